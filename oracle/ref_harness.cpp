@@ -1,0 +1,892 @@
+// Fixture harness for the reference's CFD hot path.
+//
+// Test infrastructure only: this file is compiled (by oracle/Makefile.ref) against the
+// reference sources under /root/reference/src and linked with the archive built from them.
+// It never ships with the product and never runs on a GPU host.  It sets up one small
+// single-rank REEF3D domain the way driver::driver / driver::cfd_driver do
+// (driver.cpp:33-118, driver_makegrid.cpp:29-62, driver_ini_cfd.cpp:45-160), writes the
+// DIVEMesh grid file the reference expects (read_grid.cpp:29-911), runs one group of
+// hot-path operations through the reference's own classes and dumps every input and
+// output as FP64 .npy arrays in the i-fastest, 3-ghost layout used by the GPU code:
+// array[k+3][j+3][i+3] = field(i,j,k).
+//
+//   ref_harness <case> <outdir>
+//   cases: box  (operator-level fixtures on a 12x10x8 walled box)
+//          nonuni (operator-level fixtures on a stretched 10x9x11 grid)
+//          vortex (C0-style level-set benchmark, 16^3, a few steps)
+//          tank (momentum_FC3 wave-tank steps, 16x6x12, free-surface top)
+//          tank2d (momentum_FC3 wave-tank steps, 24x1x12, j_dir=0)
+#include<mpi.h>
+#include<cstdio>
+#include<cstdlib>
+#include<cstring>
+#include<cstdint>
+#include<cmath>
+#include<string>
+#include<vector>
+#include<fstream>
+#include<sys/stat.h>
+#include<unistd.h>
+
+#include"lexer.h"
+#include"ghostcell.h"
+#include"fdm.h"
+#include"field1.h"
+#include"field2.h"
+#include"field3.h"
+#include"field4.h"
+#include"grid_helper.h"
+#include"mgcslice1.h"
+#include"mgcslice2.h"
+#include"mgcslice4.h"
+#include"weno_hj_df_nug.h"
+#include"weno_flux_nug.h"
+#include"reini_RK3.h"
+#include"levelset_RK3.h"
+#include"bicgstab_ijk.h"
+#include"poisson_pcorr.h"
+#include"pjm_corr.h"
+#include"fluid_update_fsf.h"
+#include"ioflow_void.h"
+#include"patchBC_void.h"
+#include"particle_pls_void.h"
+#include"momentum_FC3.h"
+#include"momentum_RK3.h"
+#include"diff_void.h"
+#include"kepsilon_void.h"
+#include"heat_void.h"
+#include"concentration_void.h"
+#include"vrans_v.h"
+#include"6DOF_void.h"
+#include"FSI_void.h"
+#include"pressure_void.h"
+#include"convection_void.h"
+#include"benchmark_vortex3D.h"
+#include"initialize.h"
+
+using std::string;
+using std::vector;
+
+// ---------------------------------------------------------------- rng (SplitMix64)
+struct Rng
+{
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed) {}
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uni() { return double(next() >> 11) * (1.0 / 9007199254740992.0); }
+    double sym() { return 2.0 * uni() - 1.0; }
+};
+
+// ---------------------------------------------------------------- npy writer
+static string g_out;
+
+static void write_npy(const string& name, const double* data, const vector<long>& shape)
+{
+    string hdr = "{'descr': '<f8', 'fortran_order': False, 'shape': (";
+    long n = 1;
+    for(size_t d = 0; d < shape.size(); ++d)
+    {
+        hdr += std::to_string(shape[d]);
+        hdr += (shape.size() == 1 || d + 1 < shape.size()) ? ", " : "";
+        n *= shape[d];
+    }
+    hdr += "), }";
+    size_t total = 10 + hdr.size() + 1;
+    size_t pad = (64 - total % 64) % 64;
+    hdr.append(pad, ' ');
+    hdr += '\n';
+    std::ofstream f(g_out + "/" + name + ".npy", std::ios::binary);
+    const char magic[] = "\x93NUMPY\x01\x00";
+    f.write(magic, 8);
+    uint16_t hl = (uint16_t)hdr.size();
+    f.write((const char*)&hl, 2);
+    f.write(hdr.data(), hdr.size());
+    f.write((const char*)data, sizeof(double) * n);
+}
+
+static void write_scalars(const string& name, const vector<double>& v)
+{
+    write_npy(name, v.data(), {(long)v.size()});
+}
+
+// ---------------------------------------------------------------- case description
+struct Case
+{
+    string name;
+    int nx, ny, nz;
+    int jdir;
+    double x0, y0, z0;
+    double dx, dy, dz;
+    int nonuniform;         // stretched node coordinates
+    int topgroup;           // boundary group of the +z side (21 wall, 3 free surface)
+    int N40;                // time scheme
+    double dt;
+    double W22;
+    double N44;
+    int N46;
+};
+
+static double node_coord(int i, int n, double x0, double d, int nonuni)
+{
+    if(!nonuni)
+        return x0 + double(i) * d;
+    double L = double(n) * d;
+    double s = double(i) / double(n);
+    return x0 + L * (s + 0.08 * sin(2.0 * 3.141592653589793 * s) / (2.0 * 3.141592653589793));
+}
+
+static void put_i(std::ofstream& f, int v) { f.write((const char*)&v, sizeof(int)); }
+static void put_d(std::ofstream& f, double v) { f.write((const char*)&v, sizeof(double)); }
+
+// DIVEMesh grid file, layout of read_grid.cpp:75-911 for one rank, no solids/topo.
+static void write_grid(const Case& c)
+{
+    mkdir("DIVEMesh_Grid", 0755);
+    std::ofstream f("DIVEMesh_Grid/grid-000001.dat", std::ios::binary);
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const int marge = 5;
+    // boundary entries: LOOP order, side order 1..6 (1 -x, 2 +y, 3 -y, 4 +x, 5 -z, 6 +z)
+    vector<int> gcb;
+    for(int i = 0; i < nx; ++i)
+    for(int j = 0; j < ny; ++j)
+    for(int k = 0; k < nz; ++k)
+    {
+        int sides[7] = {0, i == 0, j == ny - 1, j == 0, i == nx - 1, k == 0, k == nz - 1};
+        for(int s = 1; s <= 6; ++s)
+            if(sides[s])
+            {
+                int group = (s == 6) ? c.topgroup : 21;
+                gcb.push_back(i); gcb.push_back(j); gcb.push_back(k);
+                gcb.push_back(s); gcb.push_back(group);
+            }
+    }
+    const int ngcb = (int)gcb.size() / 5;
+    double x1 = node_coord(nx, nx, c.x0, c.dx, c.nonuniform);
+    double y1 = c.y0 + ny * c.dy;
+    double z1 = node_coord(nz, nz, c.z0, c.dz, c.nonuniform);
+
+    put_i(f, 1);                       // M10
+    put_i(f, nx); put_i(f, ny); put_i(f, nz);
+    put_d(f, c.dx); put_d(f, c.dx); put_d(f, c.dy); put_d(f, c.dz);
+    put_d(f, c.x0); put_d(f, c.y0); put_d(f, c.z0);
+    put_d(f, x1); put_d(f, y1); put_d(f, z1);
+    put_d(f, c.x0); put_d(f, c.y0); put_d(f, c.z0);
+    put_d(f, x1); put_d(f, y1); put_d(f, z1);
+    put_i(f, nx); put_i(f, ny); put_i(f, nz);
+    put_i(f, 0); put_i(f, 0); put_i(f, 0);        // origin_i,j,k
+    put_i(f, ngcb);                                // gcwall_count
+    for(int q = 0; q < 6 + 6 + 4 + 4; ++q) put_i(f, 0);   // para / paraco / slpara / slparaco
+    for(int q = 0; q < 6; ++q) put_i(f, -2);       // nb1..nb6
+    put_i(f, 1); put_i(f, 1); put_i(f, 1);         // mx my mz
+    put_i(f, 0); put_i(f, 0); put_i(f, 0);
+    for(int q = 0; q < 6; ++q) put_i(f, 1);        // bcside1..6
+    put_i(f, 0); put_i(f, 0); put_i(f, 0);         // periodic1..3
+    for(int q = 0; q < 6; ++q) put_i(f, 0);        // periodicX1..6
+    put_i(f, 1); put_i(f, c.jdir); put_i(f, 1);    // i_dir j_dir k_dir
+    put_i(f, 0);                                   // P150
+    put_i(f, 0); put_i(f, 0);                      // solidread toporead
+    put_i(f, 0); put_i(f, 0); put_i(f, 0); put_i(f, 0); put_i(f, 0);   // estimates
+    put_i(f, 0); put_i(f, 0); put_i(f, 0); put_i(f, 0); put_i(f, 0);   // 5 ints (cms_flag 2nd)
+    put_d(f, 0.0); put_d(f, 0.0); put_d(f, 0.0);   // global_orig_x/y, alpha_grid
+    put_d(f, 0.0); put_d(f, 0.0); put_d(f, 0.0);
+    for(int i = 0; i < nx; ++i)
+    for(int j = 0; j < ny; ++j)
+    for(int k = 0; k < nz; ++k)
+        put_i(f, 1);                               // flag4: fluid
+    for(int i = -marge; i < nx + 1 + marge; ++i) put_d(f, node_coord(i, nx, c.x0, c.dx, c.nonuniform));
+    for(int j = -marge; j < ny + 1 + marge; ++j) put_d(f, c.y0 + double(j) * c.dy);
+    for(int k = -marge; k < nz + 1 + marge; ++k) put_d(f, node_coord(k, nz, c.z0, c.dz, c.nonuniform));
+    for(int q = 0; q < ngcb * 5; ++q) put_i(f, gcb[q]);
+    for(int i = 0; i < nx; ++i)
+    for(int j = 0; j < ny; ++j)
+        put_i(f, 1);                               // flagslice4
+    for(int i = 0; i < nx; ++i)
+    for(int j = 0; j < ny; ++j)
+        put_d(f, 0.0);                             // bed
+}
+
+static void write_ctrl(const Case& c)
+{
+    FILE* f = fopen("ctrl.txt", "w");
+    fprintf(f, "A 10 6\nM 10 1\n");
+    fprintf(f, "N 10 1\nN 40 %d\nN 44 %.17g\nN 46 %d\nN 48 0\nN 49 %.17g\n", c.N40, c.N44, c.N46, c.dt);
+    fprintf(f, "N 45 1000000\nP 12 1000000\n");
+    fprintf(f, "D 10 4\nD 20 0\nD 30 1\n");
+    fprintf(f, "F 30 3\nF 35 5\nF 40 3\nF 44 3\nF 49 1\n");
+    fprintf(f, "W 1 1000.0\nW 2 1.0e-6\nW 3 1.205\nW 4 1.4938e-5\nW 22 %.17g\n", c.W22);
+    fclose(f);
+}
+
+// ---------------------------------------------------------------- environment
+struct Env
+{
+    lexer* p;
+    ghostcell* pgc;
+    fdm* a;
+    patchBC_interface* pBC;
+    ioflow* pflow;
+};
+
+static Env setup(const Case& c)
+{
+    string dir = "/tmp/reef_ref_" + c.name + "_" + std::to_string(getpid());
+    mkdir(dir.c_str(), 0755);
+    if(chdir(dir.c_str()) != 0) { perror("chdir"); exit(1); }
+    write_grid(c);
+    write_ctrl(c);
+
+    Env e;
+    int argc = 1;
+    char arg0[] = "ref_harness";
+    char* argvv[] = {arg0, nullptr};
+    char** argv = argvv;
+    e.p = new lexer;
+    e.pgc = new ghostcell(argc, argv, e.p);
+    lexer* p = e.p;
+    ghostcell* pgc = e.pgc;
+
+    // driver::driver (driver.cpp:48-58)
+    pgc->mpi_check(p);
+    p->lexer_read(pgc);
+    p->vellast();
+    pgc->gc_ini(p);
+    pgc->gcx_ini(p);
+    p->gridini(pgc);
+    e.pBC = new patchBC_void(p);
+
+    // fixed grid CFD branch (driver.cpp:109-118) -> driver::makegrid (driver_makegrid.cpp:29-62)
+    p->flagini();
+    pgc->flagfield(p);
+    pgc->flagx(p, p->flagsf1);
+    pgc->flagx(p, p->flagsf2);
+    pgc->flagx(p, p->flagsf3);
+    pgc->flagx(p, p->flagsf4);
+    pgc->flagx(p, p->flag1);
+    pgc->flagx(p, p->flag2);
+    pgc->flagx(p, p->flag3);
+    pgc->flagx(p, p->flag4);
+    pgc->flagx(p, p->flag);
+    pgc->gcxupdate(p);
+    p->vecsize(pgc);
+    {
+        grid_helper gridgen(p);
+        gridgen.fillgcb1(p);
+        gridgen.fillgcb2(p);
+        gridgen.fillgcb3(p);
+        gridgen.fillgcb4a(p);
+        gridgen.fillgcb4_wall(p);
+        gridgen.make_dgc(p);
+        gridgen.fill_dgc1(p);
+        gridgen.fill_dgc2(p);
+        gridgen.fill_dgc3(p);
+        gridgen.fill_dgc4(p);
+    }
+    // driver::makegrid2D (driver_makgrid_2D.cpp)
+    {
+        pgc->gcslflagx(p, p->flagslice4);
+        mgcslice1 m1(p);
+        mgcslice2 m2(p);
+        mgcslice4 m4(p);
+        m1.makemgc(p); pgc->gcslflagx(p, p->flagslice1); m1.gcb_seed(p); m1.mgcsetup(p); m1.fillmgc(p); m1.gcdirfill(p);
+        m2.makemgc(p); pgc->gcslflagx(p, p->flagslice2); m2.gcb_seed(p); m2.mgcsetup(p); m2.fillmgc(p); m2.gcdirfill(p);
+        m4.makemgc(p); m4.gcb_seed(p); m4.mgcsetup(p); m4.fillmgc(p); m4.gcdirfill(p);
+        m1.make_ggc(p); m1.fill_ggc(p);
+    }
+
+    // driver::cfd_driver (driver.cpp:205-212) + driver_logic_cfd.cpp:45 (makegrid_cds)
+    e.a = new fdm(p);
+    pgc->fdm_update(e.a);
+    pgc->sizeM_update(p, e.a);
+    e.pflow = new ioflow_v(p, pgc, e.pBC);
+    // driver_ini_cfd.cpp:56 -> initialize::start (initialize.cpp:37-70): default fields,
+    // porosity 1, node numbering (cellnum), extents
+    {
+        initialize ini(p);
+        ini.start(e.a, p, pgc);
+    }
+    // porosity/solid/topo/fb = 1 on every cell (ghosts included): no porous media,
+    // no solids, no topography
+    for(int n = 0; n < p->imax * p->jmax * p->kmax; ++n)
+    {
+        e.a->porosity.V[n] = 1.0;
+        e.a->solid.V[n] = 1.0;
+        e.a->topo.V[n] = 1.0;
+        e.a->fb.V[n] = 1.0;
+    }
+    // driver_ini_cfd.cpp:95, :129
+    pgc->solid_forcing_ini(p, e.a);
+    e.pflow->ini(p, e.a, pgc);
+
+    // driver_ini_cfd.cpp:48-50
+    p->count = 0;
+    p->cellnumtot = pgc->globalisum(p->cellnum);
+    p->dt = c.dt;
+    p->simtime = 0.0;
+    return e;
+}
+
+// ---------------------------------------------------------------- dump helpers
+static int G_nx, G_ny, G_nz;
+
+static vector<long> fshape() { return {G_nz + 6, G_ny + 6, G_nx + 6}; }
+
+static void dump_field(const string& name, field& f)
+{
+    vector<double> buf((size_t)(G_nx + 6) * (G_ny + 6) * (G_nz + 6));
+    size_t n = 0;
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+        buf[n++] = f(i, j, k);
+    write_npy(name, buf.data(), fshape());
+}
+
+static void dump_iarr(const string& name, int* f)
+{
+    lexer* dummy = nullptr;
+    (void)dummy;
+    vector<double> buf((size_t)(G_nx + 6) * (G_ny + 6) * (G_nz + 6));
+    size_t n = 0;
+    int jmax = G_ny + 6, kmax = G_nz + 6;
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+        buf[n++] = f[(i + 3) * jmax * kmax + (j + 3) * kmax + (k + 3)];
+    write_npy(name, buf.data(), fshape());
+}
+
+static void copy_field(field& dst, field& src)
+{
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+        dst(i, j, k) = src(i, j, k);
+}
+
+static void dump_grid(lexer* p)
+{
+    // 1D arrays, offset marge=5 (iterators1D.h): index n <-> i = n - 5
+    auto d1 = [&](const string& name, double* a, int n) {
+        vector<double> v(a, a + n);
+        write_scalars(name, v);
+    };
+    d1("XN", p->XN, p->knox + 1 + 10);
+    d1("YN", p->YN, p->knoy + 1 + 10);
+    d1("ZN", p->ZN, p->knoz + 1 + 10);
+    d1("XP", p->XP, p->knox + 10);
+    d1("YP", p->YP, p->knoy + 10);
+    d1("ZP", p->ZP, p->knoz + 10);
+    d1("DXN", p->DXN, p->knox + 10);
+    d1("DYN", p->DYN, p->knoy + 10);
+    d1("DZN", p->DZN, p->knoz + 10);
+    d1("DXP", p->DXP, p->knox + 10);
+    d1("DYP", p->DYP, p->knoy + 10);
+    d1("DZP", p->DZP, p->knoz + 10);
+    write_scalars("grid_scalars", {p->DXM, p->DYM, p->DZM, p->psi, double(p->cellnumtot),
+                                   double(p->ulast), double(p->vlast), double(p->wlast)});
+    // WENO tables for indices IP in [4, knox+6)
+    auto tab = [&](const string& name, double**** q, double*** cf, double**** is, int n) {
+        vector<double> vq, vc, vi;
+        for(int ip = 4; ip < n + 6; ++ip)
+        for(int uf = 0; uf < 2; ++uf)
+        for(int r = 0; r < 6; ++r)
+        {
+            vq.push_back(q[ip][uf][r][0]);
+            vq.push_back(q[ip][uf][r][1]);
+            vc.push_back(cf[ip][uf][r]);
+            vi.push_back(is[ip][uf][r][0]);
+            vi.push_back(is[ip][uf][r][1]);
+            vi.push_back(is[ip][uf][r][2]);
+        }
+        write_npy("qf" + name, vq.data(), {(long)(n + 2), 2, 6, 2});
+        write_npy("cf" + name, vc.data(), {(long)(n + 2), 2, 6});
+        write_npy("isf" + name, vi.data(), {(long)(n + 2), 2, 6, 3});
+    };
+    tab("x", weno_nug_func::qfx, weno_nug_func::cfx, weno_nug_func::isfx, p->knox);
+    tab("y", weno_nug_func::qfy, weno_nug_func::cfy, weno_nug_func::isfy, p->knoy);
+    tab("z", weno_nug_func::qfz, weno_nug_func::cfz, weno_nug_func::isfz, p->knoz);
+}
+
+// ---------------------------------------------------------------- inputs
+static void fill_random(field& f, Rng& r, double scale)
+{
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+        f(i, j, k) = scale * r.sym();
+}
+
+// smooth random field + small noise, everywhere including ghosts
+static void fill_smooth(field& f, Rng& r, lexer* p, double amp, double noise)
+{
+    double a1 = r.sym(), a2 = r.sym(), a3 = r.sym(), ph = 6.0 * r.uni();
+    double kx = 1.0 + 2.0 * r.uni(), ky = 1.0 + 2.0 * r.uni(), kz = 1.0 + 2.0 * r.uni();
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+    {
+        double x = p->XP[i + 5], y = p->YP[j + 5], z = p->ZP[k + 5];
+        f(i, j, k) = amp * (a1 * sin(kx * x + ph) + a2 * cos(ky * y - ph) + a3 * sin(kz * z + 0.5 * ph))
+                   + noise * r.sym();
+    }
+}
+
+// signed distance to a sphere (positive inside) plus a small perturbation
+static void fill_phi_sphere(field& f, Rng& r, lexer* p, double xc, double yc, double zc, double rad, double noise)
+{
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+    {
+        double x = p->XP[i + 5], y = p->YP[j + 5], z = p->ZP[k + 5];
+        double d = sqrt((x - xc) * (x - xc) + (y - yc) * (y - yc) + (z - zc) * (z - zc));
+        f(i, j, k) = (rad - d) * (1.0 + noise * r.sym());
+    }
+}
+
+// ---------------------------------------------------------------- operation groups
+static void op_ghosts(Env& e, Rng& r)
+{
+    lexer* p = e.p;
+    ghostcell* pgc = e.pgc;
+    field4 f4(p);
+    field1 f1(p);
+    field2 f2(p);
+    field3 f3(p);
+    struct G { int kind; int gcv; };
+    G list[] = {{4, 52}, {4, 1}, {4, 40}, {4, 50}, {1, 10}, {2, 11}, {3, 12}, {1, 7}, {2, 8}, {3, 9}, {1, 14}, {3, 16}};
+    for(auto& g : list)
+    {
+        field* f = g.kind == 1 ? (field*)&f1 : g.kind == 2 ? (field*)&f2 : g.kind == 3 ? (field*)&f3 : (field*)&f4;
+        fill_random(*f, r, 1.0);
+        string tag = "gc" + std::to_string(g.kind) + "_" + std::to_string(g.gcv);
+        dump_field(tag + "_in", *f);
+        if(g.kind == 1) pgc->start1(p, *f, g.gcv);
+        if(g.kind == 2) pgc->start2(p, *f, g.gcv);
+        if(g.kind == 3) pgc->start3(p, *f, g.gcv);
+        if(g.kind == 4) pgc->start4(p, *f, g.gcv);
+        dump_field(tag + "_out", *f);
+    }
+}
+
+static void set_velocities(Env& e, Rng& r, double amp)
+{
+    lexer* p = e.p;
+    fill_smooth(e.a->u, r, p, amp, 0.05 * amp);
+    fill_smooth(e.a->v, r, p, amp, 0.05 * amp);
+    fill_smooth(e.a->w, r, p, amp, 0.05 * amp);
+    e.pgc->start1(p, e.a->u, 10);
+    e.pgc->start2(p, e.a->v, 11);
+    e.pgc->start3(p, e.a->w, 12);
+}
+
+static void op_hj(Env& e, Rng& r)
+{
+    lexer* p = e.p;
+    fdm* a = e.a;
+    set_velocities(e, r, 1.0);
+    fill_phi_sphere(a->phi, r, p, 0.45 * G_nx * p->DXM, 0.5 * G_ny * p->DXM, 0.55 * G_nz * p->DXM, 0.3 * G_nz * p->DXM, 0.02);
+    e.pgc->start4(p, a->phi, 52);
+    dump_field("hj_u", a->u);
+    dump_field("hj_v", a->v);
+    dump_field("hj_w", a->w);
+    dump_field("hj_phi", a->phi);
+    weno_hj_df_nug hj(p);
+    // L is zeroed on LOOP first, as levelset_RK3.cpp:114-115 does
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+        a->L(i, j, k) = 0.0;
+    hj.start(p, a, a->phi, 4, a->u, a->v, a->w);
+    dump_field("hj_L", a->L);
+}
+
+static void op_flux(Env& e, Rng& r)
+{
+    lexer* p = e.p;
+    fdm* a = e.a;
+    set_velocities(e, r, 1.0);
+    dump_field("fx_u", a->u);
+    dump_field("fx_v", a->v);
+    dump_field("fx_w", a->w);
+    weno_flux_nug wf(p);
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+        a->F(i, j, k) = a->G(i, j, k) = a->H(i, j, k) = 0.0;
+    wf.start(p, a, a->u, 1, a->u, a->v, a->w);
+    wf.start(p, a, a->v, 2, a->u, a->v, a->w);
+    wf.start(p, a, a->w, 3, a->u, a->v, a->w);
+    dump_field("fx_F", a->F);
+    dump_field("fx_G", a->G);
+    dump_field("fx_H", a->H);
+}
+
+static void op_reini(Env& e, Rng& r)
+{
+    lexer* p = e.p;
+    fdm* a = e.a;
+    fill_phi_sphere(a->phi, r, p, 0.5 * G_nx * p->DXM, 0.45 * G_ny * p->DXM, 0.5 * G_nz * p->DXM, 0.28 * G_nz * p->DXM, 0.15);
+    e.pgc->start4(p, a->phi, 52);
+    dump_field("ri_phi_in", a->phi);
+    reini_RK3 ri(p, 1);
+    p->count = 1;
+    p->F44 = 3;
+    ri.start(a, p, a->phi, e.pgc, e.pflow);
+    dump_field("ri_phi_out", a->phi);
+}
+
+static void op_update(Env& e, Rng& r)
+{
+    lexer* p = e.p;
+    fdm* a = e.a;
+    fill_phi_sphere(a->phi, r, p, 0.5 * G_nx * p->DXM, 0.5 * G_ny * p->DXM, 0.4 * G_nz * p->DXM, 0.3 * G_nz * p->DXM, 0.0);
+    e.pgc->start4(p, a->phi, 52);
+    initialize ini(p);
+    ini.inipsi(p, a, e.pgc);
+    dump_field("fu_phi", a->phi);
+    fluid_update_fsf fu(p, a, e.pgc);
+    p->count = 1;
+    fu.start(p, a, e.pgc, a->u, a->v, a->w);
+    dump_field("fu_ro", a->ro);
+    dump_field("fu_visc", a->visc);
+    write_scalars("fu_scalars", {p->psi, p->volume1, p->volume2});
+}
+
+static void op_poisson(Env& e, Rng& r)
+{
+    lexer* p = e.p;
+    fdm* a = e.a;
+    // two-phase density from a tilted plane interface
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+        a->phi(i, j, k) = 0.55 * G_nz * p->DXM + 0.1 * p->XP[i + 5] - p->ZP[k + 5];
+    e.pgc->start4(p, a->phi, 52);
+    initialize ini(p);
+    ini.inipsi(p, a, e.pgc);
+    set_velocities(e, r, 0.5);
+    fill_smooth(a->press, r, p, 100.0, 1.0);
+    e.pgc->start4(p, a->press, 40);
+    dump_field("pp_phi", a->phi);
+    dump_field("pp_u", a->u);
+    dump_field("pp_v", a->v);
+    dump_field("pp_w", a->w);
+    dump_field("pp_press", a->press);
+    write_scalars("pp_scalars", {p->psi, p->dt, 2.0 / 3.0});
+
+    heat* ph = new heat_void(p, a, e.pgc);
+    concentration* pc = new concentration_void(p, a, e.pgc);
+    poisson_pcorr pois(p, ph, pc);
+    bicgstab_ijk solv(p, a, e.pgc);
+    pjm_corr pj(p, a, e.pgc, ph, pc);
+    p->count = 1;
+    pj.start(a, p, &pois, &solv, e.pgc, e.pflow, a->u, a->v, a->w, 2.0 / 3.0);
+    dump_field("pp_u_out", a->u);
+    dump_field("pp_v_out", a->v);
+    dump_field("pp_w_out", a->w);
+    dump_field("pp_press_out", a->press);
+    dump_field("pp_pcorr", pj.pcorr);
+    // assembled operator + folded rhs, in LOOP (i,j,k; k fastest) order
+    int n = p->cellnum;
+    write_scalars("pp_Mp", vector<double>(a->M.p, a->M.p + n));
+    write_scalars("pp_Mn", vector<double>(a->M.n, a->M.n + n));
+    write_scalars("pp_Ms", vector<double>(a->M.s, a->M.s + n));
+    write_scalars("pp_Mw", vector<double>(a->M.w, a->M.w + n));
+    write_scalars("pp_Me", vector<double>(a->M.e, a->M.e + n));
+    write_scalars("pp_Mt", vector<double>(a->M.t, a->M.t + n));
+    write_scalars("pp_Mb", vector<double>(a->M.b, a->M.b + n));
+    write_scalars("pp_rhs", vector<double>(a->rhsvec.V, a->rhsvec.V + n));
+    write_scalars("pp_iter", {double(p->solveriter)});
+}
+
+static void op_bicgstab(Env& e, Rng& r)
+{
+    // one solve on the assembled operator of op_poisson's density field, random rhs,
+    // maxiter capped so the iteration path itself is the fixture
+    lexer* p = e.p;
+    fdm* a = e.a;
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+        a->phi(i, j, k) = 0.5 * G_nz * p->DXM - 0.05 * p->XP[i + 5] - p->ZP[k + 5];
+    e.pgc->start4(p, a->phi, 52);
+    initialize ini(p);
+    ini.inipsi(p, a, e.pgc);
+    heat* ph = new heat_void(p, a, e.pgc);
+    concentration* pc = new concentration_void(p, a, e.pgc);
+    poisson_pcorr pois(p, ph, pc);
+    field4 x(p);
+    pois.start(p, a, x);
+    for(int n = 0; n < p->cellnum; ++n)
+        a->rhsvec.V[n] = r.sym();
+    dump_field("bs_phi", a->phi);
+    write_scalars("bs_rhs", vector<double>(a->rhsvec.V, a->rhsvec.V + p->cellnum));
+    bicgstab_ijk solv(p, a, e.pgc);
+    int iters[] = {1, 2, 5, 40};
+    for(int it : iters)
+    {
+        for(int k = -3; k < G_nz + 3; ++k)
+        for(int j = -3; j < G_ny + 3; ++j)
+        for(int i = -3; i < G_nx + 3; ++i)
+            x(i, j, k) = 0.0;
+        p->N46 = it;
+        p->N44 = 1.0e-30;
+        solv.start(p, a, e.pgc, x, a->rhsvec, 5);
+        dump_field("bs_x_it" + std::to_string(it), x);
+        write_scalars("bs_iter_it" + std::to_string(it), {double(p->solveriter)});
+    }
+}
+
+// ---------------------------------------------------------------- level-set benchmark (C0-style)
+static void run_vortex(Env& e, int nsteps)
+{
+    lexer* p = e.p;
+    fdm* a = e.a;
+    ghostcell* pgc = e.pgc;
+    // exact sphere SDF, positive inside (water convention)
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+    {
+        double x = p->XP[i + 5], y = p->YP[j + 5], z = p->ZP[k + 5];
+        a->phi(i, j, k) = 0.15 - sqrt((x - 0.35) * (x - 0.35) + (y - 0.35) * (y - 0.35) + (z - 0.35) * (z - 0.35));
+    }
+    pgc->start4(p, a->phi, 52);
+    initialize ini(p);
+    ini.inipsi(p, a, pgc);
+    heat* ph = new heat_void(p, a, pgc);
+    concentration* pc = new concentration_void(p, a, pgc);
+    benchmark_vortex3D bench(p, a);
+    levelset_RK3 ls(p, a, pgc, ph, pc);
+    weno_hj_df_nug hj(p);
+    reini_RK3 ri(p, 1);
+    particle_pls_void ppart;
+    // momentum_RK3 with D10=D20=D30=0 (driver_logic_cfd.cpp:622-629)
+    convection_void cvoid(p);
+    diff_void dvoid;
+    pressure_void pvoid(p);
+    poisson_pcorr pois(p, ph, pc);
+    kepsilon_void turb(p, a, pgc);
+    bicgstab_ijk solv(p, a, pgc);
+    fsi_void fsi(p, pgc);
+    momentum_RK3 mom(p, a, &cvoid, &dvoid, &pvoid, &pois, &turb, &solv, &solv, e.pflow, &fsi);
+    vrans_v vr(p, pgc);
+    sixdof_void sd(p, pgc);
+    sd.initialize(p, a, pgc);
+    vr.initialize_cfd(p, a, pgc);
+    dump_field("vx_phi_0", a->phi);
+    for(int s = 1; s <= nsteps; ++s)
+    {
+        ++p->count;
+        // driver_loop_cfd.cpp:82-96
+        ls.start(a, p, &hj, &solv, pgc, e.pflow, &ri, &ppart, a->phi);
+        ls.update(p, a, pgc, a->phi);
+        mom.start(p, a, pgc, &vr, &sd);
+        bench.start(p, a, pgc, &hj);
+        p->simtime += p->dt;
+        string t = std::to_string(s);
+        dump_field("vx_phi_" + t, a->phi);
+        if(s == nsteps)
+        {
+            dump_field("vx_u_" + t, a->u);
+            dump_field("vx_v_" + t, a->v);
+            dump_field("vx_w_" + t, a->w);
+            dump_field("vx_ro_" + t, a->ro);
+        }
+        write_scalars("vx_vol_" + t, {p->volume1, p->volume2, p->simtime});
+    }
+    write_scalars("vx_scalars", {p->psi, p->dt, p->DXM});
+}
+
+// ---------------------------------------------------------------- wave tank (momentum_FC3)
+static void run_tank(Env& e, int nsteps, double depth, double amp, double wlen)
+{
+    lexer* p = e.p;
+    fdm* a = e.a;
+    ghostcell* pgc = e.pgc;
+    const double g = 9.81;
+    const double kw = 2.0 * 3.141592653589793 / wlen;
+    const double om = sqrt(g * kw * tanh(kw * depth));
+    // phi: vertical distance to eta(x) (positive in water); linear (Airy) velocities in water
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+    {
+        double x = p->XP[i + 5], z = p->ZP[k + 5];
+        double eta = amp * cos(kw * x);
+        a->phi(i, j, k) = depth + eta - z;
+        double xu = p->XN[i + 6], zw = p->ZN[k + 6];
+        double zu = z - depth, zz = zw - depth;
+        double etau = amp * cos(kw * xu);
+        double uval = (z <= depth + etau) ? amp * om * cosh(kw * (zu + depth)) / sinh(kw * depth) * cos(kw * xu) : 0.0;
+        double etaw = amp * cos(kw * x);
+        double wval = (zw <= depth + etaw) ? amp * om * sinh(kw * (zz + depth)) / sinh(kw * depth) * sin(kw * x) : 0.0;
+        a->u(i, j, k) = uval;
+        a->v(i, j, k) = 0.0;
+        a->w(i, j, k) = wval;
+    }
+    pgc->start4(p, a->phi, 52);
+    pgc->start1(p, a->u, 10);
+    pgc->start2(p, a->v, 11);
+    pgc->start3(p, a->w, 12);
+    initialize ini(p);
+    ini.inipsi(p, a, pgc);
+    heat* ph = new heat_void(p, a, pgc);
+    concentration* pc = new concentration_void(p, a, pgc);
+    fluid_update_fsf fu(p, a, pgc);
+    fu.start(p, a, pgc, a->u, a->v, a->w);
+    // hydrostatic pressure integrated from the top (p = 0 above the domain)
+    for(int i = -3; i < G_nx + 3; ++i)
+    for(int j = -3; j < G_ny + 3; ++j)
+    {
+        double acc = 0.0;
+        for(int k = G_nz + 2; k >= -3; --k)
+        {
+            acc += a->ro(i, j, k) * g * p->DZN[k + 5] * (k == G_nz + 2 ? 0.5 : 1.0);
+            a->press(i, j, k) = acc;
+        }
+    }
+    pgc->start4(p, a->press, 40);
+
+    weno_flux_nug conv(p);
+    weno_hj_df_nug fsfdisc(p);
+    diff_void dvoid;
+    poisson_pcorr pois(p, ph, pc);
+    pjm_corr press(p, a, pgc, ph, pc);
+    kepsilon_void turb(p, a, pgc);
+    bicgstab_ijk solv(p, a, pgc);
+    reini_RK3 ri(p, 1);
+    fsi_void fsi(p, pgc);
+    momentum_FC3 mom(p, a, pgc, &conv, &fsfdisc, &dvoid, &press, &pois, &turb, &solv, &solv, e.pflow, ph, pc, &ri, &fsi);
+    vrans_v vr(p, pgc);
+    sixdof_void sd(p, pgc);
+    sd.initialize(p, a, pgc);
+    vr.initialize_cfd(p, a, pgc);
+
+    dump_field("tk_phi_0", a->phi);
+    dump_field("tk_u_0", a->u);
+    dump_field("tk_v_0", a->v);
+    dump_field("tk_w_0", a->w);
+    dump_field("tk_press_0", a->press);
+    dump_field("tk_ro_0", a->ro);
+    dump_field("tk_visc_0", a->visc);
+    write_scalars("tk_scalars", {p->psi, p->dt, p->W22, p->N44, double(p->N46)});
+    for(int s = 1; s <= nsteps; ++s)
+    {
+        ++p->count;
+        mom.start(p, a, pgc, &vr, &sd);
+        p->simtime += p->dt;
+        string t = std::to_string(s);
+        dump_field("tk_phi_" + t, a->phi);
+        dump_field("tk_u_" + t, a->u);
+        dump_field("tk_v_" + t, a->v);
+        dump_field("tk_w_" + t, a->w);
+        dump_field("tk_press_" + t, a->press);
+        dump_field("tk_ro_" + t, a->ro);
+        dump_field("tk_visc_" + t, a->visc);
+        write_scalars("tk_vol_" + t, {p->volume1, p->volume2, double(p->solveriter)});
+    }
+}
+
+int main(int argc, char** argv)
+{
+    if(argc < 3)
+    {
+        fprintf(stderr, "usage: ref_harness <case> <outdir>\n");
+        return 2;
+    }
+    string cname = argv[1];
+    g_out = argv[2];
+    mkdir(g_out.c_str(), 0755);
+    char cwd[4096];
+    if(g_out[0] != '/' && getcwd(cwd, sizeof(cwd)))
+        g_out = string(cwd) + "/" + g_out;
+
+    Case c;
+    c.name = cname;
+    c.x0 = c.y0 = c.z0 = 0.0;
+    c.nonuniform = 0;
+    c.topgroup = 21;
+    c.N40 = 3;
+    c.dt = 0.01;
+    c.W22 = 0.0;
+    c.N44 = 1.0e-10;
+    c.N46 = 5000;
+    c.jdir = 1;
+    if(cname == "box")
+    {
+        c.nx = 12; c.ny = 10; c.nz = 8;
+        c.dx = c.dy = c.dz = 0.1;
+    }
+    else if(cname == "nonuni")
+    {
+        c.nx = 10; c.ny = 9; c.nz = 11;
+        c.dx = c.dy = c.dz = 0.1;
+        c.nonuniform = 1;
+    }
+    else if(cname == "vortex")
+    {
+        c.nx = c.ny = c.nz = 16;
+        c.dx = c.dy = c.dz = 1.0 / 16.0;
+        c.N40 = 13;
+        c.dt = 0.5 / (16.0 * 2.0);
+    }
+    else if(cname == "tank" || cname == "tank2d")
+    {
+        c.nx = cname == "tank" ? 16 : 24;
+        c.ny = cname == "tank" ? 6 : 1;
+        c.jdir = cname == "tank" ? 1 : 0;
+        c.nz = 12;
+        c.dx = c.dy = c.dz = 0.05;
+        c.topgroup = 3;
+        c.dt = 0.005;
+        c.W22 = -9.81;
+        c.N44 = 1.0e-14;
+    }
+    else
+    {
+        fprintf(stderr, "unknown case %s\n", cname.c_str());
+        return 2;
+    }
+    G_nx = c.nx; G_ny = c.ny; G_nz = c.nz;
+
+    Env e = setup(c);
+    Rng r(1000 + (uint64_t)cname.size() * 7919u);
+    // weno tables are static and built by the first weno_nug_func object
+    {
+        weno_hj_df_nug init_tables(e.p);
+        initialize ini(e.p);
+        ini.inipsi(e.p, e.a, e.pgc);
+        dump_grid(e.p);
+        dump_iarr("flag1", e.p->flag1);
+        dump_iarr("flag4", e.p->flag4);
+    }
+    if(cname == "box" || cname == "nonuni")
+    {
+        op_ghosts(e, r);
+        op_hj(e, r);
+        op_flux(e, r);
+        op_reini(e, r);
+        op_update(e, r);
+        if(cname == "box")
+        {
+            op_poisson(e, r);
+            op_bicgstab(e, r);
+        }
+    }
+    else if(cname == "vortex")
+        run_vortex(e, 4);
+    else if(cname == "tank")
+        run_tank(e, 3, 0.35, 0.02, 0.8);
+    else if(cname == "tank2d")
+        run_tank(e, 3, 0.35, 0.02, 0.6);
+    MPI_Finalize();
+    return 0;
+}
