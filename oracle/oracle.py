@@ -1,0 +1,151 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product package.  Arrays use the GPU field layout:
+numpy shape (nz+6, ny+6, nx+6), float64, C order (i fastest), 3 ghost layers.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("rho_water", ctypes.c_double), ("visc_water", ctypes.c_double),
+                ("rho_air", ctypes.c_double), ("visc_air", ctypes.c_double),
+                ("gi", ctypes.c_double), ("gj", ctypes.c_double), ("gk", ctypes.c_double),
+                ("F43", ctypes.c_double), ("F45", ctypes.c_double),
+                ("dt", ctypes.c_double), ("N44", ctypes.c_double), ("N46", ctypes.c_int)]
+
+
+def build():
+    import subprocess
+    root = os.path.dirname(_HERE)
+    subprocess.run(["make", "-s", "-f", "oracle/Makefile"], cwd=root, check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int] * 4 + [dp] * 3 + [ctypes.c_double] * 2 + \
+            [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(Params)]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        for name in ("orc_grid_array",):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_tables.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp]
+        L.orc_scalars.argtypes = [ctypes.c_void_p, dp]
+        L.orc_ghost.argtypes = [ctypes.c_void_p, dp, ctypes.c_int, ctypes.c_int]
+        L.orc_hj.argtypes = [ctypes.c_void_p] + [dp] * 5
+        L.orc_flux.argtypes = [ctypes.c_void_p, ctypes.c_int] + [dp] * 5
+        L.orc_reini.argtypes = [ctypes.c_void_p, dp, ctypes.c_int]
+        L.orc_fluid_update.argtypes = [ctypes.c_void_p] + [dp] * 4
+        L.orc_poisson.argtypes = [ctypes.c_void_p] + [dp] * 4
+        L.orc_bicgstab.argtypes = [ctypes.c_void_p, dp, dp, dp, ctypes.c_double, ctypes.c_int, dp]
+        L.orc_bicgstab.restype = ctypes.c_int
+        L.orc_pjm.argtypes = [ctypes.c_void_p] + [dp] * 6 + [ctypes.c_double]
+        L.orc_pjm.restype = ctypes.c_int
+        L.orc_fc3_step.argtypes = [ctypes.c_void_p] + [dp] * 8
+        L.orc_fc3_step.restype = ctypes.c_int
+        L.orc_vortex_step.argtypes = [ctypes.c_void_p] + [dp] * 6 + [ctypes.c_double, dp]
+        L.orc_phi_rk3.argtypes = [ctypes.c_void_p] + [dp] * 4
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Domain:
+    """One box domain of the oracle (orc_create)."""
+
+    def __init__(self, nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21,
+                 rho=(1000.0, 1.205), visc=(1.0e-6, 1.4938e-5), g=(0.0, 0.0, 0.0),
+                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000):
+        self.nx, self.ny, self.nz, self.jdir = nx, ny, nz, jdir
+        self.shape = (nz + 6, ny + 6, nx + 6)
+        groups = (ctypes.c_int * 7)(0, 21, 21, 21, 21, 21, top_group)
+        self.prm = Params(rho[0], visc[0], rho[1], visc[1], g[0], g[1], g[2], F43, F45, dt, N44, N46)
+        self._xn = [np.ascontiguousarray(a, dtype=np.float64) for a in (XN, YN, ZN)]
+        self.h = lib().orc_create(nx, ny, nz, jdir, _p(self._xn[0]), _p(self._xn[1]), _p(self._xn[2]),
+                                  DYM, DZM, groups, ctypes.byref(self.prm))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def zeros(self):
+        return np.zeros(self.shape)
+
+    def grid_array(self, which, axis):
+        n = (self.nx, self.ny, self.nz)[axis] + 10
+        out = np.zeros(n)
+        lib().orc_grid_array(self.h, which, axis, _p(out))
+        return out
+
+    def tables(self, axis):
+        n = (self.nx, self.ny, self.nz)[axis] + 2
+        qf, cf, isf = np.zeros((n, 2, 6, 2)), np.zeros((n, 2, 6)), np.zeros((n, 2, 6, 3))
+        lib().orc_tables(self.h, axis, _p(qf), _p(cf), _p(isf))
+        return qf, cf, isf
+
+    def scalars(self):
+        out = np.zeros(5)
+        lib().orc_scalars(self.h, _p(out))
+        return dict(DXM=out[0], DYM=out[1], DZM=out[2], psi=out[3], cellnum=int(out[4]))
+
+    def ghost(self, f, kind, gcv):
+        lib().orc_ghost(self.h, _p(f), kind, gcv)
+
+    def hj(self, phi, u, v, w, L):
+        lib().orc_hj(self.h, _p(phi), _p(u), _p(v), _p(w), _p(L))
+
+    def flux(self, ipol, q, u, v, w, acc):
+        lib().orc_flux(self.h, ipol, _p(q), _p(u), _p(v), _p(w), _p(acc))
+
+    def reini(self, phi, iters):
+        lib().orc_reini(self.h, _p(phi), iters)
+
+    def fluid_update(self, phi, ro, visc):
+        vol = np.zeros(2)
+        lib().orc_fluid_update(self.h, _p(phi), _p(ro), _p(visc), _p(vol))
+        return vol
+
+    def poisson(self, phi, pcorr, rhs):
+        n = self.nx * self.ny * self.nz
+        M = np.zeros(7 * n)
+        lib().orc_poisson(self.h, _p(phi), _p(pcorr), _p(M), _p(rhs))
+        return M.reshape(7, n)
+
+    def bicgstab(self, x, M, rhs, stop, maxiter):
+        res = np.zeros(1)
+        M = np.ascontiguousarray(M)
+        it = lib().orc_bicgstab(self.h, _p(x), _p(M), _p(rhs), stop, maxiter, _p(res))
+        return it, res[0]
+
+    def pjm(self, phi, u, v, w, press, pcorr, alpha):
+        return lib().orc_pjm(self.h, _p(phi), _p(u), _p(v), _p(w), _p(press), _p(pcorr), alpha)
+
+    def fc3_step(self, phi, u, v, w, press, ro, visc):
+        vol = np.zeros(2)
+        it = lib().orc_fc3_step(self.h, _p(phi), _p(u), _p(v), _p(w), _p(press), _p(ro), _p(visc), _p(vol))
+        return it, vol
+
+    def vortex_step(self, phi, u, v, w, ro, visc, simtime):
+        vol = np.zeros(2)
+        lib().orc_vortex_step(self.h, _p(phi), _p(u), _p(v), _p(w), _p(ro), _p(visc), simtime, _p(vol))
+        return vol
+
+    def phi_rk3(self, phi, u, v, w):
+        lib().orc_phi_rk3(self.h, _p(phi), _p(u), _p(v), _p(w))

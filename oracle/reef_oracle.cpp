@@ -1,0 +1,1279 @@
+// CPU oracle for the REEF3D free-surface hot path.
+//
+// TEST INFRASTRUCTURE ONLY: the parity checker of tests/ and the CPU baseline timed by
+// bench.py.  The product (reef3d_amd/) never includes, links or calls this file.
+//
+// A plain, serial restatement of the reference's algorithm, in the reference's loop order
+// and operation order (built with -ffp-contract=off) so that it reproduces the reference
+// bit for bit on the same inputs.  Every function cites the reference file:line it follows
+// (paths relative to REEF3D src/).  Box domains only: every side is a wall (group 21) or
+// the top is a free-surface boundary (group 3); one rank; no solids, porosity 1.
+#include "reef_oracle.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+const double PI_REF = 3.14159265359;   // definitions.h:37
+const int MARGE = 5;                    // 1-D array offset (iterators1D.h)
+const int OBJ = -20, WATER = 10;        // definitions.h:27-34
+
+struct Entry
+{
+    int i, j, k, side, group;
+    double dist;
+};
+
+struct Dgc
+{
+    int i, j, k, di, dj, dk;
+};
+
+}  // namespace
+
+struct orc_domain
+{
+    int nx, ny, nz, jdir;
+    long sx, sy, sz;
+    int group[7];
+    orc_params prm;
+    std::vector<double> X[3];     // node coordinates XN/YN/ZN, offset MARGE, zero padded
+    std::vector<double> P[3];     // XP/YP/ZP
+    std::vector<double> DN[3];    // DXN/DYN/DZN
+    std::vector<double> DP[3];    // DXP/DYP/DZP
+    double DXM, DYM, DZM, psi, ydir;
+    long cellnum;
+    // weno tables [IP][uf][row][coef] (weno_nug_func.h:198-200)
+    std::vector<double> qf[3], cf[3], isf[3];
+    // boundary lists
+    std::vector<Entry> gcb[5];
+    std::vector<Dgc> dgc[5];
+    std::vector<int> flag[5];     // flag1..flag4 over the whole field (ghosts included)
+    int last[4];                  // ulast, vlast, wlast
+
+    long id(int i, int j, int k) const { return (long)(i + 3) + sx * ((long)(j + 3) + sy * (long)(k + 3)); }
+    int nn(int a) const { return a == 0 ? nx : a == 1 ? ny : nz; }
+};
+
+namespace {
+
+// ------------------------------------------------------------------ grid
+// grid::gridspacing (grid.cpp:54-150); DXM averages DXP over x, DYP over y (3D), DZP over z.
+void gridspacing(orc_domain* d)
+{
+    for(int a = 0; a < 3; ++a)
+    {
+        int n = d->nn(a);
+        int len = n + 1 + 4 * MARGE;
+        std::vector<double>& X = d->X[a];
+        d->P[a].assign(len, 0.0);
+        d->DN[a].assign(len, 0.0);
+        d->DP[a].assign(len, 0.0);
+        for(int i = -MARGE; i < n + MARGE; ++i)
+        {
+            int ip = i + MARGE;
+            d->P[a][ip] = 0.5 * (X[ip] + X[ip + 1]);
+            d->DN[a][ip] = X[ip + 1] - X[ip];
+            d->DP[a][ip] = 0.5 * (X[ip + 2] + X[ip + 1]) - 0.5 * (X[ip + 1] + X[ip]);
+        }
+    }
+    double DXM = 0.0;
+    int count = 0;
+    for(int i = 0; i < d->nx; ++i) { DXM += d->DP[0][i + MARGE]; ++count; }
+    if(d->jdir == 1)
+        for(int j = 0; j < d->ny; ++j) { DXM += d->DP[1][j + MARGE]; ++count; }
+    for(int k = 0; k < d->nz; ++k) { DXM += d->DP[2][k + MARGE]; ++count; }
+    d->DXM = DXM / double(count);
+}
+
+// weno_nug_func_precalc_qf.cpp:27-322, _cf.cpp:27-188, _isf.cpp:27-890 restated for one
+// coordinate array X (XN for uf=0, XP for uf=1) and one index IP.
+void tables_at(const double* X, int ip, double* q /*6x2*/, double* c /*6*/, double* s /*6x3*/)
+{
+    const double m2 = X[ip - 2], m1 = X[ip - 1], x0 = X[ip], p1 = X[ip + 1], p2 = X[ip + 2],
+                 p3 = X[ip + 3], p4 = X[ip + 4];
+    // qf: reconstruction coefficients
+    q[0] = ((p2 - p1) / (p3 - x0)) * ((p3 - p1) / (p2 - x0));
+    q[1] = ((p2 - p1) / (p3 - x0)) * ((p1 - x0) / (p3 - p1));
+    q[2] = ((p1 - x0) / (p2 - m1)) * ((p1 - m1) / (p2 - x0));
+    q[3] = ((p1 - x0) / (p2 - m1)) * ((p2 - p1) / (p1 - m1));
+    q[4] = ((p1 - x0) / (x0 - m2)) * ((p1 - m1) / (p1 - m2));
+    q[5] = (1.0 + (p1 - x0) / (p1 - m1) + (p1 - x0) / (p1 - m2));
+    q[6] = (1.0 + (p2 - p1) / (p3 - p1) + (p2 - p1) / (p4 - p1));
+    q[7] = ((p2 - p1) / (p4 - p1)) * ((p3 - p1) / (p4 - p2));
+    q[8] = ((p2 - p1) / (p3 - x0)) * ((p3 - p1) / (p2 - x0));
+    q[9] = ((p2 - p1) / (p3 - x0)) * ((p1 - x0) / (p3 - p1));
+    q[10] = ((p1 - x0) / (p2 - m1)) * ((p1 - m1) / (p2 - x0));
+    q[11] = ((p1 - x0) / (p2 - m1)) * ((p2 - p1) / (p1 - m1));
+    // cf: linear weights
+    c[0] = ((p1 - m2) / (p3 - m2)) * ((p1 - m1) / (p3 - m1));
+    c[1] = ((p1 - m2) / (p3 - m2)) * ((p3 - p1) / (p3 - m1)) * ((p3 - m1) / (p2 - m2) + 1.0);
+    c[2] = ((p2 - p1) / (p3 - m2)) * ((p3 - p1) / (p2 - m2));
+    c[3] = ((p1 - x0) / (p4 - m1)) * ((p1 - m1) / (p4 - x0));
+    c[4] = ((p1 - m1) / (p4 - m1)) * ((p4 - p1) / (p4 - x0)) * ((p4 - x0) / (p3 - m1) + 1.0);
+    c[5] = ((p3 - p1) / (p4 - m1)) * ((p4 - p1) / (p3 - m1));
+    // isf: smoothness indicator coefficients
+    double fac;
+    fac = 4.0 * pow((p1 - x0) / (p3 - x0), 2.0);
+    s[0] = fac * ((10.0 * pow(p1 - x0, 2.0) + (p2 - x0) * (p2 - p1)) / pow(p3 - p1, 2.0));
+    s[1] = fac * ((20.0 * pow(p1 - x0, 2.0) + 2.0 * (p2 - x0) * (p2 - p1) + (p3 - x0) * (2.0 * p2 - p1 - x0))
+                  / ((p3 - p1) * (p2 - x0)));
+    s[2] = fac * ((10.0 * pow(p1 - x0, 2.0) + (p3 + p2 - 2.0 * x0) * (p3 + p2 - p1 - x0)) / pow(p2 - x0, 2.0));
+    fac = 4.0 * pow((p1 - x0) / (p2 - m1), 2.0);
+    s[3] = fac * ((10.0 * pow(p1 - x0, 2.0) + (p2 - x0) * (p2 - p1)) / pow(p1 - m1, 2.0));
+    s[4] = fac * ((20.0 * pow(p1 - x0, 2.0) - (p2 - p1) * (x0 - m1) - (p2 - x0) * (p1 - m1))
+                  / ((p2 - x0) * (p1 - m1)));
+    s[5] = fac * ((10.0 * pow(p1 - x0, 2.0) + (x0 - m1) * (p1 - m1)) / pow(p2 - x0, 2.0));
+    fac = 4.0 * pow((p1 - x0) / (p1 - m2), 2.0);
+    s[6] = fac * ((10.0 * pow(p1 - x0, 2.0) + (p1 - m1) * (x0 - m1)) / pow(x0 - m2, 2.0));
+    s[7] = fac * ((20.0 * pow(p1 - x0, 2.0) + 2.0 * (p1 - m1) * (x0 - m1) + (p1 - m2) * (p1 + x0 - 2.0 * m1))
+                  / ((p1 - m1) * (x0 - m2)));
+    s[8] = fac * ((10.0 * pow(p1 - x0, 2.0) + (2.0 * p1 - m2 - m1) * (p1 + x0 - m1 - m2)) / pow(p1 - m1, 2.0));
+    fac = 4.0 * pow((p2 - p1) / (p4 - p1), 2.0);
+    s[9] = fac * ((10.0 * pow(p2 - p1, 2.0) + (p3 - p1) * (p3 - p2)) / pow(p4 - p2, 2.0));
+    s[10] = fac * ((20.0 * pow(p2 - p1, 2.0) + 2.0 * (p3 - p1) * (p3 - p2) + (p4 - p1) * (2.0 * p3 - p2 - p1))
+                   / ((p4 - p2) * (p3 - p1)));
+    s[11] = fac * ((10.0 * pow(p2 - p1, 2.0) + (p4 + p3 - 2.0 * p1) * (p4 + p3 - p2 - p1)) / pow(p3 - p1, 2.0));
+    fac = 4.0 * pow((p2 - p1) / (p3 - x0), 2.0);
+    s[12] = fac * ((10.0 * pow(p2 - p1, 2.0) + (p3 - p1) * (p3 - p2)) / pow(p2 - x0, 2.0));
+    s[13] = fac * ((20.0 * pow(p2 - p1, 2.0) - (p3 - p2) * (p1 - x0) - (p3 - p1) * (p2 - x0))
+                   / ((p3 - p1) * (p2 - x0)));
+    s[14] = fac * ((10.0 * pow(p2 - p1, 2.0) + (p1 - x0) * (p2 - x0)) / pow(p3 - p1, 2.0));
+    fac = 4.0 * pow((p2 - p1) / (p2 - m1), 2.0);
+    s[15] = fac * ((10.0 * pow(p2 - p1, 2.0) + (p2 - x0) * (p1 - x0)) / pow(p1 - m1, 2.0));
+    s[16] = fac * ((20.0 * pow(p2 - p1, 2.0) + 2.0 * (p2 - x0) * (p1 - x0) + (p2 - m1) * (p2 + p1 - 2.0 * x0))
+                   / ((p2 - x0) * (p1 - m1)));
+    s[17] = fac * ((10.0 * pow(p2 - p1, 2.0) + (2.0 * p2 - m1 - x0) * (p2 + p1 - x0 - m1)) / pow(p2 - x0, 2.0));
+}
+
+void build_tables(orc_domain* d)
+{
+    for(int a = 0; a < 3; ++a)
+    {
+        int n = d->nn(a);
+        d->qf[a].assign((size_t)(n + 8) * 24, 0.0);
+        d->cf[a].assign((size_t)(n + 8) * 12, 0.0);
+        d->isf[a].assign((size_t)(n + 8) * 36, 0.0);
+        for(int uf = 0; uf < 2; ++uf)
+        {
+            const double* X = uf == 0 ? d->X[a].data() : d->P[a].data();
+            for(int i = -1; i < n + 1; ++i)      // IBLOOP
+            {
+                int ip = i + MARGE;
+                tables_at(X, ip, &d->qf[a][(size_t)ip * 24 + uf * 12], &d->cf[a][(size_t)ip * 12 + uf * 6],
+                          &d->isf[a][(size_t)ip * 36 + uf * 18]);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ boundary lists
+// read_grid.cpp:458-488 (gcb4 as written for a box), lexer_gridini.cpp:146-166 (gcd4),
+// gc_flagfield.cpp:26-98 (flags), grid_helper_gcb1/2/3.cpp (velocity lists),
+// grid_helper.cpp:42-59 + grid_helper_dgc1..4.cpp (edge/corner lists; the hgc marks
+// accumulate from dgc1 to dgc4 because make_dgc clears them once).
+void build_lists(orc_domain* d)
+{
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    long ncell = d->sx * d->sy * d->sz;
+    // gcb4 in LOOP order, sides 1..6
+    std::vector<Entry>& g4 = d->gcb[4];
+    g4.clear();
+    for(int i = 0; i < nx; ++i)
+    for(int j = 0; j < ny; ++j)
+    for(int k = 0; k < nz; ++k)
+    {
+        int at[7] = {0, i == 0, j == ny - 1, j == 0, i == nx - 1, k == 0, k == nz - 1};
+        for(int s = 1; s <= 6; ++s)
+            if(at[s])
+            {
+                Entry e{i, j, k, s, d->group[s], 0.0};
+                if(s == 1 || s == 4) e.dist = 0.5 * d->DP[0][i + MARGE];
+                else if(s == 2 || s == 3) e.dist = 0.5 * d->DP[1][j + MARGE];
+                else e.dist = 0.5 * d->DP[2][k + MARGE];
+                g4.push_back(e);
+            }
+    }
+    // flags
+    for(int f = 1; f <= 4; ++f)
+        d->flag[f].assign(ncell, OBJ);
+    for(int i = 0; i < nx; ++i)
+    for(int j = 0; j < ny; ++j)
+    for(int k = 0; k < nz; ++k)
+        for(int f = 1; f <= 4; ++f)
+            d->flag[f][d->id(i, j, k)] = WATER;
+    for(const Entry& e : g4)
+    {
+        if(e.side == 4) d->flag[1][d->id(e.i, e.j, e.k)] = OBJ;
+        if(e.side == 2) d->flag[2][d->id(e.i, e.j, e.k)] = OBJ;
+        if(e.side == 6) d->flag[3][d->id(e.i, e.j, e.k)] = OBJ;
+    }
+    // velocity lists (fillgcb1/2/3)
+    const int lastside[4] = {0, 4, 2, 6};
+    for(int c = 1; c <= 3; ++c)
+    {
+        std::vector<Entry>& g = d->gcb[c];
+        g = g4;
+        int ax = c - 1;
+        std::vector<char> fgc(ncell, 0);     // fgc[IJK][lastside-1]
+        for(Entry& e : g)
+        {
+            int sa = (e.side == 1 || e.side == 4) ? 0 : (e.side == 2 || e.side == 3) ? 1 : 2;
+            if(sa == ax)
+            {
+                int idx = ax == 0 ? e.i : ax == 1 ? e.j : e.k;
+                e.dist += 0.5 * d->DP[ax][idx + MARGE];
+            }
+            if(e.side == lastside[c]) fgc[d->id(e.i, e.j, e.k)] = 1;
+        }
+        for(Entry& e : g)
+            if(e.side == lastside[c])
+            {
+                if(c == 1) e.i -= 1;
+                if(c == 2) e.j -= 1;
+                if(c == 3) e.k -= 1;
+            }
+        for(Entry& e : g)
+            if(e.side != lastside[c] && fgc[d->id(e.i, e.j, e.k)] == 1)
+                e.side = -std::abs(e.side);
+    }
+    // edge / corner lists
+    static const int DIRS[20][3] = {
+        {-1, -1, 0}, {1, -1, 0}, {1, 1, 0}, {-1, 1, 0},
+        {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1}, {1, 0, 1},
+        {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1},
+        {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+        {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1}};
+    std::vector<int> hgc(ncell, 0);
+    for(int c = 1; c <= 4; ++c)
+    {
+        for(const Entry& e : d->gcb[c])
+        {
+            if(e.side <= 0) continue;
+            for(int n = 0; n < 3; ++n)
+            {
+                int i = e.i, j = e.j, k = e.k;
+                if(e.side == 1) i -= n + 1;
+                if(e.side == 4) i += n + 1;
+                if(e.side == 3) j -= n + 1;
+                if(e.side == 2) j += n + 1;
+                if(e.side == 5) k -= n + 1;
+                if(e.side == 6) k += n + 1;
+                hgc[d->id(i, j, k)] += 1;
+            }
+        }
+        const std::vector<int>& fl = d->flag[c];
+        int il = c == 1 ? d->last[1] : 0, jl = c == 2 ? d->last[2] : 0, kl = c == 3 ? d->last[3] : 0;
+        std::vector<Dgc>& out = d->dgc[c];
+        out.clear();
+        for(int i = 0; i < nx - il; ++i)
+        for(int j = 0; j < ny - jl; ++j)
+        for(int k = 0; k < nz - kl; ++k)
+        {
+            if(fl[d->id(i, j, k)] <= 0) continue;
+            for(int q = 0; q < 20; ++q)
+            {
+                int di = DIRS[q][0], dj = DIRS[q][1], dk = DIRS[q][2];
+                if(fl[d->id(i + di, j + dj, k + dk)] >= 0) continue;
+                if(di != 0 && fl[d->id(i + di, j, k)] >= 0) continue;
+                if(dj != 0 && fl[d->id(i, j + dj, k)] >= 0) continue;
+                if(dk != 0 && fl[d->id(i, j, k + dk)] >= 0) continue;
+                if(hgc[d->id(i + di, j + dj, k + dk)] != 0) continue;
+                out.push_back(Dgc{i, j, k, di, dj, dk});
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ ghost cells
+// ghostcell labels (ghostcell.cpp:45-200 with the defaults B20=2, B23=1, B75..B77=1,
+// B98=B99=0) and gceval1..4 (gc_epol1.cpp:26-121, gc_epol2.cpp, gc_epol3.cpp,
+// gc_epol4.cpp:26-317) for the groups of a box (21 wall, 3 free surface) and the field
+// tags of the hot path.  Returns 0 (untouched), 1 (dirichlet_ortho), 4 (neumann),
+// 5 (noslip).
+int label(int kind, int gcv, int bc, int cs)
+{
+    if(gcv == 50) return 4;
+    if(kind == 4)
+    {
+        if((gcv == 51 || gcv == 52 || gcv == 53 || gcv == 54) && (bc == 21 || bc == 3)) return 4;
+        if(gcv == 40 && (bc == 21 || bc == 3)) return 4;
+        if(gcv == 1) return 4;
+        return 0;
+    }
+    // velocities: tangential / normal sides of component `kind`
+    bool normal = (kind == 1 && (cs == 1 || cs == 4)) || (kind == 2 && (cs == 2 || cs == 3))
+               || (kind == 3 && (cs == 5 || cs == 6));
+    int g_main = kind == 1 ? 10 : kind == 2 ? 11 : 12;   // gcval_u/v/w
+    int g_alt = kind;                                      // gcv 1,2,3 (same rules)
+    int g_corr = kind == 1 ? 7 : kind == 2 ? 8 : 9;      // pjm_corr vel_setup
+    int g_neu = kind == 1 ? 14 : kind == 2 ? 15 : 16;
+    if(bc == 21)
+    {
+        if(!normal && (gcv == g_main || gcv == g_alt)) return 5;   // gclabel_u/v/w
+        if(gcv == g_neu) return 4;
+        if(normal && (gcv == g_main || gcv == g_alt)) return 1;    // gclabel_*_orth
+        if(normal && gcv == g_corr) return 5;                        // gclabel_vel
+        return 0;
+    }
+    if(bc == 3)
+    {
+        int g_fs = kind == 1 ? 17 : kind == 2 ? 18 : 19;
+        if(gcv == g_main || gcv == g_fs || gcv == g_alt)
+        {
+            if(!normal) return 4;
+            return kind == 3 ? 5 : 1;   // gc_epol3.cpp: free-surface w normal -> noslip (A10=6)
+        }
+        return 0;
+    }
+    return 0;
+}
+
+inline void setg(orc_domain* d, double* f, const Entry& e, int q, double val)
+{
+    int i = e.i, j = e.j, k = e.k;
+    switch(e.side)
+    {
+        case 1: i -= q + 1; break;
+        case 4: i += q + 1; break;
+        case 2: j += q + 1; break;
+        case 3: j -= q + 1; break;
+        case 5: k -= q + 1; break;
+        case 6: k += q + 1; break;
+    }
+    f[d->id(i, j, k)] = val;
+}
+
+// dirichlet_ortho (gc_dirichlet-ortho.cpp:28-137), orderdir=2 (B23=1), wall value 0, no
+// image point (needs dist < B29*dx, never true for the grids used here)
+void dirichlet_ortho(orc_domain* d, double* f, const Entry& e)
+{
+    double dx;
+    if(e.side == 1 || e.side == 4) dx = d->DP[0][e.i + MARGE];
+    else if(e.side == 2 || e.side == 3) dx = d->DP[1][e.j + MARGE];
+    else dx = d->DP[2][e.k + MARGE];
+    int ys = 1;
+    if(e.dist > d->DXM * (1.0 - 1.0e-6) && e.dist < d->DXM * (1.0 + 1.0e-6)) ys = 0;
+    const int orderdir = 2;
+    double pos[2] = {0.0, e.dist};
+    double x[3];
+    for(int m = 0; m < 3; ++m) x[m] = dx * double(m + 2 - ys);
+    double y[8];
+    y[0] = f[d->id(e.i, e.j, e.k)];
+    y[1] = 0.0;
+    for(int q = 0; q < 3; ++q)
+    {
+        y[orderdir + q] = 0.0;
+        for(int m = 0; m < orderdir; ++m)
+        {
+            double weight = 1.0;
+            for(int n = 0; n < orderdir; ++n)
+                if(m != n) weight *= (x[q] - pos[n]) / (pos[m] - pos[n] + 1.0e-20);
+            y[orderdir + q] += weight * y[m];
+        }
+    }
+    for(int q = 0; q < 3; ++q) setg(d, f, e, q, y[orderdir + q - 1 + ys]);
+}
+
+// ghostcell::start1..4 (gc_start.cpp:27-182) + dgcpol1..4 (gc_dgcpol4.cpp:27-51)
+void ghost(orc_domain* d, double* f, int kind, int gcv)
+{
+    const std::vector<Entry>& g = d->gcb[kind];
+    const std::vector<Entry>& g1 = d->gcb[1];
+    if(kind != 2 || d->jdir == 1)
+        for(size_t q = 0; q < g.size(); ++q)
+        {
+            const Entry& e = g[q];
+            if(e.side <= 0) continue;
+            if(!((g1[q].side != 2 && g1[q].side != 3) || d->jdir == 1)) continue;
+            int lb = label(kind, gcv, e.group, e.side);
+            if((e.side == 2 || e.side == 3) && d->jdir == 0 && (lb == 4 || lb == 5)) continue;
+            if(lb == 4)
+            {
+                double v = f[d->id(e.i, e.j, e.k)];
+                for(int n = 0; n < 3; ++n) setg(d, f, e, n, v);
+            }
+            else if(lb == 5)
+            {
+                for(int n = 0; n < 3; ++n) setg(d, f, e, n, 0.0);
+            }
+            else if(lb == 1)
+                dirichlet_ortho(d, f, e);
+        }
+    for(const Dgc& c : d->dgc[kind])
+        f[d->id(c.i + c.di, c.j + c.dj, c.k + c.dk)] = f[d->id(c.i, c.j, c.k)];
+}
+
+// ------------------------------------------------------------------ WENO kernels
+struct Weno
+{
+    const orc_domain* d;
+    double q1, q2, q3, q4, q5;
+
+    // is_min/max + weight_min/max + grad (weno_nug_func.h:48-196; weno_hj_df_nug.cpp:110-177)
+    double recon(int axis, int ip, int uf, bool minus) const
+    {
+        const double* I = &d->isf[axis][(size_t)ip * 36 + uf * 18];
+        const double* C = &d->cf[axis][(size_t)ip * 12 + uf * 6];
+        const double* Q = &d->qf[axis][(size_t)ip * 24 + uf * 12];
+        const double dq12 = q1 - q2, dq23 = q2 - q3, dq32 = q3 - q2, dq34 = q3 - q4, dq43 = q4 - q3,
+                     dq54 = q5 - q4;
+        int r = minus ? 0 : 3;
+        double is1 = I[(r + 0) * 3 + 0] * dq54 * dq54 + I[(r + 0) * 3 + 1] * (dq54) * (dq34) + I[(r + 0) * 3 + 2] * dq34 * dq34;
+        double is2 = I[(r + 1) * 3 + 0] * dq23 * dq23 + I[(r + 1) * 3 + 1] * (dq43) * (dq23) + I[(r + 1) * 3 + 2] * dq43 * dq43;
+        double is3 = I[(r + 2) * 3 + 0] * dq12 * dq12 + I[(r + 2) * 3 + 1] * (dq32) * (dq12) + I[(r + 2) * 3 + 2] * dq32 * dq32;
+        const double psi = 1.0e-6, eps = 0.0;
+        const double a1 = is1 + psi, a2 = is2 + psi, a3 = is3 + psi;
+        const double c0 = C[r], c1 = C[r + 1], c2 = C[r + 2];
+        double w1 = c0 / (eps + (a1 * a1) * (c0 / (a1 * a1) + c1 / (a2 * a2) + c2 / (a3 * a3)));
+        double w2 = c1 / (eps + (a2 * a2) * (c0 / (a1 * a1) + c1 / (a2 * a2) + c2 / (a3 * a3)));
+        double w3 = c2 / (eps + (a3 * a3) * (c0 / (a1 * a1) + c1 / (a2 * a2) + c2 / (a3 * a3)));
+        if(minus)
+            return w1 * (q4 + Q[0] * (q3 - q4) - Q[1] * (q5 - q4))
+                 + w2 * (q3 + Q[2] * (q4 - q3) - Q[3] * (q2 - q3))
+                 + w3 * (q2 + Q[4] * (q1 - q2) + Q[5] * (q3 - q2));
+        return w1 * (q4 + Q[6] * (q3 - q4) + Q[7] * (q5 - q4))
+             + w2 * (q3 + Q[8] * (q2 - q3) - Q[9] * (q4 - q3))
+             + w3 * (q2 + Q[10] * (q3 - q2) - Q[11] * (q1 - q2));
+    }
+};
+
+inline long stride(const orc_domain* d, int axis) { return axis == 0 ? 1 : axis == 1 ? d->sx : d->sx * d->sy; }
+
+// HJ-WENO gradient along one axis (weno_hj_df_nug.cpp:110-177 + iqmin_0/iqmax_0 :344-396;
+// identical arithmetic in ddweno_nug_sf.cpp:39-224): divided differences over DXP.
+double hj_grad(const orc_domain* d, const double* f, int i, int j, int k, int axis, double advec)
+{
+    if(!(advec > 0.0) && !(advec < 0.0)) return 0.0;
+    Weno W{d};
+    long s = stride(d, axis);
+    long c = d->id(i, j, k);
+    int ix = axis == 0 ? i : axis == 1 ? j : k;
+    const double* D = d->DP[axis].data();
+    int ip = ix + MARGE;
+    if(advec > 0.0)
+    {
+        W.q1 = (f[c - 2 * s] - f[c - 3 * s]) / D[ip - 3];
+        W.q2 = (f[c - s] - f[c - 2 * s]) / D[ip - 2];
+        W.q3 = (f[c] - f[c - s]) / D[ip - 1];
+        W.q4 = (f[c + s] - f[c]) / D[ip];
+        W.q5 = (f[c + 2 * s] - f[c + s]) / D[ip + 1];
+        return W.recon(axis, ip, 0, true);
+    }
+    W.q1 = (f[c - s] - f[c - 2 * s]) / D[ip - 2];
+    W.q2 = (f[c] - f[c - s]) / D[ip - 1];
+    W.q3 = (f[c + s] - f[c]) / D[ip];
+    W.q4 = (f[c + 2 * s] - f[c + s]) / D[ip + 1];
+    W.q5 = (f[c + 3 * s] - f[c + 2 * s]) / D[ip + 2];
+    return W.recon(axis, ip, 0, false);
+}
+
+// weno_flux_nug::fx/fy/fz (weno_flux_nug.cpp:136-258) with iqmin..kqmax (:260-313):
+// values, not differences; x upwind-max stencil repeats f(i+2) (weno_flux_nug.cpp:288-295).
+double flux_face(const orc_domain* d, const double* f, int i, int j, int k, int axis, int uf, double advec)
+{
+    if(!(advec > 0.0) && !(advec < 0.0)) return 0.0;
+    Weno W{d};
+    long s = stride(d, axis);
+    long c = d->id(i, j, k);
+    int ip = (axis == 0 ? i : axis == 1 ? j : k) + MARGE;
+    if(advec > 0.0)
+    {
+        W.q1 = f[c - 2 * s]; W.q2 = f[c - s]; W.q3 = f[c]; W.q4 = f[c + s]; W.q5 = f[c + 2 * s];
+        return W.recon(axis, ip, uf, true);
+    }
+    W.q1 = f[c - s]; W.q2 = f[c]; W.q3 = f[c + s]; W.q4 = f[c + 2 * s];
+    W.q5 = axis == 0 ? f[c + 2 * s] : f[c + 3 * s];
+    return W.recon(axis, ip, uf, false);
+}
+
+// weno_hj_df_nug::start(ipol=4)/aij (weno_hj_df_nug.cpp:58-108) with
+// flux_HJ_CDS2 ipol=4 cell-centred velocities (flux_HJ_CDS2.cpp:59-63,90-95,122-127)
+void hj(orc_domain* d, const double* phi, const double* u, const double* v, const double* w, double* L)
+{
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        long c = d->id(i, j, k);
+        double iadvec = 0.5 * (u[c] + u[c - 1]);
+        double jadvec = 0.5 * (v[c] + v[c - d->sx]);
+        double kadvec = 0.5 * (w[c] + w[c - d->sx * d->sy]);
+        double Lc = -iadvec * hj_grad(d, phi, i, j, k, 0, iadvec);
+        if(d->jdir == 1) Lc -= jadvec * hj_grad(d, phi, i, j, k, 1, jadvec);
+        Lc -= kadvec * hj_grad(d, phi, i, j, k, 2, kadvec);
+        L[c] += Lc;
+    }
+}
+
+// weno_flux_nug::start/aij (weno_flux_nug.cpp:84-134) with flux_face_CDS2 face velocities
+// (flux_face_CDS2.cpp:36-116)
+void flux(orc_domain* d, int ipol, const double* q, const double* u, const double* v, const double* w, double* acc)
+{
+    const long sx = d->sx, sxy = d->sx * d->sy;
+    int il = ipol == 1 ? d->last[1] : 0, jl = ipol == 2 ? d->last[2] : 0, kl = ipol == 3 ? d->last[3] : 0;
+    const std::vector<int>& fl = d->flag[ipol];
+    const double* DX = ipol == 1 ? d->DP[0].data() : d->DN[0].data();
+    const double* DY = ipol == 2 ? d->DP[1].data() : d->DN[1].data();
+    const double* DZ = ipol == 3 ? d->DP[2].data() : d->DN[2].data();
+    int ufx = ipol == 1, ufy = ipol == 2, ufz = ipol == 3;
+    for(int i = 0; i < d->nx - il; ++i)
+    for(int j = 0; j < d->ny - jl; ++j)
+    for(int k = 0; k < d->nz - kl; ++k)
+    {
+        long c = d->id(i, j, k);
+        if(fl[c] <= 0) continue;
+        double iv1, iv2, jv1, jv2, kv1, kv2;
+        if(ipol == 1)
+        {
+            iv1 = 0.5 * (u[c] + u[c - 1]);               iv2 = 0.5 * (u[c] + u[c + 1]);
+            jv1 = 0.5 * (v[c - sx] + v[c + 1 - sx]);      jv2 = 0.5 * (v[c] + v[c + 1]);
+            kv1 = 0.5 * (w[c - sxy] + w[c + 1 - sxy]);    kv2 = 0.5 * (w[c] + w[c + 1]);
+        }
+        else if(ipol == 2)
+        {
+            iv1 = 0.5 * (u[c - 1] + u[c - 1 + sx]);       iv2 = 0.5 * (u[c] + u[c + sx]);
+            jv1 = 0.5 * (v[c] + v[c - sx]);               jv2 = 0.5 * (v[c] + v[c + sx]);
+            kv1 = 0.5 * (w[c - sxy] + w[c - sxy + sx]);   kv2 = 0.5 * (w[c] + w[c + sx]);
+        }
+        else
+        {
+            iv1 = 0.5 * (u[c - 1] + u[c - 1 + sxy]);      iv2 = 0.5 * (u[c] + u[c + sxy]);
+            jv1 = 0.5 * (v[c - sx] + v[c - sx + sxy]);    jv2 = 0.5 * (v[c] + v[c + sxy]);
+            kv1 = 0.5 * (w[c] + w[c - sxy]);              kv2 = 0.5 * (w[c] + w[c + sxy]);
+        }
+        double fv1 = 0.0, fv2 = 0.0;
+        double fu1 = flux_face(d, q, i - 1, j, k, 0, ufx, iv1);
+        double fu2 = flux_face(d, q, i, j, k, 0, ufx, iv2);
+        if(d->jdir == 1)
+        {
+            fv1 = flux_face(d, q, i, j - 1, k, 1, ufy, jv1);
+            fv2 = flux_face(d, q, i, j, k, 1, ufy, jv2);
+        }
+        double fw1 = flux_face(d, q, i, j, k - 1, 2, ufz, kv1);
+        double fw2 = flux_face(d, q, i, j, k, 2, ufz, kv2);
+        double Lc = -((iv2 * fu2 - iv1 * fu1) / DX[i + MARGE])
+                  - ((jv2 * fv2 - jv1 * fv1) / DY[j + MARGE])
+                  - ((kv2 * fw2 - kv1 * fw1) / DZ[k + MARGE]);
+        acc[c] += Lc;
+    }
+}
+
+// ------------------------------------------------------------------ reinitialisation
+// reinidisc_f::start/disc (reinidisc_f.cpp:37-127) with ddweno_nug_sf (modus 0)
+void reini_disc(orc_domain* d, const double* f, double* L)
+{
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        long c = d->id(i, j, k);
+        double dx = 0.0, dy = 0.0, dz = 0.0;
+        double lsv = f[c];
+        double lsSig = lsv / sqrt(lsv * lsv);
+        if(fabs(lsv) < 1.0e-8) lsSig = 1.0;
+        const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
+        double xmin = (lsv - f[c - 1]) / d->DP[0][ip - 1];
+        double xplus = (f[c + 1] - lsv) / d->DP[0][ip];
+        if(xmin * lsSig > 0.0 && xplus * lsSig > -xmin * lsSig) dx = hj_grad(d, f, i, j, k, 0, 1.0);
+        if(xplus * lsSig < 0.0 && xmin * lsSig < -xplus * lsSig) dx = hj_grad(d, f, i, j, k, 0, -1.0);
+        if(xplus * lsSig > 0.0 && xmin * lsSig < 0.0) dx = 0.0;
+        if(d->jdir == 1)
+        {
+            double ymin = (lsv - f[c - d->sx]) / d->DP[1][jp - 1];
+            double yplus = (f[c + d->sx] - lsv) / d->DP[1][jp];
+            if(ymin * lsSig > 0.0 && yplus * lsSig > -ymin * lsSig) dy = hj_grad(d, f, i, j, k, 1, 1.0);
+            if(yplus * lsSig < 0.0 && ymin * lsSig < -yplus * lsSig) dy = hj_grad(d, f, i, j, k, 1, -1.0);
+            if(yplus * lsSig > 0.0 && ymin * lsSig < 0.0) dy = 0.0;
+        }
+        const long sz = d->sx * d->sy;
+        double zmin = (lsv - f[c - sz]) / d->DP[2][kp - 1];
+        double zplus = (f[c + sz] - lsv) / d->DP[2][kp];
+        if(zmin * lsSig > 0.0 && zplus * lsSig > -zmin * lsSig) dz = hj_grad(d, f, i, j, k, 2, 1.0);
+        if(zplus * lsSig < 0.0 && zmin * lsSig < -zplus * lsSig) dz = hj_grad(d, f, i, j, k, 2, -1.0);
+        if(zplus * lsSig > 0.0 && zmin * lsSig < 0.0) dz = 0.0;
+        double dnorm = sqrt(dx * dx + dy * dy + dz * dz);
+        double deltax;
+        if(d->jdir == 0) deltax = (1.0 / 2.0) * (d->DN[0][ip] + d->DN[2][kp]);
+        else deltax = (1.0 / 3.0) * (d->DN[0][ip] + d->DN[1][jp] + d->DN[2][kp]);
+        double sign = lsv / sqrt(lsv * lsv + dnorm * dnorm * deltax * deltax);
+        if(sign != sign) sign = 1.0;
+        L[c] = -(sign * dnorm - sign);
+    }
+}
+
+// reini_RK3::start/time_preproc (reini_RK3.cpp:77-160) for count>0 (gcval_phi = 52)
+void reini(orc_domain* d, double* f, int iters)
+{
+    long n = d->sx * d->sy * d->sz;
+    std::vector<double> L(n, 0.0), frk1(n, 0.0), frk2(n, 0.0), dtf(n, 0.0);
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        double a = d->DP[0][i + MARGE], b = d->DP[1][j + MARGE], c = d->DP[2][k + MARGE];
+        double m = d->jdir == 0 ? (a < c ? a : c) : (((a < b ? a : b) < c) ? (a < b ? a : b) : c);
+        dtf[d->id(i, j, k)] = d->prm.F43 * m;
+    }
+    auto interior = [&](auto fn) {
+        for(int i = 0; i < d->nx; ++i)
+        for(int j = 0; j < d->ny; ++j)
+        for(int k = 0; k < d->nz; ++k)
+            fn(d->id(i, j, k));
+    };
+    for(int q = 0; q < iters; ++q)
+    {
+        reini_disc(d, f, L.data());
+        interior([&](long c) { frk1[c] = f[c] + dtf[c] * L[c]; });
+        ghost(d, frk1.data(), 4, 52);
+        reini_disc(d, frk1.data(), L.data());
+        interior([&](long c) { frk2[c] = 0.75 * f[c] + 0.25 * frk1[c] + 0.25 * dtf[c] * L[c]; });
+        ghost(d, frk2.data(), 4, 52);
+        reini_disc(d, frk2.data(), L.data());
+        interior([&](long c) { f[c] = (1.0 / 3.0) * f[c] + (2.0 / 3.0) * frk2[c] + (2.0 / 3.0) * dtf[c] * L[c]; });
+        ghost(d, f, 4, 52);
+    }
+}
+
+// ------------------------------------------------------------------ fluid properties
+// fluid_update_fsf::start (fluid_update_fsf.cpp:39-104)
+void fluid_update(orc_domain* d, const double* phi, double* ro, double* visc, double* vol)
+{
+    const double psi = d->psi;
+    double H = 0.0, v1 = 0.0, v2 = 0.0;
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        long c = d->id(i, j, k);
+        if(phi[c] > psi) H = 1.0;
+        if(phi[c] < -psi) H = 0.0;
+        if(fabs(phi[c]) <= psi) H = 0.5 * (1.0 + phi[c] / psi + (1.0 / PI_REF) * sin((PI_REF * phi[c]) / psi));
+        ro[c] = d->prm.rho_water * H + d->prm.rho_air * (1.0 - H);
+        visc[c] = d->prm.visc_water * H + d->prm.visc_air * (1.0 - H);
+        double vol_c = d->DN[0][i + MARGE] * d->DN[1][j + MARGE] * d->DN[2][k + MARGE];
+        v1 += vol_c * (H - (1.0 - 1.0));
+        v2 += vol_c * (1.0 - H - (1.0 - 1.0));
+    }
+    ghost(d, ro, 4, 1);
+    ghost(d, visc, 4, 1);
+    if(vol) { vol[0] = v1; vol[1] = v2; }
+}
+
+// density_f::roface (density_f.cpp:35-51)
+inline double roface(const orc_domain* d, const double* phi, long c, long off)
+{
+    double phival = 0.5 * (phi[c] + phi[c + off]);
+    double H = 0.0;
+    if(phival > d->psi) H = 1.0;
+    if(phival < -d->psi) H = 0.0;
+    if(fabs(phival) <= d->psi) H = 0.5 * (1.0 + phival / d->psi + (1.0 / PI_REF) * sin((PI_REF * phival) / d->psi));
+    return d->prm.rho_water * H + d->prm.rho_air * (1.0 - H);
+}
+
+// ------------------------------------------------------------------ projection
+// poisson_pcorr::start (poisson_pcorr.cpp:78-231), porosity 1 (CPOR = PORVAL = 1)
+void poisson(orc_domain* d, const double* phi, const double* press, double* M, double* rhs)
+{
+    const long N = d->cellnum;
+    double *Mp = M, *Mn = M + N, *Ms = M + 2 * N, *Mw = M + 3 * N, *Me = M + 4 * N, *Mt = M + 5 * N, *Mb = M + 6 * N;
+    const long sx = d->sx, sxy = d->sx * d->sy;
+    const double yd = d->ydir;
+    long n = 0;
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        long c = d->id(i, j, k);
+        const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
+        const double *DXP = d->DP[0].data(), *DXN = d->DN[0].data(), *DYP = d->DP[1].data(),
+                     *DYN = d->DN[1].data(), *DZP = d->DP[2].data(), *DZN = d->DN[2].data();
+        Mp[n] = (1.0 * 1.0) / (roface(d, phi, c, 1) * DXP[ip] * DXN[ip])
+              + (1.0 * 1.0) / (roface(d, phi, c, -1) * DXP[ip - 1] * DXN[ip])
+              + (1.0 * 1.0) / (roface(d, phi, c, sx) * DYP[jp] * DYN[jp]) * yd
+              + (1.0 * 1.0) / (roface(d, phi, c, -sx) * DYP[jp - 1] * DYN[jp]) * yd
+              + (1.0 * 1.0) / (roface(d, phi, c, sxy) * DZP[kp] * DZN[kp])
+              + (1.0 * 1.0) / (roface(d, phi, c, -sxy) * DZP[kp - 1] * DZN[kp]);
+        Mn[n] = -(1.0 * 1.0) / (roface(d, phi, c, 1) * DXP[ip] * DXN[ip]);
+        Ms[n] = -(1.0 * 1.0) / (roface(d, phi, c, -1) * DXP[ip - 1] * DXN[ip]);
+        Mw[n] = -(1.0 * 1.0) / (roface(d, phi, c, sx) * DYP[jp] * DYN[jp]) * yd;
+        Me[n] = -(1.0 * 1.0) / (roface(d, phi, c, -sx) * DYP[jp - 1] * DYN[jp]) * yd;
+        Mt[n] = -(1.0 * 1.0) / (roface(d, phi, c, sxy) * DZP[kp] * DZN[kp]);
+        Mb[n] = -(1.0 * 1.0) / (roface(d, phi, c, -sxy) * DZP[kp - 1] * DZN[kp]);
+        ++n;
+    }
+    const std::vector<int>& f4 = d->flag[4];
+    n = 0;
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        long c = d->id(i, j, k);
+        if(f4[c - 1] < 0) { rhs[n] -= Ms[n] * press[c - 1]; Ms[n] = 0.0; }
+        if(f4[c + 1] < 0) { rhs[n] -= Mn[n] * press[c + 1]; Mn[n] = 0.0; }
+        if(f4[c - sx] < 0 && d->jdir == 1) { rhs[n] -= Me[n] * press[c - sx]; Me[n] = 0.0; }
+        if(f4[c + sx] < 0 && d->jdir == 1) { rhs[n] -= Mw[n] * press[c + sx]; Mw[n] = 0.0; }
+        if(f4[c - sxy] < 0) { rhs[n] -= Mb[n] * press[c - sxy]; Mb[n] = 0.0; }
+        if(f4[c + sxy] < 0) { rhs[n] -= Mt[n] * press[c + sxy]; Mt[n] = 0.0; }
+        ++n;
+    }
+}
+
+// bicgstab_ijk::solve and helpers (bicgstab_ijk.cpp:125-395), var 5: FLEXLOOP over flag4
+struct Bicg
+{
+    orc_domain* d;
+    const double* M;
+    long N;
+    std::vector<double> rhs, x, rj, r0, pj, vj, sj, tj, ph, sh, aii;
+
+    template<class F> void loop(F fn)
+    {
+        long n = 0;
+        for(int i = 0; i < d->nx; ++i)
+        for(int j = 0; j < d->ny; ++j)
+        for(int k = 0; k < d->nz; ++k)
+        {
+            fn(d->id(i, j, k), n);
+            ++n;
+        }
+    }
+    double Ax(const std::vector<double>& xx, long c, long n) const
+    {
+        const long sx = d->sx, sxy = d->sx * d->sy;
+        if(d->jdir == 1)
+            return M[n] * xx[c] + M[N + n] * xx[c + 1] + M[2 * N + n] * xx[c - 1] + M[3 * N + n] * xx[c + sx]
+                 + M[4 * N + n] * xx[c - sx] + M[5 * N + n] * xx[c + sxy] + M[6 * N + n] * xx[c - sxy];
+        // bicgstab_ijk_2D.cpp:308-320 (5-point)
+        return M[n] * xx[c] + M[N + n] * xx[c + 1] + M[2 * N + n] * xx[c - 1] + M[5 * N + n] * xx[c + sxy]
+             + M[6 * N + n] * xx[c - sxy];
+    }
+    double res_calc()
+    {
+        double resi = 0.0;
+        loop([&](long c, long n) {
+            double y = rhs[c] - (Ax(x, c, n));
+            resi += y * y;
+        });
+        return sqrt(resi) / double(d->cellnum);
+    }
+    int solve(double stop, int maxiter, double* resout)
+    {
+        int solveriter = 0;
+        double residual = 1.0e9;
+        loop([&](long c, long n) { aii[c] = -1.0 / (M[n] + 1.0e-19); });
+        double r_j, norm_r0, r_j1 = 0.0;
+    restart:
+        r_j = 0.0;
+        loop([&](long c, long n) { rj[c] = rhs[c] - (Ax(x, c, n)); });
+        loop([&](long c, long) { r0[c] = pj[c] = rj[c]; r_j += rj[c] * r0[c]; });
+        norm_r0 = sqrt(r_j);
+        if((residual >= stop) && (solveriter < maxiter))
+        {
+            do
+            {
+                double sigma = 0.0, norm_vj = 0.0, norm_rj = 0.0;
+                loop([&](long c, long) { ph[c] = pj[c] * aii[c]; });
+                loop([&](long c, long n) { vj[c] = Ax(ph, c, n); });
+                loop([&](long c, long) {
+                    sigma += vj[c] * r0[c];
+                    norm_vj += vj[c] * vj[c];
+                    norm_rj += rj[c] * rj[c];
+                });
+                norm_vj = sqrt(norm_vj);
+                norm_rj = sqrt(norm_rj);
+                double alpha = r_j / sigma;
+                if(fabs(sigma) <= (1.0e-12 * (norm_vj * norm_r0)))
+                {
+                    residual = res_calc();
+                    ++solveriter;
+                    goto restart;
+                }
+                if((fabs(alpha) * norm_vj / (norm_rj == 0 ? 1.0e-15 : norm_rj)) <= 0.08)
+                {
+                    residual = res_calc();
+                    ++solveriter;
+                    goto restart;
+                }
+                double norm_sj = 0.0;
+                loop([&](long c, long) {
+                    sj[c] = rj[c] - alpha * vj[c];
+                    norm_sj += sj[c] * sj[c];
+                });
+                norm_sj = sqrt(norm_sj);
+                if(norm_sj > stop)
+                {
+                    loop([&](long c, long) { sh[c] = sj[c] * aii[c]; });
+                    loop([&](long c, long n) { tj[c] = Ax(sh, c, n); });
+                    double w1 = 0.0, w2 = 0.0;
+                    loop([&](long c, long) {
+                        w1 += tj[c] * sj[c];
+                        w2 += tj[c] * tj[c];
+                    });
+                    double w = w1 / (w2 == 0 ? 1.0e-15 : w2);
+                    r_j1 = 0.0;
+                    loop([&](long c, long) {
+                        x[c] += alpha * ph[c] + w * sh[c];
+                        rj[c] = sj[c] - w * tj[c];
+                        r_j1 += rj[c] * r0[c];
+                    });
+                    double beta = alpha * r_j1 / (w * r_j == 0 ? 1.0e-15 : (w * r_j));
+                    loop([&](long c, long) { pj[c] = rj[c] + beta * (pj[c] - w * vj[c]); });
+                }
+                if(norm_sj <= stop)
+                {
+                    r_j1 = 0.0;
+                    loop([&](long c, long) {
+                        x[c] += alpha * ph[c];
+                        rj[c] = sj[c];
+                        r_j1 += rj[c] * r0[c];
+                    });
+                }
+                r_j = r_j1;
+                residual = 0.0;
+                loop([&](long c, long) { residual += rj[c] * rj[c]; });
+                residual = sqrt(residual) / double(d->cellnum);
+                ++solveriter;
+            } while((residual >= stop) && (solveriter < maxiter));
+        }
+        if(resout) *resout = residual;
+        return solveriter;
+    }
+};
+
+int bicgstab(orc_domain* d, double* xf, const double* M, const double* rhsvec, double stop, int maxiter, double* res)
+{
+    long n = d->sx * d->sy * d->sz;
+    Bicg b;
+    b.d = d;
+    b.M = M;
+    b.N = d->cellnum;
+    for(auto* v : {&b.rhs, &b.x, &b.rj, &b.r0, &b.pj, &b.vj, &b.sj, &b.tj, &b.ph, &b.sh, &b.aii})
+        v->assign(n, 0.0);
+    b.loop([&](long c, long k) { b.x[c] = xf[c]; b.rhs[c] = rhsvec[k]; });
+    int it = b.solve(stop, maxiter, res);
+    b.loop([&](long c, long) { xf[c] = b.x[c]; });
+    return it;
+}
+
+// pjm_corr::start (pjm_corr.cpp:90-183) with poisson_pcorr + bicgstab_ijk
+int pjm(orc_domain* d, const double* phi, double* u, double* v, double* w, double* press, double* pcorr, double alpha)
+{
+    const long n = d->sx * d->sy * d->sz;
+    const long N = d->cellnum;
+    const double dt = d->prm.dt;
+    const long sx = d->sx, sxy = d->sx * d->sy;
+    ghost(d, u, 1, 7);
+    ghost(d, v, 2, 8);
+    ghost(d, w, 3, 9);
+    std::vector<double> rhs(N, 0.0), M(7 * N, 0.0);
+    for(long c = 0; c < n; ++c) pcorr[c] = 0.0;
+    ghost(d, pcorr, 4, 1);
+    long m = 0;
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        long c = d->id(i, j, k);
+        rhs[m] = -(u[c] - u[c - 1]) / (alpha * dt * d->DN[0][i + MARGE])
+               - (v[c] - v[c - sx]) / (alpha * dt * d->DN[1][j + MARGE]) * d->ydir
+               - (w[c] - w[c - sxy]) / (alpha * dt * d->DN[2][k + MARGE]);
+        ++m;
+    }
+    poisson(d, phi, pcorr, M.data(), rhs.data());
+    double res;
+    int it = bicgstab(d, pcorr, M.data(), rhs.data(), d->prm.N44, d->prm.N46, &res);
+    ghost(d, pcorr, 4, 40);
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        long c = d->id(i, j, k);
+        press[c] += pcorr[c];
+    }
+    ghost(d, press, 4, 40);
+    // ucorr / vcorr / wcorr (pjm_corr.cpp:123-149)
+    for(int i = 0; i < d->nx - d->last[1]; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        long c = d->id(i, j, k);
+        if(d->flag[1][c] <= 0) continue;
+        u[c] -= alpha * dt * 1.0 * 1.0 * ((pcorr[c + 1] - pcorr[c]) / (d->DP[0][i + MARGE] * roface(d, phi, c, 1)));
+    }
+    if(d->jdir == 1)
+        for(int i = 0; i < d->nx; ++i)
+        for(int j = 0; j < d->ny - d->last[2]; ++j)
+        for(int k = 0; k < d->nz; ++k)
+        {
+            long c = d->id(i, j, k);
+            if(d->flag[2][c] <= 0) continue;
+            v[c] -= alpha * dt * 1.0 * 1.0 * ((pcorr[c + sx] - pcorr[c]) / (d->DP[1][j + MARGE] * roface(d, phi, c, sx)));
+        }
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz - d->last[3]; ++k)
+    {
+        long c = d->id(i, j, k);
+        if(d->flag[3][c] <= 0) continue;
+        w[c] -= alpha * dt * 1.0 * 1.0 * ((pcorr[c + sxy] - pcorr[c]) / (d->DP[2][k + MARGE] * roface(d, phi, c, sxy)));
+    }
+    return it;
+}
+
+// ------------------------------------------------------------------ time steps
+template<class F> void vel_loop(orc_domain* d, int comp, F fn)
+{
+    int il = comp == 1 ? d->last[1] : 0, jl = comp == 2 ? d->last[2] : 0, kl = comp == 3 ? d->last[3] : 0;
+    const std::vector<int>& fl = d->flag[comp];
+    for(int i = 0; i < d->nx - il; ++i)
+    for(int j = 0; j < d->ny - jl; ++j)
+    for(int k = 0; k < d->nz - kl; ++k)
+    {
+        long c = d->id(i, j, k);
+        if(fl[c] > 0) fn(c, i, j, k);
+    }
+}
+
+template<class F> void cell_loop(orc_domain* d, F fn)
+{
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+        fn(d->id(i, j, k));
+}
+
+// momentum_FC3::start (momentum_FC3.cpp:122-430) with D20=0 (diff_void), B10=0,
+// laminar (kepsilon_void zeroes F/G/H), void ioflow and forcing, pjm_corr projection.
+// The working field ls is a member of momentum_FC3 whose ghost cells are never filled,
+// so the stage-1 HJ-WENO reads zeros there (momentum_FC3.cpp:132-139).
+struct FC3
+{
+    orc_domain* d;
+    long n;
+    std::vector<double> ls, frk1, frk2, L, F, urk1, urk2, vrk1, vrk2, wrk1, wrk2, udiff, vdiff, wdiff, pcorr;
+
+    explicit FC3(orc_domain* dd) : d(dd), n(dd->sx * dd->sy * dd->sz)
+    {
+        for(auto* v : {&ls, &frk1, &frk2, &L, &F, &urk1, &urk2, &vrk1, &vrk2, &wrk1, &wrk2, &udiff, &vdiff, &wdiff, &pcorr})
+            v->assign(n, 0.0);
+    }
+
+    // F/G/H for component comp of velocity set (uu,vv,ww), pressure gradient and gravity
+    void rhs_comp(int comp, const double* phi, const double* press, const double* uu, const double* vv, const double* ww)
+    {
+        const double* q = comp == 1 ? uu : comp == 2 ? vv : ww;
+        long off = comp == 1 ? 1 : comp == 2 ? d->sx : d->sx * d->sy;
+        const double* D = d->DP[comp - 1].data();
+        double g = comp == 1 ? d->prm.gi : comp == 2 ? d->prm.gj : d->prm.gk;
+        vel_loop(d, comp, [&](long c, int i, int j, int k) { F[c] = 0.0; });
+        // pjm_corr::upgrad/vpgrad/wpgrad (pjm_corr.cpp:185-204)
+        vel_loop(d, comp, [&](long c, int i, int j, int k) {
+            int ix = comp == 1 ? i : comp == 2 ? j : k;
+            F[c] -= 1.0 * (press[c + off] - press[c]) / (D[ix + MARGE] * roface(d, phi, c, off));
+        });
+        // irhs/jrhs/krhs (momentum_FC3.cpp:432-472), rhsvec = 0, Fext = 0, W29 = 0
+        vel_loop(d, comp, [&](long c, int, int, int) { F[c] += (0.0 + g + 0.0 + 0.0) * 1.0; });
+        flux(d, comp, q, uu, vv, ww, F.data());
+    }
+
+    int step(double* phi, double* u, double* v, double* w, double* press, double* ro, double* visc, double* vol)
+    {
+        const double dt = d->prm.dt;
+        int iters = 0;
+        // ---- stage 1
+        cell_loop(d, [&](long c) { L[c] = 0.0; ls[c] = phi[c]; });
+        hj(d, ls.data(), u, v, w, L.data());
+        cell_loop(d, [&](long c) { frk1[c] = ls[c] + dt * L[c]; });
+        ghost(d, frk1.data(), 4, 52);
+        reini(d, frk1.data(), 2);
+        cell_loop(d, [&](long c) { phi[c] = frk1[c]; });
+        ghost(d, phi, 4, 52);
+        fluid_update(d, phi, ro, visc, vol);
+        velocity_stage(phi, press, u, v, w, u, v, w, urk1, vrk1, wrk1, 1.0, 0);
+        iters += pjm(d, phi, urk1.data(), vrk1.data(), wrk1.data(), press, pcorr.data(), 1.0);
+        ghost(d, urk1.data(), 1, 10);
+        ghost(d, vrk1.data(), 2, 11);
+        ghost(d, wrk1.data(), 3, 12);
+        fluid_update(d, phi, ro, visc, vol);
+        // ---- stage 2
+        cell_loop(d, [&](long c) { L[c] = 0.0; });
+        hj(d, frk1.data(), urk1.data(), vrk1.data(), wrk1.data(), L.data());
+        cell_loop(d, [&](long c) { frk2[c] = 0.75 * ls[c] + 0.25 * frk1[c] + 0.25 * dt * L[c]; });
+        ghost(d, frk2.data(), 4, 52);
+        reini(d, frk2.data(), 2);
+        cell_loop(d, [&](long c) { phi[c] = frk2[c]; });
+        ghost(d, phi, 4, 52);
+        reini(d, phi, 2);
+        fluid_update(d, phi, ro, visc, vol);
+        velocity_stage(phi, press, u, v, w, urk1.data(), vrk1.data(), wrk1.data(), urk2, vrk2, wrk2, 0.25, 1);
+        iters += pjm(d, phi, urk2.data(), vrk2.data(), wrk2.data(), press, pcorr.data(), 0.25);
+        ghost(d, urk2.data(), 1, 10);
+        ghost(d, vrk2.data(), 2, 11);
+        ghost(d, wrk2.data(), 3, 12);
+        fluid_update(d, phi, ro, visc, vol);
+        // ---- stage 3
+        cell_loop(d, [&](long c) { L[c] = 0.0; });
+        hj(d, frk2.data(), urk2.data(), vrk2.data(), wrk2.data(), L.data());
+        cell_loop(d, [&](long c) { ls[c] = (1.0 / 3.0) * ls[c] + (2.0 / 3.0) * frk2[c] + (2.0 / 3.0) * dt * L[c]; });
+        cell_loop(d, [&](long c) { phi[c] = ls[c]; });
+        ghost(d, phi, 4, 52);
+        reini(d, phi, 3);
+        ghost(d, phi, 4, 52);
+        fluid_update(d, phi, ro, visc, vol);
+        std::vector<double> un(u, u + n), vn(v, v + n), wn(w, w + n);
+        velocity_stage3(phi, press, u, v, w);
+        iters += pjm(d, phi, u, v, w, press, pcorr.data(), 2.0 / 3.0);
+        ghost(d, u, 1, 10);
+        ghost(d, v, 2, 11);
+        ghost(d, w, 3, 12);
+        fluid_update(d, phi, ro, visc, vol);
+        return iters;
+    }
+
+    // stage 1 (kind 0) and stage 2 (kind 1) velocity predictors
+    void velocity_stage(const double* phi, const double* press, const double* un, const double* vn, const double* wn,
+                        const double* uu, const double* vv, const double* ww, std::vector<double>& ur,
+                        std::vector<double>& vr, std::vector<double>& wr, double, int kind)
+    {
+        const double dt = d->prm.dt;
+        std::vector<double>* out[4] = {nullptr, &ur, &vr, &wr};
+        std::vector<double>* dif[4] = {nullptr, &udiff, &vdiff, &wdiff};
+        const double* in[4] = {nullptr, uu, vv, ww};
+        const double* old[4] = {nullptr, un, vn, wn};
+        for(int comp = 1; comp <= 3; ++comp)
+        {
+            if(comp == 2 && d->jdir == 0)
+            {
+                // jsource zeroes G but VLOOP is empty in 2D; nothing else runs
+                continue;
+            }
+            rhs_comp(comp, phi, press, uu, vv, ww);
+            std::vector<double>& df = *dif[comp];
+            vel_loop(d, comp, [&](long c, int, int, int) { df[c] = in[comp][c]; });
+            ghost(d, df.data(), comp, comp == 1 ? 10 : comp == 2 ? 11 : 12);
+            std::vector<double>& o = *out[comp];
+            if(kind == 0)
+                vel_loop(d, comp, [&](long c, int, int, int) { o[c] = df[c] + dt * 1.0 * F[c]; });
+            else
+                vel_loop(d, comp, [&](long c, int, int, int) {
+                    o[c] = 0.75 * old[comp][c] + 0.25 * df[c] + 0.25 * dt * 1.0 * F[c];
+                });
+        }
+    }
+
+    void velocity_stage3(const double* phi, const double* press, double* u, double* v, double* w)
+    {
+        const double dt = d->prm.dt;
+        double* tgt[4] = {nullptr, u, v, w};
+        std::vector<double>* dif[4] = {nullptr, &udiff, &vdiff, &wdiff};
+        const double* in[4] = {nullptr, urk2.data(), vrk2.data(), wrk2.data()};
+        for(int comp = 1; comp <= 3; ++comp)
+        {
+            if(comp == 2 && d->jdir == 0) continue;
+            rhs_comp(comp, phi, press, urk2.data(), vrk2.data(), wrk2.data());
+            std::vector<double>& df = *dif[comp];
+            vel_loop(d, comp, [&](long c, int, int, int) { df[c] = in[comp][c]; });
+            ghost(d, df.data(), comp, comp == 1 ? 10 : comp == 2 ? 11 : 12);
+            double* t = tgt[comp];
+            vel_loop(d, comp, [&](long c, int, int, int) {
+                t[c] = (1.0 / 3.0) * t[c] + (2.0 / 3.0) * df[c] + (2.0 / 3.0) * dt * 1.0 * F[c];
+            });
+        }
+    }
+};
+
+// levelset_RK3::start (levelset_RK3.cpp:103-160), phi transport stages only
+void phi_rk3(orc_domain* d, double* ls, const double* u, const double* v, const double* w)
+{
+    const long n = d->sx * d->sy * d->sz;
+    const double dt = d->prm.dt;
+    std::vector<double> L(n, 0.0), ark1(n, 0.0), ark2(n, 0.0);
+    cell_loop(d, [&](long c) { L[c] = 0.0; });
+    hj(d, ls, u, v, w, L.data());
+    cell_loop(d, [&](long c) { ark1[c] = ls[c] + dt * L[c]; });
+    ghost(d, ark1.data(), 4, 52);
+    cell_loop(d, [&](long c) { L[c] = 0.0; });
+    hj(d, ark1.data(), u, v, w, L.data());
+    cell_loop(d, [&](long c) { ark2[c] = 0.75 * ls[c] + 0.25 * ark1[c] + 0.25 * dt * L[c]; });
+    ghost(d, ark2.data(), 4, 52);
+    cell_loop(d, [&](long c) { L[c] = 0.0; });
+    hj(d, ark2.data(), u, v, w, L.data());
+    cell_loop(d, [&](long c) { ls[c] = (1.0 / 3.0) * ls[c] + (2.0 / 3.0) * ark2[c] + (2.0 / 3.0) * dt * L[c]; });
+    ghost(d, ls, 4, 52);
+}
+
+// one level-set benchmark step: driver_loop_cfd.cpp:82-96 with levelset_RK3 (F30=3),
+// momentum_RK3 with void convection/diffusion/pressure (momentum_RK3.cpp:70-326) and
+// benchmark_vortex3D::start (benchmark_vortex3D.cpp:67-101)
+void vortex_step(orc_domain* d, double* phi, double* u, double* v, double* w, double* ro, double* visc,
+                 double simtime, double* vol)
+{
+    const long n = d->sx * d->sy * d->sz;
+    const double dt = d->prm.dt;
+    phi_rk3(d, phi, u, v, w);
+    reini(d, phi, 3);
+    fluid_update(d, phi, ro, visc, vol);
+    fluid_update(d, phi, ro, visc, vol);
+    // momentum_RK3 with void convection, diffusion and pressure: per component
+    // F = 0 + g (isource zeroes it, irhs adds gravity), then the three RK stages, the
+    // zero body forcing and the ghost fill of momentum_RK3.cpp:70-326.  The benchmark
+    // below overwrites u, v, w inside and refills u, v ghosts but not w's, so w's ghost
+    // cells keep what this fill wrote.
+    {
+        double* comp[4] = {nullptr, u, v, w};
+        for(int cpt = 1; cpt <= 3; ++cpt)
+        {
+            if(cpt == 2 && d->jdir == 0) continue;
+            double* q = comp[cpt];
+            int gcv = cpt == 1 ? 10 : cpt == 2 ? 11 : 12;
+            double g = cpt == 1 ? d->prm.gi : cpt == 2 ? d->prm.gj : d->prm.gk;
+            const double Fv = 0.0 + (0.0 + g + 0.0 + 0.0) * 1.0;
+            std::vector<double> a1(q, q + n), a2(q, q + n);
+            vel_loop(d, cpt, [&](long c, int, int, int) { a1[c] = q[c] + dt * 1.0 * Fv; });
+            vel_loop(d, cpt, [&](long c, int, int, int) { a1[c] += 1.0 * dt * 1.0 * 0.0; });
+            vel_loop(d, cpt, [&](long c, int, int, int) { a2[c] = 0.75 * q[c] + 0.25 * a1[c] + 0.25 * dt * 1.0 * Fv; });
+            vel_loop(d, cpt, [&](long c, int, int, int) { a2[c] += 0.25 * dt * 1.0 * 0.0; });
+            vel_loop(d, cpt, [&](long c, int, int, int) {
+                q[c] = (1.0 / 3.0) * q[c] + (2.0 / 3.0) * a2[c] + (2.0 / 3.0) * dt * 1.0 * Fv;
+            });
+            vel_loop(d, cpt, [&](long c, int, int, int) { q[c] += (2.0 / 3.0) * dt * 1.0 * 0.0; });
+            ghost(d, q, cpt, gcv);
+        }
+    }
+    // benchmark_vortex3D::start: prescribed field at simtime (before the increment)
+    const double DXM = d->DXM;
+    vel_loop(d, 1, [&](long c, int i, int j, int k) {
+        double xc = d->P[0][i + MARGE] + 0.5 * DXM, yc = d->P[1][j + MARGE], zc = d->P[2][k + MARGE];
+        u[c] = 2.0 * pow(sin(PI_REF * xc), 2.0) * sin(2.0 * PI_REF * yc) * sin(2.0 * PI_REF * zc) * cos((PI_REF * simtime) / 3.0);
+    });
+    vel_loop(d, 2, [&](long c, int i, int j, int k) {
+        double xc = d->P[0][i + MARGE], yc = d->P[1][j + MARGE] + 0.5 * DXM, zc = d->P[2][k + MARGE];
+        v[c] = -sin(2.0 * PI_REF * xc) * pow(sin(PI_REF * yc), 2.0) * sin(2.0 * PI_REF * zc) * cos((PI_REF * simtime) / 3.0);
+    });
+    vel_loop(d, 3, [&](long c, int i, int j, int k) {
+        double xc = d->P[0][i + MARGE], yc = d->P[1][j + MARGE], zc = d->P[2][k + MARGE] + 0.5 * DXM;
+        w[c] = -sin(2.0 * PI_REF * xc) * sin(2.0 * PI_REF * yc) * pow(sin(PI_REF * zc), 2.0) * cos((PI_REF * simtime) / 3.0);
+    });
+    ghost(d, u, 1, 10);
+    ghost(d, v, 2, 11);
+    ghost(d, v, 2, 12);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C API
+extern "C" {
+
+orc_domain* orc_create(int nx, int ny, int nz, int jdir, const double* XN, const double* YN, const double* ZN,
+                       double DYM, double DZM, const int* group7, const orc_params* prm)
+{
+    orc_domain* d = new orc_domain();
+    d->nx = nx; d->ny = ny; d->nz = nz; d->jdir = jdir;
+    d->sx = nx + 6; d->sy = ny + 6; d->sz = nz + 6;
+    for(int s = 0; s < 7; ++s) d->group[s] = group7[s];
+    d->prm = *prm;
+    const double* src[3] = {XN, YN, ZN};
+    for(int a = 0; a < 3; ++a)
+    {
+        int n = d->nn(a);
+        d->X[a].assign(n + 1 + 4 * MARGE, 0.0);
+        for(int i = 0; i < n + 1 + 2 * MARGE; ++i) d->X[a][i] = src[a][i];
+    }
+    d->DYM = DYM;
+    d->DZM = DZM;
+    gridspacing(d);
+    build_tables(d);
+    // inipsi (ini_psi.cpp:28-36)
+    if(jdir == 0) d->psi = prm->F45 * (1.0 / 2.0) * (d->DXM + d->DZM);
+    else d->psi = prm->F45 * (1.0 / 3.0) * (d->DXM + d->DYM + d->DZM);
+    d->ydir = jdir == 1 ? 1.0 : 0.0;
+    d->cellnum = (long)nx * ny * nz;
+    d->last[0] = 0; d->last[1] = d->last[2] = d->last[3] = 1;   // lexer::vellast, no neighbours
+    build_lists(d);
+    return d;
+}
+
+void orc_destroy(orc_domain* d) { delete d; }
+
+long orc_field_size(const orc_domain* d) { return d->sx * d->sy * d->sz; }
+
+void orc_grid_array(const orc_domain* d, int which, int axis, double* out)
+{
+    const std::vector<double>& v = which == 0 ? d->P[axis] : which == 1 ? d->DN[axis] : d->DP[axis];
+    int n = d->nn(axis) + 2 * MARGE;
+    for(int i = 0; i < n; ++i) out[i] = v[i];
+}
+
+void orc_tables(const orc_domain* d, int axis, double* qf, double* cf, double* isf)
+{
+    int n = d->nn(axis);
+    for(int ip = 4; ip < n + 6; ++ip)
+    {
+        for(int r = 0; r < 24; ++r) qf[(ip - 4) * 24 + r] = d->qf[axis][(size_t)ip * 24 + r];
+        for(int r = 0; r < 12; ++r) cf[(ip - 4) * 12 + r] = d->cf[axis][(size_t)ip * 12 + r];
+        for(int r = 0; r < 36; ++r) isf[(ip - 4) * 36 + r] = d->isf[axis][(size_t)ip * 36 + r];
+    }
+}
+
+void orc_scalars(const orc_domain* d, double* out)
+{
+    out[0] = d->DXM; out[1] = d->DYM; out[2] = d->DZM; out[3] = d->psi; out[4] = double(d->cellnum);
+}
+
+void orc_ghost(orc_domain* d, double* f, int kind, int gcv) { ghost(d, f, kind, gcv); }
+
+void orc_hj(orc_domain* d, const double* phi, const double* u, const double* v, const double* w, double* L)
+{
+    hj(d, phi, u, v, w, L);
+}
+
+void orc_flux(orc_domain* d, int ipol, const double* q, const double* u, const double* v, const double* w, double* acc)
+{
+    flux(d, ipol, q, u, v, w, acc);
+}
+
+void orc_reini(orc_domain* d, double* phi, int iters) { reini(d, phi, iters); }
+
+void orc_fluid_update(orc_domain* d, const double* phi, double* ro, double* visc, double* vol2)
+{
+    fluid_update(d, phi, ro, visc, vol2);
+}
+
+void orc_poisson(orc_domain* d, const double* phi, const double* pcorr, double* M7, double* rhs)
+{
+    poisson(d, phi, pcorr, M7, rhs);
+}
+
+int orc_bicgstab(orc_domain* d, double* x, const double* M7, const double* rhs, double stop, int maxiter, double* residual)
+{
+    return bicgstab(d, x, M7, rhs, stop, maxiter, residual);
+}
+
+int orc_pjm(orc_domain* d, const double* phi, double* u, double* v, double* w, double* press, double* pcorr, double alpha)
+{
+    return pjm(d, phi, u, v, w, press, pcorr, alpha);
+}
+
+int orc_fc3_step(orc_domain* d, double* phi, double* u, double* v, double* w, double* press, double* ro, double* visc,
+                 double* vol2)
+{
+    FC3 s(d);
+    return s.step(phi, u, v, w, press, ro, visc, vol2);
+}
+
+void orc_vortex_step(orc_domain* d, double* phi, double* u, double* v, double* w, double* ro, double* visc,
+                     double simtime, double* vol2)
+{
+    vortex_step(d, phi, u, v, w, ro, visc, simtime, vol2);
+}
+
+void orc_phi_rk3(orc_domain* d, double* phi, const double* u, const double* v, const double* w)
+{
+    phi_rk3(d, phi, u, v, w);
+}
+
+}  // extern "C"

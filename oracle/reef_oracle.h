@@ -1,0 +1,89 @@
+/* CPU oracle for the REEF3D free-surface hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain C++ restatement of the reference's CPU
+ * algorithm (REEF3D src/, cited file:line in reef_oracle.cpp), used by tests/ as the
+ * parity checker and by bench.py as the timed CPU baseline.  Nothing in the product
+ * (reef3d_amd/) includes, links or calls it.
+ *
+ * Field layout (same as the GPU code): double[(nz+6)*(ny+6)*(nx+6)], i fastest,
+ * 3 ghost layers: f[(i+3) + (nx+6)*((j+3) + (ny+6)*(k+3))] = field(i,j,k).
+ */
+#ifndef REEF_ORACLE_H
+#define REEF_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_domain orc_domain;
+
+typedef struct orc_params
+{
+    double rho_water, visc_water;   /* W1, W2 (kinematic) */
+    double rho_air, visc_air;       /* W3, W4 */
+    double gi, gj, gk;              /* W20, W21, W22 */
+    double F43, F45;                /* reinit time step factor, epsilon factor */
+    double dt;                      /* fixed time step (N49) */
+    double N44;                     /* pressure stop criterion on ||r||_2 / cells */
+    int N46;                        /* max BiCGStab iterations */
+} orc_params;
+
+/* Box domain: nx x ny x nz cells, node coordinates XN/YN/ZN with 5 ghost nodes per side
+ * (n+11 values each), DYM/DZM as stored in the grid file header, group[s] = boundary
+ * group of side s (index 1..6: -x, +y, -y, +x, -z, +z; 21 wall, 3 free-surface top). */
+orc_domain* orc_create(int nx, int ny, int nz, int jdir, const double* XN, const double* YN,
+                       const double* ZN, double DYM, double DZM, const int* group7,
+                       const orc_params* prm);
+void orc_destroy(orc_domain*);
+
+/* grid metadata: which = 0 XP,1 DXN,2 DXP (axis 0..2; n+10 values),
+ * tables: orc_tables(d, axis, qf[(n+2)*24], cf[(n+2)*12], isf[(n+2)*36]) for IP=4..n+5 */
+void orc_grid_array(const orc_domain*, int which, int axis, double* out);
+void orc_tables(const orc_domain*, int axis, double* qf, double* cf, double* isf);
+/* [DXM, DYM, DZM, psi, cellnum] */
+void orc_scalars(const orc_domain*, double* out5);
+
+/* ghostcell::start1..4(f, gcv) */
+void orc_ghost(orc_domain*, double* f, int kind, int gcv);
+
+/* convection: L += HJ-WENO (ipol 4); acc += WENO flux divergence (ipol 1,2,3) */
+void orc_hj(orc_domain*, const double* phi, const double* u, const double* v, const double* w,
+            double* L);
+void orc_flux(orc_domain*, int ipol, const double* q, const double* u, const double* v,
+              const double* w, double* acc);
+
+/* reini_RK3::start with count>0, iters pseudo steps (phi in place, ghosts refilled) */
+void orc_reini(orc_domain*, double* phi, int iters);
+
+/* fluid_update_fsf: rho, visc from phi; volumes out2[0..1] */
+void orc_fluid_update(orc_domain*, const double* phi, double* ro, double* visc, double* vol2);
+
+/* poisson_pcorr::start on (phi) -> 7 coefficient arrays + rhs (LOOP order, cellnum each) */
+void orc_poisson(orc_domain*, const double* phi, const double* pcorr, double* M7, double* rhs);
+/* bicgstab_ijk::start(var=5): x (field layout, in/out), M7 + rhs as from orc_poisson */
+int orc_bicgstab(orc_domain*, double* x, const double* M7, const double* rhs, double stop,
+                 int maxiter, double* residual);
+
+/* pjm_corr::start with alpha: u,v,w corrected in place, press updated, pcorr out */
+int orc_pjm(orc_domain*, const double* phi, double* u, double* v, double* w, double* press,
+            double* pcorr, double alpha);
+
+/* one momentum_FC3 time step of the two-phase state (all fields in/out) */
+int orc_fc3_step(orc_domain*, double* phi, double* u, double* v, double* w, double* press,
+                 double* ro, double* visc, double* vol2);
+
+/* one level-set benchmark step (levelset_RK3 + momentum_RK3 with void operators +
+ * benchmark_vortex3D velocity update at simtime); returns volumes */
+void orc_vortex_step(orc_domain*, double* phi, double* u, double* v, double* w, double* ro,
+                     double* visc, double simtime, double* vol2);
+
+/* HJ-WENO phi transport TVD-RK3 only (levelset_RK3.cpp:109-160, no reinit) */
+void orc_phi_rk3(orc_domain*, double* phi, const double* u, const double* v, const double* w);
+
+/* number of cells of the field layout */
+long orc_field_size(const orc_domain*);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
