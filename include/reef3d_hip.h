@@ -1,0 +1,136 @@
+/* reef3d_hip.h - C-ABI of the MI355X (gfx950) implementation of REEF3D's CFD
+ * free-surface hot path: WENO5 flux convection of u, v, w, WENO5 Hamilton-Jacobi
+ * convection of the level set phi, TVD-RK3 reinitialisation, ghost-cell fill, the
+ * variable-density pressure projection with Jacobi-preconditioned BiCGStab, and the
+ * momentum_FC3 / levelset_RK3 time steps that orchestrate them.
+ *
+ * Conventions: every call returns 0 or a negative REEF_E* status and never exits.
+ * Host pointers are borrowed for the duration of the call.  Device fields belong to
+ * the context; all work is ordered on the context's stream.  One context is used by
+ * one host thread at a time.
+ *
+ * Field layouts on the host side of reef_field_put/get:
+ *   REEF_LAYOUT_IFAST: double[(nz+6)*(ny+6)*(nx+6)], i fastest, 3 ghost layers
+ *                      (element (i,j,k) at (i+3) + (nx+6)*((j+3) + (ny+6)*(k+3)))
+ *   REEF_LAYOUT_REF:   the reference's field_base storage (field_base.h:47),
+ *                      V[(i+3)*jmax*kmax + (j+3)*kmax + (k+3)], k fastest,
+ *                      jmax = ny+6, kmax = nz+6.
+ */
+#ifndef REEF3D_HIP_H
+#define REEF3D_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum reef_status
+{
+    REEF_OK = 0,
+    REEF_E_ARG = -1,        /* invalid argument (null pointer, bad id, bad size) */
+    REEF_E_UNSUPPORTED = -2,/* geometry or option outside the implemented subset */
+    REEF_E_HIP = -3,        /* HIP runtime error (message via reef_last_error) */
+    REEF_E_NOMEM = -4       /* device allocation failed */
+};
+
+enum reef_field_id
+{
+    REEF_U = 0, REEF_V, REEF_W, REEF_PRESS, REEF_PHI, REEF_RO, REEF_VISC,
+    REEF_F, REEF_G, REEF_H, REEF_L,     /* convection accumulators (fdm::F,G,H,L) */
+    REEF_PCORR,                          /* pressure correction (pjm_corr::pcorr)  */
+    REEF_WORK0, REEF_WORK1, REEF_WORK2, REEF_WORK3,   /* caller scratch fields     */
+    REEF_NFIELDS
+};
+
+enum reef_layout { REEF_LAYOUT_IFAST = 0, REEF_LAYOUT_REF = 1 };
+
+/* Grid of one box domain: what lexer holds after read_grid + gridini
+ * (read_grid.cpp:75-432, grid.cpp:54-150). */
+typedef struct reef_grid_desc
+{
+    int knox, knoy, knoz;       /* interior cells */
+    int j_dir;                  /* 1: 3-D, 0: 2-D x-z tank (knoy must be 1) */
+    const double* XN;           /* node coordinates incl. 5 ghost nodes per side: knox+11 */
+    const double* YN;           /* knoy+11 */
+    const double* ZN;           /* knoz+11 */
+    double DYM, DZM;            /* grid-file header spacings (used by the psi width) */
+    int group[7];               /* boundary group per side 1..6 (-x,+y,-y,+x,-z,+z):
+                                   21 wall, 3 free-surface top; index 0 unused */
+} reef_grid_desc;
+
+/* Control values of the hot path (ini.cpp defaults unless the case sets them). */
+typedef struct reef_params
+{
+    double W1, W2, W3, W4;      /* water/air density and kinematic viscosity */
+    double W20, W21, W22;       /* gravity components */
+    double F43, F45;            /* reinit pseudo time step factor, interface width factor */
+    double dt;                  /* fixed time step (N48=0, N49) */
+    double N44;                 /* pressure stop criterion on ||r||_2 / cells */
+    int N46;                    /* maximum BiCGStab iterations */
+} reef_params;
+
+typedef struct reef_ctx reef_ctx;
+
+/* lexer + fdm + ghostcell set-up on the device (replaces driver.cpp:33-118 and
+ * driver_makegrid.cpp:29-62 for a box domain); allocates all fields, zeroed. */
+int reef_ctx_create(const reef_grid_desc* grid, const reef_params* prm, reef_ctx** out);
+void reef_ctx_destroy(reef_ctx* ctx);
+const char* reef_last_error(void);
+
+/* host <-> device field transfer (fdm field storage, field_base.h:28-54) */
+int reef_field_put(reef_ctx* ctx, int field_id, const double* host, int layout);
+int reef_field_get(reef_ctx* ctx, int field_id, double* host, int layout);
+int reef_field_zero(reef_ctx* ctx, int field_id);
+
+/* ghostcell::start1..4(f, gcv) (gc_start.cpp:27-182): kind 1..4 = u, v, w, scalar */
+int reef_ghost_fill(reef_ctx* ctx, int field_id, int kind, int gcv);
+
+/* convection::start(p, a, q, ipol, u, v, w) (convection.h:37): ipol 1..3 accumulate the
+ * WENO flux divergence (weno_flux_nug.cpp:84-134) into REEF_F/G/H, ipol 4 the HJ-WENO
+ * term (weno_hj_df_nug.cpp:58-108) into REEF_L; nothing is zeroed first. */
+int reef_convection(reef_ctx* ctx, int ipol, int q_id, int u_id, int v_id, int w_id);
+
+/* reini::start(a, p, phi, pgc, pflow) (reini.h:39, reini_RK3.cpp:77-145) with
+ * p->count > 0: iters TVD-RK3 pseudo steps of reinidisc_f, in place, ghosts refilled. */
+int reef_reini(reef_ctx* ctx, int phi_id, int iters);
+
+/* fluid_update_fsf::start (fluid_update_fsf.cpp:39-104): REEF_RO/REEF_VISC from
+ * REEF_PHI, ghosts refilled; vol2 (may be NULL) receives volume1, volume2. */
+int reef_fluid_update(reef_ctx* ctx, double* vol2);
+
+/* poisson::start + solver::start (poisson_pcorr.cpp:78-231, bicgstab_ijk.cpp:51-278):
+ * assembles the 7-point operator on REEF_PHI and solves A x = rhs for field x_id.
+ * rhs_id holds the right-hand side in field layout (interior cells). */
+int reef_poisson_solve(reef_ctx* ctx, int x_id, int rhs_id, double stop, int maxiter,
+                       int* iters, double* residual);
+
+/* pressure::start(a, p, ppois, psolv, pgc, pflow, u, v, w, alpha) with pjm_corr
+ * (pjm_corr.cpp:90-121): projection of the three velocity fields, REEF_PRESS updated. */
+int reef_pjm_corr(reef_ctx* ctx, int u_id, int v_id, int w_id, double alpha, int* iters);
+
+/* one momentum_FC3 time step (momentum_FC3.cpp:122-430) of the two-phase state held in
+ * REEF_U/V/W/PRESS/PHI/RO/VISC; iters receives the summed BiCGStab iterations. */
+int reef_fc3_step(reef_ctx* ctx, int* iters, double* vol2);
+
+/* one level-set benchmark step (levelset_RK3.cpp:103-178 + benchmark_vortex3D.cpp:67-101,
+ * F150=3) at time simtime before the increment. */
+int reef_vortex_step(reef_ctx* ctx, double simtime, double* vol2);
+
+/* levelset_RK3 transport stages only (levelset_RK3.cpp:109-160) on REEF_PHI */
+int reef_phi_rk3(reef_ctx* ctx);
+
+int reef_sync(reef_ctx* ctx);
+
+/* Kernel timing: the context records HIP events around its launches when enabled. */
+int reef_timing_enable(reef_ctx* ctx, int on);
+/* name: kernel family ("bicgstab", "hj", "reini", "flux", ...); returns total ms and launches */
+int reef_timing_get(reef_ctx* ctx, const char* name, double* ms, long* launches);
+int reef_timing_reset(reef_ctx* ctx);
+
+/* sizes: out[0..4] = interior cells, field doubles allocated per field, row stride,
+ * plane stride, offset of element (-3,-3,-3) */
+int reef_ctx_sizes(reef_ctx* ctx, long* out5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

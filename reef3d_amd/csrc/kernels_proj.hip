@@ -1,0 +1,619 @@
+// Ghost fill, fluid properties, pressure projection and the Jacobi-preconditioned
+// BiCGStab of bicgstab_ijk on the device.
+//
+// Reductions are one-pass: every block writes its partial sums, the last block to
+// arrive (ticket counter) adds the partials in block order and updates the solver state
+// in device memory, so the iteration never waits on the host.  Partial sums are formed
+// in a fixed order, so results are run-to-run deterministic (they differ from the
+// reference's serial i-j-k sum only by rounding).
+#include "reef_internal.h"
+
+namespace reef {
+
+constexpr int TX = 64, TY = 4, NT = 256;
+
+__device__ __forceinline__ bool tile_of_p(int nbx, int nby, int nbtot, int& bx, int& by, int& bz)
+{
+    const unsigned per = gridDim.x >> 3;
+    const unsigned b = blockIdx.x;
+    const unsigned lin = (b & 7u) * per + (b >> 3);
+    if(lin >= (unsigned)nbtot) return false;
+    bz = lin / (nbx * nby);
+    const unsigned rem = lin - (unsigned)bz * nbx * nby;
+    by = rem / nbx;
+    bx = rem - (unsigned)by * nbx;
+    return true;
+}
+
+static inline dim3 tiles(const DevGrid& g, int& nbx, int& nby, int& nbtot)
+{
+    nbx = (g.nx + TX - 1) / TX;
+    nby = (g.ny + TY - 1) / TY;
+    nbtot = nbx * nby * g.nz;
+    return dim3((unsigned)((nbtot + 7) / 8 * 8));
+}
+
+// cell of this thread; false outside the interior or in padding blocks
+__device__ __forceinline__ bool cell(const DevGrid& g, int nbx, int nby, int nbtot, int& i, int& j, int& k, long& c)
+{
+    int bx, by, bz;
+    if(!tile_of_p(nbx, nby, nbtot, bx, by, bz)) return false;
+    i = bx * TX + (threadIdx.x & (TX - 1));
+    j = by * TY + (threadIdx.x >> 6);
+    k = bz;
+    if(i >= g.nx || j >= g.ny) return false;
+    c = g.base + i + g.sx * j + g.sxy * k;
+    return true;
+}
+
+// Block partial sums of M values, fixed tree order; partials[blockIdx.x*M + m].
+// Returns true in every thread of the last block to finish; that block then holds the
+// totals in tot[] (all threads).
+template<int M>
+__device__ __forceinline__ bool reduce_last(const double (&v)[M], double* partials, unsigned int* ticket,
+                                            double (&tot)[M])
+{
+    __shared__ double sh[M][NT];
+    __shared__ bool amlast;
+    const int t = threadIdx.x;
+    for(int m = 0; m < M; ++m) sh[m][t] = v[m];
+    __syncthreads();
+    for(int s = NT / 2; s > 0; s >>= 1)
+    {
+        if(t < s)
+            for(int m = 0; m < M; ++m) sh[m][t] += sh[m][t + s];
+        __syncthreads();
+    }
+    if(t == 0)
+    {
+        for(int m = 0; m < M; ++m) partials[(size_t)blockIdx.x * M + m] = sh[m][0];
+        __threadfence();
+        const unsigned int prev = atomicAdd(ticket, 1u);
+        amlast = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if(!amlast) return false;
+    __threadfence();
+    const volatile double* P = partials;
+    double acc[M];
+    for(int m = 0; m < M; ++m) acc[m] = 0.0;
+    for(unsigned b = t; b < gridDim.x; b += NT)
+        for(int m = 0; m < M; ++m) acc[m] += P[(size_t)b * M + m];
+    for(int m = 0; m < M; ++m) sh[m][t] = acc[m];
+    __syncthreads();
+    for(int s = NT / 2; s > 0; s >>= 1)
+    {
+        if(t < s)
+            for(int m = 0; m < M; ++m) sh[m][t] += sh[m][t + s];
+        __syncthreads();
+    }
+    for(int m = 0; m < M; ++m) tot[m] = sh[m][0];
+    if(t == 0) *ticket = 0u;
+    return true;
+}
+
+// ------------------------------------------------------------------ ghost program
+__global__ void k_ghost(const GhostOp* __restrict__ ops, int nops, double* __restrict__ f)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if(q >= nops) return;
+    const GhostOp op = ops[q];
+    double val;
+    if(op.src < 0)
+        val = 0.0;
+    else if(op.src & (1 << 30))
+        val = 0.0 + op.w * f[op.src & ~(1 << 30)];
+    else
+        val = f[op.src];
+    f[op.dst] = val;
+}
+
+void launch_ghost(const GhostOp* ops, int nops, double* f, hipStream_t s)
+{
+    if(nops <= 0) return;
+    hipLaunchKernelGGL(k_ghost, dim3((nops + 255) / 256), dim3(256), 0, s, ops, nops, f);
+}
+
+// ------------------------------------------------------------------ simple kernels
+__global__ void k_copy_interior(DevGrid g, int nbx, int nby, int nbtot, const double* __restrict__ src,
+                                double* __restrict__ dst)
+{
+    int i, j, k;
+    long c;
+    if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
+    dst[c] = src[c];
+}
+
+void launch_copy_interior(const DevGrid& g, const double* src, double* dst, hipStream_t s)
+{
+    int nbx, nby, nbtot;
+    dim3 grid = tiles(g, nbx, nby, nbtot);
+    hipLaunchKernelGGL(k_copy_interior, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, src, dst);
+}
+
+__global__ void k_fill(double* f, long n, double v)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if(q < n) f[q] = v;
+}
+
+void launch_fill(double* f, long n, double v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, n, v);
+}
+
+// ------------------------------------------------------------------ fluid properties
+// fluid_update_fsf::start (fluid_update_fsf.cpp:39-74): smoothed Heaviside, rho, nu and
+// the volume sums (porosity 1)
+__global__ void __launch_bounds__(NT) k_heaviside(DevGrid g, int nbx, int nby, int nbtot, const double* __restrict__ phi,
+                                                  double* __restrict__ ro, double* __restrict__ visc, double rw,
+                                                  double vw, double ra, double va, double* partials, double* vol_out,
+                                                  unsigned int* ticket)
+{
+    int i, j, k;
+    long c;
+    double vals[2] = {0.0, 0.0};
+    if(cell(g, nbx, nby, nbtot, i, j, k, c))
+    {
+        const double p = phi[c];
+        const double psi = g.psi;
+        double H = 0.0;
+        if(p > psi) H = 1.0;
+        if(p < -psi) H = 0.0;
+        if(fabs(p) <= psi) H = 0.5 * (1.0 + p / psi + (1.0 / PI_REF) * sin((PI_REF * p) / psi));
+        ro[c] = rw * H + ra * (1.0 - H);
+        visc[c] = vw * H + va * (1.0 - H);
+        const double vol = g.DXN[i + MARGE] * g.DYN[j + MARGE] * g.DZN[k + MARGE];
+        vals[0] = vol * (H - (1.0 - 1.0));
+        vals[1] = vol * (1.0 - H - (1.0 - 1.0));
+    }
+    double tot[2];
+    if(reduce_last<2>(vals, partials, ticket, tot) && threadIdx.x == 0)
+    {
+        vol_out[0] = tot[0];
+        vol_out[1] = tot[1];
+    }
+}
+
+void launch_heaviside(const DevGrid& g, const double* phi, double* ro, double* visc, const double* prm4,
+                      double* partials, double* vol_out, unsigned int* ticket, hipStream_t s)
+{
+    int nbx, nby, nbtot;
+    dim3 grid = tiles(g, nbx, nby, nbtot);
+    hipLaunchKernelGGL(k_heaviside, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, phi, ro, visc, prm4[0], prm4[1], prm4[2],
+                       prm4[3], partials, vol_out, ticket);
+}
+
+// ------------------------------------------------------------------ projection
+// pjm_corr::rhs (pjm_corr.cpp:151-176): -div(u*)/(alpha dt), in field layout
+__global__ void k_poisson_rhs(DevGrid g, int nbx, int nby, int nbtot, const double* __restrict__ u,
+                              const double* __restrict__ v, const double* __restrict__ w, double adt,
+                              double* __restrict__ rhs)
+{
+    int i, j, k;
+    long c;
+    if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
+    rhs[c] = -(u[c] - u[c - 1]) / (adt * g.DXN[i + MARGE]) - (v[c] - v[c - g.sx]) / (adt * g.DYN[j + MARGE]) * g.ydir
+           - (w[c] - w[c - g.sxy]) / (adt * g.DZN[k + MARGE]);
+}
+
+void launch_poisson_rhs(const DevGrid& g, const double* u, const double* v, const double* w, double adt, double* rhs,
+                        hipStream_t s)
+{
+    int nbx, nby, nbtot;
+    dim3 grid = tiles(g, nbx, nby, nbtot);
+    hipLaunchKernelGGL(k_poisson_rhs, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, u, v, w, adt, rhs);
+}
+
+__device__ __forceinline__ double roface(double a, double b, double psi, double rw, double ra)
+{
+    const double phival = 0.5 * (a + b);
+    double H = 0.0;
+    if(phival > psi) H = 1.0;
+    if(phival < -psi) H = 0.0;
+    if(fabs(phival) <= psi) H = 0.5 * (1.0 + phival / psi + (1.0 / PI_REF) * sin((PI_REF * phival) / psi));
+    return rw * H + ra * (1.0 - H);
+}
+
+// Couplings of poisson_pcorr.cpp:95-123 before the boundary folding: T[0..5] for the
+// +x, -x, +y, -y, +z, -z faces; face densities density_f::roface stored per "+" face.
+struct Coupling
+{
+    double T[6];
+};
+
+__device__ __forceinline__ Coupling couplings(const DevGrid& g, const double* __restrict__ rfx,
+                                              const double* __restrict__ rfy, const double* __restrict__ rfz, long c,
+                                              int i, int j, int k)
+{
+    const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
+    Coupling q;
+    q.T[0] = (1.0 * 1.0) / (rfx[c] * g.DXP[ip] * g.DXN[ip]);
+    q.T[1] = (1.0 * 1.0) / (rfx[c - 1] * g.DXP[ip - 1] * g.DXN[ip]);
+    q.T[2] = (1.0 * 1.0) / (rfy[c] * g.DYP[jp] * g.DYN[jp]);
+    q.T[3] = (1.0 * 1.0) / (rfy[c - g.sx] * g.DYP[jp - 1] * g.DYN[jp]);
+    q.T[4] = (1.0 * 1.0) / (rfz[c] * g.DZP[kp] * g.DZN[kp]);
+    q.T[5] = (1.0 * 1.0) / (rfz[c - g.sxy] * g.DZP[kp - 1] * g.DZN[kp]);
+    return q;
+}
+
+__device__ __forceinline__ double diag_of(const DevGrid& g, const Coupling& q)
+{
+    return q.T[0] + q.T[1] + q.T[2] * g.ydir + q.T[3] * g.ydir + q.T[4] + q.T[5];
+}
+
+// y = A x at cell c with the folded operator of poisson_pcorr.cpp:124-230: boundary
+// neighbours (pressure-correction ghosts are 0) drop out, the diagonal keeps them.
+// Summation order of matvec_std (bicgstab_ijk.cpp:296-310); 5 points in 2-D
+// (bicgstab_ijk_2D.cpp:308-320).
+__device__ __forceinline__ double apply_A(const DevGrid& g, const Coupling& q, const double* __restrict__ x, long c,
+                                          int i, int j, int k)
+{
+    const double Mp = diag_of(g, q);
+    const double Mn = i == g.nx - 1 ? 0.0 : -q.T[0];
+    const double Ms = i == 0 ? 0.0 : -q.T[1];
+    const double Mt = k == g.nz - 1 ? 0.0 : -q.T[4];
+    const double Mb = k == 0 ? 0.0 : -q.T[5];
+    const double xn = i == g.nx - 1 ? 0.0 : x[c + 1];
+    const double xs = i == 0 ? 0.0 : x[c - 1];
+    const double xt = k == g.nz - 1 ? 0.0 : x[c + g.sxy];
+    const double xb = k == 0 ? 0.0 : x[c - g.sxy];
+    if(g.jdir == 1)
+    {
+        const double Mw = j == g.ny - 1 ? 0.0 : -q.T[2] * g.ydir;
+        const double Me = j == 0 ? 0.0 : -q.T[3] * g.ydir;
+        const double xw = j == g.ny - 1 ? 0.0 : x[c + g.sx];
+        const double xe = j == 0 ? 0.0 : x[c - g.sx];
+        return Mp * x[c] + Mn * xn + Ms * xs + Mw * xw + Me * xe + Mt * xt + Mb * xb;
+    }
+    return Mp * x[c] + Mn * xn + Ms * xs + Mt * xt + Mb * xb;
+}
+
+// face densities of every "+" face of the interior cells and of the -1 layer faces, plus
+// the Jacobi preconditioner aii = -1/(M.p + 1e-19) (bicgstab_ijk.cpp:356-364)
+__global__ void k_face_rho(DevGrid g, int nbx, int nby, int nbtot, const double* __restrict__ phi, double rw,
+                           double ra, double* __restrict__ rfx, double* __restrict__ rfy, double* __restrict__ rfz)
+{
+    int i, j, k;
+    long c;
+    if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
+    rfx[c] = roface(phi[c], phi[c + 1], g.psi, rw, ra);
+    rfy[c] = roface(phi[c], phi[c + g.sx], g.psi, rw, ra);
+    rfz[c] = roface(phi[c], phi[c + g.sxy], g.psi, rw, ra);
+    if(i == 0) rfx[c - 1] = roface(phi[c], phi[c - 1], g.psi, rw, ra);
+    if(j == 0) rfy[c - g.sx] = roface(phi[c], phi[c - g.sx], g.psi, rw, ra);
+    if(k == 0) rfz[c - g.sxy] = roface(phi[c], phi[c - g.sxy], g.psi, rw, ra);
+}
+
+__global__ void k_aii(DevGrid g, int nbx, int nby, int nbtot, const double* __restrict__ rfx,
+                      const double* __restrict__ rfy, const double* __restrict__ rfz, double* __restrict__ aii)
+{
+    int i, j, k;
+    long c;
+    if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
+    const Coupling q = couplings(g, rfx, rfy, rfz, c, i, j, k);
+    aii[c] = -1.0 / (diag_of(g, q) + 1e-19);
+}
+
+void launch_face_rho(const DevGrid& g, const double* phi, double rw, double ra, double* rfx, double* rfy, double* rfz,
+                     double* aii, hipStream_t s)
+{
+    int nbx, nby, nbtot;
+    dim3 grid = tiles(g, nbx, nby, nbtot);
+    hipLaunchKernelGGL(k_face_rho, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, phi, rw, ra, rfx, rfy, rfz);
+    if(aii) hipLaunchKernelGGL(k_aii, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, rfx, rfy, rfz, aii);
+}
+
+// pjm_corr::presscorr + ucorr/vcorr/wcorr (pjm_corr.cpp:123-149)
+__global__ void k_correct(DevGrid g, int nbx, int nby, int nbtot, const double* __restrict__ pcorr,
+                          const double* __restrict__ phi, double rw, double ra, double adt, double* __restrict__ u,
+                          double* __restrict__ v, double* __restrict__ w, double* __restrict__ press)
+{
+    int i, j, k;
+    long c;
+    if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
+    press[c] += pcorr[c];
+    if(i < g.nx - 1)
+        u[c] -= adt * 1.0 * 1.0 * ((pcorr[c + 1] - pcorr[c]) / (g.DXP[i + MARGE] * roface(phi[c], phi[c + 1], g.psi, rw, ra)));
+    if(g.jdir == 1 && j < g.ny - 1)
+        v[c] -= adt * 1.0 * 1.0
+              * ((pcorr[c + g.sx] - pcorr[c]) / (g.DYP[j + MARGE] * roface(phi[c], phi[c + g.sx], g.psi, rw, ra)));
+    if(k < g.nz - 1)
+        w[c] -= adt * 1.0 * 1.0
+              * ((pcorr[c + g.sxy] - pcorr[c]) / (g.DZP[k + MARGE] * roface(phi[c], phi[c + g.sxy], g.psi, rw, ra)));
+}
+
+void launch_correct(const DevGrid& g, const double* pcorr, const double* phi, double rw, double ra, double adt,
+                    double* u, double* v, double* w, double* press, hipStream_t s)
+{
+    int nbx, nby, nbtot;
+    dim3 grid = tiles(g, nbx, nby, nbtot);
+    hipLaunchKernelGGL(k_correct, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, pcorr, phi, rw, ra, adt, u, v, w, press);
+}
+
+// ------------------------------------------------------------------ BiCGStab
+// State machine of bicgstab_ijk::solve (bicgstab_ijk.cpp:125-278): mode 1 = a restart
+// is pending (res_calc + restart label), 2 = p was just reset to r.
+
+// restart / initial residual: r = b - A x, r0 = p = r, r_j = (r, r0)
+__global__ void __launch_bounds__(NT) k_bicg_restart(DevGrid g, int nbx, int nby, int nbtot, BiBuffers b, int init)
+{
+    BiState* st = b.st;
+    if(st->done) return;
+    if(!init && st->mode != 1) return;
+    int i, j, k;
+    long c;
+    double vals[1] = {0.0};
+    if(cell(g, nbx, nby, nbtot, i, j, k, c))
+    {
+        const Coupling q = couplings(g, b.rfx, b.rfy, b.rfz, c, i, j, k);
+        const double r = b.rhs[c] - (apply_A(g, q, b.x, c, i, j, k));
+        b.r[c] = r;
+        b.r0[c] = r;
+        b.p[0][c] = r;
+        vals[0] = r * r;
+    }
+    double tot[1];
+    if(reduce_last<1>(vals, b.partials, &st->ticket[0], tot) && threadIdx.x == 0)
+    {
+        st->r_j = tot[0];
+        st->norm_r0 = sqrt(tot[0]);
+        if(!init)
+        {
+            st->residual = sqrt(tot[0]) / double(st->ncell);
+            st->iter += 1;
+        }
+        if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
+        st->mode = 2;
+    }
+}
+
+// p = r + beta (p - w v) (bicgstab_ijk.cpp:234-235), ph = aii p (precon_solve)
+__global__ void k_bicg_p(DevGrid g, int nbx, int nby, int nbtot, BiBuffers b)
+{
+    const BiState* st = b.st;
+    if(st->done || st->mode == 1) return;
+    int i, j, k;
+    long c;
+    if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
+    double p = b.p[0][c];
+    if(st->mode != 2) p = b.r[c] + st->beta * (p - st->w * b.v[0][c]);
+    b.p[0][c] = p;
+    b.p[1][c] = p * b.aii[c];
+}
+
+// v = A ph; sigma = (v, r0), |v|, |r|; alpha and the restart tests (bicgstab_ijk.cpp:151-191)
+__global__ void __launch_bounds__(NT) k_bicg_v(DevGrid g, int nbx, int nby, int nbtot, BiBuffers b)
+{
+    BiState* st = b.st;
+    if(st->done || st->mode == 1) return;
+    int i, j, k;
+    long c;
+    double vals[3] = {0.0, 0.0, 0.0};
+    if(cell(g, nbx, nby, nbtot, i, j, k, c))
+    {
+        const Coupling q = couplings(g, b.rfx, b.rfy, b.rfz, c, i, j, k);
+        const double v = apply_A(g, q, b.p[1], c, i, j, k);
+        b.v[0][c] = v;
+        const double r = b.r[c];
+        vals[0] = v * b.r0[c];
+        vals[1] = v * v;
+        vals[2] = r * r;
+    }
+    double tot[3];
+    if(reduce_last<3>(vals, b.partials, &st->ticket[1], tot) && threadIdx.x == 0)
+    {
+        const double sigma = tot[0];
+        const double norm_vj = sqrt(tot[1]);
+        const double norm_rj = sqrt(tot[2]);
+        const double alpha = st->r_j / sigma;
+        st->sigma = sigma;
+        st->norm_vj = norm_vj;
+        st->norm_rj = norm_rj;
+        st->alpha = alpha;
+        if(fabs(sigma) <= (1.0e-12 * (norm_vj * st->norm_r0)))
+            st->mode = 1;
+        else if((fabs(alpha) * norm_vj / (norm_rj == 0 ? 1.0e-15 : norm_rj)) <= 0.08)
+            st->mode = 1;
+        else
+            st->mode = 0;
+    }
+}
+
+// s = r - alpha v, sh = aii s, t = A sh; (t,s), (t,t), (s,s) (bicgstab_ijk.cpp:193-221)
+__global__ void __launch_bounds__(NT) k_bicg_t(DevGrid g, int nbx, int nby, int nbtot, BiBuffers b)
+{
+    BiState* st = b.st;
+    if(st->done || st->mode == 1) return;
+    int i, j, k;
+    long c;
+    double vals[3] = {0.0, 0.0, 0.0};
+    const double alpha = st->alpha;
+    if(cell(g, nbx, nby, nbtot, i, j, k, c))
+    {
+        const double* r = b.r;
+        const double* v = b.v[0];
+        const double* a = b.aii;
+        const Coupling q = couplings(g, b.rfx, b.rfy, b.rfz, c, i, j, k);
+        // sh at the centre and the neighbours (0 outside the domain)
+        auto shv = [&](long cc) { return (r[cc] - alpha * v[cc]) * a[cc]; };
+        const double Mp = diag_of(g, q);
+        const double Mn = i == g.nx - 1 ? 0.0 : -q.T[0];
+        const double Ms = i == 0 ? 0.0 : -q.T[1];
+        const double Mt = k == g.nz - 1 ? 0.0 : -q.T[4];
+        const double Mb = k == 0 ? 0.0 : -q.T[5];
+        const double xn = i == g.nx - 1 ? 0.0 : shv(c + 1);
+        const double xs = i == 0 ? 0.0 : shv(c - 1);
+        const double xt = k == g.nz - 1 ? 0.0 : shv(c + g.sxy);
+        const double xb = k == 0 ? 0.0 : shv(c - g.sxy);
+        const double s = r[c] - alpha * v[c];
+        const double shc = s * a[c];
+        double t;
+        if(g.jdir == 1)
+        {
+            const double Mw = j == g.ny - 1 ? 0.0 : -q.T[2] * g.ydir;
+            const double Me = j == 0 ? 0.0 : -q.T[3] * g.ydir;
+            const double xw = j == g.ny - 1 ? 0.0 : shv(c + g.sx);
+            const double xe = j == 0 ? 0.0 : shv(c - g.sx);
+            t = Mp * shc + Mn * xn + Ms * xs + Mw * xw + Me * xe + Mt * xt + Mb * xb;
+        }
+        else
+            t = Mp * shc + Mn * xn + Ms * xs + Mt * xt + Mb * xb;
+        b.t[c] = t;
+        vals[0] = t * s;
+        vals[1] = t * t;
+        vals[2] = s * s;
+    }
+    double tot[3];
+    if(reduce_last<3>(vals, b.partials, &st->ticket[2], tot) && threadIdx.x == 0)
+    {
+        st->w1 = tot[0];
+        st->w2 = tot[1];
+        st->norm_sj = sqrt(tot[2]);
+        st->branch = st->norm_sj > st->stop ? 1 : 0;
+        st->w = st->branch ? tot[0] / (tot[1] == 0 ? 1.0e-15 : tot[1]) : st->w;
+    }
+}
+
+// x and r updates, r_j1 = (r, r0), beta, residual = |r| / N (bicgstab_ijk.cpp:223-268)
+__global__ void __launch_bounds__(NT) k_bicg_x(DevGrid g, int nbx, int nby, int nbtot, BiBuffers b)
+{
+    BiState* st = b.st;
+    if(st->done || st->mode == 1) return;
+    int i, j, k;
+    long c;
+    double vals[2] = {0.0, 0.0};
+    const double alpha = st->alpha, w = st->w;
+    const int branch = st->branch;
+    if(cell(g, nbx, nby, nbtot, i, j, k, c))
+    {
+        const double s = b.r[c] - alpha * b.v[0][c];
+        const double ph = b.p[1][c];
+        double r;
+        if(branch)
+        {
+            const double sh = s * b.aii[c];
+            b.x[c] += alpha * ph + w * sh;
+            r = s - w * b.t[c];
+        }
+        else
+        {
+            b.x[c] += alpha * ph;
+            r = s;
+        }
+        b.r[c] = r;
+        vals[0] = r * b.r0[c];
+        vals[1] = r * r;
+    }
+    double tot[2];
+    if(reduce_last<2>(vals, b.partials, &st->ticket[3], tot) && threadIdx.x == 0)
+    {
+        const double r_j1 = tot[0];
+        if(branch) st->beta = alpha * r_j1 / (w * st->r_j == 0 ? 1.0e-15 : (w * st->r_j));
+        st->r_j1 = r_j1;
+        st->r_j = r_j1;
+        st->residual = sqrt(tot[1]) / double(st->ncell);
+        st->iter += 1;
+        st->mode = 0;
+        if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
+    }
+}
+
+__global__ void k_bicg_setup(BiState* st, double stop, int maxiter, long ncell)
+{
+    st->r_j = st->norm_r0 = st->alpha = st->w = st->beta = 0.0;
+    st->sigma = st->norm_vj = st->norm_rj = st->norm_sj = st->w1 = st->w2 = st->r_j1 = 0.0;
+    st->residual = 1.0e9;
+    st->stop = stop;
+    st->iter = 0;
+    st->maxiter = maxiter;
+    st->done = 0;
+    st->mode = 0;
+    st->branch = 0;
+    st->ncell = ncell;
+    for(int q = 0; q < 4; ++q) st->ticket[q] = 0u;
+}
+
+void launch_bicg_init(const DevGrid& g, const BiBuffers& b, int cur, double stop, int maxiter, hipStream_t s)
+{
+    (void)cur;
+    int nbx, nby, nbtot;
+    dim3 grid = tiles(g, nbx, nby, nbtot);
+    hipLaunchKernelGGL(k_bicg_setup, dim3(1), dim3(1), 0, s, b.st, stop, maxiter, (long)g.nx * g.ny * g.nz);
+    hipLaunchKernelGGL(k_bicg_restart, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b, 1);
+}
+
+void launch_bicg_iter(const DevGrid& g, const BiBuffers& b, int cur, hipStream_t s)
+{
+    (void)cur;
+    int nbx, nby, nbtot;
+    dim3 grid = tiles(g, nbx, nby, nbtot);
+    hipLaunchKernelGGL(k_bicg_restart, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b, 0);
+    hipLaunchKernelGGL(k_bicg_p, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b);
+    hipLaunchKernelGGL(k_bicg_v, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b);
+    hipLaunchKernelGGL(k_bicg_t, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b);
+    hipLaunchKernelGGL(k_bicg_x, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b);
+}
+
+// ------------------------------------------------------------------ level-set benchmark
+// benchmark_vortex3D::start (benchmark_vortex3D.cpp:67-96)
+__global__ void k_vortex_velocity(DevGrid g, int nbx, int nby, int nbtot, int comp, double* __restrict__ q,
+                                  double simtime, double DXM)
+{
+    int i, j, k;
+    long c;
+    if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
+    if((comp == 1 && i >= g.nx - 1) || (comp == 2 && j >= g.ny - 1) || (comp == 3 && k >= g.nz - 1)) return;
+    const double x = g.XP[i + MARGE], y = g.YP[j + MARGE], z = g.ZP[k + MARGE];
+    const double ct = cos((PI_REF * simtime) / 3.0);
+    if(comp == 1)
+    {
+        const double xc = x + 0.5 * DXM;
+        q[c] = 2.0 * pow(sin(PI_REF * xc), 2.0) * sin(2.0 * PI_REF * y) * sin(2.0 * PI_REF * z) * ct;
+    }
+    else if(comp == 2)
+    {
+        const double yc = y + 0.5 * DXM;
+        q[c] = -sin(2.0 * PI_REF * x) * pow(sin(PI_REF * yc), 2.0) * sin(2.0 * PI_REF * z) * ct;
+    }
+    else
+    {
+        const double zc = z + 0.5 * DXM;
+        q[c] = -sin(2.0 * PI_REF * x) * sin(2.0 * PI_REF * y) * pow(sin(PI_REF * zc), 2.0) * ct;
+    }
+}
+
+void launch_vortex_velocity(const DevGrid& g, int comp, double* q, double simtime, double DXM, hipStream_t s)
+{
+    int nbx, nby, nbtot;
+    dim3 grid = tiles(g, nbx, nby, nbtot);
+    hipLaunchKernelGGL(k_vortex_velocity, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, comp, q, simtime, DXM);
+}
+
+// momentum_RK3 with void convection, diffusion and pressure (momentum_RK3.cpp:70-326):
+// the three stages and the zero forcing of one component, F = 0 + g
+__global__ void k_void_rk3(DevGrid g, int nbx, int nby, int nbtot, int comp, double* __restrict__ q, double dt,
+                           double Fv)
+{
+    int i, j, k;
+    long c;
+    if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
+    if((comp == 1 && i >= g.nx - 1) || (comp == 2 && j >= g.ny - 1) || (comp == 3 && k >= g.nz - 1)) return;
+    const double q0 = q[c];
+    double a1 = q0 + dt * 1.0 * Fv;
+    a1 += 1.0 * dt * 1.0 * 0.0;
+    double a2 = 0.75 * q0 + 0.25 * a1 + 0.25 * dt * 1.0 * Fv;
+    a2 += 0.25 * dt * 1.0 * 0.0;
+    double qn = (1.0 / 3.0) * q0 + (2.0 / 3.0) * a2 + (2.0 / 3.0) * dt * 1.0 * Fv;
+    qn += (2.0 / 3.0) * dt * 1.0 * 0.0;
+    q[c] = qn;
+}
+
+void launch_void_rk3(const DevGrid& g, int comp, double* q, double dt, double Fv, hipStream_t s)
+{
+    int nbx, nby, nbtot;
+    dim3 grid = tiles(g, nbx, nby, nbtot);
+    hipLaunchKernelGGL(k_void_rk3, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, comp, q, dt, Fv);
+}
+
+}  // namespace reef

@@ -1,0 +1,129 @@
+// Internal declarations of the gfx950 hot-path library: host-side grid metadata, the
+// device view passed to kernels, and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/reef3d_hip.h"
+
+namespace reef {
+
+constexpr int MARGE = 5;        // 1-D metadata offset (iterators1D.h: IP = i + 5)
+constexpr int NG = 3;           // ghost layers of a field (grid.h:70)
+constexpr double PI_REF = 3.14159265359;   // definitions.h:37
+
+// Device view of one domain.  Element (i,j,k), i,j,k in [-3, n+3), lives at
+// base + i + sx*j + sxy*k.  Rows are padded so that every i = 0 is 128-byte aligned.
+struct DevGrid
+{
+    int nx, ny, nz, jdir;
+    long sx, sxy;
+    long base;                  // index of (0,0,0)
+    long nalloc;                // doubles per field
+    double ydir;
+    double psi;
+    // 1-D metadata, MARGE-offset (entry i + MARGE)
+    const double *DXP, *DYP, *DZP, *DXN, *DYN, *DZN, *XP, *YP, *ZP;
+    // WENO tables per axis: qf [IP][uf][12], cf [IP][uf][6], isf [IP][uf][18]
+    const double *qf[3], *cf[3], *isf[3];
+};
+
+// One ghost operation: f[dst] = (src < 0) ? 0 : (scaled ? 0.0 + w*f[src] : f[src])
+struct GhostOp
+{
+    int32_t dst;
+    int32_t src;     // bit 30 set: scaled; -1: zero
+    double w;
+};
+
+// Host copy of the metadata and the compiled ghost programs
+struct HostGrid
+{
+    int nx, ny, nz, jdir;
+    int group[7];
+    std::vector<double> X[3], P[3], DN[3], DP[3];
+    double DXM, DYM, DZM, psi;
+    std::vector<double> qf[3], cf[3], isf[3];
+    long sx, sy, sz, sxy, base, nalloc;
+    int last[4];
+    std::vector<int> flag[5];   // flag1..4 on the compact (nx+6)(ny+6)(nz+6) layout
+    long cid(int i, int j, int k) const
+    {
+        return (long)(i + NG) + (long)(nx + 2 * NG) * ((long)(j + NG) + (long)(ny + 2 * NG) * (long)(k + NG));
+    }
+    long did(int i, int j, int k) const { return base + i + sx * j + sxy * k; }
+};
+
+// builds metadata and lists; returns empty string or an error message
+std::string build_grid(HostGrid& g, const reef_grid_desc& d, double F45);
+// compiled ghost program of ghostcell::start<kind>(f, gcv)
+std::string build_ghost_program(const HostGrid& g, int kind, int gcv, std::vector<GhostOp>& ops);
+
+// ------------------------------------------------------------------ BiCGStab state
+struct BiState
+{
+    double r_j, norm_r0, alpha, w, beta, residual, stop, sigma, norm_vj, norm_rj, norm_sj;
+    double w1, w2, r_j1;
+    int iter, maxiter, done, mode;     // mode: 0 run, 1 restart pending, 2 fresh p
+    int branch;                        // 1: norm_sj > stop
+    int pad;
+    unsigned int ticket[4];
+    long ncell;
+};
+
+// ------------------------------------------------------------------ launchers
+void launch_ghost(const GhostOp* ops, int nops, double* f, hipStream_t s);
+
+// HJ-WENO stage: out = combo(phi_n, phi_s, cdt * (0 + L_hj(phi_s; u, v, w)))
+//   stage 1: out = phi_s + cdt*L ; stage 2: 0.75 phi_n + 0.25 phi_s + cdt*L ;
+//   stage 3: (1/3) phi_n + (2/3) phi_s + cdt*L ; stage 0: out = L (accumulate: out += L)
+void launch_hj_stage(const DevGrid& g, int stage, const double* phi_s, const double* phi_n, const double* u,
+                     const double* v, const double* w, double* out, double cdt, hipStream_t s);
+
+// reinit sub-stage q (1..3): out = combo(f0, fs, dtf * L_reini(fs))
+void launch_reini_stage(const DevGrid& g, int stage, const double* fs, const double* f0, double* out, double F43,
+                        hipStream_t s);
+
+// WENO flux: acc += flux divergence of q (component ipol) with face velocities of (u,v,w)
+void launch_flux_acc(const DevGrid& g, int ipol, const double* q, const double* u, const double* v, const double* w,
+                     double* acc, hipStream_t s);
+
+// velocity predictor of one component (momentum_FC3 stages): F = 0 - dp/(DP*rho_f) + g + conv;
+// stage 1: out = qin + (dt)*F ; stage 2: 0.75 qold + 0.25 qin + (0.25 dt) F ;
+// stage 3: (1/3) qold + (2/3) qin + ((2/3) dt) F   (written to out)
+void launch_momentum(const DevGrid& g, int comp, int stage, const double* qin, const double* qold, const double* u,
+                     const double* v, const double* w, const double* press, const double* phi, double grav,
+                     double cdt, double* out, const double* rho_w_a, hipStream_t s);
+
+void launch_heaviside(const DevGrid& g, const double* phi, double* ro, double* visc, const double* prm4,
+                      double* partials, double* vol_out, unsigned int* ticket, hipStream_t s);
+
+void launch_copy_interior(const DevGrid& g, const double* src, double* dst, hipStream_t s);
+void launch_fill(double* f, long n, double v, hipStream_t s);
+
+// projection
+void launch_poisson_rhs(const DevGrid& g, const double* u, const double* v, const double* w, double adt, double* rhs,
+                        hipStream_t s);
+void launch_face_rho(const DevGrid& g, const double* phi, double rw, double ra, double* rfx, double* rfy,
+                     double* rfz, double* aii, hipStream_t s);
+void launch_correct(const DevGrid& g, const double* pcorr, const double* phi, double rw, double ra, double adt,
+                    double* u, double* v, double* w, double* press, hipStream_t s);
+
+struct BiBuffers
+{
+    double *x, *rhs, *r, *r0, *p[2], *v[2], *t, *aii, *rfx, *rfy, *rfz;
+    double* partials;
+    BiState* st;
+};
+void launch_bicg_init(const DevGrid& g, const BiBuffers& b, int cur, double stop, int maxiter, hipStream_t s);
+void launch_bicg_iter(const DevGrid& g, const BiBuffers& b, int cur, hipStream_t s);
+
+// prescribed vortex field (benchmark_vortex3D.cpp:67-101) and the void momentum_RK3 update
+void launch_vortex_velocity(const DevGrid& g, int comp, double* q, double simtime, double DXM, hipStream_t s);
+void launch_void_rk3(const DevGrid& g, int comp, double* q, double dt, double Fv, hipStream_t s);
+
+}  // namespace reef

@@ -1,0 +1,221 @@
+"""Python host mirror of the reference's hot-path interfaces over the C-ABI.
+
+The reference drives this path through C++ strategy objects on lexer/fdm/ghostcell:
+convection::start (convection.h:37), reini::start (reini.h:39), solver::start
+(solver.h:42), pressure::start (pressure.h:41), ghostcell::start1..4 (ghostcell.h:58-61),
+momentum::start (momentum.h) and freesurface/benchmark steps.  `Domain` owns one
+device-resident domain (reef_ctx) and exposes the same operations with the same
+argument meaning; fields are addressed by name ("u", "v", "w", "press", "phi", "ro",
+"visc", "F", "G", "H", "L", "pcorr", "work0".."work3").
+
+Arrays exchanged with the host are float64 with shape (nz+6, ny+6, nx+6) (i fastest,
+3 ghost layers), or the reference's own k-fastest field storage with layout="ref".
+Every call goes to the HIP library; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "libreef3d_hip.so")
+_LIB = None
+
+FIELDS = {"u": 0, "v": 1, "w": 2, "press": 3, "phi": 4, "ro": 5, "visc": 6, "F": 7, "G": 8,
+          "H": 9, "L": 10, "pcorr": 11, "work0": 12, "work1": 13, "work2": 14, "work3": 15}
+
+
+class ReefError(RuntimeError):
+    pass
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("knox", ctypes.c_int), ("knoy", ctypes.c_int), ("knoz", ctypes.c_int),
+                ("j_dir", ctypes.c_int), ("XN", ctypes.POINTER(ctypes.c_double)),
+                ("YN", ctypes.POINTER(ctypes.c_double)), ("ZN", ctypes.POINTER(ctypes.c_double)),
+                ("DYM", ctypes.c_double), ("DZM", ctypes.c_double), ("group", ctypes.c_int * 7)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("W1", ctypes.c_double), ("W2", ctypes.c_double), ("W3", ctypes.c_double),
+                ("W4", ctypes.c_double), ("W20", ctypes.c_double), ("W21", ctypes.c_double),
+                ("W22", ctypes.c_double), ("F43", ctypes.c_double), ("F45", ctypes.c_double),
+                ("dt", ctypes.c_double), ("N44", ctypes.c_double), ("N46", ctypes.c_int)]
+
+
+def library():
+    """Loads libreef3d_hip.so; raises if it was not built (no fallback path exists)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(_LIBPATH):
+        raise ImportError(f"{_LIBPATH} missing: build it with `python reef3d_amd/build.py`")
+    L = ctypes.CDLL(_LIBPATH)
+    dp, ip, vp = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.c_void_p
+    sig = {
+        "reef_ctx_create": ([ctypes.POINTER(GridDesc), ctypes.POINTER(Params), ctypes.POINTER(vp)], ctypes.c_int),
+        "reef_ctx_destroy": ([vp], None),
+        "reef_last_error": ([], ctypes.c_char_p),
+        "reef_field_put": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
+        "reef_field_get": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
+        "reef_field_zero": ([vp, ctypes.c_int], ctypes.c_int),
+        "reef_ghost_fill": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "reef_convection": ([vp] + [ctypes.c_int] * 5, ctypes.c_int),
+        "reef_reini": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "reef_fluid_update": ([vp, dp], ctypes.c_int),
+        "reef_poisson_solve": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ip, dp], ctypes.c_int),
+        "reef_pjm_corr": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ip], ctypes.c_int),
+        "reef_fc3_step": ([vp, ip, dp], ctypes.c_int),
+        "reef_vortex_step": ([vp, ctypes.c_double, dp], ctypes.c_int),
+        "reef_phi_rk3": ([vp], ctypes.c_int),
+        "reef_sync": ([vp], ctypes.c_int),
+        "reef_timing_enable": ([vp, ctypes.c_int], ctypes.c_int),
+        "reef_timing_get": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "reef_timing_reset": ([vp], ctypes.c_int),
+        "reef_ctx_sizes": ([vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _LIB = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc):
+    if rc != 0:
+        msg = library().reef_last_error().decode(errors="replace")
+        raise ReefError(f"reef call failed ({rc}): {msg}")
+
+
+class Domain:
+    """One device-resident box domain (lexer + fdm + ghostcell of the reference).
+
+    nx, ny, nz: interior cells; XN/YN/ZN: node coordinates with 5 ghost nodes per side;
+    top_group: boundary group of the +z side (21 wall, 3 free surface); the other five
+    sides are walls.  Physical/control values default to the reference's ini.cpp
+    defaults except where the hot-path configs set them.
+    """
+
+    def __init__(self, nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21,
+                 rho=(1000.0, 1.205), visc=(1.0e-6, 1.4938e-5), g=(0.0, 0.0, 0.0),
+                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000):
+        L = library()
+        self.nx, self.ny, self.nz, self.jdir = nx, ny, nz, jdir
+        self.shape = (nz + 6, ny + 6, nx + 6)
+        self._xn = [np.ascontiguousarray(a, dtype=np.float64) for a in (XN, YN, ZN)]
+        for a, n in zip(self._xn, (nx, ny, nz)):
+            if a.size != n + 11:
+                raise ValueError("node arrays need n + 11 entries (5 ghost nodes per side)")
+        gd = GridDesc(nx, ny, nz, jdir, _dp(self._xn[0]), _dp(self._xn[1]), _dp(self._xn[2]),
+                      DYM, DZM, (ctypes.c_int * 7)(0, 21, 21, 21, 21, 21, top_group))
+        self.params = Params(rho[0], visc[0], rho[1], visc[1], g[0], g[1], g[2], F43, F45, dt, N44, N46)
+        h = ctypes.c_void_p()
+        _check(L.reef_ctx_create(ctypes.byref(gd), ctypes.byref(self.params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().reef_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- fields (fdm)
+    def put(self, name, arr, layout="ifast"):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.size != int(np.prod(self.shape)):
+            raise ValueError(f"field {name}: expected {self.shape} values")
+        _check(library().reef_field_put(self.h, FIELDS[name], _dp(a), 0 if layout == "ifast" else 1))
+
+    def get(self, name, layout="ifast"):
+        out = np.empty(self.shape if layout == "ifast" else (self.nx + 6, self.ny + 6, self.nz + 6))
+        _check(library().reef_field_get(self.h, FIELDS[name], _dp(out), 0 if layout == "ifast" else 1))
+        return out
+
+    def zero(self, name):
+        _check(library().reef_field_zero(self.h, FIELDS[name]))
+
+    def sync(self):
+        _check(library().reef_sync(self.h))
+
+    # ---------------------------------------------------------------- ghostcell
+    def start1(self, name, gcv):
+        _check(library().reef_ghost_fill(self.h, FIELDS[name], 1, gcv))
+
+    def start2(self, name, gcv):
+        _check(library().reef_ghost_fill(self.h, FIELDS[name], 2, gcv))
+
+    def start3(self, name, gcv):
+        _check(library().reef_ghost_fill(self.h, FIELDS[name], 3, gcv))
+
+    def start4(self, name, gcv):
+        _check(library().reef_ghost_fill(self.h, FIELDS[name], 4, gcv))
+
+    # ---------------------------------------------------------------- convection / reini
+    def convection(self, q, ipol, u="u", v="v", w="w"):
+        """convection::start(p, a, q, ipol, u, v, w): += into F/G/H (ipol 1-3) or L (4)."""
+        _check(library().reef_convection(self.h, ipol, FIELDS[q], FIELDS[u], FIELDS[v], FIELDS[w]))
+
+    def reini(self, name="phi", iters=3):
+        """reini::start with p->count > 0 and p->F44 = iters."""
+        _check(library().reef_reini(self.h, FIELDS[name], iters))
+
+    def fluid_update(self):
+        vol = np.zeros(2)
+        _check(library().reef_fluid_update(self.h, _dp(vol)))
+        return vol
+
+    # ---------------------------------------------------------------- projection
+    def solver(self, x, rhs, stop, maxiter):
+        """solver::start(var=5) on the operator assembled from phi; returns (iters, residual)."""
+        it, res = ctypes.c_int(), np.zeros(1)
+        _check(library().reef_poisson_solve(self.h, FIELDS[x], FIELDS[rhs], stop, maxiter,
+                                            ctypes.byref(it), _dp(res)))
+        return it.value, float(res[0])
+
+    def pressure(self, u="u", v="v", w="w", alpha=1.0):
+        """pressure::start with pjm_corr; returns the BiCGStab iteration count."""
+        it = ctypes.c_int()
+        _check(library().reef_pjm_corr(self.h, FIELDS[u], FIELDS[v], FIELDS[w], alpha, ctypes.byref(it)))
+        return it.value
+
+    # ---------------------------------------------------------------- time steps
+    def momentum_fc3(self):
+        """momentum_FC3::start: one two-phase time step; returns (iterations, volumes)."""
+        it, vol = ctypes.c_int(), np.zeros(2)
+        _check(library().reef_fc3_step(self.h, ctypes.byref(it), _dp(vol)))
+        return it.value, vol
+
+    def vortex_step(self, simtime):
+        vol = np.zeros(2)
+        _check(library().reef_vortex_step(self.h, simtime, _dp(vol)))
+        return vol
+
+    def levelset_rk3(self):
+        _check(library().reef_phi_rk3(self.h))
+
+    # ---------------------------------------------------------------- timing
+    def timing(self, on=True):
+        _check(library().reef_timing_enable(self.h, 1 if on else 0))
+
+    def timing_reset(self):
+        _check(library().reef_timing_reset(self.h))
+
+    def timing_get(self, name):
+        ms, n = np.zeros(1), ctypes.c_long()
+        _check(library().reef_timing_get(self.h, name.encode(), _dp(ms), ctypes.byref(n)))
+        return float(ms[0]), n.value
+
+    def sizes(self):
+        out = (ctypes.c_long * 5)()
+        _check(library().reef_ctx_sizes(self.h, out))
+        return list(out)
