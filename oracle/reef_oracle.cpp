@@ -771,6 +771,9 @@ struct Bicg
         loop([&](long c, long n) { rj[c] = rhs[c] - (Ax(x, c, n)); });
         loop([&](long c, long) { r0[c] = pj[c] = rj[c]; r_j += rj[c] * r0[c]; });
         norm_r0 = sqrt(r_j);
+        // negative stop: relative to the initial residual (harness mapping of a relative
+        // tolerance onto N44, not a reference feature)
+        if(stop < 0.0) stop = -stop * sqrt(r_j) / double(d->cellnum);
         if((residual >= stop) && (solveriter < maxiter))
         {
             do

@@ -79,14 +79,28 @@ struct reef_ctx
         return 0;
     }
 
-    // timing of one launch family
+    // timing of one launch family: HIP events from a reusable pool, recorded on the
+    // context stream around the launches
+    std::vector<hipEvent_t> pool;
+    hipEvent_t take()
+    {
+        if(pool.empty())
+        {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
     void tbegin(const char* name)
     {
         if(!timing) return;
         TimedRange r;
         r.name = name;
-        hipEventCreate(&r.a);
-        hipEventCreate(&r.b);
+        r.a = take();
+        r.b = take();
         hipEventRecord(r.a, stream);
         ranges.push_back(r);
     }
@@ -105,8 +119,8 @@ struct reef_ctx
             auto& t = totals[r.name];
             t.first += ms;
             t.second += 1;
-            hipEventDestroy(r.a);
-            hipEventDestroy(r.b);
+            pool.push_back(r.a);
+            pool.push_back(r.b);
         }
         ranges.clear();
     }
@@ -185,6 +199,8 @@ int do_fluid_update(reef_ctx* c, double* vol2)
     return 0;
 }
 
+const char* const BICG_NAMES[4] = {"bicg_p", "bicg_v", "bicg_t", "bicg_x"};
+
 // BiCGStab on the operator assembled from REEF_PHI; x holds the initial guess
 int do_solve(reef_ctx* c, double* x, const double* rhs, double stop, int maxiter, int* iters, double* residual)
 {
@@ -195,7 +211,7 @@ int do_solve(reef_ctx* c, double* x, const double* rhs, double stop, int maxiter
     c->tbegin("face_rho");
     launch_face_rho(g, c->field[REEF_PHI], c->prm.W1, c->prm.W3, b.rfx, b.rfy, b.rfz, b.aii, c->stream);
     c->tend();
-    c->tbegin("bicgstab");
+    c->tbegin("bicg_init");
     launch_bicg_init(g, b, 0, stop, maxiter, c->stream);
     c->tend();
     // iterations are enqueued in batches; the device state says when the solve ended
@@ -203,12 +219,17 @@ int do_solve(reef_ctx* c, double* x, const double* rhs, double stop, int maxiter
     BiState hs;
     for(int launched = 0;;)
     {
-        c->tbegin("bicgstab");
-        for(int q = 0; q < batch; ++q) launch_bicg_iter(g, b, 0, c->stream);
-        c->tend();
+        for(int q = 0; q < batch; ++q)
+            for(int ph = 0; ph < 4; ++ph)
+            {
+                c->tbegin(BICG_NAMES[ph]);
+                launch_bicg_phase(g, b, ph, c->stream);
+                c->tend();
+            }
         launched += batch;
         HIPCHK(hipMemcpyAsync(&hs, b.st, sizeof(BiState), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        if(c->timing) c->tcollect();
         if(hs.done) break;
         // every launched iteration advances iter or restarts; restarts also count, so
         // the solve ends within maxiter + (restarts) batches
@@ -440,6 +461,7 @@ void reef_ctx_destroy(reef_ctx* c)
         hipEventDestroy(r.a);
         hipEventDestroy(r.b);
     }
+    for(hipEvent_t e : c->pool) hipEventDestroy(e);
     for(void* p : c->allocs) hipFree(p);
     if(c->stream) hipStreamDestroy(c->stream);
     delete c;

@@ -1,11 +1,11 @@
 // Ghost fill, fluid properties, pressure projection and the Jacobi-preconditioned
 // BiCGStab of bicgstab_ijk on the device.
 //
-// Reductions are one-pass: every block writes its partial sums, the last block to
-// arrive (ticket counter) adds the partials in block order and updates the solver state
-// in device memory, so the iteration never waits on the host.  Partial sums are formed
-// in a fixed order, so results are run-to-run deterministic (they differ from the
-// reference's serial i-j-k sum only by rounding).
+// Reductions: every block writes its partial sums, a one-block finalize kernel adds them
+// in block order and updates the solver state in device memory, so the iteration never
+// waits on the host.  Partial sums are formed in a fixed order, so results are
+// run-to-run deterministic (they differ from the reference's serial i-j-k sum only by
+// rounding).
 #include "reef_internal.h"
 
 namespace reef {
@@ -46,15 +46,15 @@ __device__ __forceinline__ bool cell(const DevGrid& g, int nbx, int nby, int nbt
     return true;
 }
 
-// Block partial sums of M values, fixed tree order; partials[blockIdx.x*M + m].
-// Returns true in every thread of the last block to finish; that block then holds the
-// totals in tot[] (all threads).
+// Block partial sums of M values in a fixed tree order, written to
+// partials[blockIdx.x*M + m].  The totals are formed by a one-block finalize kernel
+// launched right after (sum_partials): a kernel boundary is a far cheaper way to make
+// the partials visible than a device-scope fence per block (each such fence writes the
+// L2 back on this GPU).
 template<int M>
-__device__ __forceinline__ bool reduce_last(const double (&v)[M], double* partials, unsigned int* ticket,
-                                            double (&tot)[M])
+__device__ __forceinline__ void block_partials(const double (&v)[M], double* __restrict__ partials)
 {
     __shared__ double sh[M][NT];
-    __shared__ bool amlast;
     const int t = threadIdx.x;
     for(int m = 0; m < M; ++m) sh[m][t] = v[m];
     __syncthreads();
@@ -65,20 +65,28 @@ __device__ __forceinline__ bool reduce_last(const double (&v)[M], double* partia
         __syncthreads();
     }
     if(t == 0)
-    {
         for(int m = 0; m < M; ++m) partials[(size_t)blockIdx.x * M + m] = sh[m][0];
-        __threadfence();
-        const unsigned int prev = atomicAdd(ticket, 1u);
-        amlast = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if(!amlast) return false;
-    __threadfence();
-    const volatile double* P = partials;
+}
+
+// totals of nb block partials, summed in block order by one block of NT threads
+template<int M>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int nb, double (&tot)[M])
+{
+    __shared__ double sh[M][NT];
+    const int t = threadIdx.x;
     double acc[M];
     for(int m = 0; m < M; ++m) acc[m] = 0.0;
-    for(unsigned b = t; b < gridDim.x; b += NT)
-        for(int m = 0; m < M; ++m) acc[m] += P[(size_t)b * M + m];
+    int b = t;
+    for(; b + 3 * NT < nb; b += 4 * NT)
+    {
+        double q[4][M];
+        for(int u = 0; u < 4; ++u)
+            for(int m = 0; m < M; ++m) q[u][m] = partials[(size_t)(b + u * NT) * M + m];
+        for(int u = 0; u < 4; ++u)
+            for(int m = 0; m < M; ++m) acc[m] += q[u][m];
+    }
+    for(; b < nb; b += NT)
+        for(int m = 0; m < M; ++m) acc[m] += partials[(size_t)b * M + m];
     for(int m = 0; m < M; ++m) sh[m][t] = acc[m];
     __syncthreads();
     for(int s = NT / 2; s > 0; s >>= 1)
@@ -88,8 +96,6 @@ __device__ __forceinline__ bool reduce_last(const double (&v)[M], double* partia
         __syncthreads();
     }
     for(int m = 0; m < M; ++m) tot[m] = sh[m][0];
-    if(t == 0) *ticket = 0u;
-    return true;
 }
 
 // ------------------------------------------------------------------ ghost program
@@ -167,8 +173,14 @@ __global__ void __launch_bounds__(NT) k_heaviside(DevGrid g, int nbx, int nby, i
         vals[0] = vol * (H - (1.0 - 1.0));
         vals[1] = vol * (1.0 - H - (1.0 - 1.0));
     }
+    block_partials<2>(vals, partials);
+}
+
+__global__ void k_heaviside_fin(const double* __restrict__ partials, int nb, double* vol_out)
+{
     double tot[2];
-    if(reduce_last<2>(vals, partials, ticket, tot) && threadIdx.x == 0)
+    sum_partials<2>(partials, nb, tot);
+    if(threadIdx.x == 0)
     {
         vol_out[0] = tot[0];
         vol_out[1] = tot[1];
@@ -182,6 +194,7 @@ void launch_heaviside(const DevGrid& g, const double* phi, double* ro, double* v
     dim3 grid = tiles(g, nbx, nby, nbtot);
     hipLaunchKernelGGL(k_heaviside, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, phi, ro, visc, prm4[0], prm4[1], prm4[2],
                        prm4[3], partials, vol_out, ticket);
+    hipLaunchKernelGGL(k_heaviside_fin, dim3(1), dim3(NT), 0, s, partials, (int)grid.x, vol_out);
 }
 
 // ------------------------------------------------------------------ projection
@@ -333,75 +346,191 @@ void launch_correct(const DevGrid& g, const double* pcorr, const double* phi, do
 
 // ------------------------------------------------------------------ BiCGStab
 // State machine of bicgstab_ijk::solve (bicgstab_ijk.cpp:125-278): mode 1 = a restart
-// is pending (res_calc + restart label), 2 = p was just reset to r.
+// is pending (res_calc + restart label), 2 = p and ph were just set from the initial
+// residual.
+//
+// The solver kernels run a fixed grid of RB blocks (8 per CU); block b works through a
+// contiguous range of 64x4 tiles (z fastest) inside the slab of XCD b % 8, so stencil
+// neighbours mostly hit L1 or that XCD's L2, and every reduction has RB partials.
+constexpr int RB = 2048;
 
-// restart / initial residual: r = b - A x, r0 = p = r, r_j = (r, r0)
-__global__ void __launch_bounds__(NT) k_bicg_restart(DevGrid g, int nbx, int nby, int nbtot, BiBuffers b, int init)
+struct TileWalk
+{
+    int t, t1, nbx, nby;
+};
+
+__device__ __forceinline__ TileWalk tile_walk(const DevGrid& g)
+{
+    TileWalk w;
+    w.nbx = (g.nx + TX - 1) / TX;
+    w.nby = (g.ny + TY - 1) / TY;
+    const int ntiles = w.nbx * w.nby * g.nz;
+    const int per_xcd_blocks = gridDim.x >> 3;
+    const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+    const int tiles_xcd = (ntiles + 7) >> 3;
+    const int tpb = (tiles_xcd + per_xcd_blocks - 1) / per_xcd_blocks;
+    const int lo = xcd * tiles_xcd;
+    const int hi = min(lo + tiles_xcd, ntiles);
+    w.t = min(lo + loc * tpb, hi);
+    w.t1 = min(w.t + tpb, hi);
+    return w;
+}
+
+// tiles are numbered z fastest: a block walks up one 64x4 column, so the k-1 plane of a
+// 7-point stencil is still in L1 from the previous tile
+__device__ __forceinline__ bool tile_cell(const DevGrid& g, const TileWalk& w, int t, int& i, int& j, int& k, long& c)
+{
+    const int col = t / g.nz;
+    k = t - col * g.nz;
+    const int by = col / w.nbx, bx = col - by * w.nbx;
+    i = bx * TX + (threadIdx.x & (TX - 1));
+    j = by * TY + (threadIdx.x >> 6);
+    if(i >= g.nx || j >= g.ny) return false;
+    c = g.base + i + g.sx * j + g.sxy * k;
+    return true;
+}
+
+// restart / initial residual: r = b - A x, r0 = p = r, ph = aii p, r_j = (r, r0)
+// (restart label of bicgstab_ijk.cpp:136-148; res_calc :314-338 gives the same sums)
+__device__ __forceinline__ void restart_tiles(const DevGrid& g, const BiBuffers& b, double& rr)
+{
+    const double* __restrict__ rhs = b.rhs;
+    const double* __restrict__ x = b.x;
+    const double* __restrict__ rfx = b.rfx;
+    const double* __restrict__ rfy = b.rfy;
+    const double* __restrict__ rfz = b.rfz;
+    const double* __restrict__ aii = b.aii;
+    double* __restrict__ r = b.r;
+    double* __restrict__ r0 = b.r0;
+    double* __restrict__ p = b.p[0];
+    double* __restrict__ ph = b.p[1];
+    const TileWalk tw = tile_walk(g);
+    for(int t = tw.t; t < tw.t1; ++t)
+    {
+        int i, j, k;
+        long c;
+        if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+        const Coupling q = couplings(g, rfx, rfy, rfz, c, i, j, k);
+        const double rv = rhs[c] - (apply_A(g, q, x, c, i, j, k));
+        r[c] = rv;
+        r0[c] = rv;
+        p[c] = rv;
+        ph[c] = rv * aii[c];
+        rr += rv * rv;
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_bicg_init(DevGrid g, BiBuffers b)
+{
+    double vals[1] = {0.0};
+    restart_tiles(g, b, vals[0]);
+    block_partials<1>(vals, b.partials);
+}
+
+__global__ void k_fin_init(BiBuffers b, int nb)
 {
     BiState* st = b.st;
-    if(st->done) return;
-    if(!init && st->mode != 1) return;
-    int i, j, k;
-    long c;
-    double vals[1] = {0.0};
-    if(cell(g, nbx, nby, nbtot, i, j, k, c))
-    {
-        const Coupling q = couplings(g, b.rfx, b.rfy, b.rfz, c, i, j, k);
-        const double r = b.rhs[c] - (apply_A(g, q, b.x, c, i, j, k));
-        b.r[c] = r;
-        b.r0[c] = r;
-        b.p[0][c] = r;
-        vals[0] = r * r;
-    }
     double tot[1];
-    if(reduce_last<1>(vals, b.partials, &st->ticket[0], tot) && threadIdx.x == 0)
+    sum_partials<1>(b.partials, nb, tot);
+    if(threadIdx.x == 0)
     {
         st->r_j = tot[0];
         st->norm_r0 = sqrt(tot[0]);
-        if(!init)
-        {
-            st->residual = sqrt(tot[0]) / double(st->ncell);
-            st->iter += 1;
-        }
+        // a negative stop value asks for a criterion relative to the initial residual:
+        // ||r||_2 / N < |stop| * ||b - A x0||_2 / N (= ||b||_2 / N for x0 = 0)
+        if(st->stop < 0.0) st->stop = -st->stop * sqrt(tot[0]) / double(st->ncell);
         if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
         st->mode = 2;
     }
 }
 
-// p = r + beta (p - w v) (bicgstab_ijk.cpp:234-235), ph = aii p (precon_solve)
-__global__ void k_bicg_p(DevGrid g, int nbx, int nby, int nbtot, BiBuffers b)
+// start of an iteration: either the pending restart, or p = r + beta (p - w v)
+// (bicgstab_ijk.cpp:234-235); then ph = aii p (precon_solve)
+__global__ void __launch_bounds__(NT) k_bicg_p(DevGrid g, BiBuffers b)
 {
-    const BiState* st = b.st;
-    if(st->done || st->mode == 1) return;
-    int i, j, k;
-    long c;
-    if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
-    double p = b.p[0][c];
-    if(st->mode != 2) p = b.r[c] + st->beta * (p - st->w * b.v[0][c]);
-    b.p[0][c] = p;
-    b.p[1][c] = p * b.aii[c];
+    BiState* st = b.st;
+    if(st->done) return;
+    if(st->mode == 1)
+    {
+        double vals[1] = {0.0};
+        restart_tiles(g, b, vals[0]);
+        block_partials<1>(vals, b.partials);
+        return;
+    }
+    if(st->mode == 2) return;     // p = r and ph were set by the initial residual
+    const double beta = st->beta, w = st->w;
+    const double* __restrict__ r = b.r;
+    const double* __restrict__ v = b.v[0];
+    const double* __restrict__ aii = b.aii;
+    double* __restrict__ p = b.p[0];
+    double* __restrict__ ph = b.p[1];
+    const TileWalk tw = tile_walk(g);
+    for(int t = tw.t; t < tw.t1; ++t)
+    {
+        int i, j, k;
+        long c;
+        if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+        const double pn = r[c] + beta * (p[c] - w * v[c]);
+        p[c] = pn;
+        ph[c] = pn * aii[c];
+    }
+}
+
+// totals of a restart done by k_bicg_p: ++solveriter, residual = |r|/N (bicgstab_ijk.cpp:183-191)
+__global__ void k_fin_p(BiBuffers b, int nb)
+{
+    BiState* st = b.st;
+    if(st->done || st->mode != 1) return;
+    double tot[1];
+    sum_partials<1>(b.partials, nb, tot);
+    if(threadIdx.x == 0)
+    {
+        st->r_j = tot[0];
+        st->norm_r0 = sqrt(tot[0]);
+        st->residual = sqrt(tot[0]) / double(st->ncell);
+        st->iter += 1;
+        if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
+        st->mode = 2;
+    }
 }
 
 // v = A ph; sigma = (v, r0), |v|, |r|; alpha and the restart tests (bicgstab_ijk.cpp:151-191)
-__global__ void __launch_bounds__(NT) k_bicg_v(DevGrid g, int nbx, int nby, int nbtot, BiBuffers b)
+__global__ void __launch_bounds__(NT) k_bicg_v(DevGrid g, BiBuffers b)
 {
     BiState* st = b.st;
-    if(st->done || st->mode == 1) return;
-    int i, j, k;
-    long c;
+    if(st->done) return;
     double vals[3] = {0.0, 0.0, 0.0};
-    if(cell(g, nbx, nby, nbtot, i, j, k, c))
+    const double* __restrict__ ph = b.p[1];
+    const double* __restrict__ rfx = b.rfx;
+    const double* __restrict__ rfy = b.rfy;
+    const double* __restrict__ rfz = b.rfz;
+    const double* __restrict__ r = b.r;
+    const double* __restrict__ r0 = b.r0;
+    double* __restrict__ vo = b.v[0];
+    const TileWalk tw = tile_walk(g);
+    for(int t = tw.t; t < tw.t1; ++t)
     {
-        const Coupling q = couplings(g, b.rfx, b.rfy, b.rfz, c, i, j, k);
-        const double v = apply_A(g, q, b.p[1], c, i, j, k);
-        b.v[0][c] = v;
-        const double r = b.r[c];
-        vals[0] = v * b.r0[c];
-        vals[1] = v * v;
-        vals[2] = r * r;
+        int i, j, k;
+        long c;
+        if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+        const Coupling q = couplings(g, rfx, rfy, rfz, c, i, j, k);
+        const double v = apply_A(g, q, ph, c, i, j, k);
+        vo[c] = v;
+        const double rc = r[c];
+        vals[0] += v * r0[c];
+        vals[1] += v * v;
+        vals[2] += rc * rc;
     }
+    block_partials<3>(vals, b.partials);
+}
+
+__global__ void k_fin_v(BiBuffers b, int nb)
+{
+    BiState* st = b.st;
+    if(st->done) return;
     double tot[3];
-    if(reduce_last<3>(vals, b.partials, &st->ticket[1], tot) && threadIdx.x == 0)
+    sum_partials<3>(b.partials, nb, tot);
+    if(threadIdx.x == 0)
     {
         const double sigma = tot[0];
         const double norm_vj = sqrt(tot[1]);
@@ -421,20 +550,26 @@ __global__ void __launch_bounds__(NT) k_bicg_v(DevGrid g, int nbx, int nby, int 
 }
 
 // s = r - alpha v, sh = aii s, t = A sh; (t,s), (t,t), (s,s) (bicgstab_ijk.cpp:193-221)
-__global__ void __launch_bounds__(NT) k_bicg_t(DevGrid g, int nbx, int nby, int nbtot, BiBuffers b)
+__global__ void __launch_bounds__(NT) k_bicg_t(DevGrid g, BiBuffers b)
 {
     BiState* st = b.st;
     if(st->done || st->mode == 1) return;
-    int i, j, k;
-    long c;
     double vals[3] = {0.0, 0.0, 0.0};
     const double alpha = st->alpha;
-    if(cell(g, nbx, nby, nbtot, i, j, k, c))
+    const double* __restrict__ r = b.r;
+    const double* __restrict__ v = b.v[0];
+    const double* __restrict__ a = b.aii;
+    const double* __restrict__ rfx = b.rfx;
+    const double* __restrict__ rfy = b.rfy;
+    const double* __restrict__ rfz = b.rfz;
+    double* __restrict__ to = b.t;
+    const TileWalk tw = tile_walk(g);
+    for(int tt = tw.t; tt < tw.t1; ++tt)
     {
-        const double* r = b.r;
-        const double* v = b.v[0];
-        const double* a = b.aii;
-        const Coupling q = couplings(g, b.rfx, b.rfy, b.rfz, c, i, j, k);
+        int i, j, k;
+        long c;
+        if(!tile_cell(g, tw, tt, i, j, k, c)) continue;
+        const Coupling q = couplings(g, rfx, rfy, rfz, c, i, j, k);
         // sh at the centre and the neighbours (0 outside the domain)
         auto shv = [&](long cc) { return (r[cc] - alpha * v[cc]) * a[cc]; };
         const double Mp = diag_of(g, q);
@@ -459,13 +594,21 @@ __global__ void __launch_bounds__(NT) k_bicg_t(DevGrid g, int nbx, int nby, int 
         }
         else
             t = Mp * shc + Mn * xn + Ms * xs + Mt * xt + Mb * xb;
-        b.t[c] = t;
-        vals[0] = t * s;
-        vals[1] = t * t;
-        vals[2] = s * s;
+        to[c] = t;
+        vals[0] += t * s;
+        vals[1] += t * t;
+        vals[2] += s * s;
     }
+    block_partials<3>(vals, b.partials);
+}
+
+__global__ void k_fin_t(BiBuffers b, int nb)
+{
+    BiState* st = b.st;
+    if(st->done || st->mode == 1) return;
     double tot[3];
-    if(reduce_last<3>(vals, b.partials, &st->ticket[2], tot) && threadIdx.x == 0)
+    sum_partials<3>(b.partials, nb, tot);
+    if(threadIdx.x == 0)
     {
         st->w1 = tot[0];
         st->w2 = tot[1];
@@ -476,37 +619,64 @@ __global__ void __launch_bounds__(NT) k_bicg_t(DevGrid g, int nbx, int nby, int 
 }
 
 // x and r updates, r_j1 = (r, r0), beta, residual = |r| / N (bicgstab_ijk.cpp:223-268)
-__global__ void __launch_bounds__(NT) k_bicg_x(DevGrid g, int nbx, int nby, int nbtot, BiBuffers b)
+template<bool BRANCH>
+__device__ __forceinline__ void x_tiles(const DevGrid& g, const BiBuffers& b, double alpha, double w, double (&vals)[2])
 {
-    BiState* st = b.st;
-    if(st->done || st->mode == 1) return;
-    int i, j, k;
-    long c;
-    double vals[2] = {0.0, 0.0};
-    const double alpha = st->alpha, w = st->w;
-    const int branch = st->branch;
-    if(cell(g, nbx, nby, nbtot, i, j, k, c))
+    const double* __restrict__ v = b.v[0];
+    const double* __restrict__ ph = b.p[1];
+    const double* __restrict__ a = b.aii;
+    const double* __restrict__ tt = b.t;
+    const double* __restrict__ r0 = b.r0;
+    double* __restrict__ r = b.r;
+    double* __restrict__ x = b.x;
+    const TileWalk tw = tile_walk(g);
+    for(int t = tw.t; t < tw.t1; ++t)
     {
-        const double s = b.r[c] - alpha * b.v[0][c];
-        const double ph = b.p[1][c];
-        double r;
-        if(branch)
+        int i, j, k;
+        long c;
+        if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+        const double s = r[c] - alpha * v[c];
+        double rn;
+        if(BRANCH)
         {
-            const double sh = s * b.aii[c];
-            b.x[c] += alpha * ph + w * sh;
-            r = s - w * b.t[c];
+            const double sh = s * a[c];
+            x[c] += alpha * ph[c] + w * sh;
+            rn = s - w * tt[c];
         }
         else
         {
-            b.x[c] += alpha * ph;
-            r = s;
+            x[c] += alpha * ph[c];
+            rn = s;
         }
-        b.r[c] = r;
-        vals[0] = r * b.r0[c];
-        vals[1] = r * r;
+        r[c] = rn;
+        vals[0] += rn * r0[c];
+        vals[1] += rn * rn;
     }
+}
+
+__global__ void __launch_bounds__(NT) k_bicg_x(DevGrid g, BiBuffers b)
+{
+    BiState* st = b.st;
+    if(st->done || st->mode == 1) return;
+    double vals[2] = {0.0, 0.0};
+    const double alpha = st->alpha, w = st->w;
+    const int branch = st->branch;
+    if(branch)
+        x_tiles<true>(g, b, alpha, w, vals);
+    else
+        x_tiles<false>(g, b, alpha, w, vals);
+    block_partials<2>(vals, b.partials);
+}
+
+__global__ void k_fin_x(BiBuffers b, int nb)
+{
+    BiState* st = b.st;
+    if(st->done || st->mode == 1) return;
+    const double alpha = st->alpha, w = st->w;
+    const int branch = st->branch;
     double tot[2];
-    if(reduce_last<2>(vals, b.partials, &st->ticket[3], tot) && threadIdx.x == 0)
+    sum_partials<2>(b.partials, nb, tot);
+    if(threadIdx.x == 0)
     {
         const double r_j1 = tot[0];
         if(branch) st->beta = alpha * r_j1 / (w * st->r_j == 0 ? 1.0e-15 : (w * st->r_j));
@@ -534,25 +704,44 @@ __global__ void k_bicg_setup(BiState* st, double stop, int maxiter, long ncell)
     for(int q = 0; q < 4; ++q) st->ticket[q] = 0u;
 }
 
+static inline unsigned rblocks(const DevGrid& g)
+{
+    const long ntiles = (long)((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) * g.nz;
+    long nb = ntiles < RB ? ntiles : RB;
+    return (unsigned)((nb + 7) / 8 * 8);
+}
+
 void launch_bicg_init(const DevGrid& g, const BiBuffers& b, int cur, double stop, int maxiter, hipStream_t s)
 {
     (void)cur;
-    int nbx, nby, nbtot;
-    dim3 grid = tiles(g, nbx, nby, nbtot);
     hipLaunchKernelGGL(k_bicg_setup, dim3(1), dim3(1), 0, s, b.st, stop, maxiter, (long)g.nx * g.ny * g.nz);
-    hipLaunchKernelGGL(k_bicg_restart, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b, 1);
+    hipLaunchKernelGGL(k_bicg_init, dim3(rblocks(g)), dim3(NT), 0, s, g, b);
+    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(NT), 0, s, b, (int)rblocks(g));
 }
 
-void launch_bicg_iter(const DevGrid& g, const BiBuffers& b, int cur, hipStream_t s)
+void launch_bicg_phase(const DevGrid& g, const BiBuffers& b, int phase, hipStream_t s)
 {
-    (void)cur;
-    int nbx, nby, nbtot;
-    dim3 grid = tiles(g, nbx, nby, nbtot);
-    hipLaunchKernelGGL(k_bicg_restart, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b, 0);
-    hipLaunchKernelGGL(k_bicg_p, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b);
-    hipLaunchKernelGGL(k_bicg_v, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b);
-    hipLaunchKernelGGL(k_bicg_t, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b);
-    hipLaunchKernelGGL(k_bicg_x, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, b);
+    const dim3 grid(rblocks(g));
+    const int nb = (int)grid.x;
+    switch(phase)
+    {
+        case 0:
+            hipLaunchKernelGGL(k_bicg_p, grid, dim3(NT), 0, s, g, b);
+            hipLaunchKernelGGL(k_fin_p, dim3(1), dim3(NT), 0, s, b, nb);
+            break;
+        case 1:
+            hipLaunchKernelGGL(k_bicg_v, grid, dim3(NT), 0, s, g, b);
+            hipLaunchKernelGGL(k_fin_v, dim3(1), dim3(NT), 0, s, b, nb);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_bicg_t, grid, dim3(NT), 0, s, g, b);
+            hipLaunchKernelGGL(k_fin_t, dim3(1), dim3(NT), 0, s, b, nb);
+            break;
+        default:
+            hipLaunchKernelGGL(k_bicg_x, grid, dim3(NT), 0, s, g, b);
+            hipLaunchKernelGGL(k_fin_x, dim3(1), dim3(NT), 0, s, b, nb);
+            break;
+    }
 }
 
 // ------------------------------------------------------------------ level-set benchmark

@@ -120,7 +120,8 @@ struct BiBuffers
     BiState* st;
 };
 void launch_bicg_init(const DevGrid& g, const BiBuffers& b, int cur, double stop, int maxiter, hipStream_t s);
-void launch_bicg_iter(const DevGrid& g, const BiBuffers& b, int cur, hipStream_t s);
+// one phase of a BiCGStab iteration: 0 restart or p update, 1 v = A ph, 2 t = A sh, 3 x/r update
+void launch_bicg_phase(const DevGrid& g, const BiBuffers& b, int phase, hipStream_t s);
 
 // prescribed vortex field (benchmark_vortex3D.cpp:67-101) and the void momentum_RK3 update
 void launch_vortex_velocity(const DevGrid& g, int comp, double* q, double simtime, double DXM, hipStream_t s);
