@@ -130,6 +130,16 @@ int reef_timing_reset(reef_ctx* ctx);
  * plane stride, offset of element (-3,-3,-3) */
 int reef_ctx_sizes(reef_ctx* ctx, long* out5);
 
+/* Host-only views of the set-up a context would upload (no device needed): the grid
+ * arrays and WENO tables (axis 0..2; P/DN/DP n+10 values, tables (n+2) rows of
+ * 24/12/36), scalars4 = DXM, DYM, DZM, psi; and the compiled ghostcell::start<kind>(gcv)
+ * program applied to a REEF_LAYOUT_IFAST field on the host.  These are diagnostics with no
+ * counterpart in the reference; tests use them to check the host set-up on machines
+ * without a GPU. */
+int reef_host_grid_arrays(const reef_grid_desc* grid, double F45, int axis, double* P, double* DN,
+                          double* DP, double* qf, double* cf, double* isf, double* scalars4);
+int reef_host_ghost_apply(const reef_grid_desc* grid, double F45, int kind, int gcv, double* field_ifast);
+
 #ifdef __cplusplus
 }
 #endif

@@ -467,6 +467,67 @@ void reef_ctx_destroy(reef_ctx* c)
     delete c;
 }
 
+int reef_host_grid_arrays(const reef_grid_desc* grid, double F45, int axis, double* P, double* DN, double* DP,
+                          double* qf, double* cf, double* isf, double* scalars4)
+{
+    if(!grid || axis < 0 || axis > 2) return fail(REEF_E_ARG, "bad argument");
+    HostGrid h;
+    std::string err = build_grid(h, *grid, F45);
+    if(!err.empty()) return fail(REEF_E_UNSUPPORTED, err);
+    const int n = axis == 0 ? h.nx : axis == 1 ? h.ny : h.nz;
+    for(int q = 0; q < n + 10; ++q)
+    {
+        if(P) P[q] = h.P[axis][q];
+        if(DN) DN[q] = h.DN[axis][q];
+        if(DP) DP[q] = h.DP[axis][q];
+    }
+    for(int ip = MARGE - 1; ip < n + MARGE + 1; ++ip)
+    {
+        const int r = ip - (MARGE - 1);
+        for(int m = 0; m < 24; ++m) if(qf) qf[r * 24 + m] = h.qf[axis][(size_t)ip * 24 + m];
+        for(int m = 0; m < 12; ++m) if(cf) cf[r * 12 + m] = h.cf[axis][(size_t)ip * 12 + m];
+        for(int m = 0; m < 36; ++m) if(isf) isf[r * 36 + m] = h.isf[axis][(size_t)ip * 36 + m];
+    }
+    if(scalars4)
+    {
+        scalars4[0] = h.DXM;
+        scalars4[1] = h.DYM;
+        scalars4[2] = h.DZM;
+        scalars4[3] = h.psi;
+    }
+    return 0;
+}
+
+int reef_host_ghost_apply(const reef_grid_desc* grid, double F45, int kind, int gcv, double* f)
+{
+    if(!grid || !f || kind < 1 || kind > 4) return fail(REEF_E_ARG, "bad argument");
+    HostGrid h;
+    std::string err = build_grid(h, *grid, F45);
+    if(!err.empty()) return fail(REEF_E_UNSUPPORTED, err);
+    std::vector<GhostOp> ops;
+    err = build_ghost_program(h, kind, gcv, ops);
+    if(!err.empty()) return fail(REEF_E_UNSUPPORTED, err);
+    const long origin = h.base - 3 - 3 * h.sx - 3 * h.sxy;
+    auto compact = [&](long d) {
+        const long r = d - origin;
+        const long k = r / h.sxy, rem = r - k * h.sxy;
+        const long j = rem / h.sx, i = rem - j * h.sx;
+        return i + (long)(h.nx + 6) * (j + (long)(h.ny + 6) * k);
+    };
+    for(const GhostOp& op : ops)
+    {
+        double val;
+        if(op.src < 0)
+            val = 0.0;
+        else if(op.src & (1 << 30))
+            val = 0.0 + op.w * f[compact(op.src & ~(1 << 30))];
+        else
+            val = f[compact(op.src)];
+        f[compact(op.dst)] = val;
+    }
+    return 0;
+}
+
 int reef_ctx_sizes(reef_ctx* c, long* out5)
 {
     if(check_ctx(c) || !out5) return fail(REEF_E_ARG, "null argument");

@@ -73,6 +73,9 @@ def library():
         "reef_timing_get": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "reef_timing_reset": ([vp], ctypes.c_int),
         "reef_ctx_sizes": ([vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "reef_host_grid_arrays": ([ctypes.POINTER(GridDesc), ctypes.c_double, ctypes.c_int] + [dp] * 7, ctypes.c_int),
+        "reef_host_ghost_apply": ([ctypes.POINTER(GridDesc), ctypes.c_double, ctypes.c_int, ctypes.c_int, dp],
+                                  ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -90,6 +93,31 @@ def _check(rc):
     if rc != 0:
         msg = library().reef_last_error().decode(errors="replace")
         raise ReefError(f"reef call failed ({rc}): {msg}")
+
+
+def grid_desc(nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21):
+    """reef_grid_desc of a box; the node arrays are returned too (they must stay alive)."""
+    xn = [np.ascontiguousarray(a, dtype=np.float64) for a in (XN, YN, ZN)]
+    gd = GridDesc(nx, ny, nz, jdir, _dp(xn[0]), _dp(xn[1]), _dp(xn[2]), DYM, DZM,
+                  (ctypes.c_int * 7)(0, 21, 21, 21, 21, 21, top_group))
+    return gd, xn
+
+
+def host_grid_arrays(gd, axis, n, F45=2.1):
+    """Grid arrays, WENO tables and (DXM, DYM, DZM, psi) the library builds on the host."""
+    P, DN, DP = np.zeros(n + 10), np.zeros(n + 10), np.zeros(n + 10)
+    qf, cf, isf = np.zeros((n + 2, 2, 6, 2)), np.zeros((n + 2, 2, 6)), np.zeros((n + 2, 2, 6, 3))
+    sc = np.zeros(4)
+    _check(library().reef_host_grid_arrays(ctypes.byref(gd), F45, axis, _dp(P), _dp(DN), _dp(DP), _dp(qf), _dp(cf),
+                                           _dp(isf), _dp(sc)))
+    return P, DN, DP, qf, cf, isf, sc
+
+
+def host_ghost_apply(gd, kind, gcv, field, F45=2.1):
+    """Applies the library's compiled ghost program to a host field (i-fast layout)."""
+    f = np.ascontiguousarray(field, dtype=np.float64).copy()
+    _check(library().reef_host_ghost_apply(ctypes.byref(gd), F45, kind, gcv, _dp(f)))
+    return f
 
 
 class Domain:
