@@ -1,0 +1,198 @@
+// Adapters a REEF3D maintainer adds to src/ to run the hot path on an MI355X through
+// libreef3d_hip.so.  They derive from the reference's own strategy interfaces
+// (convection.h:33-39, reini.h:35-40, momentum.h:45-50) so driver::logic_cfd can select
+// them like any other scheme, and they talk to the device only through the C-ABI of
+// include/reef3d_hip.h (plain pointers, no device types in the signatures).
+//
+// Two ways to use them:
+//  * per call (hip_weno_flux, hip_weno_hj, hip_reini): the fields are copied to the
+//    device, the operator runs there and the result is copied back; a drop-in for
+//    one scheme, bounded by PCIe traffic;
+//  * per time step (hip_momentum_FC3): the whole momentum_FC3 step runs on the device
+//    and the fdm fields are copied in and out once per step (or left on the device
+//    between steps when nothing on the host reads them).
+#ifndef REEF3D_HIP_ADAPTERS_H_
+#define REEF3D_HIP_ADAPTERS_H_
+
+#include <cstdio>
+#include <cstdlib>
+
+#include "convection.h"
+#include "fdm.h"
+#include "field.h"
+#include "ghostcell.h"
+#include "lexer.h"
+#include "momentum.h"
+#include "reini.h"
+#include "reef3d_hip.h"
+
+namespace reef_adapt {
+
+inline void check(int rc, const char* what)
+{
+    if(rc != 0)
+    {
+        std::fprintf(stderr, "reef3d_hip: %s failed (%d): %s\n", what, rc, reef_last_error());
+        std::exit(1);     // the reference's own convention for fatal errors
+    }
+}
+
+// reef_grid_desc from the lexer of a single-rank box run (read_grid.cpp:75-432): node
+// coordinates incl. 5 ghost nodes, header spacings, and the boundary group of each side
+// taken from the gcb4 list (every entry of a side must share one group)
+inline reef_grid_desc grid_from_lexer(lexer* p)
+{
+    reef_grid_desc d;
+    d.knox = p->knox;
+    d.knoy = p->knoy;
+    d.knoz = p->knoz;
+    d.j_dir = p->j_dir;
+    d.XN = p->XN;
+    d.YN = p->YN;
+    d.ZN = p->ZN;
+    d.DYM = p->DYM;
+    d.DZM = p->DZM;
+    for(int s = 0; s < 7; ++s) d.group[s] = 0;
+    for(int q = 0; q < p->gcb4_count; ++q)
+    {
+        const int side = p->gcb4[q][3], group = p->gcb4[q][4];
+        if(side < 1 || side > 6) continue;
+        if(d.group[side] != 0 && d.group[side] != group)
+        {
+            std::fprintf(stderr, "reef3d_hip: side %d mixes boundary groups\n", side);
+            std::exit(1);
+        }
+        d.group[side] = group;
+    }
+    return d;
+}
+
+inline reef_params params_from_lexer(lexer* p)
+{
+    reef_params r;
+    r.W1 = p->W1;
+    r.W2 = p->W2;
+    r.W3 = p->W3;
+    r.W4 = p->W4;
+    r.W20 = p->W20;
+    r.W21 = p->W21;
+    r.W22 = p->W22;
+    r.F43 = p->F43;
+    r.F45 = p->F45;
+    r.dt = p->dt;
+    r.N44 = p->N44;
+    r.N46 = p->N46;
+    return r;
+}
+
+// one device context per process (the reference is single-threaded per rank)
+inline reef_ctx* context(lexer* p)
+{
+    static reef_ctx* ctx = nullptr;
+    if(!ctx)
+    {
+        reef_grid_desc d = grid_from_lexer(p);
+        reef_params r = params_from_lexer(p);
+        check(reef_ctx_create(&d, &r, &ctx), "reef_ctx_create");
+    }
+    return ctx;
+}
+
+inline void put(reef_ctx* c, int id, field& f) { check(reef_field_put(c, id, f.V, REEF_LAYOUT_REF), "put"); }
+inline void get(reef_ctx* c, int id, field& f) { check(reef_field_get(c, id, f.V, REEF_LAYOUT_REF), "get"); }
+
+// convection::start for D10 = 4 (weno_flux_nug): accumulates into a->F/G/H
+class hip_weno_flux : public convection
+{
+public:
+    explicit hip_weno_flux(lexer* p) { context(p); }
+    void start(lexer* p, fdm* a, field& q, int ipol, field& u, field& v, field& w) override
+    {
+        reef_ctx* c = context(p);
+        field* accp = &a->L;
+        if(ipol == 1) accp = &a->F;
+        if(ipol == 2) accp = &a->G;
+        if(ipol == 3) accp = &a->H;
+        field& acc = *accp;
+        const int acc_id = ipol == 1 ? REEF_F : ipol == 2 ? REEF_G : ipol == 3 ? REEF_H : REEF_L;
+        put(c, REEF_WORK0, q);
+        put(c, REEF_U, u);
+        put(c, REEF_V, v);
+        put(c, REEF_W, w);
+        put(c, acc_id, acc);
+        check(reef_convection(c, ipol, REEF_WORK0, REEF_U, REEF_V, REEF_W), "reef_convection");
+        get(c, acc_id, acc);
+    }
+};
+
+// convection::start for F35 = 5 (weno_hj_df_nug): accumulates into a->L
+class hip_weno_hj : public convection
+{
+public:
+    explicit hip_weno_hj(lexer* p) { context(p); }
+    void start(lexer* p, fdm* a, field& q, int ipol, field& u, field& v, field& w) override
+    {
+        reef_ctx* c = context(p);
+        put(c, REEF_WORK0, q);
+        put(c, REEF_U, u);
+        put(c, REEF_V, v);
+        put(c, REEF_W, w);
+        put(c, REEF_L, a->L);
+        check(reef_convection(c, 4, REEF_WORK0, REEF_U, REEF_V, REEF_W), "reef_convection");
+        get(c, REEF_L, a->L);
+    }
+};
+
+// reini::start for F40 = 3 (reini_RK3 with reinidisc_f), p->count > 0
+class hip_reini : public reini
+{
+public:
+    explicit hip_reini(lexer* p) { context(p); }
+    void start(fdm* a, lexer* p, field& f, ghostcell* pgc, ioflow* pflow) override
+    {
+        (void)a;
+        (void)pgc;
+        (void)pflow;
+        reef_ctx* c = context(p);
+        put(c, REEF_WORK1, f);
+        check(reef_reini(c, REEF_WORK1, p->F44), "reef_reini");
+        get(c, REEF_WORK1, f);
+    }
+};
+
+// momentum::start for N40 = 3 (momentum_FC3 with weno_flux_nug, weno_hj_df_nug,
+// reini_RK3, diff_void, pjm_corr, bicgstab_ijk): the whole step on the device
+class hip_momentum_FC3 : public momentum
+{
+public:
+    explicit hip_momentum_FC3(lexer* p) { context(p); }
+    void start(lexer* p, fdm* a, ghostcell* pgc, vrans* pvrans, sixdof* p6dof) override
+    {
+        (void)pgc;
+        (void)pvrans;
+        (void)p6dof;
+        reef_ctx* c = context(p);
+        put(c, REEF_U, a->u);
+        put(c, REEF_V, a->v);
+        put(c, REEF_W, a->w);
+        put(c, REEF_PRESS, a->press);
+        put(c, REEF_PHI, a->phi);
+        int iters = 0;
+        double vol[2];
+        check(reef_fc3_step(c, &iters, vol), "reef_fc3_step");
+        get(c, REEF_U, a->u);
+        get(c, REEF_V, a->v);
+        get(c, REEF_W, a->w);
+        get(c, REEF_PRESS, a->press);
+        get(c, REEF_PHI, a->phi);
+        get(c, REEF_RO, a->ro);
+        get(c, REEF_VISC, a->visc);
+        p->solveriter = iters;
+        p->volume1 = vol[0];
+        p->volume2 = vol[1];
+    }
+};
+
+}  // namespace reef_adapt
+
+#endif
