@@ -13,7 +13,15 @@ Prints one JSON line: value = interior cells x steps / wall time (Mcell-updates/
 The roofline object is for the dominant kernel (the largest total time among the solver
 kernels), measured with HIP events on the library stream inside the timed region; the
 cpu_baseline object times the CPU oracle (a single-threaded restatement of the
-reference) on a y-slab of the same tank and scales it to the same metric.
+reference) on a y-slab of the same tank and scales it to the same metric.  The
+mixed_projection object repeats the timed steps from the same initial state with the
+mixed-precision projection (FP32 BiCGStab inside FP64 defect correction to the same
+1e-10 FP64 residual) and compares.
+
+--workload solves: the standalone-solve comparison on the same operator instead: --solves
+right-hand sides ~ N(0, 1) per cell (seeds 2000, 2001, ..., mean removed), each solved
+in FP64 and in mixed precision from x = 0 to ||b - Ax|| <= 1e-10 ||b||; reports
+iterations, times, the FP64 residual of both results and max |p - p_FP64|.
 """
 import argparse
 import json
@@ -34,7 +42,12 @@ SOLVER_BYTES = {
     "bicg_t": 56,   # r, v, aii, 3 face densities in; t out
     "bicg_x": 72,   # x, ph, r, v, aii, t, r0 in; x, r out
 }
+# the same kernels of the FP32 inner solve of the mixed-precision projection
+SOLVER_BYTES.update({"bicg32_p": 24, "bicg32_v": 28, "bicg32_t": 28, "bicg32_x": 36})
 HBM_PEAK_GBS = 8000.0
+TIMED_FAMILIES = ("bicg_init", "bicg_p", "bicg_v", "bicg_t", "bicg_x", "bicg32_init", "bicg32_p", "bicg32_v",
+                  "bicg32_t", "bicg32_x", "coef32", "defect", "add32", "hj", "reini", "momentum", "ghost",
+                  "face_rho", "correct", "poisson_rhs", "heaviside", "copy")
 
 
 def make_domain(cfg):
@@ -98,12 +111,94 @@ def cpu_baseline(cfg, iters_per_step):
     }
 
 
+def timed_steps(D, steps):
+    """steps momentum_FC3 steps with per-kernel HIP event timing; (seconds, iterations, families)"""
+    D.sync()
+    D.timing(True)
+    D.timing_reset()
+    iters = 0
+    D.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        it, _ = D.momentum_fc3()
+        iters += it
+    D.sync()
+    elapsed = time.perf_counter() - t0
+    fams = {name: D.timing_get(name) for name in TIMED_FAMILIES}
+    D.timing(False)
+    return elapsed, iters, fams
+
+
+def solves(args, cfg):
+    """Standalone FP64 vs mixed-precision solves on the wave tank's step-0 operator."""
+    D, _ = make_domain(cfg)
+    nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
+    cells = nx * ny * nz
+    tol = cfg["rtol"]
+    rhs = np.zeros((nz + 6, ny + 6, nx + 6))
+    rec = []
+    for s in range(args.warmup + args.solves):
+        b = np.random.default_rng(2000 + s).standard_normal(cells)
+        b -= b.mean()
+        bn = float(np.linalg.norm(b))
+        rhs[3:-3, 3:-3, 3:-3] = b.reshape(nx, ny, nz).transpose(2, 1, 0)
+        D.put("work1", rhs)
+        D.zero("work2")
+        D.zero("work0")
+        D.sync()
+        t0 = time.perf_counter()
+        it64, _ = D.solver("work2", "work1", -tol, 20000)
+        t64 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        itm, outer, _ = D.solver_mixed("work0", "work1", -tol, 20000)
+        tm = time.perf_counter() - t0
+        if s < args.warmup:
+            continue
+        r64 = D.residual("work2", "work1") / bn
+        rm = D.residual("work0", "work1") / bn
+        x64 = D.get("work2")[3:-3, 3:-3, 3:-3]
+        xm = D.get("work0")[3:-3, 3:-3, 3:-3]
+        dmax = float(np.abs(xm - x64).max())
+        rec.append((it64, t64, itm, outer, tm, r64, rm, dmax, float(np.abs(x64).max())))
+        print(f"solve {s}: fp64 {it64} it {t64 * 1e3:.1f} ms res {r64:.2e} | mixed {itm} it/{outer} corr "
+              f"{tm * 1e3:.1f} ms res {rm:.2e} | max|p-p64| {dmax:.2e}", file=sys.stderr, flush=True)
+    a = np.array(rec)
+    t64, tm = a[:, 1].sum(), a[:, 4].sum()
+    out = {
+        "metric": "projection solves/s",
+        "value": round(len(rec) / tm, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "solves": len(rec),
+        "warmup": args.warmup,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 defect",
+        "data": "synthetic",
+        "config": {"workload": f"poisson_solves_{nx}x{ny}x{nz}: wave-tank step-0 operator, b ~ N(0,1) "
+                               f"seeds 2000.., x0 = 0, ||b - Ax|| <= {tol:g} ||b||"},
+        "fp64": {"solves_per_s": round(len(rec) / t64, 3), "ms_mean": round(t64 / len(rec) * 1e3, 2),
+                 "iters_mean": round(float(a[:, 0].mean()), 1),
+                 "final_rel_residual_max": float(a[:, 5].max())},
+        "mixed": {"ms_mean": round(tm / len(rec) * 1e3, 2), "inner_iters_mean": round(float(a[:, 2].mean()), 1),
+                  "corrections_mean": round(float(a[:, 3].mean()), 2),
+                  "final_rel_residual_max": float(a[:, 6].max())},
+        "speedup_mixed_vs_fp64": round(t64 / tm, 3),
+        "max_abs_p_minus_p64": float(a[:, 7].max()),
+        "max_rel_p_minus_p64": float((a[:, 7] / a[:, 8]).max()),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison leg")
+    ap.add_argument("--workload", choices=("fc3", "solves"), default="fc3")
+    ap.add_argument("--solves", type=int, default=100, help="right-hand sides of --workload solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the hot path is single-device); got --gpus {args.gpus}",
@@ -112,29 +207,15 @@ def main():
     rank = int(os.environ.get("RANK", "0"))
     from reef3d_amd.cases import C1
     cfg = dict(C1)
+    if args.workload == "solves":
+        solves(args, cfg)
+        return
     D, _ = make_domain(cfg)
     cells = cfg["nx"] * cfg["ny"] * cfg["nz"]
     for _ in range(args.warmup):
         D.momentum_fc3()
-    D.sync()
-    D.timing(True)
-    D.timing_reset()
-    iters = 0
-    D.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        it, _ = D.momentum_fc3()
-        iters += it
-    D.sync()
-    elapsed = time.perf_counter() - t0
-    # per-kernel event timings from the timed region
-    fams = {}
-    for name in ("bicg_init", "bicg_p", "bicg_v", "bicg_t", "bicg_x", "hj", "reini", "momentum", "ghost",
-                 "face_rho", "correct", "poisson_rhs", "heaviside", "copy"):
-        ms, n = D.timing_get(name)
-        fams[name] = (ms, n)
-    D.timing(False)
-    dom = max(SOLVER_BYTES, key=lambda k: fams[k][0])
+    elapsed, iters, fams = timed_steps(D, args.steps)
+    dom = max(("bicg_p", "bicg_v", "bicg_t", "bicg_x"), key=lambda k: fams[k][0])
     ms, n = fams[dom]
     avg_s = ms / 1e3 / max(n, 1)
     achieved = SOLVER_BYTES[dom] * cells / avg_s / 1e9
@@ -175,8 +256,30 @@ def main():
             "avg_launch_us": round(avg_s * 1e6, 2),
             "bytes_per_cell": SOLVER_BYTES[dom],
         },
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in fams.items()},
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in fams.items() if v[1]},
     }
+    if not args.no_mixed:
+        del D
+        Dm, _ = make_domain(cfg)
+        Dm.solver_config("mixed", inner_tol=1e-5)
+        for _ in range(args.warmup):
+            Dm.momentum_fc3()
+        el_m, it_m, fams_m = timed_steps(Dm, args.steps)
+        _, _, rel = Dm.solver_last()
+        dm = max(("bicg32_p", "bicg32_v", "bicg32_t", "bicg32_x"), key=lambda k: fams_m[k][0])
+        ms_m, n_m = fams_m[dm]
+        avg_m = ms_m / 1e3 / max(n_m, 1)
+        out["mixed_projection"] = {
+            "ms_per_step": round(el_m / args.steps * 1e3, 3),
+            "value": round(cells * args.steps / el_m / 1e6, 4),
+            "speedup_vs_fp64": round(elapsed / el_m, 3),
+            "inner_iters_per_step": it_m / args.steps,
+            "last_solve_rel_residual": rel,
+            "dominant_kernel": dm,
+            "dominant_achieved_GBs": round(SOLVER_BYTES[dm] * cells / avg_m / 1e9, 1),
+            "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in fams_m.items() if v[1]},
+        }
+        del Dm
     if rank == 0 and not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, iters / args.steps)
     if rank == 0:

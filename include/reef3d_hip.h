@@ -43,6 +43,12 @@ enum reef_field_id
 
 enum reef_layout { REEF_LAYOUT_IFAST = 0, REEF_LAYOUT_REF = 1 };
 
+enum reef_solver_precision
+{
+    REEF_SOLVER_FP64 = 0,   /* bicgstab_ijk in FP64 (the reference solver) */
+    REEF_SOLVER_MIXED = 1   /* FP32 inner BiCGStab inside FP64 defect correction */
+};
+
 /* Grid of one box domain: what lexer holds after read_grid + gridini
  * (read_grid.cpp:75-432, grid.cpp:54-150). */
 typedef struct reef_grid_desc
@@ -102,6 +108,31 @@ int reef_fluid_update(reef_ctx* ctx, double* vol2);
  * rhs_id holds the right-hand side in field layout (interior cells). */
 int reef_poisson_solve(reef_ctx* ctx, int x_id, int rhs_id, double stop, int maxiter,
                        int* iters, double* residual);
+
+/* The same solve in mixed precision; no counterpart in the reference, which solves in
+ * FP64 only (it replaces bicgstab_ijk::start, bicgstab_ijk.cpp:51-123, where the caller
+ * accepts a result at the same FP64 residual).  The FP64 defect r = b - A x with the
+ * reference operator is corrected by FP32 BiCGStab solves (FP32 face conductances and
+ * Jacobi preconditioner, FP64 dot products) until ||r||_2 / N < stop (stop >= 0, the
+ * reference criterion) or ||r||_2 < |stop| * ||b - A x0||_2 (stop < 0).  Each inner
+ * solve reduces its residual by inner_tol (reef_solver_config, default 1e-5) or by what is
+ * left to reach the criterion; the loop also ends when a correction no longer halves the
+ * FP64 defect.  inner_iters: summed FP32 iterations (restarts included, as solveriter
+ * counts them); outer_iters: corrections applied; rel_residual: final
+ * ||b - A x||_2 / ||b - A x0||_2 in FP64.  x_id and rhs_id must differ. */
+int reef_poisson_solve_mixed(reef_ctx* ctx, int x_id, int rhs_id, double stop, int maxiter,
+                             int* inner_iters, int* outer_iters, double* rel_residual);
+
+/* ||rhs - A x||_2 in FP64 with the operator assembled from REEF_PHI: res_calc
+ * (bicgstab_ijk.cpp:314-338) without its 1/N. */
+int reef_poisson_residual(reef_ctx* ctx, int x_id, int rhs_id, double* norm);
+
+/* Solver of the projection inside reef_pjm_corr / reef_fc3_step: precision
+ * REEF_SOLVER_FP64 (default) or REEF_SOLVER_MIXED, the inner relative tolerance and the
+ * maximum number of corrections of the mixed form. */
+int reef_solver_config(reef_ctx* ctx, int precision, double inner_tol, int max_outer);
+/* counts of the last mixed-precision solve */
+int reef_solver_last(reef_ctx* ctx, long* inner_iters, long* outer_iters, double* rel_residual);
 
 /* pressure::start(a, p, ppois, psolv, pgc, pflow, u, v, w, alpha) with pjm_corr
  * (pjm_corr.cpp:90-121): projection of the three velocity fields, REEF_PRESS updated. */

@@ -49,6 +49,8 @@ def lib():
         L.orc_reini.argtypes = [ctypes.c_void_p, dp, ctypes.c_int]
         L.orc_fluid_update.argtypes = [ctypes.c_void_p] + [dp] * 4
         L.orc_poisson.argtypes = [ctypes.c_void_p] + [dp] * 4
+        L.orc_residual.argtypes = [ctypes.c_void_p, dp, dp, dp, dp]
+        L.orc_residual.restype = ctypes.c_double
         L.orc_bicgstab.argtypes = [ctypes.c_void_p, dp, dp, dp, ctypes.c_double, ctypes.c_int, dp]
         L.orc_bicgstab.restype = ctypes.c_int
         L.orc_pjm.argtypes = [ctypes.c_void_p] + [dp] * 6 + [ctypes.c_double]
@@ -133,6 +135,11 @@ class Domain:
         M = np.ascontiguousarray(M)
         it = lib().orc_bicgstab(self.h, _p(x), _p(M), _p(rhs), stop, maxiter, _p(res))
         return it, res[0]
+
+    def residual(self, x, M, rhs, r=None):
+        """||rhs - A x||_2 in the order of res_calc; r (field layout) receives the residual."""
+        M = np.ascontiguousarray(M)
+        return lib().orc_residual(self.h, _p(x), _p(M), _p(rhs), _p(r) if r is not None else None)
 
     def pjm(self, phi, u, v, w, press, pcorr, alpha):
         return lib().orc_pjm(self.h, _p(phi), _p(u), _p(v), _p(w), _p(press), _p(pcorr), alpha)

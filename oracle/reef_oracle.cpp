@@ -862,6 +862,26 @@ int bicgstab(orc_domain* d, double* xf, const double* M, const double* rhsvec, d
     return it;
 }
 
+// res_calc (bicgstab_ijk.cpp:314-338) without the 1/N: r = rhs - A x per interior cell,
+// written to rout (field layout) when given; returns ||r||_2
+double residual(orc_domain* d, const double* xf, const double* M, const double* rhsvec, double* rout)
+{
+    long n = d->sx * d->sy * d->sz;
+    Bicg b;
+    b.d = d;
+    b.M = M;
+    b.N = d->cellnum;
+    b.x.assign(n, 0.0);
+    b.loop([&](long c, long) { b.x[c] = xf[c]; });
+    double resi = 0.0;
+    b.loop([&](long c, long k) {
+        const double y = rhsvec[k] - (b.Ax(b.x, c, k));
+        if(rout) rout[c] = y;
+        resi += y * y;
+    });
+    return sqrt(resi);
+}
+
 // pjm_corr::start (pjm_corr.cpp:90-183) with poisson_pcorr + bicgstab_ijk
 int pjm(orc_domain* d, const double* phi, double* u, double* v, double* w, double* press, double* pcorr, double alpha)
 {
@@ -1254,6 +1274,11 @@ void orc_poisson(orc_domain* d, const double* phi, const double* pcorr, double* 
 int orc_bicgstab(orc_domain* d, double* x, const double* M7, const double* rhs, double stop, int maxiter, double* residual)
 {
     return bicgstab(d, x, M7, rhs, stop, maxiter, residual);
+}
+
+double orc_residual(orc_domain* d, const double* x, const double* M7, const double* rhs, double* r)
+{
+    return residual(d, x, M7, rhs, r);
 }
 
 int orc_pjm(orc_domain* d, const double* phi, double* u, double* v, double* w, double* press, double* pcorr, double alpha)

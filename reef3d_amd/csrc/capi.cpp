@@ -49,7 +49,19 @@ struct reef_ctx
     double *ls = nullptr, *frk1 = nullptr, *frk2 = nullptr, *rk1 = nullptr, *rk2 = nullptr;
     double *urk1 = nullptr, *vrk1 = nullptr, *wrk1 = nullptr, *urk2 = nullptr, *vrk2 = nullptr, *wrk2 = nullptr;
     double* rhs = nullptr;
-    BiBuffers bi{};
+    BiVec<double> bi{};
+    SolverOp64 op64{};
+    double *rfx = nullptr, *rfy = nullptr, *rfz = nullptr, *aii64 = nullptr;
+    // mixed-precision projection: FP32 inner vectors and operator, allocated on first use
+    int precision = REEF_SOLVER_FP64;
+    double inner_tol = 1.0e-5;
+    int max_outer = 40;
+    BiVec<float> bf{};
+    SolverOp32 op32{};
+    float *kx32 = nullptr, *ky32 = nullptr, *kz32 = nullptr, *aii32 = nullptr, *r32 = nullptr;
+    double* norm_dev = nullptr;
+    long last_inner = 0, last_outer = 0;
+    double last_rel = 0.0;
     double* vol_dev = nullptr;
     unsigned int* vol_ticket = nullptr;
     double* partials = nullptr;
@@ -66,6 +78,15 @@ struct reef_ctx
         allocs.push_back(q);
         if(hipMemsetAsync(q, 0, n * sizeof(double), stream) != hipSuccess) return fail(REEF_E_HIP, "hipMemset failed");
         *p = (double*)q;
+        return 0;
+    }
+    int falloc(float** p, size_t n)
+    {
+        void* q = nullptr;
+        if(hipMalloc(&q, n * sizeof(float)) != hipSuccess) return fail(REEF_E_NOMEM, "hipMalloc failed");
+        allocs.push_back(q);
+        if(hipMemsetAsync(q, 0, n * sizeof(float), stream) != hipSuccess) return fail(REEF_E_HIP, "hipMemset failed");
+        *p = (float*)q;
         return 0;
     }
     template<class T> int dupload(const T* host, size_t n, const T** out)
@@ -200,31 +221,24 @@ int do_fluid_update(reef_ctx* c, double* vol2)
 }
 
 const char* const BICG_NAMES[4] = {"bicg_p", "bicg_v", "bicg_t", "bicg_x"};
+const char* const BICG32_NAMES[4] = {"bicg32_p", "bicg32_v", "bicg32_t", "bicg32_x"};
 
-// BiCGStab on the operator assembled from REEF_PHI; x holds the initial guess
-int do_solve(reef_ctx* c, double* x, const double* rhs, double stop, int maxiter, int* iters, double* residual)
+// BiCGStab iterations after launch_bicg_init: enqueued in batches, the device state says
+// when the solve ended
+template<class V, class Op>
+int run_bicg(reef_ctx* c, const Op& op, const BiVec<V>& b, int maxiter, const char* const* names, BiState& hs)
 {
     const DevGrid& g = c->dg;
-    BiBuffers b = c->bi;
-    b.x = x;
-    b.rhs = (double*)rhs;
-    c->tbegin("face_rho");
-    launch_face_rho(g, c->field[REEF_PHI], c->prm.W1, c->prm.W3, b.rfx, b.rfy, b.rfz, b.aii, c->stream);
-    c->tend();
-    c->tbegin("bicg_init");
-    launch_bicg_init(g, b, 0, stop, maxiter, c->stream);
-    c->tend();
-    // iterations are enqueued in batches; the device state says when the solve ended
     const int batch = 8;
-    BiState hs;
     for(int launched = 0;;)
     {
         for(int q = 0; q < batch; ++q)
             for(int ph = 0; ph < 4; ++ph)
             {
-                c->tbegin(BICG_NAMES[ph]);
-                launch_bicg_phase(g, b, ph, c->stream);
+                c->tbegin(names[ph]);
+                launch_bicg_phase(g, op, b, ph, 0, c->stream);
                 c->tend();
+                launch_bicg_phase(g, op, b, ph, 1, c->stream);
             }
         launched += batch;
         HIPCHK(hipMemcpyAsync(&hs, b.st, sizeof(BiState), hipMemcpyDeviceToHost, c->stream));
@@ -235,8 +249,122 @@ int do_solve(reef_ctx* c, double* x, const double* rhs, double stop, int maxiter
         // the solve ends within maxiter + (restarts) batches
         if(launched > 4 * (maxiter + 16)) return fail(REEF_E_HIP, "BiCGStab did not terminate");
     }
+    return 0;
+}
+
+// BiCGStab on the operator assembled from REEF_PHI; x holds the initial guess
+int do_solve(reef_ctx* c, double* x, const double* rhs, double stop, int maxiter, int* iters, double* residual)
+{
+    const DevGrid& g = c->dg;
+    BiVec<double> b = c->bi;
+    b.x = x;
+    b.rhs = rhs;
+    c->tbegin("face_rho");
+    launch_face_rho(g, c->field[REEF_PHI], c->prm.W1, c->prm.W3, c->rfx, c->rfy, c->rfz, c->aii64, c->stream);
+    c->tend();
+    c->tbegin("bicg_init");
+    launch_bicg_init(g, c->op64, b, stop, maxiter, c->stream);
+    c->tend();
+    BiState hs;
+    int rc;
+    if((rc = run_bicg(c, c->op64, b, maxiter, BICG_NAMES, hs))) return rc;
     if(iters) *iters = hs.iter;
     if(residual) *residual = hs.residual;
+    return 0;
+}
+
+int alloc_mixed(reef_ctx* c)
+{
+    if(c->r32) return 0;
+    const size_t n = (size_t)c->dg.nalloc;
+    BiVec<float>& b = c->bf;
+    int rc;
+    float* fx = nullptr;
+    float** bw[] = {&b.r, &b.r0, &b.p, &b.ph, &b.v, &b.t, &fx, &c->kx32, &c->ky32, &c->kz32, &c->aii32, &c->r32};
+    for(float** p : bw)
+        if((rc = c->falloc(p, n))) return rc;
+    b.x = fx;
+    b.rhs = c->r32;
+    b.aii = c->aii32;
+    b.partials = c->bi.partials;
+    double* stmem = nullptr;
+    if((rc = c->dalloc(&stmem, (sizeof(BiState) + 7) / 8 + 8))) return rc;
+    b.st = (BiState*)stmem;
+    if(!c->norm_dev && (rc = c->dalloc(&c->norm_dev, 8))) return rc;
+    const HostGrid& h = c->hg;
+    const float** tab[3] = {&c->op32.iDXN, &c->op32.iDYN, &c->op32.iDZN};
+    for(int a = 0; a < 3; ++a)
+    {
+        std::vector<float> inv(h.DN[a].size());
+        for(size_t q = 0; q < inv.size(); ++q) inv[q] = h.DN[a][q] != 0.0 ? (float)(1.0 / h.DN[a][q]) : 0.0f;
+        if((rc = c->dupload(inv.data(), inv.size(), tab[a]))) return rc;
+    }
+    c->op32.kx = c->kx32;
+    c->op32.ky = c->ky32;
+    c->op32.kz = c->kz32;
+    return 0;
+}
+
+// Mixed-precision projection solve: FP64 defect correction around an FP32 BiCGStab.
+// Outer step: r = b - A x with the FP64 operator; stop when ||r||_2 meets the criterion
+// (stop >= 0: ||r||_2 / N < stop as bicgstab_ijk.cpp:264; stop < 0: ||r||_2 < |stop| *
+// ||b - A x0||_2); else solve A d = r in FP32 to the relative tolerance inner_tol (or
+// what the remaining reduction needs) and x += d.  iters receives the summed inner
+// iterations, outer the number of corrections.
+int do_solve_mixed(reef_ctx* c, double* x, const double* rhs, double stop, int maxiter, int* iters, int* outer,
+                   double* rel_residual)
+{
+    int rc;
+    if((rc = alloc_mixed(c))) return rc;
+    const DevGrid& g = c->dg;
+    c->tbegin("face_rho");
+    launch_face_rho(g, c->field[REEF_PHI], c->prm.W1, c->prm.W3, c->rfx, c->rfy, c->rfz, nullptr, c->stream);
+    c->tend();
+    c->tbegin("coef32");
+    launch_coef32(g, c->op64.rfx, c->op64.rfy, c->op64.rfz, c->kx32, c->ky32, c->kz32, c->aii32, c->stream);
+    c->tend();
+    const double ncell = (double)g.nx * g.ny * g.nz;
+    double ref = 0.0, target = 0.0, prev = 0.0, res = 0.0;
+    long inner_total = 0;
+    int nouter = 0;
+    for(;;)
+    {
+        c->tbegin("defect");
+        launch_defect(g, c->op64, x, rhs, c->r32, c->bf.x, c->bi.partials, c->norm_dev, c->stream);
+        c->tend();
+        HIPCHK(hipMemcpyAsync(&res, c->norm_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if(c->timing) c->tcollect();
+        if(!std::isfinite(res)) return fail(REEF_E_HIP, "mixed-precision solve: non-finite defect");
+        if(nouter == 0)
+        {
+            ref = res;
+            target = stop >= 0.0 ? stop * ncell : -stop * res;
+        }
+        // converged, out of budget, or no longer contracting (the FP32 inner solve has
+        // reached what FP32 can resolve)
+        if(res <= target || res == 0.0 || nouter >= c->max_outer || inner_total >= maxiter) break;
+        if(nouter > 0 && res > 0.5 * prev) break;
+        prev = res;
+        const double need = 0.3 * target / res;
+        const double tol = need > c->inner_tol ? need : c->inner_tol;
+        c->tbegin("bicg32_init");
+        launch_bicg_init(g, c->op32, c->bf, -tol, (int)(maxiter - inner_total), c->stream);
+        c->tend();
+        BiState hs;
+        if((rc = run_bicg(c, c->op32, c->bf, (int)(maxiter - inner_total), BICG32_NAMES, hs))) return rc;
+        inner_total += hs.iter;
+        c->tbegin("add32");
+        launch_add32(g, x, c->bf.x, c->stream);
+        c->tend();
+        ++nouter;
+    }
+    c->last_inner = inner_total;
+    c->last_outer = nouter;
+    c->last_rel = ref > 0.0 ? res / ref : 0.0;
+    if(iters) *iters = (int)inner_total;
+    if(outer) *outer = nouter;
+    if(rel_residual) *rel_residual = c->last_rel;
     return 0;
 }
 
@@ -254,7 +382,11 @@ int do_pjm(reef_ctx* c, double* u, double* v, double* w, double alpha, int* iter
     double* pc = c->field[REEF_PCORR];
     launch_fill(pc, g.nalloc, 0.0, c->stream);
     int it = 0;
-    if((rc = do_solve(c, pc, c->rhs, c->prm.N44, c->prm.N46, &it, nullptr))) return rc;
+    if(c->precision == REEF_SOLVER_MIXED)
+        rc = do_solve_mixed(c, pc, c->rhs, c->prm.N44, c->prm.N46, &it, nullptr, nullptr);
+    else
+        rc = do_solve(c, pc, c->rhs, c->prm.N44, c->prm.N46, &it, nullptr);
+    if(rc) return rc;
     if((rc = c->ghost(pc, 4, 40))) return rc;
     c->tbegin("correct");
     launch_correct(g, pc, c->field[REEF_PHI], c->prm.W1, c->prm.W3, adt, u, v, w, c->field[REEF_PRESS], c->stream);
@@ -433,10 +565,14 @@ int reef_ctx_create(const reef_grid_desc* grid, const reef_params* prm, reef_ctx
                        &c->urk2, &c->vrk2, &c->wrk2, &c->rhs};
     for(double** p : work)
         if((rc = c->dalloc(p, n))) return rc;
-    BiBuffers& b = c->bi;
-    double** bw[] = {&b.r, &b.r0, &b.p[0], &b.p[1], &b.v[0], &b.v[1], &b.t, &b.aii, &b.rfx, &b.rfy, &b.rfz};
+    BiVec<double>& b = c->bi;
+    double** bw[] = {&b.r, &b.r0, &b.p, &b.ph, &b.v, &b.t, &c->aii64, &c->rfx, &c->rfy, &c->rfz};
     for(double** p : bw)
         if((rc = c->dalloc(p, n))) return rc;
+    b.aii = c->aii64;
+    c->op64.rfx = c->rfx;
+    c->op64.rfy = c->rfy;
+    c->op64.rfz = c->rfz;
     const long nblocks = ((long)((g.nx + 63) / 64) * ((g.ny + 3) / 4) * g.nz + 7) / 8 * 8;
     if((rc = c->dalloc(&c->partials, (size_t)nblocks * 4 + 64))) return rc;
     if((rc = c->dalloc(&b.partials, (size_t)nblocks * 4 + 64))) return rc;
@@ -648,6 +784,51 @@ int reef_poisson_solve(reef_ctx* c, int x_id, int rhs_id, double stop, int maxit
     double *x = field_ptr(c, x_id), *rhs = field_ptr(c, rhs_id);
     if(!x || !rhs || maxiter < 0) return fail(REEF_E_ARG, "bad field id or maxiter");
     return do_solve(c, x, rhs, stop, maxiter, iters, residual);
+}
+
+int reef_poisson_solve_mixed(reef_ctx* c, int x_id, int rhs_id, double stop, int maxiter, int* inner_iters,
+                             int* outer_iters, double* rel_residual)
+{
+    if(check_ctx(c)) return REEF_E_ARG;
+    double *x = field_ptr(c, x_id), *rhs = field_ptr(c, rhs_id);
+    if(!x || !rhs || maxiter < 0) return fail(REEF_E_ARG, "bad field id or maxiter");
+    if(x == rhs) return fail(REEF_E_ARG, "x and rhs must be different fields");
+    return do_solve_mixed(c, x, rhs, stop, maxiter, inner_iters, outer_iters, rel_residual);
+}
+
+int reef_poisson_residual(reef_ctx* c, int x_id, int rhs_id, double* norm)
+{
+    if(check_ctx(c)) return REEF_E_ARG;
+    double *x = field_ptr(c, x_id), *rhs = field_ptr(c, rhs_id);
+    if(!x || !rhs || !norm) return fail(REEF_E_ARG, "bad field id or null norm");
+    int rc;
+    if(!c->norm_dev && (rc = c->dalloc(&c->norm_dev, 8))) return rc;
+    const DevGrid& g = c->dg;
+    launch_face_rho(g, c->field[REEF_PHI], c->prm.W1, c->prm.W3, c->rfx, c->rfy, c->rfz, nullptr, c->stream);
+    launch_defect(g, c->op64, x, rhs, nullptr, nullptr, c->bi.partials, c->norm_dev, c->stream);
+    HIPCHK(hipMemcpyAsync(norm, c->norm_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int reef_solver_config(reef_ctx* c, int precision, double inner_tol, int max_outer)
+{
+    if(check_ctx(c)) return REEF_E_ARG;
+    if(precision != REEF_SOLVER_FP64 && precision != REEF_SOLVER_MIXED) return fail(REEF_E_ARG, "bad precision");
+    if(!(inner_tol > 0.0 && inner_tol < 1.0) || max_outer < 1) return fail(REEF_E_ARG, "bad inner_tol or max_outer");
+    c->precision = precision;
+    c->inner_tol = inner_tol;
+    c->max_outer = max_outer;
+    return 0;
+}
+
+int reef_solver_last(reef_ctx* c, long* inner_iters, long* outer_iters, double* rel_residual)
+{
+    if(check_ctx(c)) return REEF_E_ARG;
+    if(inner_iters) *inner_iters = c->last_inner;
+    if(outer_iters) *outer_iters = c->last_outer;
+    if(rel_residual) *rel_residual = c->last_rel;
+    return 0;
 }
 
 int reef_pjm_corr(reef_ctx* c, int u_id, int v_id, int w_id, double alpha, int* iters)

@@ -230,10 +230,12 @@ __device__ __forceinline__ double roface(double a, double b, double psi, double 
 
 // Couplings of poisson_pcorr.cpp:95-123 before the boundary folding: T[0..5] for the
 // +x, -x, +y, -y, +z, -z faces; face densities density_f::roface stored per "+" face.
-struct Coupling
+template<class V>
+struct CouplingV
 {
-    double T[6];
+    V T[6];
 };
+using Coupling = CouplingV<double>;
 
 __device__ __forceinline__ Coupling couplings(const DevGrid& g, const double* __restrict__ rfx,
                                               const double* __restrict__ rfy, const double* __restrict__ rfz, long c,
@@ -250,33 +252,64 @@ __device__ __forceinline__ Coupling couplings(const DevGrid& g, const double* __
     return q;
 }
 
-__device__ __forceinline__ double diag_of(const DevGrid& g, const Coupling& q)
+// The FP64 operator recomputes its couplings from the face densities per application
+// (bit-identical to the reference's assembly).  The FP32 operator of the mixed-precision
+// inner solve reads face conductances 1/(rho_f * DP) rounded to FP32 and scales them by
+// the FP32 1/DN of the cell: T[0] = kx(c) / DXN(i), T[1] = kx(c-1) / DXN(i), ...
+__device__ __forceinline__ CouplingV<double> coupling_of(const DevGrid& g, const SolverOp64& op, long c, int i, int j,
+                                                         int k)
 {
-    return q.T[0] + q.T[1] + q.T[2] * g.ydir + q.T[3] * g.ydir + q.T[4] + q.T[5];
+    return couplings(g, op.rfx, op.rfy, op.rfz, c, i, j, k);
+}
+
+__device__ __forceinline__ CouplingV<float> coupling_of(const DevGrid& g, const SolverOp32& op, long c, int i, int j,
+                                                        int k)
+{
+    const float* __restrict__ kx = op.kx;
+    const float* __restrict__ ky = op.ky;
+    const float* __restrict__ kz = op.kz;
+    const float ix = op.iDXN[i + MARGE], iy = op.iDYN[j + MARGE], iz = op.iDZN[k + MARGE];
+    CouplingV<float> q;
+    q.T[0] = kx[c] * ix;
+    q.T[1] = kx[c - 1] * ix;
+    q.T[2] = ky[c] * iy;
+    q.T[3] = ky[c - g.sx] * iy;
+    q.T[4] = kz[c] * iz;
+    q.T[5] = kz[c - g.sxy] * iz;
+    return q;
+}
+
+template<class V>
+__device__ __forceinline__ V diag_of(const DevGrid& g, const CouplingV<V>& q)
+{
+    const V yd = (V)g.ydir;
+    return q.T[0] + q.T[1] + q.T[2] * yd + q.T[3] * yd + q.T[4] + q.T[5];
 }
 
 // y = A x at cell c with the folded operator of poisson_pcorr.cpp:124-230: boundary
 // neighbours (pressure-correction ghosts are 0) drop out, the diagonal keeps them.
 // Summation order of matvec_std (bicgstab_ijk.cpp:296-310); 5 points in 2-D
 // (bicgstab_ijk_2D.cpp:308-320).
-__device__ __forceinline__ double apply_A(const DevGrid& g, const Coupling& q, const double* __restrict__ x, long c,
-                                          int i, int j, int k)
+template<class V>
+__device__ __forceinline__ V apply_A(const DevGrid& g, const CouplingV<V>& q, const V* __restrict__ x, long c, int i,
+                                     int j, int k)
 {
-    const double Mp = diag_of(g, q);
-    const double Mn = i == g.nx - 1 ? 0.0 : -q.T[0];
-    const double Ms = i == 0 ? 0.0 : -q.T[1];
-    const double Mt = k == g.nz - 1 ? 0.0 : -q.T[4];
-    const double Mb = k == 0 ? 0.0 : -q.T[5];
-    const double xn = i == g.nx - 1 ? 0.0 : x[c + 1];
-    const double xs = i == 0 ? 0.0 : x[c - 1];
-    const double xt = k == g.nz - 1 ? 0.0 : x[c + g.sxy];
-    const double xb = k == 0 ? 0.0 : x[c - g.sxy];
+    const V Mp = diag_of(g, q);
+    const V Mn = i == g.nx - 1 ? V(0) : -q.T[0];
+    const V Ms = i == 0 ? V(0) : -q.T[1];
+    const V Mt = k == g.nz - 1 ? V(0) : -q.T[4];
+    const V Mb = k == 0 ? V(0) : -q.T[5];
+    const V xn = i == g.nx - 1 ? V(0) : x[c + 1];
+    const V xs = i == 0 ? V(0) : x[c - 1];
+    const V xt = k == g.nz - 1 ? V(0) : x[c + g.sxy];
+    const V xb = k == 0 ? V(0) : x[c - g.sxy];
     if(g.jdir == 1)
     {
-        const double Mw = j == g.ny - 1 ? 0.0 : -q.T[2] * g.ydir;
-        const double Me = j == 0 ? 0.0 : -q.T[3] * g.ydir;
-        const double xw = j == g.ny - 1 ? 0.0 : x[c + g.sx];
-        const double xe = j == 0 ? 0.0 : x[c - g.sx];
+        const V yd = (V)g.ydir;
+        const V Mw = j == g.ny - 1 ? V(0) : -q.T[2] * yd;
+        const V Me = j == 0 ? V(0) : -q.T[3] * yd;
+        const V xw = j == g.ny - 1 ? V(0) : x[c + g.sx];
+        const V xe = j == 0 ? V(0) : x[c - g.sx];
         return Mp * x[c] + Mn * xn + Ms * xs + Mw * xw + Me * xe + Mt * xt + Mb * xb;
     }
     return Mp * x[c] + Mn * xn + Ms * xs + Mt * xt + Mb * xb;
@@ -352,6 +385,11 @@ void launch_correct(const DevGrid& g, const double* pcorr, const double* phi, do
 // The solver kernels run a fixed grid of RB blocks (8 per CU); block b works through a
 // contiguous range of 64x4 tiles (z fastest) inside the slab of XCD b % 8, so stencil
 // neighbours mostly hit L1 or that XCD's L2, and every reduction has RB partials.
+//
+// Every kernel is a template on the vector type V and the operator: V = double with the
+// FP64 operator is the reference solver; V = float with the FP32 operator is the inner
+// solve of the mixed-precision projection.  Dot products are accumulated in FP64 and the
+// scalars (alpha, beta, omega) are formed in FP64 in both cases.
 constexpr int RB = 2048;
 
 struct TileWalk
@@ -390,48 +428,48 @@ __device__ __forceinline__ bool tile_cell(const DevGrid& g, const TileWalk& w, i
     return true;
 }
 
+__device__ __forceinline__ double sq(double a) { return a * a; }
+
 // restart / initial residual: r = b - A x, r0 = p = r, ph = aii p, r_j = (r, r0)
 // (restart label of bicgstab_ijk.cpp:136-148; res_calc :314-338 gives the same sums)
-__device__ __forceinline__ void restart_tiles(const DevGrid& g, const BiBuffers& b, double& rr)
+template<class V, class Op>
+__device__ __forceinline__ void restart_tiles(const DevGrid& g, const Op& op, const BiVec<V>& b, double& rr)
 {
-    const double* __restrict__ rhs = b.rhs;
-    const double* __restrict__ x = b.x;
-    const double* __restrict__ rfx = b.rfx;
-    const double* __restrict__ rfy = b.rfy;
-    const double* __restrict__ rfz = b.rfz;
-    const double* __restrict__ aii = b.aii;
-    double* __restrict__ r = b.r;
-    double* __restrict__ r0 = b.r0;
-    double* __restrict__ p = b.p[0];
-    double* __restrict__ ph = b.p[1];
+    const V* __restrict__ rhs = b.rhs;
+    const V* __restrict__ x = b.x;
+    const V* __restrict__ aii = b.aii;
+    V* __restrict__ r = b.r;
+    V* __restrict__ r0 = b.r0;
+    V* __restrict__ p = b.p;
+    V* __restrict__ ph = b.ph;
     const TileWalk tw = tile_walk(g);
     for(int t = tw.t; t < tw.t1; ++t)
     {
         int i, j, k;
         long c;
         if(!tile_cell(g, tw, t, i, j, k, c)) continue;
-        const Coupling q = couplings(g, rfx, rfy, rfz, c, i, j, k);
-        const double rv = rhs[c] - (apply_A(g, q, x, c, i, j, k));
+        const CouplingV<V> q = coupling_of(g, op, c, i, j, k);
+        const V rv = rhs[c] - (apply_A(g, q, x, c, i, j, k));
         r[c] = rv;
         r0[c] = rv;
         p[c] = rv;
         ph[c] = rv * aii[c];
-        rr += rv * rv;
+        rr += sq(rv);
     }
 }
 
-__global__ void __launch_bounds__(NT) k_bicg_init(DevGrid g, BiBuffers b)
+template<class V, class Op>
+__global__ void __launch_bounds__(NT) k_bicg_init(DevGrid g, Op op, BiVec<V> b)
 {
     double vals[1] = {0.0};
-    restart_tiles(g, b, vals[0]);
+    restart_tiles(g, op, b, vals[0]);
     block_partials<1>(vals, b.partials);
 }
 
-__global__ void k_fin_init(BiBuffers b, int nb)
+__global__ void k_fin_init(const double* __restrict__ partials, BiState* st, int nb)
 {
-    BiState* st = b.st;
     double tot[1];
-    sum_partials<1>(b.partials, nb, tot);
+    sum_partials<1>(partials, nb, tot);
     if(threadIdx.x == 0)
     {
         st->r_j = tot[0];
@@ -446,43 +484,43 @@ __global__ void k_fin_init(BiBuffers b, int nb)
 
 // start of an iteration: either the pending restart, or p = r + beta (p - w v)
 // (bicgstab_ijk.cpp:234-235); then ph = aii p (precon_solve)
-__global__ void __launch_bounds__(NT) k_bicg_p(DevGrid g, BiBuffers b)
+template<class V, class Op>
+__global__ void __launch_bounds__(NT) k_bicg_p(DevGrid g, Op op, BiVec<V> b)
 {
     BiState* st = b.st;
     if(st->done) return;
     if(st->mode == 1)
     {
         double vals[1] = {0.0};
-        restart_tiles(g, b, vals[0]);
+        restart_tiles(g, op, b, vals[0]);
         block_partials<1>(vals, b.partials);
         return;
     }
     if(st->mode == 2) return;     // p = r and ph were set by the initial residual
-    const double beta = st->beta, w = st->w;
-    const double* __restrict__ r = b.r;
-    const double* __restrict__ v = b.v[0];
-    const double* __restrict__ aii = b.aii;
-    double* __restrict__ p = b.p[0];
-    double* __restrict__ ph = b.p[1];
+    const V beta = (V)st->beta, w = (V)st->w;
+    const V* __restrict__ r = b.r;
+    const V* __restrict__ v = b.v;
+    const V* __restrict__ aii = b.aii;
+    V* __restrict__ p = b.p;
+    V* __restrict__ ph = b.ph;
     const TileWalk tw = tile_walk(g);
     for(int t = tw.t; t < tw.t1; ++t)
     {
         int i, j, k;
         long c;
         if(!tile_cell(g, tw, t, i, j, k, c)) continue;
-        const double pn = r[c] + beta * (p[c] - w * v[c]);
+        const V pn = r[c] + beta * (p[c] - w * v[c]);
         p[c] = pn;
         ph[c] = pn * aii[c];
     }
 }
 
 // totals of a restart done by k_bicg_p: ++solveriter, residual = |r|/N (bicgstab_ijk.cpp:183-191)
-__global__ void k_fin_p(BiBuffers b, int nb)
+__global__ void k_fin_p(const double* __restrict__ partials, BiState* st, int nb)
 {
-    BiState* st = b.st;
     if(st->done || st->mode != 1) return;
     double tot[1];
-    sum_partials<1>(b.partials, nb, tot);
+    sum_partials<1>(partials, nb, tot);
     if(threadIdx.x == 0)
     {
         st->r_j = tot[0];
@@ -495,41 +533,38 @@ __global__ void k_fin_p(BiBuffers b, int nb)
 }
 
 // v = A ph; sigma = (v, r0), |v|, |r|; alpha and the restart tests (bicgstab_ijk.cpp:151-191)
-__global__ void __launch_bounds__(NT) k_bicg_v(DevGrid g, BiBuffers b)
+template<class V, class Op>
+__global__ void __launch_bounds__(NT) k_bicg_v(DevGrid g, Op op, BiVec<V> b)
 {
     BiState* st = b.st;
     if(st->done) return;
     double vals[3] = {0.0, 0.0, 0.0};
-    const double* __restrict__ ph = b.p[1];
-    const double* __restrict__ rfx = b.rfx;
-    const double* __restrict__ rfy = b.rfy;
-    const double* __restrict__ rfz = b.rfz;
-    const double* __restrict__ r = b.r;
-    const double* __restrict__ r0 = b.r0;
-    double* __restrict__ vo = b.v[0];
+    const V* __restrict__ ph = b.ph;
+    const V* __restrict__ r = b.r;
+    const V* __restrict__ r0 = b.r0;
+    V* __restrict__ vo = b.v;
     const TileWalk tw = tile_walk(g);
     for(int t = tw.t; t < tw.t1; ++t)
     {
         int i, j, k;
         long c;
         if(!tile_cell(g, tw, t, i, j, k, c)) continue;
-        const Coupling q = couplings(g, rfx, rfy, rfz, c, i, j, k);
-        const double v = apply_A(g, q, ph, c, i, j, k);
+        const CouplingV<V> q = coupling_of(g, op, c, i, j, k);
+        const V v = apply_A(g, q, ph, c, i, j, k);
         vo[c] = v;
-        const double rc = r[c];
-        vals[0] += v * r0[c];
-        vals[1] += v * v;
+        const double vd = v, rc = r[c];
+        vals[0] += vd * double(r0[c]);
+        vals[1] += vd * vd;
         vals[2] += rc * rc;
     }
     block_partials<3>(vals, b.partials);
 }
 
-__global__ void k_fin_v(BiBuffers b, int nb)
+__global__ void k_fin_v(const double* __restrict__ partials, BiState* st, int nb)
 {
-    BiState* st = b.st;
     if(st->done) return;
     double tot[3];
-    sum_partials<3>(b.partials, nb, tot);
+    sum_partials<3>(partials, nb, tot);
     if(threadIdx.x == 0)
     {
         const double sigma = tot[0];
@@ -550,64 +585,63 @@ __global__ void k_fin_v(BiBuffers b, int nb)
 }
 
 // s = r - alpha v, sh = aii s, t = A sh; (t,s), (t,t), (s,s) (bicgstab_ijk.cpp:193-221)
-__global__ void __launch_bounds__(NT) k_bicg_t(DevGrid g, BiBuffers b)
+template<class V, class Op>
+__global__ void __launch_bounds__(NT) k_bicg_t(DevGrid g, Op op, BiVec<V> b)
 {
     BiState* st = b.st;
     if(st->done || st->mode == 1) return;
     double vals[3] = {0.0, 0.0, 0.0};
-    const double alpha = st->alpha;
-    const double* __restrict__ r = b.r;
-    const double* __restrict__ v = b.v[0];
-    const double* __restrict__ a = b.aii;
-    const double* __restrict__ rfx = b.rfx;
-    const double* __restrict__ rfy = b.rfy;
-    const double* __restrict__ rfz = b.rfz;
-    double* __restrict__ to = b.t;
+    const V alpha = (V)st->alpha;
+    const V* __restrict__ r = b.r;
+    const V* __restrict__ v = b.v;
+    const V* __restrict__ a = b.aii;
+    V* __restrict__ to = b.t;
     const TileWalk tw = tile_walk(g);
     for(int tt = tw.t; tt < tw.t1; ++tt)
     {
         int i, j, k;
         long c;
         if(!tile_cell(g, tw, tt, i, j, k, c)) continue;
-        const Coupling q = couplings(g, rfx, rfy, rfz, c, i, j, k);
+        const CouplingV<V> q = coupling_of(g, op, c, i, j, k);
         // sh at the centre and the neighbours (0 outside the domain)
         auto shv = [&](long cc) { return (r[cc] - alpha * v[cc]) * a[cc]; };
-        const double Mp = diag_of(g, q);
-        const double Mn = i == g.nx - 1 ? 0.0 : -q.T[0];
-        const double Ms = i == 0 ? 0.0 : -q.T[1];
-        const double Mt = k == g.nz - 1 ? 0.0 : -q.T[4];
-        const double Mb = k == 0 ? 0.0 : -q.T[5];
-        const double xn = i == g.nx - 1 ? 0.0 : shv(c + 1);
-        const double xs = i == 0 ? 0.0 : shv(c - 1);
-        const double xt = k == g.nz - 1 ? 0.0 : shv(c + g.sxy);
-        const double xb = k == 0 ? 0.0 : shv(c - g.sxy);
-        const double s = r[c] - alpha * v[c];
-        const double shc = s * a[c];
-        double t;
+        const V Mp = diag_of(g, q);
+        const V Mn = i == g.nx - 1 ? V(0) : -q.T[0];
+        const V Ms = i == 0 ? V(0) : -q.T[1];
+        const V Mt = k == g.nz - 1 ? V(0) : -q.T[4];
+        const V Mb = k == 0 ? V(0) : -q.T[5];
+        const V xn = i == g.nx - 1 ? V(0) : shv(c + 1);
+        const V xs = i == 0 ? V(0) : shv(c - 1);
+        const V xt = k == g.nz - 1 ? V(0) : shv(c + g.sxy);
+        const V xb = k == 0 ? V(0) : shv(c - g.sxy);
+        const V s = r[c] - alpha * v[c];
+        const V shc = s * a[c];
+        V t;
         if(g.jdir == 1)
         {
-            const double Mw = j == g.ny - 1 ? 0.0 : -q.T[2] * g.ydir;
-            const double Me = j == 0 ? 0.0 : -q.T[3] * g.ydir;
-            const double xw = j == g.ny - 1 ? 0.0 : shv(c + g.sx);
-            const double xe = j == 0 ? 0.0 : shv(c - g.sx);
+            const V yd = (V)g.ydir;
+            const V Mw = j == g.ny - 1 ? V(0) : -q.T[2] * yd;
+            const V Me = j == 0 ? V(0) : -q.T[3] * yd;
+            const V xw = j == g.ny - 1 ? V(0) : shv(c + g.sx);
+            const V xe = j == 0 ? V(0) : shv(c - g.sx);
             t = Mp * shc + Mn * xn + Ms * xs + Mw * xw + Me * xe + Mt * xt + Mb * xb;
         }
         else
             t = Mp * shc + Mn * xn + Ms * xs + Mt * xt + Mb * xb;
         to[c] = t;
-        vals[0] += t * s;
-        vals[1] += t * t;
-        vals[2] += s * s;
+        const double td = t, sd = s;
+        vals[0] += td * sd;
+        vals[1] += td * td;
+        vals[2] += sd * sd;
     }
     block_partials<3>(vals, b.partials);
 }
 
-__global__ void k_fin_t(BiBuffers b, int nb)
+__global__ void k_fin_t(const double* __restrict__ partials, BiState* st, int nb)
 {
-    BiState* st = b.st;
     if(st->done || st->mode == 1) return;
     double tot[3];
-    sum_partials<3>(b.partials, nb, tot);
+    sum_partials<3>(partials, nb, tot);
     if(threadIdx.x == 0)
     {
         st->w1 = tot[0];
@@ -619,27 +653,27 @@ __global__ void k_fin_t(BiBuffers b, int nb)
 }
 
 // x and r updates, r_j1 = (r, r0), beta, residual = |r| / N (bicgstab_ijk.cpp:223-268)
-template<bool BRANCH>
-__device__ __forceinline__ void x_tiles(const DevGrid& g, const BiBuffers& b, double alpha, double w, double (&vals)[2])
+template<class V, bool BRANCH>
+__device__ __forceinline__ void x_tiles(const DevGrid& g, const BiVec<V>& b, V alpha, V w, double (&vals)[2])
 {
-    const double* __restrict__ v = b.v[0];
-    const double* __restrict__ ph = b.p[1];
-    const double* __restrict__ a = b.aii;
-    const double* __restrict__ tt = b.t;
-    const double* __restrict__ r0 = b.r0;
-    double* __restrict__ r = b.r;
-    double* __restrict__ x = b.x;
+    const V* __restrict__ v = b.v;
+    const V* __restrict__ ph = b.ph;
+    const V* __restrict__ a = b.aii;
+    const V* __restrict__ tt = b.t;
+    const V* __restrict__ r0 = b.r0;
+    V* __restrict__ r = b.r;
+    V* __restrict__ x = b.x;
     const TileWalk tw = tile_walk(g);
     for(int t = tw.t; t < tw.t1; ++t)
     {
         int i, j, k;
         long c;
         if(!tile_cell(g, tw, t, i, j, k, c)) continue;
-        const double s = r[c] - alpha * v[c];
-        double rn;
+        const V s = r[c] - alpha * v[c];
+        V rn;
         if(BRANCH)
         {
-            const double sh = s * a[c];
+            const V sh = s * a[c];
             x[c] += alpha * ph[c] + w * sh;
             rn = s - w * tt[c];
         }
@@ -649,33 +683,34 @@ __device__ __forceinline__ void x_tiles(const DevGrid& g, const BiBuffers& b, do
             rn = s;
         }
         r[c] = rn;
-        vals[0] += rn * r0[c];
-        vals[1] += rn * rn;
+        const double rd = rn;
+        vals[0] += rd * double(r0[c]);
+        vals[1] += rd * rd;
     }
 }
 
-__global__ void __launch_bounds__(NT) k_bicg_x(DevGrid g, BiBuffers b)
+template<class V>
+__global__ void __launch_bounds__(NT) k_bicg_x(DevGrid g, BiVec<V> b)
 {
     BiState* st = b.st;
     if(st->done || st->mode == 1) return;
     double vals[2] = {0.0, 0.0};
-    const double alpha = st->alpha, w = st->w;
+    const V alpha = (V)st->alpha, w = (V)st->w;
     const int branch = st->branch;
     if(branch)
-        x_tiles<true>(g, b, alpha, w, vals);
+        x_tiles<V, true>(g, b, alpha, w, vals);
     else
-        x_tiles<false>(g, b, alpha, w, vals);
+        x_tiles<V, false>(g, b, alpha, w, vals);
     block_partials<2>(vals, b.partials);
 }
 
-__global__ void k_fin_x(BiBuffers b, int nb)
+__global__ void k_fin_x(const double* __restrict__ partials, BiState* st, int nb)
 {
-    BiState* st = b.st;
     if(st->done || st->mode == 1) return;
     const double alpha = st->alpha, w = st->w;
     const int branch = st->branch;
     double tot[2];
-    sum_partials<2>(b.partials, nb, tot);
+    sum_partials<2>(partials, nb, tot);
     if(threadIdx.x == 0)
     {
         const double r_j1 = tot[0];
@@ -711,37 +746,148 @@ static inline unsigned rblocks(const DevGrid& g)
     return (unsigned)((nb + 7) / 8 * 8);
 }
 
-void launch_bicg_init(const DevGrid& g, const BiBuffers& b, int cur, double stop, int maxiter, hipStream_t s)
+template<class V, class Op>
+static void bicg_init_t(const DevGrid& g, const Op& op, const BiVec<V>& b, double stop, int maxiter, hipStream_t s)
 {
-    (void)cur;
+    const unsigned nb = rblocks(g);
     hipLaunchKernelGGL(k_bicg_setup, dim3(1), dim3(1), 0, s, b.st, stop, maxiter, (long)g.nx * g.ny * g.nz);
-    hipLaunchKernelGGL(k_bicg_init, dim3(rblocks(g)), dim3(NT), 0, s, g, b);
-    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(NT), 0, s, b, (int)rblocks(g));
+    hipLaunchKernelGGL((k_bicg_init<V, Op>), dim3(nb), dim3(NT), 0, s, g, op, b);
+    hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(NT), 0, s, (const double*)b.partials, b.st, (int)nb);
 }
 
-void launch_bicg_phase(const DevGrid& g, const BiBuffers& b, int phase, hipStream_t s)
+// part 0 launches the phase's grid kernel, part 1 its one-block finalize kernel
+template<class V, class Op>
+static void bicg_phase_t(const DevGrid& g, const Op& op, const BiVec<V>& b, int phase, int part, hipStream_t s)
 {
     const dim3 grid(rblocks(g));
     const int nb = (int)grid.x;
+    const double* part_sums = b.partials;
+    if(part == 0)
+    {
+        switch(phase)
+        {
+            case 0: hipLaunchKernelGGL((k_bicg_p<V, Op>), grid, dim3(NT), 0, s, g, op, b); break;
+            case 1: hipLaunchKernelGGL((k_bicg_v<V, Op>), grid, dim3(NT), 0, s, g, op, b); break;
+            case 2: hipLaunchKernelGGL((k_bicg_t<V, Op>), grid, dim3(NT), 0, s, g, op, b); break;
+            default: hipLaunchKernelGGL((k_bicg_x<V>), grid, dim3(NT), 0, s, g, b); break;
+        }
+        return;
+    }
     switch(phase)
     {
-        case 0:
-            hipLaunchKernelGGL(k_bicg_p, grid, dim3(NT), 0, s, g, b);
-            hipLaunchKernelGGL(k_fin_p, dim3(1), dim3(NT), 0, s, b, nb);
-            break;
-        case 1:
-            hipLaunchKernelGGL(k_bicg_v, grid, dim3(NT), 0, s, g, b);
-            hipLaunchKernelGGL(k_fin_v, dim3(1), dim3(NT), 0, s, b, nb);
-            break;
-        case 2:
-            hipLaunchKernelGGL(k_bicg_t, grid, dim3(NT), 0, s, g, b);
-            hipLaunchKernelGGL(k_fin_t, dim3(1), dim3(NT), 0, s, b, nb);
-            break;
-        default:
-            hipLaunchKernelGGL(k_bicg_x, grid, dim3(NT), 0, s, g, b);
-            hipLaunchKernelGGL(k_fin_x, dim3(1), dim3(NT), 0, s, b, nb);
-            break;
+        case 0: hipLaunchKernelGGL(k_fin_p, dim3(1), dim3(NT), 0, s, part_sums, b.st, nb); break;
+        case 1: hipLaunchKernelGGL(k_fin_v, dim3(1), dim3(NT), 0, s, part_sums, b.st, nb); break;
+        case 2: hipLaunchKernelGGL(k_fin_t, dim3(1), dim3(NT), 0, s, part_sums, b.st, nb); break;
+        default: hipLaunchKernelGGL(k_fin_x, dim3(1), dim3(NT), 0, s, part_sums, b.st, nb); break;
     }
+}
+
+void launch_bicg_init(const DevGrid& g, const SolverOp64& op, const BiVec<double>& b, double stop, int maxiter,
+                      hipStream_t s)
+{
+    bicg_init_t(g, op, b, stop, maxiter, s);
+}
+void launch_bicg_init(const DevGrid& g, const SolverOp32& op, const BiVec<float>& b, double stop, int maxiter,
+                      hipStream_t s)
+{
+    bicg_init_t(g, op, b, stop, maxiter, s);
+}
+void launch_bicg_phase(const DevGrid& g, const SolverOp64& op, const BiVec<double>& b, int phase, int part,
+                       hipStream_t s)
+{
+    bicg_phase_t(g, op, b, phase, part, s);
+}
+void launch_bicg_phase(const DevGrid& g, const SolverOp32& op, const BiVec<float>& b, int phase, int part,
+                       hipStream_t s)
+{
+    bicg_phase_t(g, op, b, phase, part, s);
+}
+
+// ------------------------------------------------------------------ mixed precision
+// FP32 operator and preconditioner of the inner solve from the FP64 face densities:
+// kx = 1/(rho_f DXP) on every "+x" face of the interior and of the -1 layer (ky, kz
+// likewise), aii = -1/(M.p + 1e-19) from the FP64 diagonal, rounded once.
+__global__ void k_coef32(DevGrid g, int nbx, int nby, int nbtot, const double* __restrict__ rfx,
+                         const double* __restrict__ rfy, const double* __restrict__ rfz, float* __restrict__ kx,
+                         float* __restrict__ ky, float* __restrict__ kz, float* __restrict__ aii)
+{
+    int i, j, k;
+    long c;
+    if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
+    const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
+    kx[c] = (float)(1.0 / (rfx[c] * g.DXP[ip]));
+    ky[c] = (float)(1.0 / (rfy[c] * g.DYP[jp]));
+    kz[c] = (float)(1.0 / (rfz[c] * g.DZP[kp]));
+    if(i == 0) kx[c - 1] = (float)(1.0 / (rfx[c - 1] * g.DXP[ip - 1]));
+    if(j == 0) ky[c - g.sx] = (float)(1.0 / (rfy[c - g.sx] * g.DYP[jp - 1]));
+    if(k == 0) kz[c - g.sxy] = (float)(1.0 / (rfz[c - g.sxy] * g.DZP[kp - 1]));
+    const Coupling q = couplings(g, rfx, rfy, rfz, c, i, j, k);
+    aii[c] = (float)(-1.0 / (diag_of(g, q) + 1e-19));
+}
+
+void launch_coef32(const DevGrid& g, const double* rfx, const double* rfy, const double* rfz, float* kx, float* ky,
+                   float* kz, float* aii, hipStream_t s)
+{
+    int nbx, nby, nbtot;
+    dim3 grid = tiles(g, nbx, nby, nbtot);
+    hipLaunchKernelGGL(k_coef32, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, rfx, rfy, rfz, kx, ky, kz, aii);
+}
+
+// outer defect of the mixed-precision solve: r = b - A x in FP64 with the reference
+// operator, handed to the inner solve as its FP32 right-hand side; the inner solution
+// d starts at 0 (with r32 = null only ||r||_2 is formed).  The partials give ||r||_2.
+__global__ void __launch_bounds__(NT) k_defect(DevGrid g, SolverOp64 op, const double* __restrict__ x,
+                                                const double* __restrict__ rhs, float* __restrict__ r32,
+                                                float* __restrict__ d32, double* __restrict__ partials)
+{
+    double vals[1] = {0.0};
+    const TileWalk tw = tile_walk(g);
+    for(int t = tw.t; t < tw.t1; ++t)
+    {
+        int i, j, k;
+        long c;
+        if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+        const Coupling q = coupling_of(g, op, c, i, j, k);
+        const double rv = rhs[c] - apply_A(g, q, x, c, i, j, k);
+        if(r32)
+        {
+            r32[c] = (float)rv;
+            d32[c] = 0.0f;
+        }
+        vals[0] += rv * rv;
+    }
+    block_partials<1>(vals, partials);
+}
+
+__global__ void k_fin_norm(const double* __restrict__ partials, int nb, double* __restrict__ out)
+{
+    double tot[1];
+    sum_partials<1>(partials, nb, tot);
+    if(threadIdx.x == 0) out[0] = sqrt(tot[0]);
+}
+
+void launch_defect(const DevGrid& g, const SolverOp64& op, const double* x, const double* rhs, float* r32,
+                   float* d32, double* partials, double* norm_out, hipStream_t s)
+{
+    const unsigned nb = rblocks(g);
+    hipLaunchKernelGGL(k_defect, dim3(nb), dim3(NT), 0, s, g, op, x, rhs, r32, d32, partials);
+    hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(NT), 0, s, (const double*)partials, (int)nb, norm_out);
+}
+
+// x += d (FP64 accumulation of the FP32 correction)
+__global__ void k_add32(DevGrid g, int nbx, int nby, int nbtot, double* __restrict__ x, const float* __restrict__ d)
+{
+    int i, j, k;
+    long c;
+    if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
+    x[c] += (double)d[c];
+}
+
+void launch_add32(const DevGrid& g, double* x, const float* d, hipStream_t s)
+{
+    int nbx, nby, nbtot;
+    dim3 grid = tiles(g, nbx, nby, nbtot);
+    hipLaunchKernelGGL(k_add32, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, x, d);
 }
 
 // ------------------------------------------------------------------ level-set benchmark

@@ -113,15 +113,50 @@ void launch_face_rho(const DevGrid& g, const double* phi, double rw, double ra, 
 void launch_correct(const DevGrid& g, const double* pcorr, const double* phi, double rw, double ra, double adt,
                     double* u, double* v, double* w, double* press, hipStream_t s);
 
-struct BiBuffers
+// operator of the solve: FP64 recomputes the couplings of poisson_pcorr from the face
+// densities; FP32 (mixed-precision inner solve) reads FP32 face conductances 1/(rho_f DP)
+// and scales them by the FP32 1/DN tables (MARGE-offset)
+struct SolverOp64
 {
-    double *x, *rhs, *r, *r0, *p[2], *v[2], *t, *aii, *rfx, *rfy, *rfz;
+    const double *rfx, *rfy, *rfz;
+};
+struct SolverOp32
+{
+    const float *kx, *ky, *kz;
+    const float *iDXN, *iDYN, *iDZN;
+};
+
+// vectors of one BiCGStab solve (bicgstab_ijk.cpp:30-40); ph = aii p, v = A ph, t = A sh
+template<class V>
+struct BiVec
+{
+    V* x;
+    const V* rhs;
+    V *r, *r0, *p, *ph, *v, *t;
+    const V* aii;
     double* partials;
     BiState* st;
 };
-void launch_bicg_init(const DevGrid& g, const BiBuffers& b, int cur, double stop, int maxiter, hipStream_t s);
-// one phase of a BiCGStab iteration: 0 restart or p update, 1 v = A ph, 2 t = A sh, 3 x/r update
-void launch_bicg_phase(const DevGrid& g, const BiBuffers& b, int phase, hipStream_t s);
+
+void launch_bicg_init(const DevGrid& g, const SolverOp64& op, const BiVec<double>& b, double stop, int maxiter,
+                      hipStream_t s);
+void launch_bicg_init(const DevGrid& g, const SolverOp32& op, const BiVec<float>& b, double stop, int maxiter,
+                      hipStream_t s);
+// one phase of a BiCGStab iteration: 0 restart or p update, 1 v = A ph, 2 t = A sh, 3 x/r update;
+// part 0 is the grid kernel, part 1 the one-block finalize kernel that must follow it
+void launch_bicg_phase(const DevGrid& g, const SolverOp64& op, const BiVec<double>& b, int phase, int part,
+                       hipStream_t s);
+void launch_bicg_phase(const DevGrid& g, const SolverOp32& op, const BiVec<float>& b, int phase, int part,
+                       hipStream_t s);
+
+// mixed precision: FP32 operator and preconditioner from the FP64 face densities; the FP64
+// defect r = b - A x rounded into the inner right-hand side (d32 zeroed, ||r||_2 to
+// norm_out on the device); x += d
+void launch_coef32(const DevGrid& g, const double* rfx, const double* rfy, const double* rfz, float* kx, float* ky,
+                   float* kz, float* aii, hipStream_t s);
+void launch_defect(const DevGrid& g, const SolverOp64& op, const double* x, const double* rhs, float* r32,
+                   float* d32, double* partials, double* norm_out, hipStream_t s);
+void launch_add32(const DevGrid& g, double* x, const float* d, hipStream_t s);
 
 // prescribed vortex field (benchmark_vortex3D.cpp:67-101) and the void momentum_RK3 update
 void launch_vortex_velocity(const DevGrid& g, int comp, double* q, double simtime, double DXM, hipStream_t s);

@@ -176,6 +176,27 @@ def test_bicgstab_matches_direct_solve():
     assert np.allclose(got, xs, rtol=0, atol=1e-9 * np.abs(xs).max())
 
 
+def test_residual_is_rhs_minus_dense_product():
+    """orc_residual: r = b - A x per cell with the assembled operator (a sign or a
+    transposed neighbour fails it against the dense matrix)."""
+    O = dom(nx=5, ny=6, nz=4, dx=0.1)
+    x, y, z = coords(O)
+    phi = 0.2 - z + 0.03 * y
+    M, A = dense_operator(O, phi)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(A.shape[0])
+    xf = np.zeros(O.shape)
+    inner(xf)[...] = rng.standard_normal((O.nz, O.ny, O.nx))
+    xv = inner(xf).transpose(2, 1, 0).ravel()
+    r = np.zeros(O.shape)
+    norm = O.residual(xf, M, b.copy(), r)
+    want = b - A @ xv
+    got = inner(r).transpose(2, 1, 0).ravel()
+    assert np.allclose(got, want, rtol=0, atol=1e-12 * np.abs(A).max() * np.abs(xv).max())
+    assert abs(norm - np.linalg.norm(want)) <= 1e-12 * np.linalg.norm(want)
+    assert O.residual(np.zeros(O.shape), M, b.copy()) == pytest.approx(np.linalg.norm(b), rel=1e-14)
+
+
 def test_heaviside_limits_and_volume():
     O = dom(nx=8, ny=8, nz=8, dx=0.1)
     psi = O.scalars()["psi"]
