@@ -79,6 +79,11 @@ typedef struct reef_ctx reef_ctx;
 /* lexer + fdm + ghostcell set-up on the device (replaces driver.cpp:33-118 and
  * driver_makegrid.cpp:29-62 for a box domain); allocates all fields, zeroed. */
 int reef_ctx_create(const reef_grid_desc* grid, const reef_params* prm, reef_ctx** out);
+
+/* Control keys of lexer (ctrl.txt, lexer_read.cpp) that select a hot-path variant after
+ * creation.  "F49": reinitialisation discretisation, 1 = reinidisc_f (default, ini.cpp),
+ * 0 = reinidisc_fsf (reini_RK3.cpp:68-72).  Other keys: REEF_E_UNSUPPORTED. */
+int reef_set_control(reef_ctx* ctx, const char* key, double value);
 void reef_ctx_destroy(reef_ctx* ctx);
 const char* reef_last_error(void);
 

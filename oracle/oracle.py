@@ -49,6 +49,8 @@ def lib():
         L.orc_reini.argtypes = [ctypes.c_void_p, dp, ctypes.c_int]
         L.orc_fluid_update.argtypes = [ctypes.c_void_p] + [dp] * 4
         L.orc_poisson.argtypes = [ctypes.c_void_p] + [dp] * 4
+        L.orc_set_control.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double]
+        L.orc_set_control.restype = ctypes.c_int
         L.orc_residual.argtypes = [ctypes.c_void_p, dp, dp, dp, dp]
         L.orc_residual.restype = ctypes.c_double
         L.orc_bicgstab.argtypes = [ctypes.c_void_p, dp, dp, dp, ctypes.c_double, ctypes.c_int, dp]
@@ -73,7 +75,7 @@ class Domain:
 
     def __init__(self, nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21,
                  rho=(1000.0, 1.205), visc=(1.0e-6, 1.4938e-5), g=(0.0, 0.0, 0.0),
-                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000):
+                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000, F49=1):
         self.nx, self.ny, self.nz, self.jdir = nx, ny, nz, jdir
         self.shape = (nz + 6, ny + 6, nx + 6)
         groups = (ctypes.c_int * 7)(0, 21, 21, 21, 21, 21, top_group)
@@ -81,6 +83,8 @@ class Domain:
         self._xn = [np.ascontiguousarray(a, dtype=np.float64) for a in (XN, YN, ZN)]
         self.h = lib().orc_create(nx, ny, nz, jdir, _p(self._xn[0]), _p(self._xn[1]), _p(self._xn[2]),
                                   DYM, DZM, groups, ctypes.byref(self.prm))
+        if F49 != 1 and lib().orc_set_control(self.h, b"F49", float(F49)) != 0:
+            raise ValueError(f"F49={F49} not supported")
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace {
@@ -37,6 +38,7 @@ struct Dgc
 struct orc_domain
 {
     int nx, ny, nz, jdir;
+    int F49 = 1;                // reinit discretisation: 1 reinidisc_f, 0 reinidisc_fsf
     long sx, sy, sz;
     int group[7];
     orc_params prm;
@@ -563,14 +565,28 @@ void flux(orc_domain* d, int ipol, const double* q, const double* u, const doubl
 }
 
 // ------------------------------------------------------------------ reinitialisation
-// reinidisc_f::start/disc (reinidisc_f.cpp:37-127) with ddweno_nug_sf (modus 0)
+// reinidisc_f::start/disc (reinidisc_f.cpp:37-127) with ddweno_nug_sf (modus 0), F49 = 1;
+// reinidisc_fsf::start/disc (reinidisc_fsf.cpp:37-141) with ddweno_nug, F49 = 0: the same
+// per-cell update (ddweno_nug and ddweno_nug_sf in modus 0 read the same differences,
+// ddweno_nug.cpp:39-213, ddweno_nug_sf.cpp:41-277) restricted to cells whose 7-point
+// stencil has one sign; every other cell gets L = 0 (reinidisc_fsf.cpp:68-73)
 void reini_disc(orc_domain* d, const double* f, double* L)
 {
+    const long sxy = d->sx * d->sy;
     for(int i = 0; i < d->nx; ++i)
     for(int j = 0; j < d->ny; ++j)
     for(int k = 0; k < d->nz; ++k)
     {
         long c = d->id(i, j, k);
+        if(d->F49 == 0)
+        {
+            L[c] = 0.0;
+            const bool pos = f[c] >= 0.0 && f[c + 1] >= 0.0 && f[c - 1] >= 0.0 && f[c + d->sx] >= 0.0
+                          && f[c - d->sx] >= 0.0 && f[c + sxy] >= 0.0 && f[c - sxy] >= 0.0;
+            const bool neg = f[c] < 0.0 && f[c + 1] < 0.0 && f[c - 1] < 0.0 && f[c + d->sx] < 0.0
+                          && f[c - d->sx] < 0.0 && f[c + sxy] < 0.0 && f[c - sxy] < 0.0;
+            if(!(pos || neg)) continue;
+        }
         double dx = 0.0, dy = 0.0, dz = 0.0;
         double lsv = f[c];
         double lsSig = lsv / sqrt(lsv * lsv);
@@ -1274,6 +1290,16 @@ void orc_poisson(orc_domain* d, const double* phi, const double* pcorr, double* 
 int orc_bicgstab(orc_domain* d, double* x, const double* M7, const double* rhs, double stop, int maxiter, double* residual)
 {
     return bicgstab(d, x, M7, rhs, stop, maxiter, residual);
+}
+
+int orc_set_control(orc_domain* d, const char* key, double value)
+{
+    if(std::string(key) == "F49" && (value == 0.0 || value == 1.0))
+    {
+        d->F49 = (int)value;
+        return 0;
+    }
+    return -1;
 }
 
 double orc_residual(orc_domain* d, const double* x, const double* M7, const double* rhs, double* r)

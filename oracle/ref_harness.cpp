@@ -333,6 +333,7 @@ static Env setup(const Case& c)
 
 // ---------------------------------------------------------------- dump helpers
 static int G_nx, G_ny, G_nz;
+static int G_f49 = 1;       // reinit discretisation of the tank runs (F49)
 
 static vector<long> fshape() { return {G_nz + 6, G_ny + 6, G_nx + 6}; }
 
@@ -535,11 +536,29 @@ static void op_reini(Env& e, Rng& r)
     fill_phi_sphere(a->phi, r, p, 0.5 * G_nx * p->DXM, 0.45 * G_ny * p->DXM, 0.5 * G_nz * p->DXM, 0.28 * G_nz * p->DXM, 0.15);
     e.pgc->start4(p, a->phi, 52);
     dump_field("ri_phi_in", a->phi);
+    vector<double> keep;
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+        keep.push_back(a->phi(i, j, k));
     reini_RK3 ri(p, 1);
     p->count = 1;
     p->F44 = 3;
     ri.start(a, p, a->phi, e.pgc, e.pflow);
     dump_field("ri_phi_out", a->phi);
+    // the same input through reinidisc_fsf (F49 = 0, chosen in the reini_RK3 constructor)
+    size_t q = 0;
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+        a->phi(i, j, k) = keep[q++];
+    const int f49 = p->F49;
+    p->F49 = 0;
+    reini_RK3 rifsf(p, 1);
+    p->F49 = f49;
+    p->F44 = 3;
+    rifsf.start(a, p, a->phi, e.pgc, e.pflow);
+    dump_field("ri_fsf_phi_out", a->phi);
 }
 
 static void op_update(Env& e, Rng& r)
@@ -762,6 +781,7 @@ static void run_tank(Env& e, int nsteps, double depth, double amp, double wlen)
     pjm_corr press(p, a, pgc, ph, pc);
     kepsilon_void turb(p, a, pgc);
     bicgstab_ijk solv(p, a, pgc);
+    p->F49 = G_f49;
     reini_RK3 ri(p, 1);
     fsi_void fsi(p, pgc);
     momentum_FC3 mom(p, a, pgc, &conv, &fsfdisc, &dvoid, &press, &pois, &turb, &solv, &solv, e.pflow, ph, pc, &ri, &fsi);
@@ -838,11 +858,12 @@ int main(int argc, char** argv)
         c.N40 = 13;
         c.dt = 0.5 / (16.0 * 2.0);
     }
-    else if(cname == "tank" || cname == "tank2d")
+    else if(cname == "tank" || cname == "tank2d" || cname == "tankfsf")
     {
-        c.nx = cname == "tank" ? 16 : 24;
-        c.ny = cname == "tank" ? 6 : 1;
-        c.jdir = cname == "tank" ? 1 : 0;
+        c.nx = cname == "tank2d" ? 24 : 16;
+        c.ny = cname == "tank2d" ? 1 : 6;
+        c.jdir = cname == "tank2d" ? 0 : 1;
+        if(cname == "tankfsf") G_f49 = 0;
         c.nz = 12;
         c.dx = c.dy = c.dz = 0.05;
         c.topgroup = 3;
@@ -883,7 +904,7 @@ int main(int argc, char** argv)
     }
     else if(cname == "vortex")
         run_vortex(e, 4);
-    else if(cname == "tank")
+    else if(cname == "tank" || cname == "tankfsf")
         run_tank(e, 3, 0.35, 0.02, 0.8);
     else if(cname == "tank2d")
         run_tank(e, 3, 0.35, 0.02, 0.6);

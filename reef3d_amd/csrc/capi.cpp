@@ -53,6 +53,7 @@ struct reef_ctx
     SolverOp64 op64{};
     double *rfx = nullptr, *rfy = nullptr, *rfz = nullptr, *aii64 = nullptr;
     // mixed-precision projection: FP32 inner vectors and operator, allocated on first use
+    int F49 = 1;                         // reinit discretisation (ini.cpp default 1)
     int precision = REEF_SOLVER_FP64;
     double inner_tol = 1.0e-5;
     int max_outer = 40;
@@ -60,6 +61,8 @@ struct reef_ctx
     SolverOp32 op32{};
     float *kx32 = nullptr, *ky32 = nullptr, *kz32 = nullptr, *aii32 = nullptr, *r32 = nullptr;
     double* norm_dev = nullptr;
+    BiState* st_host = nullptr;          // pinned, two status slots of the batched iteration
+    hipEvent_t st_ev[2] = {nullptr, nullptr};
     long last_inner = 0, last_outer = 0;
     double last_rel = 0.0;
     double* vol_dev = nullptr;
@@ -187,15 +190,15 @@ int do_reini(reef_ctx* c, double* f, int iters)
     for(int q = 0; q < iters; ++q)
     {
         c->tbegin("reini");
-        launch_reini_stage(g, 1, f, f, c->rk1, c->prm.F43, c->stream);
+        launch_reini_stage(g, 1, f, f, c->rk1, c->prm.F43, c->F49 == 0, c->stream);
         c->tend();
         if((rc = c->ghost(c->rk1, 4, 52))) return rc;
         c->tbegin("reini");
-        launch_reini_stage(g, 2, c->rk1, f, c->rk2, c->prm.F43, c->stream);
+        launch_reini_stage(g, 2, c->rk1, f, c->rk2, c->prm.F43, c->F49 == 0, c->stream);
         c->tend();
         if((rc = c->ghost(c->rk2, 4, 52))) return rc;
         c->tbegin("reini");
-        launch_reini_stage(g, 3, c->rk2, f, f, c->prm.F43, c->stream);
+        launch_reini_stage(g, 3, c->rk2, f, f, c->prm.F43, c->F49 == 0, c->stream);
         c->tend();
         if((rc = c->ghost(f, 4, 52))) return rc;
     }
@@ -223,15 +226,16 @@ int do_fluid_update(reef_ctx* c, double* vol2)
 const char* const BICG_NAMES[4] = {"bicg_p", "bicg_v", "bicg_t", "bicg_x"};
 const char* const BICG32_NAMES[4] = {"bicg32_p", "bicg32_v", "bicg32_t", "bicg32_x"};
 
-// BiCGStab iterations after launch_bicg_init: enqueued in batches, the device state says
-// when the solve ended
+// BiCGStab iterations after launch_bicg_init, enqueued in batches; the device state says
+// when the solve ended.  Two batches are in flight: batch k+1 is enqueued before the host
+// waits for the status copied out after batch k, so the device does not idle while the
+// host polls.  Iterations launched after the end are no-ops (every kernel tests done).
 template<class V, class Op>
 int run_bicg(reef_ctx* c, const Op& op, const BiVec<V>& b, int maxiter, const char* const* names, BiState& hs)
 {
     const DevGrid& g = c->dg;
     const int batch = 8;
-    for(int launched = 0;;)
-    {
+    auto enqueue = [&](long k) -> int {
         for(int q = 0; q < batch; ++q)
             for(int ph = 0; ph < 4; ++ph)
             {
@@ -240,15 +244,24 @@ int run_bicg(reef_ctx* c, const Op& op, const BiVec<V>& b, int maxiter, const ch
                 c->tend();
                 launch_bicg_phase(g, op, b, ph, 1, c->stream);
             }
-        launched += batch;
-        HIPCHK(hipMemcpyAsync(&hs, b.st, sizeof(BiState), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        if(c->timing) c->tcollect();
+        HIPCHK(hipMemcpyAsync(&c->st_host[k & 1], b.st, sizeof(BiState), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipEventRecord(c->st_ev[k & 1], c->stream));
+        return 0;
+    };
+    int rc;
+    if((rc = enqueue(0))) return rc;
+    for(long k = 1;; ++k)
+    {
+        if((rc = enqueue(k))) return rc;
+        HIPCHK(hipEventSynchronize(c->st_ev[(k - 1) & 1]));
+        hs = c->st_host[(k - 1) & 1];
         if(hs.done) break;
+        if(c->timing && c->ranges.size() > 2048) c->tcollect();
         // every launched iteration advances iter or restarts; restarts also count, so
         // the solve ends within maxiter + (restarts) batches
-        if(launched > 4 * (maxiter + 16)) return fail(REEF_E_HIP, "BiCGStab did not terminate");
+        if(k * batch > 4L * (maxiter + 16)) return fail(REEF_E_HIP, "BiCGStab did not terminate");
     }
+    if(c->timing) c->tcollect();
     return 0;
 }
 
@@ -580,6 +593,9 @@ int reef_ctx_create(const reef_grid_desc* grid, const reef_params* prm, reef_ctx
     if((rc = c->dalloc(&stmem, (sizeof(BiState) + 7) / 8 + 8))) return rc;
     b.st = (BiState*)stmem;
     if((rc = c->dalloc(&c->vol_dev, 4))) return rc;
+    HIPCHK(hipHostMalloc((void**)&c->st_host, 2 * sizeof(BiState), hipHostMallocDefault));
+    std::memset(c->st_host, 0, 2 * sizeof(BiState));
+    for(hipEvent_t& e : c->st_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     double* tk = nullptr;
     if((rc = c->dalloc(&tk, 4))) return rc;
     c->vol_ticket = (unsigned int*)tk;
@@ -598,6 +614,9 @@ void reef_ctx_destroy(reef_ctx* c)
         hipEventDestroy(r.b);
     }
     for(hipEvent_t e : c->pool) hipEventDestroy(e);
+    for(hipEvent_t e : c->st_ev)
+        if(e) hipEventDestroy(e);
+    if(c->st_host) hipHostFree(c->st_host);
     for(void* p : c->allocs) hipFree(p);
     if(c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -809,6 +828,18 @@ int reef_poisson_residual(reef_ctx* c, int x_id, int rhs_id, double* norm)
     HIPCHK(hipMemcpyAsync(norm, c->norm_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+int reef_set_control(reef_ctx* c, const char* key, double value)
+{
+    if(check_ctx(c) || !key) return fail(REEF_E_ARG, "null argument");
+    if(std::strcmp(key, "F49") == 0)
+    {
+        if(value != 0.0 && value != 1.0) return fail(REEF_E_UNSUPPORTED, "F49 must be 0 or 1");
+        c->F49 = (int)value;
+        return 0;
+    }
+    return fail(REEF_E_UNSUPPORTED, std::string("control key not supported: ") + key);
 }
 
 int reef_solver_config(reef_ctx* c, int precision, double inner_tol, int max_outer)

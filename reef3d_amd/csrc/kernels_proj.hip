@@ -459,7 +459,7 @@ __device__ __forceinline__ void restart_tiles(const DevGrid& g, const Op& op, co
 }
 
 template<class V, class Op>
-__global__ void __launch_bounds__(NT) k_bicg_init(DevGrid g, Op op, BiVec<V> b)
+__global__ void __launch_bounds__(NT, 8) k_bicg_init(DevGrid g, Op op, BiVec<V> b)
 {
     double vals[1] = {0.0};
     restart_tiles(g, op, b, vals[0]);
@@ -485,7 +485,7 @@ __global__ void k_fin_init(const double* __restrict__ partials, BiState* st, int
 // start of an iteration: either the pending restart, or p = r + beta (p - w v)
 // (bicgstab_ijk.cpp:234-235); then ph = aii p (precon_solve)
 template<class V, class Op>
-__global__ void __launch_bounds__(NT) k_bicg_p(DevGrid g, Op op, BiVec<V> b)
+__global__ void __launch_bounds__(NT, 8) k_bicg_p(DevGrid g, Op op, BiVec<V> b)
 {
     BiState* st = b.st;
     if(st->done) return;
@@ -534,7 +534,7 @@ __global__ void k_fin_p(const double* __restrict__ partials, BiState* st, int nb
 
 // v = A ph; sigma = (v, r0), |v|, |r|; alpha and the restart tests (bicgstab_ijk.cpp:151-191)
 template<class V, class Op>
-__global__ void __launch_bounds__(NT) k_bicg_v(DevGrid g, Op op, BiVec<V> b)
+__global__ void __launch_bounds__(NT, 8) k_bicg_v(DevGrid g, Op op, BiVec<V> b)
 {
     BiState* st = b.st;
     if(st->done) return;
@@ -586,7 +586,7 @@ __global__ void k_fin_v(const double* __restrict__ partials, BiState* st, int nb
 
 // s = r - alpha v, sh = aii s, t = A sh; (t,s), (t,t), (s,s) (bicgstab_ijk.cpp:193-221)
 template<class V, class Op>
-__global__ void __launch_bounds__(NT) k_bicg_t(DevGrid g, Op op, BiVec<V> b)
+__global__ void __launch_bounds__(NT, 8) k_bicg_t(DevGrid g, Op op, BiVec<V> b)
 {
     BiState* st = b.st;
     if(st->done || st->mode == 1) return;
@@ -690,7 +690,7 @@ __device__ __forceinline__ void x_tiles(const DevGrid& g, const BiVec<V>& b, V a
 }
 
 template<class V>
-__global__ void __launch_bounds__(NT) k_bicg_x(DevGrid g, BiVec<V> b)
+__global__ void __launch_bounds__(NT, 8) k_bicg_x(DevGrid g, BiVec<V> b)
 {
     BiState* st = b.st;
     if(st->done || st->mode == 1) return;
@@ -836,7 +836,7 @@ void launch_coef32(const DevGrid& g, const double* rfx, const double* rfy, const
 // outer defect of the mixed-precision solve: r = b - A x in FP64 with the reference
 // operator, handed to the inner solve as its FP32 right-hand side; the inner solution
 // d starts at 0 (with r32 = null only ||r||_2 is formed).  The partials give ||r||_2.
-__global__ void __launch_bounds__(NT) k_defect(DevGrid g, SolverOp64 op, const double* __restrict__ x,
+__global__ void __launch_bounds__(NT, 8) k_defect(DevGrid g, SolverOp64 op, const double* __restrict__ x,
                                                 const double* __restrict__ rhs, float* __restrict__ r32,
                                                 float* __restrict__ d32, double* __restrict__ partials)
 {

@@ -126,9 +126,12 @@ void launch_hj_stage(const DevGrid& g, int stage, const double* phi_s, const dou
 // ------------------------------------------------------------------ reinitialisation
 // reinidisc_f::disc (reinidisc_f.cpp:57-127) fused with the reini_RK3 sub-stage
 // (reini_RK3.cpp:114-139) and the per-cell pseudo time step (reini_RK3.cpp:150-160).
+// fsf = 1 is reinidisc_fsf::disc (reinidisc_fsf.cpp:66-141, F49 = 0): the same update
+// on cells whose 7-point stencil has one sign, L = 0 elsewhere (ddweno_nug and
+// ddweno_nug_sf in modus 0 compute the same gradients).
 __global__ void __launch_bounds__(256) k_reini_stage(DevGrid g, int nbx, int nby, int nbtot, int stage,
                                                      const double* __restrict__ f, const double* __restrict__ f0,
-                                                     double* __restrict__ out, double F43)
+                                                     double* __restrict__ out, double F43, int fsf)
 {
     int bx, by, bz;
     if(!tile_of(nbx, nby, nbtot, bx, by, bz)) return;
@@ -138,8 +141,19 @@ __global__ void __launch_bounds__(256) k_reini_stage(DevGrid g, int nbx, int nby
     if(i >= g.nx || j >= g.ny) return;
     const long c = g.base + i + g.sx * j + g.sxy * k;
     const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
-    double dx = 0.0, dy = 0.0, dz = 0.0;
     const double lsv = f[c];
+    bool active = true;
+    if(fsf)
+    {
+        const double n1 = f[c + 1], n2 = f[c - 1], n3 = f[c + g.sx], n4 = f[c - g.sx], n5 = f[c + g.sxy],
+                     n6 = f[c - g.sxy];
+        active = (lsv >= 0.0 && n1 >= 0.0 && n2 >= 0.0 && n3 >= 0.0 && n4 >= 0.0 && n5 >= 0.0 && n6 >= 0.0)
+              || (lsv < 0.0 && n1 < 0.0 && n2 < 0.0 && n3 < 0.0 && n4 < 0.0 && n5 < 0.0 && n6 < 0.0);
+    }
+    double L = 0.0;
+    if(active)
+    {
+    double dx = 0.0, dy = 0.0, dz = 0.0;
     double lsSig = lsv / sqrt(lsv * lsv);
     if(fabs(lsv) < 1.0e-8) lsSig = 1.0;
     const double xmin = (lsv - f[c - 1]) / g.DXP[ip - 1];
@@ -168,7 +182,8 @@ __global__ void __launch_bounds__(256) k_reini_stage(DevGrid g, int nbx, int nby
         deltax = (1.0 / 3.0) * (g.DXN[ip] + g.DYN[jp] + g.DZN[kp]);
     double sign = lsv / sqrt(lsv * lsv + dnorm * dnorm * deltax * deltax);
     if(sign != sign) sign = 1.0;
-    const double L = -(sign * dnorm - sign);
+    L = -(sign * dnorm - sign);
+    }
     // pseudo time step F43 * min(DXP, DYP, DZP)
     const double a = g.DXP[ip], b = g.DYP[jp], cc = g.DZP[kp];
     const double m = g.jdir == 0 ? (a < cc ? a : cc) : (((a < b ? a : b) < cc) ? (a < b ? a : b) : cc);
@@ -182,11 +197,11 @@ __global__ void __launch_bounds__(256) k_reini_stage(DevGrid g, int nbx, int nby
 }
 
 void launch_reini_stage(const DevGrid& g, int stage, const double* fs, const double* f0, double* out, double F43,
-                        hipStream_t s)
+                        int fsf, hipStream_t s)
 {
     int nbx, nby, nbtot;
     dim3 grid = tile_grid(g, g.nx, g.ny, g.nz, nbx, nby, nbtot);
-    hipLaunchKernelGGL(k_reini_stage, grid, dim3(256), 0, s, g, nbx, nby, nbtot, stage, fs, f0, out, F43);
+    hipLaunchKernelGGL(k_reini_stage, grid, dim3(256), 0, s, g, nbx, nby, nbtot, stage, fs, f0, out, F43, fsf);
 }
 
 // ------------------------------------------------------------------ WENO flux

@@ -84,9 +84,10 @@ void launch_ghost(const GhostOp* ops, int nops, double* f, hipStream_t s);
 void launch_hj_stage(const DevGrid& g, int stage, const double* phi_s, const double* phi_n, const double* u,
                      const double* v, const double* w, double* out, double cdt, hipStream_t s);
 
-// reinit sub-stage q (1..3): out = combo(f0, fs, dtf * L_reini(fs))
+// reinit sub-stage q (1..3): out = combo(f0, fs, dtf * L_reini(fs)); fsf = 1 for
+// reinidisc_fsf (F49 = 0), 0 for reinidisc_f (F49 = 1)
 void launch_reini_stage(const DevGrid& g, int stage, const double* fs, const double* f0, double* out, double F43,
-                        hipStream_t s);
+                        int fsf, hipStream_t s);
 
 // WENO flux: acc += flux divergence of q (component ipol) with face velocities of (u,v,w)
 void launch_flux_acc(const DevGrid& g, int ipol, const double* q, const double* u, const double* v, const double* w,

@@ -55,6 +55,7 @@ def library():
     sig = {
         "reef_ctx_create": ([ctypes.POINTER(GridDesc), ctypes.POINTER(Params), ctypes.POINTER(vp)], ctypes.c_int),
         "reef_ctx_destroy": ([vp], None),
+        "reef_set_control": ([vp, ctypes.c_char_p, ctypes.c_double], ctypes.c_int),
         "reef_last_error": ([], ctypes.c_char_p),
         "reef_field_put": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
         "reef_field_get": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
@@ -136,7 +137,7 @@ class Domain:
 
     def __init__(self, nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21,
                  rho=(1000.0, 1.205), visc=(1.0e-6, 1.4938e-5), g=(0.0, 0.0, 0.0),
-                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000):
+                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000, F49=1):
         L = library()
         self.nx, self.ny, self.nz, self.jdir = nx, ny, nz, jdir
         self.shape = (nz + 6, ny + 6, nx + 6)
@@ -150,6 +151,12 @@ class Domain:
         h = ctypes.c_void_p()
         _check(L.reef_ctx_create(ctypes.byref(gd), ctypes.byref(self.params), ctypes.byref(h)))
         self.h = h
+        if F49 != 1:
+            self.set_control("F49", F49)
+
+    def set_control(self, key, value):
+        """lexer control key of a hot-path variant (reef_set_control), e.g. F49 = 0."""
+        _check(library().reef_set_control(self.h, key.encode(), float(value)))
 
     def close(self):
         if getattr(self, "h", None):

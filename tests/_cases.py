@@ -12,6 +12,8 @@ CASES = {
     "vortex": dict(nx=16, ny=16, nz=16, jdir=1, top_group=21, g=(0.0, 0.0, 0.0), dt=0.5 / 32.0, N44=1e-10),
     "tank": dict(nx=16, ny=6, nz=12, jdir=1, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14),
     "tank2d": dict(nx=24, ny=1, nz=12, jdir=0, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14),
+    # the tank with reinidisc_fsf (F49 = 0)
+    "tankfsf": dict(nx=16, ny=6, nz=12, jdir=1, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14, F49=0),
 }
 
 
@@ -23,7 +25,8 @@ def domain_args(case, d):
     c = CASES[case]
     gs = d["grid_scalars"]
     return dict(nx=c["nx"], ny=c["ny"], nz=c["nz"], XN=d["XN"], YN=d["YN"], ZN=d["ZN"], DYM=gs[1], DZM=gs[2],
-                jdir=c["jdir"], top_group=c["top_group"], g=c["g"], dt=c["dt"], N44=c["N44"], N46=5000)
+                jdir=c["jdir"], top_group=c["top_group"], g=c["g"], dt=c["dt"], N44=c["N44"], N46=5000,
+                F49=c.get("F49", 1))
 
 
 def inner(a):

@@ -80,6 +80,18 @@ def test_reini_bitexact(case):
     assert np.array_equal(got[mask], ref[mask])
 
 
+@pytest.mark.parametrize("case", ["box", "nonuni"])
+def test_reini_fsf_bitexact(case):
+    """reinidisc_fsf (F49 = 0) against the reference's own output."""
+    d = load(case)
+    D = dev(case, d)
+    D.set_control("F49", 0)
+    D.put("phi", d["ri_phi_in"])
+    D.reini("phi", 3)
+    got = D.get("phi")
+    assert bitexact(inner(got), inner(d["ri_fsf_phi_out"]))
+
+
 def test_fluid_update_matches_reference():
     d = load("box")
     D = dev("box", d)
@@ -120,7 +132,7 @@ def test_bicgstab_first_iterations_match_reference_path():
         np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10 * np.abs(ref).max(), err_msg=f"it={it}")
 
 
-@pytest.mark.parametrize("case", ["tank", "tank2d"])
+@pytest.mark.parametrize("case", ["tank", "tank2d", "tankfsf"])
 def test_fc3_step_matches_reference(case):
     d = load(case)
     D = dev(case, d)

@@ -20,7 +20,7 @@ def same(a, b):
     return np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("case", ["box", "nonuni", "vortex", "tank", "tank2d"])
+@pytest.mark.parametrize("case", ["box", "nonuni", "vortex", "tank", "tank2d", "tankfsf"])
 def test_grid_metadata_and_tables(case):
     """grid::gridspacing and the non-uniform WENO tables (weno_nug_func_precalc_*)."""
     d = load(case)
@@ -72,6 +72,18 @@ def test_reinitialisation(case):
     assert same(phi, d["ri_phi_out"])
 
 
+@pytest.mark.parametrize("case", ["box", "nonuni"])
+def test_reinitialisation_fsf(case):
+    """reinidisc_fsf (F49 = 0): interface-touching cells frozen, the rest as reinidisc_f."""
+    from oracle.oracle import Domain
+    d = load(case)
+    O = Domain(**dict(domain_args(case, d), F49=0))
+    phi = d["ri_phi_in"].copy()
+    O.reini(phi, 3)
+    assert same(phi, d["ri_fsf_phi_out"])
+    assert not same(phi, d["ri_phi_out"])
+
+
 def test_fluid_update():
     d = load("box")
     O = oracle("box", d)
@@ -118,7 +130,7 @@ def test_bicgstab_iteration_path(it):
     assert same(inner(x), inner(d[f"bs_x_it{it}"]))
 
 
-@pytest.mark.parametrize("case", ["tank", "tank2d"])
+@pytest.mark.parametrize("case", ["tank", "tank2d", "tankfsf"])
 def test_momentum_fc3_steps(case):
     d = load(case)
     O = oracle(case, d)
