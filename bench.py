@@ -18,6 +18,11 @@ mixed_projection object repeats the timed steps from the same initial state with
 mixed-precision projection (FP32 BiCGStab inside FP64 defect correction to the same
 1e-10 FP64 residual) and compares.
 
+--workload vortex: the level-set vortex deformation benchmark instead: unit cube, 128^3
+cells, sphere r = 0.15 at (0.35, 0.35, 0.35), levelset_RK3 + 3 reinit iterations per
+step, dt = 1/512 to t = T = 3 (1536 steps); reports Mcell-updates/s, the enclosed volume
+against 4/3 pi r^3, and the phi difference to the CPU oracle over the first --check steps.
+
 --workload solves: the standalone-solve comparison on the same operator instead: --solves
 right-hand sides ~ N(0, 1) per cell (seeds 2000, 2001, ..., mean removed), each solved
 in FP64 and in mixed precision from x = 0 to ||b - Ax|| <= 1e-10 ||b||; reports
@@ -129,6 +134,103 @@ def timed_steps(D, steps):
     return elapsed, iters, fams
 
 
+# algorithmic bytes per interior cell, averaged over the three stages of one launch family
+STENCIL_BYTES = {
+    "hj": (40 + 48 + 48) / 3.0,      # phi_s, u, v, w [, phi_n] in; phi out
+    "reini": (16 + 24 + 24) / 3.0,   # phi_s [, phi_0] in; phi out
+}
+
+
+def vortex(args):
+    """Level-set vortex deformation benchmark (levelset_RK3 + reini_RK3 on 128^3)."""
+    from oracle.oracle import Domain as ODomain
+    from reef3d_amd import Domain
+    from reef3d_amd.cases import C0, vortex3d
+    n, dt = C0["n"], C0["dt"]
+    steps = args.steps if args.steps_given else int(round(C0["T"] / dt))
+    cells = n ** 3
+    f = vortex3d(n)
+    kw = dict(nx=n, ny=n, nz=n, XN=f["XN"], YN=f["YN"], ZN=f["ZN"], DYM=1.0 / n, DZM=1.0 / n, jdir=1,
+              top_group=21, dt=dt)
+
+    def fresh():
+        D = Domain(**kw)
+        D.put("phi", f["phi"])
+        D.start4("phi", 52)
+        for name in ("u", "v", "w", "ro", "visc"):
+            D.zero(name)
+        return D
+
+    Dw = fresh()
+    for s in range(args.warmup):
+        Dw.vortex_step(s * dt, volumes=False)
+    Dw.sync()
+    del Dw
+    D = fresh()
+    vol0 = D.fluid_update()
+    D.sync()
+    D.timing(True)
+    D.timing_reset()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        D.vortex_step(s * dt, volumes=False)
+    D.sync()
+    elapsed = time.perf_counter() - t0
+    fams = {name: D.timing_get(name) for name in TIMED_FAMILIES}
+    D.timing(False)
+    vol_end = D.fluid_update()
+    exact = 4.0 / 3.0 * np.pi * 0.15 ** 3
+    dom = max(STENCIL_BYTES, key=lambda k: fams[k][0])
+    ms, nl = fams[dom]
+    avg_s = ms / 1e3 / max(nl, 1)
+    achieved = STENCIL_BYTES[dom] * cells / avg_s / 1e9
+    out = {
+        "metric": "Mcell-updates/s",
+        "value": round(cells * steps / elapsed / 1e6, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / steps * 1e3, 4),
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"vortex3d_{n}: levelset_RK3 (HJ-WENO5) + 3 reinit iterations per step, "
+                               f"dt = 1/512, t = {steps * dt:g}"},
+        "volume": {"initial": float(vol0[0]), "final": float(vol_end[0]), "exact_sphere": exact,
+                   "rel_change": float(vol_end[0] / vol0[0] - 1.0),
+                   "rel_err_vs_exact": float(vol_end[0] / exact - 1.0)},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_cell": round(STENCIL_BYTES[dom], 2)},
+        "kernel_ms_per_step": {k: round(v[0] / steps, 4) for k, v in fams.items() if v[1]},
+    }
+    if args.check > 0:
+        Dp = fresh()
+        O = ODomain(**kw)
+        phi = f["phi"].copy()
+        O.ghost(phi, 4, 52)
+        u, v, w, ro, visc = (np.zeros_like(phi) for _ in range(5))
+        worst = 0.0
+        t0 = time.perf_counter()
+        for s in range(args.check):
+            O.vortex_step(phi, u, v, w, ro, visc, s * dt)
+        t_cpu = (time.perf_counter() - t0) / args.check
+        for s in range(args.check):
+            Dp.vortex_step(s * dt, volumes=False)
+        got = Dp.get("phi")[3:-3, 3:-3, 3:-3]
+        ref = phi[3:-3, 3:-3, 3:-3]
+        worst = float(np.abs(got - ref).max() / np.abs(ref).max())
+        out["parity"] = {"steps": args.check, "phi_max_rel_err_vs_oracle": worst}
+        if not args.no_cpu:
+            out["cpu_baseline"] = {"value": round(cells / t_cpu / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1,
+                                   "kind": "port",
+                                   "sample": f"CPU oracle, 1 thread, the first {args.check} steps of the same "
+                                             f"128^3 run ({t_cpu:.2f} s per step)"}
+    print(json.dumps(out), flush=True)
+
+
 def solves(args, cfg):
     """Standalone FP64 vs mixed-precision solves on the wave tank's step-0 operator."""
     D, _ = make_domain(cfg)
@@ -193,13 +295,17 @@ def solves(args, cfg):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison leg")
-    ap.add_argument("--workload", choices=("fc3", "solves"), default="fc3")
+    ap.add_argument("--workload", choices=("fc3", "solves", "vortex"), default="fc3")
+    ap.add_argument("--check", type=int, default=3, help="oracle parity steps of --workload vortex")
     ap.add_argument("--solves", type=int, default=100, help="right-hand sides of --workload solves")
     args = ap.parse_args()
+    args.steps_given = args.steps is not None
+    if args.steps is None:
+        args.steps = 3
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the hot path is single-device); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -209,6 +315,9 @@ def main():
     cfg = dict(C1)
     if args.workload == "solves":
         solves(args, cfg)
+        return
+    if args.workload == "vortex":
+        vortex(args)
         return
     D, _ = make_domain(cfg)
     cells = cfg["nx"] * cfg["ny"] * cfg["nz"]

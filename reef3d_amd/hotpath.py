@@ -260,7 +260,12 @@ class Domain:
         _check(library().reef_fc3_step(self.h, ctypes.byref(it), _dp(vol)))
         return it.value, vol
 
-    def vortex_step(self, simtime):
+    def vortex_step(self, simtime, volumes=True):
+        """levelset_RK3 + reinit + benchmark_vortex3D step; volumes=False skips the
+        volume read-back (and the stream synchronisation it needs)."""
+        if not volumes:
+            _check(library().reef_vortex_step(self.h, simtime, None))
+            return None
         vol = np.zeros(2)
         _check(library().reef_vortex_step(self.h, simtime, _dp(vol)))
         return vol
