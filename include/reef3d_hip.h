@@ -116,28 +116,29 @@ int reef_poisson_solve(reef_ctx* ctx, int x_id, int rhs_id, double stop, int max
 
 /* The same solve in mixed precision; no counterpart in the reference, which solves in
  * FP64 only (it replaces bicgstab_ijk::start, bicgstab_ijk.cpp:51-123, where the caller
- * accepts a result at the same FP64 residual).  The FP64 defect r = b - A x with the
- * reference operator is corrected by FP32 BiCGStab solves (FP32 face conductances and
- * Jacobi preconditioner, FP64 dot products) until ||r||_2 / N < stop (stop >= 0, the
- * reference criterion) or ||r||_2 < |stop| * ||b - A x0||_2 (stop < 0).  Each inner
- * solve reduces its residual by inner_tol (reef_solver_config, default 1e-5) or by what is
- * left to reach the criterion; the loop also ends when a correction no longer halves the
- * FP64 defect.  inner_iters: summed FP32 iterations (restarts included, as solveriter
- * counts them); outer_iters: corrections applied; rel_residual: final
- * ||b - A x||_2 / ||b - A x0||_2 in FP64.  x_id and rhs_id must differ. */
+ * accepts a result at the same FP64 residual).  bicgstab_ijk's iteration runs on FP32
+ * vectors with the FP32 operator (face conductances 1/(rho_f DXP) and the Jacobi
+ * preconditioner rounded once; dot products and scalars in FP64); the solution is
+ * accumulated in FP64, and reliable updates replace the iterated residual by the true
+ * FP64 residual b - A x of the reference operator whenever the iterated one has dropped by
+ * the factor delta (reef_solver_config, default 0.1) and before any exit.  The solve ends
+ * when the true residual meets ||r||_2 / N < stop (stop >= 0, the reference criterion) or
+ * ||r||_2 < |stop| * ||b - A x0||_2 (stop < 0), or at maxiter.  iters: iterations
+ * (restarts counted as solveriter counts them); updates: reliable updates;
+ * rel_residual: final ||b - A x||_2 / ||b - A x0||_2 in FP64.  x_id, rhs_id must differ. */
 int reef_poisson_solve_mixed(reef_ctx* ctx, int x_id, int rhs_id, double stop, int maxiter,
-                             int* inner_iters, int* outer_iters, double* rel_residual);
+                             int* iters, int* updates, double* rel_residual);
 
 /* ||rhs - A x||_2 in FP64 with the operator assembled from REEF_PHI: res_calc
  * (bicgstab_ijk.cpp:314-338) without its 1/N. */
 int reef_poisson_residual(reef_ctx* ctx, int x_id, int rhs_id, double* norm);
 
 /* Solver of the projection inside reef_pjm_corr / reef_fc3_step: precision
- * REEF_SOLVER_FP64 (default) or REEF_SOLVER_MIXED, the inner relative tolerance and the
- * maximum number of corrections of the mixed form. */
-int reef_solver_config(reef_ctx* ctx, int precision, double inner_tol, int max_outer);
+ * REEF_SOLVER_FP64 (default) or REEF_SOLVER_MIXED, and the reliable-update threshold
+ * delta of the mixed form, 0 < delta < 1. */
+int reef_solver_config(reef_ctx* ctx, int precision, double delta);
 /* counts of the last mixed-precision solve */
-int reef_solver_last(reef_ctx* ctx, long* inner_iters, long* outer_iters, double* rel_residual);
+int reef_solver_last(reef_ctx* ctx, long* iters, long* updates, double* rel_residual);
 
 /* pressure::start(a, p, ppois, psolv, pgc, pflow, u, v, w, alpha) with pjm_corr
  * (pjm_corr.cpp:90-121): projection of the three velocity fields, REEF_PRESS updated. */

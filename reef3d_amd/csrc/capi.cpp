@@ -55,8 +55,7 @@ struct reef_ctx
     // mixed-precision projection: FP32 inner vectors and operator, allocated on first use
     int F49 = 1;                         // reinit discretisation (ini.cpp default 1)
     int precision = REEF_SOLVER_FP64;
-    double inner_tol = 1.0e-5;
-    int max_outer = 40;
+    double delta = 0.1;                  // reliable-update threshold of the mixed solve
     BiVec<float> bf{};
     SolverOp32 op32{};
     float *kx32 = nullptr, *ky32 = nullptr, *kz32 = nullptr, *aii32 = nullptr, *r32 = nullptr;
@@ -223,28 +222,30 @@ int do_fluid_update(reef_ctx* c, double* vol2)
     return 0;
 }
 
-const char* const BICG_NAMES[4] = {"bicg_p", "bicg_v", "bicg_t", "bicg_x"};
-const char* const BICG32_NAMES[4] = {"bicg32_p", "bicg32_v", "bicg32_t", "bicg32_x"};
+const char* const BICG_NAMES[5] = {"bicg_p", "bicg_v", "bicg_t", "bicg_x", ""};
+const char* const MIX_NAMES[5] = {"mix_p", "bicg32_v", "bicg32_t", "bicg32_x", "mix_ru"};
 
-// BiCGStab iterations after launch_bicg_init, enqueued in batches; the device state says
-// when the solve ended.  Two batches are in flight: batch k+1 is enqueued before the host
-// waits for the status copied out after batch k, so the device does not idle while the
-// host polls.  Iterations launched after the end are no-ops (every kernel tests done).
-template<class V, class Op>
-int run_bicg(reef_ctx* c, const Op& op, const BiVec<V>& b, int maxiter, const char* const* names, BiState& hs)
+// BiCGStab iterations after the solver's init kernels, enqueued in batches; the device
+// state says when the solve ended.  Two batches are in flight: batch k+1 is enqueued
+// before the host waits for the status copied out after batch k, so the device does not
+// idle while the host polls.  Iterations launched after the end are no-ops (every kernel
+// tests done).  launch(phase, part) enqueues part 0 (grid kernel) or 1 (finalize) of a
+// phase; nphase phases make one iteration.
+template<class Launch>
+int run_iterations(reef_ctx* c, const BiState* st_dev, int nphase, const char* const* names, Launch launch,
+                   int maxiter, BiState& hs)
 {
-    const DevGrid& g = c->dg;
     const int batch = 8;
     auto enqueue = [&](long k) -> int {
         for(int q = 0; q < batch; ++q)
-            for(int ph = 0; ph < 4; ++ph)
+            for(int ph = 0; ph < nphase; ++ph)
             {
                 c->tbegin(names[ph]);
-                launch_bicg_phase(g, op, b, ph, 0, c->stream);
+                launch(ph, 0);
                 c->tend();
-                launch_bicg_phase(g, op, b, ph, 1, c->stream);
+                launch(ph, 1);
             }
-        HIPCHK(hipMemcpyAsync(&c->st_host[k & 1], b.st, sizeof(BiState), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(&c->st_host[k & 1], st_dev, sizeof(BiState), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipEventRecord(c->st_ev[k & 1], c->stream));
         return 0;
     };
@@ -257,8 +258,8 @@ int run_bicg(reef_ctx* c, const Op& op, const BiVec<V>& b, int maxiter, const ch
         hs = c->st_host[(k - 1) & 1];
         if(hs.done) break;
         if(c->timing && c->ranges.size() > 2048) c->tcollect();
-        // every launched iteration advances iter or restarts; restarts also count, so
-        // the solve ends within maxiter + (restarts) batches
+        // every launched iteration advances iter, restarts or updates; each of these
+        // counts toward maxiter, so the solve ends within a bounded number of batches
         if(k * batch > 4L * (maxiter + 16)) return fail(REEF_E_HIP, "BiCGStab did not terminate");
     }
     if(c->timing) c->tcollect();
@@ -280,7 +281,8 @@ int do_solve(reef_ctx* c, double* x, const double* rhs, double stop, int maxiter
     c->tend();
     BiState hs;
     int rc;
-    if((rc = run_bicg(c, c->op64, b, maxiter, BICG_NAMES, hs))) return rc;
+    auto launch = [&](int ph, int part) { launch_bicg_phase(g, c->op64, b, ph, part, c->stream); };
+    if((rc = run_iterations(c, b.st, 4, BICG_NAMES, launch, maxiter, hs))) return rc;
     if(iters) *iters = hs.iter;
     if(residual) *residual = hs.residual;
     return 0;
@@ -297,7 +299,7 @@ int alloc_mixed(reef_ctx* c)
     for(float** p : bw)
         if((rc = c->falloc(p, n))) return rc;
     b.x = fx;
-    b.rhs = c->r32;
+    b.rhs = nullptr;
     b.aii = c->aii32;
     b.partials = c->bi.partials;
     double* stmem = nullptr;
@@ -318,13 +320,13 @@ int alloc_mixed(reef_ctx* c)
     return 0;
 }
 
-// Mixed-precision projection solve: FP64 defect correction around an FP32 BiCGStab.
-// Outer step: r = b - A x with the FP64 operator; stop when ||r||_2 meets the criterion
-// (stop >= 0: ||r||_2 / N < stop as bicgstab_ijk.cpp:264; stop < 0: ||r||_2 < |stop| *
-// ||b - A x0||_2); else solve A d = r in FP32 to the relative tolerance inner_tol (or
-// what the remaining reduction needs) and x += d.  iters receives the summed inner
-// iterations, outer the number of corrections.
-int do_solve_mixed(reef_ctx* c, double* x, const double* rhs, double stop, int maxiter, int* iters, int* outer,
+// Mixed-precision projection solve: BiCGStab iterating in FP32 (FP32 face conductances and
+// Jacobi preconditioner, FP64 dot products) with FP64 reliable updates of the residual
+// (launch_mixed_phase); stops when the true FP64 residual meets the criterion (stop >= 0:
+// ||r||_2 / N < stop as bicgstab_ijk.cpp:264; stop < 0: ||r||_2 < |stop| ||b - A x0||_2).
+// iters receives the iterations (restarts counted as the reference counts them), nru the
+// reliable updates, rel_residual the final ||b - A x||_2 / ||b - A x0||_2.
+int do_solve_mixed(reef_ctx* c, double* x, const double* rhs, double stop, int maxiter, int* iters, int* nru,
                    double* rel_residual)
 {
     int rc;
@@ -334,49 +336,23 @@ int do_solve_mixed(reef_ctx* c, double* x, const double* rhs, double stop, int m
     launch_face_rho(g, c->field[REEF_PHI], c->prm.W1, c->prm.W3, c->rfx, c->rfy, c->rfz, nullptr, c->stream);
     c->tend();
     c->tbegin("coef32");
-    launch_coef32(g, c->op64.rfx, c->op64.rfy, c->op64.rfz, c->kx32, c->ky32, c->kz32, c->aii32, c->stream);
+    launch_coef32(g, c->rfx, c->rfy, c->rfz, c->kx32, c->ky32, c->kz32, c->aii32, c->stream);
     c->tend();
-    const double ncell = (double)g.nx * g.ny * g.nz;
-    double ref = 0.0, target = 0.0, prev = 0.0, res = 0.0;
-    long inner_total = 0;
-    int nouter = 0;
-    for(;;)
-    {
-        c->tbegin("defect");
-        launch_defect(g, c->op64, x, rhs, c->r32, c->bf.x, c->bi.partials, c->norm_dev, c->stream);
-        c->tend();
-        HIPCHK(hipMemcpyAsync(&res, c->norm_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        if(c->timing) c->tcollect();
-        if(!std::isfinite(res)) return fail(REEF_E_HIP, "mixed-precision solve: non-finite defect");
-        if(nouter == 0)
-        {
-            ref = res;
-            target = stop >= 0.0 ? stop * ncell : -stop * res;
-        }
-        // converged, out of budget, or no longer contracting (the FP32 inner solve has
-        // reached what FP32 can resolve)
-        if(res <= target || res == 0.0 || nouter >= c->max_outer || inner_total >= maxiter) break;
-        if(nouter > 0 && res > 0.5 * prev) break;
-        prev = res;
-        const double need = 0.3 * target / res;
-        const double tol = need > c->inner_tol ? need : c->inner_tol;
-        c->tbegin("bicg32_init");
-        launch_bicg_init(g, c->op32, c->bf, -tol, (int)(maxiter - inner_total), c->stream);
-        c->tend();
-        BiState hs;
-        if((rc = run_bicg(c, c->op32, c->bf, (int)(maxiter - inner_total), BICG32_NAMES, hs))) return rc;
-        inner_total += hs.iter;
-        c->tbegin("add32");
-        launch_add32(g, x, c->bf.x, c->stream);
-        c->tend();
-        ++nouter;
-    }
-    c->last_inner = inner_total;
-    c->last_outer = nouter;
-    c->last_rel = ref > 0.0 ? res / ref : 0.0;
-    if(iters) *iters = (int)inner_total;
-    if(outer) *outer = nouter;
+    const MixedVec m{x, rhs};
+    c->tbegin("mix_init");
+    launch_mixed_init(g, c->op64, m, c->bf, stop, maxiter, c->delta, c->stream);
+    c->tend();
+    BiState hs;
+    auto launch = [&](int ph, int part) { launch_mixed_phase(g, c->op64, c->op32, m, c->bf, ph, part, c->stream); };
+    if((rc = run_iterations(c, c->bf.st, 5, MIX_NAMES, launch, maxiter, hs))) return rc;
+    c->tbegin("add32");
+    launch_add32(g, x, c->bf.x, c->stream);
+    c->tend();
+    c->last_inner = hs.iter;
+    c->last_outer = hs.nru;
+    c->last_rel = hs.res0 > 0.0 ? hs.residual * double(hs.ncell) / hs.res0 : 0.0;
+    if(iters) *iters = hs.iter;
+    if(nru) *nru = hs.nru;
     if(rel_residual) *rel_residual = c->last_rel;
     return 0;
 }
@@ -805,14 +781,14 @@ int reef_poisson_solve(reef_ctx* c, int x_id, int rhs_id, double stop, int maxit
     return do_solve(c, x, rhs, stop, maxiter, iters, residual);
 }
 
-int reef_poisson_solve_mixed(reef_ctx* c, int x_id, int rhs_id, double stop, int maxiter, int* inner_iters,
-                             int* outer_iters, double* rel_residual)
+int reef_poisson_solve_mixed(reef_ctx* c, int x_id, int rhs_id, double stop, int maxiter, int* iters,
+                             int* updates, double* rel_residual)
 {
     if(check_ctx(c)) return REEF_E_ARG;
     double *x = field_ptr(c, x_id), *rhs = field_ptr(c, rhs_id);
     if(!x || !rhs || maxiter < 0) return fail(REEF_E_ARG, "bad field id or maxiter");
     if(x == rhs) return fail(REEF_E_ARG, "x and rhs must be different fields");
-    return do_solve_mixed(c, x, rhs, stop, maxiter, inner_iters, outer_iters, rel_residual);
+    return do_solve_mixed(c, x, rhs, stop, maxiter, iters, updates, rel_residual);
 }
 
 int reef_poisson_residual(reef_ctx* c, int x_id, int rhs_id, double* norm)
@@ -824,7 +800,7 @@ int reef_poisson_residual(reef_ctx* c, int x_id, int rhs_id, double* norm)
     if(!c->norm_dev && (rc = c->dalloc(&c->norm_dev, 8))) return rc;
     const DevGrid& g = c->dg;
     launch_face_rho(g, c->field[REEF_PHI], c->prm.W1, c->prm.W3, c->rfx, c->rfy, c->rfz, nullptr, c->stream);
-    launch_defect(g, c->op64, x, rhs, nullptr, nullptr, c->bi.partials, c->norm_dev, c->stream);
+    launch_defect(g, c->op64, x, rhs, c->bi.partials, c->norm_dev, c->stream);
     HIPCHK(hipMemcpyAsync(norm, c->norm_dev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
@@ -842,22 +818,21 @@ int reef_set_control(reef_ctx* c, const char* key, double value)
     return fail(REEF_E_UNSUPPORTED, std::string("control key not supported: ") + key);
 }
 
-int reef_solver_config(reef_ctx* c, int precision, double inner_tol, int max_outer)
+int reef_solver_config(reef_ctx* c, int precision, double delta)
 {
     if(check_ctx(c)) return REEF_E_ARG;
     if(precision != REEF_SOLVER_FP64 && precision != REEF_SOLVER_MIXED) return fail(REEF_E_ARG, "bad precision");
-    if(!(inner_tol > 0.0 && inner_tol < 1.0) || max_outer < 1) return fail(REEF_E_ARG, "bad inner_tol or max_outer");
+    if(!(delta > 0.0 && delta < 1.0)) return fail(REEF_E_ARG, "delta must lie in (0, 1)");
     c->precision = precision;
-    c->inner_tol = inner_tol;
-    c->max_outer = max_outer;
+    c->delta = delta;
     return 0;
 }
 
-int reef_solver_last(reef_ctx* c, long* inner_iters, long* outer_iters, double* rel_residual)
+int reef_solver_last(reef_ctx* c, long* iters, long* updates, double* rel_residual)
 {
     if(check_ctx(c)) return REEF_E_ARG;
-    if(inner_iters) *inner_iters = c->last_inner;
-    if(outer_iters) *outer_iters = c->last_outer;
+    if(iters) *iters = c->last_inner;
+    if(updates) *updates = c->last_outer;
     if(rel_residual) *rel_residual = c->last_rel;
     return 0;
 }

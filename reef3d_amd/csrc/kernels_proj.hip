@@ -289,30 +289,36 @@ __device__ __forceinline__ V diag_of(const DevGrid& g, const CouplingV<V>& q)
 // y = A x at cell c with the folded operator of poisson_pcorr.cpp:124-230: boundary
 // neighbours (pressure-correction ghosts are 0) drop out, the diagonal keeps them.
 // Summation order of matvec_std (bicgstab_ijk.cpp:296-310); 5 points in 2-D
-// (bicgstab_ijk_2D.cpp:308-320).
-template<class V>
-__device__ __forceinline__ V apply_A(const DevGrid& g, const CouplingV<V>& q, const V* __restrict__ x, long c, int i,
-                                     int j, int k)
+// (bicgstab_ijk_2D.cpp:308-320).  xv(cc) gives the value of x at element cc.
+template<class V, class XV>
+__device__ __forceinline__ V apply_A_fn(const DevGrid& g, const CouplingV<V>& q, XV xv, long c, int i, int j, int k)
 {
     const V Mp = diag_of(g, q);
     const V Mn = i == g.nx - 1 ? V(0) : -q.T[0];
     const V Ms = i == 0 ? V(0) : -q.T[1];
     const V Mt = k == g.nz - 1 ? V(0) : -q.T[4];
     const V Mb = k == 0 ? V(0) : -q.T[5];
-    const V xn = i == g.nx - 1 ? V(0) : x[c + 1];
-    const V xs = i == 0 ? V(0) : x[c - 1];
-    const V xt = k == g.nz - 1 ? V(0) : x[c + g.sxy];
-    const V xb = k == 0 ? V(0) : x[c - g.sxy];
+    const V xn = i == g.nx - 1 ? V(0) : xv(c + 1);
+    const V xs = i == 0 ? V(0) : xv(c - 1);
+    const V xt = k == g.nz - 1 ? V(0) : xv(c + g.sxy);
+    const V xb = k == 0 ? V(0) : xv(c - g.sxy);
     if(g.jdir == 1)
     {
         const V yd = (V)g.ydir;
         const V Mw = j == g.ny - 1 ? V(0) : -q.T[2] * yd;
         const V Me = j == 0 ? V(0) : -q.T[3] * yd;
-        const V xw = j == g.ny - 1 ? V(0) : x[c + g.sx];
-        const V xe = j == 0 ? V(0) : x[c - g.sx];
-        return Mp * x[c] + Mn * xn + Ms * xs + Mw * xw + Me * xe + Mt * xt + Mb * xb;
+        const V xw = j == g.ny - 1 ? V(0) : xv(c + g.sx);
+        const V xe = j == 0 ? V(0) : xv(c - g.sx);
+        return Mp * xv(c) + Mn * xn + Ms * xs + Mw * xw + Me * xe + Mt * xt + Mb * xb;
     }
-    return Mp * x[c] + Mn * xn + Ms * xs + Mt * xt + Mb * xb;
+    return Mp * xv(c) + Mn * xn + Ms * xs + Mt * xt + Mb * xb;
+}
+
+template<class V>
+__device__ __forceinline__ V apply_A(const DevGrid& g, const CouplingV<V>& q, const V* __restrict__ x, long c, int i,
+                                     int j, int k)
+{
+    return apply_A_fn(g, q, [&](long cc) { return x[cc]; }, c, i, j, k);
 }
 
 // face densities of every "+" face of the interior cells and of the -1 layer faces, plus
@@ -537,7 +543,7 @@ template<class V, class Op>
 __global__ void __launch_bounds__(NT, 8) k_bicg_v(DevGrid g, Op op, BiVec<V> b)
 {
     BiState* st = b.st;
-    if(st->done) return;
+    if(st->done || st->mode == 1 || st->mode == 3) return;
     double vals[3] = {0.0, 0.0, 0.0};
     const V* __restrict__ ph = b.ph;
     const V* __restrict__ r = b.r;
@@ -562,7 +568,7 @@ __global__ void __launch_bounds__(NT, 8) k_bicg_v(DevGrid g, Op op, BiVec<V> b)
 
 __global__ void k_fin_v(const double* __restrict__ partials, BiState* st, int nb)
 {
-    if(st->done) return;
+    if(st->done || st->mode == 1 || st->mode == 3) return;
     double tot[3];
     sum_partials<3>(partials, nb, tot);
     if(threadIdx.x == 0)
@@ -589,7 +595,7 @@ template<class V, class Op>
 __global__ void __launch_bounds__(NT, 8) k_bicg_t(DevGrid g, Op op, BiVec<V> b)
 {
     BiState* st = b.st;
-    if(st->done || st->mode == 1) return;
+    if(st->done || st->mode == 1 || st->mode == 3) return;
     double vals[3] = {0.0, 0.0, 0.0};
     const V alpha = (V)st->alpha;
     const V* __restrict__ r = b.r;
@@ -639,7 +645,7 @@ __global__ void __launch_bounds__(NT, 8) k_bicg_t(DevGrid g, Op op, BiVec<V> b)
 
 __global__ void k_fin_t(const double* __restrict__ partials, BiState* st, int nb)
 {
-    if(st->done || st->mode == 1) return;
+    if(st->done || st->mode == 1 || st->mode == 3) return;
     double tot[3];
     sum_partials<3>(partials, nb, tot);
     if(threadIdx.x == 0)
@@ -693,7 +699,7 @@ template<class V>
 __global__ void __launch_bounds__(NT, 8) k_bicg_x(DevGrid g, BiVec<V> b)
 {
     BiState* st = b.st;
-    if(st->done || st->mode == 1) return;
+    if(st->done || st->mode == 1 || st->mode == 3) return;
     double vals[2] = {0.0, 0.0};
     const V alpha = (V)st->alpha, w = (V)st->w;
     const int branch = st->branch;
@@ -706,7 +712,7 @@ __global__ void __launch_bounds__(NT, 8) k_bicg_x(DevGrid g, BiVec<V> b)
 
 __global__ void k_fin_x(const double* __restrict__ partials, BiState* st, int nb)
 {
-    if(st->done || st->mode == 1) return;
+    if(st->done || st->mode == 1 || st->mode == 3) return;
     const double alpha = st->alpha, w = st->w;
     const int branch = st->branch;
     double tot[2];
@@ -714,13 +720,28 @@ __global__ void k_fin_x(const double* __restrict__ partials, BiState* st, int nb
     if(threadIdx.x == 0)
     {
         const double r_j1 = tot[0];
+        st->r_j_prev = st->r_j;
         if(branch) st->beta = alpha * r_j1 / (w * st->r_j == 0 ? 1.0e-15 : (w * st->r_j));
         st->r_j1 = r_j1;
         st->r_j = r_j1;
         st->residual = sqrt(tot[1]) / double(st->ncell);
         st->iter += 1;
         st->mode = 0;
-        if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
+        if(!st->mixed)
+        {
+            if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
+            return;
+        }
+        // FP32 iteration: the solve only ends on a true FP64 residual.  A step that took
+        // the norm_sj <= stop exit, or any restart, is continued by a restart from the true
+        // residual; convergence of the iterated residual, the iteration limit, or a drop
+        // below delta * (largest ||r|| since the last update) asks for a reliable update.
+        const double rn = sqrt(tot[1]);
+        if(rn > st->rmax) st->rmax = rn;
+        if(!branch)
+            st->mode = 1;
+        else if(!(st->residual >= st->stop) || st->iter >= st->maxiter || rn < st->delta * st->rmax)
+            st->mode = 3;
     }
 }
 
@@ -735,8 +756,11 @@ __global__ void k_bicg_setup(BiState* st, double stop, int maxiter, long ncell)
     st->done = 0;
     st->mode = 0;
     st->branch = 0;
+    st->mixed = 0;
+    st->r_j_prev = st->rmax = st->delta = st->res0 = 0.0;
+    st->pending = 0;
+    st->nru = 0;
     st->ncell = ncell;
-    for(int q = 0; q < 4; ++q) st->ticket[q] = 0u;
 }
 
 static inline unsigned rblocks(const DevGrid& g)
@@ -833,12 +857,9 @@ void launch_coef32(const DevGrid& g, const double* rfx, const double* rfy, const
     hipLaunchKernelGGL(k_coef32, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, rfx, rfy, rfz, kx, ky, kz, aii);
 }
 
-// outer defect of the mixed-precision solve: r = b - A x in FP64 with the reference
-// operator, handed to the inner solve as its FP32 right-hand side; the inner solution
-// d starts at 0 (with r32 = null only ||r||_2 is formed).  The partials give ||r||_2.
+// ||b - A x||_2 with the FP64 operator (res_calc, bicgstab_ijk.cpp:314-338, without 1/N)
 __global__ void __launch_bounds__(NT, 8) k_defect(DevGrid g, SolverOp64 op, const double* __restrict__ x,
-                                                const double* __restrict__ rhs, float* __restrict__ r32,
-                                                float* __restrict__ d32, double* __restrict__ partials)
+                                                   const double* __restrict__ rhs, double* __restrict__ partials)
 {
     double vals[1] = {0.0};
     const TileWalk tw = tile_walk(g);
@@ -849,11 +870,6 @@ __global__ void __launch_bounds__(NT, 8) k_defect(DevGrid g, SolverOp64 op, cons
         if(!tile_cell(g, tw, t, i, j, k, c)) continue;
         const Coupling q = coupling_of(g, op, c, i, j, k);
         const double rv = rhs[c] - apply_A(g, q, x, c, i, j, k);
-        if(r32)
-        {
-            r32[c] = (float)rv;
-            d32[c] = 0.0f;
-        }
         vals[0] += rv * rv;
     }
     block_partials<1>(vals, partials);
@@ -866,24 +882,238 @@ __global__ void k_fin_norm(const double* __restrict__ partials, int nb, double* 
     if(threadIdx.x == 0) out[0] = sqrt(tot[0]);
 }
 
-void launch_defect(const DevGrid& g, const SolverOp64& op, const double* x, const double* rhs, float* r32,
-                   float* d32, double* partials, double* norm_out, hipStream_t s)
+void launch_defect(const DevGrid& g, const SolverOp64& op, const double* x, const double* rhs, double* partials,
+                   double* norm_out, hipStream_t s)
 {
     const unsigned nb = rblocks(g);
-    hipLaunchKernelGGL(k_defect, dim3(nb), dim3(NT), 0, s, g, op, x, rhs, r32, d32, partials);
+    hipLaunchKernelGGL(k_defect, dim3(nb), dim3(NT), 0, s, g, op, x, rhs, partials);
     hipLaunchKernelGGL(k_fin_norm, dim3(1), dim3(NT), 0, s, (const double*)partials, (int)nb, norm_out);
 }
 
-// x += d (FP64 accumulation of the FP32 correction)
-__global__ void k_add32(DevGrid g, int nbx, int nby, int nbtot, double* __restrict__ x, const float* __restrict__ d)
+// ---- FP32 BiCGStab with FP64 reliable updates.  The iteration is bicgstab_ijk's on the
+// FP32 operator; x32 holds the solution increment since the last reliable update, x64 the
+// rest.  A reliable update replaces the iterated residual by the true one,
+// r = b - A64 (x64 + x32), keeps the search direction and recomputes beta from the new
+// (r0, r); a restart (the reference's tests, or the norm_sj <= stop exit) also resets
+// r0 = p = r.  The solve ends only on a true FP64 residual.
+
+// initial residual in FP64: r = b - A64 x64; r0 = p = r, ph = aii r, x32 = 0
+__global__ void __launch_bounds__(NT, 8) k_mix_start(DevGrid g, SolverOp64 op, MixedVec m, BiVec<float> b)
+{
+    double vals[2] = {0.0, 0.0};
+    const TileWalk tw = tile_walk(g);
+    for(int t = tw.t; t < tw.t1; ++t)
+    {
+        int i, j, k;
+        long c;
+        if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+        const Coupling q = coupling_of(g, op, c, i, j, k);
+        const double rv = m.b64[c] - apply_A(g, q, (const double*)m.x64, c, i, j, k);
+        const float r = (float)rv;
+        b.r[c] = r;
+        b.r0[c] = r;
+        b.p[c] = r;
+        b.ph[c] = r * b.aii[c];
+        b.x[c] = 0.0f;
+        vals[0] += rv * rv;
+        vals[1] += double(r) * double(r);
+    }
+    block_partials<2>(vals, b.partials);
+}
+
+__global__ void k_mix_setup(BiState* st, double stop, int maxiter, long ncell, double delta)
+{
+    st->r_j = st->norm_r0 = st->alpha = st->w = st->beta = 0.0;
+    st->sigma = st->norm_vj = st->norm_rj = st->norm_sj = st->w1 = st->w2 = st->r_j1 = 0.0;
+    st->residual = 1.0e9;
+    st->stop = stop;
+    st->iter = 0;
+    st->maxiter = maxiter;
+    st->done = 0;
+    st->mode = 0;
+    st->branch = 0;
+    st->mixed = 1;
+    st->r_j_prev = st->rmax = st->res0 = 0.0;
+    st->delta = delta;
+    st->pending = 0;
+    st->nru = 0;
+    st->ncell = ncell;
+}
+
+__global__ void k_fin_mix_start(const double* __restrict__ partials, BiState* st, int nb)
+{
+    double tot[2];
+    sum_partials<2>(partials, nb, tot);
+    if(threadIdx.x == 0)
+    {
+        const double rn = sqrt(tot[0]);
+        st->r_j = tot[1];
+        st->norm_r0 = sqrt(tot[1]);
+        if(st->stop < 0.0) st->stop = -st->stop * rn / double(st->ncell);
+        st->residual = rn / double(st->ncell);
+        st->rmax = rn;
+        st->res0 = rn;
+        if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
+        st->mode = 2;
+    }
+}
+
+// start of an FP32 iteration: x64 += x32 after a reliable update, then the p update
+__global__ void __launch_bounds__(NT, 8) k_mix_p(DevGrid g, MixedVec m, BiVec<float> b)
+{
+    BiState* st = b.st;
+    if(st->done || st->mode == 1 || st->mode == 3) return;
+    const int upd = st->mode == 0, pend = st->pending;
+    if(!upd && !pend) return;
+    const float beta = (float)st->beta, w = (float)st->w;
+    const float* __restrict__ r = b.r;
+    const float* __restrict__ v = b.v;
+    const float* __restrict__ aii = b.aii;
+    float* __restrict__ p = b.p;
+    float* __restrict__ ph = b.ph;
+    float* __restrict__ x32 = b.x;
+    double* __restrict__ x64 = m.x64;
+    const TileWalk tw = tile_walk(g);
+    for(int t = tw.t; t < tw.t1; ++t)
+    {
+        int i, j, k;
+        long c;
+        if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+        if(pend)
+        {
+            x64[c] += (double)x32[c];
+            x32[c] = 0.0f;
+        }
+        if(upd)
+        {
+            const float pn = r[c] + beta * (p[c] - w * v[c]);
+            p[c] = pn;
+            ph[c] = pn * aii[c];
+        }
+    }
+}
+
+__global__ void k_fin_mix_p(BiState* st)
+{
+    if(st->done || st->mode == 1 || st->mode == 3) return;
+    st->pending = 0;
+}
+
+// reliable update (mode 3) or restart (mode 1): r = b - A64 (x64 + x32) in FP64, rounded
+// to the FP32 residual; a restart also sets r0 = p = r, ph = aii r
+__global__ void __launch_bounds__(NT, 8) k_mix_ru(DevGrid g, SolverOp64 op, MixedVec m, BiVec<float> b)
+{
+    BiState* st = b.st;
+    if(st->done || (st->mode != 1 && st->mode != 3)) return;
+    const int restart = st->mode == 1;
+    double vals[2] = {0.0, 0.0};
+    const double* __restrict__ x64 = m.x64;
+    const float* __restrict__ x32 = b.x;
+    const TileWalk tw = tile_walk(g);
+    for(int t = tw.t; t < tw.t1; ++t)
+    {
+        int i, j, k;
+        long c;
+        if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+        const Coupling q = coupling_of(g, op, c, i, j, k);
+        const double rv =
+            m.b64[c] - apply_A_fn(g, q, [&](long cc) { return x64[cc] + (double)x32[cc]; }, c, i, j, k);
+        const float r = (float)rv;
+        b.r[c] = r;
+        float r0v;
+        if(restart)
+        {
+            b.r0[c] = r;
+            b.p[c] = r;
+            b.ph[c] = r * b.aii[c];
+            r0v = r;
+        }
+        else
+            r0v = b.r0[c];
+        vals[0] += rv * rv;
+        vals[1] += double(r0v) * double(r);
+    }
+    block_partials<2>(vals, b.partials);
+}
+
+__global__ void k_fin_mix_ru(const double* __restrict__ partials, BiState* st, int nb)
+{
+    if(st->done || (st->mode != 1 && st->mode != 3)) return;
+    double tot[2];
+    sum_partials<2>(partials, nb, tot);
+    if(threadIdx.x == 0)
+    {
+        const double rn = sqrt(tot[0]);
+        st->residual = rn / double(st->ncell);
+        st->nru += 1;
+        st->pending = 1;
+        st->rmax = rn;
+        if(st->mode == 1) st->iter += 1;     // a restart counts as an iteration (bicgstab_ijk.cpp:183-191)
+        if(!((st->residual >= st->stop) && (st->iter < st->maxiter)))
+        {
+            st->done = 1;
+            return;
+        }
+        if(st->mode == 1)
+        {
+            st->r_j = tot[1];
+            st->norm_r0 = sqrt(tot[1]);
+            st->mode = 2;
+        }
+        else
+        {
+            const double wr = st->w * st->r_j_prev;
+            st->beta = st->alpha * tot[1] / (wr == 0 ? 1.0e-15 : wr);
+            st->r_j = tot[1];
+            st->mode = 0;
+        }
+    }
+}
+
+void launch_mixed_init(const DevGrid& g, const SolverOp64& op64, const MixedVec& m, const BiVec<float>& b32,
+                       double stop, int maxiter, double delta, hipStream_t s)
+{
+    const unsigned nb = rblocks(g);
+    hipLaunchKernelGGL(k_mix_setup, dim3(1), dim3(1), 0, s, b32.st, stop, maxiter, (long)g.nx * g.ny * g.nz, delta);
+    hipLaunchKernelGGL(k_mix_start, dim3(nb), dim3(NT), 0, s, g, op64, m, b32);
+    hipLaunchKernelGGL(k_fin_mix_start, dim3(1), dim3(NT), 0, s, (const double*)b32.partials, b32.st, (int)nb);
+}
+
+void launch_mixed_phase(const DevGrid& g, const SolverOp64& op64, const SolverOp32& op32, const MixedVec& m,
+                        const BiVec<float>& b32, int phase, int part, hipStream_t s)
+{
+    const dim3 grid(rblocks(g));
+    const int nb = (int)grid.x;
+    if(phase == 0)
+    {
+        if(part == 0)
+            hipLaunchKernelGGL(k_mix_p, grid, dim3(NT), 0, s, g, m, b32);
+        else
+            hipLaunchKernelGGL(k_fin_mix_p, dim3(1), dim3(1), 0, s, b32.st);
+        return;
+    }
+    if(phase == 4)
+    {
+        if(part == 0)
+            hipLaunchKernelGGL(k_mix_ru, grid, dim3(NT), 0, s, g, op64, m, b32);
+        else
+            hipLaunchKernelGGL(k_fin_mix_ru, dim3(1), dim3(NT), 0, s, (const double*)b32.partials, b32.st, nb);
+        return;
+    }
+    bicg_phase_t(g, op32, b32, phase, part, s);
+}
+
+// x += d, d = 0 (FP64 accumulation of the FP32 increment)
+__global__ void k_add32(DevGrid g, int nbx, int nby, int nbtot, double* __restrict__ x, float* __restrict__ d)
 {
     int i, j, k;
     long c;
     if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
     x[c] += (double)d[c];
+    d[c] = 0.0f;
 }
 
-void launch_add32(const DevGrid& g, double* x, const float* d, hipStream_t s)
+void launch_add32(const DevGrid& g, double* x, float* d, hipStream_t s)
 {
     int nbx, nby, nbtot;
     dim3 grid = tiles(g, nbx, nby, nbtot);

@@ -68,10 +68,16 @@ struct BiState
 {
     double r_j, norm_r0, alpha, w, beta, residual, stop, sigma, norm_vj, norm_rj, norm_sj;
     double w1, w2, r_j1;
-    int iter, maxiter, done, mode;     // mode: 0 run, 1 restart pending, 2 fresh p
+    int iter, maxiter, done, mode;     // mode: 0 run, 1 restart pending, 2 fresh p,
+                                       // 3 reliable update pending (mixed precision)
     int branch;                        // 1: norm_sj > stop
-    int pad;
-    unsigned int ticket[4];
+    int mixed;                         // 1: FP32 iteration with FP64 reliable updates
+    // mixed precision: r_j before the last x phase, largest ||r|| since the last
+    // reliable update, the update threshold, x32 waiting to be added to x64, count of
+    // reliable updates
+    double r_j_prev, rmax, delta;
+    double res0;                       // ||b - A x0||_2 (FP64) of a mixed solve
+    int pending, nru;
     long ncell;
 };
 
@@ -150,14 +156,28 @@ void launch_bicg_phase(const DevGrid& g, const SolverOp64& op, const BiVec<doubl
 void launch_bicg_phase(const DevGrid& g, const SolverOp32& op, const BiVec<float>& b, int phase, int part,
                        hipStream_t s);
 
-// mixed precision: FP32 operator and preconditioner from the FP64 face densities; the FP64
-// defect r = b - A x rounded into the inner right-hand side (d32 zeroed, ||r||_2 to
-// norm_out on the device); x += d
+// mixed precision: FP32 operator and preconditioner from the FP64 face densities;
+// ||b - A x||_2 in FP64 (norm_out on the device); x64 += x32, x32 = 0
 void launch_coef32(const DevGrid& g, const double* rfx, const double* rfy, const double* rfz, float* kx, float* ky,
                    float* kz, float* aii, hipStream_t s);
-void launch_defect(const DevGrid& g, const SolverOp64& op, const double* x, const double* rhs, float* r32,
-                   float* d32, double* partials, double* norm_out, hipStream_t s);
-void launch_add32(const DevGrid& g, double* x, const float* d, hipStream_t s);
+void launch_defect(const DevGrid& g, const SolverOp64& op, const double* x, const double* rhs, double* partials,
+                   double* norm_out, hipStream_t s);
+void launch_add32(const DevGrid& g, double* x, float* d, hipStream_t s);
+
+// FP32 BiCGStab with reliable updates: the iteration of launch_bicg_phase on the FP32
+// vectors of b32 (operator op32) while x64 accumulates the solution and every reliable
+// update recomputes the residual as b64 - A64 (x64 + x32) in FP64
+struct MixedVec
+{
+    double* x64;
+    const double* b64;
+};
+void launch_mixed_init(const DevGrid& g, const SolverOp64& op64, const MixedVec& m, const BiVec<float>& b32,
+                       double stop, int maxiter, double delta, hipStream_t s);
+// phase 0..3 as launch_bicg_phase, 4 = reliable update / restart (a no-op unless the state
+// asks for one); part 0 grid kernel, part 1 finalize
+void launch_mixed_phase(const DevGrid& g, const SolverOp64& op64, const SolverOp32& op32, const MixedVec& m,
+                        const BiVec<float>& b32, int phase, int part, hipStream_t s);
 
 // prescribed vortex field (benchmark_vortex3D.cpp:67-101) and the void momentum_RK3 update
 void launch_vortex_velocity(const DevGrid& g, int comp, double* q, double simtime, double DXM, hipStream_t s);

@@ -68,7 +68,7 @@ def library():
         "reef_poisson_solve_mixed": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ip, ip, dp],
                                      ctypes.c_int),
         "reef_poisson_residual": ([vp, ctypes.c_int, ctypes.c_int, dp], ctypes.c_int),
-        "reef_solver_config": ([vp, ctypes.c_int, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+        "reef_solver_config": ([vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "reef_solver_last": ([vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long), dp], ctypes.c_int),
         "reef_pjm_corr": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ip], ctypes.c_int),
         "reef_fc3_step": ([vp, ip, dp], ctypes.c_int),
@@ -223,12 +223,12 @@ class Domain:
         return it.value, float(res[0])
 
     def solver_mixed(self, x, rhs, stop, maxiter):
-        """Mixed-precision solve (FP32 BiCGStab inside FP64 defect correction) of the same
-        system; returns (inner iterations, corrections, final relative FP64 residual)."""
-        it, outer, rel = ctypes.c_int(), ctypes.c_int(), np.zeros(1)
+        """Mixed-precision solve (FP32 BiCGStab with FP64 reliable updates) of the same
+        system; returns (iterations, reliable updates, final relative FP64 residual)."""
+        it, nru, rel = ctypes.c_int(), ctypes.c_int(), np.zeros(1)
         _check(library().reef_poisson_solve_mixed(self.h, FIELDS[x], FIELDS[rhs], stop, maxiter,
-                                                  ctypes.byref(it), ctypes.byref(outer), _dp(rel)))
-        return it.value, outer.value, float(rel[0])
+                                                  ctypes.byref(it), ctypes.byref(nru), _dp(rel)))
+        return it.value, nru.value, float(rel[0])
 
     def residual(self, x, rhs):
         """||rhs - A x||_2 in FP64 on the device (operator assembled from phi)."""
@@ -236,13 +236,13 @@ class Domain:
         _check(library().reef_poisson_residual(self.h, FIELDS[x], FIELDS[rhs], _dp(out)))
         return float(out[0])
 
-    def solver_config(self, precision="fp64", inner_tol=1e-5, max_outer=40):
+    def solver_config(self, precision="fp64", delta=0.1):
         """Solver used by pressure() and momentum_fc3(): "fp64" or "mixed"."""
         prec = {"fp64": 0, "mixed": 1}[precision]
-        _check(library().reef_solver_config(self.h, prec, inner_tol, max_outer))
+        _check(library().reef_solver_config(self.h, prec, delta))
 
     def solver_last(self):
-        """(inner iterations, corrections, relative residual) of the last mixed solve."""
+        """(iterations, reliable updates, relative residual) of the last mixed solve."""
         a, b, r = ctypes.c_long(), ctypes.c_long(), np.zeros(1)
         _check(library().reef_solver_last(self.h, ctypes.byref(a), ctypes.byref(b), _dp(r)))
         return a.value, b.value, float(r[0])
