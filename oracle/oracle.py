@@ -140,6 +140,10 @@ class Domain:
         it = lib().orc_bicgstab(self.h, _p(x), _p(M), _p(rhs), stop, maxiter, _p(res))
         return it, res[0]
 
+    def set_control(self, key, value):
+        if lib().orc_set_control(self.h, key.encode(), float(value)) != 0:
+            raise ValueError(f"control {key}={value} not supported")
+
     def residual(self, x, M, rhs, r=None):
         """||rhs - A x||_2 in the order of res_calc; r (field layout) receives the residual."""
         M = np.ascontiguousarray(M)

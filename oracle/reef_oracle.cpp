@@ -39,6 +39,7 @@ struct orc_domain
 {
     int nx, ny, nz, jdir;
     int F49 = 1;                // reinit discretisation: 1 reinidisc_f, 0 reinidisc_fsf
+    int dot_ext = 0;            // 1: BiCGStab dot products accumulated in long double
     long sx, sy, sz;
     int group[7];
     orc_params prm;
@@ -767,25 +768,35 @@ struct Bicg
         return M[n] * xx[c] + M[N + n] * xx[c + 1] + M[2 * N + n] * xx[c - 1] + M[5 * N + n] * xx[c + sxy]
              + M[6 * N + n] * xx[c - sxy];
     }
-    double res_calc()
+    // A: accumulator type of the dot products; double reproduces the reference, long
+    // double (the "dot_ext" control) perturbs only their rounding, to measure how far the
+    // solution moves when a different summation order is used (a device reduction)
+    template<class A> double res_calc()
     {
-        double resi = 0.0;
+        A resi = 0.0;
         loop([&](long c, long n) {
             double y = rhs[c] - (Ax(x, c, n));
             resi += y * y;
         });
-        return sqrt(resi) / double(d->cellnum);
+        return sqrt((double)resi) / double(d->cellnum);
     }
     int solve(double stop, int maxiter, double* resout)
+    {
+        return d->dot_ext ? solve_t<long double>(stop, maxiter, resout) : solve_t<double>(stop, maxiter, resout);
+    }
+    template<class A> int solve_t(double stop, int maxiter, double* resout)
     {
         int solveriter = 0;
         double residual = 1.0e9;
         loop([&](long c, long n) { aii[c] = -1.0 / (M[n] + 1.0e-19); });
         double r_j, norm_r0, r_j1 = 0.0;
     restart:
-        r_j = 0.0;
-        loop([&](long c, long n) { rj[c] = rhs[c] - (Ax(x, c, n)); });
-        loop([&](long c, long) { r0[c] = pj[c] = rj[c]; r_j += rj[c] * r0[c]; });
+        {
+            A acc = 0.0;
+            loop([&](long c, long n) { rj[c] = rhs[c] - (Ax(x, c, n)); });
+            loop([&](long c, long) { r0[c] = pj[c] = rj[c]; acc += rj[c] * r0[c]; });
+            r_j = (double)acc;
+        }
         norm_r0 = sqrt(r_j);
         // negative stop: relative to the initial residual (harness mapping of a relative
         // tolerance onto N44, not a reference feature)
@@ -794,67 +805,70 @@ struct Bicg
         {
             do
             {
-                double sigma = 0.0, norm_vj = 0.0, norm_rj = 0.0;
+                A sig = 0.0, nvj = 0.0, nrj = 0.0;
                 loop([&](long c, long) { ph[c] = pj[c] * aii[c]; });
                 loop([&](long c, long n) { vj[c] = Ax(ph, c, n); });
                 loop([&](long c, long) {
-                    sigma += vj[c] * r0[c];
-                    norm_vj += vj[c] * vj[c];
-                    norm_rj += rj[c] * rj[c];
+                    sig += vj[c] * r0[c];
+                    nvj += vj[c] * vj[c];
+                    nrj += rj[c] * rj[c];
                 });
-                norm_vj = sqrt(norm_vj);
-                norm_rj = sqrt(norm_rj);
+                const double sigma = (double)sig;
+                const double norm_vj = sqrt((double)nvj);
+                const double norm_rj = sqrt((double)nrj);
                 double alpha = r_j / sigma;
                 if(fabs(sigma) <= (1.0e-12 * (norm_vj * norm_r0)))
                 {
-                    residual = res_calc();
+                    residual = res_calc<A>();
                     ++solveriter;
                     goto restart;
                 }
                 if((fabs(alpha) * norm_vj / (norm_rj == 0 ? 1.0e-15 : norm_rj)) <= 0.08)
                 {
-                    residual = res_calc();
+                    residual = res_calc<A>();
                     ++solveriter;
                     goto restart;
                 }
-                double norm_sj = 0.0;
+                A nsj = 0.0;
                 loop([&](long c, long) {
                     sj[c] = rj[c] - alpha * vj[c];
-                    norm_sj += sj[c] * sj[c];
+                    nsj += sj[c] * sj[c];
                 });
-                norm_sj = sqrt(norm_sj);
+                const double norm_sj = sqrt((double)nsj);
                 if(norm_sj > stop)
                 {
                     loop([&](long c, long) { sh[c] = sj[c] * aii[c]; });
                     loop([&](long c, long n) { tj[c] = Ax(sh, c, n); });
-                    double w1 = 0.0, w2 = 0.0;
+                    A w1 = 0.0, w2 = 0.0;
                     loop([&](long c, long) {
                         w1 += tj[c] * sj[c];
                         w2 += tj[c] * tj[c];
                     });
-                    double w = w1 / (w2 == 0 ? 1.0e-15 : w2);
-                    r_j1 = 0.0;
+                    double w = (double)w1 / ((double)w2 == 0 ? 1.0e-15 : (double)w2);
+                    A rj1 = 0.0;
                     loop([&](long c, long) {
                         x[c] += alpha * ph[c] + w * sh[c];
                         rj[c] = sj[c] - w * tj[c];
-                        r_j1 += rj[c] * r0[c];
+                        rj1 += rj[c] * r0[c];
                     });
+                    r_j1 = (double)rj1;
                     double beta = alpha * r_j1 / (w * r_j == 0 ? 1.0e-15 : (w * r_j));
                     loop([&](long c, long) { pj[c] = rj[c] + beta * (pj[c] - w * vj[c]); });
                 }
                 if(norm_sj <= stop)
                 {
-                    r_j1 = 0.0;
+                    A rj1 = 0.0;
                     loop([&](long c, long) {
                         x[c] += alpha * ph[c];
                         rj[c] = sj[c];
-                        r_j1 += rj[c] * r0[c];
+                        rj1 += rj[c] * r0[c];
                     });
+                    r_j1 = (double)rj1;
                 }
                 r_j = r_j1;
-                residual = 0.0;
-                loop([&](long c, long) { residual += rj[c] * rj[c]; });
-                residual = sqrt(residual) / double(d->cellnum);
+                A res = 0.0;
+                loop([&](long c, long) { res += rj[c] * rj[c]; });
+                residual = sqrt((double)res) / double(d->cellnum);
                 ++solveriter;
             } while((residual >= stop) && (solveriter < maxiter));
         }
@@ -1297,6 +1311,11 @@ int orc_set_control(orc_domain* d, const char* key, double value)
     if(std::string(key) == "F49" && (value == 0.0 || value == 1.0))
     {
         d->F49 = (int)value;
+        return 0;
+    }
+    if(std::string(key) == "dot_ext" && (value == 0.0 || value == 1.0))
+    {
+        d->dot_ext = (int)value;
         return 0;
     }
     return -1;

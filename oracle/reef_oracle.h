@@ -61,7 +61,9 @@ void orc_fluid_update(orc_domain*, const double* phi, double* ro, double* visc, 
 /* poisson_pcorr::start on (phi) -> 7 coefficient arrays + rhs (LOOP order, cellnum each) */
 void orc_poisson(orc_domain*, const double* phi, const double* pcorr, double* M7, double* rhs);
 /* bicgstab_ijk::start(var=5): x (field layout, in/out), M7 + rhs as from orc_poisson */
-/* control key of the hot path: "F49" (1 reinidisc_f, 0 reinidisc_fsf); 0 or -1 */
+/* control key of the hot path: "F49" (1 reinidisc_f, 0 reinidisc_fsf); or of the oracle
+ * itself: "dot_ext" (1: BiCGStab dot products accumulated in long double, a rounding
+ * perturbation for sensitivity checks); returns 0 or -1 */
 int orc_set_control(orc_domain*, const char* key, double value);
 /* r = rhs - A x (res_calc, bicgstab_ijk.cpp:314-338) per cell into r (may be NULL); returns ||r||_2 */
 double orc_residual(orc_domain*, const double* x, const double* M7, const double* rhs, double* r);

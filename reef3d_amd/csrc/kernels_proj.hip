@@ -68,6 +68,24 @@ __device__ __forceinline__ void block_partials(const double (&v)[M], double* __r
         for(int m = 0; m < M; ++m) partials[(size_t)blockIdx.x * M + m] = sh[m][0];
 }
 
+// The same partial sums with a wavefront butterfly (fixed order, lane 0's sum) and a
+// 4-entry LDS combine: needs M * 4 doubles of LDS instead of M * NT, so kernels that keep
+// stencil planes in LDS still fit eight blocks per CU.
+template<int M>
+__device__ __forceinline__ void block_partials_w(double (&v)[M], double* __restrict__ partials)
+{
+    __shared__ double ws[M][NT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for(int m = 0; m < M; ++m)
+        for(int off = 32; off > 0; off >>= 1) v[m] += __shfl_xor(v[m], off);
+    if(lane == 0)
+        for(int m = 0; m < M; ++m) ws[m][wave] = v[m];
+    __syncthreads();
+    if(threadIdx.x == 0)
+        for(int m = 0; m < M; ++m)
+            partials[(size_t)blockIdx.x * M + m] = ((ws[m][0] + ws[m][1]) + ws[m][2]) + ws[m][3];
+}
+
 // totals of nb block partials, summed in block order by one block of NT threads
 template<int M>
 __device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int nb, double (&tot)[M])
@@ -279,6 +297,45 @@ __device__ __forceinline__ CouplingV<float> coupling_of(const DevGrid& g, const 
     return q;
 }
 
+// couplings from the six face values (+x, -x, +y, -y, +z, -z) of cell (i, j, k): the face
+// densities for the FP64 operator, the FP32 conductances for the FP32 one
+__device__ __forceinline__ CouplingV<double> coupling_faces(const DevGrid& g, const SolverOp64&, double F0, double F1,
+                                                            double F2, double F3, double F4, double F5, int i, int j,
+                                                            int k)
+{
+    const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
+    CouplingV<double> q;
+    q.T[0] = (1.0 * 1.0) / (F0 * g.DXP[ip] * g.DXN[ip]);
+    q.T[1] = (1.0 * 1.0) / (F1 * g.DXP[ip - 1] * g.DXN[ip]);
+    q.T[2] = (1.0 * 1.0) / (F2 * g.DYP[jp] * g.DYN[jp]);
+    q.T[3] = (1.0 * 1.0) / (F3 * g.DYP[jp - 1] * g.DYN[jp]);
+    q.T[4] = (1.0 * 1.0) / (F4 * g.DZP[kp] * g.DZN[kp]);
+    q.T[5] = (1.0 * 1.0) / (F5 * g.DZP[kp - 1] * g.DZN[kp]);
+    return q;
+}
+
+__device__ __forceinline__ CouplingV<float> coupling_faces(const DevGrid&, const SolverOp32& op, float F0, float F1,
+                                                           float F2, float F3, float F4, float F5, int i, int j,
+                                                           int k)
+{
+    const float ix = op.iDXN[i + MARGE], iy = op.iDYN[j + MARGE], iz = op.iDZN[k + MARGE];
+    CouplingV<float> q;
+    q.T[0] = F0 * ix;
+    q.T[1] = F1 * ix;
+    q.T[2] = F2 * iy;
+    q.T[3] = F3 * iy;
+    q.T[4] = F4 * iz;
+    q.T[5] = F5 * iz;
+    return q;
+}
+
+__device__ __forceinline__ const double* face_x(const SolverOp64& op) { return op.rfx; }
+__device__ __forceinline__ const double* face_y(const SolverOp64& op) { return op.rfy; }
+__device__ __forceinline__ const double* face_z(const SolverOp64& op) { return op.rfz; }
+__device__ __forceinline__ const float* face_x(const SolverOp32& op) { return op.kx; }
+__device__ __forceinline__ const float* face_y(const SolverOp32& op) { return op.ky; }
+__device__ __forceinline__ const float* face_z(const SolverOp32& op) { return op.kz; }
+
 template<class V>
 __device__ __forceinline__ V diag_of(const DevGrid& g, const CouplingV<V>& q)
 {
@@ -290,6 +347,29 @@ __device__ __forceinline__ V diag_of(const DevGrid& g, const CouplingV<V>& q)
 // neighbours (pressure-correction ghosts are 0) drop out, the diagonal keeps them.
 // Summation order of matvec_std (bicgstab_ijk.cpp:296-310); 5 points in 2-D
 // (bicgstab_ijk_2D.cpp:308-320).  xv(cc) gives the value of x at element cc.
+// the same sum from the seven values (xn/xs: i+-1, xw/xe: j+-1, xt/xb: k+-1), values
+// outside the domain are ignored
+template<class V>
+__device__ __forceinline__ V stencil7(const DevGrid& g, const CouplingV<V>& q, V xc, V xn_, V xs_, V xw_, V xe_, V xt_,
+                                      V xb_, int i, int j, int k)
+{
+    const V Mp = diag_of(g, q);
+    const V Mn = i == g.nx - 1 ? V(0) : -q.T[0];
+    const V Ms = i == 0 ? V(0) : -q.T[1];
+    const V Mt = k == g.nz - 1 ? V(0) : -q.T[4];
+    const V Mb = k == 0 ? V(0) : -q.T[5];
+    const V xn = i == g.nx - 1 ? V(0) : xn_;
+    const V xs = i == 0 ? V(0) : xs_;
+    const V xt = k == g.nz - 1 ? V(0) : xt_;
+    const V xb = k == 0 ? V(0) : xb_;
+    const V yd = (V)g.ydir;
+    const V Mw = j == g.ny - 1 ? V(0) : -q.T[2] * yd;
+    const V Me = j == 0 ? V(0) : -q.T[3] * yd;
+    const V xw = j == g.ny - 1 ? V(0) : xw_;
+    const V xe = j == 0 ? V(0) : xe_;
+    return Mp * xc + Mn * xn + Ms * xs + Mw * xw + Me * xe + Mt * xt + Mb * xb;
+}
+
 template<class V, class XV>
 __device__ __forceinline__ V apply_A_fn(const DevGrid& g, const CouplingV<V>& q, XV xv, long c, int i, int j, int k)
 {
@@ -436,6 +516,84 @@ __device__ __forceinline__ bool tile_cell(const DevGrid& g, const TileWalk& w, i
 
 __device__ __forceinline__ double sq(double a) { return a * a; }
 
+// ---- plane walk of the 3-D stencil kernels.  A block's tiles are consecutive k planes
+// of one 64 x 4 column (z fastest), so the block walks up the column: the stencil input
+// of plane k (rows j-1 .. j+4, columns i-1 .. i+64) and the +x / +y face values sit in
+// LDS, planes k-1 and k+1 of the own cell and the -z face stay in registers.  Every
+// array is then read from HBM about once per cell; the LDS planes are double buffered,
+// so one barrier per plane suffices.
+template<class V>
+struct PlaneLDS
+{
+    V x[TY + 2][TX + 2];   // stencil input, rows j-1..j+4, columns i-1..i+64
+    V fx[TY][TX + 1];      // +x face of columns i-1..i+63
+    V fy[TY + 1][TX];      // +y face of rows j-1..j+3
+};
+
+// load(cc, aux) returns the stencil input at element cc (aux: a by-product the cell
+// epilogue wants, e.g. s = r - alpha v); cell(c, i, j, k, Ax, aux) consumes the result
+template<class V, class Op, class Load, class Cell>
+__device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, Load load, Cell cell)
+{
+    __shared__ PlaneLDS<V> L[2];
+    const V* __restrict__ fxa = face_x(op);
+    const V* __restrict__ fya = face_y(op);
+    const V* __restrict__ fza = face_z(op);
+    const TileWalk tw = tile_walk(g);
+    const int lane = threadIdx.x & (TX - 1), row = threadIdx.x >> 6;
+    int curcol = -1, buf = 0;
+    V xd = V(0), xc = V(0), fzd = V(0), auxc = V(0);
+    for(int t = tw.t; t < tw.t1; ++t)
+    {
+        const int col = t / g.nz;
+        const int k = t - col * g.nz;
+        const int by = col / tw.nbx, bx = col - by * tw.nbx;
+        const int i = bx * TX + lane, j = by * TY + row;
+        const bool in = i < g.nx && j < g.ny;
+        const long c = g.base + i + g.sx * j + g.sxy * k;
+        if(col != curcol)
+        {
+            // first plane of this column in the block: prime the register window
+            curcol = col;
+            V dummy;
+            xd = (in && k > 0) ? load(c - g.sxy, dummy) : V(0);
+            xc = in ? load(c, auxc) : V(0);
+            fzd = in ? fza[c - g.sxy] : V(0);
+        }
+        V auxu = V(0);
+        const V xu = (in && k + 1 < g.nz) ? load(c + g.sxy, auxu) : V(0);
+        PlaneLDS<V>& P = L[buf];
+        P.x[row + 1][lane + 1] = xc;
+        {
+            V dummy;
+            if(row == 0) P.x[0][lane + 1] = (i < g.nx && j > 0 && j - 1 < g.ny) ? load(c - g.sx, dummy) : V(0);
+            if(row == TY - 1) P.x[TY + 1][lane + 1] = (i < g.nx && j + 1 < g.ny) ? load(c + g.sx, dummy) : V(0);
+            if(lane == 0) P.x[row + 1][0] = (j < g.ny && i > 0) ? load(c - 1, dummy) : V(0);
+            if(lane == TX - 1) P.x[row + 1][TX + 1] = (j < g.ny && i + 1 < g.nx) ? load(c + 1, dummy) : V(0);
+        }
+        const V f0 = in ? fxa[c] : V(0);
+        const V f2 = in ? fya[c] : V(0);
+        const V f4 = in ? fza[c] : V(0);
+        P.fx[row][lane + 1] = f0;
+        if(lane == 0) P.fx[row][0] = (j < g.ny) ? fxa[c - 1] : V(0);
+        P.fy[row + 1][lane] = f2;
+        if(row == 0) P.fy[0][lane] = (i < g.nx && j < g.ny) ? fya[c - g.sx] : V(0);
+        __syncthreads();
+        if(in)
+        {
+            const CouplingV<V> q = coupling_faces(g, op, f0, P.fx[row][lane], f2, P.fy[row][lane], f4, fzd, i, j, k);
+            const V Ax = stencil7(g, q, xc, P.x[row + 1][lane + 2], P.x[row + 1][lane], P.x[row + 2][lane + 1],
+                                  P.x[row][lane + 1], xu, xd, i, j, k);
+            cell(c, i, j, k, Ax, auxc);
+        }
+        buf ^= 1;
+        xd = xc;
+        xc = xu;
+        auxc = auxu;
+        fzd = f4;
+    }
+}
+
 // restart / initial residual: r = b - A x, r0 = p = r, ph = aii p, r_j = (r, r0)
 // (restart label of bicgstab_ijk.cpp:136-148; res_calc :314-338 gives the same sums)
 template<class V, class Op>
@@ -549,6 +707,20 @@ __global__ void __launch_bounds__(NT, 8) k_bicg_v(DevGrid g, Op op, BiVec<V> b)
     const V* __restrict__ r = b.r;
     const V* __restrict__ r0 = b.r0;
     V* __restrict__ vo = b.v;
+    auto epilogue = [&](long c, V v) {
+        vo[c] = v;
+        const double vd = v, rc = r[c];
+        vals[0] += vd * double(r0[c]);
+        vals[1] += vd * vd;
+        vals[2] += rc * rc;
+    };
+    if(g.jdir == 1)
+    {
+        plane_walk<V>(g, op, [&](long cc, V&) { return ph[cc]; },
+                      [&](long c, int, int, int, V Ax, V) { epilogue(c, Ax); });
+        block_partials_w<3>(vals, b.partials);
+        return;
+    }
     const TileWalk tw = tile_walk(g);
     for(int t = tw.t; t < tw.t1; ++t)
     {
@@ -556,14 +728,9 @@ __global__ void __launch_bounds__(NT, 8) k_bicg_v(DevGrid g, Op op, BiVec<V> b)
         long c;
         if(!tile_cell(g, tw, t, i, j, k, c)) continue;
         const CouplingV<V> q = coupling_of(g, op, c, i, j, k);
-        const V v = apply_A(g, q, ph, c, i, j, k);
-        vo[c] = v;
-        const double vd = v, rc = r[c];
-        vals[0] += vd * double(r0[c]);
-        vals[1] += vd * vd;
-        vals[2] += rc * rc;
+        epilogue(c, apply_A(g, q, ph, c, i, j, k));
     }
-    block_partials<3>(vals, b.partials);
+    block_partials_w<3>(vals, b.partials);
 }
 
 __global__ void k_fin_v(const double* __restrict__ partials, BiState* st, int nb)
@@ -602,6 +769,24 @@ __global__ void __launch_bounds__(NT, 8) k_bicg_t(DevGrid g, Op op, BiVec<V> b)
     const V* __restrict__ v = b.v;
     const V* __restrict__ a = b.aii;
     V* __restrict__ to = b.t;
+    auto epilogue = [&](long c, V t, V s) {
+        to[c] = t;
+        const double td = t, sd = s;
+        vals[0] += td * sd;
+        vals[1] += td * td;
+        vals[2] += sd * sd;
+    };
+    // sh at an element (0 outside the domain is applied by the stencil)
+    auto shv = [&](long cc, V& sv) {
+        sv = r[cc] - alpha * v[cc];
+        return sv * a[cc];
+    };
+    if(g.jdir == 1)
+    {
+        plane_walk<V>(g, op, shv, [&](long c, int, int, int, V Ax, V s) { epilogue(c, Ax, s); });
+        block_partials_w<3>(vals, b.partials);
+        return;
+    }
     const TileWalk tw = tile_walk(g);
     for(int tt = tw.t; tt < tw.t1; ++tt)
     {
@@ -609,38 +794,13 @@ __global__ void __launch_bounds__(NT, 8) k_bicg_t(DevGrid g, Op op, BiVec<V> b)
         long c;
         if(!tile_cell(g, tw, tt, i, j, k, c)) continue;
         const CouplingV<V> q = coupling_of(g, op, c, i, j, k);
-        // sh at the centre and the neighbours (0 outside the domain)
-        auto shv = [&](long cc) { return (r[cc] - alpha * v[cc]) * a[cc]; };
-        const V Mp = diag_of(g, q);
-        const V Mn = i == g.nx - 1 ? V(0) : -q.T[0];
-        const V Ms = i == 0 ? V(0) : -q.T[1];
-        const V Mt = k == g.nz - 1 ? V(0) : -q.T[4];
-        const V Mb = k == 0 ? V(0) : -q.T[5];
-        const V xn = i == g.nx - 1 ? V(0) : shv(c + 1);
-        const V xs = i == 0 ? V(0) : shv(c - 1);
-        const V xt = k == g.nz - 1 ? V(0) : shv(c + g.sxy);
-        const V xb = k == 0 ? V(0) : shv(c - g.sxy);
-        const V s = r[c] - alpha * v[c];
-        const V shc = s * a[c];
-        V t;
-        if(g.jdir == 1)
-        {
-            const V yd = (V)g.ydir;
-            const V Mw = j == g.ny - 1 ? V(0) : -q.T[2] * yd;
-            const V Me = j == 0 ? V(0) : -q.T[3] * yd;
-            const V xw = j == g.ny - 1 ? V(0) : shv(c + g.sx);
-            const V xe = j == 0 ? V(0) : shv(c - g.sx);
-            t = Mp * shc + Mn * xn + Ms * xs + Mw * xw + Me * xe + Mt * xt + Mb * xb;
-        }
-        else
-            t = Mp * shc + Mn * xn + Ms * xs + Mt * xt + Mb * xb;
-        to[c] = t;
-        const double td = t, sd = s;
-        vals[0] += td * sd;
-        vals[1] += td * td;
-        vals[2] += sd * sd;
+        V s;
+        const V shc = shv(c, s);
+        V dummy;
+        const V t = apply_A_fn(g, q, [&](long cc) { return cc == c ? shc : shv(cc, dummy); }, c, i, j, k);
+        epilogue(c, t, s);
     }
-    block_partials<3>(vals, b.partials);
+    block_partials_w<3>(vals, b.partials);
 }
 
 __global__ void k_fin_t(const double* __restrict__ partials, BiState* st, int nb)

@@ -132,6 +132,14 @@ def test_bicgstab_first_iterations_match_reference_path():
         np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10 * np.abs(ref).max(), err_msg=f"it={it}")
 
 
+# Tolerance of fields downstream of a BiCGStab solve.  The fixtures stop on
+# ||r||_2 / N < 1e-14; changing only the summation order of the solver's dot products
+# moves u, w and phi after one step by up to 8e-10 relative (measured with the oracle
+# accumulating its dot products in extended precision, tank case), so the device, whose
+# reductions are ordered differently, is held to 1e-8.
+SOLVED_RTOL = 1e-8
+
+
 @pytest.mark.parametrize("case", ["tank", "tank2d", "tankfsf"])
 def test_fc3_step_matches_reference(case):
     d = load(case)
@@ -145,7 +153,7 @@ def test_fc3_step_matches_reference(case):
             got, ref = inner(D.get(n)), inner(d[f"tk_{n}_{s}"])
             scale = np.abs(ref).max() or 1.0
             err = np.abs(got - ref).max() / scale
-            assert err < 1e-10, f"step {s} {n}: rel err {err:.2e}"
+            assert err < SOLVED_RTOL, f"step {s} {n}: rel err {err:.2e}"
         np.testing.assert_allclose(vol, d[f"tk_vol_{s}"][:2], rtol=1e-12)
 
 
@@ -237,4 +245,4 @@ def test_fc3_step_vs_oracle_multi_tile():
     for n, ref in (("phi", phi), ("u", u), ("v", v), ("w", w), ("press", press)):
         got = D.get(n)
         err = np.abs(inner(got) - inner(ref)).max() / (np.abs(inner(ref)).max() or 1.0)
-        assert err < 1e-9, f"{n}: rel err {err:.2e}"
+        assert err < SOLVED_RTOL, f"{n}: rel err {err:.2e}"
