@@ -6,11 +6,10 @@
 // waits on the host.  Partial sums are formed in a fixed order, so results are
 // run-to-run deterministic (they differ from the reference's serial i-j-k sum only by
 // rounding).
-#include "reef_internal.h"
+#include "cell_ops.h"
 
 namespace reef {
 
-constexpr int TX = 64, TY = 4, NT = 256;
 
 __device__ __forceinline__ bool tile_of_p(int nbx, int nby, int nbtot, int& bx, int& by, int& bz)
 {
@@ -44,76 +43,6 @@ __device__ __forceinline__ bool cell(const DevGrid& g, int nbx, int nby, int nbt
     if(i >= g.nx || j >= g.ny) return false;
     c = g.base + i + g.sx * j + g.sxy * k;
     return true;
-}
-
-// Block partial sums of M values in a fixed tree order, written to
-// partials[blockIdx.x*M + m].  The totals are formed by a one-block finalize kernel
-// launched right after (sum_partials): a kernel boundary is a far cheaper way to make
-// the partials visible than a device-scope fence per block (each such fence writes the
-// L2 back on this GPU).
-template<int M>
-__device__ __forceinline__ void block_partials(const double (&v)[M], double* __restrict__ partials)
-{
-    __shared__ double sh[M][NT];
-    const int t = threadIdx.x;
-    for(int m = 0; m < M; ++m) sh[m][t] = v[m];
-    __syncthreads();
-    for(int s = NT / 2; s > 0; s >>= 1)
-    {
-        if(t < s)
-            for(int m = 0; m < M; ++m) sh[m][t] += sh[m][t + s];
-        __syncthreads();
-    }
-    if(t == 0)
-        for(int m = 0; m < M; ++m) partials[(size_t)blockIdx.x * M + m] = sh[m][0];
-}
-
-// The same partial sums with a wavefront butterfly (fixed order, lane 0's sum) and a
-// 4-entry LDS combine: needs M * 4 doubles of LDS instead of M * NT, so kernels that keep
-// stencil planes in LDS still fit eight blocks per CU.
-template<int M>
-__device__ __forceinline__ void block_partials_w(double (&v)[M], double* __restrict__ partials)
-{
-    __shared__ double ws[M][NT / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for(int m = 0; m < M; ++m)
-        for(int off = 32; off > 0; off >>= 1) v[m] += __shfl_xor(v[m], off);
-    if(lane == 0)
-        for(int m = 0; m < M; ++m) ws[m][wave] = v[m];
-    __syncthreads();
-    if(threadIdx.x == 0)
-        for(int m = 0; m < M; ++m)
-            partials[(size_t)blockIdx.x * M + m] = ((ws[m][0] + ws[m][1]) + ws[m][2]) + ws[m][3];
-}
-
-// totals of nb block partials, summed in block order by one block of NT threads
-template<int M>
-__device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int nb, double (&tot)[M])
-{
-    __shared__ double sh[M][NT];
-    const int t = threadIdx.x;
-    double acc[M];
-    for(int m = 0; m < M; ++m) acc[m] = 0.0;
-    int b = t;
-    for(; b + 3 * NT < nb; b += 4 * NT)
-    {
-        double q[4][M];
-        for(int u = 0; u < 4; ++u)
-            for(int m = 0; m < M; ++m) q[u][m] = partials[(size_t)(b + u * NT) * M + m];
-        for(int u = 0; u < 4; ++u)
-            for(int m = 0; m < M; ++m) acc[m] += q[u][m];
-    }
-    for(; b < nb; b += NT)
-        for(int m = 0; m < M; ++m) acc[m] += partials[(size_t)b * M + m];
-    for(int m = 0; m < M; ++m) sh[m][t] = acc[m];
-    __syncthreads();
-    for(int s = NT / 2; s > 0; s >>= 1)
-    {
-        if(t < s)
-            for(int m = 0; m < M; ++m) sh[m][t] += sh[m][t + s];
-        __syncthreads();
-    }
-    for(int m = 0; m < M; ++m) tot[m] = sh[m][0];
 }
 
 // ------------------------------------------------------------------ ghost program
@@ -179,12 +108,7 @@ __global__ void __launch_bounds__(NT) k_heaviside(DevGrid g, int nbx, int nby, i
     double vals[2] = {0.0, 0.0};
     if(cell(g, nbx, nby, nbtot, i, j, k, c))
     {
-        const double p = phi[c];
-        const double psi = g.psi;
-        double H = 0.0;
-        if(p > psi) H = 1.0;
-        if(p < -psi) H = 0.0;
-        if(fabs(p) <= psi) H = 0.5 * (1.0 + p / psi + (1.0 / PI_REF) * sin((PI_REF * p) / psi));
+        const double H = heaviside_H(phi[c], g.psi);
         ro[c] = rw * H + ra * (1.0 - H);
         visc[c] = vw * H + va * (1.0 - H);
         const double vol = g.DXN[i + MARGE] * g.DYN[j + MARGE] * g.DZN[k + MARGE];
@@ -224,8 +148,7 @@ __global__ void k_poisson_rhs(DevGrid g, int nbx, int nby, int nbtot, const doub
     int i, j, k;
     long c;
     if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
-    rhs[c] = -(u[c] - u[c - 1]) / (adt * g.DXN[i + MARGE]) - (v[c] - v[c - g.sx]) / (adt * g.DYN[j + MARGE]) * g.ydir
-           - (w[c] - w[c - g.sxy]) / (adt * g.DZN[k + MARGE]);
+    rhs[c] = poisson_rhs_cell(g, u, v, w, adt, c, i, j, k);
 }
 
 void launch_poisson_rhs(const DevGrid& g, const double* u, const double* v, const double* w, double adt, double* rhs,
@@ -234,171 +157,6 @@ void launch_poisson_rhs(const DevGrid& g, const double* u, const double* v, cons
     int nbx, nby, nbtot;
     dim3 grid = tiles(g, nbx, nby, nbtot);
     hipLaunchKernelGGL(k_poisson_rhs, grid, dim3(NT), 0, s, g, nbx, nby, nbtot, u, v, w, adt, rhs);
-}
-
-__device__ __forceinline__ double roface(double a, double b, double psi, double rw, double ra)
-{
-    const double phival = 0.5 * (a + b);
-    double H = 0.0;
-    if(phival > psi) H = 1.0;
-    if(phival < -psi) H = 0.0;
-    if(fabs(phival) <= psi) H = 0.5 * (1.0 + phival / psi + (1.0 / PI_REF) * sin((PI_REF * phival) / psi));
-    return rw * H + ra * (1.0 - H);
-}
-
-// Couplings of poisson_pcorr.cpp:95-123 before the boundary folding: T[0..5] for the
-// +x, -x, +y, -y, +z, -z faces; face densities density_f::roface stored per "+" face.
-template<class V>
-struct CouplingV
-{
-    V T[6];
-};
-using Coupling = CouplingV<double>;
-
-__device__ __forceinline__ Coupling couplings(const DevGrid& g, const double* __restrict__ rfx,
-                                              const double* __restrict__ rfy, const double* __restrict__ rfz, long c,
-                                              int i, int j, int k)
-{
-    const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
-    Coupling q;
-    q.T[0] = (1.0 * 1.0) / (rfx[c] * g.DXP[ip] * g.DXN[ip]);
-    q.T[1] = (1.0 * 1.0) / (rfx[c - 1] * g.DXP[ip - 1] * g.DXN[ip]);
-    q.T[2] = (1.0 * 1.0) / (rfy[c] * g.DYP[jp] * g.DYN[jp]);
-    q.T[3] = (1.0 * 1.0) / (rfy[c - g.sx] * g.DYP[jp - 1] * g.DYN[jp]);
-    q.T[4] = (1.0 * 1.0) / (rfz[c] * g.DZP[kp] * g.DZN[kp]);
-    q.T[5] = (1.0 * 1.0) / (rfz[c - g.sxy] * g.DZP[kp - 1] * g.DZN[kp]);
-    return q;
-}
-
-// The FP64 operator recomputes its couplings from the face densities per application
-// (bit-identical to the reference's assembly).  The FP32 operator of the mixed-precision
-// inner solve reads face conductances 1/(rho_f * DP) rounded to FP32 and scales them by
-// the FP32 1/DN of the cell: T[0] = kx(c) / DXN(i), T[1] = kx(c-1) / DXN(i), ...
-__device__ __forceinline__ CouplingV<double> coupling_of(const DevGrid& g, const SolverOp64& op, long c, int i, int j,
-                                                         int k)
-{
-    return couplings(g, op.rfx, op.rfy, op.rfz, c, i, j, k);
-}
-
-__device__ __forceinline__ CouplingV<float> coupling_of(const DevGrid& g, const SolverOp32& op, long c, int i, int j,
-                                                        int k)
-{
-    const float* __restrict__ kx = op.kx;
-    const float* __restrict__ ky = op.ky;
-    const float* __restrict__ kz = op.kz;
-    const float ix = op.iDXN[i + MARGE], iy = op.iDYN[j + MARGE], iz = op.iDZN[k + MARGE];
-    CouplingV<float> q;
-    q.T[0] = kx[c] * ix;
-    q.T[1] = kx[c - 1] * ix;
-    q.T[2] = ky[c] * iy;
-    q.T[3] = ky[c - g.sx] * iy;
-    q.T[4] = kz[c] * iz;
-    q.T[5] = kz[c - g.sxy] * iz;
-    return q;
-}
-
-// couplings from the six face values (+x, -x, +y, -y, +z, -z) of cell (i, j, k): the face
-// densities for the FP64 operator, the FP32 conductances for the FP32 one
-__device__ __forceinline__ CouplingV<double> coupling_faces(const DevGrid& g, const SolverOp64&, double F0, double F1,
-                                                            double F2, double F3, double F4, double F5, int i, int j,
-                                                            int k)
-{
-    const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
-    CouplingV<double> q;
-    q.T[0] = (1.0 * 1.0) / (F0 * g.DXP[ip] * g.DXN[ip]);
-    q.T[1] = (1.0 * 1.0) / (F1 * g.DXP[ip - 1] * g.DXN[ip]);
-    q.T[2] = (1.0 * 1.0) / (F2 * g.DYP[jp] * g.DYN[jp]);
-    q.T[3] = (1.0 * 1.0) / (F3 * g.DYP[jp - 1] * g.DYN[jp]);
-    q.T[4] = (1.0 * 1.0) / (F4 * g.DZP[kp] * g.DZN[kp]);
-    q.T[5] = (1.0 * 1.0) / (F5 * g.DZP[kp - 1] * g.DZN[kp]);
-    return q;
-}
-
-__device__ __forceinline__ CouplingV<float> coupling_faces(const DevGrid&, const SolverOp32& op, float F0, float F1,
-                                                           float F2, float F3, float F4, float F5, int i, int j,
-                                                           int k)
-{
-    const float ix = op.iDXN[i + MARGE], iy = op.iDYN[j + MARGE], iz = op.iDZN[k + MARGE];
-    CouplingV<float> q;
-    q.T[0] = F0 * ix;
-    q.T[1] = F1 * ix;
-    q.T[2] = F2 * iy;
-    q.T[3] = F3 * iy;
-    q.T[4] = F4 * iz;
-    q.T[5] = F5 * iz;
-    return q;
-}
-
-__device__ __forceinline__ const double* face_x(const SolverOp64& op) { return op.rfx; }
-__device__ __forceinline__ const double* face_y(const SolverOp64& op) { return op.rfy; }
-__device__ __forceinline__ const double* face_z(const SolverOp64& op) { return op.rfz; }
-__device__ __forceinline__ const float* face_x(const SolverOp32& op) { return op.kx; }
-__device__ __forceinline__ const float* face_y(const SolverOp32& op) { return op.ky; }
-__device__ __forceinline__ const float* face_z(const SolverOp32& op) { return op.kz; }
-
-template<class V>
-__device__ __forceinline__ V diag_of(const DevGrid& g, const CouplingV<V>& q)
-{
-    const V yd = (V)g.ydir;
-    return q.T[0] + q.T[1] + q.T[2] * yd + q.T[3] * yd + q.T[4] + q.T[5];
-}
-
-// y = A x at cell c with the folded operator of poisson_pcorr.cpp:124-230: boundary
-// neighbours (pressure-correction ghosts are 0) drop out, the diagonal keeps them.
-// Summation order of matvec_std (bicgstab_ijk.cpp:296-310); 5 points in 2-D
-// (bicgstab_ijk_2D.cpp:308-320).  xv(cc) gives the value of x at element cc.
-// the same sum from the seven values (xn/xs: i+-1, xw/xe: j+-1, xt/xb: k+-1), values
-// outside the domain are ignored
-template<class V>
-__device__ __forceinline__ V stencil7(const DevGrid& g, const CouplingV<V>& q, V xc, V xn_, V xs_, V xw_, V xe_, V xt_,
-                                      V xb_, int i, int j, int k)
-{
-    const V Mp = diag_of(g, q);
-    const V Mn = i == g.nx - 1 ? V(0) : -q.T[0];
-    const V Ms = i == 0 ? V(0) : -q.T[1];
-    const V Mt = k == g.nz - 1 ? V(0) : -q.T[4];
-    const V Mb = k == 0 ? V(0) : -q.T[5];
-    const V xn = i == g.nx - 1 ? V(0) : xn_;
-    const V xs = i == 0 ? V(0) : xs_;
-    const V xt = k == g.nz - 1 ? V(0) : xt_;
-    const V xb = k == 0 ? V(0) : xb_;
-    const V yd = (V)g.ydir;
-    const V Mw = j == g.ny - 1 ? V(0) : -q.T[2] * yd;
-    const V Me = j == 0 ? V(0) : -q.T[3] * yd;
-    const V xw = j == g.ny - 1 ? V(0) : xw_;
-    const V xe = j == 0 ? V(0) : xe_;
-    return Mp * xc + Mn * xn + Ms * xs + Mw * xw + Me * xe + Mt * xt + Mb * xb;
-}
-
-template<class V, class XV>
-__device__ __forceinline__ V apply_A_fn(const DevGrid& g, const CouplingV<V>& q, XV xv, long c, int i, int j, int k)
-{
-    const V Mp = diag_of(g, q);
-    const V Mn = i == g.nx - 1 ? V(0) : -q.T[0];
-    const V Ms = i == 0 ? V(0) : -q.T[1];
-    const V Mt = k == g.nz - 1 ? V(0) : -q.T[4];
-    const V Mb = k == 0 ? V(0) : -q.T[5];
-    const V xn = i == g.nx - 1 ? V(0) : xv(c + 1);
-    const V xs = i == 0 ? V(0) : xv(c - 1);
-    const V xt = k == g.nz - 1 ? V(0) : xv(c + g.sxy);
-    const V xb = k == 0 ? V(0) : xv(c - g.sxy);
-    if(g.jdir == 1)
-    {
-        const V yd = (V)g.ydir;
-        const V Mw = j == g.ny - 1 ? V(0) : -q.T[2] * yd;
-        const V Me = j == 0 ? V(0) : -q.T[3] * yd;
-        const V xw = j == g.ny - 1 ? V(0) : xv(c + g.sx);
-        const V xe = j == 0 ? V(0) : xv(c - g.sx);
-        return Mp * xv(c) + Mn * xn + Ms * xs + Mw * xw + Me * xe + Mt * xt + Mb * xb;
-    }
-    return Mp * xv(c) + Mn * xn + Ms * xs + Mt * xt + Mb * xb;
-}
-
-template<class V>
-__device__ __forceinline__ V apply_A(const DevGrid& g, const CouplingV<V>& q, const V* __restrict__ x, long c, int i,
-                                     int j, int k)
-{
-    return apply_A_fn(g, q, [&](long cc) { return x[cc]; }, c, i, j, k);
 }
 
 // face densities of every "+" face of the interior cells and of the -1 layer faces, plus
@@ -444,15 +202,7 @@ __global__ void k_correct(DevGrid g, int nbx, int nby, int nbtot, const double* 
     int i, j, k;
     long c;
     if(!cell(g, nbx, nby, nbtot, i, j, k, c)) return;
-    press[c] += pcorr[c];
-    if(i < g.nx - 1)
-        u[c] -= adt * 1.0 * 1.0 * ((pcorr[c + 1] - pcorr[c]) / (g.DXP[i + MARGE] * roface(phi[c], phi[c + 1], g.psi, rw, ra)));
-    if(g.jdir == 1 && j < g.ny - 1)
-        v[c] -= adt * 1.0 * 1.0
-              * ((pcorr[c + g.sx] - pcorr[c]) / (g.DYP[j + MARGE] * roface(phi[c], phi[c + g.sx], g.psi, rw, ra)));
-    if(k < g.nz - 1)
-        w[c] -= adt * 1.0 * 1.0
-              * ((pcorr[c + g.sxy] - pcorr[c]) / (g.DZP[k + MARGE] * roface(phi[c], phi[c + g.sxy], g.psi, rw, ra)));
+    correct_cell(g, pcorr, phi, rw, ra, adt, u, v, w, press, c, i, j, k);
 }
 
 void launch_correct(const DevGrid& g, const double* pcorr, const double* phi, double rw, double ra, double adt,
