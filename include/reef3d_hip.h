@@ -157,6 +157,39 @@ int reef_phi_rk3(reef_ctx* ctx);
 
 int reef_sync(reef_ctx* ctx);
 
+/* ---- Batches: many independent tanks advanced by the same launches ------------------
+ * No counterpart in the reference, which runs one domain per process; this is the
+ * serving form of reef_fc3_step for many small tanks.  Every tank keeps its own fields,
+ * grid, time step (reef_params.dt) and solver state; the other reef_params values, the
+ * F49 control and the solver configuration are shared by the batch.  Each tank's work
+ * is cut into chunks that depend on the tank alone, and every reduction of a tank sums
+ * that tank's chunks in a fixed order, so a tank's results are the same alone and in
+ * any batch.  Tank ids are slots: a removed tank's slot is reused by a later add. */
+typedef struct reef_batch reef_batch;
+/* hbm_budget_bytes: device bytes the batch's tanks may hold (0 = no limit);
+ * reef_batch_add fails with REEF_E_NOMEM beyond it */
+int reef_batch_create(double hbm_budget_bytes, reef_batch** out);
+void reef_batch_destroy(reef_batch* b);
+/* device bytes one tank of this grid needs (fields, solver arrays, metadata) */
+long reef_batch_tank_bytes(const reef_grid_desc* grid);
+int reef_batch_add(reef_batch* b, const reef_grid_desc* grid, const reef_params* prm, int* tank);
+int reef_batch_remove(reef_batch* b, int tank);
+/* tanks the next operations advance (n = 0: every tank) */
+int reef_batch_set_active(reef_batch* b, const int* tanks, int n);
+int reef_batch_info(reef_batch* b, int* slots, int* active, double* bytes_in_use);
+int reef_batch_put(reef_batch* b, int tank, int field_id, const double* host, int layout);
+int reef_batch_get(reef_batch* b, int tank, int field_id, double* host, int layout);
+int reef_batch_ghost_fill(reef_batch* b, int field_id, int kind, int gcv);
+int reef_batch_fluid_update(reef_batch* b);
+int reef_batch_control(reef_batch* b, const char* key, double value);
+int reef_batch_solver_config(reef_batch* b, int precision, double delta);
+/* one momentum_FC3 step of every active tank; iters[slot] (may be NULL) receives the
+ * summed BiCGStab iterations, vol[2*slot..] (may be NULL) volume1, volume2 */
+int reef_batch_fc3_step(reef_batch* b, long* iters, double* vol);
+int reef_batch_sync(reef_batch* b);
+int reef_batch_timing_enable(reef_batch* b, int on);
+int reef_batch_timing_get(reef_batch* b, const char* name, double* ms, long* launches);
+
 /* Kernel timing: the context records HIP events around its launches when enabled. */
 int reef_timing_enable(reef_ctx* ctx, int on);
 /* name: kernel family ("bicgstab", "hj", "reini", "flux", ...); returns total ms and launches */

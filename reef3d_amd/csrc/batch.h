@@ -1,0 +1,89 @@
+// Batched tanks: many independent box domains ("tanks") advanced by the same kernel
+// launches.  Each tank keeps its own fields, metadata, work decomposition, time step and
+// solver state in device memory (TankDev); a launch walks the work items of every tank of
+// the batch (BatchView).  A single domain is a batch of one.
+#pragma once
+
+#include "reef_internal.h"
+#include "tiling.h"
+
+namespace reef {
+
+// FP64 arrays of a tank: the public REEF_* fields (0..15) and the step / solver work arrays
+enum Arr64
+{
+    A_LS = REEF_NFIELDS, A_FRK1, A_FRK2, A_RK1, A_RK2, A_URK1, A_VRK1, A_WRK1, A_URK2, A_VRK2, A_WRK2, A_RHS,
+    A_R, A_R0, A_P, A_PH, A_V, A_T, A_AII, A_RFX, A_RFY, A_RFZ,
+    NARR64
+};
+// FP32 arrays of the mixed-precision solve
+enum Arr32
+{
+    S_R, S_R0, S_P, S_PH, S_V, S_T, S_X, S_KX, S_KY, S_KZ, S_AII,
+    NARR32
+};
+
+// ghost programs a step uses: (kind, gcv)
+constexpr int NPROG = 13;
+constexpr int PROG_KIND_GCV[NPROG][2] = {{4, 52}, {4, 1}, {4, 40}, {1, 10}, {2, 11}, {3, 12}, {1, 7},
+                                         {2, 8},  {3, 9}, {2, 12}, {4, 50}, {1, 14}, {3, 16}};
+
+struct TankDev
+{
+    DevGrid g;                     // geometry and metadata (base relative to each array)
+    double* f[NARR64];
+    float* s[NARR32];
+    const float *iDXN32, *iDYN32, *iDZN32;   // FP32 1/DN of the mixed solve (MARGE-offset)
+    TileShape shape;               // tiles, chunk length, items
+    int item0;                     // first work item of the tank in the batch
+    int active;                    // 0: skipped by every launch
+    double dt;                     // time step of the tank
+    const GhostOp* gops[NPROG];    // compiled ghost programs
+    int gnops[NPROG];
+    BiState* st;                   // solver state (FP64 or mixed)
+    double* vol;                   // volume1, volume2 of the last fluid update
+};
+
+struct BatchView
+{
+    const TankDev* tanks;          // [ntank]
+    const int* item_tank;          // [nitems]
+    int ntank, nitems;
+    double* partials;              // [nitems * 4]
+    int has_pw, has_direct;        // tanks walked by the 3-D plane walk / by the direct stencil
+};
+
+// ---- launchers (kernels_batch.hip); field arguments are Arr64 ids; every launch covers
+// the active tanks of the batch.  grid: blocks of the persistent launch (multiple of 8).
+void bl_copy(const BatchView& b, int grid, int src, int dst, hipStream_t s);
+void bl_fill(const BatchView& b, int grid, int id, double v, hipStream_t s);
+void bl_ghost(const BatchView& b, int grid, int prog, int id, hipStream_t s);
+void bl_hj_stage(const BatchView& b, int grid, int stage, int phi_s, int phi_n, int u, int v, int w, int out,
+                 double cfac, hipStream_t s);
+void bl_reini_stage(const BatchView& b, int grid, int stage, int fs, int f0, int out, double F43, int fsf,
+                    hipStream_t s);
+void bl_flux_acc(const BatchView& b, int grid, int ipol, int q, int u, int v, int w, int acc, hipStream_t s);
+void bl_momentum(const BatchView& b, int grid, int comp, int stage, int qin, int qold, int u, int v, int w,
+                 double grav, double cfac, int out, double rw, double ra, hipStream_t s);
+void bl_heaviside(const BatchView& b, int grid, const double* prm4, hipStream_t s);
+void bl_poisson_rhs(const BatchView& b, int grid, int u, int v, int w, double afac, hipStream_t s);
+void bl_face_rho(const BatchView& b, int grid, double rw, double ra, int with_aii, hipStream_t s);
+void bl_correct(const BatchView& b, int grid, double rw, double ra, double afac, int u, int v, int w,
+                hipStream_t s);
+// BiCGStab on every active tank: x, rhs array ids; stop / maxiter per call
+void bl_bicg_init(const BatchView& b, int grid, int x, int rhs, double stop, int maxiter, hipStream_t s);
+void bl_bicg_phase(const BatchView& b, int grid, int x, int rhs, int phase, int part, hipStream_t s);
+// mixed precision (FP32 iteration, FP64 reliable updates)
+void bl_coef32(const BatchView& b, int grid, hipStream_t s);
+void bl_mixed_init(const BatchView& b, int grid, int x, int rhs, double stop, int maxiter, double delta,
+                   hipStream_t s);
+void bl_mixed_phase(const BatchView& b, int grid, int x, int rhs, int phase, int part, hipStream_t s);
+void bl_add32(const BatchView& b, int grid, int x, hipStream_t s);
+// level-set benchmark: prescribed vortex velocity (benchmark_vortex3D) and the void
+// momentum_RK3 update of one component
+void bl_vortex_velocity(const BatchView& b, int grid, int comp, int id, double simtime, double DXM, hipStream_t s);
+void bl_void_rk3(const BatchView& b, int grid, int comp, int id, double Fv, hipStream_t s);
+// ||rhs - A x||_2 per tank into norms[tank]
+void bl_residual(const BatchView& b, int grid, int x, int rhs, double* norms, hipStream_t s);
+
+}  // namespace reef

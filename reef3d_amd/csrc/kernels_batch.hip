@@ -1,0 +1,1482 @@
+// Kernels of the batched engine.  Every launch is a fixed grid of blocks (a multiple of 8)
+// that walks the work items of a batch: block b belongs to XCD group b % 8 and takes the
+// items of that group's eighth of the item range in turn, so neighbouring chunks of a
+// tank share an L2.  An item is a run of consecutive tiles (z fastest) of one tank; the
+// tank's grid, fields, time step and solver state come from its TankDev.  Reductions
+// produce one partial per item and a per-tank finalize block adds a tank's partials in
+// item order, so a tank's arithmetic does not depend on the other tanks of the batch.
+//
+// The per-cell arithmetic is cell_ops.h's (the reference's operation order), so the
+// stencil results are bit-identical to the reference whatever the batch.
+#include "batch.h"
+#include "cell_ops.h"
+
+namespace reef {
+namespace bk {
+
+// ------------------------------------------------------------------ work walk
+struct ItemIter
+{
+    int it, hi, step;
+};
+
+__device__ __forceinline__ ItemIter block_items(const BatchView& b)
+{
+    int lo, hi;
+    group_range(b.nitems, (int)(blockIdx.x & 7), lo, hi);
+    ItemIter q;
+    q.it = lo + (int)(blockIdx.x >> 3);
+    q.hi = hi;
+    q.step = (int)(gridDim.x >> 3);
+    return q;
+}
+
+// The tank table is never written by a kernel: reading it through the constant address
+// space lets the compiler use scalar loads (the tank's pointers and sizes stay in SGPRs).
+// Kernels take the tank by reference there and copy only its DevGrid: indexing a local
+// copy of the pointer table with a run-time array id would keep the whole table in
+// registers.
+// (the host compilation pass of this file never runs device code and has no such
+// address space)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(4))) const TankDev CTank;
+typedef __attribute__((address_space(4))) const int CInt;
+#else
+typedef const TankDev CTank;
+typedef const int CInt;
+#endif
+
+__device__ __forceinline__ CTank& tank_at(const BatchView& b, int tk) { return ((CTank*)b.tanks)[tk]; }
+__device__ __forceinline__ CTank& item_tank_at(const BatchView& b, int it)
+{
+    return ((CTank*)b.tanks)[((CInt*)b.item_tank)[it]];
+}
+
+struct TileWalk
+{
+    int t, t1;
+    TileShape s;
+};
+
+__device__ __forceinline__ TileWalk item_walk(CTank& T, int it)
+{
+    TileWalk w;
+    w.s = T.shape;
+    w.t = (it - T.item0) * T.shape.cpt;
+    w.t1 = min(w.t + T.shape.cpt, T.shape.ntiles);
+    return w;
+}
+
+__device__ __forceinline__ bool tile_cell(const DevGrid& g, const TileWalk& w, int t, int& i, int& j, int& k, long& c)
+{
+    if(!tile_cell_of(w.s, g.nx, g.ny, g.nz, t, (int)threadIdx.x, i, j, k)) return false;
+    c = g.base + i + g.sx * j + g.sxy * k;
+    return true;
+}
+
+// block sum of M values (wavefront butterfly, lane 0's sum, 4-wave combine) into dst[0..M)
+template<int M>
+__device__ __forceinline__ void block_sum_to(double (&v)[M], double* __restrict__ dst)
+{
+    __shared__ double ws[M][NT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for(int m = 0; m < M; ++m)
+        for(int off = 32; off > 0; off >>= 1) v[m] += __shfl_xor(v[m], off);
+    if(lane == 0)
+        for(int m = 0; m < M; ++m) ws[m][wave] = v[m];
+    __syncthreads();
+    if(threadIdx.x == 0)
+        for(int m = 0; m < M; ++m) dst[m] = ((ws[m][0] + ws[m][1]) + ws[m][2]) + ws[m][3];
+    __syncthreads();
+}
+
+// a tank's partials: items [item0, item0 + nitems), M values each
+template<int M>
+__device__ __forceinline__ void tank_totals(const BatchView& b, CTank& T, double (&tot)[M])
+{
+    sum_partials<M>(b.partials + (size_t)T.item0 * M, T.shape.nitems, tot);
+}
+
+// ------------------------------------------------------------------ per-precision views
+template<class V>
+struct Prec;
+
+template<>
+struct Prec<double>
+{
+    using Op = SolverOp64;
+    __device__ static SolverOp64 op(CTank& T)
+    {
+        SolverOp64 o;
+        o.rfx = T.f[A_RFX];
+        o.rfy = T.f[A_RFY];
+        o.rfz = T.f[A_RFZ];
+        return o;
+    }
+    __device__ static BiVec<double> vec(CTank& T, int x, int rhs)
+    {
+        BiVec<double> v;
+        v.x = T.f[x];
+        v.rhs = T.f[rhs];
+        v.r = T.f[A_R];
+        v.r0 = T.f[A_R0];
+        v.p = T.f[A_P];
+        v.ph = T.f[A_PH];
+        v.v = T.f[A_V];
+        v.t = T.f[A_T];
+        v.aii = T.f[A_AII];
+        v.partials = nullptr;
+        v.st = T.st;
+        return v;
+    }
+};
+
+template<>
+struct Prec<float>
+{
+    using Op = SolverOp32;
+    __device__ static SolverOp32 op(CTank& T)
+    {
+        SolverOp32 o;
+        o.kx = T.s[S_KX];
+        o.ky = T.s[S_KY];
+        o.kz = T.s[S_KZ];
+        o.iDXN = T.iDXN32;
+        o.iDYN = T.iDYN32;
+        o.iDZN = T.iDZN32;
+        return o;
+    }
+    __device__ static BiVec<float> vec(CTank& T, int, int)
+    {
+        BiVec<float> v;
+        v.x = T.s[S_X];
+        v.rhs = nullptr;
+        v.r = T.s[S_R];
+        v.r0 = T.s[S_R0];
+        v.p = T.s[S_P];
+        v.ph = T.s[S_PH];
+        v.v = T.s[S_V];
+        v.t = T.s[S_T];
+        v.aii = T.s[S_AII];
+        v.partials = nullptr;
+        v.st = T.st;
+        return v;
+    }
+};
+
+// ------------------------------------------------------------------ simple kernels
+__global__ void __launch_bounds__(NT) kb_copy(BatchView b, int src, int dst)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const double* __restrict__ s = T.f[src];
+        double* __restrict__ d = T.f[dst];
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(tile_cell(g, tw, t, i, j, k, c)) d[c] = s[c];
+        }
+    }
+}
+
+// every element of array id (ghosts and padding included) of every active tank
+__global__ void __launch_bounds__(NT) kb_fill(BatchView b, int id, double v)
+{
+    for(int tk = 0; tk < b.ntank; ++tk)
+    {
+        CTank& T = tank_at(b, tk);
+        if(!T.active) continue;
+        double* __restrict__ f = T.f[id];
+        const long n = T.g.nalloc;
+        for(long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) f[q] = v;
+    }
+}
+
+// compiled ghost program prog of every active tank on array id
+__global__ void __launch_bounds__(NT) kb_ghost(BatchView b, int prog, int id)
+{
+    for(int tk = 0; tk < b.ntank; ++tk)
+    {
+        CTank& T = tank_at(b, tk);
+        if(!T.active) continue;
+        const GhostOp* __restrict__ ops = T.gops[prog];
+        const int n = T.gnops[prog];
+        double* __restrict__ f = T.f[id];
+        for(int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+        {
+            const GhostOp op = ops[q];
+            double val;
+            if(op.src < 0)
+                val = 0.0;
+            else if(op.src & (1 << 30))
+                val = 0.0 + op.w * f[op.src & ~(1 << 30)];
+            else
+                val = f[op.src];
+            f[op.dst] = val;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ level set, reinit
+// HJ-WENO transport stage fused with the TVD-RK3 combination (levelset_RK3.cpp:118-157,
+// momentum_FC3.cpp:132-358); cdt = cfac * dt of the tank; stage 0 accumulates out += L
+__global__ void __launch_bounds__(NT, 4) kb_hj_stage(BatchView b, int stage, int phi_s_, int phi_n_, int u_, int v_,
+                                                   int w_, int out_, double cfac)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const double* __restrict__ phi_s = T.f[phi_s_];
+        const double* __restrict__ phi_n = phi_n_ >= 0 ? T.f[phi_n_] : nullptr;
+        const double* __restrict__ u = T.f[u_];
+        const double* __restrict__ v = T.f[v_];
+        const double* __restrict__ w = T.f[w_];
+        double* __restrict__ out = T.f[out_];
+        const double cdt = cfac * T.dt;
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            const double L = hj_L(g, phi_s, u, v, w, c, i, j, k);
+            const double La = 0.0 + L;
+            if(stage == 0)
+                out[c] += L;
+            else if(stage == 1)
+                out[c] = phi_s[c] + cdt * La;
+            else if(stage == 2)
+                out[c] = 0.75 * phi_n[c] + 0.25 * phi_s[c] + cdt * La;
+            else
+                out[c] = (1.0 / 3.0) * phi_n[c] + (2.0 / 3.0) * phi_s[c] + cdt * La;
+        }
+    }
+}
+
+// reinit sub-stage (reini_RK3.cpp:114-139) with the per-cell pseudo time step
+__global__ void __launch_bounds__(NT, 4) kb_reini_stage(BatchView b, int stage, int fs_, int f0_, int out_, double F43,
+                                                      int fsf)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const double* __restrict__ f = T.f[fs_];
+        const double* __restrict__ f0 = T.f[f0_];
+        double* __restrict__ out = T.f[out_];
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            const double lsv = f[c];
+            const double L = reini_L(g, f, c, i, j, k, fsf);
+            const double dtc = reini_dtc(g, i, j, k, F43);
+            if(stage == 1)
+                out[c] = f0[c] + dtc * L;
+            else if(stage == 2)
+                out[c] = 0.75 * f0[c] + 0.25 * lsv + 0.25 * dtc * L;
+            else
+                out[c] = (1.0 / 3.0) * f0[c] + (2.0 / 3.0) * lsv + (2.0 / 3.0) * dtc * L;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ momentum
+// WENO flux divergence accumulated into acc (convection::start, ipol 1..3)
+__global__ void __launch_bounds__(NT, 4) kb_flux_acc(BatchView b, int ipol, int q_, int u_, int v_, int w_, int acc_)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        if(ipol == 2 && g.jdir == 0) continue;
+        const double* __restrict__ qf = T.f[q_];
+        const double* __restrict__ u = T.f[u_];
+        const double* __restrict__ v = T.f[v_];
+        const double* __restrict__ w = T.f[w_];
+        double* __restrict__ acc = T.f[acc_];
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c) || !comp_cell(g, ipol, i, j, k)) continue;
+            acc[c] += flux_div(g, ipol, qf, u, v, w, c, i, j, k);
+        }
+    }
+}
+
+// velocity predictor of one component (momentum_FC3.cpp:160-208 and the later stages);
+// cdt = cfac * dt of the tank
+__global__ void __launch_bounds__(NT, 3) kb_momentum(BatchView b, int comp, int stage, int qin_, int qold_, int u_,
+                                                   int v_, int w_, double grav, double cfac, int out_, double rw,
+                                                   double ra)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        if(comp == 2 && g.jdir == 0) continue;
+        const double* __restrict__ qin = T.f[qin_];
+        const double* __restrict__ qold = T.f[qold_];
+        const double* __restrict__ u = T.f[u_];
+        const double* __restrict__ v = T.f[v_];
+        const double* __restrict__ w = T.f[w_];
+        const double* __restrict__ press = T.f[REEF_PRESS];
+        const double* __restrict__ phi = T.f[REEF_PHI];
+        double* __restrict__ out = T.f[out_];
+        const double cdt = cfac * T.dt;
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c) || !comp_cell(g, comp, i, j, k)) continue;
+            const double F = momentum_F(g, comp, qin, u, v, w, press, phi, grav, rw, ra, c, i, j, k);
+            const double df = qin[c];
+            if(stage == 1)
+                out[c] = df + cdt * 1.0 * F;
+            else if(stage == 2)
+                out[c] = 0.75 * qold[c] + 0.25 * df + cdt * 1.0 * F;
+            else
+                out[c] = (1.0 / 3.0) * qold[c] + (2.0 / 3.0) * df + cdt * 1.0 * F;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ fluid update
+// fluid_update_fsf::start (fluid_update_fsf.cpp:39-74): rho, nu and the volume partials
+__global__ void __launch_bounds__(NT) kb_heaviside(BatchView b, double rw, double vw, double ra, double va)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const double* __restrict__ phi = T.f[REEF_PHI];
+        double* __restrict__ ro = T.f[REEF_RO];
+        double* __restrict__ visc = T.f[REEF_VISC];
+        double vals[2] = {0.0, 0.0};
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            const double H = heaviside_H(phi[c], g.psi);
+            ro[c] = rw * H + ra * (1.0 - H);
+            visc[c] = vw * H + va * (1.0 - H);
+            const double vol = g.DXN[i + MARGE] * g.DYN[j + MARGE] * g.DZN[k + MARGE];
+            vals[0] += vol * (H - (1.0 - 1.0));
+            vals[1] += vol * (1.0 - H - (1.0 - 1.0));
+        }
+        block_sum_to<2>(vals, b.partials + (size_t)q.it * 2);
+    }
+}
+
+__global__ void __launch_bounds__(NT) kb_heaviside_fin(BatchView b)
+{
+    CTank& T = tank_at(b, blockIdx.x);
+    if(!T.active) return;
+    double tot[2];
+    tank_totals<2>(b, T, tot);
+    if(threadIdx.x == 0)
+    {
+        T.vol[0] = tot[0];
+        T.vol[1] = tot[1];
+    }
+}
+
+// ------------------------------------------------------------------ projection
+__global__ void __launch_bounds__(NT) kb_poisson_rhs(BatchView b, int u_, int v_, int w_, double afac)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const double* __restrict__ u = T.f[u_];
+        const double* __restrict__ v = T.f[v_];
+        const double* __restrict__ w = T.f[w_];
+        double* __restrict__ rhs = T.f[A_RHS];
+        const double adt = afac * T.dt;
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(tile_cell(g, tw, t, i, j, k, c)) rhs[c] = poisson_rhs_cell(g, u, v, w, adt, c, i, j, k);
+        }
+    }
+}
+
+// face densities of the "+" faces of the interior and of the -1 layer, optionally the
+// Jacobi diagonal aii = -1/(M.p + 1e-19) (needs all faces: second pass)
+__global__ void __launch_bounds__(NT) kb_face_rho(BatchView b, double rw, double ra)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const double* __restrict__ phi = T.f[REEF_PHI];
+        double* __restrict__ rfx = T.f[A_RFX];
+        double* __restrict__ rfy = T.f[A_RFY];
+        double* __restrict__ rfz = T.f[A_RFZ];
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            rfx[c] = roface(phi[c], phi[c + 1], g.psi, rw, ra);
+            rfy[c] = roface(phi[c], phi[c + g.sx], g.psi, rw, ra);
+            rfz[c] = roface(phi[c], phi[c + g.sxy], g.psi, rw, ra);
+            if(i == 0) rfx[c - 1] = roface(phi[c], phi[c - 1], g.psi, rw, ra);
+            if(j == 0) rfy[c - g.sx] = roface(phi[c], phi[c - g.sx], g.psi, rw, ra);
+            if(k == 0) rfz[c - g.sxy] = roface(phi[c], phi[c - g.sxy], g.psi, rw, ra);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(NT) kb_aii(BatchView b)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const double* __restrict__ rfx = T.f[A_RFX];
+        const double* __restrict__ rfy = T.f[A_RFY];
+        const double* __restrict__ rfz = T.f[A_RFZ];
+        double* __restrict__ aii = T.f[A_AII];
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            const Coupling cq = couplings(g, rfx, rfy, rfz, c, i, j, k);
+            aii[c] = -1.0 / (diag_of(g, cq) + 1e-19);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(NT) kb_correct(BatchView b, double rw, double ra, double afac, int u_, int v_,
+                                                  int w_)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const double adt = afac * T.dt;
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(tile_cell(g, tw, t, i, j, k, c))
+                correct_cell(g, T.f[REEF_PCORR], T.f[REEF_PHI], rw, ra, adt, T.f[u_], T.f[v_], T.f[w_],
+                             T.f[REEF_PRESS], c, i, j, k);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ BiCGStab
+// The state machine of bicgstab_ijk::solve (bicgstab_ijk.cpp:125-278) per tank: mode 1 = a
+// restart is pending, 2 = p and ph were just set from the residual, 3 = reliable update
+// pending (mixed precision).  Dot products are accumulated in FP64 for both precisions.
+
+// plane walk of the 3-D stencil kernels (tanks with 64 x 4 x 1 tiles): a block walks up
+// the k planes of its item's column; the stencil input of plane k (rows j-1 .. j+4,
+// columns i-1 .. i+64) and the +x / +y faces sit in LDS, planes k-1, k+1 of the own cell
+// and the -z face stay in registers, so every array crosses HBM about once per cell.
+// The LDS planes are double buffered: one barrier per plane.
+template<class V>
+struct PlaneLDS
+{
+    V x[TY + 2][TX + 2];
+    V fx[TY][TX + 1];
+    V fy[TY + 1][TX];
+};
+
+template<class V, class Op, class Load, class Cell>
+__device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const TileWalk& tw, Load load, Cell cell)
+{
+    __shared__ PlaneLDS<V> L[2];
+    const V* __restrict__ fxa = face_x(op);
+    const V* __restrict__ fya = face_y(op);
+    const V* __restrict__ fza = face_z(op);
+    const int lane = threadIdx.x & (TX - 1), row = threadIdx.x >> 6;
+    int curcol = -1, buf = 0;
+    V xd = V(0), xc = V(0), fzd = V(0), auxc = V(0);
+    #pragma unroll 1
+    for(int t = tw.t; t < tw.t1; ++t)
+    {
+        const int col = t / tw.s.nbz;
+        const int k = t - col * tw.s.nbz;
+        const int by = col / tw.s.nbx, bx = col - by * tw.s.nbx;
+        const int i = bx * TX + lane, j = by * TY + row;
+        const bool in = i < g.nx && j < g.ny;
+        const long c = g.base + i + g.sx * j + g.sxy * k;
+        if(col != curcol)
+        {
+            curcol = col;
+            V dummy;
+            xd = (in && k > 0) ? load(c - g.sxy, dummy) : V(0);
+            xc = in ? load(c, auxc) : V(0);
+            fzd = in ? fza[c - g.sxy] : V(0);
+        }
+        V auxu = V(0);
+        const V xu = (in && k + 1 < g.nz) ? load(c + g.sxy, auxu) : V(0);
+        PlaneLDS<V>& P = L[buf];
+        P.x[row + 1][lane + 1] = xc;
+        {
+            V dummy;
+            if(row == 0) P.x[0][lane + 1] = (i < g.nx && j > 0 && j - 1 < g.ny) ? load(c - g.sx, dummy) : V(0);
+            if(row == TY - 1) P.x[TY + 1][lane + 1] = (i < g.nx && j + 1 < g.ny) ? load(c + g.sx, dummy) : V(0);
+            if(lane == 0) P.x[row + 1][0] = (j < g.ny && i > 0) ? load(c - 1, dummy) : V(0);
+            if(lane == TX - 1) P.x[row + 1][TX + 1] = (j < g.ny && i + 1 < g.nx) ? load(c + 1, dummy) : V(0);
+        }
+        const V f0 = in ? fxa[c] : V(0);
+        const V f2 = in ? fya[c] : V(0);
+        const V f4 = in ? fza[c] : V(0);
+        P.fx[row][lane + 1] = f0;
+        if(lane == 0) P.fx[row][0] = (j < g.ny) ? fxa[c - 1] : V(0);
+        P.fy[row + 1][lane] = f2;
+        if(row == 0) P.fy[0][lane] = (i < g.nx && j < g.ny) ? fya[c - g.sx] : V(0);
+        __syncthreads();
+        if(in)
+        {
+            const CouplingV<V> cq = coupling_faces(g, op, f0, P.fx[row][lane], f2, P.fy[row][lane], f4, fzd, i, j, k);
+            const V Ax = stencil7(g, cq, xc, P.x[row + 1][lane + 2], P.x[row + 1][lane], P.x[row + 2][lane + 1],
+                                  P.x[row][lane + 1], xu, xd, i, j, k);
+            cell(c, Ax, auxc);
+        }
+        buf ^= 1;
+        xd = xc;
+        xc = xu;
+        auxc = auxu;
+        fzd = f4;
+    }
+}
+
+__device__ __forceinline__ bool plane_walk_ok(CTank& T)
+{
+    return T.g.jdir == 1 && T.shape.ty == TY && T.shape.tz == 1;
+}
+
+// restart / initial residual of the iterated precision: r = b - A x, r0 = p = r,
+// ph = aii p (restart label, bicgstab_ijk.cpp:136-148)
+template<class V, class Op>
+__device__ __forceinline__ void restart_body(const DevGrid& g, const Op& op, const BiVec<V>& v, const TileWalk& tw,
+                                             double& rr)
+{
+    #pragma unroll 1
+    for(int t = tw.t; t < tw.t1; ++t)
+    {
+        int i, j, k;
+        long c;
+        if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+        const CouplingV<V> cq = coupling_of(g, op, c, i, j, k);
+        const V rv = v.rhs[c] - (apply_A(g, cq, (const V*)v.x, c, i, j, k));
+        v.r[c] = rv;
+        v.r0[c] = rv;
+        v.p[c] = rv;
+        v.ph[c] = rv * v.aii[c];
+        rr += double(rv) * double(rv);
+    }
+}
+
+__global__ void kb_setup(BatchView b, double stop, int maxiter, int mixed, double delta)
+{
+    const int tk = blockIdx.x * blockDim.x + threadIdx.x;
+    if(tk >= b.ntank) return;
+    CTank& T = tank_at(b, tk);
+    if(!T.active) return;
+    BiState* st = T.st;
+    st->r_j = st->norm_r0 = st->alpha = st->w = st->beta = 0.0;
+    st->sigma = st->norm_vj = st->norm_rj = st->norm_sj = st->w1 = st->w2 = st->r_j1 = 0.0;
+    st->residual = 1.0e9;
+    st->stop = stop;
+    st->iter = 0;
+    st->maxiter = maxiter;
+    st->done = 0;
+    st->mode = 0;
+    st->branch = 0;
+    st->mixed = mixed;
+    st->r_j_prev = st->rmax = st->res0 = 0.0;
+    st->delta = delta;
+    st->pending = 0;
+    st->nru = 0;
+    st->ncell = (long)T.g.nx * T.g.ny * T.g.nz;
+}
+
+template<class V>
+__global__ void __launch_bounds__(NT, 8) kb_bicg_init(BatchView b, int x, int rhs)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        double vals[1] = {0.0};
+        restart_body(g, Prec<V>::op(T), Prec<V>::vec(T, x, rhs), item_walk(T, q.it), vals[0]);
+        block_sum_to<1>(vals, b.partials + (size_t)q.it);
+    }
+}
+
+__global__ void kb_fin_init(BatchView b)
+{
+    CTank& T = tank_at(b, blockIdx.x);
+    if(!T.active) return;
+    BiState* st = T.st;
+    double tot[1];
+    tank_totals<1>(b, T, tot);
+    if(threadIdx.x == 0)
+    {
+        st->r_j = tot[0];
+        st->norm_r0 = sqrt(tot[0]);
+        // a negative stop asks for ||r||_2 / N < |stop| * ||b - A x0||_2 / N
+        if(st->stop < 0.0) st->stop = -st->stop * sqrt(tot[0]) / double(st->ncell);
+        if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
+        st->mode = 2;
+    }
+}
+
+// start of an iteration: the pending restart, or p = r + beta (p - w v), ph = aii p
+// (bicgstab_ijk.cpp:234-235 and precon_solve)
+template<class V>
+__global__ void __launch_bounds__(NT, 8) kb_bicg_p(BatchView b, int x, int rhs)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const BiState* st = T.st;
+        if(st->done || st->mode == 2) continue;
+        const BiVec<V> v = Prec<V>::vec(T, x, rhs);
+        const TileWalk tw = item_walk(T, q.it);
+        if(st->mode == 1)
+        {
+            double vals[1] = {0.0};
+            restart_body(g, Prec<V>::op(T), v, tw, vals[0]);
+            block_sum_to<1>(vals, b.partials + (size_t)q.it);
+            continue;
+        }
+        const V beta = (V)st->beta, w = (V)st->w;
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            const V pn = v.r[c] + beta * (v.p[c] - w * v.v[c]);
+            v.p[c] = pn;
+            v.ph[c] = pn * v.aii[c];
+        }
+    }
+}
+
+// totals of a restart: ++solveriter, residual = |r|/N (bicgstab_ijk.cpp:183-191)
+__global__ void kb_fin_p(BatchView b)
+{
+    CTank& T = tank_at(b, blockIdx.x);
+    if(!T.active) return;
+    BiState* st = T.st;
+    if(st->done || st->mode != 1) return;
+    double tot[1];
+    tank_totals<1>(b, T, tot);
+    if(threadIdx.x == 0)
+    {
+        st->r_j = tot[0];
+        st->norm_r0 = sqrt(tot[0]);
+        st->residual = sqrt(tot[0]) / double(st->ncell);
+        st->iter += 1;
+        if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
+        st->mode = 2;
+    }
+}
+
+// v = A ph; sigma = (v, r0), |v|^2, |r|^2 (bicgstab_ijk.cpp:151-160)
+// PW: items of the tanks walked by the plane walk, else the others (direct stencil)
+template<class V, bool PW>
+__global__ void __launch_bounds__(NT, 8) kb_bicg_v(BatchView b, int x, int rhs)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const BiState* st = T.st;
+        if(st->done || st->mode == 1 || st->mode == 3 || plane_walk_ok(T) != PW) continue;
+        const BiVec<V> bv = Prec<V>::vec(T, x, rhs);
+        const typename Prec<V>::Op op = Prec<V>::op(T);
+        const TileWalk tw = item_walk(T, q.it);
+        double vals[3] = {0.0, 0.0, 0.0};
+        const V* __restrict__ ph = bv.ph;
+        const V* __restrict__ r = bv.r;
+        const V* __restrict__ r0 = bv.r0;
+        V* __restrict__ vo = bv.v;
+        auto epilogue = [&](long c, V v) {
+            vo[c] = v;
+            const double vd = v, rc = r[c];
+            vals[0] += vd * double(r0[c]);
+            vals[1] += vd * vd;
+            vals[2] += rc * rc;
+        };
+        if(plane_walk_ok(T))
+            plane_walk<V>(g, op, tw, [&](long cc, V&) { return ph[cc]; }, [&](long c, V Ax, V) { epilogue(c, Ax); });
+        else
+            #pragma unroll 1
+            for(int t = tw.t; t < tw.t1; ++t)
+            {
+                int i, j, k;
+                long c;
+                if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+                const CouplingV<V> cq = coupling_of(g, op, c, i, j, k);
+                epilogue(c, apply_A(g, cq, ph, c, i, j, k));
+            }
+        block_sum_to<3>(vals, b.partials + (size_t)q.it * 3);
+    }
+}
+
+// alpha and the restart tests (bicgstab_ijk.cpp:162-191)
+__global__ void kb_fin_v(BatchView b)
+{
+    CTank& T = tank_at(b, blockIdx.x);
+    if(!T.active) return;
+    BiState* st = T.st;
+    if(st->done || st->mode == 1 || st->mode == 3) return;
+    double tot[3];
+    tank_totals<3>(b, T, tot);
+    if(threadIdx.x == 0)
+    {
+        const double sigma = tot[0];
+        const double norm_vj = sqrt(tot[1]);
+        const double norm_rj = sqrt(tot[2]);
+        const double alpha = st->r_j / sigma;
+        st->sigma = sigma;
+        st->norm_vj = norm_vj;
+        st->norm_rj = norm_rj;
+        st->alpha = alpha;
+        if(fabs(sigma) <= (1.0e-12 * (norm_vj * st->norm_r0)))
+            st->mode = 1;
+        else if((fabs(alpha) * norm_vj / (norm_rj == 0 ? 1.0e-15 : norm_rj)) <= 0.08)
+            st->mode = 1;
+        else
+            st->mode = 0;
+    }
+}
+
+// s = r - alpha v, sh = aii s, t = A sh; (t,s), (t,t), (s,s) (bicgstab_ijk.cpp:193-221)
+template<class V, bool PW>
+__global__ void __launch_bounds__(NT, 8) kb_bicg_t(BatchView b, int x, int rhs)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const BiState* st = T.st;
+        if(st->done || st->mode == 1 || st->mode == 3 || plane_walk_ok(T) != PW) continue;
+        const BiVec<V> bv = Prec<V>::vec(T, x, rhs);
+        const typename Prec<V>::Op op = Prec<V>::op(T);
+        const TileWalk tw = item_walk(T, q.it);
+        double vals[3] = {0.0, 0.0, 0.0};
+        const V alpha = (V)st->alpha;
+        const V* __restrict__ r = bv.r;
+        const V* __restrict__ v = bv.v;
+        const V* __restrict__ a = bv.aii;
+        V* __restrict__ to = bv.t;
+        auto epilogue = [&](long c, V t, V s) {
+            to[c] = t;
+            const double td = t, sd = s;
+            vals[0] += td * sd;
+            vals[1] += td * td;
+            vals[2] += sd * sd;
+        };
+        auto shv = [&](long cc, V& sv) {
+            sv = r[cc] - alpha * v[cc];
+            return sv * a[cc];
+        };
+        if(plane_walk_ok(T))
+            plane_walk<V>(g, op, tw, shv, [&](long c, V Ax, V s) { epilogue(c, Ax, s); });
+        else
+            #pragma unroll 1
+            for(int tt = tw.t; tt < tw.t1; ++tt)
+            {
+                int i, j, k;
+                long c;
+                if(!tile_cell(g, tw, tt, i, j, k, c)) continue;
+                const CouplingV<V> cq = coupling_of(g, op, c, i, j, k);
+                V s;
+                const V shc = shv(c, s);
+                V dummy;
+                const V t = apply_A_fn(g, cq, [&](long cc) { return cc == c ? shc : shv(cc, dummy); }, c, i, j, k);
+                epilogue(c, t, s);
+            }
+        block_sum_to<3>(vals, b.partials + (size_t)q.it * 3);
+    }
+}
+
+__global__ void kb_fin_t(BatchView b)
+{
+    CTank& T = tank_at(b, blockIdx.x);
+    if(!T.active) return;
+    BiState* st = T.st;
+    if(st->done || st->mode == 1 || st->mode == 3) return;
+    double tot[3];
+    tank_totals<3>(b, T, tot);
+    if(threadIdx.x == 0)
+    {
+        st->w1 = tot[0];
+        st->w2 = tot[1];
+        st->norm_sj = sqrt(tot[2]);
+        st->branch = st->norm_sj > st->stop ? 1 : 0;
+        st->w = st->branch ? tot[0] / (tot[1] == 0 ? 1.0e-15 : tot[1]) : st->w;
+    }
+}
+
+// x and r updates, (r, r0), |r|^2 (bicgstab_ijk.cpp:223-268)
+template<class V>
+__global__ void __launch_bounds__(NT, 8) kb_bicg_x(BatchView b, int x, int rhs)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const BiState* st = T.st;
+        if(st->done || st->mode == 1 || st->mode == 3) continue;
+        const BiVec<V> bv = Prec<V>::vec(T, x, rhs);
+        const TileWalk tw = item_walk(T, q.it);
+        const V alpha = (V)st->alpha, w = (V)st->w;
+        const int branch = st->branch;
+        double vals[2] = {0.0, 0.0};
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            const V s = bv.r[c] - alpha * bv.v[c];
+            V rn;
+            if(branch)
+            {
+                const V sh = s * bv.aii[c];
+                bv.x[c] += alpha * bv.ph[c] + w * sh;
+                rn = s - w * bv.t[c];
+            }
+            else
+            {
+                bv.x[c] += alpha * bv.ph[c];
+                rn = s;
+            }
+            bv.r[c] = rn;
+            const double rd = rn;
+            vals[0] += rd * double(bv.r0[c]);
+            vals[1] += rd * rd;
+        }
+        block_sum_to<2>(vals, b.partials + (size_t)q.it * 2);
+    }
+}
+
+// beta, residual, iteration count; the mixed form asks for reliable updates instead of
+// ending on the iterated residual
+__global__ void kb_fin_x(BatchView b)
+{
+    CTank& T = tank_at(b, blockIdx.x);
+    if(!T.active) return;
+    BiState* st = T.st;
+    if(st->done || st->mode == 1 || st->mode == 3) return;
+    const double alpha = st->alpha, w = st->w;
+    const int branch = st->branch;
+    double tot[2];
+    tank_totals<2>(b, T, tot);
+    if(threadIdx.x == 0)
+    {
+        const double r_j1 = tot[0];
+        st->r_j_prev = st->r_j;
+        if(branch) st->beta = alpha * r_j1 / (w * st->r_j == 0 ? 1.0e-15 : (w * st->r_j));
+        st->r_j1 = r_j1;
+        st->r_j = r_j1;
+        st->residual = sqrt(tot[1]) / double(st->ncell);
+        st->iter += 1;
+        st->mode = 0;
+        if(!st->mixed)
+        {
+            if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
+            return;
+        }
+        const double rn = sqrt(tot[1]);
+        if(rn > st->rmax) st->rmax = rn;
+        if(!branch)
+            st->mode = 1;
+        else if(!(st->residual >= st->stop) || st->iter >= st->maxiter || rn < st->delta * st->rmax)
+            st->mode = 3;
+    }
+}
+
+// ------------------------------------------------------------------ mixed precision
+// FP32 face conductances kx = 1/(rho_f DXP) (+x faces of the interior and the -1 layer,
+// likewise y, z) and aii = -1/(M.p + 1e-19) from the FP64 diagonal, rounded once
+__global__ void __launch_bounds__(NT) kb_coef32(BatchView b)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const double* __restrict__ rfx = T.f[A_RFX];
+        const double* __restrict__ rfy = T.f[A_RFY];
+        const double* __restrict__ rfz = T.f[A_RFZ];
+        float* __restrict__ kx = T.s[S_KX];
+        float* __restrict__ ky = T.s[S_KY];
+        float* __restrict__ kz = T.s[S_KZ];
+        float* __restrict__ aii = T.s[S_AII];
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
+            kx[c] = (float)(1.0 / (rfx[c] * g.DXP[ip]));
+            ky[c] = (float)(1.0 / (rfy[c] * g.DYP[jp]));
+            kz[c] = (float)(1.0 / (rfz[c] * g.DZP[kp]));
+            if(i == 0) kx[c - 1] = (float)(1.0 / (rfx[c - 1] * g.DXP[ip - 1]));
+            if(j == 0) ky[c - g.sx] = (float)(1.0 / (rfy[c - g.sx] * g.DYP[jp - 1]));
+            if(k == 0) kz[c - g.sxy] = (float)(1.0 / (rfz[c - g.sxy] * g.DZP[kp - 1]));
+            const Coupling cq = couplings(g, rfx, rfy, rfz, c, i, j, k);
+            aii[c] = (float)(-1.0 / (diag_of(g, cq) + 1e-19));
+        }
+    }
+}
+
+// initial residual in FP64: r = b - A64 x64; r0 = p = r, ph = aii r, x32 = 0
+__global__ void __launch_bounds__(NT, 8) kb_mix_start(BatchView b, int x, int rhs)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const SolverOp64 op = Prec<double>::op(T);
+        const BiVec<float> v = Prec<float>::vec(T, x, rhs);
+        const double* __restrict__ x64 = T.f[x];
+        const double* __restrict__ b64 = T.f[rhs];
+        const TileWalk tw = item_walk(T, q.it);
+        double vals[2] = {0.0, 0.0};
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            const Coupling cq = coupling_of(g, op, c, i, j, k);
+            const double rv = b64[c] - apply_A(g, cq, x64, c, i, j, k);
+            const float r = (float)rv;
+            v.r[c] = r;
+            v.r0[c] = r;
+            v.p[c] = r;
+            v.ph[c] = r * v.aii[c];
+            v.x[c] = 0.0f;
+            vals[0] += rv * rv;
+            vals[1] += double(r) * double(r);
+        }
+        block_sum_to<2>(vals, b.partials + (size_t)q.it * 2);
+    }
+}
+
+__global__ void kb_fin_mix_start(BatchView b)
+{
+    CTank& T = tank_at(b, blockIdx.x);
+    if(!T.active) return;
+    BiState* st = T.st;
+    double tot[2];
+    tank_totals<2>(b, T, tot);
+    if(threadIdx.x == 0)
+    {
+        const double rn = sqrt(tot[0]);
+        st->r_j = tot[1];
+        st->norm_r0 = sqrt(tot[1]);
+        if(st->stop < 0.0) st->stop = -st->stop * rn / double(st->ncell);
+        st->residual = rn / double(st->ncell);
+        st->rmax = rn;
+        st->res0 = rn;
+        if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
+        st->mode = 2;
+    }
+}
+
+// start of an FP32 iteration: x64 += x32 after a reliable update, then the p update
+__global__ void __launch_bounds__(NT, 8) kb_mix_p(BatchView b, int x)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const BiState* st = T.st;
+        if(st->done || st->mode == 1 || st->mode == 3) continue;
+        const int upd = st->mode == 0, pend = st->pending;
+        if(!upd && !pend) continue;
+        const BiVec<float> v = Prec<float>::vec(T, x, x);
+        double* __restrict__ x64 = T.f[x];
+        const float beta = (float)st->beta, w = (float)st->w;
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            if(pend)
+            {
+                x64[c] += (double)v.x[c];
+                v.x[c] = 0.0f;
+            }
+            if(upd)
+            {
+                const float pn = v.r[c] + beta * (v.p[c] - w * v.v[c]);
+                v.p[c] = pn;
+                v.ph[c] = pn * v.aii[c];
+            }
+        }
+    }
+}
+
+__global__ void kb_fin_mix_p(BatchView b)
+{
+    const int tk = blockIdx.x * blockDim.x + threadIdx.x;
+    if(tk >= b.ntank) return;
+    CTank& T = tank_at(b, tk);
+    if(!T.active) return;
+    BiState* st = T.st;
+    if(st->done || st->mode == 1 || st->mode == 3) return;
+    st->pending = 0;
+}
+
+// reliable update (mode 3) or restart (mode 1): r = b - A64 (x64 + x32) in FP64, rounded to
+// the FP32 residual; a restart also sets r0 = p = r, ph = aii r
+__global__ void __launch_bounds__(NT, 8) kb_mix_ru(BatchView b, int x, int rhs)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const BiState* st = T.st;
+        if(st->done || (st->mode != 1 && st->mode != 3)) continue;
+        const int restart = st->mode == 1;
+        const DevGrid g = T.g;
+        const SolverOp64 op = Prec<double>::op(T);
+        const BiVec<float> v = Prec<float>::vec(T, x, rhs);
+        const double* __restrict__ x64 = T.f[x];
+        const double* __restrict__ b64 = T.f[rhs];
+        const float* __restrict__ x32 = v.x;
+        const TileWalk tw = item_walk(T, q.it);
+        double vals[2] = {0.0, 0.0};
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            const Coupling cq = coupling_of(g, op, c, i, j, k);
+            const double rv =
+                b64[c] - apply_A_fn(g, cq, [&](long cc) { return x64[cc] + (double)x32[cc]; }, c, i, j, k);
+            const float r = (float)rv;
+            v.r[c] = r;
+            float r0v;
+            if(restart)
+            {
+                v.r0[c] = r;
+                v.p[c] = r;
+                v.ph[c] = r * v.aii[c];
+                r0v = r;
+            }
+            else
+                r0v = v.r0[c];
+            vals[0] += rv * rv;
+            vals[1] += double(r0v) * double(r);
+        }
+        block_sum_to<2>(vals, b.partials + (size_t)q.it * 2);
+    }
+}
+
+__global__ void kb_fin_mix_ru(BatchView b)
+{
+    CTank& T = tank_at(b, blockIdx.x);
+    if(!T.active) return;
+    BiState* st = T.st;
+    if(st->done || (st->mode != 1 && st->mode != 3)) return;
+    double tot[2];
+    tank_totals<2>(b, T, tot);
+    if(threadIdx.x == 0)
+    {
+        const double rn = sqrt(tot[0]);
+        st->residual = rn / double(st->ncell);
+        st->nru += 1;
+        st->pending = 1;
+        st->rmax = rn;
+        if(st->mode == 1) st->iter += 1;     // a restart counts as an iteration (bicgstab_ijk.cpp:183-191)
+        if(!((st->residual >= st->stop) && (st->iter < st->maxiter)))
+        {
+            st->done = 1;
+            return;
+        }
+        if(st->mode == 1)
+        {
+            st->r_j = tot[1];
+            st->norm_r0 = sqrt(tot[1]);
+            st->mode = 2;
+        }
+        else
+        {
+            const double wr = st->w * st->r_j_prev;
+            st->beta = st->alpha * tot[1] / (wr == 0 ? 1.0e-15 : wr);
+            st->r_j = tot[1];
+            st->mode = 0;
+        }
+    }
+}
+
+// x64 += x32, x32 = 0
+__global__ void __launch_bounds__(NT) kb_add32(BatchView b, int x)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        double* __restrict__ x64 = T.f[x];
+        float* __restrict__ x32 = T.s[S_X];
+        const TileWalk tw = item_walk(T, q.it);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            x64[c] += (double)x32[c];
+            x32[c] = 0.0f;
+        }
+    }
+}
+
+// ||rhs - A x||_2 (res_calc, bicgstab_ijk.cpp:314-338, without 1/N)
+__global__ void __launch_bounds__(NT, 8) kb_defect(BatchView b, int x, int rhs)
+{
+    #pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        const SolverOp64 op = Prec<double>::op(T);
+        const double* __restrict__ xv = T.f[x];
+        const double* __restrict__ bv = T.f[rhs];
+        const TileWalk tw = item_walk(T, q.it);
+        double vals[1] = {0.0};
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            const Coupling cq = coupling_of(g, op, c, i, j, k);
+            const double rv = bv[c] - apply_A(g, cq, xv, c, i, j, k);
+            vals[0] += rv * rv;
+        }
+        block_sum_to<1>(vals, b.partials + (size_t)q.it);
+    }
+}
+
+__global__ void kb_fin_norm(BatchView b, double* __restrict__ norms)
+{
+    CTank& T = tank_at(b, blockIdx.x);
+    if(!T.active) return;
+    double tot[1];
+    tank_totals<1>(b, T, tot);
+    if(threadIdx.x == 0) norms[blockIdx.x] = sqrt(tot[0]);
+}
+
+// ------------------------------------------------------------------ level-set benchmark
+// benchmark_vortex3D::start (benchmark_vortex3D.cpp:67-96): the prescribed deformation
+// velocity at the face positions, at time simtime
+__global__ void __launch_bounds__(NT) kb_vortex_velocity(BatchView b, int comp, int id, double simtime, double DXM)
+{
+#pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        double* __restrict__ qf = T.f[id];
+        const TileWalk tw = item_walk(T, q.it);
+#pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            if((comp == 1 && i >= g.nx - 1) || (comp == 2 && j >= g.ny - 1) || (comp == 3 && k >= g.nz - 1)) continue;
+            const double x = g.XP[i + MARGE], y = g.YP[j + MARGE], z = g.ZP[k + MARGE];
+            const double ct = cos((PI_REF * simtime) / 3.0);
+            if(comp == 1)
+            {
+                const double xc = x + 0.5 * DXM;
+                qf[c] = 2.0 * pow(sin(PI_REF * xc), 2.0) * sin(2.0 * PI_REF * y) * sin(2.0 * PI_REF * z) * ct;
+            }
+            else if(comp == 2)
+            {
+                const double yc = y + 0.5 * DXM;
+                qf[c] = -sin(2.0 * PI_REF * x) * pow(sin(PI_REF * yc), 2.0) * sin(2.0 * PI_REF * z) * ct;
+            }
+            else
+            {
+                const double zc = z + 0.5 * DXM;
+                qf[c] = -sin(2.0 * PI_REF * x) * sin(2.0 * PI_REF * y) * pow(sin(PI_REF * zc), 2.0) * ct;
+            }
+        }
+    }
+}
+
+// momentum_RK3 with void convection, diffusion and pressure (momentum_RK3.cpp:70-326):
+// the three stages of one component with the constant forcing Fv
+__global__ void __launch_bounds__(NT) kb_void_rk3(BatchView b, int comp, int id, double Fv)
+{
+#pragma unroll 1
+    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    {
+        CTank& T = item_tank_at(b, q.it);
+        if(!T.active) continue;
+        const DevGrid g = T.g;
+        double* __restrict__ qf = T.f[id];
+        const double dt = T.dt;
+        const TileWalk tw = item_walk(T, q.it);
+#pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            if((comp == 1 && i >= g.nx - 1) || (comp == 2 && j >= g.ny - 1) || (comp == 3 && k >= g.nz - 1)) continue;
+            const double q0 = qf[c];
+            double a1 = q0 + dt * 1.0 * Fv;
+            a1 += 1.0 * dt * 1.0 * 0.0;
+            double a2 = 0.75 * q0 + 0.25 * a1 + 0.25 * dt * 1.0 * Fv;
+            a2 += 0.25 * dt * 1.0 * 0.0;
+            double qn = (1.0 / 3.0) * q0 + (2.0 / 3.0) * a2 + (2.0 / 3.0) * dt * 1.0 * Fv;
+            qn += (2.0 / 3.0) * dt * 1.0 * 0.0;
+            qf[c] = qn;
+        }
+    }
+}
+
+}  // namespace bk
+
+// ------------------------------------------------------------------ launchers
+using namespace bk;
+
+static inline dim3 tankgrid(const BatchView& b) { return dim3((unsigned)b.ntank); }
+static inline dim3 pgrid(int grid) { return dim3((unsigned)grid); }
+
+void bl_copy(const BatchView& b, int grid, int src, int dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_copy, pgrid(grid), dim3(NT), 0, s, b, src, dst);
+}
+void bl_fill(const BatchView& b, int grid, int id, double v, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_fill, pgrid(grid), dim3(NT), 0, s, b, id, v);
+}
+void bl_ghost(const BatchView& b, int grid, int prog, int id, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_ghost, pgrid(grid), dim3(NT), 0, s, b, prog, id);
+}
+void bl_hj_stage(const BatchView& b, int grid, int stage, int phi_s, int phi_n, int u, int v, int w, int out,
+                 double cfac, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_hj_stage, pgrid(grid), dim3(NT), 0, s, b, stage, phi_s, phi_n, u, v, w, out, cfac);
+}
+void bl_reini_stage(const BatchView& b, int grid, int stage, int fs, int f0, int out, double F43, int fsf,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_reini_stage, pgrid(grid), dim3(NT), 0, s, b, stage, fs, f0, out, F43, fsf);
+}
+void bl_flux_acc(const BatchView& b, int grid, int ipol, int q, int u, int v, int w, int acc, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_flux_acc, pgrid(grid), dim3(NT), 0, s, b, ipol, q, u, v, w, acc);
+}
+void bl_momentum(const BatchView& b, int grid, int comp, int stage, int qin, int qold, int u, int v, int w,
+                 double grav, double cfac, int out, double rw, double ra, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_momentum, pgrid(grid), dim3(NT), 0, s, b, comp, stage, qin, qold, u, v, w, grav, cfac, out,
+                       rw, ra);
+}
+void bl_heaviside(const BatchView& b, int grid, const double* prm4, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_heaviside, pgrid(grid), dim3(NT), 0, s, b, prm4[0], prm4[1], prm4[2], prm4[3]);
+    hipLaunchKernelGGL(kb_heaviside_fin, tankgrid(b), dim3(NT), 0, s, b);
+}
+void bl_poisson_rhs(const BatchView& b, int grid, int u, int v, int w, double afac, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_poisson_rhs, pgrid(grid), dim3(NT), 0, s, b, u, v, w, afac);
+}
+void bl_face_rho(const BatchView& b, int grid, double rw, double ra, int with_aii, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_face_rho, pgrid(grid), dim3(NT), 0, s, b, rw, ra);
+    if(with_aii) hipLaunchKernelGGL(kb_aii, pgrid(grid), dim3(NT), 0, s, b);
+}
+void bl_correct(const BatchView& b, int grid, double rw, double ra, double afac, int u, int v, int w, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_correct, pgrid(grid), dim3(NT), 0, s, b, rw, ra, afac, u, v, w);
+}
+
+static inline dim3 setup_grid(const BatchView& b) { return dim3((unsigned)((b.ntank + 63) / 64)); }
+
+void bl_bicg_init(const BatchView& b, int grid, int x, int rhs, double stop, int maxiter, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_setup, setup_grid(b), dim3(64), 0, s, b, stop, maxiter, 0, 0.0);
+    hipLaunchKernelGGL(kb_bicg_init<double>, pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+    hipLaunchKernelGGL(kb_fin_init, tankgrid(b), dim3(NT), 0, s, b);
+}
+
+void bl_bicg_phase(const BatchView& b, int grid, int x, int rhs, int phase, int part, hipStream_t s)
+{
+    if(part == 0)
+    {
+        switch(phase)
+        {
+            case 0: hipLaunchKernelGGL(kb_bicg_p<double>, pgrid(grid), dim3(NT), 0, s, b, x, rhs); break;
+            case 1:
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<double, true>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_v<double, false>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                break;
+            case 2:
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<double, true>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_t<double, false>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                break;
+            default: hipLaunchKernelGGL(kb_bicg_x<double>, pgrid(grid), dim3(NT), 0, s, b, x, rhs); break;
+        }
+        return;
+    }
+    switch(phase)
+    {
+        case 0: hipLaunchKernelGGL(kb_fin_p, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 1: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 2: hipLaunchKernelGGL(kb_fin_t, tankgrid(b), dim3(NT), 0, s, b); break;
+        default: hipLaunchKernelGGL(kb_fin_x, tankgrid(b), dim3(NT), 0, s, b); break;
+    }
+}
+
+void bl_coef32(const BatchView& b, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_coef32, pgrid(grid), dim3(NT), 0, s, b);
+}
+
+void bl_mixed_init(const BatchView& b, int grid, int x, int rhs, double stop, int maxiter, double delta,
+                   hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_setup, setup_grid(b), dim3(64), 0, s, b, stop, maxiter, 1, delta);
+    hipLaunchKernelGGL(kb_mix_start, pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+    hipLaunchKernelGGL(kb_fin_mix_start, tankgrid(b), dim3(NT), 0, s, b);
+}
+
+void bl_mixed_phase(const BatchView& b, int grid, int x, int rhs, int phase, int part, hipStream_t s)
+{
+    if(part == 0)
+    {
+        switch(phase)
+        {
+            case 0: hipLaunchKernelGGL(kb_mix_p, pgrid(grid), dim3(NT), 0, s, b, x); break;
+            case 1:
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<float, true>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_v<float, false>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                break;
+            case 2:
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<float, true>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_t<float, false>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                break;
+            case 3: hipLaunchKernelGGL(kb_bicg_x<float>, pgrid(grid), dim3(NT), 0, s, b, x, rhs); break;
+            default: hipLaunchKernelGGL(kb_mix_ru, pgrid(grid), dim3(NT), 0, s, b, x, rhs); break;
+        }
+        return;
+    }
+    switch(phase)
+    {
+        case 0: hipLaunchKernelGGL(kb_fin_mix_p, setup_grid(b), dim3(64), 0, s, b); break;
+        case 1: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 2: hipLaunchKernelGGL(kb_fin_t, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 3: hipLaunchKernelGGL(kb_fin_x, tankgrid(b), dim3(NT), 0, s, b); break;
+        default: hipLaunchKernelGGL(kb_fin_mix_ru, tankgrid(b), dim3(NT), 0, s, b); break;
+    }
+}
+
+void bl_add32(const BatchView& b, int grid, int x, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_add32, pgrid(grid), dim3(NT), 0, s, b, x);
+}
+
+void bl_vortex_velocity(const BatchView& b, int grid, int comp, int id, double simtime, double DXM, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_vortex_velocity, pgrid(grid), dim3(NT), 0, s, b, comp, id, simtime, DXM);
+}
+void bl_void_rk3(const BatchView& b, int grid, int comp, int id, double Fv, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_void_rk3, pgrid(grid), dim3(NT), 0, s, b, comp, id, Fv);
+}
+
+void bl_residual(const BatchView& b, int grid, int x, int rhs, double* norms, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_defect, pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+    hipLaunchKernelGGL(kb_fin_norm, tankgrid(b), dim3(NT), 0, s, b, norms);
+}
+
+}  // namespace reef

@@ -1,5 +1,5 @@
 // Internal declarations of the gfx950 hot-path library: host-side grid metadata, the
-// device view passed to kernels, and the kernel launchers.
+// device view of a domain, the solver state and the solver's vector / operator views.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -81,45 +81,6 @@ struct BiState
     long ncell;
 };
 
-// ------------------------------------------------------------------ launchers
-void launch_ghost(const GhostOp* ops, int nops, double* f, hipStream_t s);
-
-// HJ-WENO stage: out = combo(phi_n, phi_s, cdt * (0 + L_hj(phi_s; u, v, w)))
-//   stage 1: out = phi_s + cdt*L ; stage 2: 0.75 phi_n + 0.25 phi_s + cdt*L ;
-//   stage 3: (1/3) phi_n + (2/3) phi_s + cdt*L ; stage 0: out = L (accumulate: out += L)
-void launch_hj_stage(const DevGrid& g, int stage, const double* phi_s, const double* phi_n, const double* u,
-                     const double* v, const double* w, double* out, double cdt, hipStream_t s);
-
-// reinit sub-stage q (1..3): out = combo(f0, fs, dtf * L_reini(fs)); fsf = 1 for
-// reinidisc_fsf (F49 = 0), 0 for reinidisc_f (F49 = 1)
-void launch_reini_stage(const DevGrid& g, int stage, const double* fs, const double* f0, double* out, double F43,
-                        int fsf, hipStream_t s);
-
-// WENO flux: acc += flux divergence of q (component ipol) with face velocities of (u,v,w)
-void launch_flux_acc(const DevGrid& g, int ipol, const double* q, const double* u, const double* v, const double* w,
-                     double* acc, hipStream_t s);
-
-// velocity predictor of one component (momentum_FC3 stages): F = 0 - dp/(DP*rho_f) + g + conv;
-// stage 1: out = qin + (dt)*F ; stage 2: 0.75 qold + 0.25 qin + (0.25 dt) F ;
-// stage 3: (1/3) qold + (2/3) qin + ((2/3) dt) F   (written to out)
-void launch_momentum(const DevGrid& g, int comp, int stage, const double* qin, const double* qold, const double* u,
-                     const double* v, const double* w, const double* press, const double* phi, double grav,
-                     double cdt, double* out, const double* rho_w_a, hipStream_t s);
-
-void launch_heaviside(const DevGrid& g, const double* phi, double* ro, double* visc, const double* prm4,
-                      double* partials, double* vol_out, unsigned int* ticket, hipStream_t s);
-
-void launch_copy_interior(const DevGrid& g, const double* src, double* dst, hipStream_t s);
-void launch_fill(double* f, long n, double v, hipStream_t s);
-
-// projection
-void launch_poisson_rhs(const DevGrid& g, const double* u, const double* v, const double* w, double adt, double* rhs,
-                        hipStream_t s);
-void launch_face_rho(const DevGrid& g, const double* phi, double rw, double ra, double* rfx, double* rfy,
-                     double* rfz, double* aii, hipStream_t s);
-void launch_correct(const DevGrid& g, const double* pcorr, const double* phi, double rw, double ra, double adt,
-                    double* u, double* v, double* w, double* press, hipStream_t s);
-
 // operator of the solve: FP64 recomputes the couplings of poisson_pcorr from the face
 // densities; FP32 (mixed-precision inner solve) reads FP32 face conductances 1/(rho_f DP)
 // and scales them by the FP32 1/DN tables (MARGE-offset)
@@ -144,43 +105,5 @@ struct BiVec
     double* partials;
     BiState* st;
 };
-
-void launch_bicg_init(const DevGrid& g, const SolverOp64& op, const BiVec<double>& b, double stop, int maxiter,
-                      hipStream_t s);
-void launch_bicg_init(const DevGrid& g, const SolverOp32& op, const BiVec<float>& b, double stop, int maxiter,
-                      hipStream_t s);
-// one phase of a BiCGStab iteration: 0 restart or p update, 1 v = A ph, 2 t = A sh, 3 x/r update;
-// part 0 is the grid kernel, part 1 the one-block finalize kernel that must follow it
-void launch_bicg_phase(const DevGrid& g, const SolverOp64& op, const BiVec<double>& b, int phase, int part,
-                       hipStream_t s);
-void launch_bicg_phase(const DevGrid& g, const SolverOp32& op, const BiVec<float>& b, int phase, int part,
-                       hipStream_t s);
-
-// mixed precision: FP32 operator and preconditioner from the FP64 face densities;
-// ||b - A x||_2 in FP64 (norm_out on the device); x64 += x32, x32 = 0
-void launch_coef32(const DevGrid& g, const double* rfx, const double* rfy, const double* rfz, float* kx, float* ky,
-                   float* kz, float* aii, hipStream_t s);
-void launch_defect(const DevGrid& g, const SolverOp64& op, const double* x, const double* rhs, double* partials,
-                   double* norm_out, hipStream_t s);
-void launch_add32(const DevGrid& g, double* x, float* d, hipStream_t s);
-
-// FP32 BiCGStab with reliable updates: the iteration of launch_bicg_phase on the FP32
-// vectors of b32 (operator op32) while x64 accumulates the solution and every reliable
-// update recomputes the residual as b64 - A64 (x64 + x32) in FP64
-struct MixedVec
-{
-    double* x64;
-    const double* b64;
-};
-void launch_mixed_init(const DevGrid& g, const SolverOp64& op64, const MixedVec& m, const BiVec<float>& b32,
-                       double stop, int maxiter, double delta, hipStream_t s);
-// phase 0..3 as launch_bicg_phase, 4 = reliable update / restart (a no-op unless the state
-// asks for one); part 0 grid kernel, part 1 finalize
-void launch_mixed_phase(const DevGrid& g, const SolverOp64& op64, const SolverOp32& op32, const MixedVec& m,
-                        const BiVec<float>& b32, int phase, int part, hipStream_t s);
-
-// prescribed vortex field (benchmark_vortex3D.cpp:67-101) and the void momentum_RK3 update
-void launch_vortex_velocity(const DevGrid& g, int comp, double* q, double simtime, double DXM, hipStream_t s);
-void launch_void_rk3(const DevGrid& g, int comp, double* q, double dt, double Fv, hipStream_t s);
 
 }  // namespace reef

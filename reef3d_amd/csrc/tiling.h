@@ -27,6 +27,7 @@ constexpr int TILE_X = 64;
 struct TileShape
 {
     int ty, tz;          // tile rows in y and z, ty * tz == 4
+    int tzl;             // log2(tz)
     int nbx, nby, nbz;   // tiles per axis
     int ntiles;
     int cpt;             // tiles per work item
@@ -38,6 +39,7 @@ REEF_HD TileShape make_shape(int nx, int ny, int nz, int ty, int tz, int cpt)
     TileShape s;
     s.ty = ty;
     s.tz = tz;
+    s.tzl = tz == 4 ? 2 : tz == 2 ? 1 : 0;
     s.nbx = (nx + TILE_X - 1) / TILE_X;
     s.nby = (ny + ty - 1) / ty;
     s.nbz = (nz + tz - 1) / tz;
@@ -55,7 +57,7 @@ REEF_HD bool tile_cell_of(const TileShape& s, int nx, int ny, int nz, int t, int
     const int by = col / s.nbx;
     const int bx = col - by * s.nbx;
     const int row = thread >> 6;               // 0..3: one wavefront per row
-    const int ry = row / s.tz, rz = row - ry * s.tz;
+    const int ry = row >> s.tzl, rz = row & (s.tz - 1);
     i = bx * TILE_X + (thread & (TILE_X - 1));
     j = by * s.ty + ry;
     k = bz * s.tz + rz;

@@ -79,6 +79,23 @@ def library():
         "reef_timing_get": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "reef_timing_reset": ([vp], ctypes.c_int),
         "reef_ctx_sizes": ([vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "reef_batch_create": ([ctypes.c_double, ctypes.POINTER(vp)], ctypes.c_int),
+        "reef_batch_destroy": ([vp], None),
+        "reef_batch_tank_bytes": ([ctypes.POINTER(GridDesc)], ctypes.c_long),
+        "reef_batch_add": ([vp, ctypes.POINTER(GridDesc), ctypes.POINTER(Params), ip], ctypes.c_int),
+        "reef_batch_remove": ([vp, ctypes.c_int], ctypes.c_int),
+        "reef_batch_set_active": ([vp, ip, ctypes.c_int], ctypes.c_int),
+        "reef_batch_info": ([vp, ip, ip, dp], ctypes.c_int),
+        "reef_batch_put": ([vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
+        "reef_batch_get": ([vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
+        "reef_batch_ghost_fill": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "reef_batch_fluid_update": ([vp], ctypes.c_int),
+        "reef_batch_control": ([vp, ctypes.c_char_p, ctypes.c_double], ctypes.c_int),
+        "reef_batch_solver_config": ([vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+        "reef_batch_fc3_step": ([vp, ctypes.POINTER(ctypes.c_long), dp], ctypes.c_int),
+        "reef_batch_sync": ([vp], ctypes.c_int),
+        "reef_batch_timing_enable": ([vp, ctypes.c_int], ctypes.c_int),
+        "reef_batch_timing_get": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "reef_host_grid_arrays": ([ctypes.POINTER(GridDesc), ctypes.c_double, ctypes.c_int] + [dp] * 7, ctypes.c_int),
         "reef_host_ghost_apply": ([ctypes.POINTER(GridDesc), ctypes.c_double, ctypes.c_int, ctypes.c_int, dp],
                                   ctypes.c_int),
@@ -289,3 +306,97 @@ class Domain:
         out = (ctypes.c_long * 5)()
         _check(library().reef_ctx_sizes(self.h, out))
         return list(out)
+
+
+class Batch:
+    """Many independent tanks advanced by the same launches (reef_batch).
+
+    Tanks are added with the same arguments as Domain; each keeps its own fields, grid and
+    time step.  fc3_step() advances every active tank by one momentum_FC3 step.  A tank's
+    results do not depend on the other tanks of the batch.
+    """
+
+    def __init__(self, hbm_budget_bytes=0.0):
+        h = ctypes.c_void_p()
+        _check(library().reef_batch_create(float(hbm_budget_bytes), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().reef_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def tank_bytes(nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21):
+        gd, keep = grid_desc(nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir, top_group)
+        return library().reef_batch_tank_bytes(ctypes.byref(gd))
+
+    def add(self, nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21, rho=(1000.0, 1.205),
+            visc=(1.0e-6, 1.4938e-5), g=(0.0, 0.0, 0.0), F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000):
+        """Adds a tank; returns its id (slot)."""
+        gd, keep = grid_desc(nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir, top_group)
+        prm = Params(rho[0], visc[0], rho[1], visc[1], g[0], g[1], g[2], F43, F45, dt, N44, N46)
+        tid = ctypes.c_int()
+        _check(library().reef_batch_add(self.h, ctypes.byref(gd), ctypes.byref(prm), ctypes.byref(tid)))
+        self._keep[tid.value] = ((nz + 6, ny + 6, nx + 6), keep)
+        return tid.value
+
+    def remove(self, tank):
+        _check(library().reef_batch_remove(self.h, tank))
+        self._keep.pop(tank, None)
+
+    def set_active(self, tanks=()):
+        arr = (ctypes.c_int * max(len(tanks), 1))(*tanks)
+        _check(library().reef_batch_set_active(self.h, arr, len(tanks)))
+
+    def info(self):
+        slots, active, used = ctypes.c_int(), ctypes.c_int(), np.zeros(1)
+        _check(library().reef_batch_info(self.h, ctypes.byref(slots), ctypes.byref(active), _dp(used)))
+        return slots.value, active.value, float(used[0])
+
+    def put(self, tank, name, arr, layout="ifast"):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        _check(library().reef_batch_put(self.h, tank, FIELDS[name], _dp(a), 0 if layout == "ifast" else 1))
+
+    def get(self, tank, name, layout="ifast"):
+        out = np.zeros(self._keep[tank][0])
+        _check(library().reef_batch_get(self.h, tank, FIELDS[name], _dp(out), 0 if layout == "ifast" else 1))
+        return out
+
+    def ghost_fill(self, name, kind, gcv):
+        _check(library().reef_batch_ghost_fill(self.h, FIELDS[name], kind, gcv))
+
+    def fluid_update(self):
+        _check(library().reef_batch_fluid_update(self.h))
+
+    def set_control(self, key, value):
+        _check(library().reef_batch_control(self.h, key.encode(), float(value)))
+
+    def solver_config(self, precision="fp64", delta=0.1):
+        _check(library().reef_batch_solver_config(self.h, {"fp64": 0, "mixed": 1}[precision], delta))
+
+    def fc3_step(self):
+        """One momentum_FC3 step of every active tank; returns (iterations, volumes) per slot."""
+        slots = self.info()[0]
+        it = (ctypes.c_long * max(slots, 1))()
+        vol = np.zeros(2 * max(slots, 1))
+        _check(library().reef_batch_fc3_step(self.h, it, _dp(vol)))
+        return list(it)[:slots], vol[:2 * slots].reshape(-1, 2)
+
+    def sync(self):
+        _check(library().reef_batch_sync(self.h))
+
+    def timing(self, on=True):
+        _check(library().reef_batch_timing_enable(self.h, 1 if on else 0))
+
+    def timing_get(self, name):
+        ms, n = np.zeros(1), ctypes.c_long()
+        _check(library().reef_batch_timing_get(self.h, name.encode(), _dp(ms), ctypes.byref(n)))
+        return float(ms[0]), n.value
