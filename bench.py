@@ -23,12 +23,21 @@ cells, sphere r = 0.15 at (0.35, 0.35, 0.35), levelset_RK3 + 3 reinit iterations
 step, dt = 1/512 to t = T = 3 (1536 steps); reports Mcell-updates/s, the enclosed volume
 against 4/3 pi r^3, and the phi difference to the CPU oracle over the first --check steps.
 
+--workload tanks: 64 independent 2-D (x-z) wave tanks advanced together by the batched
+engine (one launch per kernel for all tanks): seed 1003, nx in {256..2048 step 128}, nz in
+{64..256 step 32}, dx = dz ~ U[0.01, 0.05], depth = 0.6 nz dz, amplitude ~ U[0.01, 0.1]
+depth, wavelength ~ U[1, 4] m, Airy initial fields, per-tank dt = 0.25 dx / max(sqrt(g depth),
+max |u|, |w| of the initial fields), mixed-precision projection, each step replayed from
+hipGraphs (--no-graphs: launch by launch); --steps momentum_FC3 steps (100 in the full
+run).  The same tanks are also run one at a time (a sample of --sequential tanks).
+
 --workload solves: the standalone-solve comparison on the same operator instead: --solves
 right-hand sides ~ N(0, 1) per cell (seeds 2000, 2001, ..., mean removed), each solved
 in FP64 and in mixed precision from x = 0 to ||b - Ax|| <= 1e-10 ||b||; reports
 iterations, times, the FP64 residual of both results and max |p - p_FP64|.
 """
 import argparse
+import math
 import json
 import os
 import sys
@@ -231,6 +240,122 @@ def vortex(args):
     print(json.dumps(out), flush=True)
 
 
+def c3_specs(n=64, seed=1003):
+    """Seeded 2-D tank set of the batched workload."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n):
+        nx = int(rng.choice(np.arange(256, 2049, 128)))
+        nz = int(rng.choice(np.arange(64, 257, 32)))
+        dx = float(rng.uniform(0.01, 0.05))
+        depth = 0.6 * nz * dx
+        amp = float(rng.uniform(0.01, 0.1)) * depth
+        lam = float(rng.uniform(1.0, 4.0))
+        specs.append(dict(nx=nx, nz=nz, dx=dx, depth=depth, amp=amp, wavelength=lam))
+    return specs
+
+
+def tank_batch(specs, precision, graphs=True, rtol=1e-10):
+    from reef3d_amd import Batch
+    from reef3d_amd.cases import wave_tank
+    B = Batch()
+    ids = []
+    for sp in specs:
+        f = wave_tank(sp["nx"], 1, sp["nz"], sp["dx"], sp["depth"], sp["amp"], sp["wavelength"])
+        umax = max(float(np.abs(f["u"]).max()), float(np.abs(f["w"]).max()))
+        dt = 0.25 * sp["dx"] / max(math.sqrt(9.81 * sp["depth"]), umax)
+        tid = B.add(nx=sp["nx"], ny=1, nz=sp["nz"], XN=f["XN"], YN=f["YN"], ZN=f["ZN"], DYM=sp["dx"], DZM=sp["dx"],
+                    jdir=0, top_group=3, g=(0.0, 0.0, -9.81), dt=dt, N44=-rtol, N46=20000)
+        for n in ("phi", "u", "v", "w", "press"):
+            B.put(tid, n, f[n])
+        ids.append(tid)
+    B.solver_config(precision)
+    B.set_control("graphs", 1 if graphs else 0)
+    B.ghost_fill("phi", 4, 52)
+    B.ghost_fill("u", 1, 10)
+    B.ghost_fill("v", 2, 11)
+    B.ghost_fill("w", 3, 12)
+    B.ghost_fill("press", 4, 40)
+    B.fluid_update()
+    B.sync()
+    return B, ids
+
+
+def tanks(args):
+    """Batched 2-D tanks versus the same tanks one at a time."""
+    specs = c3_specs()
+    cells = sum(sp["nx"] * sp["nz"] for sp in specs)
+    graphs = not args.no_graphs
+    B, _ = tank_batch(specs, "mixed", graphs)
+    t_tune = time.perf_counter()
+    if not args.no_tune:
+        B.autotune(reps=2)
+    t_tune = time.perf_counter() - t_tune
+    plans = B.plans()
+    for _ in range(args.warmup):
+        B.fc3_step()
+    B.sync()
+    t0 = time.perf_counter()
+    iters = np.zeros(len(specs))
+    for q in range(args.steps):
+        it, _ = B.fc3_step()
+        iters += np.array(it[:len(specs)])
+        if (q + 1) % 20 == 0:
+            print(f"# tanks: {q + 1} steps", file=sys.stderr, flush=True)
+    B.sync()
+    elapsed = time.perf_counter() - t0
+    # one more step launch by launch with HIP events around each launch family
+    B.timing(True)
+    B.fc3_step()
+    B.sync()
+    fams = {name: B.timing_get(name) for name in TIMED_FAMILIES}
+    B.close()
+    # the plain drop-in on the same tanks: one tank at a time, FP64 projection to the same
+    # 1e-10 residual, launch by launch (a sample of the tanks, same steps)
+    sample = specs[:max(1, min(args.sequential, len(specs)))]
+    seq_time, seq_cells = 0.0, 0
+    for sp in sample:
+        S, _ = tank_batch([sp], "fp64", False)
+        for _ in range(args.warmup):
+            S.fc3_step()
+        S.sync()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            S.fc3_step()
+        S.sync()
+        seq_time += time.perf_counter() - t1
+        seq_cells += sp["nx"] * sp["nz"]
+        S.close()
+    batch_rate = cells * args.steps / elapsed / 1e6
+    seq_rate = seq_cells * args.steps / seq_time / 1e6
+    out = {
+        "metric": "Mcell-updates/s",
+        "value": round(batch_rate, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "f64 (FP32 inner solve)",
+        "data": "synthetic",
+        "config": {"workload": f"batched_2d_tanks_64: seed 1003, {cells} cells in total, momentum_FC3 steps with "
+                               f"mixed-precision projection to ||r|| <= 1e-10 ||b|| per tank"},
+        "tank_steps_per_s": round(len(specs) * args.steps / elapsed, 2),
+        "iters_per_step_mean": round(float(iters.mean() / args.steps), 1),
+        "sequential": {"kind": "one tank at a time, FP64 projection, launch by launch", "tanks": len(sample),
+                       "Mcell_updates_per_s": round(seq_rate, 3),
+                       "ms_per_tank_step": round(seq_time / (len(sample) * args.steps) * 1e3, 3)},
+        "speedup_vs_sequential": round(batch_rate / seq_rate, 3),
+        "graphs": graphs,
+        "plans": [dict(jdir=p[0], size_log2=p[1], tile=f"64x{p[2]}x{p[3]}", cpt=p[4]) for p in plans],
+        "autotune_s": round(t_tune, 2),
+        "kernel_ms_one_step": {k: round(v[0], 3) for k, v in fams.items() if v[1]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def solves(args, cfg):
     """Standalone FP64 vs mixed-precision solves on the wave tank's step-0 operator."""
     D, _ = make_domain(cfg)
@@ -299,7 +424,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison leg")
-    ap.add_argument("--workload", choices=("fc3", "solves", "vortex"), default="fc3")
+    ap.add_argument("--workload", choices=("fc3", "solves", "vortex", "tanks"), default="fc3")
+    ap.add_argument("--sequential", type=int, default=8, help="tanks run one at a time in --workload tanks")
+    ap.add_argument("--no-graphs", action="store_true", help="--workload tanks: launch by launch")
+    ap.add_argument("--no-tune", action="store_true", help="--workload tanks: default tile plans")
     ap.add_argument("--check", type=int, default=3, help="oracle parity steps of --workload vortex")
     ap.add_argument("--solves", type=int, default=100, help="right-hand sides of --workload solves")
     args = ap.parse_args()
@@ -318,6 +446,9 @@ def main():
         return
     if args.workload == "vortex":
         vortex(args)
+        return
+    if args.workload == "tanks":
+        tanks(args)
         return
     D, _ = make_domain(cfg)
     cells = cfg["nx"] * cfg["ny"] * cfg["nz"]
@@ -365,7 +496,10 @@ def main():
             "avg_launch_us": round(avg_s * 1e6, 2),
             "bytes_per_cell": SOLVER_BYTES[dom],
         },
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in fams.items() if v[1]},
+        "graphs": graphs,
+        "plans": [dict(jdir=p[0], size_log2=p[1], tile=f"64x{p[2]}x{p[3]}", cpt=p[4]) for p in plans],
+        "autotune_s": round(t_tune, 2),
+        "kernel_ms_one_step": {k: round(v[0], 3) for k, v in fams.items() if v[1]},
     }
     if not args.no_mixed:
         del D

@@ -82,7 +82,11 @@ int reef_ctx_create(const reef_grid_desc* grid, const reef_params* prm, reef_ctx
 
 /* Control keys of lexer (ctrl.txt, lexer_read.cpp) that select a hot-path variant after
  * creation.  "F49": reinitialisation discretisation, 1 = reinidisc_f (default, ini.cpp),
- * 0 = reinidisc_fsf (reini_RK3.cpp:68-72).  Other keys: REEF_E_UNSUPPORTED. */
+ * 0 = reinidisc_fsf (reini_RK3.cpp:68-72).  "graphs" (no counterpart in the reference):
+ * 1 = replay the launch sequences of a momentum_FC3 step and the BiCGStab iteration
+ * batches as hipGraphs captured on their first run (re-captured after tanks or controls
+ * change), 0 = launch by launch (default); results are identical either way.
+ * Other keys: REEF_E_UNSUPPORTED. */
 int reef_set_control(reef_ctx* ctx, const char* key, double value);
 void reef_ctx_destroy(reef_ctx* ctx);
 const char* reef_last_error(void);
@@ -181,8 +185,26 @@ int reef_batch_put(reef_batch* b, int tank, int field_id, const double* host, in
 int reef_batch_get(reef_batch* b, int tank, int field_id, double* host, int layout);
 int reef_batch_ghost_fill(reef_batch* b, int field_id, int kind, int gcv);
 int reef_batch_fluid_update(reef_batch* b);
+/* the keys of reef_set_control ("F49", "graphs") for every tank of the batch */
 int reef_batch_control(reef_batch* b, const char* key, double value);
 int reef_batch_solver_config(reef_batch* b, int precision, double delta);
+/* Tile plans per size bucket (no counterpart in the reference).  A bucket is (jdir,
+ * size_log2 = floor(log2(nx*ny*nz))); its plan is the tile's rows ty x tz (64 x 4 x 1,
+ * 64 x 2 x 2 or 64 x 1 x 4; 2-D tanks 64 x 1 x 4) and the chunk length cpt (tiles per work
+ * item, 1..64).  Buckets without a plan use the default (64x4x1 in 3-D, 64x1x4 in 2-D,
+ * about 2048 chunks per tank).  A tank takes its bucket's plan when added and when that
+ * plan is set; its results depend on its plan (the chunks fix the reduction order), not on
+ * the other tanks.
+ * reef_batch_autotune: for each bucket of the active tanks that has no plan, probes every
+ * candidate plan on that bucket's tanks (reps timed runs after a warm-up; only step-internal
+ * arrays are written, the tanks' state is unchanged) and keeps the fastest; nbuckets
+ * (may be NULL) receives the number of buckets tuned.
+ * reef_batch_plan_set: sets (cpt = 0: removes) a bucket's plan, e.g. one saved earlier.
+ * reef_batch_plans: the table, 5 ints per entry (jdir, size_log2, ty, tz, cpt); n receives
+ * the number of entries, at most capacity are written. */
+int reef_batch_autotune(reef_batch* b, int reps, int* nbuckets);
+int reef_batch_plan_set(reef_batch* b, int jdir, int size_log2, int ty, int tz, int cpt);
+int reef_batch_plans(reef_batch* b, int* entries, int capacity, int* n);
 /* one momentum_FC3 step of every active tank; iters[slot] (may be NULL) receives the
  * summed BiCGStab iterations, vol[2*slot..] (may be NULL) volume1, volume2 */
 int reef_batch_fc3_step(reef_batch* b, long* iters, double* vol);

@@ -77,7 +77,15 @@ int control(Engine& e, const char* key, double value)
     if(std::strcmp(key, "F49") == 0)
     {
         if(value != 0.0 && value != 1.0) return fail(REEF_E_UNSUPPORTED, "F49 must be 0 or 1");
+        e.drop_graphs();
         e.fsf = value == 0.0 ? 1 : 0;
+        return 0;
+    }
+    if(std::strcmp(key, "graphs") == 0)
+    {
+        if(value != 0.0 && value != 1.0) return fail(REEF_E_ARG, "graphs must be 0 or 1");
+        e.drop_graphs();
+        e.graphs = value != 0.0;
         return 0;
     }
     return fail(REEF_E_UNSUPPORTED, std::string("control key not supported: ") + key);
@@ -87,6 +95,7 @@ int solver_config(Engine& e, int precision, double delta)
 {
     if(precision != REEF_SOLVER_FP64 && precision != REEF_SOLVER_MIXED) return fail(REEF_E_ARG, "bad precision");
     if(!(delta > 0.0 && delta < 1.0)) return fail(REEF_E_ARG, "delta must lie in (0, 1)");
+    e.drop_graphs();
     e.precision = precision;
     e.delta = delta;
     return 0;
@@ -503,6 +512,46 @@ int reef_batch_solver_config(reef_batch* b, int precision, double delta)
 {
     if(check_batch(b)) return REEF_E_ARG;
     return solver_config(*b->e, precision, delta);
+}
+
+int reef_batch_autotune(reef_batch* b, int reps, int* nbuckets)
+{
+    if(check_batch(b)) return REEF_E_ARG;
+    std::vector<Engine::TuneRecord> log;
+    const int rc = b->e->autotune(reps, &log);
+    if(nbuckets) *nbuckets = (int)log.size();
+    return eng(*b->e, rc);
+}
+
+int reef_batch_plan_set(reef_batch* b, int jdir, int size_log2, int ty, int tz, int cpt)
+{
+    if(check_batch(b)) return REEF_E_ARG;
+    Engine::Plan p;
+    p.ty = ty;
+    p.tz = tz;
+    p.cpt = cpt;
+    return eng(*b->e, b->e->set_plan(jdir, size_log2, p));
+}
+
+int reef_batch_plans(reef_batch* b, int* entries, int capacity, int* n)
+{
+    if(check_batch(b) || !n || (capacity > 0 && !entries)) return fail(REEF_E_ARG, "bad argument");
+    int q = 0;
+    for(const auto& e : b->e->plans)
+    {
+        if(q < capacity)
+        {
+            int* o = entries + 5 * q;
+            o[0] = e.first.first;
+            o[1] = e.first.second;
+            o[2] = e.second.ty;
+            o[3] = e.second.tz;
+            o[4] = e.second.cpt;
+        }
+        ++q;
+    }
+    *n = q;
+    return 0;
 }
 
 int reef_batch_fc3_step(reef_batch* b, long* iters, double* vol)

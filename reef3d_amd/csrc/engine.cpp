@@ -110,10 +110,46 @@ TileShape plan_shape(const HostGrid& h, int cpt_override)
     return make_shape(h.nx, h.ny, h.nz, ty, tz, cpt);
 }
 
+Engine::Bucket Engine::bucket_of(const HostGrid& h)
+{
+    const double cells = (double)h.nx * h.ny * h.nz;
+    return {h.jdir, (int)std::floor(std::log2(std::max(cells, 1.0)))};
+}
+
+TileShape Engine::shape_of(const HostGrid& h) const
+{
+    auto it = plans.find(bucket_of(h));
+    if(it == plans.end() || it->second.cpt <= 0) return plan_shape(h, 0);
+    const Plan& p = it->second;
+    return make_shape(h.nx, h.ny, h.nz, p.ty, p.tz, p.cpt);
+}
+
+static bool plan_valid(int jdir, const Engine::Plan& p)
+{
+    const bool shape = (p.ty == 4 && p.tz == 1) || (p.ty == 2 && p.tz == 2) || (p.ty == 1 && p.tz == 4);
+    return shape && (jdir == 1 || p.ty == 1) && p.cpt >= 1 && p.cpt <= 64;
+}
+
+int Engine::set_plan(int jdir, int lg, const Plan& p)
+{
+    if((jdir != 0 && jdir != 1) || lg < 0 || lg > 40) return fail(REEF_E_ARG, "bad size bucket");
+    if(p.cpt == 0)
+        plans.erase({jdir, lg});
+    else if(!plan_valid(jdir, p))
+        return fail(REEF_E_ARG, "bad plan: tile rows (4,1), (2,2) or (1,4) (2-D tanks: (1,4)), 1 <= cpt <= 64");
+    else
+        plans[{jdir, lg}] = p;
+    for(TankHost& t : tanks)
+        if(t.used && bucket_of(t.hg) == Bucket{jdir, lg}) t.dev.shape = shape_of(t.hg);
+    dirty_ = true;
+    return 0;
+}
+
 // ------------------------------------------------------------------ tanks
 Engine::~Engine()
 {
     if(stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
     for(auto& r : ranges_)
     {
         (void)hipEventDestroy(r.a);
@@ -251,7 +287,7 @@ int Engine::add_tank(const reef_grid_desc& gd, const reef_params& p, int* id)
         D.gops[q] = opd + op0[q];
         D.gnops[q] = nop[q];
     }
-    D.shape = plan_shape(h, plan_cpt_override);
+    D.shape = shape_of(h);
     D.active = 1;
     D.dt = p.dt;
     // slot
@@ -311,6 +347,7 @@ int Engine::sync_tables()
 {
     if(!dirty_) return 0;
     EHIP(hipStreamSynchronize(stream));
+    drop_graphs();   // their launches carry the old tables, grids and parameters
     const int nslot = (int)tanks.size();
     if(nslot > cap_tanks_ || !d_tanks_)
     {
@@ -586,17 +623,61 @@ int Engine::read_volumes()
     return 0;
 }
 
+// ------------------------------------------------------------------ graphs
+void Engine::drop_graphs()
+{
+    if(graphs_.empty()) return;
+    if(stream) (void)hipStreamSynchronize(stream);
+    for(auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
+    graphs_.clear();
+}
+
+// runs body() (a launch sequence on the stream), or replays it as a graph captured from
+// its first run under this key
+template<class Body>
+int Engine::replay(const GraphKey& key, Body body)
+{
+    if(!graphs || timing) return body();
+    auto it = graphs_.find(key);
+    if(it == graphs_.end())
+    {
+        EHIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        const int rc = body();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(stream, &g);
+        if(rc || ec != hipSuccess)
+        {
+            if(g) (void)hipGraphDestroy(g);
+            return rc ? rc : fail(REEF_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if(ei != hipSuccess) return fail(REEF_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        it = graphs_.emplace(key, ex).first;
+    }
+    EHIP(hipGraphLaunch(it->second, stream));
+    return 0;
+}
+
+const TankHost* Engine::first_used() const
+{
+    for(const TankHost& t : tanks)
+        if(t.used) return &t;
+    return nullptr;
+}
+
 // BiCGStab iterations after the init kernels, enqueued in batches; the device states say
 // when every active tank has finished.  Two batches are in flight: batch k+1 is enqueued
 // before the host waits for the states copied out after batch k.  Iterations launched
 // after a tank's end are no-ops for it.
 template<class Launch>
-int Engine::run_iterations(int nphase, const char* const* names, Launch launch, int maxiter)
+int Engine::run_iterations(const GraphKey& key, int nphase, const char* const* names, Launch launch, int maxiter)
 {
     const int batch = 8;
     const int nslot = (int)tanks.size();
     BiState* states = (BiState*)((char*)d_norms_ + 4 * sizeof(double) * cap_tanks_);
-    auto enqueue = [&](long k) -> int {
+    auto body = [&]() -> int {
         for(int q = 0; q < batch; ++q)
             for(int ph = 0; ph < nphase; ++ph)
             {
@@ -605,6 +686,11 @@ int Engine::run_iterations(int nphase, const char* const* names, Launch launch, 
                 tend();
                 launch(ph, 1);
             }
+        return 0;
+    };
+    auto enqueue = [&](long k) -> int {
+        int rc;
+        if((rc = replay(key, body))) return rc;
         EHIP(hipMemcpyAsync(h_status_ + (k & 1) * cap_status_, states, sizeof(BiState) * nslot,
                             hipMemcpyDeviceToHost, stream));
         EHIP(hipEventRecord(st_ev_[k & 1], stream));
@@ -636,20 +722,12 @@ int Engine::run_iterations(int nphase, const char* const* names, Launch launch, 
 static const char* const BICG_NAMES[5] = {"bicg_p", "bicg_v", "bicg_t", "bicg_x", ""};
 static const char* const MIX_NAMES[5] = {"mix_p", "bicg32_v", "bicg32_t", "bicg32_x", "mix_ru"};
 
-// poisson::start + solver::start on every active tank: FP64 bicgstab_ijk, or the mixed
-// form (FP32 iteration with FP64 reliable updates) ending on the true FP64 residual
-int Engine::solve(int x, int rhs, double stop, int maxiter, bool mixed)
+// face densities and the solver's init kernels (FP64, or FP32 coefficients and the mixed
+// solver's init)
+int Engine::solve_begin(int x, int rhs, double stop, int maxiter, bool mixed)
 {
-    int rc;
-    if((rc = sync_tables())) return rc;
-    double rw = 1000.0, ra = 1.205;
-    for(const TankHost& t : tanks)
-        if(t.used)
-        {
-            rw = t.prm.W1;
-            ra = t.prm.W3;
-            break;
-        }
+    const TankHost* t0 = first_used();
+    const double rw = t0 ? t0->prm.W1 : 1000.0, ra = t0 ? t0->prm.W3 : 1.205;
     tbegin("face_rho");
     bl_face_rho(view_, grid_simple_, rw, ra, mixed ? 0 : 1, stream);
     tend();
@@ -658,8 +736,7 @@ int Engine::solve(int x, int rhs, double stop, int maxiter, bool mixed)
         tbegin("bicg_init");
         bl_bicg_init(view_, grid_solver_, x, rhs, stop, maxiter, stream);
         tend();
-        auto launch = [&](int ph, int part) { bl_bicg_phase(view_, grid_solver_, x, rhs, ph, part, stream); };
-        return run_iterations(4, BICG_NAMES, launch, maxiter);
+        return 0;
     }
     tbegin("coef32");
     bl_coef32(view_, grid_simple_, stream);
@@ -667,26 +744,47 @@ int Engine::solve(int x, int rhs, double stop, int maxiter, bool mixed)
     tbegin("mix_init");
     bl_mixed_init(view_, grid_solver_, x, rhs, stop, maxiter, delta, stream);
     tend();
+    return 0;
+}
+
+int Engine::solve_iterate(int x, int rhs, int maxiter, bool mixed)
+{
+    if(!mixed)
+    {
+        auto launch = [&](int ph, int part) { bl_bicg_phase(view_, grid_solver_, x, rhs, ph, part, stream); };
+        return run_iterations({1, 0, x, rhs}, 4, BICG_NAMES, launch, maxiter);
+    }
     auto launch = [&](int ph, int part) { bl_mixed_phase(view_, grid_solver_, x, rhs, ph, part, stream); };
-    if((rc = run_iterations(5, MIX_NAMES, launch, maxiter))) return rc;
+    return run_iterations({1, 1, x, rhs}, 5, MIX_NAMES, launch, maxiter);
+}
+
+// the mixed solve keeps its FP64 iterate split until the end: x64 += x32
+int Engine::solve_end(int x, bool mixed)
+{
+    if(!mixed) return 0;
     tbegin("add32");
     bl_add32(view_, grid_simple_, x, stream);
     tend();
     return 0;
 }
 
+// poisson::start + solver::start on every active tank: FP64 bicgstab_ijk, or the mixed
+// form (FP32 iteration with FP64 reliable updates) ending on the true FP64 residual
+int Engine::solve(int x, int rhs, double stop, int maxiter, bool mixed)
+{
+    int rc;
+    if((rc = sync_tables())) return rc;
+    if((rc = solve_begin(x, rhs, stop, maxiter, mixed))) return rc;
+    if((rc = solve_iterate(x, rhs, maxiter, mixed))) return rc;
+    return solve_end(x, mixed);
+}
+
 int Engine::residual(int x, int rhs, std::vector<double>& norms)
 {
     int rc;
     if((rc = sync_tables())) return rc;
-    double rw = 1000.0, ra = 1.205;
-    for(const TankHost& t : tanks)
-        if(t.used)
-        {
-            rw = t.prm.W1;
-            ra = t.prm.W3;
-            break;
-        }
+    const TankHost* t0 = first_used();
+    const double rw = t0 ? t0->prm.W1 : 1000.0, ra = t0 ? t0->prm.W3 : 1.205;
     bl_face_rho(view_, grid_simple_, rw, ra, 0, stream);
     bl_residual(view_, grid_solver_, x, rhs, d_norms_, stream);
     norms.assign(tanks.size(), 0.0);
@@ -695,26 +793,28 @@ int Engine::residual(int x, int rhs, std::vector<double>& norms)
     return 0;
 }
 
-// pjm_corr::start (pjm_corr.cpp:90-121) on every active tank
-int Engine::pjm(int u, int v, int w, double alpha)
+// pjm_corr::start (pjm_corr.cpp:90-121) up to the solve: velocity ghosts, rhs, p' = 0,
+// solver set-up
+int Engine::pjm_pre(int u, int v, int w, double alpha, bool mixed)
 {
     int rc;
     if((rc = ghost(u, 1, 7)) || (rc = ghost(v, 2, 8)) || (rc = ghost(w, 3, 9))) return rc;
-    const TankHost* t0 = nullptr;
-    for(const TankHost& t : tanks)
-        if(t.used)
-        {
-            t0 = &t;
-            break;
-        }
+    const TankHost* t0 = first_used();
     if(!t0) return 0;
     tbegin("poisson_rhs");
     bl_poisson_rhs(view_, grid_simple_, u, v, w, alpha, stream);
     tend();
     bl_fill(view_, grid_simple_, REEF_PCORR, 0.0, stream);
-    if((rc = solve(REEF_PCORR, A_RHS, t0->prm.N44, t0->prm.N46, precision == REEF_SOLVER_MIXED))) return rc;
-    for(size_t q = 0; q < tanks.size(); ++q)
-        if(tanks[q].used && tanks[q].dev.active) step_iters[q] += last_state[q].iter;
+    return solve_begin(REEF_PCORR, A_RHS, t0->prm.N44, t0->prm.N46, mixed);
+}
+
+// pjm_corr::start after the solve: p' ghosts, velocity and pressure correction
+int Engine::pjm_post(int u, int v, int w, double alpha, bool mixed)
+{
+    int rc;
+    const TankHost* t0 = first_used();
+    if(!t0) return 0;
+    if((rc = solve_end(REEF_PCORR, mixed))) return rc;
     if((rc = ghost(REEF_PCORR, 4, 40))) return rc;
     tbegin("correct");
     bl_correct(view_, grid_simple_, t0->prm.W1, t0->prm.W3, alpha, u, v, w, stream);
@@ -722,16 +822,25 @@ int Engine::pjm(int u, int v, int w, double alpha)
     return ghost(REEF_PRESS, 4, 40);
 }
 
+// pjm_corr::start (pjm_corr.cpp:90-121) on every active tank
+int Engine::pjm(int u, int v, int w, double alpha)
+{
+    int rc;
+    if((rc = sync_tables())) return rc;
+    const TankHost* t0 = first_used();
+    const bool mixed = precision == REEF_SOLVER_MIXED;
+    if((rc = pjm_pre(u, v, w, alpha, mixed))) return rc;
+    if(!t0) return 0;
+    if((rc = solve_iterate(REEF_PCORR, A_RHS, t0->prm.N46, mixed))) return rc;
+    for(size_t q = 0; q < tanks.size(); ++q)
+        if(tanks[q].used && tanks[q].dev.active) step_iters[q] += last_state[q].iter;
+    return pjm_post(u, v, w, alpha, mixed);
+}
+
 // velocity predictor of the three components (momentum_FC3.cpp:160-208 and the later stages)
 int Engine::momentum_stage(int stage, const int qin[3], const int qold[3], const int vel[3], const int out[3])
 {
-    const TankHost* t0 = nullptr;
-    for(const TankHost& t : tanks)
-        if(t.used)
-        {
-            t0 = &t;
-            break;
-        }
+    const TankHost* t0 = first_used();
     if(!t0) return 0;
     const double grav[3] = {t0->prm.W20, t0->prm.W21, t0->prm.W22};
     const double cfac = stage == 1 ? 1.0 : stage == 2 ? 0.25 : (2.0 / 3.0);
@@ -749,6 +858,73 @@ int Engine::momentum_stage(int stage, const int qin[3], const int qold[3], const
     return 0;
 }
 
+// The launches of momentum_FC3::start (momentum_FC3.cpp:122-430) between its pressure
+// solves.  Segment 0: stage-1 phi transport, reinit, predictor and the stage-1 projection
+// up to its solve; segments 1 and 2: the end of the previous projection, velocity ghosts,
+// the next stage up to its solve; segment 3: the end of the stage-3 projection, velocity
+// ghosts and the fluid update (rho and nu depend on phi only, so the last fluid_update of
+// the step is the one that stays).
+int Engine::fc3_segment(int s, bool mixed)
+{
+    static const int VEL[3][3] = {{A_URK1, A_VRK1, A_WRK1}, {A_URK2, A_VRK2, A_WRK2}, {REEF_U, REEF_V, REEF_W}};
+    static const double ALPHA[3] = {1.0, 0.25, 2.0 / 3.0};
+    int rc;
+    if(s > 0)
+    {
+        const int* vel = VEL[s - 1];
+        if((rc = pjm_post(vel[0], vel[1], vel[2], ALPHA[s - 1], mixed))) return rc;
+        if((rc = ghost(vel[0], 1, 10)) || (rc = ghost(vel[1], 2, 11)) || (rc = ghost(vel[2], 3, 12))) return rc;
+    }
+    if(s == 0)
+    {
+        // stage 1 (momentum_FC3.cpp:132-230)
+        tbegin("copy");
+        bl_copy(view_, grid_simple_, REEF_PHI, A_LS, stream);   // ls ghosts stay zero, as in the reference
+        tend();
+        tbegin("hj");
+        bl_hj_stage(view_, grid_stencil_, 1, A_LS, -1, REEF_U, REEF_V, REEF_W, A_FRK1, 1.0, stream);
+        tend();
+        if((rc = ghost(A_FRK1, 4, 52))) return rc;
+        if((rc = reini(A_FRK1, 2))) return rc;
+        tbegin("copy");
+        bl_copy(view_, grid_simple_, A_FRK1, REEF_PHI, stream);
+        tend();
+        if((rc = ghost(REEF_PHI, 4, 52))) return rc;
+        const int qin[3] = {REEF_U, REEF_V, REEF_W};
+        if((rc = momentum_stage(1, qin, qin, qin, VEL[0]))) return rc;
+    }
+    else if(s == 1)
+    {
+        // stage 2 (momentum_FC3.cpp:232-332)
+        tbegin("hj");
+        bl_hj_stage(view_, grid_stencil_, 2, A_FRK1, A_LS, A_URK1, A_VRK1, A_WRK1, A_FRK2, 0.25, stream);
+        tend();
+        if((rc = ghost(A_FRK2, 4, 52))) return rc;
+        if((rc = reini(A_FRK2, 2))) return rc;
+        tbegin("copy");
+        bl_copy(view_, grid_simple_, A_FRK2, REEF_PHI, stream);
+        tend();
+        if((rc = ghost(REEF_PHI, 4, 52))) return rc;
+        if((rc = reini(REEF_PHI, 2))) return rc;
+        const int qold[3] = {REEF_U, REEF_V, REEF_W};
+        if((rc = momentum_stage(2, VEL[0], qold, VEL[0], VEL[1]))) return rc;
+    }
+    else if(s == 2)
+    {
+        // stage 3 (momentum_FC3.cpp:334-430)
+        tbegin("hj");
+        bl_hj_stage(view_, grid_stencil_, 3, A_FRK2, A_LS, A_URK2, A_VRK2, A_WRK2, REEF_PHI, 2.0 / 3.0, stream);
+        tend();
+        if((rc = ghost(REEF_PHI, 4, 52))) return rc;
+        if((rc = reini(REEF_PHI, 3))) return rc;
+        const int qold[3] = {REEF_U, REEF_V, REEF_W};
+        if((rc = momentum_stage(3, VEL[1], qold, VEL[1], VEL[2]))) return rc;
+    }
+    else
+        return fluid_update();
+    return pjm_pre(VEL[s][0], VEL[s][1], VEL[s][2], ALPHA[s], mixed);
+}
+
 // momentum_FC3::start (momentum_FC3.cpp:122-430): three TVD-RK3 stages of phi transport,
 // reinitialisation, velocity predictor and projection; rho / nu from the final phi
 int Engine::fc3_step()
@@ -756,58 +932,18 @@ int Engine::fc3_step()
     int rc;
     if((rc = sync_tables())) return rc;
     std::fill(step_iters.begin(), step_iters.end(), 0L);
-    // ---- stage 1 (momentum_FC3.cpp:132-230)
-    tbegin("copy");
-    bl_copy(view_, grid_simple_, REEF_PHI, A_LS, stream);   // ls ghosts stay zero, as in the reference
-    tend();
-    tbegin("hj");
-    bl_hj_stage(view_, grid_stencil_, 1, A_LS, -1, REEF_U, REEF_V, REEF_W, A_FRK1, 1.0, stream);
-    tend();
-    if((rc = ghost(A_FRK1, 4, 52))) return rc;
-    if((rc = reini(A_FRK1, 2))) return rc;
-    tbegin("copy");
-    bl_copy(view_, grid_simple_, A_FRK1, REEF_PHI, stream);
-    tend();
-    if((rc = ghost(REEF_PHI, 4, 52))) return rc;
+    const TankHost* t0 = first_used();
+    if(!t0) return 0;
+    const bool mixed = precision == REEF_SOLVER_MIXED;
+    for(int s = 0; s < 4; ++s)
     {
-        const int qin[3] = {REEF_U, REEF_V, REEF_W}, out[3] = {A_URK1, A_VRK1, A_WRK1};
-        if((rc = momentum_stage(1, qin, qin, qin, out))) return rc;
+        if((rc = replay({0, s, mixed ? 1 : 0, fsf}, [&] { return fc3_segment(s, mixed); }))) return rc;
+        if(s == 3) break;
+        if((rc = solve_iterate(REEF_PCORR, A_RHS, t0->prm.N46, mixed))) return rc;
+        for(size_t q = 0; q < tanks.size(); ++q)
+            if(tanks[q].used && tanks[q].dev.active) step_iters[q] += last_state[q].iter;
     }
-    if((rc = pjm(A_URK1, A_VRK1, A_WRK1, 1.0))) return rc;
-    if((rc = ghost(A_URK1, 1, 10)) || (rc = ghost(A_VRK1, 2, 11)) || (rc = ghost(A_WRK1, 3, 12))) return rc;
-    // ---- stage 2 (momentum_FC3.cpp:232-332)
-    tbegin("hj");
-    bl_hj_stage(view_, grid_stencil_, 2, A_FRK1, A_LS, A_URK1, A_VRK1, A_WRK1, A_FRK2, 0.25, stream);
-    tend();
-    if((rc = ghost(A_FRK2, 4, 52))) return rc;
-    if((rc = reini(A_FRK2, 2))) return rc;
-    tbegin("copy");
-    bl_copy(view_, grid_simple_, A_FRK2, REEF_PHI, stream);
-    tend();
-    if((rc = ghost(REEF_PHI, 4, 52))) return rc;
-    if((rc = reini(REEF_PHI, 2))) return rc;
-    {
-        const int qin[3] = {A_URK1, A_VRK1, A_WRK1}, qold[3] = {REEF_U, REEF_V, REEF_W},
-                  out[3] = {A_URK2, A_VRK2, A_WRK2};
-        if((rc = momentum_stage(2, qin, qold, qin, out))) return rc;
-    }
-    if((rc = pjm(A_URK2, A_VRK2, A_WRK2, 0.25))) return rc;
-    if((rc = ghost(A_URK2, 1, 10)) || (rc = ghost(A_VRK2, 2, 11)) || (rc = ghost(A_WRK2, 3, 12))) return rc;
-    // ---- stage 3 (momentum_FC3.cpp:334-430)
-    tbegin("hj");
-    bl_hj_stage(view_, grid_stencil_, 3, A_FRK2, A_LS, A_URK2, A_VRK2, A_WRK2, REEF_PHI, 2.0 / 3.0, stream);
-    tend();
-    if((rc = ghost(REEF_PHI, 4, 52))) return rc;
-    if((rc = reini(REEF_PHI, 3))) return rc;
-    {
-        const int qin[3] = {A_URK2, A_VRK2, A_WRK2}, qold[3] = {REEF_U, REEF_V, REEF_W},
-                  out[3] = {REEF_U, REEF_V, REEF_W};
-        if((rc = momentum_stage(3, qin, qold, qin, out))) return rc;
-    }
-    if((rc = pjm(REEF_U, REEF_V, REEF_W, 2.0 / 3.0))) return rc;
-    if((rc = ghost(REEF_U, 1, 10)) || (rc = ghost(REEF_V, 2, 11)) || (rc = ghost(REEF_W, 3, 12))) return rc;
-    // rho and nu depend on phi only: the last fluid_update of the step is the one that stays
-    return fluid_update();
+    return 0;
 }
 
 // levelset_RK3 transport stages (levelset_RK3.cpp:109-160)
@@ -857,6 +993,131 @@ int Engine::vortex_step(double simtime)
     if((rc = ghost(REEF_V, 2, 11))) return rc;
     if((rc = ghost(REEF_V, 2, 12))) return rc;
     return fluid_update();
+}
+
+// ------------------------------------------------------------------ autotuner
+// One sample of the step's kernels on step-internal arrays only: phi copied to ls, an HJ
+// stage into frk1, one reinit iteration (3 stages) of frk1, the stage-1 predictor into
+// urk1..wrk1, then the projection set-up on those and 16 iterations of the configured
+// solver on p'.  phi, u, v, w, press, ro and visc are read, never written; ls, frk1,
+// urk1..wrk1, the rhs, p' and the solver arrays are rewritten by every step before use.
+int Engine::probe(double& stencil_ms, double& solver_ms)
+{
+    int rc;
+    stencil_ms = solver_ms = 0.0;
+    if((rc = sync_tables())) return rc;
+    if(!first_used()) return 0;
+    struct Ev
+    {
+        hipEvent_t e = nullptr;
+        ~Ev()
+        {
+            if(e) (void)hipEventDestroy(e);
+        }
+    } ev[3];
+    for(Ev& x : ev) EHIP(hipEventCreate(&x.e));
+    const bool mixed = precision == REEF_SOLVER_MIXED;
+    EHIP(hipEventRecord(ev[0].e, stream));
+    bl_copy(view_, grid_simple_, REEF_PHI, A_LS, stream);
+    bl_hj_stage(view_, grid_stencil_, 1, A_LS, -1, REEF_U, REEF_V, REEF_W, A_FRK1, 1.0, stream);
+    if((rc = ghost(A_FRK1, 4, 52)) || (rc = reini(A_FRK1, 1))) return rc;
+    const int qin[3] = {REEF_U, REEF_V, REEF_W}, out[3] = {A_URK1, A_VRK1, A_WRK1};
+    if((rc = momentum_stage(1, qin, qin, qin, out))) return rc;
+    EHIP(hipEventRecord(ev[1].e, stream));
+    if((rc = ghost(A_URK1, 1, 7)) || (rc = ghost(A_VRK1, 2, 8)) || (rc = ghost(A_WRK1, 3, 9))) return rc;
+    bl_poisson_rhs(view_, grid_simple_, A_URK1, A_VRK1, A_WRK1, 1.0, stream);
+    bl_fill(view_, grid_simple_, REEF_PCORR, 0.0, stream);
+    if((rc = solve_begin(REEF_PCORR, A_RHS, 1e-300, 16, mixed))) return rc;
+    for(int it = 0; it < 16; ++it)
+        for(int ph = 0; ph < (mixed ? 5 : 4); ++ph)
+            for(int part = 0; part < 2; ++part)
+                if(mixed)
+                    bl_mixed_phase(view_, grid_solver_, REEF_PCORR, A_RHS, ph, part, stream);
+                else
+                    bl_bicg_phase(view_, grid_solver_, REEF_PCORR, A_RHS, ph, part, stream);
+    if((rc = solve_end(REEF_PCORR, mixed))) return rc;
+    EHIP(hipEventRecord(ev[2].e, stream));
+    EHIP(hipEventSynchronize(ev[2].e));
+    float a = 0.0f, b = 0.0f;
+    EHIP(hipEventElapsedTime(&a, ev[0].e, ev[1].e));
+    EHIP(hipEventElapsedTime(&b, ev[1].e, ev[2].e));
+    stencil_ms = a;
+    solver_ms = b;
+    return 0;
+}
+
+// For each bucket of the active tanks without a plan: every tile shape (3-D: 64x4x1,
+// 64x2x2, 64x1x4; 2-D: 64x1x4) and chunk length 1..64 (powers of two) is probed on that
+// bucket's tanks alone, launch by launch, reps times after one warm-up; the score is the
+// best probe time weighted to about one momentum_FC3 step (9 x the stencil sample, which
+// holds a third of a stage's HJ / reinit / predictor work, plus 300 / 16 x the 16-iteration
+// solver sample).  The fastest plan is kept.
+int Engine::autotune(int reps, std::vector<TuneRecord>* log)
+{
+    int rc;
+    if((rc = sync_tables())) return rc;
+    std::map<Bucket, std::vector<int>> members;
+    std::vector<int> was_active;
+    for(size_t q = 0; q < tanks.size(); ++q)
+        if(tanks[q].used && tanks[q].dev.active)
+        {
+            was_active.push_back((int)q);
+            if(!plans.count(bucket_of(tanks[q].hg))) members[bucket_of(tanks[q].hg)].push_back((int)q);
+        }
+    if(members.empty()) return 0;
+    const bool keep_graphs = graphs, keep_timing = timing;
+    graphs = false;
+    timing = false;
+    reps = std::max(1, reps);
+    static const int SHAPES[3][2] = {{4, 1}, {2, 2}, {1, 4}};
+    auto tune_bucket = [&](const Bucket& bk, const std::vector<int>& ids) -> int {
+        int r;
+        if((r = set_active(ids))) return r;
+        int max_tiles = 1;
+        for(int id : ids) max_tiles = std::max(max_tiles, plan_shape(tanks[id].hg, 1).ntiles);
+        Plan best{};
+        double best_score = 1e300;
+        for(const auto& sh : SHAPES)
+        {
+            if(bk.first == 0 && sh[0] != 1) continue;
+            for(int cpt = 1; cpt <= 64; cpt *= 2)
+            {
+                if(cpt / 2 >= max_tiles) break;   // one item per tank already
+                const Plan p{sh[0], sh[1], cpt};
+                for(int id : ids)
+                {
+                    const HostGrid& h = tanks[id].hg;
+                    tanks[id].dev.shape = make_shape(h.nx, h.ny, h.nz, p.ty, p.tz, p.cpt);
+                }
+                dirty_ = true;
+                double score = 1e300;
+                for(int k = 0; k <= reps; ++k)
+                {
+                    double a, b;
+                    if((r = probe(a, b))) return r;
+                    if(k > 0) score = std::min(score, 9.0 * a + (300.0 / 16.0) * b);
+                }
+                if(score < best_score)
+                {
+                    best_score = score;
+                    best = p;
+                }
+            }
+        }
+        plans[bk] = best;
+        if(log) log->push_back({bk, best, best_score});
+        return 0;
+    };
+    for(const auto& m : members)
+        if((rc = tune_bucket(m.first, m.second))) break;
+    // every tank back on its bucket's plan, the active set as before
+    for(TankHost& t : tanks)
+        if(t.used) t.dev.shape = shape_of(t.hg);
+    dirty_ = true;
+    const int rc2 = set_active(was_active);
+    graphs = keep_graphs;
+    timing = keep_timing;
+    return rc ? rc : rc2;
 }
 
 }  // namespace reef

@@ -7,6 +7,7 @@
 
 #include "batch.h"
 
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
@@ -60,8 +61,35 @@ public:
     int fsf = 0;                    // reinidisc_fsf (F49 = 0) for every tank
     int precision = REEF_SOLVER_FP64;
     double delta = 0.1;             // reliable-update threshold of the mixed solve
-    // tile shape / chunk overrides per size bucket (autotuner); 0 = default plan
-    int plan_cpt_override = 0;
+    // tile plans per size bucket (jdir, floor(log2(nx ny nz))): tile rows ty x tz and
+    // chunk length cpt; a bucket without an entry uses the default plan.  A tank takes the
+    // plan of its bucket when it is added (and when its bucket's plan is set), so its
+    // chunks, and with them its reduction order, are the same alone and in any batch that
+    // shares the plan table.
+    struct Plan
+    {
+        int ty = 0, tz = 0, cpt = 0;
+    };
+    using Bucket = std::pair<int, int>;
+    std::map<Bucket, Plan> plans;
+    static Bucket bucket_of(const HostGrid& h);
+    TileShape shape_of(const HostGrid& h) const;
+    int set_plan(int jdir, int lg, const Plan& p);
+    // times every candidate plan of each bucket of the active tanks that has none and
+    // keeps the fastest; leaves the tanks' fields and results unchanged
+    struct TuneRecord
+    {
+        Bucket bucket;
+        Plan plan;
+        double score_ms;
+    };
+    int autotune(int reps, std::vector<TuneRecord>* log);
+    // replay fc3_step's launch sequences (the four segments between its pressure solves)
+    // and the BiCGStab iteration batches as hipGraphs, captured on first use; the graphs
+    // are dropped whenever the tank tables or the controls change.  Launch-by-launch while
+    // timing is on.
+    bool graphs = false;
+    void drop_graphs();
 
     // ---- tanks
     static size_t tank_bytes(const reef_grid_desc& g);
@@ -138,12 +166,24 @@ private:
         err = m;
         return code;
     }
+    using GraphKey = std::array<int, 4>;
+    std::map<GraphKey, hipGraphExec_t> graphs_;
+    template<class Body>
+    int replay(const GraphKey& key, Body body);
     template<class Launch>
-    int run_iterations(int nphase, const char* const* names, Launch launch, int maxiter);
+    int run_iterations(const GraphKey& key, int nphase, const char* const* names, Launch launch, int maxiter);
+    const TankHost* first_used() const;
+    int solve_begin(int x, int rhs, double stop, int maxiter, bool mixed);
+    int solve_iterate(int x, int rhs, int maxiter, bool mixed);
+    int solve_end(int x, bool mixed);
+    int pjm_pre(int u, int v, int w, double alpha, bool mixed);
+    int pjm_post(int u, int v, int w, double alpha, bool mixed);
+    int fc3_segment(int s, bool mixed);
+    int probe(double& stencil_ms, double& solver_ms);
     int momentum_stage(int stage, const int qin[3], const int qold[3], const int vel[3], const int out[3]);
 };
 
-// tile shape and chunk length of a tank (the plan the batch launches use)
+// default tile shape and chunk length of a tank
 TileShape plan_shape(const HostGrid& h, int cpt_override);
 
 }  // namespace reef

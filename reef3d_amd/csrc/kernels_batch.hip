@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(NT, 8) kb_bicg_v(BatchView b, int x, int rhs)
             vals[1] += vd * vd;
             vals[2] += rc * rc;
         };
-        if(plane_walk_ok(T))
+        if constexpr(PW)
             plane_walk<V>(g, op, tw, [&](long cc, V&) { return ph[cc]; }, [&](long c, V Ax, V) { epilogue(c, Ax); });
         else
             #pragma unroll 1
@@ -838,7 +838,7 @@ __global__ void __launch_bounds__(NT, 8) kb_bicg_t(BatchView b, int x, int rhs)
             sv = r[cc] - alpha * v[cc];
             return sv * a[cc];
         };
-        if(plane_walk_ok(T))
+        if constexpr(PW)
             plane_walk<V>(g, op, tw, shv, [&](long c, V Ax, V s) { epilogue(c, Ax, s); });
         else
             #pragma unroll 1

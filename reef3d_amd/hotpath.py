@@ -93,6 +93,9 @@ def library():
         "reef_batch_control": ([vp, ctypes.c_char_p, ctypes.c_double], ctypes.c_int),
         "reef_batch_solver_config": ([vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "reef_batch_fc3_step": ([vp, ctypes.POINTER(ctypes.c_long), dp], ctypes.c_int),
+        "reef_batch_autotune": ([vp, ctypes.c_int, ip], ctypes.c_int),
+        "reef_batch_plan_set": ([vp] + [ctypes.c_int] * 5, ctypes.c_int),
+        "reef_batch_plans": ([vp, ip, ctypes.c_int, ip], ctypes.c_int),
         "reef_batch_sync": ([vp], ctypes.c_int),
         "reef_batch_timing_enable": ([vp, ctypes.c_int], ctypes.c_int),
         "reef_batch_timing_get": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
@@ -381,6 +384,24 @@ class Batch:
 
     def solver_config(self, precision="fp64", delta=0.1):
         _check(library().reef_batch_solver_config(self.h, {"fp64": 0, "mixed": 1}[precision], delta))
+
+    def autotune(self, reps=2):
+        """Tunes the tile plan of each size bucket of the active tanks that has none;
+        returns the number of buckets tuned."""
+        n = ctypes.c_int()
+        _check(library().reef_batch_autotune(self.h, int(reps), ctypes.byref(n)))
+        return n.value
+
+    def plans(self):
+        """The plan table: list of (jdir, size_log2, ty, tz, cpt)."""
+        n = ctypes.c_int()
+        _check(library().reef_batch_plans(self.h, None, 0, ctypes.byref(n)))
+        buf = (ctypes.c_int * max(5 * n.value, 1))()
+        _check(library().reef_batch_plans(self.h, buf, n.value, ctypes.byref(n)))
+        return [tuple(buf[5 * q:5 * q + 5]) for q in range(n.value)]
+
+    def set_plan(self, jdir, size_log2, ty, tz, cpt):
+        _check(library().reef_batch_plan_set(self.h, jdir, size_log2, ty, tz, cpt))
 
     def fc3_step(self):
         """One momentum_FC3 step of every active tank; returns (iterations, volumes) per slot."""

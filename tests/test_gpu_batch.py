@@ -47,21 +47,24 @@ def _prepare(B):
     B.fluid_update()
 
 
-def _run(tanks, steps, precision):
+def _run(tanks, steps, precision, graphs=False, every_step=False):
     from reef3d_amd import Batch
     B = Batch()
     ids = [B.add(**kw) for kw, _ in tanks]
     for tid, (kw, f) in zip(ids, tanks):
         _initial(B, tid, kw, f)
     B.solver_config(precision)
+    B.set_control("graphs", 1 if graphs else 0)
     _prepare(B)
-    iters = []
+    iters, history = [], []
     for _ in range(steps):
         it, _ = B.fc3_step()
         iters.append(it)
+        if every_step:
+            history.append([{n: B.get(tid, n) for n in FIELDS} for tid in ids])
     out = [{n: B.get(tid, n) for n in FIELDS} for tid in ids]
     B.close()
-    return out, iters
+    return (history if every_step else out), iters
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -72,6 +75,21 @@ def test_batch_results_identical_to_alone(precision):
         for n in FIELDS:
             assert np.array_equal(mixed[q][n], alone[0][n]), f"tank {q} {n} differs between batch and alone"
         assert [s[q] for s in it_mixed] == [s[0] for s in it_alone]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replay_identical(precision):
+    """Steps replayed from captured hipGraphs (segments and iteration batches) give the
+    same bits as launch-by-launch steps, on every replay; the host reads all fields back
+    between the replays."""
+    eager, it_eager = _run(TANKS[:3], 3, precision, graphs=False, every_step=True)
+    graph, it_graph = _run(TANKS[:3], 3, precision, graphs=True, every_step=True)
+    assert it_eager == it_graph
+    for step in range(3):
+        for q in range(3):
+            for n in FIELDS:
+                assert np.array_equal(eager[step][q][n], graph[step][q][n]), \
+                    f"step {step} tank {q} {n}: graph replay differs"
 
 
 def test_batched_tank_matches_oracle():
@@ -114,3 +132,39 @@ def test_batch_slots_and_budget():
     it, _ = B.fc3_step()
     assert it[c] > 0 and it[b] == 0
     B.close()
+
+
+def test_autotune_plans_and_identity():
+    """The autotuner leaves every tank's state unchanged, records a valid plan for each
+    size bucket, and with that plan table a tank's step is the same alone and in the batch."""
+    from reef3d_amd import Batch
+    B = Batch()
+    ids = [B.add(**kw) for kw, _ in TANKS]
+    for tid, (kw, f) in zip(ids, TANKS):
+        _initial(B, tid, kw, f)
+    B.solver_config("mixed")
+    _prepare(B)
+    before = [{n: B.get(tid, n) for n in FIELDS} for tid in ids]
+    nb = B.autotune(reps=1)
+    plans = B.plans()
+    assert nb == len(plans) >= 2
+    for jdir, lg, ty, tz, cpt in plans:
+        assert (ty, tz) in ((4, 1), (2, 2), (1, 4)) and (jdir == 1 or ty == 1) and 1 <= cpt <= 64
+    for q, tid in enumerate(ids):
+        for n in FIELDS:
+            assert np.array_equal(before[q][n], B.get(tid, n)), f"autotune changed tank {q} {n}"
+    B.fc3_step()
+    batch = [{n: B.get(tid, n) for n in FIELDS} for tid in ids]
+    B.close()
+    for q, (kw, f) in enumerate(TANKS):
+        A = Batch()
+        for p in plans:
+            A.set_plan(*p)
+        t = A.add(**kw)
+        _initial(A, t, kw, f)
+        A.solver_config("mixed")
+        _prepare(A)
+        A.fc3_step()
+        for n in FIELDS:
+            assert np.array_equal(batch[q][n], A.get(t, n)), f"tank {q} {n}: tuned plan, alone != batch"
+        A.close()
