@@ -31,6 +31,15 @@ max |u|, |w| of the initial fields), mixed-precision projection, each step repla
 hipGraphs (--no-graphs: launch by launch); --steps momentum_FC3 steps (100 in the full
 run).  The same tanks are also run one at a time (a sample of --sequential tanks).
 
+--workload serve: a served request stream: --requests tank requests with seeded Poisson
+arrivals (lambda = 200/s, seed 1004), 70% 2-D tanks (nx 256..2048 step 128, nz 64..256
+step 32) and 30% 3-D tanks (nx 128..256, ny 32..64, nz 64..128), dx, depth, amplitude and
+wavelength drawn as in --workload tanks, each asking for 50..500 momentum_FC3 steps
+(divided by --step-div to bound the run); the server (continuous batching, HBM budget
+200 GB, mixed-precision projection to 1e-10) reports p50 / p99 latency (submit to done)
+and solves/s; the same requests are then solved one at a time with the plain FP64
+drop-in, and their latencies under the same arrivals follow from FIFO order.
+
 --workload solves: the standalone-solve comparison on the same operator instead: --solves
 right-hand sides ~ N(0, 1) per cell (seeds 2000, 2001, ..., mean removed), each solved
 in FP64 and in mixed precision from x = 0 to ||b - Ax|| <= 1e-10 ||b||; reports
@@ -356,6 +365,118 @@ def tanks(args):
     print(json.dumps(out), flush=True)
 
 
+def c4_requests(n, step_div, seed=1004, rate=200.0):
+    """Seeded request stream of the served workload."""
+    rng = np.random.default_rng(seed)
+    t, reqs = 0.0, []
+    for _ in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        if rng.random() < 0.7:
+            nx, ny, nz, jdir = int(rng.choice(np.arange(256, 2049, 128))), 1, int(rng.choice(np.arange(64, 257, 32))), 0
+        else:
+            nx, ny, nz, jdir = (int(rng.choice(np.arange(128, 257, 32))), int(rng.choice(np.arange(32, 65, 8))),
+                                int(rng.choice(np.arange(64, 129, 16))), 1)
+        dx = float(rng.uniform(0.01, 0.05))
+        depth = 0.6 * nz * dx
+        amp = float(rng.uniform(0.01, 0.1)) * depth
+        lam = float(rng.uniform(1.0, 4.0))
+        steps = int(rng.integers(50, 501))
+        reqs.append(dict(arrival=t, nx=nx, ny=ny, nz=nz, jdir=jdir, dx=dx, depth=depth, amp=amp, wavelength=lam,
+                         steps=max(1, steps // step_div), steps_full=steps))
+    return reqs
+
+
+def request_fields(r):
+    from reef3d_amd.cases import wave_tank
+    f = wave_tank(r["nx"], r["ny"], r["nz"], r["dx"], r["depth"], r["amp"], r["wavelength"])
+    umax = max(float(np.abs(f["u"]).max()), float(np.abs(f["w"]).max()))
+    f["dt"] = 0.25 * r["dx"] / max(math.sqrt(9.81 * r["depth"]), umax)
+    return f
+
+
+def serve(args):
+    """Served request stream versus the same requests solved one at a time in FP64."""
+    from reef3d_amd import Batch, Server
+    reqs = c4_requests(args.requests, args.step_div)
+    fields = [request_fields(r) for r in reqs]
+    S = Server(hbm_budget_bytes=200e9, precision="mixed", graphs=False)
+    ids = []
+    t_start = time.perf_counter()
+    for r, f in zip(reqs, fields):
+        delay = r["arrival"] - (time.perf_counter() - t_start)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(S.submit(r["nx"], r["ny"], r["nz"], f["XN"], f["YN"], f["ZN"], r["dx"], r["dx"], f["dt"], r["steps"],
+                            f["phi"], f["u"], f["v"], f["w"], f["press"], jdir=r["jdir"], top_group=3))
+    done = 0
+    for rid in ids:
+        st = S.wait(rid)
+        assert st == "done", st
+        done += 1
+        if done % 8 == 0:
+            print(f"# serve: {done}/{len(ids)} done", file=sys.stderr, flush=True)
+    info = [S.info(rid) for rid in ids]
+    rounds = S.stats()["rounds"]
+    S.close()
+    lat = np.array([i["t_done"] - i["t_submit"] for i in info])
+    makespan = max(i["t_done"] for i in info) - min(i["t_submit"] for i in info)
+    # the plain drop-in: one request at a time, FP64 projection, launch by launch
+    service = []
+    for q, (r, f) in enumerate(zip(reqs, fields)):
+        B = Batch()
+        t = B.add(nx=r["nx"], ny=r["ny"], nz=r["nz"], XN=f["XN"], YN=f["YN"], ZN=f["ZN"], DYM=r["dx"], DZM=r["dx"],
+                  jdir=r["jdir"], top_group=3, g=(0.0, 0.0, -9.81), dt=f["dt"], N44=-1e-10, N46=20000)
+        for n in ("phi", "u", "v", "w", "press"):
+            B.put(t, n, f[n])
+        B.solver_config("fp64")
+        B.sync()
+        t1 = time.perf_counter()
+        B.ghost_fill("phi", 4, 52)
+        B.ghost_fill("u", 1, 10)
+        B.ghost_fill("v", 2, 11)
+        B.ghost_fill("w", 3, 12)
+        B.ghost_fill("press", 4, 40)
+        B.fluid_update()
+        for _ in range(r["steps"]):
+            B.fc3_step()
+        B.sync()
+        service.append(time.perf_counter() - t1)
+        B.close()
+        if (q + 1) % 8 == 0:
+            print(f"# sequential: {q + 1}/{len(reqs)} done", file=sys.stderr, flush=True)
+    seq_done, clock = [], 0.0
+    for r, sv in zip(reqs, service):
+        clock = max(clock, r["arrival"]) + sv
+        seq_done.append(clock)
+    seq_lat = np.array([d - r["arrival"] for d, r in zip(seq_done, reqs)])
+    seq_span = seq_done[-1] - reqs[0]["arrival"]
+    n2d = sum(r["jdir"] == 0 for r in reqs)
+    out = {
+        "metric": "served tank solves/s",
+        "value": round(len(reqs) / makespan, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_baseline": None,
+        "dtype": "f64 (FP32 inner solve)",
+        "data": "synthetic",
+        "config": {"workload": f"served_tank_stream: {len(reqs)} requests, Poisson arrivals 200/s (seed 1004), "
+                               f"{n2d} 2-D / {len(reqs) - n2d} 3-D, steps 50..500 / {args.step_div}, HBM budget 200 GB, "
+                               f"mixed-precision projection to ||r|| <= 1e-10 ||b||"},
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                      "max": round(float(lat.max()), 3)},
+        "makespan_s": round(makespan, 3),
+        "server_rounds": rounds,
+        "tank_steps": int(sum(r["steps"] for r in reqs)),
+        "sequential_fp64": {"solves_per_s": round(len(reqs) / seq_span, 4),
+                            "latency_s": {"p50": round(float(np.percentile(seq_lat, 50)), 3),
+                                          "p99": round(float(np.percentile(seq_lat, 99)), 3)},
+                            "service_s_total": round(float(sum(service)), 3)},
+        "speedup_vs_sequential": round((len(reqs) / makespan) / (len(reqs) / seq_span), 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def solves(args, cfg):
     """Standalone FP64 vs mixed-precision solves on the wave tank's step-0 operator."""
     D, _ = make_domain(cfg)
@@ -424,7 +545,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison leg")
-    ap.add_argument("--workload", choices=("fc3", "solves", "vortex", "tanks"), default="fc3")
+    ap.add_argument("--workload", choices=("fc3", "solves", "vortex", "tanks", "serve"), default="fc3")
+    ap.add_argument("--requests", type=int, default=16, help="requests of --workload serve")
+    ap.add_argument("--step-div", type=int, default=10, help="--workload serve: steps per request divided by this")
     ap.add_argument("--sequential", type=int, default=8, help="tanks run one at a time in --workload tanks")
     ap.add_argument("--no-graphs", action="store_true", help="--workload tanks: launch by launch")
     ap.add_argument("--no-tune", action="store_true", help="--workload tanks: default tile plans")
@@ -449,6 +572,9 @@ def main():
         return
     if args.workload == "tanks":
         tanks(args)
+        return
+    if args.workload == "serve":
+        serve(args)
         return
     D, _ = make_domain(cfg)
     cells = cfg["nx"] * cfg["ny"] * cfg["nz"]

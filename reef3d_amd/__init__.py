@@ -4,6 +4,6 @@ The compute runs in hand-written HIP kernels in libreef3d_hip.so (C-ABI in
 include/reef3d_hip.h); `reef3d_amd.hotpath.Domain` is the Python host mirror of the
 reference's convection / reini / solver / pressure / ghostcell interfaces.
 """
-from .hotpath import Batch, Domain, ReefError, FIELDS, library  # noqa: F401
+from .hotpath import Batch, Domain, ReefError, Server, FIELDS, library  # noqa: F401
 
-__all__ = ["Batch", "Domain", "ReefError", "FIELDS", "library"]
+__all__ = ["Batch", "Domain", "ReefError", "Server", "FIELDS", "library"]

@@ -43,6 +43,12 @@ class Params(ctypes.Structure):
                 ("dt", ctypes.c_double), ("N44", ctypes.c_double), ("N46", ctypes.c_int)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_double), ("max_live", ctypes.c_int), ("precision", ctypes.c_int),
+                ("delta", ctypes.c_double), ("graphs", ctypes.c_int), ("F49", ctypes.c_int), ("shared", Params),
+                ("plans", ctypes.POINTER(ctypes.c_int)), ("nplans", ctypes.c_int)]
+
+
 def library():
     """Loads libreef3d_hip.so; raises if it was not built (no fallback path exists)."""
     global _LIB
@@ -99,6 +105,16 @@ def library():
         "reef_batch_sync": ([vp], ctypes.c_int),
         "reef_batch_timing_enable": ([vp, ctypes.c_int], ctypes.c_int),
         "reef_batch_timing_get": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "reef_server_create": ([ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)], ctypes.c_int),
+        "reef_server_destroy": ([vp], None),
+        "reef_server_last_error": ([], ctypes.c_char_p),
+        "reef_server_submit": ([vp, ctypes.POINTER(GridDesc), ctypes.c_double, ctypes.c_int] + [dp] * 5
+                               + [ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "reef_server_wait": ([vp, ctypes.c_long, ctypes.c_double, ip], ctypes.c_int),
+        "reef_server_result": ([vp, ctypes.c_long, ctypes.c_int, dp], ctypes.c_int),
+        "reef_server_request_info": ([vp, ctypes.c_long, dp, ip, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "reef_server_release": ([vp, ctypes.c_long], ctypes.c_int),
+        "reef_server_stats": ([vp, ip, ip, ctypes.POINTER(ctypes.c_long), dp], ctypes.c_int),
         "reef_host_grid_arrays": ([ctypes.POINTER(GridDesc), ctypes.c_double, ctypes.c_int] + [dp] * 7, ctypes.c_int),
         "reef_host_ghost_apply": ([ctypes.POINTER(GridDesc), ctypes.c_double, ctypes.c_int, ctypes.c_int, dp],
                                   ctypes.c_int),
@@ -421,3 +437,86 @@ class Batch:
         ms, n = np.zeros(1), ctypes.c_long()
         _check(library().reef_batch_timing_get(self.h, name.encode(), _dp(ms), ctypes.byref(n)))
         return float(ms[0]), n.value
+
+
+def _scheck(rc):
+    if rc != 0:
+        msg = library().reef_server_last_error().decode(errors="replace")
+        raise ReefError(f"reef server call failed ({rc}): {msg}")
+
+
+class Server:
+    """Continuous-batching server of tank requests (reef_server, include/reef3d_serve.h).
+
+    A worker thread in the library admits requests in arrival order by the HBM budget,
+    advances every live tank one momentum_FC3 step per round in the same launches and
+    retires finished tanks.  All requests share the physical and solver parameters given
+    here; each has its own grid, dt and step count.
+    """
+
+    STATES = ("queued", "live", "done", "failed")
+
+    def __init__(self, hbm_budget_bytes=0.0, max_live=0, precision="mixed", delta=0.1, graphs=False, F49=1,
+                 rho=(1000.0, 1.205), visc=(1.0e-6, 1.4938e-5), g=(0.0, 0.0, -9.81), F43=0.55, F45=2.1,
+                 N44=-1e-10, N46=20000, plans=()):
+        flat = [int(x) for p in plans for x in p]
+        self._plans = (ctypes.c_int * max(len(flat), 1))(*flat)
+        cfg = ServerConfig(float(hbm_budget_bytes), int(max_live), {"fp64": 0, "mixed": 1}[precision], float(delta),
+                           1 if graphs else 0, int(F49),
+                           Params(rho[0], visc[0], rho[1], visc[1], g[0], g[1], g[2], F43, F45, 0.0, N44, N46),
+                           self._plans, len(plans))
+        h = ctypes.c_void_p()
+        _scheck(library().reef_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self._shapes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            library().reef_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, nx, ny, nz, XN, YN, ZN, DYM, DZM, dt, steps, phi, u, v, w, press, jdir=1, top_group=21):
+        """Queues a tank; returns the request id.  The arrays are copied."""
+        gd, keep = grid_desc(nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir, top_group)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (phi, u, v, w, press)]
+        shape = (nz + 6, ny + 6, nx + 6)
+        for a in arrs:
+            if a.shape != shape:
+                raise ValueError(f"field shape {a.shape}, expected {shape}")
+        rid = ctypes.c_long()
+        _scheck(library().reef_server_submit(self.h, ctypes.byref(gd), float(dt), int(steps), *(_dp(a) for a in arrs),
+                                             ctypes.byref(rid)))
+        self._shapes[rid.value] = shape
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        """Waits for a request; returns its state ("queued", "live", "done", "failed")."""
+        st = ctypes.c_int()
+        _scheck(library().reef_server_wait(self.h, rid, float(timeout), ctypes.byref(st)))
+        return self.STATES[st.value]
+
+    def result(self, rid, name):
+        out = np.zeros(self._shapes[rid])
+        _scheck(library().reef_server_result(self.h, rid, FIELDS[name], _dp(out)))
+        return out
+
+    def info(self, rid):
+        t = np.zeros(3)
+        steps, iters = ctypes.c_int(), ctypes.c_long()
+        _scheck(library().reef_server_request_info(self.h, rid, _dp(t), ctypes.byref(steps), ctypes.byref(iters)))
+        return dict(t_submit=t[0], t_admit=t[1], t_done=t[2], steps=steps.value, iters=iters.value)
+
+    def release(self, rid):
+        _scheck(library().reef_server_release(self.h, rid))
+        self._shapes.pop(rid, None)
+
+    def stats(self):
+        q, lv, st, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_long(), np.zeros(1)
+        _scheck(library().reef_server_stats(self.h, ctypes.byref(q), ctypes.byref(lv), ctypes.byref(st), _dp(b)))
+        return dict(queued=q.value, live=lv.value, rounds=st.value, bytes_in_use=float(b[0]))

@@ -47,3 +47,11 @@ def test_context_rejects_bad_arguments_without_a_device():
     assert lib.reef_ctx_create(None, None, None) == -1
     assert b"null" in lib.reef_last_error()
     assert lib.reef_sync(None) == -1
+
+
+def test_server_rejects_bad_arguments_without_a_device():
+    from reef3d_amd import hotpath
+    lib = hotpath.library()
+    assert lib.reef_server_create(None, None) == -1
+    assert b"null" in lib.reef_server_last_error()
+    assert lib.reef_server_wait(None, 1, 0.0, None) == -1

@@ -1,0 +1,78 @@
+"""The serving loop on the GPU: continuous batching of tank requests.
+
+A request's result must be the same bits whether it was served alone or while other
+requests of other sizes and step counts were admitted and retired around it, and the
+same as the batch engine's result for that tank; admission must respect the HBM budget.
+"""
+import numpy as np
+import pytest
+
+from tests.test_gpu_batch import FIELDS, TANKS, _initial, _prepare
+
+pytestmark = pytest.mark.gpu
+
+STEPS = [2, 3, 1, 2]
+
+
+def _submit(S, q, steps):
+    kw, f = TANKS[q]
+    return S.submit(kw["nx"], kw["ny"], kw["nz"], kw["XN"], kw["YN"], kw["ZN"], kw["DYM"], kw["DZM"], kw["dt"], steps,
+                    f["phi"], f["u"], f["v"], f["w"], f["press"], jdir=kw["jdir"], top_group=kw["top_group"])
+
+
+def _results(S, rid):
+    assert S.wait(rid, 120.0) == "done"
+    return {n: S.result(rid, n) for n in FIELDS}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_served_alone_equals_mix(precision):
+    from reef3d_amd import Server
+    S = Server(precision=precision)
+    ids = [_submit(S, q, STEPS[q]) for q in range(len(TANKS))]
+    mix = [_results(S, rid) for rid in ids]
+    infos = [S.info(rid) for rid in ids]
+    assert [i["steps"] for i in infos] == STEPS
+    # alone: one request at a time on the same server
+    for q in range(len(TANKS)):
+        rid = _submit(S, q, STEPS[q])
+        alone = _results(S, rid)
+        for n in FIELDS:
+            assert np.array_equal(alone[n], mix[q][n]), f"request {q} {n}: alone != mix"
+        assert S.info(rid)["iters"] == infos[q]["iters"]
+    S.close()
+
+
+def test_served_equals_batch_engine():
+    from reef3d_amd import Batch, Server
+    S = Server(precision="mixed")
+    rid = _submit(S, 1, 2)
+    served = _results(S, rid)
+    S.close()
+    kw, f = TANKS[1]
+    B = Batch()
+    t = B.add(**kw)
+    _initial(B, t, kw, f)
+    B.solver_config("mixed")
+    _prepare(B)
+    for _ in range(2):
+        B.fc3_step()
+    for n in FIELDS:
+        assert np.array_equal(served[n], B.get(t, n)), n
+    B.close()
+
+
+def test_admission_by_budget():
+    from reef3d_amd import Batch, Server
+    kw, _ = TANKS[0]
+    need = Batch.tank_bytes(**{k: kw[k] for k in ("nx", "ny", "nz", "XN", "YN", "ZN", "DYM", "DZM", "jdir")})
+    S = Server(hbm_budget_bytes=2.5 * need, precision="mixed")
+    ids = [_submit(S, 0, 2) for _ in range(4)]
+    for rid in ids:
+        assert S.wait(rid, 120.0) == "done"
+    info = [S.info(rid) for rid in ids]
+    # at most two tanks fit: the third and fourth are admitted only after earlier ones left
+    done_sorted = sorted(i["t_done"] for i in info[:2])
+    assert info[2]["t_admit"] >= done_sorted[0] and info[3]["t_admit"] >= done_sorted[0]
+    assert S.stats()["bytes_in_use"] <= 2.5 * need
+    S.close()
