@@ -291,16 +291,29 @@ def tank_batch(specs, precision, graphs=True, rtol=1e-10):
 
 
 def tanks(args):
-    """Batched 2-D tanks versus the same tanks one at a time."""
+    """Batched 2-D tanks: lockstep steps replayed from hipGraphs, the same steps with
+    iteration-level scheduling, and the same tanks one at a time with the plain FP64
+    drop-in."""
     specs = c3_specs()
     cells = sum(sp["nx"] * sp["nz"] for sp in specs)
     graphs = not args.no_graphs
+    # the tile plans: tuned once on the batch, shared by both runs
     B, _ = tank_batch(specs, "mixed", graphs)
     t_tune = time.perf_counter()
     if not args.no_tune:
         B.autotune(reps=2)
     t_tune = time.perf_counter() - t_tune
     plans = B.plans()
+    B.close()
+
+    def fresh():
+        Bx, _ = tank_batch(specs, "mixed", graphs)
+        for p_ in plans:
+            Bx.set_plan(*p_)
+        return Bx
+
+    # lockstep: every step's solves run until the slowest tank has converged
+    B = fresh()
     for _ in range(args.warmup):
         B.fc3_step()
     B.sync()
@@ -310,14 +323,28 @@ def tanks(args):
         it, _ = B.fc3_step()
         iters += np.array(it[:len(specs)])
         if (q + 1) % 20 == 0:
-            print(f"# tanks: {q + 1} steps", file=sys.stderr, flush=True)
+            print(f"# tanks lockstep: {q + 1} steps", file=sys.stderr, flush=True)
     B.sync()
-    elapsed = time.perf_counter() - t0
+    t_lock = time.perf_counter() - t0
     # one more step launch by launch with HIP events around each launch family
     B.timing(True)
     B.fc3_step()
     B.sync()
     fams = {name: B.timing_get(name) for name in TIMED_FAMILIES}
+    lock_fields = [B.get(t, "phi") for t in range(len(specs))] if args.steps <= 5 else None
+    B.close()
+    # iteration-level scheduling: each tank moves through its own steps
+    B = fresh()
+    B.run(args.warmup)
+    B.sync()
+    t0 = time.perf_counter()
+    B.run(args.steps)
+    B.sync()
+    t_sched = time.perf_counter() - t0
+    same = None
+    if lock_fields is not None:
+        B.run(1)
+        same = all(np.array_equal(B.get(t, "phi"), lock_fields[t]) for t in range(len(specs)))
     B.close()
     # the plain drop-in on the same tanks: one tank at a time, FP64 projection to the same
     # 1e-10 residual, launch by launch (a sample of the tanks, same steps)
@@ -335,32 +362,39 @@ def tanks(args):
         seq_time += time.perf_counter() - t1
         seq_cells += sp["nx"] * sp["nz"]
         S.close()
-    batch_rate = cells * args.steps / elapsed / 1e6
+    lock_rate = cells * args.steps / t_lock / 1e6
+    sched_rate = cells * args.steps / t_sched / 1e6
     seq_rate = seq_cells * args.steps / seq_time / 1e6
+    best = max(lock_rate, sched_rate)
     out = {
         "metric": "Mcell-updates/s",
-        "value": round(batch_rate, 3),
+        "value": round(best, 3),
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "ms_per_step": round(min(t_lock, t_sched) / args.steps * 1e3, 3),
         "higher_is_better": True,
         "vs_baseline": None,
         "dtype": "f64 (FP32 inner solve)",
         "data": "synthetic",
         "config": {"workload": f"batched_2d_tanks_64: seed 1003, {cells} cells in total, momentum_FC3 steps with "
-                               f"mixed-precision projection to ||r|| <= 1e-10 ||b|| per tank"},
-        "tank_steps_per_s": round(len(specs) * args.steps / elapsed, 2),
+                               f"mixed-precision projection to ||r|| <= 1e-10 ||b|| per tank",
+                   "value_from": "scheduled" if sched_rate >= lock_rate else "lockstep_graphs"},
+        "lockstep_graphs": {"Mcell_updates_per_s": round(lock_rate, 3), "ms_per_step": round(t_lock / args.steps * 1e3, 3),
+                            "graphs": graphs},
+        "scheduled": {"Mcell_updates_per_s": round(sched_rate, 3), "ms_per_step": round(t_sched / args.steps * 1e3, 3),
+                      "phi_identical_to_lockstep": same},
+        "tank_steps_per_s": round(len(specs) * args.steps / min(t_lock, t_sched), 2),
         "iters_per_step_mean": round(float(iters.mean() / args.steps), 1),
+        "iters_per_step_max": round(float(iters.max() / args.steps), 1),
         "sequential": {"kind": "one tank at a time, FP64 projection, launch by launch", "tanks": len(sample),
                        "Mcell_updates_per_s": round(seq_rate, 3),
                        "ms_per_tank_step": round(seq_time / (len(sample) * args.steps) * 1e3, 3)},
-        "speedup_vs_sequential": round(batch_rate / seq_rate, 3),
-        "graphs": graphs,
+        "speedup_vs_sequential": round(best / seq_rate, 3),
         "plans": [dict(jdir=p[0], size_log2=p[1], tile=f"64x{p[2]}x{p[3]}", cpt=p[4]) for p in plans],
         "autotune_s": round(t_tune, 2),
-        "kernel_ms_one_step": {k: round(v[0], 3) for k, v in fams.items() if v[1]},
+        "kernel_ms_one_lockstep_step": {k: round(v[0], 3) for k, v in fams.items() if v[1]},
     }
     print(json.dumps(out), flush=True)
 
@@ -622,10 +656,7 @@ def main():
             "avg_launch_us": round(avg_s * 1e6, 2),
             "bytes_per_cell": SOLVER_BYTES[dom],
         },
-        "graphs": graphs,
-        "plans": [dict(jdir=p[0], size_log2=p[1], tile=f"64x{p[2]}x{p[3]}", cpt=p[4]) for p in plans],
-        "autotune_s": round(t_tune, 2),
-        "kernel_ms_one_step": {k: round(v[0], 3) for k, v in fams.items() if v[1]},
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in fams.items() if v[1]},
     }
     if not args.no_mixed:
         del D

@@ -208,6 +208,12 @@ int reef_batch_plans(reef_batch* b, int* entries, int capacity, int* n);
 /* one momentum_FC3 step of every active tank; iters[slot] (may be NULL) receives the
  * summed BiCGStab iterations, vol[2*slot..] (may be NULL) volume1, volume2 */
 int reef_batch_fc3_step(reef_batch* b, long* iters, double* vol);
+/* nsteps momentum_FC3 steps of every active tank with iteration-level scheduling: each
+ * tank moves through its own steps, so a tank whose pressure solve has converged goes on
+ * while others still iterate (no lockstep at the solves); iters[slot] (may be NULL)
+ * receives the BiCGStab iterations of each tank's last step.  Results are those of
+ * nsteps calls of reef_batch_fc3_step. */
+int reef_batch_run(reef_batch* b, int nsteps, long* iters);
 int reef_batch_sync(reef_batch* b);
 int reef_batch_timing_enable(reef_batch* b, int on);
 int reef_batch_timing_get(reef_batch* b, const char* name, double* ms, long* launches);

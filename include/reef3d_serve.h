@@ -61,7 +61,7 @@ int reef_server_result(reef_server* s, long id, int field_id, double* host);
 int reef_server_request_info(reef_server* s, long id, double* times, int* steps_done, long* iters);
 /* forgets a done or failed request */
 int reef_server_release(reef_server* s, long id);
-/* queued and live requests, rounds stepped, device bytes held by live tanks; returns the
+/* queued and live requests, scheduler rounds run, device bytes held by live tanks; returns the
  * worker's error once it has stopped on one */
 int reef_server_stats(reef_server* s, int* queued, int* live, long* steps, double* bytes_in_use);
 

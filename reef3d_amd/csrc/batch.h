@@ -1,7 +1,7 @@
 // Batched tanks: many independent box domains ("tanks") advanced by the same kernel
 // launches.  Each tank keeps its own fields, metadata, work decomposition, time step and
-// solver state in device memory (TankDev); a launch walks the work items of every tank of
-// the batch (BatchView).  A single domain is a batch of one.
+// solver state in device memory (TankDev); a launch covers a set of tanks (TankSet).  A
+// single domain is a batch of one.
 #pragma once
 
 #include "reef_internal.h"
@@ -35,8 +35,7 @@ struct TankDev
     float* s[NARR32];
     const float *iDXN32, *iDYN32, *iDZN32;   // FP32 1/DN of the mixed solve (MARGE-offset)
     TileShape shape;               // tiles, chunk length, items
-    int item0;                     // first work item of the tank in the batch
-    int active;                    // 0: skipped by every launch
+    int item0;                     // first reduction slot (work item) of the tank
     double dt;                     // time step of the tank
     const GhostOp* gops[NPROG];    // compiled ghost programs
     int gnops[NPROG];
@@ -44,46 +43,56 @@ struct TankDev
     double* vol;                   // volume1, volume2 of the last fluid update
 };
 
-struct BatchView
+// The tanks one launch covers, passed by value (it lives in the kernel-argument segment,
+// read with scalar loads): their slots in the tank table and the prefix sums of their
+// tiles and work items.  A block finds its tank by a search over the prefix sums.
+constexpr int SET_MAX = 192;
+struct TankSet
 {
-    const TankDev* tanks;          // [ntank]
-    const int* item_tank;          // [nitems]
-    int ntank, nitems;
-    double* partials;              // [nitems * 4]
+    const TankDev* tanks;          // device tank table (every slot)
+    double* partials;              // reduction partials: [TankDev::item0 + item] * 4
+    int n;                         // tanks in the set
+    int ntiles, nitems;            // tiles and work items of the set
     int has_pw, has_direct;        // tanks walked by the 3-D plane walk / by the direct stencil
+    int slot[SET_MAX];             // tank slots
+    int tile0[SET_MAX + 1];        // first tile of each tank in the set's tile range
+    int item0[SET_MAX + 1];        // first work item of each tank in the set's item range
 };
 
 // ---- launchers (kernels_batch.hip); field arguments are Arr64 ids; every launch covers
-// the active tanks of the batch.  grid: blocks of the persistent launch (multiple of 8).
-void bl_copy(const BatchView& b, int grid, int src, int dst, hipStream_t s);
-void bl_fill(const BatchView& b, int grid, int id, double v, hipStream_t s);
-void bl_ghost(const BatchView& b, int grid, int prog, int id, hipStream_t s);
-void bl_hj_stage(const BatchView& b, int grid, int stage, int phi_s, int phi_n, int u, int v, int w, int out,
+// the tanks of the set.  Stencil and elementwise kernels run one 256-cell tile per block;
+// reductions run a persistent grid over the set's work items.
+void bl_copy(const TankSet& b, int src, int dst, hipStream_t s);
+void bl_fill(const TankSet& b, int id, double v, hipStream_t s);
+void bl_ghost(const TankSet& b, int prog, int id, hipStream_t s);
+void bl_hj_stage(const TankSet& b, int stage, int phi_s, int phi_n, int u, int v, int w, int out,
                  double cfac, hipStream_t s);
-void bl_reini_stage(const BatchView& b, int grid, int stage, int fs, int f0, int out, double F43, int fsf,
+void bl_reini_stage(const TankSet& b, int stage, int fs, int f0, int out, double F43, int fsf,
                     hipStream_t s);
-void bl_flux_acc(const BatchView& b, int grid, int ipol, int q, int u, int v, int w, int acc, hipStream_t s);
-void bl_momentum(const BatchView& b, int grid, int comp, int stage, int qin, int qold, int u, int v, int w,
+void bl_flux_acc(const TankSet& b, int ipol, int q, int u, int v, int w, int acc, hipStream_t s);
+void bl_momentum(const TankSet& b, int comp, int stage, int qin, int qold, int u, int v, int w,
                  double grav, double cfac, int out, double rw, double ra, hipStream_t s);
-void bl_heaviside(const BatchView& b, int grid, const double* prm4, hipStream_t s);
-void bl_poisson_rhs(const BatchView& b, int grid, int u, int v, int w, double afac, hipStream_t s);
-void bl_face_rho(const BatchView& b, int grid, double rw, double ra, int with_aii, hipStream_t s);
-void bl_correct(const BatchView& b, int grid, double rw, double ra, double afac, int u, int v, int w,
+void bl_heaviside(const TankSet& b, const double* prm4, hipStream_t s);
+void bl_poisson_rhs(const TankSet& b, int u, int v, int w, double afac, hipStream_t s);
+void bl_face_rho(const TankSet& b, double rw, double ra, int with_aii, hipStream_t s);
+void bl_correct(const TankSet& b, double rw, double ra, double afac, int u, int v, int w,
                 hipStream_t s);
 // BiCGStab on every active tank: x, rhs array ids; stop / maxiter per call
-void bl_bicg_init(const BatchView& b, int grid, int x, int rhs, double stop, int maxiter, hipStream_t s);
-void bl_bicg_phase(const BatchView& b, int grid, int x, int rhs, int phase, int part, hipStream_t s);
+void bl_bicg_init(const TankSet& b, int x, int rhs, double stop, int maxiter, hipStream_t s);
+void bl_bicg_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStream_t s);
 // mixed precision (FP32 iteration, FP64 reliable updates)
-void bl_coef32(const BatchView& b, int grid, hipStream_t s);
-void bl_mixed_init(const BatchView& b, int grid, int x, int rhs, double stop, int maxiter, double delta,
+void bl_coef32(const TankSet& b, hipStream_t s);
+void bl_mixed_init(const TankSet& b, int x, int rhs, double stop, int maxiter, double delta,
                    hipStream_t s);
-void bl_mixed_phase(const BatchView& b, int grid, int x, int rhs, int phase, int part, hipStream_t s);
-void bl_add32(const BatchView& b, int grid, int x, hipStream_t s);
+void bl_mixed_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStream_t s);
+void bl_add32(const TankSet& b, int x, hipStream_t s);
 // level-set benchmark: prescribed vortex velocity (benchmark_vortex3D) and the void
 // momentum_RK3 update of one component
-void bl_vortex_velocity(const BatchView& b, int grid, int comp, int id, double simtime, double DXM, hipStream_t s);
-void bl_void_rk3(const BatchView& b, int grid, int comp, int id, double Fv, hipStream_t s);
-// ||rhs - A x||_2 per tank into norms[tank]
-void bl_residual(const BatchView& b, int grid, int x, int rhs, double* norms, hipStream_t s);
+void bl_vortex_velocity(const TankSet& b, int comp, int id, double simtime, double DXM, hipStream_t s);
+void bl_void_rk3(const TankSet& b, int comp, int id, double Fv, hipStream_t s);
+// solver states of the set's tanks into dst[slot]
+void bl_gather_states(const TankSet& b, BiState* dst, hipStream_t s);
+// ||rhs - A x||_2 per tank into norms[slot]
+void bl_residual(const TankSet& b, int x, int rhs, double* norms, hipStream_t s);
 
 }  // namespace reef

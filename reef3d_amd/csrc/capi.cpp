@@ -570,6 +570,20 @@ int reef_batch_fc3_step(reef_batch* b, long* iters, double* vol)
     return 0;
 }
 
+int reef_batch_run(reef_batch* b, int nsteps, long* iters)
+{
+    if(check_batch(b) || nsteps < 0) return fail(REEF_E_ARG, "bad argument");
+    Engine& e = *b->e;
+    int rc;
+    if((rc = e.prepare())) return eng(e, rc);
+    for(size_t q = 0; q < e.tanks.size(); ++q)
+        if(e.tanks[q].used && e.tanks[q].active && (rc = e.start_steps((int)q, nsteps))) return eng(e, rc);
+    if((rc = e.run_scheduled())) return eng(e, rc);
+    if(iters)
+        for(size_t q = 0; q < e.tanks.size(); ++q) iters[q] = e.tanks[q].used ? e.tanks[q].iters_last : 0;
+    return 0;
+}
+
 int reef_batch_sync(reef_batch* b)
 {
     if(check_batch(b)) return REEF_E_ARG;

@@ -1,6 +1,7 @@
 // Host side of the batched engine (engine.h): device memory pool, tank set-up, the device
-// tank / work-item tables, and the orchestration of the reference's operations on every
-// active tank (momentum_FC3, levelset_RK3, reini_RK3, pjm_corr + bicgstab_ijk).
+// tank table and the tank sets of the launches, and the orchestration of the reference's
+// operations (momentum_FC3, levelset_RK3, reini_RK3, pjm_corr + bicgstab_ijk) in lockstep
+// or by the iteration-level scheduler.
 #include "engine.h"
 
 #include <algorithm>
@@ -159,8 +160,8 @@ Engine::~Engine()
     for(hipEvent_t e : st_ev_)
         if(e) (void)hipEventDestroy(e);
     if(h_status_) (void)hipHostFree(h_status_);
+    if(d_status_) (void)hipFree(d_status_);
     if(d_tanks_) (void)hipFree(d_tanks_);
-    if(d_items_) (void)hipFree(d_items_);
     if(d_partials_) (void)hipFree(d_partials_);
     if(d_norms_) (void)hipFree(d_norms_);
 }
@@ -174,6 +175,7 @@ size_t Engine::tank_bytes(const reef_grid_desc& gd)
     for(int a = 0; a < 3; ++a)
         meta += (h.P[a].size() * 3 + h.qf[a].size() + h.cf[a].size() + h.isf[a].size()) * 8 + h.DN[a].size() * 4;
     b += up(meta) + (size_t)2 * 8 * (size_t)(h.nx + h.ny + h.nz) * 64;   // ghost programs (estimate)
+    b += up(sizeof(BiState));
     return b;
 }
 
@@ -263,6 +265,11 @@ int Engine::add_tank(const reef_grid_desc& gd, const reef_params& p, int* id)
     const size_t metabytes = up(meta.size() * 8);
     void* mb;
     if((rc = take(metabytes + inv.size() * 4 + ops.size() * sizeof(GhostOp) + 64, &mb))) return release(), rc;
+    // the solver state stays with the tank (iterations in flight survive table rebuilds)
+    void* stb;
+    if((rc = take(sizeof(BiState), &stb))) return release(), rc;
+    EHIP(hipMemsetAsync(stb, 0, sizeof(BiState), stream));
+    D.st = (BiState*)stb;
     EHIP(hipMemcpy(mb, meta.data(), meta.size() * 8, hipMemcpyHostToDevice));
     float* invd = (float*)((char*)mb + metabytes);
     EHIP(hipMemcpy(invd, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
@@ -288,7 +295,6 @@ int Engine::add_tank(const reef_grid_desc& gd, const reef_params& p, int* id)
         D.gnops[q] = nop[q];
     }
     D.shape = shape_of(h);
-    D.active = 1;
     D.dt = p.dt;
     // slot
     int slot = -1;
@@ -314,6 +320,8 @@ int Engine::remove_tank(int id)
     TankHost* t = tank(id);
     if(!t) return fail(REEF_E_ARG, "no such tank");
     EHIP(hipStreamSynchronize(stream));
+    int rc;
+    if((rc = drain())) return rc;
     for(void* m : t->mem) pool.free(m);
     *t = TankHost();
     dirty_ = true;
@@ -323,107 +331,125 @@ int Engine::remove_tank(int id)
 int Engine::active_count() const
 {
     int n = 0;
-    for(const TankHost& t : tanks) n += t.used && t.dev.active;
+    for(const TankHost& t : tanks) n += t.used && t.active;
     return n;
 }
 
 int Engine::set_active(const std::vector<int>& ids)
 {
     for(size_t q = 0; q < tanks.size(); ++q)
-        if(tanks[q].used) tanks[q].dev.active = ids.empty() ? 1 : 0;
+        if(tanks[q].used) tanks[q].active = ids.empty();
     for(int id : ids)
     {
         TankHost* t = tank(id);
         if(!t) return fail(REEF_E_ARG, "no such tank");
-        t->dev.active = 1;
+        t->active = true;
     }
     dirty_ = true;
     return 0;
 }
 
-// device tank table (every slot; unused or inactive slots have active = 0), work items
-// of the active tanks in slot order, per-slot solver states, launch sizes
+// device tank table (every slot), reduction slots of the used tanks, per-slot norms,
+// volumes and gathered solver states, and the tank sets of the active tanks
 int Engine::sync_tables()
 {
     if(!dirty_) return 0;
     EHIP(hipStreamSynchronize(stream));
-    drop_graphs();   // their launches carry the old tables, grids and parameters
+    int rc;
+    if((rc = drain())) return rc;
+    drop_graphs();   // their launches carry the old tables, sets and parameters
     const int nslot = (int)tanks.size();
     if(nslot > cap_tanks_ || !d_tanks_)
     {
-        const int cap = std::max(nslot, 8);
-        if(d_tanks_) hipFree(d_tanks_);
-        if(d_norms_) hipFree(d_norms_);
-        if(h_status_) hipHostFree(h_status_);
+        const int cap = std::max(2 * nslot, 8);
+        if(d_tanks_) (void)hipFree(d_tanks_);
+        if(d_norms_) (void)hipFree(d_norms_);
+        if(d_status_) (void)hipFree(d_status_);
+        if(h_status_) (void)hipHostFree(h_status_);
+        d_tanks_ = nullptr;
+        d_norms_ = nullptr;
+        d_status_ = nullptr;
+        h_status_ = nullptr;
         EHIP(hipMalloc(&d_tanks_, sizeof(TankDev) * cap));
-        // solver states, volumes and norms of every slot in one array
-        EHIP(hipMalloc(&d_norms_, (sizeof(BiState) + 4 * sizeof(double)) * cap));
-        EHIP(hipMemset(d_norms_, 0, (sizeof(BiState) + 4 * sizeof(double)) * cap));
+        // norms (1 per slot) and volumes (2 per slot)
+        EHIP(hipMalloc(&d_norms_, 3 * sizeof(double) * cap));
+        EHIP(hipMemset(d_norms_, 0, 3 * sizeof(double) * cap));
+        EHIP(hipMalloc(&d_status_, 2 * sizeof(BiState) * cap));
+        EHIP(hipMemset(d_status_, 0, 2 * sizeof(BiState) * cap));
         EHIP(hipHostMalloc((void**)&h_status_, 2 * sizeof(BiState) * cap, hipHostMallocDefault));
         std::memset(h_status_, 0, 2 * sizeof(BiState) * cap);
         cap_tanks_ = cap;
-        cap_status_ = cap;
     }
-    BiState* states = (BiState*)((char*)d_norms_ + 4 * sizeof(double) * cap_tanks_);
     double* vols = d_norms_ + cap_tanks_;
     std::vector<TankDev> host(nslot);
-    std::vector<int> items;
-    bool pw = false, direct = false;
+    size_t items = 0;
     for(int q = 0; q < nslot; ++q)
     {
         TankHost& t = tanks[q];
-        TankDev D{};
-        if(t.used)
-        {
-            D = t.dev;
-            D.st = states + q;
-            D.vol = vols + 2 * q;
-            D.item0 = (int)items.size();
-            if(D.active)
-            {
-                for(int it = 0; it < D.shape.nitems; ++it) items.push_back(q);
-                if(D.g.jdir == 1 && D.shape.ty == 4 && D.shape.tz == 1)
-                    pw = true;
-                else
-                    direct = true;
-            }
-            t.dev = D;
-        }
-        host[q] = D;
+        if(!t.used) continue;
+        t.dev.item0 = (int)items;
+        t.dev.vol = vols + 2 * q;
+        items += (size_t)t.dev.shape.nitems;
+        host[q] = t.dev;
     }
     if(!host.empty()) EHIP(hipMemcpy(d_tanks_, host.data(), sizeof(TankDev) * nslot, hipMemcpyHostToDevice));
-    nitems_ = (int)items.size();
-    if(nitems_ > cap_items_ || !d_items_)
+    if(4 * std::max(items, (size_t)64) > cap_partials_)
     {
-        const int cap = std::max(nitems_, 64);
-        if(d_items_) hipFree(d_items_);
-        if(d_partials_) hipFree(d_partials_);
-        EHIP(hipMalloc(&d_items_, sizeof(int) * cap));
-        EHIP(hipMalloc(&d_partials_, sizeof(double) * 4 * cap));
-        cap_items_ = cap;
+        if(d_partials_) (void)hipFree(d_partials_);
+        d_partials_ = nullptr;
+        cap_partials_ = 8 * std::max(items, (size_t)64);
+        EHIP(hipMalloc(&d_partials_, sizeof(double) * cap_partials_));
     }
-    if(nitems_) EHIP(hipMemcpy(d_items_, items.data(), sizeof(int) * nitems_, hipMemcpyHostToDevice));
     if(!st_ev_[0])
         for(hipEvent_t& e : st_ev_) EHIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    view_.tanks = d_tanks_;
-    view_.item_tank = d_items_;
-    view_.ntank = nslot;
-    view_.nitems = nitems_;
-    view_.partials = d_partials_;
-    view_.has_pw = pw ? 1 : 0;
-    view_.has_direct = direct ? 1 : 0;
-    auto grid_of = [&](int cap) {
-        int gsz = std::min(cap, (nitems_ + 7) / 8 * 8);
-        return std::max(gsz, 8);
-    };
-    grid_solver_ = grid_of(2048);
-    grid_simple_ = grid_of(2048);
-    grid_stencil_ = grid_of(1024);
-    last_state.assign(nslot, BiState{});
-    step_iters.assign(nslot, 0);
-    vol.assign(2 * (size_t)nslot, 0.0);
+    std::vector<int> act;
+    for(int q = 0; q < nslot; ++q)
+        if(tanks[q].used && tanks[q].active) act.push_back(q);
+    all_ = make_sets(act);
+    cur_ = &all_;
+    solving_key_.clear();
+    solving_sets_.clear();
+    last_state.resize(nslot, BiState{});
+    step_iters.resize(nslot, 0);
+    vol.resize(2 * (size_t)nslot, 0.0);
     dirty_ = false;
     return 0;
+}
+
+// the launch sets of the given slots, at most SET_MAX tanks each
+std::vector<TankSet> Engine::make_sets(const std::vector<int>& slots) const
+{
+    std::vector<TankSet> out;
+    for(size_t a = 0; a < slots.size(); a += SET_MAX)
+    {
+        TankSet S;
+        std::memset(&S, 0, sizeof(S));
+        S.tanks = d_tanks_;
+        S.partials = d_partials_;
+        const size_t b = std::min(slots.size(), a + (size_t)SET_MAX);
+        int nt = 0, ni = 0;
+        for(size_t q = a; q < b; ++q)
+        {
+            const TankHost& t = tanks[slots[q]];
+            const int p = (int)(q - a);
+            S.slot[p] = slots[q];
+            S.tile0[p] = nt;
+            S.item0[p] = ni;
+            nt += t.dev.shape.ntiles;
+            ni += t.dev.shape.nitems;
+            if(t.hg.jdir == 1 && t.dev.shape.ty == 4 && t.dev.shape.tz == 1)
+                S.has_pw = 1;
+            else
+                S.has_direct = 1;
+        }
+        S.n = (int)(b - a);
+        S.tile0[S.n] = nt;
+        S.item0[S.n] = ni;
+        S.ntiles = nt;
+        S.nitems = ni;
+        out.push_back(S);
+    }
+    return out;
 }
 
 // ------------------------------------------------------------------ fields
@@ -482,7 +508,7 @@ int Engine::zero(int field)
 {
     int rc;
     if((rc = sync_tables())) return rc;
-    bl_fill(view_, grid_simple_, field, 0.0, stream);
+    each([&](const TankSet& S) { bl_fill(S, field, 0.0, stream); });
     return 0;
 }
 
@@ -534,6 +560,13 @@ void Engine::tcollect()
 }
 
 // ------------------------------------------------------------------ operations
+const TankHost* Engine::first_used() const
+{
+    for(const TankHost& t : tanks)
+        if(t.used) return &t;
+    return nullptr;
+}
+
 int Engine::ghost(int field, int kind, int gcv)
 {
     const int q = prog_index(kind, gcv);
@@ -541,7 +574,7 @@ int Engine::ghost(int field, int kind, int gcv)
     int rc;
     if((rc = sync_tables())) return rc;
     tbegin("ghost");
-    bl_ghost(view_, 256, q, field, stream);
+    each([&](const TankSet& S) { bl_ghost(S, q, field, stream); });
     tend();
     return 0;
 }
@@ -553,12 +586,13 @@ int Engine::convection(int ipol, int q, int u, int v, int w)
     if(ipol == 4)
     {
         tbegin("hj");
-        bl_hj_stage(view_, grid_stencil_, 0, q, -1, u, v, w, REEF_L, 0.0, stream);
+        each([&](const TankSet& S) { bl_hj_stage(S, 0, q, -1, u, v, w, REEF_L, 0.0, stream); });
         tend();
         return 0;
     }
+    const int acc = ipol == 1 ? REEF_F : ipol == 2 ? REEF_G : REEF_H;
     tbegin("flux");
-    bl_flux_acc(view_, grid_stencil_, ipol, q, u, v, w, ipol == 1 ? REEF_F : ipol == 2 ? REEF_G : REEF_H, stream);
+    each([&](const TankSet& S) { bl_flux_acc(S, ipol, q, u, v, w, acc, stream); });
     tend();
     return 0;
 }
@@ -568,25 +602,20 @@ int Engine::reini(int f, int iters)
     int rc;
     if((rc = sync_tables())) return rc;
     // F43 is a control value shared by the batch (the first used tank's)
-    double f43 = 0.55;
-    for(const TankHost& t : tanks)
-        if(t.used)
-        {
-            f43 = t.prm.F43;
-            break;
-        }
+    const TankHost* t0 = first_used();
+    const double f43 = t0 ? t0->prm.F43 : 0.55;
     for(int q = 0; q < iters; ++q)
     {
         tbegin("reini");
-        bl_reini_stage(view_, grid_stencil_, 1, f, f, A_RK1, f43, fsf, stream);
+        each([&](const TankSet& S) { bl_reini_stage(S, 1, f, f, A_RK1, f43, fsf, stream); });
         tend();
         if((rc = ghost(A_RK1, 4, 52))) return rc;
         tbegin("reini");
-        bl_reini_stage(view_, grid_stencil_, 2, A_RK1, f, A_RK2, f43, fsf, stream);
+        each([&](const TankSet& S) { bl_reini_stage(S, 2, A_RK1, f, A_RK2, f43, fsf, stream); });
         tend();
         if((rc = ghost(A_RK2, 4, 52))) return rc;
         tbegin("reini");
-        bl_reini_stage(view_, grid_stencil_, 3, A_RK2, f, f, f43, fsf, stream);
+        each([&](const TankSet& S) { bl_reini_stage(S, 3, A_RK2, f, f, f43, fsf, stream); });
         tend();
         if((rc = ghost(f, 4, 52))) return rc;
     }
@@ -597,17 +626,12 @@ int Engine::fluid_update()
 {
     int rc;
     if((rc = sync_tables())) return rc;
-    const reef_params* p = nullptr;
-    for(const TankHost& t : tanks)
-        if(t.used)
-        {
-            p = &t.prm;
-            break;
-        }
-    if(!p) return 0;
+    const TankHost* t0 = first_used();
+    if(!t0) return 0;
+    const reef_params* p = &t0->prm;
     const double prm4[4] = {p->W1, p->W2, p->W3, p->W4};
     tbegin("heaviside");
-    bl_heaviside(view_, grid_simple_, prm4, stream);
+    each([&](const TankSet& S) { bl_heaviside(S, prm4, stream); });
     tend();
     if((rc = ghost(REEF_RO, 4, 1))) return rc;
     return ghost(REEF_VISC, 4, 1);
@@ -621,6 +645,21 @@ int Engine::read_volumes()
     EHIP(hipMemcpyAsync(vol.data(), d_norms_ + cap_tanks_, sizeof(double) * 2 * nslot, hipMemcpyDeviceToHost, stream));
     EHIP(hipStreamSynchronize(stream));
     return 0;
+}
+
+int Engine::prepare_tanks(const std::vector<int>& ids)
+{
+    int rc;
+    if((rc = sync_tables())) return rc;
+    for(int id : ids)
+        if(!tank(id)) return fail(REEF_E_ARG, "no such tank");
+    const std::vector<TankSet> sets = make_sets(ids);
+    cur_ = &sets;
+    if(!(rc = ghost(REEF_PHI, 4, 52)) && !(rc = ghost(REEF_U, 1, 10)) && !(rc = ghost(REEF_V, 2, 11)) &&
+       !(rc = ghost(REEF_W, 3, 12)) && !(rc = ghost(REEF_PRESS, 4, 40)))
+        rc = fluid_update();
+    cur_ = &all_;
+    return rc;
 }
 
 // ------------------------------------------------------------------ graphs
@@ -660,67 +699,76 @@ int Engine::replay(const GraphKey& key, Body body)
     return 0;
 }
 
-const TankHost* Engine::first_used() const
-{
-    for(const TankHost& t : tanks)
-        if(t.used) return &t;
-    return nullptr;
-}
+// ------------------------------------------------------------------ BiCGStab iterations
+static const char* const BICG_NAMES[5] = {"bicg_p", "bicg_v", "bicg_t", "bicg_x", ""};
+static const char* const MIX_NAMES[5] = {"mix_p", "bicg32_v", "bicg32_t", "bicg32_x", "mix_ru"};
 
-// BiCGStab iterations after the init kernels, enqueued in batches; the device states say
-// when every active tank has finished.  Two batches are in flight: batch k+1 is enqueued
-// before the host waits for the states copied out after batch k.  Iterations launched
-// after a tank's end are no-ops for it.
-template<class Launch>
-int Engine::run_iterations(const GraphKey& key, int nphase, const char* const* names, Launch launch, int maxiter)
+// 8 iterations of the tanks of the current sets, then their states gathered into ring
+// slot `ring` and copied to the host.  Iterations after a tank's end are no-ops for it.
+int Engine::enqueue_batch(bool mixed, int x, int rhs, int ring)
 {
-    const int batch = 8;
-    const int nslot = (int)tanks.size();
-    BiState* states = (BiState*)((char*)d_norms_ + 4 * sizeof(double) * cap_tanks_);
+    const int nph = mixed ? 5 : 4;
+    const char* const* names = mixed ? MIX_NAMES : BICG_NAMES;
     auto body = [&]() -> int {
-        for(int q = 0; q < batch; ++q)
-            for(int ph = 0; ph < nphase; ++ph)
-            {
-                tbegin(names[ph]);
-                launch(ph, 0);
-                tend();
-                launch(ph, 1);
-            }
-        return 0;
-    };
-    auto enqueue = [&](long k) -> int {
-        int rc;
-        if((rc = replay(key, body))) return rc;
-        EHIP(hipMemcpyAsync(h_status_ + (k & 1) * cap_status_, states, sizeof(BiState) * nslot,
-                            hipMemcpyDeviceToHost, stream));
-        EHIP(hipEventRecord(st_ev_[k & 1], stream));
+        for(int q = 0; q < 8; ++q)
+            for(int ph = 0; ph < nph; ++ph)
+                for(int part = 0; part < 2; ++part)
+                {
+                    if(part == 0) tbegin(names[ph]);
+                    each([&](const TankSet& S) {
+                        if(mixed)
+                            bl_mixed_phase(S, x, rhs, ph, part, stream);
+                        else
+                            bl_bicg_phase(S, x, rhs, ph, part, stream);
+                    });
+                    if(part == 0) tend();
+                }
         return 0;
     };
     int rc;
-    if((rc = enqueue(0))) return rc;
+    // the lockstep sets only change with the tables; the scheduler's change every round
+    if(cur_ == &all_)
+        rc = replay({1, mixed ? 1 : 0, x, rhs}, body);
+    else
+        rc = body();
+    if(rc) return rc;
+    BiState* dst = d_status_ + (size_t)ring * cap_tanks_;
+    each([&](const TankSet& S) { bl_gather_states(S, dst, stream); });
+    EHIP(hipMemcpyAsync(h_status_ + (size_t)ring * cap_tanks_, dst, sizeof(BiState) * tanks.size(),
+                        hipMemcpyDeviceToHost, stream));
+    EHIP(hipEventRecord(st_ev_[ring], stream));
+    return 0;
+}
+
+// BiCGStab iterations of the current sets until every tank of them has finished.  Two
+// batches are in flight: batch k+1 is enqueued before the host waits for the states of
+// batch k.
+int Engine::run_iterations(bool mixed, int x, int rhs, int maxiter)
+{
+    std::vector<int> members;
+    for(const TankSet& S : *cur_)
+        for(int p = 0; p < S.n; ++p) members.push_back(S.slot[p]);
+    int rc;
+    if((rc = enqueue_batch(mixed, x, rhs, 0))) return rc;
     for(long k = 1;; ++k)
     {
-        if((rc = enqueue(k))) return rc;
+        if((rc = enqueue_batch(mixed, x, rhs, (int)(k & 1)))) return rc;
         EHIP(hipEventSynchronize(st_ev_[(k - 1) & 1]));
-        const BiState* hs = h_status_ + ((k - 1) & 1) * cap_status_;
+        const BiState* hs = h_status_ + (size_t)((k - 1) & 1) * cap_tanks_;
         bool all = true;
-        for(int q = 0; q < nslot; ++q)
-            if(tanks[q].used && tanks[q].dev.active && !hs[q].done) all = false;
+        for(int q : members)
+            if(!hs[q].done) all = false;
         if(all)
         {
-            for(int q = 0; q < nslot; ++q)
-                if(tanks[q].used && tanks[q].dev.active) last_state[q] = hs[q];
+            for(int q : members) last_state[q] = hs[q];
             break;
         }
         if(timing && ranges_.size() > 2048) tcollect();
-        if(k * batch > 4L * (maxiter + 16)) return fail(REEF_E_HIP, "BiCGStab did not terminate");
+        if(k * 8 > 4L * (maxiter + 16)) return fail(REEF_E_HIP, "BiCGStab did not terminate");
     }
     if(timing) tcollect();
     return 0;
 }
-
-static const char* const BICG_NAMES[5] = {"bicg_p", "bicg_v", "bicg_t", "bicg_x", ""};
-static const char* const MIX_NAMES[5] = {"mix_p", "bicg32_v", "bicg32_t", "bicg32_x", "mix_ru"};
 
 // face densities and the solver's init kernels (FP64, or FP32 coefficients and the mixed
 // solver's init)
@@ -729,41 +777,32 @@ int Engine::solve_begin(int x, int rhs, double stop, int maxiter, bool mixed)
     const TankHost* t0 = first_used();
     const double rw = t0 ? t0->prm.W1 : 1000.0, ra = t0 ? t0->prm.W3 : 1.205;
     tbegin("face_rho");
-    bl_face_rho(view_, grid_simple_, rw, ra, mixed ? 0 : 1, stream);
+    each([&](const TankSet& S) { bl_face_rho(S, rw, ra, mixed ? 0 : 1, stream); });
     tend();
     if(!mixed)
     {
         tbegin("bicg_init");
-        bl_bicg_init(view_, grid_solver_, x, rhs, stop, maxiter, stream);
+        each([&](const TankSet& S) { bl_bicg_init(S, x, rhs, stop, maxiter, stream); });
         tend();
         return 0;
     }
     tbegin("coef32");
-    bl_coef32(view_, grid_simple_, stream);
+    each([&](const TankSet& S) { bl_coef32(S, stream); });
     tend();
     tbegin("mix_init");
-    bl_mixed_init(view_, grid_solver_, x, rhs, stop, maxiter, delta, stream);
+    each([&](const TankSet& S) { bl_mixed_init(S, x, rhs, stop, maxiter, delta, stream); });
     tend();
     return 0;
 }
 
-int Engine::solve_iterate(int x, int rhs, int maxiter, bool mixed)
-{
-    if(!mixed)
-    {
-        auto launch = [&](int ph, int part) { bl_bicg_phase(view_, grid_solver_, x, rhs, ph, part, stream); };
-        return run_iterations({1, 0, x, rhs}, 4, BICG_NAMES, launch, maxiter);
-    }
-    auto launch = [&](int ph, int part) { bl_mixed_phase(view_, grid_solver_, x, rhs, ph, part, stream); };
-    return run_iterations({1, 1, x, rhs}, 5, MIX_NAMES, launch, maxiter);
-}
+int Engine::solve_iterate(int x, int rhs, int maxiter, bool mixed) { return run_iterations(mixed, x, rhs, maxiter); }
 
 // the mixed solve keeps its FP64 iterate split until the end: x64 += x32
 int Engine::solve_end(int x, bool mixed)
 {
     if(!mixed) return 0;
     tbegin("add32");
-    bl_add32(view_, grid_simple_, x, stream);
+    each([&](const TankSet& S) { bl_add32(S, x, stream); });
     tend();
     return 0;
 }
@@ -773,7 +812,7 @@ int Engine::solve_end(int x, bool mixed)
 int Engine::solve(int x, int rhs, double stop, int maxiter, bool mixed)
 {
     int rc;
-    if((rc = sync_tables())) return rc;
+    if((rc = sync_tables()) || (rc = drain())) return rc;
     if((rc = solve_begin(x, rhs, stop, maxiter, mixed))) return rc;
     if((rc = solve_iterate(x, rhs, maxiter, mixed))) return rc;
     return solve_end(x, mixed);
@@ -785,8 +824,10 @@ int Engine::residual(int x, int rhs, std::vector<double>& norms)
     if((rc = sync_tables())) return rc;
     const TankHost* t0 = first_used();
     const double rw = t0 ? t0->prm.W1 : 1000.0, ra = t0 ? t0->prm.W3 : 1.205;
-    bl_face_rho(view_, grid_simple_, rw, ra, 0, stream);
-    bl_residual(view_, grid_solver_, x, rhs, d_norms_, stream);
+    each([&](const TankSet& S) {
+        bl_face_rho(S, rw, ra, 0, stream);
+        bl_residual(S, x, rhs, d_norms_, stream);
+    });
     norms.assign(tanks.size(), 0.0);
     EHIP(hipMemcpyAsync(norms.data(), d_norms_, sizeof(double) * tanks.size(), hipMemcpyDeviceToHost, stream));
     EHIP(hipStreamSynchronize(stream));
@@ -802,9 +843,11 @@ int Engine::pjm_pre(int u, int v, int w, double alpha, bool mixed)
     const TankHost* t0 = first_used();
     if(!t0) return 0;
     tbegin("poisson_rhs");
-    bl_poisson_rhs(view_, grid_simple_, u, v, w, alpha, stream);
+    each([&](const TankSet& S) {
+        bl_poisson_rhs(S, u, v, w, alpha, stream);
+        bl_fill(S, REEF_PCORR, 0.0, stream);
+    });
     tend();
-    bl_fill(view_, grid_simple_, REEF_PCORR, 0.0, stream);
     return solve_begin(REEF_PCORR, A_RHS, t0->prm.N44, t0->prm.N46, mixed);
 }
 
@@ -817,7 +860,7 @@ int Engine::pjm_post(int u, int v, int w, double alpha, bool mixed)
     if((rc = solve_end(REEF_PCORR, mixed))) return rc;
     if((rc = ghost(REEF_PCORR, 4, 40))) return rc;
     tbegin("correct");
-    bl_correct(view_, grid_simple_, t0->prm.W1, t0->prm.W3, alpha, u, v, w, stream);
+    each([&](const TankSet& S) { bl_correct(S, t0->prm.W1, t0->prm.W3, alpha, u, v, w, stream); });
     tend();
     return ghost(REEF_PRESS, 4, 40);
 }
@@ -826,14 +869,14 @@ int Engine::pjm_post(int u, int v, int w, double alpha, bool mixed)
 int Engine::pjm(int u, int v, int w, double alpha)
 {
     int rc;
-    if((rc = sync_tables())) return rc;
+    if((rc = sync_tables()) || (rc = drain())) return rc;
     const TankHost* t0 = first_used();
     const bool mixed = precision == REEF_SOLVER_MIXED;
     if((rc = pjm_pre(u, v, w, alpha, mixed))) return rc;
     if(!t0) return 0;
     if((rc = solve_iterate(REEF_PCORR, A_RHS, t0->prm.N46, mixed))) return rc;
-    for(size_t q = 0; q < tanks.size(); ++q)
-        if(tanks[q].used && tanks[q].dev.active) step_iters[q] += last_state[q].iter;
+    for(const TankSet& S : all_)
+        for(int p = 0; p < S.n; ++p) step_iters[S.slot[p]] += last_state[S.slot[p]].iter;
     return pjm_post(u, v, w, alpha, mixed);
 }
 
@@ -848,8 +891,10 @@ int Engine::momentum_stage(int stage, const int qin[3], const int qold[3], const
     for(int comp = 1; comp <= 3; ++comp)
     {
         tbegin("momentum");
-        bl_momentum(view_, grid_stencil_, comp, stage, qin[comp - 1], qold[comp - 1], vel[0], vel[1], vel[2],
-                    grav[comp - 1], cfac, out[comp - 1], t0->prm.W1, t0->prm.W3, stream);
+        each([&](const TankSet& S) {
+            bl_momentum(S, comp, stage, qin[comp - 1], qold[comp - 1], vel[0], vel[1], vel[2], grav[comp - 1], cfac,
+                        out[comp - 1], t0->prm.W1, t0->prm.W3, stream);
+        });
         tend();
     }
     const int kinds[3] = {1, 2, 3}, gcvs[3] = {10, 11, 12};
@@ -879,15 +924,15 @@ int Engine::fc3_segment(int s, bool mixed)
     {
         // stage 1 (momentum_FC3.cpp:132-230)
         tbegin("copy");
-        bl_copy(view_, grid_simple_, REEF_PHI, A_LS, stream);   // ls ghosts stay zero, as in the reference
+        each([&](const TankSet& S) { bl_copy(S, REEF_PHI, A_LS, stream); });   // ls ghosts stay zero, as in the reference
         tend();
         tbegin("hj");
-        bl_hj_stage(view_, grid_stencil_, 1, A_LS, -1, REEF_U, REEF_V, REEF_W, A_FRK1, 1.0, stream);
+        each([&](const TankSet& S) { bl_hj_stage(S, 1, A_LS, -1, REEF_U, REEF_V, REEF_W, A_FRK1, 1.0, stream); });
         tend();
         if((rc = ghost(A_FRK1, 4, 52))) return rc;
         if((rc = reini(A_FRK1, 2))) return rc;
         tbegin("copy");
-        bl_copy(view_, grid_simple_, A_FRK1, REEF_PHI, stream);
+        each([&](const TankSet& S) { bl_copy(S, A_FRK1, REEF_PHI, stream); });
         tend();
         if((rc = ghost(REEF_PHI, 4, 52))) return rc;
         const int qin[3] = {REEF_U, REEF_V, REEF_W};
@@ -897,12 +942,12 @@ int Engine::fc3_segment(int s, bool mixed)
     {
         // stage 2 (momentum_FC3.cpp:232-332)
         tbegin("hj");
-        bl_hj_stage(view_, grid_stencil_, 2, A_FRK1, A_LS, A_URK1, A_VRK1, A_WRK1, A_FRK2, 0.25, stream);
+        each([&](const TankSet& S) { bl_hj_stage(S, 2, A_FRK1, A_LS, A_URK1, A_VRK1, A_WRK1, A_FRK2, 0.25, stream); });
         tend();
         if((rc = ghost(A_FRK2, 4, 52))) return rc;
         if((rc = reini(A_FRK2, 2))) return rc;
         tbegin("copy");
-        bl_copy(view_, grid_simple_, A_FRK2, REEF_PHI, stream);
+        each([&](const TankSet& S) { bl_copy(S, A_FRK2, REEF_PHI, stream); });
         tend();
         if((rc = ghost(REEF_PHI, 4, 52))) return rc;
         if((rc = reini(REEF_PHI, 2))) return rc;
@@ -913,7 +958,9 @@ int Engine::fc3_segment(int s, bool mixed)
     {
         // stage 3 (momentum_FC3.cpp:334-430)
         tbegin("hj");
-        bl_hj_stage(view_, grid_stencil_, 3, A_FRK2, A_LS, A_URK2, A_VRK2, A_WRK2, REEF_PHI, 2.0 / 3.0, stream);
+        each([&](const TankSet& S) {
+            bl_hj_stage(S, 3, A_FRK2, A_LS, A_URK2, A_VRK2, A_WRK2, REEF_PHI, 2.0 / 3.0, stream);
+        });
         tend();
         if((rc = ghost(REEF_PHI, 4, 52))) return rc;
         if((rc = reini(REEF_PHI, 3))) return rc;
@@ -930,7 +977,7 @@ int Engine::fc3_segment(int s, bool mixed)
 int Engine::fc3_step()
 {
     int rc;
-    if((rc = sync_tables())) return rc;
+    if((rc = sync_tables()) || (rc = drain())) return rc;
     std::fill(step_iters.begin(), step_iters.end(), 0L);
     const TankHost* t0 = first_used();
     if(!t0) return 0;
@@ -940,9 +987,139 @@ int Engine::fc3_step()
         if((rc = replay({0, s, mixed ? 1 : 0, fsf}, [&] { return fc3_segment(s, mixed); }))) return rc;
         if(s == 3) break;
         if((rc = solve_iterate(REEF_PCORR, A_RHS, t0->prm.N46, mixed))) return rc;
-        for(size_t q = 0; q < tanks.size(); ++q)
-            if(tanks[q].used && tanks[q].dev.active) step_iters[q] += last_state[q].iter;
+        for(const TankSet& S : all_)
+            for(int p = 0; p < S.n; ++p) step_iters[S.slot[p]] += last_state[S.slot[p]].iter;
     }
+    return 0;
+}
+
+// ------------------------------------------------------------------ iteration-level scheduling
+int Engine::start_steps(int id, int n)
+{
+    TankHost* t = tank(id);
+    if(!t || n < 0) return fail(REEF_E_ARG, "bad tank or step count");
+    t->steps_left += n;
+    if(t->pc < 0 && t->steps_left > 0) t->pc = 0;
+    return 0;
+}
+
+bool Engine::busy() const
+{
+    if(!inflight_.empty()) return true;
+    for(const TankHost& t : tanks)
+        if(t.used && t.pc >= 0) return true;
+    return false;
+}
+
+// states of an iteration batch: a tank whose solve (the one the batch ran) has ended moves
+// on to its next segment
+int Engine::read_batch(const Inflight& f)
+{
+    EHIP(hipEventSynchronize(st_ev_[f.ring]));
+    const BiState* hs = h_status_ + (size_t)f.ring * cap_tanks_;
+    for(const auto& m : f.members)
+    {
+        TankHost& t = tanks[m.first];
+        if(!t.used || t.seq != m.second || (t.pc & 1) == 0) continue;
+        const BiState& st = hs[m.first];
+        if(!st.done) continue;
+        t.iters_step += st.iter;
+        last_state[m.first] = st;
+        t.pc += 1;
+    }
+    return 0;
+}
+
+int Engine::drain()
+{
+    int rc;
+    while(!inflight_.empty())
+    {
+        const Inflight f = std::move(inflight_.front());
+        inflight_.pop_front();
+        if((rc = read_batch(f))) return rc;
+    }
+    return 0;
+}
+
+// One round: the segments the tanks have reached (grouped by segment, in the order 3, 0,
+// 1, 2 so that a tank that ends a step starts the next one in the same round), then one
+// batch of 8 iterations for the tanks that are solving.  The states of the batch before
+// are read while this one runs.
+int Engine::pump(std::vector<int>& stepped)
+{
+    stepped.clear();
+    int rc;
+    if((rc = sync_tables())) return rc;
+    if(!first_used()) return 0;
+    const bool mixed = precision == REEF_SOLVER_MIXED;
+    static const int ORDER[4] = {3, 0, 1, 2};
+    for(int s : ORDER)
+    {
+        std::vector<int> group;
+        for(size_t q = 0; q < tanks.size(); ++q)
+            if(tanks[q].used && tanks[q].pc == 2 * s) group.push_back((int)q);
+        if(group.empty()) continue;
+        const std::vector<TankSet> sets = make_sets(group);
+        cur_ = &sets;
+        rc = fc3_segment(s, mixed);
+        cur_ = &all_;
+        if(rc) return rc;
+        for(int q : group)
+        {
+            TankHost& t = tanks[q];
+            if(s < 3)
+            {
+                t.pc = 2 * s + 1;
+                t.seq += 1;
+                continue;
+            }
+            t.steps_done += 1;
+            t.steps_left -= 1;
+            t.iters_last = t.iters_step;
+            t.iters_step = 0;
+            step_iters[q] = t.iters_last;
+            stepped.push_back(q);
+            t.pc = t.steps_left > 0 ? 0 : -1;
+        }
+    }
+    std::vector<int> solving;
+    for(size_t q = 0; q < tanks.size(); ++q)
+        if(tanks[q].used && (tanks[q].pc & 1) == 1) solving.push_back((int)q);
+    if(!solving.empty())
+    {
+        if(solving != solving_key_)
+        {
+            solving_sets_ = make_sets(solving);
+            solving_key_ = solving;
+        }
+        const int ring = (int)(batch_no_++ & 1);
+        cur_ = &solving_sets_;
+        rc = enqueue_batch(mixed, REEF_PCORR, A_RHS, ring);
+        cur_ = &all_;
+        if(rc) return rc;
+        Inflight f;
+        f.ring = ring;
+        for(int q : solving) f.members.push_back({q, tanks[q].seq});
+        inflight_.push_back(std::move(f));
+    }
+    if(inflight_.size() > 1 || (solving.empty() && !inflight_.empty()))
+    {
+        const Inflight f = std::move(inflight_.front());
+        inflight_.pop_front();
+        if((rc = read_batch(f))) return rc;
+    }
+    if(timing && ranges_.size() > 2048) tcollect();
+    return 0;
+}
+
+int Engine::run_scheduled()
+{
+    std::vector<int> stepped;
+    int rc;
+    while(busy())
+        if((rc = pump(stepped))) return rc;
+    if(timing) tcollect();
     return 0;
 }
 
@@ -952,15 +1129,17 @@ int Engine::phi_rk3()
     int rc;
     if((rc = sync_tables())) return rc;
     tbegin("hj");
-    bl_hj_stage(view_, grid_stencil_, 1, REEF_PHI, -1, REEF_U, REEF_V, REEF_W, A_FRK1, 1.0, stream);
+    each([&](const TankSet& S) { bl_hj_stage(S, 1, REEF_PHI, -1, REEF_U, REEF_V, REEF_W, A_FRK1, 1.0, stream); });
     tend();
     if((rc = ghost(A_FRK1, 4, 52))) return rc;
     tbegin("hj");
-    bl_hj_stage(view_, grid_stencil_, 2, A_FRK1, REEF_PHI, REEF_U, REEF_V, REEF_W, A_FRK2, 0.25, stream);
+    each([&](const TankSet& S) { bl_hj_stage(S, 2, A_FRK1, REEF_PHI, REEF_U, REEF_V, REEF_W, A_FRK2, 0.25, stream); });
     tend();
     if((rc = ghost(A_FRK2, 4, 52))) return rc;
     tbegin("hj");
-    bl_hj_stage(view_, grid_stencil_, 3, A_FRK2, REEF_PHI, REEF_U, REEF_V, REEF_W, REEF_PHI, 2.0 / 3.0, stream);
+    each([&](const TankSet& S) {
+        bl_hj_stage(S, 3, A_FRK2, REEF_PHI, REEF_U, REEF_V, REEF_W, REEF_PHI, 2.0 / 3.0, stream);
+    });
     tend();
     return ghost(REEF_PHI, 4, 52);
 }
@@ -971,24 +1150,18 @@ int Engine::vortex_step(double simtime)
     int rc;
     if((rc = phi_rk3())) return rc;
     if((rc = reini(REEF_PHI, 3))) return rc;
-    const TankHost* t0 = nullptr;
-    for(const TankHost& t : tanks)
-        if(t.used)
-        {
-            t0 = &t;
-            break;
-        }
+    const TankHost* t0 = first_used();
     if(!t0) return 0;
     const double grav[3] = {t0->prm.W20, t0->prm.W21, t0->prm.W22};
     for(int comp = 1; comp <= 3; ++comp)
     {
         if(comp == 2 && t0->hg.jdir == 0) continue;
         const double Fv = 0.0 + (0.0 + grav[comp - 1] + 0.0 + 0.0) * 1.0;
-        bl_void_rk3(view_, grid_simple_, comp, REEF_U + comp - 1, Fv, stream);
+        each([&](const TankSet& S) { bl_void_rk3(S, comp, REEF_U + comp - 1, Fv, stream); });
         if((rc = ghost(REEF_U + comp - 1, comp, 9 + comp))) return rc;
     }
     for(int comp = 1; comp <= 3; ++comp)
-        bl_vortex_velocity(view_, grid_simple_, comp, REEF_U + comp - 1, simtime, t0->hg.DXM, stream);
+        each([&](const TankSet& S) { bl_vortex_velocity(S, comp, REEF_U + comp - 1, simtime, t0->hg.DXM, stream); });
     if((rc = ghost(REEF_U, 1, 10))) return rc;
     if((rc = ghost(REEF_V, 2, 11))) return rc;
     if((rc = ghost(REEF_V, 2, 12))) return rc;
@@ -1018,23 +1191,29 @@ int Engine::probe(double& stencil_ms, double& solver_ms)
     for(Ev& x : ev) EHIP(hipEventCreate(&x.e));
     const bool mixed = precision == REEF_SOLVER_MIXED;
     EHIP(hipEventRecord(ev[0].e, stream));
-    bl_copy(view_, grid_simple_, REEF_PHI, A_LS, stream);
-    bl_hj_stage(view_, grid_stencil_, 1, A_LS, -1, REEF_U, REEF_V, REEF_W, A_FRK1, 1.0, stream);
+    each([&](const TankSet& S) {
+        bl_copy(S, REEF_PHI, A_LS, stream);
+        bl_hj_stage(S, 1, A_LS, -1, REEF_U, REEF_V, REEF_W, A_FRK1, 1.0, stream);
+    });
     if((rc = ghost(A_FRK1, 4, 52)) || (rc = reini(A_FRK1, 1))) return rc;
     const int qin[3] = {REEF_U, REEF_V, REEF_W}, out[3] = {A_URK1, A_VRK1, A_WRK1};
     if((rc = momentum_stage(1, qin, qin, qin, out))) return rc;
     EHIP(hipEventRecord(ev[1].e, stream));
     if((rc = ghost(A_URK1, 1, 7)) || (rc = ghost(A_VRK1, 2, 8)) || (rc = ghost(A_WRK1, 3, 9))) return rc;
-    bl_poisson_rhs(view_, grid_simple_, A_URK1, A_VRK1, A_WRK1, 1.0, stream);
-    bl_fill(view_, grid_simple_, REEF_PCORR, 0.0, stream);
+    each([&](const TankSet& S) {
+        bl_poisson_rhs(S, A_URK1, A_VRK1, A_WRK1, 1.0, stream);
+        bl_fill(S, REEF_PCORR, 0.0, stream);
+    });
     if((rc = solve_begin(REEF_PCORR, A_RHS, 1e-300, 16, mixed))) return rc;
     for(int it = 0; it < 16; ++it)
         for(int ph = 0; ph < (mixed ? 5 : 4); ++ph)
             for(int part = 0; part < 2; ++part)
-                if(mixed)
-                    bl_mixed_phase(view_, grid_solver_, REEF_PCORR, A_RHS, ph, part, stream);
-                else
-                    bl_bicg_phase(view_, grid_solver_, REEF_PCORR, A_RHS, ph, part, stream);
+                each([&](const TankSet& S) {
+                    if(mixed)
+                        bl_mixed_phase(S, REEF_PCORR, A_RHS, ph, part, stream);
+                    else
+                        bl_bicg_phase(S, REEF_PCORR, A_RHS, ph, part, stream);
+                });
     if((rc = solve_end(REEF_PCORR, mixed))) return rc;
     EHIP(hipEventRecord(ev[2].e, stream));
     EHIP(hipEventSynchronize(ev[2].e));
@@ -1059,7 +1238,7 @@ int Engine::autotune(int reps, std::vector<TuneRecord>* log)
     std::map<Bucket, std::vector<int>> members;
     std::vector<int> was_active;
     for(size_t q = 0; q < tanks.size(); ++q)
-        if(tanks[q].used && tanks[q].dev.active)
+        if(tanks[q].used && tanks[q].active)
         {
             was_active.push_back((int)q);
             if(!plans.count(bucket_of(tanks[q].hg))) members[bucket_of(tanks[q].hg)].push_back((int)q);

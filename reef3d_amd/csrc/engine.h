@@ -1,13 +1,23 @@
 // Host side of the batched engine: tanks (box domains) live in a device memory pool, the
-// engine keeps the device tank table and work-item table, plans the launches and runs
+// engine keeps the device tank table, builds the tank sets its launches cover and runs
 // the reference's operations (ghost fills, convection, reinit, fluid update, projection
-// and BiCGStab, momentum_FC3 and level-set steps) on all active tanks at once.  The
-// single-domain C-ABI context is an engine with one tank.
+// and BiCGStab, momentum_FC3 and level-set steps) on the active tanks.  The single-domain
+// C-ABI context is an engine with one tank.
+//
+// Two ways to advance momentum_FC3 steps:
+//   fc3_step(): every active tank by one step in lockstep (each pressure solve runs until
+//     the slowest tank has converged); its launch sequences can be replayed as hipGraphs;
+//   start_steps() + pump(): iteration-level scheduling.  Each tank moves through its own
+//     steps: a round launches the step segment each tank has reached (tanks grouped by
+//     segment) and one batch of BiCGStab iterations for the tanks that are solving, so a
+//     tank whose solve has converged goes on without waiting for the others.
+// A tank's arithmetic is the same either way, alone or with any other tanks.
 #pragma once
 
 #include "batch.h"
 
 #include <array>
+#include <deque>
 #include <map>
 #include <string>
 #include <vector>
@@ -41,12 +51,20 @@ private:
 struct TankHost
 {
     bool used = false;
+    bool active = true;             // covered by the launches of the whole batch
     HostGrid hg;
     reef_params prm{};
     TankDev dev{};                  // host copy of the table entry
     std::vector<void*> mem;         // pool blocks
     size_t bytes = 0;               // device bytes of the tank
     std::vector<double> staging;
+    // iteration-level scheduling (start_steps / pump)
+    int pc = -1;                    // -1 idle; 0, 2, 4, 6: segment pc / 2 next; 1, 3, 5: solving
+    int seq = 0;                    // solves started
+    int steps_left = 0;
+    int steps_done = 0;
+    long iters_step = 0;            // iterations of the step in progress
+    long iters_last = 0;            // iterations of the last completed step
 };
 
 class Engine
@@ -61,6 +79,12 @@ public:
     int fsf = 0;                    // reinidisc_fsf (F49 = 0) for every tank
     int precision = REEF_SOLVER_FP64;
     double delta = 0.1;             // reliable-update threshold of the mixed solve
+    // replay fc3_step's launch sequences (the four segments between its pressure solves)
+    // and the BiCGStab iteration batches as hipGraphs, captured on first use; the graphs
+    // are dropped whenever the tank tables or the controls change.  Launch-by-launch while
+    // timing is on.
+    bool graphs = false;
+    void drop_graphs();
     // tile plans per size bucket (jdir, floor(log2(nx ny nz))): tile rows ty x tz and
     // chunk length cpt; a bucket without an entry uses the default plan.  A tank takes the
     // plan of its bucket when it is added (and when its bucket's plan is set), so its
@@ -84,12 +108,6 @@ public:
         double score_ms;
     };
     int autotune(int reps, std::vector<TuneRecord>* log);
-    // replay fc3_step's launch sequences (the four segments between its pressure solves)
-    // and the BiCGStab iteration batches as hipGraphs, captured on first use; the graphs
-    // are dropped whenever the tank tables or the controls change.  Launch-by-launch while
-    // timing is on.
-    bool graphs = false;
-    void drop_graphs();
 
     // ---- tanks
     static size_t tank_bytes(const reef_grid_desc& g);
@@ -119,6 +137,16 @@ public:
     int fc3_step();
     int phi_rk3();
     int vortex_step(double simtime);
+    // the ghost fills and fluid update a run starts with, on the given tanks only
+    int prepare_tanks(const std::vector<int>& ids);
+
+    // ---- iteration-level scheduling of momentum_FC3 steps
+    int start_steps(int id, int n);
+    // one round; stepped receives the slots that completed a step in it
+    int pump(std::vector<int>& stepped);
+    bool busy() const;
+    // pump() until no tank has steps left
+    int run_scheduled();
 
     // results of the last operation per tank slot
     std::vector<BiState> last_state;       // solver state after the last solve
@@ -148,30 +176,47 @@ private:
     // device tables
     TankDev* d_tanks_ = nullptr;
     int cap_tanks_ = 0;
-    int* d_items_ = nullptr;
-    int cap_items_ = 0;
     double* d_partials_ = nullptr;
-    double* d_norms_ = nullptr;
-    BiState* h_status_ = nullptr;          // pinned, 2 slots x tanks
-    int cap_status_ = 0;
+    size_t cap_partials_ = 0;
+    double* d_norms_ = nullptr;            // per slot
+    BiState* d_status_ = nullptr;          // gathered states, 2 ring slots x cap_tanks_
+    BiState* h_status_ = nullptr;          // pinned mirror
     hipEvent_t st_ev_[2] = {nullptr, nullptr};
+    long batch_no_ = 0;
     bool dirty_ = true;
-    BatchView view_{};
-    int nitems_ = 0;
-    int grid_solver_ = 8, grid_stencil_ = 8, grid_simple_ = 8;
+    std::vector<TankSet> all_;             // the active tanks (chunks of SET_MAX)
+    const std::vector<TankSet>* cur_ = &all_;
+
+    // iteration batches in flight (pump)
+    struct Inflight
+    {
+        int ring;
+        std::vector<std::pair<int, int>> members;   // (slot, seq)
+    };
+    std::deque<Inflight> inflight_;
+    std::vector<int> solving_key_;
+    std::vector<TankSet> solving_sets_;
 
     int sync_tables();
+    std::vector<TankSet> make_sets(const std::vector<int>& slots) const;
     int fail(int code, const std::string& m)
     {
         err = m;
         return code;
     }
+    template<class F>
+    void each(F f)
+    {
+        for(const TankSet& S : *cur_) f(S);
+    }
     using GraphKey = std::array<int, 4>;
     std::map<GraphKey, hipGraphExec_t> graphs_;
     template<class Body>
     int replay(const GraphKey& key, Body body);
-    template<class Launch>
-    int run_iterations(const GraphKey& key, int nphase, const char* const* names, Launch launch, int maxiter);
+    int enqueue_batch(bool mixed, int x, int rhs, int ring);
+    int drain();
+    int read_batch(const Inflight& b);
+    int run_iterations(bool mixed, int x, int rhs, int maxiter);
     const TankHost* first_used() const;
     int solve_begin(int x, int rhs, double stop, int maxiter, bool mixed);
     int solve_iterate(int x, int rhs, int maxiter, bool mixed);

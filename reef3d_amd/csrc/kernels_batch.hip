@@ -1,36 +1,24 @@
-// Kernels of the batched engine.  Every launch is a fixed grid of blocks (a multiple of 8)
-// that walks the work items of a batch: block b belongs to XCD group b % 8 and takes the
-// items of that group's eighth of the item range in turn, so neighbouring chunks of a
-// tank share an L2.  An item is a run of consecutive tiles (z fastest) of one tank; the
-// tank's grid, fields, time step and solver state come from its TankDev.  Reductions
-// produce one partial per item and a per-tank finalize block adds a tank's partials in
-// item order, so a tank's arithmetic does not depend on the other tanks of the batch.
+// Kernels of the batched engine.  A launch covers a set of tanks (TankSet, by value).
+// Stencil and elementwise kernels run one 256-cell tile per block; the tiles of the set
+// are numbered tank by tank, z fastest, and block b takes tile (b % 8) * per + b / 8, so
+// the blocks the hardware sends to XCD x (b % 8 == x) work on one contiguous eighth of
+// the tiles and share that XCD's L2 for their halos.  Reductions run a persistent grid
+// (a multiple of 8 blocks) over the set's work items (runs of cpt tiles of one tank):
+// block group x walks the x-th eighth of the item range.  Every reduction writes one
+// partial per work item and a per-tank finalize block adds a tank's partials in item
+// order, so a tank's arithmetic does not depend on the other tanks of a launch.
 //
 // The per-cell arithmetic is cell_ops.h's (the reference's operation order), so the
-// stencil results are bit-identical to the reference whatever the batch.
+// stencil results are bit-identical to the reference whatever the set.
 #include "batch.h"
 #include "cell_ops.h"
+
+#include <algorithm>
 
 namespace reef {
 namespace bk {
 
 // ------------------------------------------------------------------ work walk
-struct ItemIter
-{
-    int it, hi, step;
-};
-
-__device__ __forceinline__ ItemIter block_items(const BatchView& b)
-{
-    int lo, hi;
-    group_range(b.nitems, (int)(blockIdx.x & 7), lo, hi);
-    ItemIter q;
-    q.it = lo + (int)(blockIdx.x >> 3);
-    q.hi = hi;
-    q.step = (int)(gridDim.x >> 3);
-    return q;
-}
-
 // The tank table is never written by a kernel: reading it through the constant address
 // space lets the compiler use scalar loads (the tank's pointers and sizes stay in SGPRs).
 // Kernels take the tank by reference there and copy only its DevGrid: indexing a local
@@ -40,30 +28,101 @@ __device__ __forceinline__ ItemIter block_items(const BatchView& b)
 // address space)
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef __attribute__((address_space(4))) const TankDev CTank;
-typedef __attribute__((address_space(4))) const int CInt;
 #else
 typedef const TankDev CTank;
-typedef const int CInt;
 #endif
 
-__device__ __forceinline__ CTank& tank_at(const BatchView& b, int tk) { return ((CTank*)b.tanks)[tk]; }
-__device__ __forceinline__ CTank& item_tank_at(const BatchView& b, int it)
+// index p with a[p] <= g < a[p + 1] in a set's prefix array a[0..n]
+__device__ __forceinline__ int set_find(const int* a, int n, int g)
 {
-    return ((CTank*)b.tanks)[((CInt*)b.item_tank)[it]];
+    int lo = 0, hi = n;
+    while(hi - lo > 1)
+    {
+        const int mid = (lo + hi) >> 1;
+        if(a[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
 }
 
+__device__ __forceinline__ CTank& tank_at(const TankSet& b, int p) { return ((CTank*)b.tanks)[b.slot[p]]; }
+
+// persistent walk over the set's work items: q the item, p its tank in the set
+struct ItemIter
+{
+    int q, hi, step, p;
+};
+
+__device__ __forceinline__ ItemIter set_items(const TankSet& b)
+{
+    int lo, hi;
+    group_range(b.nitems, (int)(blockIdx.x & 7), lo, hi);
+    ItemIter it;
+    it.q = lo + (int)(blockIdx.x >> 3);
+    it.hi = hi;
+    it.step = (int)(gridDim.x >> 3);
+    it.p = it.q < hi ? set_find(b.item0, b.n, it.q) : 0;
+    return it;
+}
+
+__device__ __forceinline__ void next_item(ItemIter& it, const TankSet& b)
+{
+    it.q += it.step;
+    if(it.q < it.hi)
+        while(b.item0[it.p + 1] <= it.q) ++it.p;
+}
+
+// the one tile of a block of a per-tile launch
+struct TileIter
+{
+    bool ok;
+    int p, t;
+};
+
+__device__ __forceinline__ TileIter set_tile(const TankSet& b)
+{
+    TileIter it;
+    const int per = (b.ntiles + 7) >> 3;
+    const int g = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    it.ok = g < b.ntiles;
+    it.p = it.ok ? set_find(b.tile0, b.n, g) : 0;
+    it.t = it.ok ? g - b.tile0[it.p] : 0;
+    return it;
+}
+
+// tiles [t, t1) of the walk; gi = reduction slot of an item walk
 struct TileWalk
 {
-    int t, t1;
+    int t, t1, gi;
     TileShape s;
 };
 
-__device__ __forceinline__ TileWalk item_walk(CTank& T, int it)
+__device__ __forceinline__ TileWalk item_walk(CTank& T, const TankSet& b, const ItemIter& q)
+{
+    const int li = q.q - b.item0[q.p];
+    TileWalk w;
+    w.s = T.shape;
+    w.t = li * T.shape.cpt;
+    w.t1 = min(w.t + T.shape.cpt, T.shape.ntiles);
+    w.gi = T.item0 + li;
+    return w;
+}
+
+// reduction slot of the current item of a persistent walk
+__device__ __forceinline__ int item_slot(CTank& T, const TankSet& b, const ItemIter& q)
+{
+    return T.item0 + (q.q - b.item0[q.p]);
+}
+
+__device__ __forceinline__ TileWalk tile_walk(CTank& T, int t)
 {
     TileWalk w;
     w.s = T.shape;
-    w.t = (it - T.item0) * T.shape.cpt;
-    w.t1 = min(w.t + T.shape.cpt, T.shape.ntiles);
+    w.t = t;
+    w.t1 = t + 1;
+    w.gi = -1;
     return w;
 }
 
@@ -92,7 +151,7 @@ __device__ __forceinline__ void block_sum_to(double (&v)[M], double* __restrict_
 
 // a tank's partials: items [item0, item0 + nitems), M values each
 template<int M>
-__device__ __forceinline__ void tank_totals(const BatchView& b, CTank& T, double (&tot)[M])
+__device__ __forceinline__ void tank_totals(CTank& T, const TankSet& b, double (&tot)[M])
 {
     sum_partials<M>(b.partials + (size_t)T.item0 * M, T.shape.nitems, tot);
 }
@@ -165,17 +224,16 @@ struct Prec<float>
 };
 
 // ------------------------------------------------------------------ simple kernels
-__global__ void __launch_bounds__(NT) kb_copy(BatchView b, int src, int dst)
+__global__ void __launch_bounds__(NT) kb_copy(TankSet b, int src, int dst)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const double* __restrict__ s = T.f[src];
         double* __restrict__ d = T.f[dst];
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -187,12 +245,11 @@ __global__ void __launch_bounds__(NT) kb_copy(BatchView b, int src, int dst)
 }
 
 // every element of array id (ghosts and padding included) of every active tank
-__global__ void __launch_bounds__(NT) kb_fill(BatchView b, int id, double v)
+__global__ void __launch_bounds__(NT) kb_fill(TankSet b, int id, double v)
 {
-    for(int tk = 0; tk < b.ntank; ++tk)
+    for(int tk = 0; tk < b.n; ++tk)
     {
         CTank& T = tank_at(b, tk);
-        if(!T.active) continue;
         double* __restrict__ f = T.f[id];
         const long n = T.g.nalloc;
         for(long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) f[q] = v;
@@ -200,12 +257,11 @@ __global__ void __launch_bounds__(NT) kb_fill(BatchView b, int id, double v)
 }
 
 // compiled ghost program prog of every active tank on array id
-__global__ void __launch_bounds__(NT) kb_ghost(BatchView b, int prog, int id)
+__global__ void __launch_bounds__(NT) kb_ghost(TankSet b, int prog, int id)
 {
-    for(int tk = 0; tk < b.ntank; ++tk)
+    for(int tk = 0; tk < b.n; ++tk)
     {
         CTank& T = tank_at(b, tk);
-        if(!T.active) continue;
         const GhostOp* __restrict__ ops = T.gops[prog];
         const int n = T.gnops[prog];
         double* __restrict__ f = T.f[id];
@@ -227,14 +283,13 @@ __global__ void __launch_bounds__(NT) kb_ghost(BatchView b, int prog, int id)
 // ------------------------------------------------------------------ level set, reinit
 // HJ-WENO transport stage fused with the TVD-RK3 combination (levelset_RK3.cpp:118-157,
 // momentum_FC3.cpp:132-358); cdt = cfac * dt of the tank; stage 0 accumulates out += L
-__global__ void __launch_bounds__(NT, 4) kb_hj_stage(BatchView b, int stage, int phi_s_, int phi_n_, int u_, int v_,
+__global__ void __launch_bounds__(NT, 4) kb_hj_stage(TankSet b, int stage, int phi_s_, int phi_n_, int u_, int v_,
                                                    int w_, int out_, double cfac)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const double* __restrict__ phi_s = T.f[phi_s_];
         const double* __restrict__ phi_n = phi_n_ >= 0 ? T.f[phi_n_] : nullptr;
@@ -243,7 +298,7 @@ __global__ void __launch_bounds__(NT, 4) kb_hj_stage(BatchView b, int stage, int
         const double* __restrict__ w = T.f[w_];
         double* __restrict__ out = T.f[out_];
         const double cdt = cfac * T.dt;
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -265,19 +320,18 @@ __global__ void __launch_bounds__(NT, 4) kb_hj_stage(BatchView b, int stage, int
 }
 
 // reinit sub-stage (reini_RK3.cpp:114-139) with the per-cell pseudo time step
-__global__ void __launch_bounds__(NT, 4) kb_reini_stage(BatchView b, int stage, int fs_, int f0_, int out_, double F43,
+__global__ void __launch_bounds__(NT, 4) kb_reini_stage(TankSet b, int stage, int fs_, int f0_, int out_, double F43,
                                                       int fsf)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const double* __restrict__ f = T.f[fs_];
         const double* __restrict__ f0 = T.f[f0_];
         double* __restrict__ out = T.f[out_];
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -299,13 +353,12 @@ __global__ void __launch_bounds__(NT, 4) kb_reini_stage(BatchView b, int stage, 
 
 // ------------------------------------------------------------------ momentum
 // WENO flux divergence accumulated into acc (convection::start, ipol 1..3)
-__global__ void __launch_bounds__(NT, 4) kb_flux_acc(BatchView b, int ipol, int q_, int u_, int v_, int w_, int acc_)
+__global__ void __launch_bounds__(NT, 4) kb_flux_acc(TankSet b, int ipol, int q_, int u_, int v_, int w_, int acc_)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         if(ipol == 2 && g.jdir == 0) continue;
         const double* __restrict__ qf = T.f[q_];
@@ -313,7 +366,7 @@ __global__ void __launch_bounds__(NT, 4) kb_flux_acc(BatchView b, int ipol, int 
         const double* __restrict__ v = T.f[v_];
         const double* __restrict__ w = T.f[w_];
         double* __restrict__ acc = T.f[acc_];
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -327,15 +380,14 @@ __global__ void __launch_bounds__(NT, 4) kb_flux_acc(BatchView b, int ipol, int 
 
 // velocity predictor of one component (momentum_FC3.cpp:160-208 and the later stages);
 // cdt = cfac * dt of the tank
-__global__ void __launch_bounds__(NT, 3) kb_momentum(BatchView b, int comp, int stage, int qin_, int qold_, int u_,
-                                                   int v_, int w_, double grav, double cfac, int out_, double rw,
-                                                   double ra)
+template<int comp>
+__global__ void __launch_bounds__(NT, 2) kb_momentum(TankSet b, int stage, int qin_, int qold_, int u_, int v_, int w_,
+                                                   double grav, double cfac, int out_, double rw, double ra)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         if(comp == 2 && g.jdir == 0) continue;
         const double* __restrict__ qin = T.f[qin_];
@@ -347,7 +399,7 @@ __global__ void __launch_bounds__(NT, 3) kb_momentum(BatchView b, int comp, int 
         const double* __restrict__ phi = T.f[REEF_PHI];
         double* __restrict__ out = T.f[out_];
         const double cdt = cfac * T.dt;
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -368,19 +420,18 @@ __global__ void __launch_bounds__(NT, 3) kb_momentum(BatchView b, int comp, int 
 
 // ------------------------------------------------------------------ fluid update
 // fluid_update_fsf::start (fluid_update_fsf.cpp:39-74): rho, nu and the volume partials
-__global__ void __launch_bounds__(NT) kb_heaviside(BatchView b, double rw, double vw, double ra, double va)
+__global__ void __launch_bounds__(NT) kb_heaviside(TankSet b, double rw, double vw, double ra, double va)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const double* __restrict__ phi = T.f[REEF_PHI];
         double* __restrict__ ro = T.f[REEF_RO];
         double* __restrict__ visc = T.f[REEF_VISC];
         double vals[2] = {0.0, 0.0};
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = item_walk(T, b, q);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -394,16 +445,15 @@ __global__ void __launch_bounds__(NT) kb_heaviside(BatchView b, double rw, doubl
             vals[0] += vol * (H - (1.0 - 1.0));
             vals[1] += vol * (1.0 - H - (1.0 - 1.0));
         }
-        block_sum_to<2>(vals, b.partials + (size_t)q.it * 2);
+        block_sum_to<2>(vals, b.partials + (size_t)item_slot(T, b, q) * 2);
     }
 }
 
-__global__ void __launch_bounds__(NT) kb_heaviside_fin(BatchView b)
+__global__ void __launch_bounds__(NT) kb_heaviside_fin(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
-    if(!T.active) return;
     double tot[2];
-    tank_totals<2>(b, T, tot);
+    tank_totals<2>(T, b, tot);
     if(threadIdx.x == 0)
     {
         T.vol[0] = tot[0];
@@ -412,20 +462,19 @@ __global__ void __launch_bounds__(NT) kb_heaviside_fin(BatchView b)
 }
 
 // ------------------------------------------------------------------ projection
-__global__ void __launch_bounds__(NT) kb_poisson_rhs(BatchView b, int u_, int v_, int w_, double afac)
+__global__ void __launch_bounds__(NT) kb_poisson_rhs(TankSet b, int u_, int v_, int w_, double afac)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const double* __restrict__ u = T.f[u_];
         const double* __restrict__ v = T.f[v_];
         const double* __restrict__ w = T.f[w_];
         double* __restrict__ rhs = T.f[A_RHS];
         const double adt = afac * T.dt;
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -438,19 +487,18 @@ __global__ void __launch_bounds__(NT) kb_poisson_rhs(BatchView b, int u_, int v_
 
 // face densities of the "+" faces of the interior and of the -1 layer, optionally the
 // Jacobi diagonal aii = -1/(M.p + 1e-19) (needs all faces: second pass)
-__global__ void __launch_bounds__(NT) kb_face_rho(BatchView b, double rw, double ra)
+__global__ void __launch_bounds__(NT) kb_face_rho(TankSet b, double rw, double ra)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const double* __restrict__ phi = T.f[REEF_PHI];
         double* __restrict__ rfx = T.f[A_RFX];
         double* __restrict__ rfy = T.f[A_RFY];
         double* __restrict__ rfz = T.f[A_RFZ];
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -467,19 +515,18 @@ __global__ void __launch_bounds__(NT) kb_face_rho(BatchView b, double rw, double
     }
 }
 
-__global__ void __launch_bounds__(NT) kb_aii(BatchView b)
+__global__ void __launch_bounds__(NT) kb_aii(TankSet b)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const double* __restrict__ rfx = T.f[A_RFX];
         const double* __restrict__ rfy = T.f[A_RFY];
         const double* __restrict__ rfz = T.f[A_RFZ];
         double* __restrict__ aii = T.f[A_AII];
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -492,17 +539,16 @@ __global__ void __launch_bounds__(NT) kb_aii(BatchView b)
     }
 }
 
-__global__ void __launch_bounds__(NT) kb_correct(BatchView b, double rw, double ra, double afac, int u_, int v_,
+__global__ void __launch_bounds__(NT) kb_correct(TankSet b, double rw, double ra, double afac, int u_, int v_,
                                                   int w_)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const double adt = afac * T.dt;
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -516,6 +562,11 @@ __global__ void __launch_bounds__(NT) kb_correct(BatchView b, double rw, double 
 }
 
 // ------------------------------------------------------------------ BiCGStab
+// Occupancy of the persistent solver kernels (waves per SIMD = blocks per CU): 8 for the
+// plane walk and the FP32 / elementwise kernels; 5 for the kernels that apply the FP64
+// operator cell by cell (about 100 registers), whose grids are sized to match.
+constexpr int OCC_FULL = 8, OCC_A64 = 5;
+
 // The state machine of bicgstab_ijk::solve (bicgstab_ijk.cpp:125-278) per tank: mode 1 = a
 // restart is pending, 2 = p and ph were just set from the residual, 3 = reliable update
 // pending (mixed precision).  Dot products are accumulated in FP64 for both precisions.
@@ -621,12 +672,11 @@ __device__ __forceinline__ void restart_body(const DevGrid& g, const Op& op, con
     }
 }
 
-__global__ void kb_setup(BatchView b, double stop, int maxiter, int mixed, double delta)
+__global__ void kb_setup(TankSet b, double stop, int maxiter, int mixed, double delta)
 {
     const int tk = blockIdx.x * blockDim.x + threadIdx.x;
-    if(tk >= b.ntank) return;
+    if(tk >= b.n) return;
     CTank& T = tank_at(b, tk);
-    if(!T.active) return;
     BiState* st = T.st;
     st->r_j = st->norm_r0 = st->alpha = st->w = st->beta = 0.0;
     st->sigma = st->norm_vj = st->norm_rj = st->norm_sj = st->w1 = st->w2 = st->r_j1 = 0.0;
@@ -646,27 +696,25 @@ __global__ void kb_setup(BatchView b, double stop, int maxiter, int mixed, doubl
 }
 
 template<class V>
-__global__ void __launch_bounds__(NT, 8) kb_bicg_init(BatchView b, int x, int rhs)
+__global__ void __launch_bounds__(NT, OCC_A64) kb_bicg_init(TankSet b, int x, int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         double vals[1] = {0.0};
-        restart_body(g, Prec<V>::op(T), Prec<V>::vec(T, x, rhs), item_walk(T, q.it), vals[0]);
-        block_sum_to<1>(vals, b.partials + (size_t)q.it);
+        restart_body(g, Prec<V>::op(T), Prec<V>::vec(T, x, rhs), item_walk(T, b, q), vals[0]);
+        block_sum_to<1>(vals, b.partials + (size_t)item_slot(T, b, q));
     }
 }
 
-__global__ void kb_fin_init(BatchView b)
+__global__ void kb_fin_init(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
-    if(!T.active) return;
     BiState* st = T.st;
     double tot[1];
-    tank_totals<1>(b, T, tot);
+    tank_totals<1>(T, b, tot);
     if(threadIdx.x == 0)
     {
         st->r_j = tot[0];
@@ -681,23 +729,22 @@ __global__ void kb_fin_init(BatchView b)
 // start of an iteration: the pending restart, or p = r + beta (p - w v), ph = aii p
 // (bicgstab_ijk.cpp:234-235 and precon_solve)
 template<class V>
-__global__ void __launch_bounds__(NT, 8) kb_bicg_p(BatchView b, int x, int rhs)
+__global__ void __launch_bounds__(NT, OCC_A64) kb_bicg_p(TankSet b, int x, int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const BiState* st = T.st;
         if(st->done || st->mode == 2) continue;
         const BiVec<V> v = Prec<V>::vec(T, x, rhs);
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = item_walk(T, b, q);
         if(st->mode == 1)
         {
             double vals[1] = {0.0};
             restart_body(g, Prec<V>::op(T), v, tw, vals[0]);
-            block_sum_to<1>(vals, b.partials + (size_t)q.it);
+            block_sum_to<1>(vals, b.partials + (size_t)item_slot(T, b, q));
             continue;
         }
         const V beta = (V)st->beta, w = (V)st->w;
@@ -715,14 +762,13 @@ __global__ void __launch_bounds__(NT, 8) kb_bicg_p(BatchView b, int x, int rhs)
 }
 
 // totals of a restart: ++solveriter, residual = |r|/N (bicgstab_ijk.cpp:183-191)
-__global__ void kb_fin_p(BatchView b)
+__global__ void kb_fin_p(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
-    if(!T.active) return;
     BiState* st = T.st;
     if(st->done || st->mode != 1) return;
     double tot[1];
-    tank_totals<1>(b, T, tot);
+    tank_totals<1>(T, b, tot);
     if(threadIdx.x == 0)
     {
         st->r_j = tot[0];
@@ -737,19 +783,18 @@ __global__ void kb_fin_p(BatchView b)
 // v = A ph; sigma = (v, r0), |v|^2, |r|^2 (bicgstab_ijk.cpp:151-160)
 // PW: items of the tanks walked by the plane walk, else the others (direct stencil)
 template<class V, bool PW>
-__global__ void __launch_bounds__(NT, 8) kb_bicg_v(BatchView b, int x, int rhs)
+__global__ void __launch_bounds__(NT, (PW || sizeof(V) == 4) ? OCC_FULL : OCC_A64) kb_bicg_v(TankSet b, int x, int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const BiState* st = T.st;
         if(st->done || st->mode == 1 || st->mode == 3 || plane_walk_ok(T) != PW) continue;
         const BiVec<V> bv = Prec<V>::vec(T, x, rhs);
         const typename Prec<V>::Op op = Prec<V>::op(T);
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = item_walk(T, b, q);
         double vals[3] = {0.0, 0.0, 0.0};
         const V* __restrict__ ph = bv.ph;
         const V* __restrict__ r = bv.r;
@@ -774,19 +819,18 @@ __global__ void __launch_bounds__(NT, 8) kb_bicg_v(BatchView b, int x, int rhs)
                 const CouplingV<V> cq = coupling_of(g, op, c, i, j, k);
                 epilogue(c, apply_A(g, cq, ph, c, i, j, k));
             }
-        block_sum_to<3>(vals, b.partials + (size_t)q.it * 3);
+        block_sum_to<3>(vals, b.partials + (size_t)item_slot(T, b, q) * 3);
     }
 }
 
 // alpha and the restart tests (bicgstab_ijk.cpp:162-191)
-__global__ void kb_fin_v(BatchView b)
+__global__ void kb_fin_v(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
-    if(!T.active) return;
     BiState* st = T.st;
     if(st->done || st->mode == 1 || st->mode == 3) return;
     double tot[3];
-    tank_totals<3>(b, T, tot);
+    tank_totals<3>(T, b, tot);
     if(threadIdx.x == 0)
     {
         const double sigma = tot[0];
@@ -808,19 +852,18 @@ __global__ void kb_fin_v(BatchView b)
 
 // s = r - alpha v, sh = aii s, t = A sh; (t,s), (t,t), (s,s) (bicgstab_ijk.cpp:193-221)
 template<class V, bool PW>
-__global__ void __launch_bounds__(NT, 8) kb_bicg_t(BatchView b, int x, int rhs)
+__global__ void __launch_bounds__(NT, (PW || sizeof(V) == 4) ? OCC_FULL : OCC_A64) kb_bicg_t(TankSet b, int x, int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const BiState* st = T.st;
         if(st->done || st->mode == 1 || st->mode == 3 || plane_walk_ok(T) != PW) continue;
         const BiVec<V> bv = Prec<V>::vec(T, x, rhs);
         const typename Prec<V>::Op op = Prec<V>::op(T);
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = item_walk(T, b, q);
         double vals[3] = {0.0, 0.0, 0.0};
         const V alpha = (V)st->alpha;
         const V* __restrict__ r = bv.r;
@@ -854,18 +897,17 @@ __global__ void __launch_bounds__(NT, 8) kb_bicg_t(BatchView b, int x, int rhs)
                 const V t = apply_A_fn(g, cq, [&](long cc) { return cc == c ? shc : shv(cc, dummy); }, c, i, j, k);
                 epilogue(c, t, s);
             }
-        block_sum_to<3>(vals, b.partials + (size_t)q.it * 3);
+        block_sum_to<3>(vals, b.partials + (size_t)item_slot(T, b, q) * 3);
     }
 }
 
-__global__ void kb_fin_t(BatchView b)
+__global__ void kb_fin_t(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
-    if(!T.active) return;
     BiState* st = T.st;
     if(st->done || st->mode == 1 || st->mode == 3) return;
     double tot[3];
-    tank_totals<3>(b, T, tot);
+    tank_totals<3>(T, b, tot);
     if(threadIdx.x == 0)
     {
         st->w1 = tot[0];
@@ -878,18 +920,17 @@ __global__ void kb_fin_t(BatchView b)
 
 // x and r updates, (r, r0), |r|^2 (bicgstab_ijk.cpp:223-268)
 template<class V>
-__global__ void __launch_bounds__(NT, 8) kb_bicg_x(BatchView b, int x, int rhs)
+__global__ void __launch_bounds__(NT, OCC_FULL) kb_bicg_x(TankSet b, int x, int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const BiState* st = T.st;
         if(st->done || st->mode == 1 || st->mode == 3) continue;
         const BiVec<V> bv = Prec<V>::vec(T, x, rhs);
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = item_walk(T, b, q);
         const V alpha = (V)st->alpha, w = (V)st->w;
         const int branch = st->branch;
         double vals[2] = {0.0, 0.0};
@@ -917,22 +958,21 @@ __global__ void __launch_bounds__(NT, 8) kb_bicg_x(BatchView b, int x, int rhs)
             vals[0] += rd * double(bv.r0[c]);
             vals[1] += rd * rd;
         }
-        block_sum_to<2>(vals, b.partials + (size_t)q.it * 2);
+        block_sum_to<2>(vals, b.partials + (size_t)item_slot(T, b, q) * 2);
     }
 }
 
 // beta, residual, iteration count; the mixed form asks for reliable updates instead of
 // ending on the iterated residual
-__global__ void kb_fin_x(BatchView b)
+__global__ void kb_fin_x(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
-    if(!T.active) return;
     BiState* st = T.st;
     if(st->done || st->mode == 1 || st->mode == 3) return;
     const double alpha = st->alpha, w = st->w;
     const int branch = st->branch;
     double tot[2];
-    tank_totals<2>(b, T, tot);
+    tank_totals<2>(T, b, tot);
     if(threadIdx.x == 0)
     {
         const double r_j1 = tot[0];
@@ -960,13 +1000,12 @@ __global__ void kb_fin_x(BatchView b)
 // ------------------------------------------------------------------ mixed precision
 // FP32 face conductances kx = 1/(rho_f DXP) (+x faces of the interior and the -1 layer,
 // likewise y, z) and aii = -1/(M.p + 1e-19) from the FP64 diagonal, rounded once
-__global__ void __launch_bounds__(NT) kb_coef32(BatchView b)
+__global__ void __launch_bounds__(NT) kb_coef32(TankSet b)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const double* __restrict__ rfx = T.f[A_RFX];
         const double* __restrict__ rfy = T.f[A_RFY];
@@ -975,7 +1014,7 @@ __global__ void __launch_bounds__(NT) kb_coef32(BatchView b)
         float* __restrict__ ky = T.s[S_KY];
         float* __restrict__ kz = T.s[S_KZ];
         float* __restrict__ aii = T.s[S_AII];
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -996,19 +1035,18 @@ __global__ void __launch_bounds__(NT) kb_coef32(BatchView b)
 }
 
 // initial residual in FP64: r = b - A64 x64; r0 = p = r, ph = aii r, x32 = 0
-__global__ void __launch_bounds__(NT, 8) kb_mix_start(BatchView b, int x, int rhs)
+__global__ void __launch_bounds__(NT, OCC_A64) kb_mix_start(TankSet b, int x, int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const SolverOp64 op = Prec<double>::op(T);
         const BiVec<float> v = Prec<float>::vec(T, x, rhs);
         const double* __restrict__ x64 = T.f[x];
         const double* __restrict__ b64 = T.f[rhs];
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = item_walk(T, b, q);
         double vals[2] = {0.0, 0.0};
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
@@ -1027,17 +1065,16 @@ __global__ void __launch_bounds__(NT, 8) kb_mix_start(BatchView b, int x, int rh
             vals[0] += rv * rv;
             vals[1] += double(r) * double(r);
         }
-        block_sum_to<2>(vals, b.partials + (size_t)q.it * 2);
+        block_sum_to<2>(vals, b.partials + (size_t)item_slot(T, b, q) * 2);
     }
 }
 
-__global__ void kb_fin_mix_start(BatchView b)
+__global__ void kb_fin_mix_start(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
-    if(!T.active) return;
     BiState* st = T.st;
     double tot[2];
-    tank_totals<2>(b, T, tot);
+    tank_totals<2>(T, b, tot);
     if(threadIdx.x == 0)
     {
         const double rn = sqrt(tot[0]);
@@ -1053,13 +1090,12 @@ __global__ void kb_fin_mix_start(BatchView b)
 }
 
 // start of an FP32 iteration: x64 += x32 after a reliable update, then the p update
-__global__ void __launch_bounds__(NT, 8) kb_mix_p(BatchView b, int x)
+__global__ void __launch_bounds__(NT, OCC_FULL) kb_mix_p(TankSet b, int x)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const BiState* st = T.st;
         if(st->done || st->mode == 1 || st->mode == 3) continue;
@@ -1068,7 +1104,7 @@ __global__ void __launch_bounds__(NT, 8) kb_mix_p(BatchView b, int x)
         const BiVec<float> v = Prec<float>::vec(T, x, x);
         double* __restrict__ x64 = T.f[x];
         const float beta = (float)st->beta, w = (float)st->w;
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = item_walk(T, b, q);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -1090,12 +1126,11 @@ __global__ void __launch_bounds__(NT, 8) kb_mix_p(BatchView b, int x)
     }
 }
 
-__global__ void kb_fin_mix_p(BatchView b)
+__global__ void kb_fin_mix_p(TankSet b)
 {
     const int tk = blockIdx.x * blockDim.x + threadIdx.x;
-    if(tk >= b.ntank) return;
+    if(tk >= b.n) return;
     CTank& T = tank_at(b, tk);
-    if(!T.active) return;
     BiState* st = T.st;
     if(st->done || st->mode == 1 || st->mode == 3) return;
     st->pending = 0;
@@ -1103,13 +1138,12 @@ __global__ void kb_fin_mix_p(BatchView b)
 
 // reliable update (mode 3) or restart (mode 1): r = b - A64 (x64 + x32) in FP64, rounded to
 // the FP32 residual; a restart also sets r0 = p = r, ph = aii r
-__global__ void __launch_bounds__(NT, 8) kb_mix_ru(BatchView b, int x, int rhs)
+__global__ void __launch_bounds__(NT, OCC_A64) kb_mix_ru(TankSet b, int x, int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const BiState* st = T.st;
         if(st->done || (st->mode != 1 && st->mode != 3)) continue;
         const int restart = st->mode == 1;
@@ -1119,7 +1153,7 @@ __global__ void __launch_bounds__(NT, 8) kb_mix_ru(BatchView b, int x, int rhs)
         const double* __restrict__ x64 = T.f[x];
         const double* __restrict__ b64 = T.f[rhs];
         const float* __restrict__ x32 = v.x;
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = item_walk(T, b, q);
         double vals[2] = {0.0, 0.0};
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
@@ -1145,18 +1179,17 @@ __global__ void __launch_bounds__(NT, 8) kb_mix_ru(BatchView b, int x, int rhs)
             vals[0] += rv * rv;
             vals[1] += double(r0v) * double(r);
         }
-        block_sum_to<2>(vals, b.partials + (size_t)q.it * 2);
+        block_sum_to<2>(vals, b.partials + (size_t)item_slot(T, b, q) * 2);
     }
 }
 
-__global__ void kb_fin_mix_ru(BatchView b)
+__global__ void kb_fin_mix_ru(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
-    if(!T.active) return;
     BiState* st = T.st;
     if(st->done || (st->mode != 1 && st->mode != 3)) return;
     double tot[2];
-    tank_totals<2>(b, T, tot);
+    tank_totals<2>(T, b, tot);
     if(threadIdx.x == 0)
     {
         const double rn = sqrt(tot[0]);
@@ -1187,17 +1220,16 @@ __global__ void kb_fin_mix_ru(BatchView b)
 }
 
 // x64 += x32, x32 = 0
-__global__ void __launch_bounds__(NT) kb_add32(BatchView b, int x)
+__global__ void __launch_bounds__(NT) kb_add32(TankSet b, int x)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         double* __restrict__ x64 = T.f[x];
         float* __restrict__ x32 = T.s[S_X];
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -1211,18 +1243,17 @@ __global__ void __launch_bounds__(NT) kb_add32(BatchView b, int x)
 }
 
 // ||rhs - A x||_2 (res_calc, bicgstab_ijk.cpp:314-338, without 1/N)
-__global__ void __launch_bounds__(NT, 8) kb_defect(BatchView b, int x, int rhs)
+__global__ void __launch_bounds__(NT, OCC_A64) kb_defect(TankSet b, int x, int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const SolverOp64 op = Prec<double>::op(T);
         const double* __restrict__ xv = T.f[x];
         const double* __restrict__ bv = T.f[rhs];
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = item_walk(T, b, q);
         double vals[1] = {0.0};
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
@@ -1234,32 +1265,39 @@ __global__ void __launch_bounds__(NT, 8) kb_defect(BatchView b, int x, int rhs)
             const double rv = bv[c] - apply_A(g, cq, xv, c, i, j, k);
             vals[0] += rv * rv;
         }
-        block_sum_to<1>(vals, b.partials + (size_t)q.it);
+        block_sum_to<1>(vals, b.partials + (size_t)item_slot(T, b, q));
     }
 }
 
-__global__ void kb_fin_norm(BatchView b, double* __restrict__ norms)
+__global__ void kb_fin_norm(TankSet b, double* __restrict__ norms)
 {
     CTank& T = tank_at(b, blockIdx.x);
-    if(!T.active) return;
     double tot[1];
-    tank_totals<1>(b, T, tot);
-    if(threadIdx.x == 0) norms[blockIdx.x] = sqrt(tot[0]);
+    tank_totals<1>(T, b, tot);
+    if(threadIdx.x == 0) norms[b.slot[blockIdx.x]] = sqrt(tot[0]);
+}
+
+// solver states of the set's tanks into dst[slot] (one copy-out per iteration batch)
+__global__ void kb_gather_states(TankSet b, BiState* __restrict__ dst)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p >= b.n) return;
+    CTank& T = tank_at(b, p);
+    dst[b.slot[p]] = *T.st;
 }
 
 // ------------------------------------------------------------------ level-set benchmark
 // benchmark_vortex3D::start (benchmark_vortex3D.cpp:67-96): the prescribed deformation
 // velocity at the face positions, at time simtime
-__global__ void __launch_bounds__(NT) kb_vortex_velocity(BatchView b, int comp, int id, double simtime, double DXM)
+__global__ void __launch_bounds__(NT) kb_vortex_velocity(TankSet b, int comp, int id, double simtime, double DXM)
 {
 #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         double* __restrict__ qf = T.f[id];
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
 #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -1290,17 +1328,16 @@ __global__ void __launch_bounds__(NT) kb_vortex_velocity(BatchView b, int comp, 
 
 // momentum_RK3 with void convection, diffusion and pressure (momentum_RK3.cpp:70-326):
 // the three stages of one component with the constant forcing Fv
-__global__ void __launch_bounds__(NT) kb_void_rk3(BatchView b, int comp, int id, double Fv)
+__global__ void __launch_bounds__(NT) kb_void_rk3(TankSet b, int comp, int id, double Fv)
 {
 #pragma unroll 1
-    for(ItemIter q = block_items(b); q.it < q.hi; q.it += q.step)
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
     {
-        CTank& T = item_tank_at(b, q.it);
-        if(!T.active) continue;
+        CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         double* __restrict__ qf = T.f[id];
         const double dt = T.dt;
-        const TileWalk tw = item_walk(T, q.it);
+        const TileWalk tw = tile_walk(T, q.t);
 #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
         {
@@ -1325,85 +1362,95 @@ __global__ void __launch_bounds__(NT) kb_void_rk3(BatchView b, int comp, int id,
 // ------------------------------------------------------------------ launchers
 using namespace bk;
 
-static inline dim3 tankgrid(const BatchView& b) { return dim3((unsigned)b.ntank); }
-static inline dim3 pgrid(int grid) { return dim3((unsigned)grid); }
+static inline dim3 tankgrid(const TankSet& b) { return dim3((unsigned)b.n); }
+static inline dim3 setup_grid(const TankSet& b) { return dim3((unsigned)((b.n + 63) / 64)); }
+// per-tile launch: 8 * ceil(tiles / 8) blocks
+static inline dim3 tgrid(const TankSet& b) { return dim3((unsigned)(8 * std::max(1, (b.ntiles + 7) / 8))); }
+// persistent launch over the work items: one block per work item, at most occ blocks per
+// CU (the kernel's occupancy) on the 256 CUs, a multiple of 8
+static inline dim3 pgrid(const TankSet& b, int occ = OCC_FULL)
+{
+    return dim3((unsigned)std::max(8, std::min(256 * occ, (b.nitems + 7) / 8 * 8)));
+}
 
-void bl_copy(const BatchView& b, int grid, int src, int dst, hipStream_t s)
+void bl_copy(const TankSet& b, int src, int dst, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_copy, pgrid(grid), dim3(NT), 0, s, b, src, dst);
+    hipLaunchKernelGGL(kb_copy, tgrid(b), dim3(NT), 0, s, b, src, dst);
 }
-void bl_fill(const BatchView& b, int grid, int id, double v, hipStream_t s)
+void bl_fill(const TankSet& b, int id, double v, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_fill, pgrid(grid), dim3(NT), 0, s, b, id, v);
+    hipLaunchKernelGGL(kb_fill, dim3(1024), dim3(NT), 0, s, b, id, v);
 }
-void bl_ghost(const BatchView& b, int grid, int prog, int id, hipStream_t s)
+void bl_ghost(const TankSet& b, int prog, int id, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_ghost, pgrid(grid), dim3(NT), 0, s, b, prog, id);
+    hipLaunchKernelGGL(kb_ghost, dim3(256), dim3(NT), 0, s, b, prog, id);
 }
-void bl_hj_stage(const BatchView& b, int grid, int stage, int phi_s, int phi_n, int u, int v, int w, int out,
-                 double cfac, hipStream_t s)
+void bl_hj_stage(const TankSet& b, int stage, int phi_s, int phi_n, int u, int v, int w, int out, double cfac,
+                 hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_hj_stage, pgrid(grid), dim3(NT), 0, s, b, stage, phi_s, phi_n, u, v, w, out, cfac);
+    hipLaunchKernelGGL(kb_hj_stage, tgrid(b), dim3(NT), 0, s, b, stage, phi_s, phi_n, u, v, w, out, cfac);
 }
-void bl_reini_stage(const BatchView& b, int grid, int stage, int fs, int f0, int out, double F43, int fsf,
-                    hipStream_t s)
+void bl_reini_stage(const TankSet& b, int stage, int fs, int f0, int out, double F43, int fsf, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_reini_stage, pgrid(grid), dim3(NT), 0, s, b, stage, fs, f0, out, F43, fsf);
+    hipLaunchKernelGGL(kb_reini_stage, tgrid(b), dim3(NT), 0, s, b, stage, fs, f0, out, F43, fsf);
 }
-void bl_flux_acc(const BatchView& b, int grid, int ipol, int q, int u, int v, int w, int acc, hipStream_t s)
+void bl_flux_acc(const TankSet& b, int ipol, int q, int u, int v, int w, int acc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_flux_acc, pgrid(grid), dim3(NT), 0, s, b, ipol, q, u, v, w, acc);
+    hipLaunchKernelGGL(kb_flux_acc, tgrid(b), dim3(NT), 0, s, b, ipol, q, u, v, w, acc);
 }
-void bl_momentum(const BatchView& b, int grid, int comp, int stage, int qin, int qold, int u, int v, int w,
-                 double grav, double cfac, int out, double rw, double ra, hipStream_t s)
+void bl_momentum(const TankSet& b, int comp, int stage, int qin, int qold, int u, int v, int w, double grav,
+                 double cfac, int out, double rw, double ra, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_momentum, pgrid(grid), dim3(NT), 0, s, b, comp, stage, qin, qold, u, v, w, grav, cfac, out,
-                       rw, ra);
+    if(comp == 1)
+        hipLaunchKernelGGL(kb_momentum<1>, tgrid(b), dim3(NT), 0, s, b, stage, qin, qold, u, v, w, grav, cfac, out, rw, ra);
+    else if(comp == 2)
+        hipLaunchKernelGGL(kb_momentum<2>, tgrid(b), dim3(NT), 0, s, b, stage, qin, qold, u, v, w, grav, cfac, out, rw, ra);
+    else
+        hipLaunchKernelGGL(kb_momentum<3>, tgrid(b), dim3(NT), 0, s, b, stage, qin, qold, u, v, w, grav, cfac, out, rw, ra);
 }
-void bl_heaviside(const BatchView& b, int grid, const double* prm4, hipStream_t s)
+void bl_heaviside(const TankSet& b, const double* prm4, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_heaviside, pgrid(grid), dim3(NT), 0, s, b, prm4[0], prm4[1], prm4[2], prm4[3]);
+    hipLaunchKernelGGL(kb_heaviside, pgrid(b), dim3(NT), 0, s, b, prm4[0], prm4[1], prm4[2], prm4[3]);
     hipLaunchKernelGGL(kb_heaviside_fin, tankgrid(b), dim3(NT), 0, s, b);
 }
-void bl_poisson_rhs(const BatchView& b, int grid, int u, int v, int w, double afac, hipStream_t s)
+void bl_poisson_rhs(const TankSet& b, int u, int v, int w, double afac, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_poisson_rhs, pgrid(grid), dim3(NT), 0, s, b, u, v, w, afac);
+    hipLaunchKernelGGL(kb_poisson_rhs, tgrid(b), dim3(NT), 0, s, b, u, v, w, afac);
 }
-void bl_face_rho(const BatchView& b, int grid, double rw, double ra, int with_aii, hipStream_t s)
+void bl_face_rho(const TankSet& b, double rw, double ra, int with_aii, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_face_rho, pgrid(grid), dim3(NT), 0, s, b, rw, ra);
-    if(with_aii) hipLaunchKernelGGL(kb_aii, pgrid(grid), dim3(NT), 0, s, b);
+    hipLaunchKernelGGL(kb_face_rho, tgrid(b), dim3(NT), 0, s, b, rw, ra);
+    if(with_aii) hipLaunchKernelGGL(kb_aii, tgrid(b), dim3(NT), 0, s, b);
 }
-void bl_correct(const BatchView& b, int grid, double rw, double ra, double afac, int u, int v, int w, hipStream_t s)
+void bl_correct(const TankSet& b, double rw, double ra, double afac, int u, int v, int w, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_correct, pgrid(grid), dim3(NT), 0, s, b, rw, ra, afac, u, v, w);
+    hipLaunchKernelGGL(kb_correct, tgrid(b), dim3(NT), 0, s, b, rw, ra, afac, u, v, w);
 }
 
-static inline dim3 setup_grid(const BatchView& b) { return dim3((unsigned)((b.ntank + 63) / 64)); }
-
-void bl_bicg_init(const BatchView& b, int grid, int x, int rhs, double stop, int maxiter, hipStream_t s)
+void bl_bicg_init(const TankSet& b, int x, int rhs, double stop, int maxiter, hipStream_t s)
 {
     hipLaunchKernelGGL(kb_setup, setup_grid(b), dim3(64), 0, s, b, stop, maxiter, 0, 0.0);
-    hipLaunchKernelGGL(kb_bicg_init<double>, pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+    hipLaunchKernelGGL(kb_bicg_init<double>, pgrid(b, OCC_A64), dim3(NT), 0, s, b, x, rhs);
     hipLaunchKernelGGL(kb_fin_init, tankgrid(b), dim3(NT), 0, s, b);
 }
 
-void bl_bicg_phase(const BatchView& b, int grid, int x, int rhs, int phase, int part, hipStream_t s)
+void bl_bicg_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStream_t s)
 {
+    const dim3 grid = pgrid(b), grid64 = pgrid(b, OCC_A64);
     if(part == 0)
     {
         switch(phase)
         {
-            case 0: hipLaunchKernelGGL(kb_bicg_p<double>, pgrid(grid), dim3(NT), 0, s, b, x, rhs); break;
+            case 0: hipLaunchKernelGGL(kb_bicg_p<double>, grid64, dim3(NT), 0, s, b, x, rhs); break;
             case 1:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<double, true>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
-                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_v<double, false>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<double, true>), grid, dim3(NT), 0, s, b, x, rhs);
+                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_v<double, false>), grid64, dim3(NT), 0, s, b, x, rhs);
                 break;
             case 2:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<double, true>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
-                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_t<double, false>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<double, true>), grid, dim3(NT), 0, s, b, x, rhs);
+                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_t<double, false>), grid64, dim3(NT), 0, s, b, x, rhs);
                 break;
-            default: hipLaunchKernelGGL(kb_bicg_x<double>, pgrid(grid), dim3(NT), 0, s, b, x, rhs); break;
+            default: hipLaunchKernelGGL(kb_bicg_x<double>, grid, dim3(NT), 0, s, b, x, rhs); break;
         }
         return;
     }
@@ -1416,36 +1463,36 @@ void bl_bicg_phase(const BatchView& b, int grid, int x, int rhs, int phase, int 
     }
 }
 
-void bl_coef32(const BatchView& b, int grid, hipStream_t s)
+void bl_coef32(const TankSet& b, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_coef32, pgrid(grid), dim3(NT), 0, s, b);
+    hipLaunchKernelGGL(kb_coef32, tgrid(b), dim3(NT), 0, s, b);
 }
 
-void bl_mixed_init(const BatchView& b, int grid, int x, int rhs, double stop, int maxiter, double delta,
-                   hipStream_t s)
+void bl_mixed_init(const TankSet& b, int x, int rhs, double stop, int maxiter, double delta, hipStream_t s)
 {
     hipLaunchKernelGGL(kb_setup, setup_grid(b), dim3(64), 0, s, b, stop, maxiter, 1, delta);
-    hipLaunchKernelGGL(kb_mix_start, pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+    hipLaunchKernelGGL(kb_mix_start, pgrid(b, OCC_A64), dim3(NT), 0, s, b, x, rhs);
     hipLaunchKernelGGL(kb_fin_mix_start, tankgrid(b), dim3(NT), 0, s, b);
 }
 
-void bl_mixed_phase(const BatchView& b, int grid, int x, int rhs, int phase, int part, hipStream_t s)
+void bl_mixed_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStream_t s)
 {
+    const dim3 grid = pgrid(b);
     if(part == 0)
     {
         switch(phase)
         {
-            case 0: hipLaunchKernelGGL(kb_mix_p, pgrid(grid), dim3(NT), 0, s, b, x); break;
+            case 0: hipLaunchKernelGGL(kb_mix_p, grid, dim3(NT), 0, s, b, x); break;
             case 1:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<float, true>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
-                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_v<float, false>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<float, true>), grid, dim3(NT), 0, s, b, x, rhs);
+                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_v<float, false>), grid, dim3(NT), 0, s, b, x, rhs);
                 break;
             case 2:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<float, true>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
-                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_t<float, false>), pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<float, true>), grid, dim3(NT), 0, s, b, x, rhs);
+                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_t<float, false>), grid, dim3(NT), 0, s, b, x, rhs);
                 break;
-            case 3: hipLaunchKernelGGL(kb_bicg_x<float>, pgrid(grid), dim3(NT), 0, s, b, x, rhs); break;
-            default: hipLaunchKernelGGL(kb_mix_ru, pgrid(grid), dim3(NT), 0, s, b, x, rhs); break;
+            case 3: hipLaunchKernelGGL(kb_bicg_x<float>, grid, dim3(NT), 0, s, b, x, rhs); break;
+            default: hipLaunchKernelGGL(kb_mix_ru, pgrid(b, OCC_A64), dim3(NT), 0, s, b, x, rhs); break;
         }
         return;
     }
@@ -1459,23 +1506,28 @@ void bl_mixed_phase(const BatchView& b, int grid, int x, int rhs, int phase, int
     }
 }
 
-void bl_add32(const BatchView& b, int grid, int x, hipStream_t s)
+void bl_add32(const TankSet& b, int x, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_add32, pgrid(grid), dim3(NT), 0, s, b, x);
+    hipLaunchKernelGGL(kb_add32, tgrid(b), dim3(NT), 0, s, b, x);
 }
 
-void bl_vortex_velocity(const BatchView& b, int grid, int comp, int id, double simtime, double DXM, hipStream_t s)
+void bl_vortex_velocity(const TankSet& b, int comp, int id, double simtime, double DXM, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_vortex_velocity, pgrid(grid), dim3(NT), 0, s, b, comp, id, simtime, DXM);
+    hipLaunchKernelGGL(kb_vortex_velocity, tgrid(b), dim3(NT), 0, s, b, comp, id, simtime, DXM);
 }
-void bl_void_rk3(const BatchView& b, int grid, int comp, int id, double Fv, hipStream_t s)
+void bl_void_rk3(const TankSet& b, int comp, int id, double Fv, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_void_rk3, pgrid(grid), dim3(NT), 0, s, b, comp, id, Fv);
+    hipLaunchKernelGGL(kb_void_rk3, tgrid(b), dim3(NT), 0, s, b, comp, id, Fv);
 }
 
-void bl_residual(const BatchView& b, int grid, int x, int rhs, double* norms, hipStream_t s)
+void bl_gather_states(const TankSet& b, BiState* dst, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_defect, pgrid(grid), dim3(NT), 0, s, b, x, rhs);
+    hipLaunchKernelGGL(kb_gather_states, setup_grid(b), dim3(64), 0, s, b, dst);
+}
+
+void bl_residual(const TankSet& b, int x, int rhs, double* norms, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_defect, pgrid(b, OCC_A64), dim3(NT), 0, s, b, x, rhs);
     hipLaunchKernelGGL(kb_fin_norm, tankgrid(b), dim3(NT), 0, s, b, norms);
 }
 

@@ -3,10 +3,12 @@
 // the engine and its stream.  Each round the worker
 //   1. admits waiting requests in arrival order while their device bytes fit in the HBM
 //      budget and the live count allows (the first that does not fit stops admission, so
-//      a large request is not overtaken for ever),
-//   2. prepares the admitted tanks alone (ghost fills and fluid update of those tanks),
-//   3. advances every live tank by one momentum_FC3 step in the same launches,
-//   4. retires the tanks that reached their step count: fields copied to the request,
+//      a large request is not overtaken for ever), prepares the admitted tanks alone
+//      (ghost fills and fluid update of those tanks) and schedules their steps,
+//   2. runs one round of the engine's iteration-level scheduler: every live tank moves on
+//      through its own momentum_FC3 steps (a tank whose pressure solve has converged
+//      goes on to its next segment while the others keep iterating),
+//   3. retires the tanks that have done all their steps: fields copied to the request,
 //      tank removed, its device memory back in the pool for the next admission.
 // A tank's arithmetic does not depend on the other tanks of a launch, so a request's
 // result is the same whether it was served alone or among others (given the same plan
@@ -15,6 +17,7 @@
 
 #include "../../include/reef3d_serve.h"
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -99,17 +102,6 @@ namespace {
 
 const int INIT_FIELDS[5] = {REEF_PHI, REEF_U, REEF_V, REEF_W, REEF_PRESS};
 
-// set-up of newly admitted tanks only: the ghost fills and fluid update a run starts with
-int prepare_new(Engine& e, const std::vector<int>& ids)
-{
-    int rc;
-    if((rc = e.set_active(ids))) return rc;
-    if((rc = e.ghost(REEF_PHI, 4, 52)) || (rc = e.ghost(REEF_U, 1, 10)) || (rc = e.ghost(REEF_V, 2, 11)) ||
-       (rc = e.ghost(REEF_W, 3, 12)) || (rc = e.ghost(REEF_PRESS, 4, 40)) || (rc = e.fluid_update()))
-        return rc;
-    return e.set_active({});   // every tank of the engine is live
-}
-
 }  // namespace
 
 // one admission + step + retirement round of the worker; a non-zero status is fatal
@@ -151,7 +143,7 @@ int reef_server::round(std::vector<RequestPtr>& admitted)
         }
         new_ids.push_back(tank);
     }
-    if(!new_ids.empty() && (rc = prepare_new(E, new_ids))) return rc;
+    if(!new_ids.empty() && (rc = E.prepare_tanks(new_ids))) return rc;
     {
         std::lock_guard<std::mutex> lk(mu);
         const double t = now();
@@ -163,19 +155,27 @@ int reef_server::round(std::vector<RequestPtr>& admitted)
         }
         bytes_in_use = E.pool.in_use();
     }
+    for(RequestPtr& r : admitted)
+        if((rc = E.start_steps(r->tank, r->steps))) return rc;
     admitted.clear();
     if(live.empty()) return 0;
-    if((rc = E.fc3_step())) return rc;
+    std::vector<int> stepped;
+    if((rc = E.pump(stepped))) return rc;
     ++steps;
+    if(stepped.empty()) return 0;
     std::vector<RequestPtr> still;
     std::vector<RequestPtr> finished;
     {
         std::lock_guard<std::mutex> lk(mu);
         for(RequestPtr& r : live)
         {
-            r->steps_done += 1;
-            r->iters += E.step_iters[r->tank];
-            (r->steps_done >= r->steps ? finished : still).push_back(r);
+            const TankHost* t = E.tank(r->tank);
+            if(std::find(stepped.begin(), stepped.end(), r->tank) != stepped.end())
+            {
+                r->steps_done += 1;
+                r->iters += t->iters_last;
+            }
+            (t->steps_left == 0 && t->pc < 0 ? finished : still).push_back(r);
         }
     }
     for(RequestPtr& r : finished)

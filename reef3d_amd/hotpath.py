@@ -103,6 +103,7 @@ def library():
         "reef_batch_plan_set": ([vp] + [ctypes.c_int] * 5, ctypes.c_int),
         "reef_batch_plans": ([vp, ip, ctypes.c_int, ip], ctypes.c_int),
         "reef_batch_sync": ([vp], ctypes.c_int),
+        "reef_batch_run": ([vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "reef_batch_timing_enable": ([vp, ctypes.c_int], ctypes.c_int),
         "reef_batch_timing_get": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "reef_server_create": ([ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)], ctypes.c_int),
@@ -426,6 +427,14 @@ class Batch:
         vol = np.zeros(2 * max(slots, 1))
         _check(library().reef_batch_fc3_step(self.h, it, _dp(vol)))
         return list(it)[:slots], vol[:2 * slots].reshape(-1, 2)
+
+    def run(self, nsteps):
+        """nsteps momentum_FC3 steps of every active tank, iteration-level scheduled;
+        returns the iterations of each slot's last step."""
+        slots = self.info()[0]
+        it = (ctypes.c_long * max(slots, 1))()
+        _check(library().reef_batch_run(self.h, int(nsteps), it))
+        return list(it)[:slots]
 
     def sync(self):
         _check(library().reef_batch_sync(self.h))
