@@ -1003,6 +1003,9 @@ int Engine::start_steps(int id, int n)
     return 0;
 }
 
+// pc 1, 3, 5: in the pressure solve of stage 1, 2, 3 (pc -1, idle, is not odd here)
+static inline bool solving_pc(int pc) { return pc == 1 || pc == 3 || pc == 5; }
+
 bool Engine::busy() const
 {
     if(!inflight_.empty()) return true;
@@ -1020,7 +1023,7 @@ int Engine::read_batch(const Inflight& f)
     for(const auto& m : f.members)
     {
         TankHost& t = tanks[m.first];
-        if(!t.used || t.seq != m.second || (t.pc & 1) == 0) continue;
+        if(!t.used || t.seq != m.second || !solving_pc(t.pc)) continue;
         const BiState& st = hs[m.first];
         if(!st.done) continue;
         t.iters_step += st.iter;
@@ -1085,7 +1088,7 @@ int Engine::pump(std::vector<int>& stepped)
     }
     std::vector<int> solving;
     for(size_t q = 0; q < tanks.size(); ++q)
-        if(tanks[q].used && (tanks[q].pc & 1) == 1) solving.push_back((int)q);
+        if(tanks[q].used && solving_pc(tanks[q].pc)) solving.push_back((int)q);
     if(!solving.empty())
     {
         if(solving != solving_key_)
