@@ -261,35 +261,67 @@ __device__ __forceinline__ CouplingV<float> coupling_of(const DevGrid& g, const 
     return q;
 }
 
-// couplings from the six face values (+x, -x, +y, -y, +z, -z) of cell (i, j, k): the face
-// densities for the FP64 operator, the FP32 conductances for the FP32 one
-__device__ __forceinline__ CouplingV<double> coupling_faces(const DevGrid& g, const SolverOp64&, double F0, double F1,
-                                                            double F2, double F3, double F4, double F5, int i, int j,
-                                                            int k)
+// The grid factors of the couplings along one axis at index ix: DP(ix), DN(ix), DP(ix-1)
+// for the FP64 operator, the FP32 1/DN(ix) for the FP32 one.  The plane walk loads the
+// x and y factors once per column and the z factors with each plane.
+template<class Op>
+struct AxisFactor;
+template<>
+struct AxisFactor<SolverOp64>
 {
-    const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
+    double p0, n, p1;
+};
+template<>
+struct AxisFactor<SolverOp32>
+{
+    float inv;
+};
+
+__device__ __forceinline__ AxisFactor<SolverOp64> axis_factor(const DevGrid& g, const SolverOp64&, int axis, int ix)
+{
+    const double* P = axis == 0 ? g.DXP : axis == 1 ? g.DYP : g.DZP;
+    const double* N = axis == 0 ? g.DXN : axis == 1 ? g.DYN : g.DZN;
+    const int ip = ix + MARGE;
+    return AxisFactor<SolverOp64>{P[ip], N[ip], P[ip - 1]};
+}
+
+__device__ __forceinline__ AxisFactor<SolverOp32> axis_factor(const DevGrid&, const SolverOp32& op, int axis, int ix)
+{
+    const float* I = axis == 0 ? op.iDXN : axis == 1 ? op.iDYN : op.iDZN;
+    return AxisFactor<SolverOp32>{I[ix + MARGE]};
+}
+
+// couplings from the six face values (+x, -x, +y, -y, +z, -z) and the axis factors of the
+// cell: the face densities for the FP64 operator (the operation order of couplings()),
+// the FP32 conductances for the FP32 one (that of coupling_of)
+__device__ __forceinline__ CouplingV<double> coupling_axes(const SolverOp64&, double F0, double F1, double F2,
+                                                           double F3, double F4, double F5,
+                                                           const AxisFactor<SolverOp64>& ax,
+                                                           const AxisFactor<SolverOp64>& ay,
+                                                           const AxisFactor<SolverOp64>& az)
+{
     CouplingV<double> q;
-    q.T[0] = (1.0 * 1.0) / (F0 * g.DXP[ip] * g.DXN[ip]);
-    q.T[1] = (1.0 * 1.0) / (F1 * g.DXP[ip - 1] * g.DXN[ip]);
-    q.T[2] = (1.0 * 1.0) / (F2 * g.DYP[jp] * g.DYN[jp]);
-    q.T[3] = (1.0 * 1.0) / (F3 * g.DYP[jp - 1] * g.DYN[jp]);
-    q.T[4] = (1.0 * 1.0) / (F4 * g.DZP[kp] * g.DZN[kp]);
-    q.T[5] = (1.0 * 1.0) / (F5 * g.DZP[kp - 1] * g.DZN[kp]);
+    q.T[0] = (1.0 * 1.0) / (F0 * ax.p0 * ax.n);
+    q.T[1] = (1.0 * 1.0) / (F1 * ax.p1 * ax.n);
+    q.T[2] = (1.0 * 1.0) / (F2 * ay.p0 * ay.n);
+    q.T[3] = (1.0 * 1.0) / (F3 * ay.p1 * ay.n);
+    q.T[4] = (1.0 * 1.0) / (F4 * az.p0 * az.n);
+    q.T[5] = (1.0 * 1.0) / (F5 * az.p1 * az.n);
     return q;
 }
 
-__device__ __forceinline__ CouplingV<float> coupling_faces(const DevGrid&, const SolverOp32& op, float F0, float F1,
-                                                           float F2, float F3, float F4, float F5, int i, int j,
-                                                           int k)
+__device__ __forceinline__ CouplingV<float> coupling_axes(const SolverOp32&, float F0, float F1, float F2, float F3,
+                                                          float F4, float F5, const AxisFactor<SolverOp32>& ax,
+                                                          const AxisFactor<SolverOp32>& ay,
+                                                          const AxisFactor<SolverOp32>& az)
 {
-    const float ix = op.iDXN[i + MARGE], iy = op.iDYN[j + MARGE], iz = op.iDZN[k + MARGE];
     CouplingV<float> q;
-    q.T[0] = F0 * ix;
-    q.T[1] = F1 * ix;
-    q.T[2] = F2 * iy;
-    q.T[3] = F3 * iy;
-    q.T[4] = F4 * iz;
-    q.T[5] = F5 * iz;
+    q.T[0] = F0 * ax.inv;
+    q.T[1] = F1 * ax.inv;
+    q.T[2] = F2 * ay.inv;
+    q.T[3] = F3 * ay.inv;
+    q.T[4] = F4 * az.inv;
+    q.T[5] = F5 * az.inv;
     return q;
 }
 

@@ -99,14 +99,16 @@ void DevicePool::free(void* p)
 }
 
 // ------------------------------------------------------------------ plan
-// 64 x 4 x 1 tiles for 3-D tanks (plane walk of the solver), 64 x 1 x 4 for 2-D x-z tanks;
-// chunks of up to 16 tiles so that a large tank has about 2048 work items.  The plan is a
+// 64 x 4 x 1 tiles for 3-D tanks (plane walk of the solver), 64 x 1 x 4 for 2-D x-z tanks.
+// A 3-D tank gets about one work item per block of the plane-walk grid (a large tank's
+// items are columns of several planes); a 2-D tank about 2048 items.  The plan is a
 // function of the tank alone, so its reductions are the same in any batch.
 TileShape plan_shape(const HostGrid& h, int cpt_override)
 {
     const int ty = h.jdir == 1 ? 4 : 1, tz = h.jdir == 1 ? 1 : 4;
     TileShape s = make_shape(h.nx, h.ny, h.nz, ty, tz, 1);
-    int cpt = cpt_override > 0 ? cpt_override : (s.ntiles + 2047) / 2048;
+    const int target = h.jdir == 1 ? PW_BLOCKS : 2048;
+    int cpt = cpt_override > 0 ? cpt_override : (s.ntiles + target - 1) / target;
     cpt = std::max(1, std::min(cpt, 64));
     return make_shape(h.nx, h.ny, h.nz, ty, tz, cpt);
 }

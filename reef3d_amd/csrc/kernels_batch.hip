@@ -566,6 +566,9 @@ __global__ void __launch_bounds__(NT) kb_correct(TankSet b, double rw, double ra
 // plane walk and the FP32 / elementwise kernels; 5 for the kernels that apply the FP64
 // operator cell by cell (about 100 registers), whose grids are sized to match.
 constexpr int OCC_FULL = 8, OCC_A64 = 5;
+// plane walk: 4 blocks per CU (it keeps one plane of loads in flight per block, about 100
+// registers); the default plan of a 3-D tank gives it one work item per block
+constexpr int OCC_PW = REEF_OCC_PW;
 
 // The state machine of bicgstab_ijk::solve (bicgstab_ijk.cpp:125-278) per tank: mode 1 = a
 // restart is pending, 2 = p and ph were just set from the residual, 3 = reliable update
@@ -584,64 +587,196 @@ struct PlaneLDS
     V fy[TY + 1][TX];
 };
 
-template<class V, class Op, class Load, class Cell>
-__device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const TileWalk& tw, Load load, Cell cell)
+// Loads of one plane of the walk, issued one plane ahead: the operand's raw inputs at one
+// halo cell of the block and at the cell above (k + 1), the faces, and the epilogue's
+// inputs.  Halo cells are spread over the threads: thread h < 64 loads the south row
+// (j0-1, i0+h), 64 <= h < 128 the north row (j0+TY, i0+h-64), 128 <= h < 132 the west
+// column (j0+h-128, i0-1), 132 <= h < 136 the east column (j0+h-132, i0+TX); the halo
+// faces: thread h < 64 the -y face of column i0+h, 128 <= h < 132 the -x face of row
+// j0+h-128.
+template<class V, class R, class E, class AF>
+struct PlaneIn
 {
+    R h, xu;
+    V f0, f2, f4, fh;
+    AF az;
+    E e;
+};
+
+struct NoEpi
+{
+};
+
+// raw(cc) -> R: the loads of the stencil operand at cc; val(R, V& aux) -> V: the operand
+// (aux: what the epilogue needs of the centre); pre(c) -> E: the epilogue's loads;
+// post(c, Ax, aux, E): the epilogue.  The loads of plane k+1 are issued after plane k is
+// staged in LDS and before its barrier, so they overlap plane k's barrier and arithmetic.
+template<class V, class Op, class Raw, class Val, class Pre, class Post>
+__device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const TileWalk& tw, Raw raw, Val val, Pre pre,
+                                           Post post)
+{
+    using R = decltype(raw(0L));
+    using E = decltype(pre(0L));
+    using AF = AxisFactor<Op>;
+    using In = PlaneIn<V, R, E, AF>;
     __shared__ PlaneLDS<V> L[2];
     const V* __restrict__ fxa = face_x(op);
     const V* __restrict__ fya = face_y(op);
     const V* __restrict__ fza = face_z(op);
-    const int lane = threadIdx.x & (TX - 1), row = threadIdx.x >> 6;
-    int curcol = -1, buf = 0;
-    V xd = V(0), xc = V(0), fzd = V(0), auxc = V(0);
-    #pragma unroll 1
-    for(int t = tw.t; t < tw.t1; ++t)
+    const int h = threadIdx.x;
+    const int lane = h & (TX - 1), row = h >> 6;
+    // the halo cell of thread h relative to the tile origin, and its LDS slot
+    int hi = 0, hj = 0, hrow = 0, hcol = 0;
+    if(h < TX)
     {
-        const int col = t / tw.s.nbz;
-        const int k = t - col * tw.s.nbz;
+        hi = h;
+        hj = -1;
+        hrow = 0;
+        hcol = h + 1;
+    }
+    else if(h < 2 * TX)
+    {
+        hi = h - TX;
+        hj = TY;
+        hrow = TY + 1;
+        hcol = h - TX + 1;
+    }
+    else if(h < 2 * TX + TY)
+    {
+        hi = -1;
+        hj = h - 2 * TX;
+        hrow = h - 2 * TX + 1;
+        hcol = 0;
+    }
+    else if(h < 2 * TX + 2 * TY)
+    {
+        hi = TX;
+        hj = h - 2 * TX - TY;
+        hrow = h - 2 * TX - TY + 1;
+        hcol = TX + 1;
+    }
+    const bool has_halo = h < 2 * TX + 2 * TY;
+    auto geo = [&](int t, int& i0, int& j0, int& k, int& col) {
+        col = t / tw.s.nbz;
+        k = t - col * tw.s.nbz;
         const int by = col / tw.s.nbx, bx = col - by * tw.s.nbx;
-        const int i = bx * TX + lane, j = by * TY + row;
+        i0 = bx * TX;
+        j0 = by * TY;
+    };
+    auto fetch = [&](int i0, int j0, int k, In& P) {
+        const int i = i0 + lane, j = j0 + row;
         const bool in = i < g.nx && j < g.ny;
         const long c = g.base + i + g.sx * j + g.sxy * k;
-        if(col != curcol)
+        P.h = R{};
+        if(has_halo)
         {
-            curcol = col;
-            V dummy;
-            xd = (in && k > 0) ? load(c - g.sxy, dummy) : V(0);
-            xc = in ? load(c, auxc) : V(0);
-            fzd = in ? fza[c - g.sxy] : V(0);
+            const int ih = i0 + hi, jh = j0 + hj;
+            if(ih >= 0 && ih < g.nx && jh >= 0 && jh < g.ny) P.h = raw(g.base + ih + g.sx * jh + g.sxy * k);
         }
-        V auxu = V(0);
-        const V xu = (in && k + 1 < g.nz) ? load(c + g.sxy, auxu) : V(0);
-        PlaneLDS<V>& P = L[buf];
-        P.x[row + 1][lane + 1] = xc;
+        P.fh = V(0);
+        if(h < TX)
         {
-            V dummy;
-            if(row == 0) P.x[0][lane + 1] = (i < g.nx && j > 0 && j - 1 < g.ny) ? load(c - g.sx, dummy) : V(0);
-            if(row == TY - 1) P.x[TY + 1][lane + 1] = (i < g.nx && j + 1 < g.ny) ? load(c + g.sx, dummy) : V(0);
-            if(lane == 0) P.x[row + 1][0] = (j < g.ny && i > 0) ? load(c - 1, dummy) : V(0);
-            if(lane == TX - 1) P.x[row + 1][TX + 1] = (j < g.ny && i + 1 < g.nx) ? load(c + 1, dummy) : V(0);
+            if(i < g.nx && j0 < g.ny) P.fh = fya[g.base + i + g.sx * j0 + g.sxy * k - g.sx];
         }
-        const V f0 = in ? fxa[c] : V(0);
-        const V f2 = in ? fya[c] : V(0);
-        const V f4 = in ? fza[c] : V(0);
-        P.fx[row][lane + 1] = f0;
-        if(lane == 0) P.fx[row][0] = (j < g.ny) ? fxa[c - 1] : V(0);
-        P.fy[row + 1][lane] = f2;
-        if(row == 0) P.fy[0][lane] = (i < g.nx && j < g.ny) ? fya[c - g.sx] : V(0);
-        __syncthreads();
+        else if(h >= 2 * TX && h < 2 * TX + TY)
+        {
+            const int jr = j0 + (h - 2 * TX);
+            if(jr < g.ny) P.fh = fxa[g.base + i0 + g.sx * jr + g.sxy * k - 1];
+        }
+        P.f0 = in ? fxa[c] : V(0);
+        P.f2 = in ? fya[c] : V(0);
+        P.f4 = in ? fza[c] : V(0);
+        P.xu = (in && k + 1 < g.nz) ? raw(c + g.sxy) : R{};
+        P.az = axis_factor(g, op, 2, k);
+        P.e = in ? pre(c) : E{};
+    };
+    int t = tw.t;
+    if(t >= tw.t1) return;
+    int i0, j0, k, col;
+    geo(t, i0, j0, k, col);
+    int i = i0 + lane, j = j0 + row;
+    // k-window of the operand: xd (k-1), xc (k) and the centre's aux; fzd: the -z face;
+    // ax, ay: the column's x and y grid factors
+    V xd, xc, auxc, fzd;
+    AF ax{}, ay{};
+    auto column_start = [&]() {
+        const bool in = i < g.nx && j < g.ny;
+        const long c = g.base + i + g.sx * j + g.sxy * k;
+        V d;
+        xd = (in && k > 0) ? val(raw(c - g.sxy), d) : V(0);
+        auxc = V(0);
+        xc = in ? val(raw(c), auxc) : V(0);
+        fzd = in ? fza[c - g.sxy] : V(0);
         if(in)
         {
-            const CouplingV<V> cq = coupling_faces(g, op, f0, P.fx[row][lane], f2, P.fy[row][lane], f4, fzd, i, j, k);
+            ax = axis_factor(g, op, 0, i);
+            ay = axis_factor(g, op, 1, j);
+        }
+    };
+    column_start();
+    In cur;
+    fetch(i0, j0, k, cur);
+    int buf = 0;
+    #pragma unroll 1
+    for(;; ++t)
+    {
+        const bool in = i < g.nx && j < g.ny;
+        const long c = g.base + i + g.sx * j + g.sxy * k;
+        PlaneLDS<V>& P = L[buf];
+        {
+            P.x[row + 1][lane + 1] = xc;
+            if(has_halo)
+            {
+                V d;
+                P.x[hrow][hcol] = val(cur.h, d);
+            }
+            P.fx[row][lane + 1] = cur.f0;
+            P.fy[row + 1][lane] = cur.f2;
+            if(h < TX)
+                P.fy[0][h] = cur.fh;
+            else if(h >= 2 * TX && h < 2 * TX + TY)
+                P.fx[h - 2 * TX][0] = cur.fh;
+        }
+        // the next plane of the same column: its loads go out now
+        const bool more = t + 1 < tw.t1;
+        int i2 = i0, j2 = j0, k2 = k, col2 = col;
+        if(more) geo(t + 1, i2, j2, k2, col2);
+        const bool same = more && col2 == col;
+        In nxt;
+        if(same) fetch(i2, j2, k2, nxt);
+        __syncthreads();
+        V auxu = V(0);
+        const V xu = val(cur.xu, auxu);
+        if(in)
+        {
+            const CouplingV<V> cq =
+                coupling_axes(op, cur.f0, P.fx[row][lane], cur.f2, P.fy[row][lane], cur.f4, fzd, ax, ay, cur.az);
             const V Ax = stencil7(g, cq, xc, P.x[row + 1][lane + 2], P.x[row + 1][lane], P.x[row + 2][lane + 1],
                                   P.x[row][lane + 1], xu, xd, i, j, k);
-            cell(c, Ax, auxc);
+            post(c, Ax, auxc, cur.e);
         }
         buf ^= 1;
-        xd = xc;
-        xc = xu;
-        auxc = auxu;
-        fzd = f4;
+        if(!more) break;
+        if(same)
+        {
+            xd = xc;
+            xc = xu;
+            auxc = auxu;
+            fzd = cur.f4;
+            cur = nxt;
+            k = k2;
+        }
+        else
+        {
+            i0 = i2;
+            j0 = j2;
+            k = k2;
+            col = col2;
+            i = i0 + lane;
+            j = j0 + row;
+            column_start();
+            fetch(i0, j0, k, cur);
+        }
     }
 }
 
@@ -783,7 +918,8 @@ __global__ void kb_fin_p(TankSet b)
 // v = A ph; sigma = (v, r0), |v|^2, |r|^2 (bicgstab_ijk.cpp:151-160)
 // PW: items of the tanks walked by the plane walk, else the others (direct stencil)
 template<class V, bool PW>
-__global__ void __launch_bounds__(NT, (PW || sizeof(V) == 4) ? OCC_FULL : OCC_A64) kb_bicg_v(TankSet b, int x, int rhs)
+__global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL : OCC_A64)) kb_bicg_v(TankSet b, int x,
+                                                                                               int rhs)
 {
     #pragma unroll 1
     for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
@@ -808,7 +944,22 @@ __global__ void __launch_bounds__(NT, (PW || sizeof(V) == 4) ? OCC_FULL : OCC_A6
             vals[2] += rc * rc;
         };
         if constexpr(PW)
-            plane_walk<V>(g, op, tw, [&](long cc, V&) { return ph[cc]; }, [&](long c, V Ax, V) { epilogue(c, Ax); });
+        {
+            struct EpiIn
+            {
+                V r0, r;
+            };
+            plane_walk<V>(
+                g, op, tw, [&](long cc) { return ph[cc]; }, [](V x, V&) { return x; },
+                [&](long c) { return EpiIn{r0[c], r[c]}; },
+                [&](long c, V Ax, V, const EpiIn& e) {
+                    vo[c] = Ax;
+                    const double vd = Ax, rc = e.r;
+                    vals[0] += vd * double(e.r0);
+                    vals[1] += vd * vd;
+                    vals[2] += rc * rc;
+                });
+        }
         else
             #pragma unroll 1
             for(int t = tw.t; t < tw.t1; ++t)
@@ -852,7 +1003,8 @@ __global__ void kb_fin_v(TankSet b)
 
 // s = r - alpha v, sh = aii s, t = A sh; (t,s), (t,t), (s,s) (bicgstab_ijk.cpp:193-221)
 template<class V, bool PW>
-__global__ void __launch_bounds__(NT, (PW || sizeof(V) == 4) ? OCC_FULL : OCC_A64) kb_bicg_t(TankSet b, int x, int rhs)
+__global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL : OCC_A64)) kb_bicg_t(TankSet b, int x,
+                                                                                               int rhs)
 {
     #pragma unroll 1
     for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
@@ -882,7 +1034,19 @@ __global__ void __launch_bounds__(NT, (PW || sizeof(V) == 4) ? OCC_FULL : OCC_A6
             return sv * a[cc];
         };
         if constexpr(PW)
-            plane_walk<V>(g, op, tw, shv, [&](long c, V Ax, V s) { epilogue(c, Ax, s); });
+        {
+            struct Rin
+            {
+                V r, v, a;
+            };
+            plane_walk<V>(
+                g, op, tw, [&](long cc) { return Rin{r[cc], v[cc], a[cc]}; },
+                [&](const Rin& q, V& sv) {
+                    sv = q.r - alpha * q.v;
+                    return sv * q.a;
+                },
+                [](long) { return NoEpi{}; }, [&](long c, V Ax, V s, const NoEpi&) { epilogue(c, Ax, s); });
+        }
         else
             #pragma unroll 1
             for(int tt = tw.t; tt < tw.t1; ++tt)
@@ -1436,18 +1600,18 @@ void bl_bicg_init(const TankSet& b, int x, int rhs, double stop, int maxiter, hi
 
 void bl_bicg_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStream_t s)
 {
-    const dim3 grid = pgrid(b), grid64 = pgrid(b, OCC_A64);
+    const dim3 grid = pgrid(b), grid64 = pgrid(b, OCC_A64), gridpw = pgrid(b, OCC_PW);
     if(part == 0)
     {
         switch(phase)
         {
             case 0: hipLaunchKernelGGL(kb_bicg_p<double>, grid64, dim3(NT), 0, s, b, x, rhs); break;
             case 1:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<double, true>), grid, dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<double, true>), gridpw, dim3(NT), 0, s, b, x, rhs);
                 if(b.has_direct) hipLaunchKernelGGL((kb_bicg_v<double, false>), grid64, dim3(NT), 0, s, b, x, rhs);
                 break;
             case 2:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<double, true>), grid, dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<double, true>), gridpw, dim3(NT), 0, s, b, x, rhs);
                 if(b.has_direct) hipLaunchKernelGGL((kb_bicg_t<double, false>), grid64, dim3(NT), 0, s, b, x, rhs);
                 break;
             default: hipLaunchKernelGGL(kb_bicg_x<double>, grid, dim3(NT), 0, s, b, x, rhs); break;
@@ -1477,18 +1641,18 @@ void bl_mixed_init(const TankSet& b, int x, int rhs, double stop, int maxiter, d
 
 void bl_mixed_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStream_t s)
 {
-    const dim3 grid = pgrid(b);
+    const dim3 grid = pgrid(b), gridpw = pgrid(b, OCC_PW);
     if(part == 0)
     {
         switch(phase)
         {
             case 0: hipLaunchKernelGGL(kb_mix_p, grid, dim3(NT), 0, s, b, x); break;
             case 1:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<float, true>), grid, dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<float, true>), gridpw, dim3(NT), 0, s, b, x, rhs);
                 if(b.has_direct) hipLaunchKernelGGL((kb_bicg_v<float, false>), grid, dim3(NT), 0, s, b, x, rhs);
                 break;
             case 2:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<float, true>), grid, dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<float, true>), gridpw, dim3(NT), 0, s, b, x, rhs);
                 if(b.has_direct) hipLaunchKernelGGL((kb_bicg_t<float, false>), grid, dim3(NT), 0, s, b, x, rhs);
                 break;
             case 3: hipLaunchKernelGGL(kb_bicg_x<float>, grid, dim3(NT), 0, s, b, x, rhs); break;

@@ -24,6 +24,10 @@ namespace reef {
 
 constexpr int TILE_X = 64;
 
+// blocks per CU of the plane-walk solver kernels (256 CUs)
+#define REEF_OCC_PW 4
+constexpr int PW_BLOCKS = 256 * REEF_OCC_PW;
+
 struct TileShape
 {
     int ty, tz;          // tile rows in y and z, ty * tz == 4
