@@ -587,6 +587,19 @@ struct PlaneLDS
     V fy[TY + 1][TX];
 };
 
+// element e of an array through a 32-bit byte offset from its (uniform) base, so the load
+// takes the scalar-base + 32-bit vector-offset form (a tank's arrays are below 4 GiB)
+template<class T>
+__device__ __forceinline__ T ld32(const T* __restrict__ p, unsigned e)
+{
+    return *(const T*)((const char*)p + e * (unsigned)sizeof(T));
+}
+template<class T>
+__device__ __forceinline__ void st32(T* __restrict__ p, unsigned e, T v)
+{
+    *(T*)((char*)p + e * (unsigned)sizeof(T)) = v;
+}
+
 // Loads of one plane of the walk, issued one plane ahead: the operand's raw inputs at one
 // halo cell of the block and at the cell above (k + 1), the faces, and the epilogue's
 // inputs.  Halo cells are spread over the threads: thread h < 64 loads the south row
@@ -609,23 +622,31 @@ struct NoEpi
 
 // raw(cc) -> R: the loads of the stencil operand at cc; val(R, V& aux) -> V: the operand
 // (aux: what the epilogue needs of the centre); pre(c) -> E: the epilogue's loads;
-// post(c, Ax, aux, E): the epilogue.  The loads of plane k+1 are issued after plane k is
-// staged in LDS and before its barrier, so they overlap plane k's barrier and arithmetic.
+// post(c, Ax, aux, E): the epilogue.  The loads of planes t+1 .. t+D are in flight while
+// plane t is staged, synchronised and computed: D = 2 for 4-byte operands, 1 for 8-byte
+// ones.  Every load of a plane is issued unconditionally at a clamped (valid) address
+// and masked afterwards, and the loop is unrolled over D+1 plane buffers, so no plane's
+// loads wait for the next plane's to be issued.
 template<class V, class Op, class Raw, class Val, class Pre, class Post>
 __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const TileWalk& tw, Raw raw, Val val, Pre pre,
                                            Post post)
 {
-    using R = decltype(raw(0L));
-    using E = decltype(pre(0L));
+    using R = decltype(raw(0u));
+    using E = decltype(pre(0u));
     using AF = AxisFactor<Op>;
     using In = PlaneIn<V, R, E, AF>;
+    constexpr int D = sizeof(V) == 4 ? 2 : 1;
     __shared__ PlaneLDS<V> L[2];
     const V* __restrict__ fxa = face_x(op);
     const V* __restrict__ fya = face_y(op);
     const V* __restrict__ fza = face_z(op);
     const int h = threadIdx.x;
     const int lane = h & (TX - 1), row = h >> 6;
-    // the halo cell of thread h relative to the tile origin, and its LDS slot
+    // the halo cell of thread h relative to the tile origin (h < 136), and its LDS slot:
+    // h < 64 the south row (j0-1, i0+h), 64 <= h < 128 the north row (j0+TY, i0+h-64),
+    // 128 <= h < 132 the west column (j0+h-128, i0-1), 132 <= h < 136 the east column
+    // (j0+h-132, i0+TX); the halo faces: h < 64 the -y face of column i0+h,
+    // 128 <= h < 132 the -x face of row j0+h-128
     int hi = 0, hj = 0, hrow = 0, hcol = 0;
     if(h < TX)
     {
@@ -656,6 +677,7 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
         hcol = TX + 1;
     }
     const bool has_halo = h < 2 * TX + 2 * TY;
+    const bool fh_y = h < TX, fh_x = h >= 2 * TX && h < 2 * TX + TY;
     auto geo = [&](int t, int& i0, int& j0, int& k, int& col) {
         col = t / tw.s.nbz;
         k = t - col * tw.s.nbz;
@@ -663,92 +685,107 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
         i0 = bx * TX;
         j0 = by * TY;
     };
-    auto fetch = [&](int i0, int j0, int k, In& P) {
-        const int i = i0 + lane, j = j0 + row;
-        const bool in = i < g.nx && j < g.ny;
-        const long c = g.base + i + g.sx * j + g.sxy * k;
-        P.h = R{};
-        if(has_halo)
-        {
-            const int ih = i0 + hi, jh = j0 + hj;
-            if(ih >= 0 && ih < g.nx && jh >= 0 && jh < g.ny) P.h = raw(g.base + ih + g.sx * jh + g.sxy * k);
-        }
-        P.fh = V(0);
-        if(h < TX)
-        {
-            if(i < g.nx && j0 < g.ny) P.fh = fya[g.base + i + g.sx * j0 + g.sxy * k - g.sx];
-        }
-        else if(h >= 2 * TX && h < 2 * TX + TY)
-        {
-            const int jr = j0 + (h - 2 * TX);
-            if(jr < g.ny) P.fh = fxa[g.base + i0 + g.sx * jr + g.sxy * k - 1];
-        }
-        P.f0 = in ? fxa[c] : V(0);
-        P.f2 = in ? fya[c] : V(0);
-        P.f4 = in ? fza[c] : V(0);
-        P.xu = (in && k + 1 < g.nz) ? raw(c + g.sxy) : R{};
-        P.az = axis_factor(g, op, 2, k);
-        P.e = in ? pre(c) : E{};
+    // per-column state: the thread's cell, which halo/face loads are real
+    int i0, j0, k, col, i, j;
+    bool in = false, hval = false, fhval = false;
+    unsigned coff = 0, hoff = 0, fhoff = 0;   // element offsets within plane k = 0
+    auto column_setup = [&]() {
+        i = i0 + lane;
+        j = j0 + row;
+        in = i < g.nx && j < g.ny;
+        const int ic = min(i, g.nx - 1), jc = min(j, g.ny - 1);
+        coff = (unsigned)(g.base + ic + g.sx * jc);
+        const int ih = i0 + hi, jh = j0 + hj;
+        hval = has_halo && ih >= 0 && ih < g.nx && jh >= 0 && jh < g.ny;
+        hoff = (unsigned)(g.base + max(0, min(ih, g.nx - 1)) + g.sx * max(0, min(jh, g.ny - 1)));
+        const int jr = j0 + (h - 2 * TX);
+        fhval = (fh_y && i < g.nx && j0 < g.ny) || (fh_x && jr < g.ny);
+        if(fh_y)
+            fhoff = (unsigned)(g.base + ic + g.sx * (min(j0, g.ny - 1) - 1));
+        else if(fh_x)
+            fhoff = (unsigned)(g.base + i0 - 1 + g.sx * min(jr, g.ny - 1));
+        else
+            fhoff = coff;
     };
-    int t = tw.t;
-    if(t >= tw.t1) return;
-    int i0, j0, k, col;
-    geo(t, i0, j0, k, col);
-    int i = i0 + lane, j = j0 + row;
+    auto fetch = [&](int kk, In& P) {
+        const unsigned pk = (unsigned)(g.sxy * kk);
+        const unsigned c = coff + pk;
+        P.h = raw(hoff + pk);
+        P.fh = ld32(fh_y ? fya : fxa, fhoff + pk);
+        P.f0 = ld32(fxa, c);
+        P.f2 = ld32(fya, c);
+        P.f4 = ld32(fza, c);
+        P.xu = raw(c + (unsigned)g.sxy);   // k + 1 <= nz: a ghost plane at worst
+        P.az = axis_factor(g, op, 2, kk);
+        P.e = pre(c);
+    };
     // k-window of the operand: xd (k-1), xc (k) and the centre's aux; fzd: the -z face;
     // ax, ay: the column's x and y grid factors
     V xd, xc, auxc, fzd;
-    AF ax{}, ay{};
-    auto column_start = [&]() {
-        const bool in = i < g.nx && j < g.ny;
-        const long c = g.base + i + g.sx * j + g.sxy * k;
+    AF ax, ay;
+    int t = tw.t;
+    if(t >= tw.t1) return;
+    geo(t, i0, j0, k, col);
+    bool v1 = false;   // D = 2: the buffer after the current one holds plane t+1
+    auto start = [&](In& P0, In& P1) {
+        column_setup();
+        const unsigned c = coff + (unsigned)(g.sxy * k);
         V d;
-        xd = (in && k > 0) ? val(raw(c - g.sxy), d) : V(0);
+        xd = (in && k > 0) ? val(raw(c - (unsigned)g.sxy), d) : V(0);
         auxc = V(0);
         xc = in ? val(raw(c), auxc) : V(0);
-        fzd = in ? fza[c - g.sxy] : V(0);
-        if(in)
-        {
-            ax = axis_factor(g, op, 0, i);
-            ay = axis_factor(g, op, 1, j);
-        }
+        fzd = in ? ld32(fza, c - (unsigned)g.sxy) : V(0);
+        ax = axis_factor(g, op, 0, min(i, g.nx - 1));
+        ay = axis_factor(g, op, 1, min(j, g.ny - 1));
+        fetch(k, P0);
+        v1 = false;
+        if constexpr(D == 2)
+            if(t + 1 < tw.t1)
+            {
+                int a, b2, kk, cc;
+                geo(t + 1, a, b2, kk, cc);
+                if(cc == col)
+                {
+                    fetch(kk, P1);
+                    v1 = true;
+                }
+            }
     };
-    column_start();
-    In cur;
-    fetch(i0, j0, k, cur);
     int buf = 0;
-    #pragma unroll 1
-    for(;; ++t)
-    {
-        const bool in = i < g.nx && j < g.ny;
-        const long c = g.base + i + g.sx * j + g.sxy * k;
+    // one plane: cur is plane t, n1 plane t+1 (D = 2), nf receives plane t+D; false at the end
+    auto step = [&](In& cur, In& n1, In& nf) -> bool {
         PlaneLDS<V>& P = L[buf];
         {
+            V d;
+            const V hv = val(cur.h, d);
             P.x[row + 1][lane + 1] = xc;
-            if(has_halo)
-            {
-                V d;
-                P.x[hrow][hcol] = val(cur.h, d);
-            }
-            P.fx[row][lane + 1] = cur.f0;
-            P.fy[row + 1][lane] = cur.f2;
-            if(h < TX)
-                P.fy[0][h] = cur.fh;
-            else if(h >= 2 * TX && h < 2 * TX + TY)
-                P.fx[h - 2 * TX][0] = cur.fh;
+            if(has_halo) P.x[hrow][hcol] = hval ? hv : V(0);
+            P.fx[row][lane + 1] = in ? cur.f0 : V(0);
+            P.fy[row + 1][lane] = in ? cur.f2 : V(0);
+            const V fh = fhval ? cur.fh : V(0);
+            if(fh_y)
+                P.fy[0][h] = fh;
+            else if(fh_x)
+                P.fx[h - 2 * TX][0] = fh;
         }
-        // the next plane of the same column: its loads go out now
-        const bool more = t + 1 < tw.t1;
-        int i2 = i0, j2 = j0, k2 = k, col2 = col;
-        if(more) geo(t + 1, i2, j2, k2, col2);
-        const bool same = more && col2 == col;
-        In nxt;
-        if(same) fetch(i2, j2, k2, nxt);
+        bool vf = false;
+        if(t + D < tw.t1 && (D == 1 || v1))
+        {
+            int a, b2, kk, cc;
+            geo(t + D, a, b2, kk, cc);
+            if(cc == col)
+            {
+                fetch(kk, nf);
+                vf = true;
+            }
+        }
         __syncthreads();
         V auxu = V(0);
-        const V xu = val(cur.xu, auxu);
+        const V xu = (in && k + 1 < g.nz) ? val(cur.xu, auxu) : V(0);
+        if(!(in && k + 1 < g.nz)) auxu = V(0);
         if(in)
         {
+            const unsigned c = coff + (unsigned)(g.sxy * k);
             const CouplingV<V> cq =
                 coupling_axes(op, cur.f0, P.fx[row][lane], cur.f2, P.fy[row][lane], cur.f4, fzd, ax, ay, cur.az);
             const V Ax = stencil7(g, cq, xc, P.x[row + 1][lane + 2], P.x[row + 1][lane], P.x[row + 2][lane + 1],
@@ -756,15 +793,18 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
             post(c, Ax, auxc, cur.e);
         }
         buf ^= 1;
-        if(!more) break;
-        if(same)
+        if(t + 1 >= tw.t1) return false;
+        int i2, j2, k2, col2;
+        geo(t + 1, i2, j2, k2, col2);
+        ++t;
+        if(col2 == col)
         {
             xd = xc;
             xc = xu;
             auxc = auxu;
-            fzd = cur.f4;
-            cur = nxt;
+            fzd = in ? cur.f4 : V(0);
             k = k2;
+            v1 = vf;
         }
         else
         {
@@ -772,10 +812,37 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
             j0 = j2;
             k = k2;
             col = col2;
-            i = i0 + lane;
-            j = j0 + row;
-            column_start();
-            fetch(i0, j0, k, cur);
+            // the next step's current buffer (and its successor) restart the column
+            if constexpr(D == 1)
+                start(nf, n1);
+            else
+                start(n1, nf);
+        }
+        return true;
+    };
+    if constexpr(D == 1)
+    {
+        // one buffer ahead: the next plane lands in nx and is moved into cur (registers
+        // allow no second buffer for 8-byte operands at this occupancy)
+        In cur, nx;
+        start(cur, nx);
+        #pragma unroll 1
+        for(;;)
+        {
+            if(!step(cur, nx, nx)) break;
+            cur = nx;
+        }
+    }
+    else
+    {
+        In X, Y, Z;
+        start(X, Y);
+        #pragma unroll 1
+        for(;;)
+        {
+            if(!step(X, Y, Z)) break;
+            if(!step(Y, Z, X)) break;
+            if(!step(Z, X, Y)) break;
         }
     }
 }
@@ -883,15 +950,21 @@ __global__ void __launch_bounds__(NT, OCC_A64) kb_bicg_p(TankSet b, int x, int r
             continue;
         }
         const V beta = (V)st->beta, w = (V)st->w;
-        #pragma unroll 1
+        const V* __restrict__ r = v.r;
+        const V* __restrict__ vv = v.v;
+        const V* __restrict__ aii = v.aii;
+        V* __restrict__ p = v.p;
+        V* __restrict__ ph = v.ph;
+        // independent tiles: unrolled so that several tiles' loads are in flight
+        #pragma unroll 4
         for(int t = tw.t; t < tw.t1; ++t)
         {
             int i, j, k;
             long c;
             if(!tile_cell(g, tw, t, i, j, k, c)) continue;
-            const V pn = v.r[c] + beta * (v.p[c] - w * v.v[c]);
-            v.p[c] = pn;
-            v.ph[c] = pn * v.aii[c];
+            const V pn = r[c] + beta * (p[c] - w * vv[c]);
+            p[c] = pn;
+            ph[c] = pn * aii[c];
         }
     }
 }
@@ -950,10 +1023,10 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL :
                 V r0, r;
             };
             plane_walk<V>(
-                g, op, tw, [&](long cc) { return ph[cc]; }, [](V x, V&) { return x; },
-                [&](long c) { return EpiIn{r0[c], r[c]}; },
-                [&](long c, V Ax, V, const EpiIn& e) {
-                    vo[c] = Ax;
+                g, op, tw, [&](unsigned cc) { return ld32(ph, cc); }, [](V x, V&) { return x; },
+                [&](unsigned c) { return EpiIn{ld32(r0, c), ld32(r, c)}; },
+                [&](unsigned c, V Ax, V, const EpiIn& e) {
+                    st32(vo, c, Ax);
                     const double vd = Ax, rc = e.r;
                     vals[0] += vd * double(e.r0);
                     vals[1] += vd * vd;
@@ -1040,12 +1113,19 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL :
                 V r, v, a;
             };
             plane_walk<V>(
-                g, op, tw, [&](long cc) { return Rin{r[cc], v[cc], a[cc]}; },
+                g, op, tw, [&](unsigned cc) { return Rin{ld32(r, cc), ld32(v, cc), ld32(a, cc)}; },
                 [&](const Rin& q, V& sv) {
                     sv = q.r - alpha * q.v;
                     return sv * q.a;
                 },
-                [](long) { return NoEpi{}; }, [&](long c, V Ax, V s, const NoEpi&) { epilogue(c, Ax, s); });
+                [](unsigned) { return NoEpi{}; },
+                [&](unsigned c, V Ax, V s, const NoEpi&) {
+                    st32(to, c, Ax);
+                    const double td = Ax, sd = s;
+                    vals[0] += td * sd;
+                    vals[1] += td * td;
+                    vals[2] += sd * sd;
+                });
         }
         else
             #pragma unroll 1
@@ -1098,28 +1178,37 @@ __global__ void __launch_bounds__(NT, OCC_FULL) kb_bicg_x(TankSet b, int x, int 
         const V alpha = (V)st->alpha, w = (V)st->w;
         const int branch = st->branch;
         double vals[2] = {0.0, 0.0};
-        #pragma unroll 1
+        V* __restrict__ xv = bv.x;
+        V* __restrict__ r = bv.r;
+        const V* __restrict__ vv = bv.v;
+        const V* __restrict__ aii = bv.aii;
+        const V* __restrict__ ph = bv.ph;
+        const V* __restrict__ tv = bv.t;
+        const V* __restrict__ r0 = bv.r0;
+        // independent tiles (the dot sums keep their order): unrolled so that several
+        // tiles' loads are in flight
+        #pragma unroll 4
         for(int t = tw.t; t < tw.t1; ++t)
         {
             int i, j, k;
             long c;
             if(!tile_cell(g, tw, t, i, j, k, c)) continue;
-            const V s = bv.r[c] - alpha * bv.v[c];
+            const V s = r[c] - alpha * vv[c];
             V rn;
             if(branch)
             {
-                const V sh = s * bv.aii[c];
-                bv.x[c] += alpha * bv.ph[c] + w * sh;
-                rn = s - w * bv.t[c];
+                const V sh = s * aii[c];
+                xv[c] += alpha * ph[c] + w * sh;
+                rn = s - w * tv[c];
             }
             else
             {
-                bv.x[c] += alpha * bv.ph[c];
+                xv[c] += alpha * ph[c];
                 rn = s;
             }
-            bv.r[c] = rn;
+            r[c] = rn;
             const double rd = rn;
-            vals[0] += rd * double(bv.r0[c]);
+            vals[0] += rd * double(r0[c]);
             vals[1] += rd * rd;
         }
         block_sum_to<2>(vals, b.partials + (size_t)item_slot(T, b, q) * 2);
@@ -1269,7 +1358,13 @@ __global__ void __launch_bounds__(NT, OCC_FULL) kb_mix_p(TankSet b, int x)
         double* __restrict__ x64 = T.f[x];
         const float beta = (float)st->beta, w = (float)st->w;
         const TileWalk tw = item_walk(T, b, q);
-        #pragma unroll 1
+        float* __restrict__ x32 = v.x;
+        const float* __restrict__ r = v.r;
+        const float* __restrict__ vv = v.v;
+        const float* __restrict__ aii = v.aii;
+        float* __restrict__ p = v.p;
+        float* __restrict__ ph = v.ph;
+        #pragma unroll 4
         for(int t = tw.t; t < tw.t1; ++t)
         {
             int i, j, k;
@@ -1277,14 +1372,14 @@ __global__ void __launch_bounds__(NT, OCC_FULL) kb_mix_p(TankSet b, int x)
             if(!tile_cell(g, tw, t, i, j, k, c)) continue;
             if(pend)
             {
-                x64[c] += (double)v.x[c];
-                v.x[c] = 0.0f;
+                x64[c] += (double)x32[c];
+                x32[c] = 0.0f;
             }
             if(upd)
             {
-                const float pn = v.r[c] + beta * (v.p[c] - w * v.v[c]);
-                v.p[c] = pn;
-                v.ph[c] = pn * v.aii[c];
+                const float pn = r[c] + beta * (p[c] - w * vv[c]);
+                p[c] = pn;
+                ph[c] = pn * aii[c];
             }
         }
     }
