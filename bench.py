@@ -66,9 +66,10 @@ SOLVER_BYTES = {
     "bicg_x": 72,   # x, ph, r, v, aii, t, r0 in; x, r out
 }
 # the same kernels of the FP32 inner solve of the mixed-precision projection
-SOLVER_BYTES.update({"mix_p": 24, "bicg32_v": 28, "bicg32_t": 28, "bicg32_x": 36})
+# mix_v: the FP32 p update fused with v = A ph (r, p, v, aii, 3 conductances, r0 in; v, p, ph out)
+SOLVER_BYTES.update({"mix_v": 44, "bicg32_t": 28, "bicg32_x": 36})
 HBM_PEAK_GBS = 8000.0
-TIMED_FAMILIES = ("bicg_init", "bicg_p", "bicg_v", "bicg_t", "bicg_x", "mix_init", "mix_p", "bicg32_v",
+TIMED_FAMILIES = ("bicg_init", "bicg_p", "bicg_v", "bicg_t", "bicg_x", "mix_init", "mix_v",
                   "bicg32_t", "bicg32_x", "mix_ru", "coef32", "add32", "hj", "reini", "momentum", "ghost",
                   "face_rho", "correct", "poisson_rhs", "heaviside", "copy", "flux")
 
@@ -300,7 +301,11 @@ def tanks(args):
     # the tile plans: tuned once on the batch, shared by both runs
     B, _ = tank_batch(specs, "mixed", graphs)
     t_tune = time.perf_counter()
-    if not args.no_tune:
+    if args.cpt:
+        # a fixed chunk length for every size bucket of the batch
+        for sp in specs:
+            B.set_plan(0, int(math.floor(math.log2(sp["nx"] * sp["nz"]))), 1, 4, args.cpt)
+    elif not args.no_tune:
         B.autotune(reps=2)
     t_tune = time.perf_counter() - t_tune
     plans = B.plans()
@@ -433,7 +438,10 @@ def serve(args):
     from reef3d_amd import Batch, Server
     reqs = c4_requests(args.requests, args.step_div)
     fields = [request_fields(r) for r in reqs]
-    S = Server(hbm_budget_bytes=200e9, precision="mixed", graphs=False)
+    # tile plans of the served tanks: 2-D chunks of args.cpt tiles per size bucket
+    plans = sorted({(0, int(math.floor(math.log2(r["nx"] * r["nz"]))), 1, 4, args.cpt)
+                    for r in reqs if r["jdir"] == 0}) if args.cpt else []
+    S = Server(hbm_budget_bytes=200e9, precision=args.serve_precision, graphs=False, plans=plans)
     ids = []
     t_start = time.perf_counter()
     for r, f in zip(reqs, fields):
@@ -492,11 +500,12 @@ def serve(args):
         "n_gpus": 1,
         "higher_is_better": True,
         "vs_baseline": None,
-        "dtype": "f64 (FP32 inner solve)",
+        "dtype": "f64 (FP32 inner solve)" if args.serve_precision == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": f"served_tank_stream: {len(reqs)} requests, Poisson arrivals 200/s (seed 1004), "
                                f"{n2d} 2-D / {len(reqs) - n2d} 3-D, steps 50..500 / {args.step_div}, HBM budget 200 GB, "
-                               f"mixed-precision projection to ||r|| <= 1e-10 ||b||"},
+                               f"{args.serve_precision} projection to ||r|| <= 1e-10 ||b||",
+                   "plans": [list(p_) for p_ in plans]},
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                       "max": round(float(lat.max()), 3)},
         "makespan_s": round(makespan, 3),
@@ -514,6 +523,7 @@ def serve(args):
 def solves(args, cfg):
     """Standalone FP64 vs mixed-precision solves on the wave tank's step-0 operator."""
     D, _ = make_domain(cfg)
+    D.solver_config("fp64", args.delta)   # the reliable-update threshold of the mixed solves
     nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
     cells = nx * ny * nz
     tol = cfg["rtol"]
@@ -566,6 +576,7 @@ def solves(args, cfg):
                   "reliable_updates_mean": round(float(a[:, 3].mean()), 2),
                   "final_rel_residual_max": float(a[:, 6].max())},
         "speedup_mixed_vs_fp64": round(t64 / tm, 3),
+        "reliable_update_delta": args.delta,
         "max_abs_p_minus_p64": float(a[:, 7].max()),
         "max_rel_p_minus_p64": float((a[:, 7] / a[:, 8]).max()),
     }
@@ -582,11 +593,14 @@ def main():
     ap.add_argument("--workload", choices=("fc3", "solves", "vortex", "tanks", "serve"), default="fc3")
     ap.add_argument("--requests", type=int, default=16, help="requests of --workload serve")
     ap.add_argument("--step-div", type=int, default=10, help="--workload serve: steps per request divided by this")
+    ap.add_argument("--serve-precision", choices=("fp64", "mixed"), default="mixed", help="--workload serve: projection")
     ap.add_argument("--sequential", type=int, default=8, help="tanks run one at a time in --workload tanks")
     ap.add_argument("--no-graphs", action="store_true", help="--workload tanks: launch by launch")
     ap.add_argument("--no-tune", action="store_true", help="--workload tanks: default tile plans")
+    ap.add_argument("--cpt", type=int, default=0, help="--workload tanks / serve: this chunk length for every 2-D bucket")
     ap.add_argument("--check", type=int, default=3, help="oracle parity steps of --workload vortex")
     ap.add_argument("--solves", type=int, default=100, help="right-hand sides of --workload solves")
+    ap.add_argument("--delta", type=float, default=0.1, help="reliable-update threshold of the mixed solve")
     args = ap.parse_args()
     args.steps_given = args.steps is not None
     if args.steps is None:
@@ -666,7 +680,7 @@ def main():
             Dm.momentum_fc3()
         el_m, it_m, fams_m = timed_steps(Dm, args.steps)
         _, _, rel = Dm.solver_last()
-        dm = max(("mix_p", "bicg32_v", "bicg32_t", "bicg32_x"), key=lambda k: fams_m[k][0])
+        dm = max(("mix_v", "bicg32_t", "bicg32_x"), key=lambda k: fams_m[k][0])
         ms_m, n_m = fams_m[dm]
         avg_m = ms_m / 1e3 / max(n_m, 1)
         out["mixed_projection"] = {

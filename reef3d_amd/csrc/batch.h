@@ -16,10 +16,11 @@ enum Arr64
     A_R, A_R0, A_P, A_PH, A_V, A_T, A_AII, A_RFX, A_RFY, A_RFZ,
     NARR64
 };
-// FP32 arrays of the mixed-precision solve
+// FP32 arrays of the mixed-precision solve; p and v are double buffered (S_P / S_P2,
+// S_V / S_V2, BiState::buf selects the current half)
 enum Arr32
 {
-    S_R, S_R0, S_P, S_PH, S_V, S_T, S_X, S_KX, S_KY, S_KZ, S_AII,
+    S_R, S_R0, S_P, S_PH, S_V, S_T, S_X, S_KX, S_KY, S_KZ, S_AII, S_P2, S_V2,
     NARR32
 };
 

@@ -325,6 +325,36 @@ __device__ __forceinline__ CouplingV<float> coupling_axes(const SolverOp32&, flo
     return q;
 }
 
+// the x and z couplings of a 2-D (x-z) cell; the y couplings are 0 (the operator weights
+// them by 0)
+__device__ __forceinline__ CouplingV<double> coupling_xz(const SolverOp64&, double F0, double F1, double F4,
+                                                         double F5, const AxisFactor<SolverOp64>& ax,
+                                                         const AxisFactor<SolverOp64>& az)
+{
+    CouplingV<double> q;
+    q.T[0] = (1.0 * 1.0) / (F0 * ax.p0 * ax.n);
+    q.T[1] = (1.0 * 1.0) / (F1 * ax.p1 * ax.n);
+    q.T[2] = 0.0;
+    q.T[3] = 0.0;
+    q.T[4] = (1.0 * 1.0) / (F4 * az.p0 * az.n);
+    q.T[5] = (1.0 * 1.0) / (F5 * az.p1 * az.n);
+    return q;
+}
+
+__device__ __forceinline__ CouplingV<float> coupling_xz(const SolverOp32&, float F0, float F1, float F4, float F5,
+                                                        const AxisFactor<SolverOp32>& ax,
+                                                        const AxisFactor<SolverOp32>& az)
+{
+    CouplingV<float> q;
+    q.T[0] = F0 * ax.inv;
+    q.T[1] = F1 * ax.inv;
+    q.T[2] = 0.0f;
+    q.T[3] = 0.0f;
+    q.T[4] = F4 * az.inv;
+    q.T[5] = F5 * az.inv;
+    return q;
+}
+
 __device__ __forceinline__ const double* face_x(const SolverOp64& op) { return op.rfx; }
 __device__ __forceinline__ const double* face_y(const SolverOp64& op) { return op.rfy; }
 __device__ __forceinline__ const double* face_z(const SolverOp64& op) { return op.rfz; }
@@ -364,6 +394,23 @@ __device__ __forceinline__ V stencil7(const DevGrid& g, const CouplingV<V>& q, V
     const V xw = j == g.ny - 1 ? V(0) : xw_;
     const V xe = j == 0 ? V(0) : xe_;
     return Mp * xc + Mn * xn + Ms * xs + Mw * xw + Me * xe + Mt * xt + Mb * xb;
+}
+
+// the 2-D (x-z) operator of apply_A_fn from the five values (xn/xs: i+-1, xt/xb: k+-1)
+template<class V>
+__device__ __forceinline__ V stencil5(const DevGrid& g, const CouplingV<V>& q, V xc, V xn_, V xs_, V xt_, V xb_, int i,
+                                      int k)
+{
+    const V Mp = diag_of(g, q);
+    const V Mn = i == g.nx - 1 ? V(0) : -q.T[0];
+    const V Ms = i == 0 ? V(0) : -q.T[1];
+    const V Mt = k == g.nz - 1 ? V(0) : -q.T[4];
+    const V Mb = k == 0 ? V(0) : -q.T[5];
+    const V xn = i == g.nx - 1 ? V(0) : xn_;
+    const V xs = i == 0 ? V(0) : xs_;
+    const V xt = k == g.nz - 1 ? V(0) : xt_;
+    const V xb = k == 0 ? V(0) : xb_;
+    return Mp * xc + Mn * xn + Ms * xs + Mt * xt + Mb * xb;
 }
 
 template<class V, class XV>

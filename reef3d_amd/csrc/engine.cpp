@@ -181,26 +181,65 @@ size_t Engine::tank_bytes(const reef_grid_desc& gd)
     return b;
 }
 
+int Engine::build_tank(const reef_grid_desc& gd, const reef_params& p, TankBuild& b, std::string& err)
+{
+    b.prm = p;
+    std::string e = build_grid(b.hg, gd, p.F45);
+    if(!e.empty()) return err = e, REEF_E_UNSUPPORTED;
+    const HostGrid& h = b.hg;
+    // ghost programs first (may refuse the geometry)
+    std::vector<GhostOp> progs[NPROG];
+    e = build_ghost_programs(h, NPROG, PROG_KIND_GCV, progs);
+    if(!e.empty()) return err = "ghost program: " + e, REEF_E_UNSUPPORTED;
+    b.ops.clear();
+    for(int q = 0; q < NPROG; ++q)
+    {
+        b.op0[q] = (int)b.ops.size();
+        b.nop[q] = (int)progs[q].size();
+        b.ops.insert(b.ops.end(), progs[q].begin(), progs[q].end());
+    }
+    // metadata: P, DN, DP per axis, WENO tables, FP32 1/DN, in one block
+    b.meta.clear();
+    b.inv.clear();
+    for(int a = 0; a < 3; ++a)
+    {
+        b.offP[a] = b.meta.size();
+        b.meta.insert(b.meta.end(), h.P[a].begin(), h.P[a].end());
+        b.offDN[a] = b.meta.size();
+        b.meta.insert(b.meta.end(), h.DN[a].begin(), h.DN[a].end());
+        b.offDP[a] = b.meta.size();
+        b.meta.insert(b.meta.end(), h.DP[a].begin(), h.DP[a].end());
+        b.offqf[a] = b.meta.size();
+        b.meta.insert(b.meta.end(), h.qf[a].begin(), h.qf[a].end());
+        b.offcf[a] = b.meta.size();
+        b.meta.insert(b.meta.end(), h.cf[a].begin(), h.cf[a].end());
+        b.offisf[a] = b.meta.size();
+        b.meta.insert(b.meta.end(), h.isf[a].begin(), h.isf[a].end());
+    }
+    for(int a = 0; a < 3; ++a)
+    {
+        b.offinv[a] = b.inv.size();
+        for(double d : h.DN[a]) b.inv.push_back(d != 0.0 ? (float)(1.0 / d) : 0.0f);
+    }
+    return 0;
+}
+
 int Engine::add_tank(const reef_grid_desc& gd, const reef_params& p, int* id)
+{
+    TankBuild b;
+    std::string e;
+    const int rc = build_tank(gd, p, b, e);
+    if(rc) return fail(rc, e);
+    return add_built(b, id);
+}
+
+int Engine::add_built(TankBuild& b, int* id)
 {
     TankHost t;
     t.used = true;
-    t.prm = p;
-    std::string e = build_grid(t.hg, gd, p.F45);
-    if(!e.empty()) return fail(REEF_E_UNSUPPORTED, e);
+    t.prm = b.prm;
+    t.hg = std::move(b.hg);
     const HostGrid& h = t.hg;
-    // ghost programs first (may refuse the geometry)
-    std::vector<GhostOp> ops;
-    std::vector<int> op0(NPROG), nop(NPROG);
-    for(int q = 0; q < NPROG; ++q)
-    {
-        std::vector<GhostOp> one;
-        e = build_ghost_program(h, PROG_KIND_GCV[q][0], PROG_KIND_GCV[q][1], one);
-        if(!e.empty()) return fail(REEF_E_UNSUPPORTED, "ghost program: " + e);
-        op0[q] = (int)ops.size();
-        nop[q] = (int)one.size();
-        ops.insert(ops.end(), one.begin(), one.end());
-    }
     TankDev& D = t.dev;
     std::memset(&D, 0, sizeof(D));
     DevGrid& g = D.g;
@@ -224,6 +263,7 @@ int Engine::add_tank(const reef_grid_desc& gd, const reef_params& p, int* id)
     };
     auto release = [&]() {
         for(void* m : t.mem) pool.free(m);
+        b.hg = std::move(t.hg);   // the build stays usable for another attempt
     };
     for(int a = 0; a < NARR64; ++a)
     {
@@ -239,31 +279,9 @@ int Engine::add_tank(const reef_grid_desc& gd, const reef_params& p, int* id)
         D.s[a] = (float*)m;
         EHIP(hipMemsetAsync(m, 0, (size_t)h.nalloc * 4, stream));
     }
-    // metadata: P, DN, DP per axis, WENO tables, FP32 1/DN, in one block
-    std::vector<double> meta;
-    size_t offP[3], offDN[3], offDP[3], offqf[3], offcf[3], offisf[3];
-    for(int a = 0; a < 3; ++a)
-    {
-        offP[a] = meta.size();
-        meta.insert(meta.end(), h.P[a].begin(), h.P[a].end());
-        offDN[a] = meta.size();
-        meta.insert(meta.end(), h.DN[a].begin(), h.DN[a].end());
-        offDP[a] = meta.size();
-        meta.insert(meta.end(), h.DP[a].begin(), h.DP[a].end());
-        offqf[a] = meta.size();
-        meta.insert(meta.end(), h.qf[a].begin(), h.qf[a].end());
-        offcf[a] = meta.size();
-        meta.insert(meta.end(), h.cf[a].begin(), h.cf[a].end());
-        offisf[a] = meta.size();
-        meta.insert(meta.end(), h.isf[a].begin(), h.isf[a].end());
-    }
-    size_t offinv[3];
-    std::vector<float> inv;
-    for(int a = 0; a < 3; ++a)
-    {
-        offinv[a] = inv.size();
-        for(double d : h.DN[a]) inv.push_back(d != 0.0 ? (float)(1.0 / d) : 0.0f);
-    }
+    const std::vector<double>& meta = b.meta;
+    const std::vector<float>& inv = b.inv;
+    const std::vector<GhostOp>& ops = b.ops;
     const size_t metabytes = up(meta.size() * 8);
     void* mb;
     if((rc = take(metabytes + inv.size() * 4 + ops.size() * sizeof(GhostOp) + 64, &mb))) return release(), rc;
@@ -281,23 +299,23 @@ int Engine::add_tank(const reef_grid_desc& gd, const reef_params& p, int* id)
     const double** d1[3][3] = {{&g.XP, &g.DXN, &g.DXP}, {&g.YP, &g.DYN, &g.DYP}, {&g.ZP, &g.DZN, &g.DZP}};
     for(int a = 0; a < 3; ++a)
     {
-        *d1[a][0] = md + offP[a];
-        *d1[a][1] = md + offDN[a];
-        *d1[a][2] = md + offDP[a];
-        g.qf[a] = md + offqf[a];
-        g.cf[a] = md + offcf[a];
-        g.isf[a] = md + offisf[a];
+        *d1[a][0] = md + b.offP[a];
+        *d1[a][1] = md + b.offDN[a];
+        *d1[a][2] = md + b.offDP[a];
+        g.qf[a] = md + b.offqf[a];
+        g.cf[a] = md + b.offcf[a];
+        g.isf[a] = md + b.offisf[a];
     }
-    D.iDXN32 = invd + offinv[0];
-    D.iDYN32 = invd + offinv[1];
-    D.iDZN32 = invd + offinv[2];
+    D.iDXN32 = invd + b.offinv[0];
+    D.iDYN32 = invd + b.offinv[1];
+    D.iDZN32 = invd + b.offinv[2];
     for(int q = 0; q < NPROG; ++q)
     {
-        D.gops[q] = opd + op0[q];
-        D.gnops[q] = nop[q];
+        D.gops[q] = opd + b.op0[q];
+        D.gnops[q] = b.nop[q];
     }
     D.shape = shape_of(h);
-    D.dt = p.dt;
+    D.dt = t.prm.dt;
     // slot
     int slot = -1;
     for(size_t q = 0; q < tanks.size(); ++q)
@@ -703,16 +721,16 @@ int Engine::replay(const GraphKey& key, Body body)
 
 // ------------------------------------------------------------------ BiCGStab iterations
 static const char* const BICG_NAMES[5] = {"bicg_p", "bicg_v", "bicg_t", "bicg_x", ""};
-static const char* const MIX_NAMES[5] = {"mix_p", "bicg32_v", "bicg32_t", "bicg32_x", "mix_ru"};
+static const char* const MIX_NAMES[4] = {"mix_v", "bicg32_t", "bicg32_x", "mix_ru"};
 
 // 8 iterations of the tanks of the current sets, then their states gathered into ring
 // slot `ring` and copied to the host.  Iterations after a tank's end are no-ops for it.
-int Engine::enqueue_batch(bool mixed, int x, int rhs, int ring)
+int Engine::enqueue_batch(bool mixed, int x, int rhs, int ring, int iters)
 {
-    const int nph = mixed ? 5 : 4;
+    const int nph = 4;
     const char* const* names = mixed ? MIX_NAMES : BICG_NAMES;
     auto body = [&]() -> int {
-        for(int q = 0; q < 8; ++q)
+        for(int q = 0; q < iters; ++q)
             for(int ph = 0; ph < nph; ++ph)
                 for(int part = 0; part < 2; ++part)
                 {
@@ -730,7 +748,7 @@ int Engine::enqueue_batch(bool mixed, int x, int rhs, int ring)
     int rc;
     // the lockstep sets only change with the tables; the scheduler's change every round
     if(cur_ == &all_)
-        rc = replay({1, mixed ? 1 : 0, x, rhs}, body);
+        rc = replay({1, mixed ? 1 : 0, x, rhs * 64 + iters}, body);
     else
         rc = body();
     if(rc) return rc;
@@ -751,10 +769,10 @@ int Engine::run_iterations(bool mixed, int x, int rhs, int maxiter)
     for(const TankSet& S : *cur_)
         for(int p = 0; p < S.n; ++p) members.push_back(S.slot[p]);
     int rc;
-    if((rc = enqueue_batch(mixed, x, rhs, 0))) return rc;
+    if((rc = enqueue_batch(mixed, x, rhs, 0, 8))) return rc;
     for(long k = 1;; ++k)
     {
-        if((rc = enqueue_batch(mixed, x, rhs, (int)(k & 1)))) return rc;
+        if((rc = enqueue_batch(mixed, x, rhs, (int)(k & 1), 8))) return rc;
         EHIP(hipEventSynchronize(st_ev_[(k - 1) & 1]));
         const BiState* hs = h_status_ + (size_t)((k - 1) & 1) * cap_tanks_;
         bool all = true;
@@ -1100,7 +1118,9 @@ int Engine::pump(std::vector<int>& stepped)
         }
         const int ring = (int)(batch_no_++ & 1);
         cur_ = &solving_sets_;
-        rc = enqueue_batch(mixed, REEF_PCORR, A_RHS, ring);
+        // many tanks solving: a deeper batch per round amortises the round's segment
+        // launches and host synchronisation (iterations after a tank's end are no-ops)
+        rc = enqueue_batch(mixed, REEF_PCORR, A_RHS, ring, (int)solving.size() >= 8 ? 32 : 8);
         cur_ = &all_;
         if(rc) return rc;
         Inflight f;
@@ -1211,7 +1231,7 @@ int Engine::probe(double& stencil_ms, double& solver_ms)
     });
     if((rc = solve_begin(REEF_PCORR, A_RHS, 1e-300, 16, mixed))) return rc;
     for(int it = 0; it < 16; ++it)
-        for(int ph = 0; ph < (mixed ? 5 : 4); ++ph)
+        for(int ph = 0; ph < 4; ++ph)
             for(int part = 0; part < 2; ++part)
                 each([&](const TankSet& S) {
                     if(mixed)

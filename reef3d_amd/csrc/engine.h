@@ -67,6 +67,19 @@ struct TankHost
     long iters_last = 0;            // iterations of the last completed step
 };
 
+// host half of a tank (grid metadata, ghost programs and the tables its device block
+// holds); Engine::build_tank makes it without touching the engine, on any thread
+struct TankBuild
+{
+    HostGrid hg;
+    reef_params prm{};
+    std::vector<GhostOp> ops;           // the NPROG ghost programs, concatenated
+    int op0[NPROG] = {}, nop[NPROG] = {};
+    std::vector<double> meta;           // P, DN, DP and WENO tables per axis
+    std::vector<float> inv;             // FP32 1/DN per axis
+    size_t offP[3] = {}, offDN[3] = {}, offDP[3] = {}, offqf[3] = {}, offcf[3] = {}, offisf[3] = {}, offinv[3] = {};
+};
+
 class Engine
 {
 public:
@@ -112,6 +125,10 @@ public:
     // ---- tanks
     static size_t tank_bytes(const reef_grid_desc& g);
     int add_tank(const reef_grid_desc& g, const reef_params& p, int* id);
+    // add_tank in two halves: the host build (thread-safe, no engine state) and the device
+    // part; add_built leaves b usable again when it fails
+    static int build_tank(const reef_grid_desc& g, const reef_params& p, TankBuild& b, std::string& err);
+    int add_built(TankBuild& b, int* id);
     int remove_tank(int id);
     int active_count() const;
     TankHost* tank(int id) { return id >= 0 && id < (int)tanks.size() && tanks[id].used ? &tanks[id] : nullptr; }
@@ -213,7 +230,7 @@ private:
     std::map<GraphKey, hipGraphExec_t> graphs_;
     template<class Body>
     int replay(const GraphKey& key, Body body);
-    int enqueue_batch(bool mixed, int x, int rhs, int ring);
+    int enqueue_batch(bool mixed, int x, int rhs, int ring, int iters);
     int drain();
     int read_batch(const Inflight& b);
     int run_iterations(bool mixed, int x, int rhs, int maxiter);

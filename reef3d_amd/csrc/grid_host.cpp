@@ -2,6 +2,7 @@
 // the compiled ghost-cell programs.  This is what lexer/grid_helper/ghostcell hold in
 // the reference (grid.cpp, weno_nug_func_precalc_*.cpp, grid_helper_*.cpp, gc_*.cpp),
 // computed once per context and uploaded.
+#include <algorithm>
 #include "reef_internal.h"
 
 #include <cmath>
@@ -85,6 +86,7 @@ int side_axis(int s) { return (s == 1 || s == 4) ? 0 : (s == 2 || s == 3) ? 1 : 
 std::vector<Face> box_faces(const HostGrid& g, int comp)
 {
     std::vector<Face> out;
+    out.reserve(2 * ((size_t)g.ny * g.nz + (size_t)g.nx * g.nz + (size_t)g.nx * g.ny));
     for(int i = 0; i < g.nx; ++i)
     for(int j = 0; j < g.ny; ++j)
     for(int k = 0; k < g.nz; ++k)
@@ -240,7 +242,12 @@ std::string build_grid(HostGrid& g, const reef_grid_desc& d, double F45)
     return "";
 }
 
-std::string build_ghost_program(const HostGrid& g, int kind, int gcv, std::vector<GhostOp>& ops)
+namespace {
+
+// one program of ghostcell::start<kind>(f, gcv); lists[1..4] are the boundary entries of
+// the u, v, w and scalar grids, hgc the mark array accumulated over lists[1..kind]
+std::string ghost_program(const HostGrid& g, const std::vector<Face>* lists, const std::vector<int>& hgc, int kind,
+                          int gcv, std::vector<GhostOp>& ops)
 {
     ops.clear();
     const long ncomp = (long)(g.nx + 6) * (g.ny + 6) * (g.nz + 6);
@@ -262,11 +269,6 @@ std::string build_ghost_program(const HostGrid& g, int kind, int gcv, std::vecto
         ops.push_back(op);
         return true;
     };
-    // the four velocity/scalar lists share one mark array (grid_helper.cpp:42-59: it is
-    // cleared once before fill_dgc1 and accumulates through fill_dgc4)
-    std::vector<int> hgc(ncomp, 0);
-    std::vector<Face> lists[5];
-    for(int c = 1; c <= 4; ++c) lists[c] = box_faces(g, c == 4 ? 0 : c);
     const std::vector<Face>& faces = lists[kind];
     const std::vector<Face>& faces1 = lists[1];
     // face ghosts (gcdistro1..4, gc_start.cpp:38-40 etc.)
@@ -318,20 +320,22 @@ std::string build_ghost_program(const HostGrid& g, int kind, int gcv, std::vecto
         {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1}, {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
         {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1}};
     std::vector<char> dwritten(ncomp, 0);
-    for(int c = 1; c <= kind; ++c)
     {
-        for(const Face& f : lists[c])
-        {
-            if(!f.active) continue;
-            const int* o = SIDE_D[f.side];
-            for(int n = 1; n <= 3; ++n) hgc[g.cid(f.i + o[0] * n, f.j + o[1] * n, f.k + o[2] * n)] += 1;
-        }
-        if(c != kind) continue;
+        const int c = kind;
         const std::vector<int>& fl = g.flag[c];
         const int il = c == 1, jl = c == 2, kl = c == 3;
-        for(int i = 0; i < g.nx - il; ++i)
-        for(int j = 0; j < g.ny - jl; ++j)
-        for(int k = 0; k < g.nz - kl; ++k)
+        const int ni = g.nx - il, nj = g.ny - jl, nk = g.nz - kl;
+        // the flags are positive exactly on the loop range.  Every direction has two or
+        // three nonzero components and each must lead out of the range, so only cells with
+        // at least two coordinates on its rim can emit: columns with i and j on the rim
+        // visit every k, columns with one of them their first and last k, others none
+        for(int i = 0; i < ni; ++i)
+        for(int j = 0; j < nj; ++j)
+        {
+            const int rim = (i == 0 || i == ni - 1) + (j == 0 || j == nj - 1);
+            if(rim == 0) continue;
+            const int kstep = rim == 2 ? 1 : std::max(1, nk - 1);
+        for(int k = 0; k < nk; k += kstep)
         {
             if(fl[g.cid(i, j, k)] <= 0) continue;
             for(const auto& dd : DIRS)
@@ -353,11 +357,54 @@ std::string build_ghost_program(const HostGrid& g, int kind, int gcv, std::vecto
                 ops.push_back(op);
             }
         }
+        }
     }
     // every source must be a cell this program does not write: one parallel pass is exact
     for(long c = 0; c < ncomp; ++c)
         if(read[c] && (written[c] || dwritten[c])) return "ghost program reads a cell it writes";
     return "";
+}
+
+}  // namespace
+
+std::string build_ghost_programs(const HostGrid& g, int n, const int (*kind_gcv)[2], std::vector<GhostOp>* ops)
+{
+    for(int q = 0; q < n; ++q)
+        if(kind_gcv[q][0] < 1 || kind_gcv[q][0] > 4) return "bad ghost program kind";
+    std::vector<Face> lists[5];
+    for(int c = 1; c <= 4; ++c) lists[c] = box_faces(g, c == 4 ? 0 : c);
+    // the four velocity/scalar lists share one mark array (grid_helper.cpp:42-59: it is
+    // cleared once before fill_dgc1 and accumulates through fill_dgc4); programs are
+    // compiled in increasing kind so the array only ever gains lists
+    std::vector<int> order(n);
+    for(int q = 0; q < n; ++q) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return kind_gcv[a][0] < kind_gcv[b][0]; });
+    const long ncomp = (long)(g.nx + 6) * (g.ny + 6) * (g.nz + 6);
+    std::vector<int> hgc(ncomp, 0);
+    int marked = 0;
+    for(int q : order)
+    {
+        const int kind = kind_gcv[q][0];
+        for(; marked < kind; )
+        {
+            ++marked;
+            for(const Face& f : lists[marked])
+            {
+                if(!f.active) continue;
+                const int* o = SIDE_D[f.side];
+                for(int m = 1; m <= 3; ++m) hgc[g.cid(f.i + o[0] * m, f.j + o[1] * m, f.k + o[2] * m)] += 1;
+            }
+        }
+        std::string e = ghost_program(g, lists, hgc, kind, kind_gcv[q][1], ops[q]);
+        if(!e.empty()) return e;
+    }
+    return "";
+}
+
+std::string build_ghost_program(const HostGrid& g, int kind, int gcv, std::vector<GhostOp>& ops)
+{
+    const int kg[1][2] = {{kind, gcv}};
+    return build_ghost_programs(g, 1, kg, &ops);
 }
 
 }  // namespace reef

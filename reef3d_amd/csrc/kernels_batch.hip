@@ -74,6 +74,14 @@ __device__ __forceinline__ void next_item(ItemIter& it, const TankSet& b)
         while(b.item0[it.p + 1] <= it.q) ++it.p;
 }
 
+// a tank the walk skips: go on to this block's last item of it (the tank's items are
+// contiguous in the set), so the next step leaves the tank after a single state check
+__device__ __forceinline__ void skip_tank(ItemIter& it, const TankSet& b)
+{
+    const int end = b.item0[it.p + 1];
+    if(it.q < end) it.q += ((end - 1 - it.q) / it.step) * it.step;
+}
+
 // the one tile of a block of a per-tile launch
 struct TileIter
 {
@@ -207,14 +215,15 @@ struct Prec<float>
     }
     __device__ static BiVec<float> vec(CTank& T, int, int)
     {
+        const int h = T.st->buf;
         BiVec<float> v;
         v.x = T.s[S_X];
         v.rhs = nullptr;
         v.r = T.s[S_R];
         v.r0 = T.s[S_R0];
-        v.p = T.s[S_P];
+        v.p = T.s[h ? S_P2 : S_P];
         v.ph = T.s[S_PH];
-        v.v = T.s[S_V];
+        v.v = T.s[h ? S_V2 : S_V];
         v.t = T.s[S_T];
         v.aii = T.s[S_AII];
         v.partials = nullptr;
@@ -847,6 +856,81 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
     }
 }
 
+// Row walk of the 2-D (x-z) solver kernels: the item is a run of 64 x 1 x 4 tiles of one
+// x-column (z rows k0 .. k0 + 4 cpt); wave w of the block walks rows
+// [k0 + w cpt, k0 + (w+1) cpt) up z with the operand of rows k-1, k, k+1 in registers;
+// the i+-1 neighbours come from the neighbouring lanes (wavefront shuffles), lanes 0 and
+// 63 load the one outside the wavefront.  No block barrier until the item's reduction.
+// The y couplings of a 2-D tank are multiplied by 0 in the operator, so they are 0 here.
+__device__ __forceinline__ bool row_walk_ok(const DevGrid& g, const TileWalk& tw)
+{
+    return g.jdir == 0 && tw.s.ty == 1 && tw.s.tz == 4 && tw.t / tw.s.nbz == (tw.t1 - 1) / tw.s.nbz;
+}
+
+template<class V>
+__device__ __forceinline__ V shfl_dn1(V x)
+{
+    return __shfl_down(x, 1, 64);
+}
+template<class V>
+__device__ __forceinline__ V shfl_up1(V x)
+{
+    return __shfl_up(x, 1, 64);
+}
+
+template<class V, class Op, class Raw, class Val, class Pre, class Post>
+__device__ __forceinline__ void row_walk(const DevGrid& g, const Op& op, const TileWalk& tw, Raw raw, Val val, Pre pre,
+                                         Post post)
+{
+    using AF = AxisFactor<Op>;
+    const V* __restrict__ fxa = face_x(op);
+    const V* __restrict__ fza = face_z(op);
+    const int lane = threadIdx.x & (TX - 1), wave = threadIdx.x >> 6;
+    const int col = tw.t / tw.s.nbz;
+    const int bz0 = tw.t - col * tw.s.nbz;
+    const int per = tw.t1 - tw.t;                 // rows per wave (4 waves, 4 rows per tile)
+    const int k0 = bz0 * 4 + wave * per;
+    const int k1 = min(k0 + per, g.nz);
+    const int i = col * TX + lane;
+    const bool in = i < g.nx;
+    const int ic = min(i, g.nx - 1);
+    const unsigned c0 = (unsigned)(g.base + ic);   // (ic, 0, 0)
+    const unsigned sxy = (unsigned)g.sxy;
+    const AF ax = axis_factor(g, op, 0, ic);
+    if(k0 >= k1) return;
+    V d, auxc = V(0);
+    unsigned c = c0 + (unsigned)k0 * sxy;
+    V xd = k0 > 0 ? val(raw(c - sxy), d) : V(0);
+    V xc = val(raw(c), auxc);
+    V fzd = ld32(fza, c - sxy);
+    #pragma unroll 1
+    for(int k = k0; k < k1; ++k, c += sxy)
+    {
+        V auxu = V(0);
+        const V xu = k + 1 < g.nz ? val(raw(c + sxy), auxu) : V(0);
+        const auto e = pre(c);
+        const V f0 = ld32(fxa, c), f4 = ld32(fza, c);
+        // the -x face and the x neighbours: from the neighbouring lanes, the lanes at the
+        // wavefront's edges load theirs
+        V f1 = shfl_up1(f0), xs = shfl_up1(xc), xn = shfl_dn1(xc);
+        if(lane == 0)
+        {
+            f1 = ld32(fxa, c - 1u);
+            xs = i > 0 ? val(raw(c - 1u), d) : V(0);
+        }
+        if(lane == TX - 1) xn = i + 1 < g.nx ? val(raw(c + 1u), d) : V(0);
+        if(in)
+        {
+            const AF az = axis_factor(g, op, 2, k);
+            post(c, stencil5(g, coupling_xz(op, f0, f1, f4, fzd, ax, az), xc, xn, xs, xu, xd, i, k), auxc, e);
+        }
+        xd = xc;
+        xc = xu;
+        auxc = auxu;
+        fzd = f4;
+    }
+}
+
 __device__ __forceinline__ bool plane_walk_ok(CTank& T)
 {
     return T.g.jdir == 1 && T.shape.ty == TY && T.shape.tz == 1;
@@ -894,6 +978,7 @@ __global__ void kb_setup(TankSet b, double stop, int maxiter, int mixed, double 
     st->delta = delta;
     st->pending = 0;
     st->nru = 0;
+    st->buf = 0;
     st->ncell = (long)T.g.nx * T.g.ny * T.g.nz;
 }
 
@@ -939,7 +1024,11 @@ __global__ void __launch_bounds__(NT, OCC_A64) kb_bicg_p(TankSet b, int x, int r
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const BiState* st = T.st;
-        if(st->done || st->mode == 2) continue;
+        if(st->done || st->mode == 2)
+        {
+            skip_tank(q, b);
+            continue;
+        }
         const BiVec<V> v = Prec<V>::vec(T, x, rhs);
         const TileWalk tw = item_walk(T, b, q);
         if(st->mode == 1)
@@ -1000,7 +1089,11 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL :
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const BiState* st = T.st;
-        if(st->done || st->mode == 1 || st->mode == 3 || plane_walk_ok(T) != PW) continue;
+        if(st->done || st->mode == 1 || st->mode == 3 || plane_walk_ok(T) != PW)
+        {
+            skip_tank(q, b);
+            continue;
+        }
         const BiVec<V> bv = Prec<V>::vec(T, x, rhs);
         const typename Prec<V>::Op op = Prec<V>::op(T);
         const TileWalk tw = item_walk(T, b, q);
@@ -1033,6 +1126,23 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL :
                     vals[2] += rc * rc;
                 });
         }
+        else if(row_walk_ok(g, tw))
+        {
+            struct EpiIn
+            {
+                V r0, r;
+            };
+            row_walk<V>(
+                g, op, tw, [&](unsigned cc) { return ld32(ph, cc); }, [](V x, V&) { return x; },
+                [&](unsigned c) { return EpiIn{ld32(r0, c), ld32(r, c)}; },
+                [&](unsigned c, V Ax, V, const EpiIn& e) {
+                    st32(vo, c, Ax);
+                    const double vd = Ax, rc = e.r;
+                    vals[0] += vd * double(e.r0);
+                    vals[1] += vd * vd;
+                    vals[2] += rc * rc;
+                });
+        }
         else
             #pragma unroll 1
             for(int t = tw.t; t < tw.t1; ++t)
@@ -1042,6 +1152,116 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL :
                 if(!tile_cell(g, tw, t, i, j, k, c)) continue;
                 const CouplingV<V> cq = coupling_of(g, op, c, i, j, k);
                 epilogue(c, apply_A(g, cq, ph, c, i, j, k));
+            }
+        block_sum_to<3>(vals, b.partials + (size_t)item_slot(T, b, q) * 3);
+    }
+}
+
+// Start of an FP32 iteration fused with v = A ph (mixed precision): the pending
+// x64 += x32 after a reliable update, and, in mode 0, p = r + beta (p - w v), ph = aii p
+// evaluated at every stencil point (the FP32 p update, bicgstab_ijk.cpp:234-235 with
+// precon_solve) so that no separate p pass runs; in mode 2 (p, ph fresh from the start
+// or a restart) ph is read.  The new p and v go to the other half of their double
+// buffers (kb_fin_v swaps the halves), ph is written at the cell for the x phase;
+// (v, r0), |v|^2, |r|^2 as kb_bicg_v.
+template<bool PW>
+__global__ void __launch_bounds__(NT, PW ? OCC_PW : OCC_FULL) kb_mix_v(TankSet b, int x)
+{
+    #pragma unroll 1
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    {
+        CTank& T = tank_at(b, q.p);
+        const DevGrid g = T.g;
+        const BiState* st = T.st;
+        if(st->done || st->mode == 1 || st->mode == 3 || plane_walk_ok(T) != PW)
+        {
+            skip_tank(q, b);
+            continue;
+        }
+        const int upd = st->mode == 0, pend = st->pending, hsel = st->buf;
+        const float beta = (float)st->beta, w = (float)st->w;
+        const SolverOp32 op = Prec<float>::op(T);
+        const float* __restrict__ r = T.s[S_R];
+        const float* __restrict__ r0 = T.s[S_R0];
+        const float* __restrict__ aii = T.s[S_AII];
+        const float* __restrict__ pc = T.s[hsel ? S_P2 : S_P];
+        const float* __restrict__ vc = T.s[hsel ? S_V2 : S_V];
+        float* __restrict__ pn = T.s[hsel ? S_P : S_P2];
+        float* __restrict__ vn = T.s[hsel ? S_V : S_V2];
+        float* __restrict__ ph = T.s[S_PH];
+        float* __restrict__ x32 = T.s[S_X];
+        double* __restrict__ x64 = T.f[x];
+        const TileWalk tw = item_walk(T, b, q);
+        double vals[3] = {0.0, 0.0, 0.0};
+        struct Epi
+        {
+            float r0, r, a, p;
+        };
+        auto pre = [&](unsigned c) { return Epi{ld32(r0, c), ld32(r, c), ld32(aii, c), upd ? 0.0f : ld32(pc, c)}; };
+        // aux: the new p at the centre (mode 0)
+        auto post = [&](unsigned c, float Ax, float pnew, const Epi& e) {
+            st32(vn, c, Ax);
+            if(upd)
+            {
+                st32(pn, c, pnew);
+                st32(ph, c, pnew * e.a);
+            }
+            else
+                st32(pn, c, e.p);
+            if(pend)
+            {
+                st32(x64, c, ld32(x64, c) + (double)ld32(x32, c));
+                st32(x32, c, 0.0f);
+            }
+            const double vd = Ax, rc = e.r;
+            vals[0] += vd * double(e.r0);
+            vals[1] += vd * vd;
+            vals[2] += rc * rc;
+        };
+        struct Rp
+        {
+            float r, p, v, a;
+        };
+        auto raw_u = [&](unsigned cc) { return Rp{ld32(r, cc), ld32(pc, cc), ld32(vc, cc), ld32(aii, cc)}; };
+        auto val_u = [&](const Rp& z, float& pnew) {
+            pnew = z.r + beta * (z.p - w * z.v);
+            return pnew * z.a;
+        };
+        auto raw_f = [&](unsigned cc) { return ld32(ph, cc); };
+        auto val_f = [](float z, float&) { return z; };
+        if constexpr(PW)
+        {
+            if(upd)
+                plane_walk<float>(g, op, tw, raw_u, val_u, pre, post);
+            else
+                plane_walk<float>(g, op, tw, raw_f, val_f, pre, post);
+        }
+        else if(row_walk_ok(g, tw))
+        {
+            if(upd)
+                row_walk<float>(g, op, tw, raw_u, val_u, pre, post);
+            else
+                row_walk<float>(g, op, tw, raw_f, val_f, pre, post);
+        }
+        else
+            #pragma unroll 1
+            for(int t = tw.t; t < tw.t1; ++t)
+            {
+                int i, j, k;
+                long c;
+                if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+                const CouplingV<float> cq = coupling_of(g, op, c, i, j, k);
+                float pnew = 0.0f, d;
+                float Ax;
+                if(upd)
+                {
+                    const float phc = val_u(raw_u((unsigned)c), pnew);
+                    Ax = apply_A_fn(g, cq, [&](long cc) { return cc == c ? phc : val_u(raw_u((unsigned)cc), d); }, c,
+                                    i, j, k);
+                }
+                else
+                    Ax = apply_A(g, cq, (const float*)ph, c, i, j, k);
+                post((unsigned)c, Ax, pnew, pre((unsigned)c));
             }
         block_sum_to<3>(vals, b.partials + (size_t)item_slot(T, b, q) * 3);
     }
@@ -1065,6 +1285,12 @@ __global__ void kb_fin_v(TankSet b)
         st->norm_vj = norm_vj;
         st->norm_rj = norm_rj;
         st->alpha = alpha;
+        if(st->mixed)
+        {
+            // the fused v phase wrote p and v into the other half and added x32
+            st->buf ^= 1;
+            st->pending = 0;
+        }
         if(fabs(sigma) <= (1.0e-12 * (norm_vj * st->norm_r0)))
             st->mode = 1;
         else if((fabs(alpha) * norm_vj / (norm_rj == 0 ? 1.0e-15 : norm_rj)) <= 0.08)
@@ -1085,7 +1311,11 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL :
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const BiState* st = T.st;
-        if(st->done || st->mode == 1 || st->mode == 3 || plane_walk_ok(T) != PW) continue;
+        if(st->done || st->mode == 1 || st->mode == 3 || plane_walk_ok(T) != PW)
+        {
+            skip_tank(q, b);
+            continue;
+        }
         const BiVec<V> bv = Prec<V>::vec(T, x, rhs);
         const typename Prec<V>::Op op = Prec<V>::op(T);
         const TileWalk tw = item_walk(T, b, q);
@@ -1122,6 +1352,27 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL :
                 [&](unsigned c, V Ax, V s, const NoEpi&) {
                     st32(to, c, Ax);
                     const double td = Ax, sd = s;
+                    vals[0] += td * sd;
+                    vals[1] += td * td;
+                    vals[2] += sd * sd;
+                });
+        }
+        else if(row_walk_ok(g, tw))
+        {
+            struct Rin
+            {
+                V r, v, a;
+            };
+            row_walk<V>(
+                g, op, tw, [&](unsigned cc) { return Rin{ld32(r, cc), ld32(v, cc), ld32(a, cc)}; },
+                [&](const Rin& q, V& sv) {
+                    sv = q.r - alpha * q.v;
+                    return sv * q.a;
+                },
+                [](unsigned) { return NoEpi{}; },
+                [&](unsigned c, V Ax, V sc, const NoEpi&) {
+                    st32(to, c, Ax);
+                    const double td = Ax, sd = sc;
                     vals[0] += td * sd;
                     vals[1] += td * td;
                     vals[2] += sd * sd;
@@ -1172,7 +1423,11 @@ __global__ void __launch_bounds__(NT, OCC_FULL) kb_bicg_x(TankSet b, int x, int 
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const BiState* st = T.st;
-        if(st->done || st->mode == 1 || st->mode == 3) continue;
+        if(st->done || st->mode == 1 || st->mode == 3)
+        {
+            skip_tank(q, b);
+            continue;
+        }
         const BiVec<V> bv = Prec<V>::vec(T, x, rhs);
         const TileWalk tw = item_walk(T, b, q);
         const V alpha = (V)st->alpha, w = (V)st->w;
@@ -1342,59 +1597,6 @@ __global__ void kb_fin_mix_start(TankSet b)
     }
 }
 
-// start of an FP32 iteration: x64 += x32 after a reliable update, then the p update
-__global__ void __launch_bounds__(NT, OCC_FULL) kb_mix_p(TankSet b, int x)
-{
-    #pragma unroll 1
-    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
-    {
-        CTank& T = tank_at(b, q.p);
-        const DevGrid g = T.g;
-        const BiState* st = T.st;
-        if(st->done || st->mode == 1 || st->mode == 3) continue;
-        const int upd = st->mode == 0, pend = st->pending;
-        if(!upd && !pend) continue;
-        const BiVec<float> v = Prec<float>::vec(T, x, x);
-        double* __restrict__ x64 = T.f[x];
-        const float beta = (float)st->beta, w = (float)st->w;
-        const TileWalk tw = item_walk(T, b, q);
-        float* __restrict__ x32 = v.x;
-        const float* __restrict__ r = v.r;
-        const float* __restrict__ vv = v.v;
-        const float* __restrict__ aii = v.aii;
-        float* __restrict__ p = v.p;
-        float* __restrict__ ph = v.ph;
-        #pragma unroll 4
-        for(int t = tw.t; t < tw.t1; ++t)
-        {
-            int i, j, k;
-            long c;
-            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
-            if(pend)
-            {
-                x64[c] += (double)x32[c];
-                x32[c] = 0.0f;
-            }
-            if(upd)
-            {
-                const float pn = r[c] + beta * (p[c] - w * vv[c]);
-                p[c] = pn;
-                ph[c] = pn * aii[c];
-            }
-        }
-    }
-}
-
-__global__ void kb_fin_mix_p(TankSet b)
-{
-    const int tk = blockIdx.x * blockDim.x + threadIdx.x;
-    if(tk >= b.n) return;
-    CTank& T = tank_at(b, tk);
-    BiState* st = T.st;
-    if(st->done || st->mode == 1 || st->mode == 3) return;
-    st->pending = 0;
-}
-
 // reliable update (mode 3) or restart (mode 1): r = b - A64 (x64 + x32) in FP64, rounded to
 // the FP32 residual; a restart also sets r0 = p = r, ph = aii r
 __global__ void __launch_bounds__(NT, OCC_A64) kb_mix_ru(TankSet b, int x, int rhs)
@@ -1404,7 +1606,11 @@ __global__ void __launch_bounds__(NT, OCC_A64) kb_mix_ru(TankSet b, int x, int r
     {
         CTank& T = tank_at(b, q.p);
         const BiState* st = T.st;
-        if(st->done || (st->mode != 1 && st->mode != 3)) continue;
+        if(st->done || (st->mode != 1 && st->mode != 3))
+        {
+            skip_tank(q, b);
+            continue;
+        }
         const int restart = st->mode == 1;
         const DevGrid g = T.g;
         const SolverOp64 op = Prec<double>::op(T);
@@ -1741,26 +1947,24 @@ void bl_mixed_phase(const TankSet& b, int x, int rhs, int phase, int part, hipSt
     {
         switch(phase)
         {
-            case 0: hipLaunchKernelGGL(kb_mix_p, grid, dim3(NT), 0, s, b, x); break;
-            case 1:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<float, true>), gridpw, dim3(NT), 0, s, b, x, rhs);
-                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_v<float, false>), grid, dim3(NT), 0, s, b, x, rhs);
+            case 0:
+                if(b.has_pw) hipLaunchKernelGGL((kb_mix_v<true>), gridpw, dim3(NT), 0, s, b, x);
+                if(b.has_direct) hipLaunchKernelGGL((kb_mix_v<false>), grid, dim3(NT), 0, s, b, x);
                 break;
-            case 2:
+            case 1:
                 if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<float, true>), gridpw, dim3(NT), 0, s, b, x, rhs);
                 if(b.has_direct) hipLaunchKernelGGL((kb_bicg_t<float, false>), grid, dim3(NT), 0, s, b, x, rhs);
                 break;
-            case 3: hipLaunchKernelGGL(kb_bicg_x<float>, grid, dim3(NT), 0, s, b, x, rhs); break;
+            case 2: hipLaunchKernelGGL(kb_bicg_x<float>, grid, dim3(NT), 0, s, b, x, rhs); break;
             default: hipLaunchKernelGGL(kb_mix_ru, pgrid(b, OCC_A64), dim3(NT), 0, s, b, x, rhs); break;
         }
         return;
     }
     switch(phase)
     {
-        case 0: hipLaunchKernelGGL(kb_fin_mix_p, setup_grid(b), dim3(64), 0, s, b); break;
-        case 1: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b); break;
-        case 2: hipLaunchKernelGGL(kb_fin_t, tankgrid(b), dim3(NT), 0, s, b); break;
-        case 3: hipLaunchKernelGGL(kb_fin_x, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 0: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 1: hipLaunchKernelGGL(kb_fin_t, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 2: hipLaunchKernelGGL(kb_fin_x, tankgrid(b), dim3(NT), 0, s, b); break;
         default: hipLaunchKernelGGL(kb_fin_mix_ru, tankgrid(b), dim3(NT), 0, s, b); break;
     }
 }

@@ -62,6 +62,9 @@ struct HostGrid
 std::string build_grid(HostGrid& g, const reef_grid_desc& d, double F45);
 // compiled ghost program of ghostcell::start<kind>(f, gcv)
 std::string build_ghost_program(const HostGrid& g, int kind, int gcv, std::vector<GhostOp>& ops);
+// the programs of n (kind, gcv) pairs of one grid, ops[q] for pair q (shares the boundary
+// lists and mark array between them)
+std::string build_ghost_programs(const HostGrid& g, int n, const int (*kind_gcv)[2], std::vector<GhostOp>* ops);
 
 // ------------------------------------------------------------------ BiCGStab state
 struct BiState
@@ -78,6 +81,7 @@ struct BiState
     double r_j_prev, rmax, delta;
     double res0;                       // ||b - A x0||_2 (FP64) of a mixed solve
     int pending, nru;
+    int buf;                           // mixed precision: current half of the FP32 p / v buffers
     long ncell;
 };
 

@@ -1,7 +1,9 @@
 // Serving loop over a batch engine (engine.h).  Requests (one tank: grid, initial fields,
-// time step and step count) are submitted from any host thread; one worker thread owns
-// the engine and its stream.  Each round the worker
-//   1. admits waiting requests in arrival order while their device bytes fit in the HBM
+// time step and step count) are submitted from any host thread.  Builder threads make the
+// host half of each tank (grid metadata and compiled ghost programs, Engine::build_tank)
+// as requests arrive, while the GPU keeps working; one worker thread owns the engine and
+// its stream.  Each round the worker
+//   1. admits built requests in arrival order while their device bytes fit in the HBM
 //      budget and the live count allows (the first that does not fit stops admission, so
 //      a large request is not overtaken for ever), prepares the admitted tanks alone
 //      (ghost fills and fluid update of those tanks) and schedules their steps,
@@ -58,6 +60,8 @@ struct Request
     int steps = 0;
     size_t bytes = 0;
     std::vector<double> init[5];      // phi, u, v, w, press (i fastest, 3 ghost layers)
+    std::unique_ptr<TankBuild> built; // host half of the tank, once a builder made it
+    bool ready = false;               // built, or failed to build (code != 0)
     std::vector<double> out[NOUT];    // every field id up to REEF_VISC after the last step
     int state = RQ_QUEUED;
     int tank = -1;
@@ -79,8 +83,9 @@ struct reef_server
     reef_server_config cfg{};
     std::vector<int> plans;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<RequestPtr> queue;
+    std::condition_variable cv_work, cv_done, cv_build;
+    std::deque<RequestPtr> queue;     // admission order
+    std::deque<RequestPtr> to_build;  // not yet taken by a builder
     std::unordered_map<long, RequestPtr> all;
     std::vector<RequestPtr> live;     // worker only
     long next_id = 1;
@@ -90,11 +95,13 @@ struct reef_server
     long steps = 0;
     size_t bytes_in_use = 0;
     std::thread worker;
+    std::vector<std::thread> builders;
     int device = 0;
     std::chrono::steady_clock::time_point t0;
 
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
     void run();
+    void build_loop();
     int round(std::vector<RequestPtr>& admitted);
 };
 
@@ -113,10 +120,8 @@ int reef_server::round(std::vector<RequestPtr>& admitted)
     for(size_t q = 0; q < admitted.size(); ++q)
     {
         RequestPtr& r = admitted[q];
-        reef_params p = cfg.shared;
-        p.dt = r->dt;
         int tank = -1;
-        rc = E.add_tank(r->gd, p, &tank);
+        rc = E.add_built(*r->built, &tank);
         if(rc == REEF_E_NOMEM && !(live.empty() && new_ids.empty()))
         {
             // the pool is fuller than the estimate said: back to the head of the queue
@@ -136,6 +141,7 @@ int reef_server::round(std::vector<RequestPtr>& admitted)
             continue;
         }
         r->tank = tank;
+        r->built.reset();
         for(int f = 0; f < 5; ++f)
         {
             if((rc = E.put(tank, INIT_FIELDS[f], r->init[f].data(), REEF_LAYOUT_IFAST))) return rc;
@@ -205,6 +211,39 @@ int reef_server::round(std::vector<RequestPtr>& admitted)
     return 0;
 }
 
+// builder thread: the host half of each submitted tank, in submission order
+void reef_server::build_loop()
+{
+    for(;;)
+    {
+        RequestPtr r;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_build.wait(lk, [&] { return stop || !to_build.empty(); });
+            if(stop) return;
+            r = to_build.front();
+            to_build.pop_front();
+        }
+        reef_params p = cfg.shared;
+        p.dt = r->dt;
+        std::unique_ptr<TankBuild> b(new TankBuild());
+        std::string err;
+        const int rc = Engine::build_tank(r->gd, p, *b, err);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if(rc)
+            {
+                r->code = rc;
+                r->err = err;
+            }
+            else
+                r->built = std::move(b);
+            r->ready = true;
+        }
+        cv_work.notify_all();
+    }
+}
+
 void reef_server::run()
 {
     (void)hipSetDevice(device);   // the device of the creating thread
@@ -213,13 +252,21 @@ void reef_server::run()
     {
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv_work.wait(lk, [&] { return stop || !queue.empty() || !live.empty(); });
+            cv_work.wait(lk, [&] { return stop || (!queue.empty() && queue.front()->ready) || !live.empty(); });
             if(stop) break;
-            // admission in arrival order by the HBM budget and the live limit
+            // admission in arrival order by the HBM budget and the live limit; a request
+            // still being built holds back the ones behind it
             size_t in_use = e->pool.in_use();
-            while(!queue.empty())
+            while(!queue.empty() && queue.front()->ready)
             {
                 RequestPtr& r = queue.front();
+                if(r->code)
+                {
+                    r->state = RQ_FAILED;
+                    queue.pop_front();
+                    cv_done.notify_all();
+                    continue;
+                }
                 if(cfg.max_live > 0 && (int)(live.size() + admitted.size()) >= cfg.max_live) break;
                 if(cfg.hbm_budget_bytes > 0.0 && (double)(in_use + r->bytes) > cfg.hbm_budget_bytes) break;
                 in_use += r->bytes;
@@ -246,6 +293,7 @@ void reef_server::run()
             kv.second->err = fatal ? fatal_msg : "server stopped";
         }
     queue.clear();
+    to_build.clear();
     cv_done.notify_all();
 }
 
@@ -296,6 +344,8 @@ int reef_server_create(const reef_server_config* cfg, reef_server** out)
     s->t0 = std::chrono::steady_clock::now();
     reef_server* raw = s.release();
     raw->worker = std::thread([raw] { raw->run(); });
+    const int nb = (int)std::max(1u, std::min(4u, std::thread::hardware_concurrency()));
+    for(int q = 0; q < nb; ++q) raw->builders.emplace_back([raw] { raw->build_loop(); });
     *out = raw;
     return 0;
 }
@@ -308,7 +358,10 @@ void reef_server_destroy(reef_server* s)
         s->stop = true;
     }
     s->cv_work.notify_all();
+    s->cv_build.notify_all();
     if(s->worker.joinable()) s->worker.join();
+    for(std::thread& b : s->builders)
+        if(b.joinable()) b.join();
     s->e.reset();
     if(s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -345,9 +398,10 @@ int reef_server_submit(reef_server* s, const reef_grid_desc* grid, double dt, in
         r->t_submit = s->now();
         s->all[r->id] = r;
         s->queue.push_back(r);
+        s->to_build.push_back(r);
         *id = r->id;
     }
-    s->cv_work.notify_all();
+    s->cv_build.notify_one();
     return 0;
 }
 
