@@ -42,7 +42,8 @@ drop-in, and their latencies under the same arrivals follow from FIFO order.
 
 --workload solves: the standalone-solve comparison on the same operator instead: --solves
 right-hand sides ~ N(0, 1) per cell (seeds 2000, 2001, ..., mean removed), each solved
-in FP64 and in mixed precision from x = 0 to ||b - Ax|| <= 1e-10 ||b||; reports
+in FP64 and in mixed precision from x = 0 to ||b - Ax|| <= 1e-10 ||b||, both with their
+8-iteration batches replayed from hipGraphs (--no-graphs: launch by launch); reports
 iterations, times, the FP64 residual of both results and max |p - p_FP64|.
 """
 import argparse
@@ -438,9 +439,26 @@ def serve(args):
     from reef3d_amd import Batch, Server
     reqs = c4_requests(args.requests, args.step_div)
     fields = [request_fields(r) for r in reqs]
-    # tile plans of the served tanks: 2-D chunks of args.cpt tiles per size bucket
-    plans = sorted({(0, int(math.floor(math.log2(r["nx"] * r["nz"]))), 1, 4, args.cpt)
-                    for r in reqs if r["jdir"] == 0}) if args.cpt else []
+    # tile plans of the served tanks: 2-D chunks of --cpt tiles (default 4, the fastest in a
+    # sweep of 2, 4 and 8 on this stream) per size bucket, or with --tune the autotuner's
+    # plans for the size buckets of the stream (tuned once before serving, each bucket alone)
+    t_tune = 0.0
+    if not args.tune:
+        plans = sorted({(0, int(math.floor(math.log2(r["nx"] * r["nz"]))), 1, 4, args.cpt or 4)
+                        for r in reqs if r["jdir"] == 0})
+    else:
+        t0 = time.perf_counter()
+        T = Batch()
+        for r, f in zip(reqs, fields):
+            t = T.add(nx=r["nx"], ny=r["ny"], nz=r["nz"], XN=f["XN"], YN=f["YN"], ZN=f["ZN"], DYM=r["dx"], DZM=r["dx"],
+                      jdir=r["jdir"], top_group=3, g=(0.0, 0.0, -9.81), dt=f["dt"], N44=-1e-10, N46=20000)
+            for n in ("phi", "u", "v", "w", "press"):
+                T.put(t, n, f[n])
+        T.solver_config(args.serve_precision)
+        T.autotune()
+        plans = T.plans()
+        T.close()
+        t_tune = time.perf_counter() - t0
     S = Server(hbm_budget_bytes=200e9, precision=args.serve_precision, graphs=False, plans=plans)
     ids = []
     t_start = time.perf_counter()
@@ -464,7 +482,7 @@ def serve(args):
     makespan = max(i["t_done"] for i in info) - min(i["t_submit"] for i in info)
     # the plain drop-in: one request at a time, FP64 projection, launch by launch
     service = []
-    for q, (r, f) in enumerate(zip(reqs, fields)):
+    for q, (r, f) in enumerate([] if args.serve_only else zip(reqs, fields)):
         B = Batch()
         t = B.add(nx=r["nx"], ny=r["ny"], nz=r["nz"], XN=f["XN"], YN=f["YN"], ZN=f["ZN"], DYM=r["dx"], DZM=r["dx"],
                   jdir=r["jdir"], top_group=3, g=(0.0, 0.0, -9.81), dt=f["dt"], N44=-1e-10, N46=20000)
@@ -490,8 +508,8 @@ def serve(args):
     for r, sv in zip(reqs, service):
         clock = max(clock, r["arrival"]) + sv
         seq_done.append(clock)
-    seq_lat = np.array([d - r["arrival"] for d, r in zip(seq_done, reqs)])
-    seq_span = seq_done[-1] - reqs[0]["arrival"]
+    seq_lat = np.array([d - r["arrival"] for d, r in zip(seq_done, reqs)]) if service else np.zeros(1)
+    seq_span = seq_done[-1] - reqs[0]["arrival"] if service else float("nan")
     n2d = sum(r["jdir"] == 0 for r in reqs)
     out = {
         "metric": "served tank solves/s",
@@ -505,17 +523,18 @@ def serve(args):
         "config": {"workload": f"served_tank_stream: {len(reqs)} requests, Poisson arrivals 200/s (seed 1004), "
                                f"{n2d} 2-D / {len(reqs) - n2d} 3-D, steps 50..500 / {args.step_div}, HBM budget 200 GB, "
                                f"{args.serve_precision} projection to ||r|| <= 1e-10 ||b||",
-                   "plans": [list(p_) for p_ in plans]},
+                   "plans": [list(p_) for p_ in plans], "autotune_s": round(t_tune, 2)},
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                       "max": round(float(lat.max()), 3)},
         "makespan_s": round(makespan, 3),
         "server_rounds": rounds,
         "tank_steps": int(sum(r["steps"] for r in reqs)),
-        "sequential_fp64": {"solves_per_s": round(len(reqs) / seq_span, 4),
-                            "latency_s": {"p50": round(float(np.percentile(seq_lat, 50)), 3),
-                                          "p99": round(float(np.percentile(seq_lat, 99)), 3)},
-                            "service_s_total": round(float(sum(service)), 3)},
-        "speedup_vs_sequential": round((len(reqs) / makespan) / (len(reqs) / seq_span), 3),
+        "sequential_fp64": None if not service else {
+            "solves_per_s": round(len(reqs) / seq_span, 4),
+            "latency_s": {"p50": round(float(np.percentile(seq_lat, 50)), 3),
+                          "p99": round(float(np.percentile(seq_lat, 99)), 3)},
+            "service_s_total": round(float(sum(service)), 3)},
+        "speedup_vs_sequential": round((len(reqs) / makespan) / (len(reqs) / seq_span), 3) if service else None,
     }
     print(json.dumps(out), flush=True)
 
@@ -524,6 +543,8 @@ def solves(args, cfg):
     """Standalone FP64 vs mixed-precision solves on the wave tank's step-0 operator."""
     D, _ = make_domain(cfg)
     D.solver_config("fp64", args.delta)   # the reliable-update threshold of the mixed solves
+    # both solvers replay their 8-iteration batches as hipGraphs (the same for both)
+    D.set_control("graphs", 0 if args.no_graphs else 1)
     nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
     cells = nx * ny * nz
     tol = cfg["rtol"]
@@ -568,7 +589,8 @@ def solves(args, cfg):
         "dtype": "f32 inner / f64 defect",
         "data": "synthetic",
         "config": {"workload": f"poisson_solves_{nx}x{ny}x{nz}: wave-tank step-0 operator, b ~ N(0,1) "
-                               f"seeds 2000.., x0 = 0, ||b - Ax|| <= {tol:g} ||b||"},
+                               f"seeds 2000.., x0 = 0, ||b - Ax|| <= {tol:g} ||b||",
+                   "graphs": not args.no_graphs},
         "fp64": {"solves_per_s": round(len(rec) / t64, 3), "ms_mean": round(t64 / len(rec) * 1e3, 2),
                  "iters_mean": round(float(a[:, 0].mean()), 1),
                  "final_rel_residual_max": float(a[:, 5].max())},
@@ -595,8 +617,10 @@ def main():
     ap.add_argument("--step-div", type=int, default=10, help="--workload serve: steps per request divided by this")
     ap.add_argument("--serve-precision", choices=("fp64", "mixed"), default="mixed", help="--workload serve: projection")
     ap.add_argument("--sequential", type=int, default=8, help="tanks run one at a time in --workload tanks")
-    ap.add_argument("--no-graphs", action="store_true", help="--workload tanks: launch by launch")
+    ap.add_argument("--no-graphs", action="store_true", help="--workload tanks / solves: launch by launch")
     ap.add_argument("--no-tune", action="store_true", help="--workload tanks: default tile plans")
+    ap.add_argument("--tune", action="store_true", help="--workload serve: autotuned tile plans")
+    ap.add_argument("--serve-only", action="store_true", help="--workload serve: skip the sequential FP64 leg")
     ap.add_argument("--cpt", type=int, default=0, help="--workload tanks / serve: this chunk length for every 2-D bucket")
     ap.add_argument("--check", type=int, default=3, help="oracle parity steps of --workload vortex")
     ap.add_argument("--solves", type=int, default=100, help="right-hand sides of --workload solves")

@@ -473,28 +473,36 @@ std::vector<TankSet> Engine::make_sets(const std::vector<int>& slots) const
 }
 
 // ------------------------------------------------------------------ fields
+// the ghosted box of a field (X x Y x Z with X = nx + 6, ...) is one span of the device
+// layout: rows sx apart, planes sx * Y apart, so a field moves in one contiguous copy and
+// the host gathers or scatters the rows (a pitched copy of Y * Z short rows is slow)
+static size_t box_span(const HostGrid& h)
+{
+    return (size_t)((long)(h.ny + 6) * (h.nz + 6) - 1) * h.sx + (size_t)(h.nx + 6);
+}
+
 int Engine::put(int id, int field, const double* host, int layout)
 {
     TankHost* t = tank(id);
     if(!t || field < 0 || field >= NARR64) return fail(REEF_E_ARG, "bad tank or field id");
+    if(layout != REEF_LAYOUT_IFAST && layout != REEF_LAYOUT_REF) return fail(REEF_E_ARG, "bad layout");
     const HostGrid& h = t->hg;
-    const long X = h.nx + 6, Y = h.ny + 6, Z = h.nz + 6;
-    const double* src = host;
+    const long X = h.nx + 6, Y = h.ny + 6, Z = h.nz + 6, sx = h.sx;
+    const size_t span = box_span(h);
+    t->staging.assign(span, 0.0);   // the row padding beyond i = nx + 2 stays zero
+    double* st = t->staging.data();
     if(layout == REEF_LAYOUT_REF)
     {
-        t->staging.resize((size_t)X * Y * Z);
         for(long i = 0; i < X; ++i)
         for(long j = 0; j < Y; ++j)
         for(long k = 0; k < Z; ++k)
-            t->staging[(size_t)(i + X * (j + Y * k))] = host[(i * Y + j) * Z + k];
-        src = t->staging.data();
+            st[(size_t)(i + sx * (j + Y * k))] = host[(i * Y + j) * Z + k];
     }
-    else if(layout != REEF_LAYOUT_IFAST)
-        return fail(REEF_E_ARG, "bad layout");
+    else
+        for(long r = 0; r < Y * Z; ++r) std::memcpy(st + (size_t)r * sx, host + (size_t)r * X, X * sizeof(double));
     double* dst0 = t->dev.f[field] + (h.base - 3 - 3 * h.sx - 3 * h.sxy);
     EHIP(hipStreamSynchronize(stream));
-    EHIP(hipMemcpy2D(dst0, h.sx * sizeof(double), src, X * sizeof(double), X * sizeof(double), Y * Z,
-                     hipMemcpyHostToDevice));
+    EHIP(hipMemcpy(dst0, st, span * sizeof(double), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -502,25 +510,24 @@ int Engine::get(int id, int field, double* host, int layout)
 {
     TankHost* t = tank(id);
     if(!t || field < 0 || field >= NARR64) return fail(REEF_E_ARG, "bad tank or field id");
+    if(layout != REEF_LAYOUT_IFAST && layout != REEF_LAYOUT_REF) return fail(REEF_E_ARG, "bad layout");
     const HostGrid& h = t->hg;
-    const long X = h.nx + 6, Y = h.ny + 6, Z = h.nz + 6;
+    const long X = h.nx + 6, Y = h.ny + 6, Z = h.nz + 6, sx = h.sx;
+    const size_t span = box_span(h);
     const double* src0 = t->dev.f[field] + (h.base - 3 - 3 * h.sx - 3 * h.sxy);
-    double* dst = host;
+    t->staging.resize(span);
+    double* st = t->staging.data();
+    EHIP(hipStreamSynchronize(stream));
+    EHIP(hipMemcpy(st, src0, span * sizeof(double), hipMemcpyDeviceToHost));
     if(layout == REEF_LAYOUT_REF)
     {
-        t->staging.resize((size_t)X * Y * Z);
-        dst = t->staging.data();
-    }
-    else if(layout != REEF_LAYOUT_IFAST)
-        return fail(REEF_E_ARG, "bad layout");
-    EHIP(hipStreamSynchronize(stream));
-    EHIP(hipMemcpy2D(dst, X * sizeof(double), src0, h.sx * sizeof(double), X * sizeof(double), Y * Z,
-                     hipMemcpyDeviceToHost));
-    if(layout == REEF_LAYOUT_REF)
         for(long i = 0; i < X; ++i)
         for(long j = 0; j < Y; ++j)
         for(long k = 0; k < Z; ++k)
-            host[(i * Y + j) * Z + k] = t->staging[(size_t)(i + X * (j + Y * k))];
+            host[(i * Y + j) * Z + k] = st[(size_t)(i + sx * (j + Y * k))];
+    }
+    else
+        for(long r = 0; r < Y * Z; ++r) std::memcpy(host + (size_t)r * X, st + (size_t)r * sx, X * sizeof(double));
     return 0;
 }
 
