@@ -55,6 +55,7 @@ struct TankSet
     int n;                         // tanks in the set
     int ntiles, nitems;            // tiles and work items of the set
     int has_pw, has_direct;        // tanks walked by the 3-D plane walk / by the direct stencil
+    int pw_items;                  // work items of the plane-walked tanks, which come first
     int slot[SET_MAX];             // tank slots
     int tile0[SET_MAX + 1];        // first tile of each tank in the set's tile range
     int item0[SET_MAX + 1];        // first work item of each tank in the set's item range

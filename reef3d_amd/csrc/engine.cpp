@@ -437,8 +437,16 @@ int Engine::sync_tables()
 }
 
 // the launch sets of the given slots, at most SET_MAX tanks each
-std::vector<TankSet> Engine::make_sets(const std::vector<int>& slots) const
+std::vector<TankSet> Engine::make_sets(const std::vector<int>& slots_in) const
 {
+    // plane-walked tanks first: their items are a prefix of each set's items, so the
+    // launches that take one kind of tank spread that kind's items over every XCD group
+    auto pw = [&](int q) {
+        const TankHost& t = tanks[q];
+        return t.hg.jdir == 1 && t.dev.shape.ty == 4 && t.dev.shape.tz == 1;
+    };
+    std::vector<int> slots(slots_in);
+    std::stable_partition(slots.begin(), slots.end(), pw);
     std::vector<TankSet> out;
     for(size_t a = 0; a < slots.size(); a += SET_MAX)
     {
@@ -457,8 +465,11 @@ std::vector<TankSet> Engine::make_sets(const std::vector<int>& slots) const
             S.item0[p] = ni;
             nt += t.dev.shape.ntiles;
             ni += t.dev.shape.nitems;
-            if(t.hg.jdir == 1 && t.dev.shape.ty == 4 && t.dev.shape.tz == 1)
+            if(pw(slots[q]))
+            {
                 S.has_pw = 1;
+                S.pw_items = ni;
+            }
             else
                 S.has_direct = 1;
         }

@@ -55,16 +55,28 @@ struct ItemIter
     int q, hi, step, p;
 };
 
-__device__ __forceinline__ ItemIter set_items(const TankSet& b)
+// items [first, first + count) of the set, split into 8 XCD groups
+__device__ __forceinline__ ItemIter set_items(const TankSet& b, int first, int count)
 {
     int lo, hi;
-    group_range(b.nitems, (int)(blockIdx.x & 7), lo, hi);
+    group_range(count, (int)(blockIdx.x & 7), lo, hi);
+    lo += first;
+    hi += first;
     ItemIter it;
     it.q = lo + (int)(blockIdx.x >> 3);
     it.hi = hi;
     it.step = (int)(gridDim.x >> 3);
     it.p = it.q < hi ? set_find(b.item0, b.n, it.q) : 0;
     return it;
+}
+
+__device__ __forceinline__ ItemIter set_items(const TankSet& b) { return set_items(b, 0, b.nitems); }
+
+// the items of the plane-walked tanks (a prefix of the set) or of the others
+template<bool PW>
+__device__ __forceinline__ ItemIter set_items_of(const TankSet& b)
+{
+    return PW ? set_items(b, 0, b.pw_items) : set_items(b, b.pw_items, b.nitems - b.pw_items);
 }
 
 __device__ __forceinline__ void next_item(ItemIter& it, const TankSet& b)
@@ -1084,7 +1096,7 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL :
                                                                                                int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    for(ItemIter q = set_items_of<PW>(b); q.q < q.hi; next_item(q, b))
     {
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
@@ -1168,7 +1180,7 @@ template<bool PW>
 __global__ void __launch_bounds__(NT, PW ? OCC_PW : OCC_FULL) kb_mix_v(TankSet b, int x)
 {
     #pragma unroll 1
-    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    for(ItemIter q = set_items_of<PW>(b); q.q < q.hi; next_item(q, b))
     {
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
@@ -1306,7 +1318,7 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL :
                                                                                                int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    for(ItemIter q = set_items_of<PW>(b); q.q < q.hi; next_item(q, b))
     {
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
@@ -1833,10 +1845,14 @@ static inline dim3 setup_grid(const TankSet& b) { return dim3((unsigned)((b.n + 
 static inline dim3 tgrid(const TankSet& b) { return dim3((unsigned)(8 * std::max(1, (b.ntiles + 7) / 8))); }
 // persistent launch over the work items: one block per work item, at most occ blocks per
 // CU (the kernel's occupancy) on the 256 CUs, a multiple of 8
-static inline dim3 pgrid(const TankSet& b, int occ = OCC_FULL)
+static inline dim3 pgrid_n(int nitems, int occ)
 {
-    return dim3((unsigned)std::max(8, std::min(256 * occ, (b.nitems + 7) / 8 * 8)));
+    return dim3((unsigned)std::max(8, std::min(256 * occ, (nitems + 7) / 8 * 8)));
 }
+static inline dim3 pgrid(const TankSet& b, int occ = OCC_FULL) { return pgrid_n(b.nitems, occ); }
+// the launches of one kind of tank (set_items_of): plane-walked or the others
+static inline dim3 pgrid_pw(const TankSet& b, int occ) { return pgrid_n(b.pw_items, occ); }
+static inline dim3 pgrid_direct(const TankSet& b, int occ) { return pgrid_n(b.nitems - b.pw_items, occ); }
 
 void bl_copy(const TankSet& b, int src, int dst, hipStream_t s)
 {
@@ -1901,19 +1917,21 @@ void bl_bicg_init(const TankSet& b, int x, int rhs, double stop, int maxiter, hi
 
 void bl_bicg_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStream_t s)
 {
-    const dim3 grid = pgrid(b), grid64 = pgrid(b, OCC_A64), gridpw = pgrid(b, OCC_PW);
+    const dim3 grid = pgrid(b), grid64 = pgrid(b, OCC_A64);
     if(part == 0)
     {
         switch(phase)
         {
             case 0: hipLaunchKernelGGL(kb_bicg_p<double>, grid64, dim3(NT), 0, s, b, x, rhs); break;
             case 1:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<double, true>), gridpw, dim3(NT), 0, s, b, x, rhs);
-                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_v<double, false>), grid64, dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<double, true>), pgrid_pw(b, OCC_PW), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_direct)
+                    hipLaunchKernelGGL((kb_bicg_v<double, false>), pgrid_direct(b, OCC_A64), dim3(NT), 0, s, b, x, rhs);
                 break;
             case 2:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<double, true>), gridpw, dim3(NT), 0, s, b, x, rhs);
-                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_t<double, false>), grid64, dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<double, true>), pgrid_pw(b, OCC_PW), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_direct)
+                    hipLaunchKernelGGL((kb_bicg_t<double, false>), pgrid_direct(b, OCC_A64), dim3(NT), 0, s, b, x, rhs);
                 break;
             default: hipLaunchKernelGGL(kb_bicg_x<double>, grid, dim3(NT), 0, s, b, x, rhs); break;
         }
@@ -1942,18 +1960,19 @@ void bl_mixed_init(const TankSet& b, int x, int rhs, double stop, int maxiter, d
 
 void bl_mixed_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStream_t s)
 {
-    const dim3 grid = pgrid(b), gridpw = pgrid(b, OCC_PW);
+    const dim3 grid = pgrid(b);
     if(part == 0)
     {
         switch(phase)
         {
             case 0:
-                if(b.has_pw) hipLaunchKernelGGL((kb_mix_v<true>), gridpw, dim3(NT), 0, s, b, x);
-                if(b.has_direct) hipLaunchKernelGGL((kb_mix_v<false>), grid, dim3(NT), 0, s, b, x);
+                if(b.has_pw) hipLaunchKernelGGL((kb_mix_v<true>), pgrid_pw(b, OCC_PW), dim3(NT), 0, s, b, x);
+                if(b.has_direct) hipLaunchKernelGGL((kb_mix_v<false>), pgrid_direct(b, OCC_FULL), dim3(NT), 0, s, b, x);
                 break;
             case 1:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<float, true>), gridpw, dim3(NT), 0, s, b, x, rhs);
-                if(b.has_direct) hipLaunchKernelGGL((kb_bicg_t<float, false>), grid, dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<float, true>), pgrid_pw(b, OCC_PW), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_direct)
+                    hipLaunchKernelGGL((kb_bicg_t<float, false>), pgrid_direct(b, OCC_FULL), dim3(NT), 0, s, b, x, rhs);
                 break;
             case 2: hipLaunchKernelGGL(kb_bicg_x<float>, grid, dim3(NT), 0, s, b, x, rhs); break;
             default: hipLaunchKernelGGL(kb_mix_ru, pgrid(b, OCC_A64), dim3(NT), 0, s, b, x, rhs); break;
