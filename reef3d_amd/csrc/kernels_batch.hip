@@ -72,6 +72,19 @@ __device__ __forceinline__ ItemIter set_items(const TankSet& b, int first, int c
 
 __device__ __forceinline__ ItemIter set_items(const TankSet& b) { return set_items(b, 0, b.nitems); }
 
+// all blocks stride over all items, so item q runs on XCD q % 8 (blocks go to the XCDs
+// in turn) and the items of any one tank spread over every XCD: for kernels without
+// neighbour reuse between items, where only some tanks of the set may have work
+__device__ __forceinline__ ItemIter set_items_spread(const TankSet& b)
+{
+    ItemIter it;
+    it.q = (int)blockIdx.x;
+    it.hi = b.nitems;
+    it.step = (int)gridDim.x;
+    it.p = it.q < it.hi ? set_find(b.item0, b.n, it.q) : 0;
+    return it;
+}
+
 // the items of the plane-walked tanks (a prefix of the set) or of the others
 template<bool PW>
 __device__ __forceinline__ ItemIter set_items_of(const TankSet& b)
@@ -1031,7 +1044,7 @@ template<class V>
 __global__ void __launch_bounds__(NT, OCC_A64) kb_bicg_p(TankSet b, int x, int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    for(ItemIter q = set_items_spread(b); q.q < q.hi; next_item(q, b))
     {
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
@@ -1430,7 +1443,7 @@ template<class V>
 __global__ void __launch_bounds__(NT, OCC_FULL) kb_bicg_x(TankSet b, int x, int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    for(ItemIter q = set_items_spread(b); q.q < q.hi; next_item(q, b))
     {
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
@@ -1614,7 +1627,7 @@ __global__ void kb_fin_mix_start(TankSet b)
 __global__ void __launch_bounds__(NT, OCC_A64) kb_mix_ru(TankSet b, int x, int rhs)
 {
     #pragma unroll 1
-    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    for(ItemIter q = set_items_spread(b); q.q < q.hi; next_item(q, b))
     {
         CTank& T = tank_at(b, q.p);
         const BiState* st = T.st;
