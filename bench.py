@@ -36,7 +36,8 @@ arrivals (lambda = 200/s, seed 1004), 70% 2-D tanks (nx 256..2048 step 128, nz 6
 step 32) and 30% 3-D tanks (nx 128..256, ny 32..64, nz 64..128), dx, depth, amplitude and
 wavelength drawn as in --workload tanks, each asking for 50..500 momentum_FC3 steps
 (divided by --step-div to bound the run); the server (continuous batching, HBM budget
-200 GB, mixed-precision projection to 1e-10) reports p50 / p99 latency (submit to done)
+200 GB, mixed-precision projection to 1e-10, the iteration batches of a recurring set of
+solving tanks replayed from hipGraphs unless --no-graphs) reports p50 / p99 latency (submit to done)
 and solves/s; the same requests are then solved one at a time with the plain FP64
 drop-in, and their latencies under the same arrivals follow from FIFO order.
 
@@ -459,7 +460,7 @@ def serve(args):
         plans = T.plans()
         T.close()
         t_tune = time.perf_counter() - t0
-    S = Server(hbm_budget_bytes=200e9, precision=args.serve_precision, graphs=False, plans=plans)
+    S = Server(hbm_budget_bytes=200e9, precision=args.serve_precision, graphs=not args.no_graphs, plans=plans)
     ids = []
     t_start = time.perf_counter()
     for r, f in zip(reqs, fields):
@@ -523,7 +524,8 @@ def serve(args):
         "config": {"workload": f"served_tank_stream: {len(reqs)} requests, Poisson arrivals 200/s (seed 1004), "
                                f"{n2d} 2-D / {len(reqs) - n2d} 3-D, steps 50..500 / {args.step_div}, HBM budget 200 GB, "
                                f"{args.serve_precision} projection to ||r|| <= 1e-10 ||b||",
-                   "plans": [list(p_) for p_ in plans], "autotune_s": round(t_tune, 2)},
+                   "plans": [list(p_) for p_ in plans], "autotune_s": round(t_tune, 2),
+                   "graphs": not args.no_graphs},
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                       "max": round(float(lat.max()), 3)},
         "makespan_s": round(makespan, 3),
@@ -617,7 +619,7 @@ def main():
     ap.add_argument("--step-div", type=int, default=10, help="--workload serve: steps per request divided by this")
     ap.add_argument("--serve-precision", choices=("fp64", "mixed"), default="mixed", help="--workload serve: projection")
     ap.add_argument("--sequential", type=int, default=8, help="tanks run one at a time in --workload tanks")
-    ap.add_argument("--no-graphs", action="store_true", help="--workload tanks / solves: launch by launch")
+    ap.add_argument("--no-graphs", action="store_true", help="--workload tanks / solves / serve: launch by launch")
     ap.add_argument("--no-tune", action="store_true", help="--workload tanks: default tile plans")
     ap.add_argument("--tune", action="store_true", help="--workload serve: autotuned tile plans")
     ap.add_argument("--serve-only", action="store_true", help="--workload serve: skip the sequential FP64 leg")

@@ -703,10 +703,34 @@ int Engine::prepare_tanks(const std::vector<int>& ids)
 // ------------------------------------------------------------------ graphs
 void Engine::drop_graphs()
 {
-    if(graphs_.empty()) return;
+    if(graphs_.empty() && set_graphs_.empty()) return;
     if(stream) (void)hipStreamSynchronize(stream);
     for(auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
     graphs_.clear();
+    for(auto& g : set_graphs_)
+        if(g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+    set_graphs_.clear();
+}
+
+// captures body() (a launch sequence on the stream) into an executable graph
+template<class Body>
+int Engine::capture(Body body, hipGraphExec_t* out)
+{
+    EHIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(stream, &g);
+    if(rc || ec != hipSuccess)
+    {
+        if(g) (void)hipGraphDestroy(g);
+        return rc ? rc : fail(REEF_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if(ei != hipSuccess) return fail(REEF_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    *out = ex;
+    return 0;
 }
 
 // runs body() (a launch sequence on the stream), or replays it as a graph captured from
@@ -718,19 +742,9 @@ int Engine::replay(const GraphKey& key, Body body)
     auto it = graphs_.find(key);
     if(it == graphs_.end())
     {
-        EHIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        const int rc = body();
-        hipGraph_t g = nullptr;
-        const hipError_t ec = hipStreamEndCapture(stream, &g);
-        if(rc || ec != hipSuccess)
-        {
-            if(g) (void)hipGraphDestroy(g);
-            return rc ? rc : fail(REEF_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-        }
         hipGraphExec_t ex = nullptr;
-        const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if(ei != hipSuccess) return fail(REEF_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        const int rc = capture(body, &ex);
+        if(rc) return rc;
         it = graphs_.emplace(key, ex).first;
     }
     EHIP(hipGraphLaunch(it->second, stream));
@@ -764,9 +778,25 @@ int Engine::enqueue_batch(bool mixed, int x, int rhs, int ring, int iters)
         return 0;
     };
     int rc;
-    // the lockstep sets only change with the tables; the scheduler's change every round
+    // the lockstep sets only change with the tables; the scheduler's solving sets change
+    // from round to round, and a set's batch graph is captured when the set comes back
     if(cur_ == &all_)
         rc = replay({1, mixed ? 1 : 0, x, rhs * 64 + iters}, body);
+    else if(graphs && !timing && cur_ == &solving_sets_)
+    {
+        std::vector<int> key(solving_key_);
+        key.insert(key.end(), {-1, mixed ? 1 : 0, x, rhs, iters});
+        if(set_graphs_.size() > 4096) drop_graphs();
+        SetGraph& sg = set_graphs_[key];
+        if(!sg.exec && ++sg.seen >= 2 && (rc = capture(body, &sg.exec))) return rc;
+        if(sg.exec)
+        {
+            EHIP(hipGraphLaunch(sg.exec, stream));
+            rc = 0;
+        }
+        else
+            rc = body();
+    }
     else
         rc = body();
     if(rc) return rc;

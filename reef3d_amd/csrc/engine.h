@@ -228,6 +228,17 @@ private:
     }
     using GraphKey = std::array<int, 4>;
     std::map<GraphKey, hipGraphExec_t> graphs_;
+    // the scheduler's iteration batches, keyed by the solving slots and the batch's
+    // arguments: captured when a key comes back (the tanks solving together at the end of
+    // a stream repeat their batch many times), launched directly the first time
+    struct SetGraph
+    {
+        int seen = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::map<std::vector<int>, SetGraph> set_graphs_;
+    template<class Body>
+    int capture(Body body, hipGraphExec_t* out);
     template<class Body>
     int replay(const GraphKey& key, Body body);
     int enqueue_batch(bool mixed, int x, int rhs, int ring, int iters);

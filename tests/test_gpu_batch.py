@@ -170,10 +170,11 @@ def test_autotune_plans_and_identity():
         A.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_scheduled_run_identical_to_lockstep(precision):
+@pytest.mark.parametrize("precision,graphs", [("fp64", False), ("mixed", False), ("mixed", True)])
+def test_scheduled_run_identical_to_lockstep(precision, graphs):
     """Iteration-level scheduling (each tank moves through its own steps, no lockstep at the
-    pressure solves) gives the same bits and iteration counts as lockstep steps."""
+    pressure solves) gives the same bits and iteration counts as lockstep steps, also when
+    the batches of recurring solving sets replay from captured graphs."""
     from reef3d_amd import Batch
     lock, it_lock = _run(TANKS, 2, precision)
     B = Batch()
@@ -181,6 +182,7 @@ def test_scheduled_run_identical_to_lockstep(precision):
     for tid, (kw, f) in zip(ids, TANKS):
         _initial(B, tid, kw, f)
     B.solver_config(precision)
+    B.set_control("graphs", 1 if graphs else 0)
     _prepare(B)
     it = B.run(2)
     sched = [{n: B.get(tid, n) for n in FIELDS} for tid in ids]

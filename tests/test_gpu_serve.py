@@ -25,10 +25,12 @@ def _results(S, rid):
     return {n: S.result(rid, n) for n in FIELDS}
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_served_alone_equals_mix(precision):
+@pytest.mark.parametrize("precision,graphs", [("fp64", False), ("mixed", False), ("mixed", True)])
+def test_served_alone_equals_mix(precision, graphs):
+    """graphs: the scheduler's iteration batches of a recurring solving set replay from a
+    captured graph."""
     from reef3d_amd import Server
-    S = Server(precision=precision)
+    S = Server(precision=precision, graphs=graphs)
     ids = [_submit(S, q, STEPS[q]) for q in range(len(TANKS))]
     mix = [_results(S, rid) for rid in ids]
     infos = [S.info(rid) for rid in ids]
