@@ -86,6 +86,10 @@ int reef_ctx_create(const reef_grid_desc* grid, const reef_params* prm, reef_ctx
  * 1 = replay the launch sequences of a momentum_FC3 step and the BiCGStab iteration
  * batches as hipGraphs captured on their first run (re-captured after tanks or controls
  * change), 0 = launch by launch (default); results are identical either way.
+ * "N48": time stepping, 0 = the fixed dt of reef_params (fixtimestep, default), 1 = every
+ * reef_fc3_step first computes its own dt on the device as reef_timestep does (ietimestep
+ * ::start, selected for N48 = 1 with D20 = 0 by driver_logic_cfd.cpp:59-66); "N47": its
+ * factor (ini.cpp:593, default 0.3).
  * Other keys: REEF_E_UNSUPPORTED. */
 int reef_set_control(reef_ctx* ctx, const char* key, double value);
 void reef_ctx_destroy(reef_ctx* ctx);
@@ -152,6 +156,14 @@ int reef_pjm_corr(reef_ctx* ctx, int u_id, int v_id, int w_id, double alpha, int
  * REEF_U/V/W/PRESS/PHI/RO/VISC; iters receives the summed BiCGStab iterations. */
 int reef_fc3_step(reef_ctx* ctx, int* iters, double* vol2);
 
+/* ietimestep::start (ini = 0; ietimestep.cpp:38-207) or ietimestep::ini (ini = 1; :209-304)
+ * with N50 = 1, the adaptive time step of N48 = 1 (driver_logic_cfd.cpp:59-66): from the
+ * largest |u|, |v|, |w| over the staggered loops, dt = N47 * 2 min(dx) / |u|max (::ini: |u|max
+ * at least 2, a quarter of that, at least 1e-6), computed on the device and used by the
+ * following steps.  Inflow, floating-body and EARSM terms are zero on the supported box
+ * domains.  out4 (may be null): dt, umax, vmax, wmax. */
+int reef_timestep(reef_ctx* c, double N47, int ini, double* out4);
+
 /* one level-set benchmark step (levelset_RK3.cpp:103-178 + benchmark_vortex3D.cpp:67-101,
  * F150=3) at time simtime before the increment. */
 int reef_vortex_step(reef_ctx* ctx, double simtime, double* vol2);
@@ -185,7 +197,7 @@ int reef_batch_put(reef_batch* b, int tank, int field_id, const double* host, in
 int reef_batch_get(reef_batch* b, int tank, int field_id, double* host, int layout);
 int reef_batch_ghost_fill(reef_batch* b, int field_id, int kind, int gcv);
 int reef_batch_fluid_update(reef_batch* b);
-/* the keys of reef_set_control ("F49", "graphs") for every tank of the batch */
+/* the keys of reef_set_control ("F49", "graphs", "N48", "N47") for every tank of the batch */
 int reef_batch_control(reef_batch* b, const char* key, double value);
 int reef_batch_solver_config(reef_batch* b, int precision, double delta);
 /* Tile plans per size bucket (no counterpart in the reference).  A bucket is (jdir,
@@ -208,6 +220,10 @@ int reef_batch_plans(reef_batch* b, int* entries, int capacity, int* n);
 /* one momentum_FC3 step of every active tank; iters[slot] (may be NULL) receives the
  * summed BiCGStab iterations, vol[2*slot..] (may be NULL) volume1, volume2 */
 int reef_batch_fc3_step(reef_batch* b, long* iters, double* vol);
+/* reef_timestep for every active tank (each its own dt); out (may be null) receives
+ * dt, umax, vmax, wmax per slot */
+int reef_batch_timestep(reef_batch* b, double N47, int ini, double* out);
+
 /* nsteps momentum_FC3 steps of every active tank with iteration-level scheduling: each
  * tank moves through its own steps, so a tank whose pressure solve has converged goes on
  * while others still iterate (no lockstep at the solves); iters[slot] (may be NULL)

@@ -61,6 +61,7 @@ def lib():
         L.orc_fc3_step.restype = ctypes.c_int
         L.orc_vortex_step.argtypes = [ctypes.c_void_p] + [dp] * 6 + [ctypes.c_double, dp]
         L.orc_phi_rk3.argtypes = [ctypes.c_void_p] + [dp] * 4
+        L.orc_timestep.argtypes = [ctypes.c_void_p] + [dp] * 3 + [ctypes.c_double, ctypes.c_int, dp]
         _LIB = L
     return _LIB
 
@@ -164,3 +165,9 @@ class Domain:
 
     def phi_rk3(self, phi, u, v, w):
         lib().orc_phi_rk3(self.h, _p(phi), _p(u), _p(v), _p(w))
+
+    def timestep(self, u, v, w, N47=0.3, ini=False):
+        """ietimestep::start / ::ini: (dt, umax, vmax, wmax)."""
+        out = np.zeros(4)
+        lib().orc_timestep(self.h, _p(u), _p(v), _p(w), float(N47), 1 if ini else 0, _p(out))
+        return tuple(float(x) for x in out)

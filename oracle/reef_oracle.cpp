@@ -1349,4 +1349,67 @@ void orc_phi_rk3(orc_domain* d, double* phi, const double* u, const double* v, c
     phi_rk3(d, phi, u, v, w);
 }
 
+// ietimestep::start (ietimestep.cpp:38-207) with N50 = 1 and ietimestep::ini
+// (ietimestep.cpp:209-304): the adaptive time step of N48 = 1 (driver_logic_cfd.cpp:59-66).
+// Inflow velocities W10..W16, the floating-body and fixed velocities (ufbmax, X210) are
+// zero on the box domains here; maxF/G/H are zero without the EARSM closure, their only
+// writer (EARSM.cpp:175-192).  out4 = dt, umax, vmax, wmax.
+void orc_timestep(orc_domain* d, const double* u, const double* v, const double* w, double N47, int ini,
+                  double* out4)
+{
+    auto MAXd = [](double a, double b) { return a > b ? a : b; };
+    auto MINd = [](double a, double b) { return a < b ? a : b; };
+    auto MIN3d = [&](double a, double b, double c) { return MINd(MINd(a, b), c); };
+    const double W = 0.0;                 // W11_u .. W16_w, W10, ufbmax, X210_u ...
+    double umax = ini ? -1.0e19 : 0.0, vmax = umax, wmax = umax;
+    if(ini) umax = MAXd(W, umax);         // W10
+    umax = MAXd(W, umax);                 // W11_u .. W16_u
+    vmax = MAXd(W, vmax);
+    wmax = MAXd(W, wmax);
+    for(int i = 0; i < d->nx - d->last[1]; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+        if(d->flag[1][d->id(i, j, k)] > 0) umax = MAXd(umax, fabs(u[d->id(i, j, k)]));
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny - d->last[2]; ++j)
+    for(int k = 0; k < d->nz; ++k)
+        if(d->flag[2][d->id(i, j, k)] > 0) vmax = MAXd(vmax, fabs(v[d->id(i, j, k)]));
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz - d->last[3]; ++k)
+        if(d->flag[3][d->id(i, j, k)] > 0) wmax = MAXd(wmax, fabs(w[d->id(i, j, k)]));
+    if(ini)
+    {
+        umax = MAXd(umax, 2.0 * W);       // 2 ufbmax, 2 vfbmax, 2 wfbmax, 2 X210_u/v/w
+        umax = MAXd(umax, 2.0);
+    }
+    else
+    {
+        umax = MAXd(umax, W);             // ufbmax, vfbmax, wfbmax
+        vmax = MAXd(vmax, W);
+        wmax = MAXd(wmax, W);
+    }
+    const double maxFGH = 0.0;
+    double cu = 1.0e10;
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        if(d->flag[4][d->id(i, j, k)] <= 0) continue;
+        const double dx = MIN3d(d->DN[0][i + MARGE], d->DN[1][j + MARGE], d->DN[2][k + MARGE]);
+        cu = MINd(cu, 2.0 / ((sqrt(umax * umax + vmax * vmax + wmax * wmax)) / dx + sqrt((4.0 * fabs(maxFGH)) / dx)));
+    }
+    cu = MIN3d(cu, 1.0e10, 1.0e10);
+    double dt = N47 * cu;
+    if(ini)
+    {
+        dt = N47 * cu * 0.25;
+        dt = MAXd(dt, 1.0e-6);
+    }
+    out4[0] = dt;
+    out4[1] = umax;
+    out4[2] = vmax;
+    out4[3] = wmax;
+}
+
 }  // extern "C"

@@ -55,6 +55,10 @@ void orc_flux(orc_domain*, int ipol, const double* q, const double* u, const dou
 /* reini_RK3::start with count>0, iters pseudo steps (phi in place, ghosts refilled) */
 void orc_reini(orc_domain*, double* phi, int iters);
 
+/* ietimestep::start (ini = 0) / ::ini (ini = 1), N50 = 1: out4 = dt, umax, vmax, wmax */
+void orc_timestep(orc_domain*, const double* u, const double* v, const double* w, double N47, int ini,
+                  double* out4);
+
 /* fluid_update_fsf: rho, visc from phi; volumes out2[0..1] */
 void orc_fluid_update(orc_domain*, const double* phi, double* ro, double* visc, double* vol2);
 

@@ -57,6 +57,7 @@
 #include"heat_void.h"
 #include"concentration_void.h"
 #include"vrans_v.h"
+#include"ietimestep.h"
 #include"6DOF_void.h"
 #include"FSI_void.h"
 #include"pressure_void.h"
@@ -815,6 +816,26 @@ static void run_tank(Env& e, int nsteps, double depth, double amp, double wlen)
     }
 }
 
+// adaptive time step: ietimestep::start and ::ini (selected for N48 = 1 with D20 = 0 or
+// >= 2, driver_logic_cfd.cpp:59-66) on random staggered velocities; maxF/G/H are zero
+// without the EARSM closure (the only writer, EARSM.cpp:175-192)
+static void op_timestep(Env& e, Rng& r, double amp)
+{
+    lexer* p = e.p;
+    fdm* a = e.a;
+    set_velocities(e, r, amp);
+    dump_field("ts_u", a->u);
+    dump_field("ts_v", a->v);
+    dump_field("ts_w", a->w);
+    a->maxF = a->maxG = a->maxH = 0.0;
+    kepsilon_void turb(p, a, e.pgc);
+    ietimestep ts(p);
+    ts.start(a, p, e.pgc, &turb);
+    write_scalars("ts_start", {p->dt, p->umax, p->vmax, p->wmax, p->N47});
+    ts.ini(a, p, e.pgc);
+    write_scalars("ts_ini", {p->dt, p->umax, p->vmax, p->wmax, p->N47});
+}
+
 int main(int argc, char** argv)
 {
     if(argc < 3)
@@ -901,6 +922,7 @@ int main(int argc, char** argv)
             op_poisson(e, r);
             op_bicgstab(e, r);
         }
+        op_timestep(e, r, 0.7);
     }
     else if(cname == "vortex")
         run_vortex(e, 4);
@@ -908,6 +930,7 @@ int main(int argc, char** argv)
         run_tank(e, 3, 0.35, 0.02, 0.8);
     else if(cname == "tank2d")
         run_tank(e, 3, 0.35, 0.02, 0.6);
+    if(cname == "tank2d") op_timestep(e, r, 1.3);
     MPI_Finalize();
     return 0;
 }

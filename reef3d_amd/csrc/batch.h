@@ -37,7 +37,8 @@ struct TankDev
     const float *iDXN32, *iDYN32, *iDZN32;   // FP32 1/DN of the mixed solve (MARGE-offset)
     TileShape shape;               // tiles, chunk length, items
     int item0;                     // first reduction slot (work item) of the tank
-    double dt;                     // time step of the tank
+    double* tstep;                 // device: [0] dt of the next step, [1..3] umax, vmax, wmax of the
+                                   // last time-step pass, [4] smallest cell width (ietimestep)
     const GhostOp* gops[NPROG];    // compiled ghost programs
     int gnops[NPROG];
     BiState* st;                   // solver state (FP64 or mixed)
@@ -88,6 +89,8 @@ void bl_mixed_init(const TankSet& b, int x, int rhs, double stop, int maxiter, d
                    hipStream_t s);
 void bl_mixed_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStream_t s);
 void bl_add32(const TankSet& b, int x, hipStream_t s);
+// ietimestep::start (ini = 0) / ::ini (ini = 1): velocity maxima, then dt into tstep[0]
+void bl_timestep(const TankSet& b, int u, int v, int w, double N47, int ini, hipStream_t s);
 // level-set benchmark: prescribed vortex velocity (benchmark_vortex3D) and the void
 // momentum_RK3 update of one component
 void bl_vortex_velocity(const TankSet& b, int comp, int id, double simtime, double DXM, hipStream_t s);

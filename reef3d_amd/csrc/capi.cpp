@@ -88,6 +88,20 @@ int control(Engine& e, const char* key, double value)
         e.graphs = value != 0.0;
         return 0;
     }
+    if(std::strcmp(key, "N48") == 0)
+    {
+        if(value != 0.0 && value != 1.0) return fail(REEF_E_UNSUPPORTED, "N48 must be 0 (fixed) or 1 (adaptive)");
+        e.drop_graphs();
+        e.n48 = value != 0.0;
+        return 0;
+    }
+    if(std::strcmp(key, "N47") == 0)
+    {
+        if(!(value > 0.0)) return fail(REEF_E_ARG, "N47 must be > 0");
+        e.drop_graphs();
+        e.n47 = value;
+        return 0;
+    }
     return fail(REEF_E_UNSUPPORTED, std::string("control key not supported: ") + key);
 }
 
@@ -369,6 +383,19 @@ int reef_fc3_step(reef_ctx* c, int* iters, double* vol2)
     return 0;
 }
 
+int reef_timestep(reef_ctx* c, double N47, int ini, double* out4)
+{
+    if(check_ctx(c)) return REEF_E_ARG;
+    if(!(N47 > 0.0) || (ini != 0 && ini != 1)) return fail(REEF_E_ARG, "N47 must be > 0 and ini 0 or 1");
+    Engine& e = *c->b.e;
+    std::vector<double> out;
+    int rc;
+    if((rc = e.timestep(N47, ini, out4 ? &out : nullptr))) return eng(e, rc);
+    if(out4)
+        for(int q = 0; q < 4; ++q) out4[q] = out[4 * c->tank + q];
+    return 0;
+}
+
 int reef_vortex_step(reef_ctx* c, double simtime, double* vol2)
 {
     if(check_ctx(c)) return REEF_E_ARG;
@@ -567,6 +594,18 @@ int reef_batch_fc3_step(reef_batch* b, long* iters, double* vol)
         if((rc = e.read_volumes())) return eng(e, rc);
         for(size_t q = 0; q < 2 * e.tanks.size(); ++q) vol[q] = e.vol[q];
     }
+    return 0;
+}
+
+int reef_batch_timestep(reef_batch* b, double N47, int ini, double* out)
+{
+    if(check_batch(b)) return REEF_E_ARG;
+    if(!(N47 > 0.0) || (ini != 0 && ini != 1)) return fail(REEF_E_ARG, "N47 must be > 0 and ini 0 or 1");
+    Engine& e = *b->e;
+    std::vector<double> v;
+    int rc;
+    if((rc = e.timestep(N47, ini, out ? &v : nullptr))) return eng(e, rc);
+    if(out) std::copy(v.begin(), v.end(), out);
     return 0;
 }
 

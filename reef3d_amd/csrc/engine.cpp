@@ -290,6 +290,21 @@ int Engine::add_built(TankBuild& b, int* id)
     if((rc = take(sizeof(BiState), &stb))) return release(), rc;
     EHIP(hipMemsetAsync(stb, 0, sizeof(BiState), stream));
     D.st = (BiState*)stb;
+    // the time-step block: dt, the velocity maxima of the last time-step pass, and the
+    // smallest cell width MIN over the cells of MIN3(DXN, DYN, DZN) (ietimestep.cpp:180-184)
+    void* tsb;
+    if((rc = take(8 * sizeof(double), &tsb))) return release(), rc;
+    D.tstep = (double*)tsb;
+    {
+        double mindx = 1.0e300;
+        for(int a = 0; a < 3; ++a)
+        {
+            const int n = a == 0 ? h.nx : a == 1 ? h.ny : h.nz;
+            for(int q = 0; q < n; ++q) mindx = std::min(mindx, h.DN[a][q + MARGE]);
+        }
+        const double init[8] = {t.prm.dt, 0.0, 0.0, 0.0, mindx, 0.0, 0.0, 0.0};
+        EHIP(hipMemcpy(tsb, init, sizeof(init), hipMemcpyHostToDevice));
+    }
     EHIP(hipMemcpy(mb, meta.data(), meta.size() * 8, hipMemcpyHostToDevice));
     float* invd = (float*)((char*)mb + metabytes);
     EHIP(hipMemcpy(invd, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
@@ -315,7 +330,6 @@ int Engine::add_built(TankBuild& b, int* id)
         D.gnops[q] = b.nop[q];
     }
     D.shape = shape_of(h);
-    D.dt = t.prm.dt;
     // slot
     int slot = -1;
     for(size_t q = 0; q < tanks.size(); ++q)
@@ -988,6 +1002,13 @@ int Engine::fc3_segment(int s, bool mixed)
         if((rc = pjm_post(vel[0], vel[1], vel[2], ALPHA[s - 1], mixed))) return rc;
         if((rc = ghost(vel[0], 1, 10)) || (rc = ghost(vel[1], 2, 11)) || (rc = ghost(vel[2], 3, 12))) return rc;
     }
+    if(s == 0 && n48)
+    {
+        // adaptive time step at the start of the step (driver_loop_cfd.cpp:103-105)
+        tbegin("timestep");
+        each([&](const TankSet& S) { bl_timestep(S, REEF_U, REEF_V, REEF_W, n47, 0, stream); });
+        tend();
+    }
     if(s == 0)
     {
         // stage 1 (momentum_FC3.cpp:132-230)
@@ -1058,6 +1079,24 @@ int Engine::fc3_step()
         for(const TankSet& S : all_)
             for(int p = 0; p < S.n; ++p) step_iters[S.slot[p]] += last_state[S.slot[p]].iter;
     }
+    return 0;
+}
+
+// ietimestep::start / ::ini on the active tanks; out (optional) receives dt, umax, vmax,
+// wmax per slot
+int Engine::timestep(double N47, int ini, std::vector<double>* out)
+{
+    int rc;
+    if((rc = sync_tables()) || (rc = drain())) return rc;
+    tbegin("timestep");
+    each([&](const TankSet& S) { bl_timestep(S, REEF_U, REEF_V, REEF_W, N47, ini, stream); });
+    tend();
+    if(!out) return 0;
+    out->assign(4 * tanks.size(), 0.0);
+    EHIP(hipStreamSynchronize(stream));
+    for(size_t q = 0; q < tanks.size(); ++q)
+        if(tanks[q].used && tanks[q].active)
+            EHIP(hipMemcpy(out->data() + 4 * q, tanks[q].dev.tstep, 4 * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 

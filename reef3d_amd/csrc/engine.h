@@ -91,6 +91,10 @@ public:
     std::vector<TankHost> tanks;    // slots
     int fsf = 0;                    // reinidisc_fsf (F49 = 0) for every tank
     int precision = REEF_SOLVER_FP64;
+    // N48 = 1: every momentum_FC3 step starts with the adaptive time step (ietimestep::start
+    // with the factor N47) computed on the device; 0: the fixed dt of each tank
+    int n48 = 0;
+    double n47 = 0.3;
     double delta = 0.1;             // reliable-update threshold of the mixed solve
     // replay fc3_step's launch sequences (the four segments between its pressure solves)
     // and the BiCGStab iteration batches as hipGraphs, captured on first use; the graphs
@@ -152,6 +156,9 @@ public:
     int residual(int x, int rhs, std::vector<double>& norms);
     int pjm(int u, int v, int w, double alpha);
     int fc3_step();
+    // ietimestep::start (ini = 0) or ::ini (ini = 1): the tanks' next dt on the device; out
+    // (optional) receives dt, umax, vmax, wmax per slot
+    int timestep(double N47, int ini, std::vector<double>* out);
     int phi_rk3();
     int vortex_step(double simtime);
     // the ghost fills and fluid update a run starts with, on the given tanks only

@@ -331,7 +331,7 @@ __global__ void __launch_bounds__(NT, 4) kb_hj_stage(TankSet b, int stage, int p
         const double* __restrict__ v = T.f[v_];
         const double* __restrict__ w = T.f[w_];
         double* __restrict__ out = T.f[out_];
-        const double cdt = cfac * T.dt;
+        const double cdt = cfac * T.tstep[0];
         const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
@@ -432,7 +432,7 @@ __global__ void __launch_bounds__(NT, 2) kb_momentum(TankSet b, int stage, int q
         const double* __restrict__ press = T.f[REEF_PRESS];
         const double* __restrict__ phi = T.f[REEF_PHI];
         double* __restrict__ out = T.f[out_];
-        const double cdt = cfac * T.dt;
+        const double cdt = cfac * T.tstep[0];
         const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(NT) kb_poisson_rhs(TankSet b, int u_, int v_, 
         const double* __restrict__ v = T.f[v_];
         const double* __restrict__ w = T.f[w_];
         double* __restrict__ rhs = T.f[A_RHS];
-        const double adt = afac * T.dt;
+        const double adt = afac * T.tstep[0];
         const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
@@ -581,7 +581,7 @@ __global__ void __launch_bounds__(NT) kb_correct(TankSet b, double rw, double ra
     {
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
-        const double adt = afac * T.dt;
+        const double adt = afac * T.tstep[0];
         const TileWalk tw = tile_walk(T, q.t);
         #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
@@ -1709,6 +1709,90 @@ __global__ void kb_fin_mix_ru(TankSet b)
     }
 }
 
+// ------------------------------------------------------------------ adaptive time step
+// ietimestep::start (ietimestep.cpp:38-207) and ::ini (:209-304) with N50 = 1, first pass:
+// per work item the largest |u| over the u loop (i < nx - 1), |v| over the v loop
+// (j < ny - 1) and |w| over the w loop (k < nz - 1): ULOOP / VLOOP / WLOOP
+// (looping.h:149-151) without the last rows of the staggered grids
+__global__ void __launch_bounds__(NT) kb_velmax(TankSet b, int u_, int v_, int w_)
+{
+    __shared__ double ws[3][NT / 64];
+    #pragma unroll 1
+    for(ItemIter q = set_items_spread(b); q.q < q.hi; next_item(q, b))
+    {
+        CTank& T = tank_at(b, q.p);
+        const DevGrid g = T.g;
+        const double* __restrict__ u = T.f[u_];
+        const double* __restrict__ v = T.f[v_];
+        const double* __restrict__ w = T.f[w_];
+        const TileWalk tw = item_walk(T, b, q);
+        double m[3] = {0.0, 0.0, 0.0};
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            if(i < g.nx - 1) m[0] = fmax(m[0], fabs(u[c]));
+            if(j < g.ny - 1) m[1] = fmax(m[1], fabs(v[c]));
+            if(k < g.nz - 1) m[2] = fmax(m[2], fabs(w[c]));
+        }
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for(int a = 0; a < 3; ++a)
+            for(int off = 32; off > 0; off >>= 1) m[a] = fmax(m[a], __shfl_xor(m[a], off));
+        if(lane == 0)
+            for(int a = 0; a < 3; ++a) ws[a][wave] = m[a];
+        __syncthreads();
+        if(threadIdx.x == 0)
+            for(int a = 0; a < 3; ++a)
+                b.partials[(size_t)item_slot(T, b, q) * 3 + a] =
+                    fmax(fmax(ws[a][0], ws[a][1]), fmax(ws[a][2], ws[a][3]));
+        __syncthreads();
+    }
+}
+
+// second pass, one block per tank: the maxima over the tank's items, then
+//   dt = N47 * MIN over cells of 2 / (|u|max / dx + sqrt(4 |maxF,G,H| / dx)), dx = MIN3(DXN,
+//   DYN, DZN) of the cell, with maxF = maxG = maxH = 0 (only the EARSM closure sets them)
+// (::ini: umax at least 2, dt * 0.25, at least 1e-6).  The term is monotone in dx, so the
+// minimum over the cells is the term at the smallest cell width (tstep[4], from the grid).
+__global__ void kb_fin_dt(TankSet b, double N47, int ini)
+{
+    __shared__ double sh[3][NT];
+    CTank& T = tank_at(b, blockIdx.x);
+    const double* __restrict__ part = b.partials + (size_t)T.item0 * 3;
+    double m[3] = {0.0, 0.0, 0.0};
+    for(int it = threadIdx.x; it < T.shape.nitems; it += NT)
+        for(int a = 0; a < 3; ++a) m[a] = fmax(m[a], part[(size_t)it * 3 + a]);
+    for(int a = 0; a < 3; ++a) sh[a][threadIdx.x] = m[a];
+    __syncthreads();
+    for(int s = NT / 2; s > 0; s >>= 1)
+    {
+        if(threadIdx.x < s)
+            for(int a = 0; a < 3; ++a) sh[a][threadIdx.x] = fmax(sh[a][threadIdx.x], sh[a][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if(threadIdx.x == 0)
+    {
+        double umax = sh[0][0];
+        const double vmax = sh[1][0], wmax = sh[2][0];
+        if(ini) umax = umax > 2.0 ? umax : 2.0;
+        const double dx = T.tstep[4], maxFGH = 0.0;
+        const double c = 2.0 / ((sqrt(umax * umax + vmax * vmax + wmax * wmax)) / dx + sqrt((4.0 * fabs(maxFGH)) / dx));
+        double cu = 1.0e10 < c ? 1.0e10 : c;
+        double dt = N47 * cu;
+        if(ini)
+        {
+            dt = N47 * cu * 0.25;
+            dt = dt > 1.0e-6 ? dt : 1.0e-6;
+        }
+        T.tstep[0] = dt;
+        T.tstep[1] = umax;
+        T.tstep[2] = vmax;
+        T.tstep[3] = wmax;
+    }
+}
+
 // x64 += x32, x32 = 0
 __global__ void __launch_bounds__(NT) kb_add32(TankSet b, int x)
 {
@@ -1826,7 +1910,7 @@ __global__ void __launch_bounds__(NT) kb_void_rk3(TankSet b, int comp, int id, d
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         double* __restrict__ qf = T.f[id];
-        const double dt = T.dt;
+        const double dt = T.tstep[0];
         const TileWalk tw = tile_walk(T, q.t);
 #pragma unroll 1
         for(int t = tw.t; t < tw.t1; ++t)
@@ -2004,6 +2088,12 @@ void bl_mixed_phase(const TankSet& b, int x, int rhs, int phase, int part, hipSt
 void bl_add32(const TankSet& b, int x, hipStream_t s)
 {
     hipLaunchKernelGGL(kb_add32, tgrid(b), dim3(NT), 0, s, b, x);
+}
+
+void bl_timestep(const TankSet& b, int u, int v, int w, double N47, int ini, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_velmax, pgrid(b), dim3(NT), 0, s, b, u, v, w);
+    hipLaunchKernelGGL(kb_fin_dt, tankgrid(b), dim3(NT), 0, s, b, N47, ini);
 }
 
 void bl_vortex_velocity(const TankSet& b, int comp, int id, double simtime, double DXM, hipStream_t s)

@@ -79,6 +79,7 @@ def library():
         "reef_pjm_corr": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ip], ctypes.c_int),
         "reef_fc3_step": ([vp, ip, dp], ctypes.c_int),
         "reef_vortex_step": ([vp, ctypes.c_double, dp], ctypes.c_int),
+        "reef_timestep": ([vp, ctypes.c_double, ctypes.c_int, dp], ctypes.c_int),
         "reef_phi_rk3": ([vp], ctypes.c_int),
         "reef_sync": ([vp], ctypes.c_int),
         "reef_timing_enable": ([vp, ctypes.c_int], ctypes.c_int),
@@ -104,6 +105,7 @@ def library():
         "reef_batch_plans": ([vp, ip, ctypes.c_int, ip], ctypes.c_int),
         "reef_batch_sync": ([vp], ctypes.c_int),
         "reef_batch_run": ([vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "reef_batch_timestep": ([vp, ctypes.c_double, ctypes.c_int, dp], ctypes.c_int),
         "reef_batch_timing_enable": ([vp, ctypes.c_int], ctypes.c_int),
         "reef_batch_timing_get": ([vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "reef_server_create": ([ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)], ctypes.c_int),
@@ -297,6 +299,13 @@ class Domain:
         _check(library().reef_fc3_step(self.h, ctypes.byref(it), _dp(vol)))
         return it.value, vol
 
+    def timestep(self, N47=0.3, ini=False):
+        """ietimestep::start (ini=False) / ::ini: sets the next step's dt on the device;
+        returns (dt, umax, vmax, wmax)."""
+        out = np.zeros(4)
+        _check(library().reef_timestep(self.h, float(N47), 1 if ini else 0, _dp(out)))
+        return tuple(float(x) for x in out)
+
     def vortex_step(self, simtime, volumes=True):
         """levelset_RK3 + reinit + benchmark_vortex3D step; volumes=False skips the
         volume read-back (and the stream synchronisation it needs)."""
@@ -398,6 +407,13 @@ class Batch:
 
     def set_control(self, key, value):
         _check(library().reef_batch_control(self.h, key.encode(), float(value)))
+
+    def timestep(self, N47=0.3, ini=False):
+        """ietimestep for every active tank: {slot: (dt, umax, vmax, wmax)}."""
+        slots = self.info()[0]
+        out = np.zeros(4 * max(slots, 1))
+        _check(library().reef_batch_timestep(self.h, float(N47), 1 if ini else 0, _dp(out)))
+        return {q: tuple(float(x) for x in out[4 * q:4 * q + 4]) for q in range(slots)}
 
     def solver_config(self, precision="fp64", delta=0.1):
         _check(library().reef_batch_solver_config(self.h, {"fp64": 0, "mixed": 1}[precision], delta))

@@ -246,3 +246,53 @@ def test_fc3_step_vs_oracle_multi_tile():
         got = D.get(n)
         err = np.abs(inner(got) - inner(ref)).max() / (np.abs(inner(ref)).max() or 1.0)
         assert err < SOLVED_RTOL, f"{n}: rel err {err:.2e}"
+
+
+@pytest.mark.parametrize("case", ["box", "nonuni", "tank2d"])
+def test_adaptive_timestep_bitexact(case):
+    """ietimestep::start and ::ini on the device: dt and the velocity maxima equal the
+    reference's own (golden ts_start / ts_ini), bit for bit."""
+    d = load(case)
+    D = dev(case, d)
+    for n in ("u", "v", "w"):
+        D.put(n, d["ts_" + n])
+    for ini, key in ((False, "ts_start"), (True, "ts_ini")):
+        got = D.timestep(N47=float(d[key][4]), ini=ini)
+        assert got == tuple(float(x) for x in d[key][:4]), key
+
+
+def test_adaptive_step_n48():
+    """N48 = 1: each step first sets its dt on the device (ietimestep::start).  The step
+    equals the fixed-dt step of an oracle domain built with that dt (within the solver
+    tolerance), and an explicit reef_timestep before a fixed-dt step gives the same bits."""
+    from oracle.oracle import Domain as ODomain
+    from reef3d_amd import Domain as DDomain
+    kw = _medium(nx=140, ny=9, nz=24)
+    phi, u, v, w = _fields(kw, 23)
+    u *= 0.1
+    v *= 0.1
+    w *= 0.1
+    O0 = ODomain(**kw)
+    for f, kind, gcv in ((phi, 4, 52), (u, 1, 10), (v, 2, 11), (w, 3, 12)):
+        O0.ghost(f, kind, gcv)
+    dt = O0.timestep(u, v, w, N47=0.3)[0]
+    O = ODomain(**dict(kw, dt=dt))
+    press = np.zeros(O.shape)
+    ro, visc = np.zeros(O.shape), np.zeros(O.shape)
+    O.fluid_update(phi, ro, visc)
+    start = {"phi": phi.copy(), "u": u.copy(), "v": v.copy(), "w": w.copy(), "press": press.copy(),
+             "ro": ro.copy(), "visc": visc.copy()}
+    A, B = DDomain(**kw), DDomain(**kw)
+    for D in (A, B):
+        for n, a in start.items():
+            D.put(n, a)
+    A.set_control("N48", 1)
+    A.momentum_fc3()
+    assert B.timestep(N47=0.3)[0] == dt
+    B.momentum_fc3()
+    O.fc3_step(phi, u, v, w, press, ro, visc)
+    for n, ref in (("phi", phi), ("u", u), ("v", v), ("w", w), ("press", press)):
+        a = A.get(n)
+        assert bitexact(a, B.get(n)), n
+        err = np.abs(inner(a) - inner(ref)).max() / (np.abs(inner(ref)).max() or 1.0)
+        assert err < SOLVED_RTOL, f"{n}: rel err {err:.2e}"

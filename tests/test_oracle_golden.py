@@ -155,3 +155,15 @@ def test_vortex_benchmark_steps():
         assert same(vol, d[f"vx_vol_{s}"][:2])
     for n, m in (("u", u), ("v", v), ("w", w), ("ro", ro)):
         assert same(inner(m), inner(d[f"vx_{n}_4"])), n
+
+
+@pytest.mark.parametrize("case", ["box", "nonuni", "tank2d"])
+def test_adaptive_timestep(case):
+    """ietimestep::start and ::ini (N48 = 1, N50 = 1): dt and the velocity maxima over the
+    staggered loops, bit-exact.  tank2d has an empty v loop (ny = 1), nonuni stretched
+    cells, and ::ini floors umax at 2 and scales by 0.25."""
+    d = load(case)
+    O = oracle(case, d)
+    for ini, key in ((False, "ts_start"), (True, "ts_ini")):
+        got = O.timestep(d["ts_u"], d["ts_v"], d["ts_w"], N47=float(d[key][4]), ini=ini)
+        assert got == tuple(float(x) for x in d[key][:4]), key
