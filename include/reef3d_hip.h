@@ -163,6 +163,9 @@ int reef_fc3_step(reef_ctx* ctx, int* iters, double* vol2);
  * following steps.  Inflow, floating-body and EARSM terms are zero on the supported box
  * domains.  out4 (may be null): dt, umax, vmax, wmax. */
 int reef_timestep(reef_ctx* c, double N47, int ini, double* out4);
+/* the dt of the following steps (the p->dt that fixtimestep::start, ietimestep::start or the
+ * caller sets before momentum::start, fixtimestep.cpp:38-114) */
+int reef_set_timestep(reef_ctx* c, double dt);
 
 /* one level-set benchmark step (levelset_RK3.cpp:103-178 + benchmark_vortex3D.cpp:67-101,
  * F150=3) at time simtime before the increment. */

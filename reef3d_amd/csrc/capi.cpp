@@ -383,6 +383,15 @@ int reef_fc3_step(reef_ctx* c, int* iters, double* vol2)
     return 0;
 }
 
+int reef_set_timestep(reef_ctx* c, double dt)
+{
+    if(check_ctx(c)) return REEF_E_ARG;
+    if(!(dt > 0.0)) return fail(REEF_E_ARG, "dt must be > 0");
+    Engine& e = *c->b.e;
+    const int rc = e.set_dt(dt);
+    return rc ? eng(e, rc) : 0;
+}
+
 int reef_timestep(reef_ctx* c, double N47, int ini, double* out4)
 {
     if(check_ctx(c)) return REEF_E_ARG;

@@ -1082,6 +1082,22 @@ int Engine::fc3_step()
     return 0;
 }
 
+// the step size of the active tanks' following steps (fixtimestep / the reference's own
+// time stepping setting p->dt)
+int Engine::set_dt(double dt)
+{
+    int rc;
+    if((rc = sync_tables()) || (rc = drain())) return rc;
+    EHIP(hipStreamSynchronize(stream));
+    for(TankHost& t : tanks)
+        if(t.used && t.active)
+        {
+            t.prm.dt = dt;
+            EHIP(hipMemcpy(t.dev.tstep, &dt, sizeof(double), hipMemcpyHostToDevice));
+        }
+    return 0;
+}
+
 // ietimestep::start / ::ini on the active tanks; out (optional) receives dt, umax, vmax,
 // wmax per slot
 int Engine::timestep(double N47, int ini, std::vector<double>* out)

@@ -159,6 +159,8 @@ public:
     // ietimestep::start (ini = 0) or ::ini (ini = 1): the tanks' next dt on the device; out
     // (optional) receives dt, umax, vmax, wmax per slot
     int timestep(double N47, int ini, std::vector<double>* out);
+    // the dt of the active tanks' following steps
+    int set_dt(double dt);
     int phi_rk3();
     int vortex_step(double simtime);
     // the ghost fills and fluid update a run starts with, on the given tanks only

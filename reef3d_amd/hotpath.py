@@ -80,6 +80,7 @@ def library():
         "reef_fc3_step": ([vp, ip, dp], ctypes.c_int),
         "reef_vortex_step": ([vp, ctypes.c_double, dp], ctypes.c_int),
         "reef_timestep": ([vp, ctypes.c_double, ctypes.c_int, dp], ctypes.c_int),
+        "reef_set_timestep": ([vp, ctypes.c_double], ctypes.c_int),
         "reef_phi_rk3": ([vp], ctypes.c_int),
         "reef_sync": ([vp], ctypes.c_int),
         "reef_timing_enable": ([vp, ctypes.c_int], ctypes.c_int),
@@ -305,6 +306,10 @@ class Domain:
         out = np.zeros(4)
         _check(library().reef_timestep(self.h, float(N47), 1 if ini else 0, _dp(out)))
         return tuple(float(x) for x in out)
+
+    def set_timestep(self, dt):
+        """dt of the following steps (p->dt)."""
+        _check(library().reef_set_timestep(self.h, float(dt)))
 
     def vortex_step(self, simtime, volumes=True):
         """levelset_RK3 + reinit + benchmark_vortex3D step; volumes=False skips the

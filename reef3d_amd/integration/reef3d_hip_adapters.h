@@ -24,6 +24,7 @@
 #include "lexer.h"
 #include "momentum.h"
 #include "reini.h"
+#include "timestep.h"
 #include "reef3d_hip.h"
 
 namespace reef_adapt {
@@ -160,6 +161,42 @@ public:
     }
 };
 
+// timestep for N48 = 1 with D20 = 0 or >= 2 (ietimestep, driver_logic_cfd.cpp:59-66):
+// p->dt, p->umax, p->vmax, p->wmax from the device (reef_timestep).  The diagnostics the
+// reference also gathers there (viscmax, kinmax, epsmax, pressmax/min, fbdt) are not set.
+class hip_ietimestep : public timestep
+{
+public:
+    explicit hip_ietimestep(lexer* p) { context(p); }
+    void start(fdm* a, lexer* p, ghostcell* pgc, turbulence* pturb) override
+    {
+        (void)pgc;
+        (void)pturb;
+        run(a, p, 0);
+    }
+    void ini(fdm* a, lexer* p, ghostcell* pgc) override
+    {
+        (void)pgc;
+        run(a, p, 1);
+        p->dt_old = p->dt;
+    }
+
+private:
+    static void run(fdm* a, lexer* p, int ini)
+    {
+        reef_ctx* c = context(p);
+        put(c, REEF_U, a->u);
+        put(c, REEF_V, a->v);
+        put(c, REEF_W, a->w);
+        double out[4];
+        check(reef_timestep(c, p->N47, ini, out), "reef_timestep");
+        p->dt = out[0];
+        p->umax = out[1];
+        p->vmax = out[2];
+        p->wmax = out[3];
+    }
+};
+
 // momentum::start for N40 = 3 (momentum_FC3 with weno_flux_nug, weno_hj_df_nug,
 // reini_RK3, diff_void, pjm_corr, bicgstab_ijk): the whole step on the device
 class hip_momentum_FC3 : public momentum
@@ -172,6 +209,7 @@ public:
         (void)pvrans;
         (void)p6dof;
         reef_ctx* c = context(p);
+        check(reef_set_timestep(c, p->dt), "reef_set_timestep");   // this step's p->dt
         put(c, REEF_U, a->u);
         put(c, REEF_V, a->v);
         put(c, REEF_W, a->w);

@@ -296,3 +296,25 @@ def test_adaptive_step_n48():
         assert bitexact(a, B.get(n)), n
         err = np.abs(inner(a) - inner(ref)).max() / (np.abs(inner(ref)).max() or 1.0)
         assert err < SOLVED_RTOL, f"{n}: rel err {err:.2e}"
+
+
+def test_set_timestep_matches_fixed_dt_context():
+    """reef_set_timestep: a context created with one dt and switched to another steps like
+    a context created with the other dt (the adapter's per-step p->dt)."""
+    from reef3d_amd import Domain as DDomain
+    kw = _medium(nx=70, ny=5, nz=16)
+    phi, u, v, w = _fields(kw, 31)
+    A, B = DDomain(**kw), DDomain(**dict(kw, dt=0.5 * kw["dt"]))
+    for D in (A, B):
+        for n, a in (("phi", phi), ("u", 0.1 * u), ("v", 0.1 * v), ("w", 0.1 * w)):
+            D.put(n, a)
+        D.start4("phi", 52)
+        D.start1("u", 10)
+        D.start2("v", 11)
+        D.start3("w", 12)
+        D.fluid_update()
+    A.set_timestep(0.5 * kw["dt"])
+    A.momentum_fc3()
+    B.momentum_fc3()
+    for n in ("phi", "u", "v", "w", "press"):
+        assert bitexact(A.get(n), B.get(n)), n
