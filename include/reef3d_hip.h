@@ -76,6 +76,22 @@ typedef struct reef_params
 
 typedef struct reef_ctx reef_ctx;
 
+/* ---- REEF3D case files (host only; no device work) */
+typedef struct reef_grid_file reef_grid_file;
+/* lexer::read_grid (read_grid.cpp:29-560) of a single-rank box domain: the DIVEMesh grid
+ * file's header, flag4, node coordinates and boundary entries fill *desc (its node arrays
+ * live in *file until reef_grid_file_free); solids, topography, parallel or periodic
+ * grids and sides with mixed boundary groups give REEF_E_UNSUPPORTED */
+int reef_grid_read(const char* path, reef_grid_file** file, reef_grid_desc* desc);
+void reef_grid_file_free(reef_grid_file* file);
+/* the hot-path keys of a ctrl.txt (control::read_control, read_control.cpp:28-70) over the
+ * defaults of ini.cpp: prm gets W1-W4, W20-W22, F43, F45, N44, N46 and N49 as dt; ctrl4
+ * (may be null) gets F49, N48, N47, N40.  Keys choosing another scheme than the hot
+ * path's (D10, D20, D30, F30, F35, F40, N10, N40, M10) give REEF_E_UNSUPPORTED. */
+int reef_ctrl_read(const char* path, reef_params* prm, double* ctrl4);
+/* message of the last failed reef_grid_read / reef_ctrl_read of this thread */
+const char* reef_case_last_error(void);
+
 /* lexer + fdm + ghostcell set-up on the device (replaces driver.cpp:33-118 and
  * driver_makegrid.cpp:29-62 for a box domain); allocates all fields, zeroed. */
 int reef_ctx_create(const reef_grid_desc* grid, const reef_params* prm, reef_ctx** out);

@@ -25,6 +25,7 @@
 #include<string>
 #include<vector>
 #include<fstream>
+#include<iterator>
 #include<sys/stat.h>
 #include<unistd.h>
 
@@ -114,6 +115,23 @@ static void write_npy(const string& name, const double* data, const vector<long>
 static void write_scalars(const string& name, const vector<double>& v)
 {
     write_npy(name, v.data(), {(long)v.size()});
+}
+
+// raw bytes (a file the reference read) as a uint8 .npy
+static void dump_bytes(const string& name, const vector<char>& b)
+{
+    string hdr = "{'descr': '|u1', 'fortran_order': False, 'shape': (" + std::to_string(b.size()) + ",), }";
+    size_t total = 10 + hdr.size() + 1;
+    size_t pad = (64 - total % 64) % 64;
+    hdr.append(pad, ' ');
+    hdr += '\n';
+    std::ofstream f(g_out + "/" + name + ".npy", std::ios::binary);
+    const char magic[] = "\x93NUMPY\x01\x00";
+    f.write(magic, 8);
+    uint16_t hl = (uint16_t)hdr.size();
+    f.write((const char*)&hl, 2);
+    f.write(hdr.data(), hdr.size());
+    f.write(b.data(), b.size());
 }
 
 // ---------------------------------------------------------------- case description
@@ -900,6 +918,15 @@ int main(int argc, char** argv)
     G_nx = c.nx; G_ny = c.ny; G_nz = c.nz;
 
     Env e = setup(c);
+    {
+        // the grid file the reference read, kept as a fixture for the library's reader
+        std::ifstream gf("DIVEMesh_Grid/grid-000001.dat", std::ios::binary);
+        vector<char> bytes((std::istreambuf_iterator<char>(gf)), std::istreambuf_iterator<char>());
+        dump_bytes("gridfile", bytes);
+        std::ifstream cf("ctrl.txt");
+        vector<char> ctext((std::istreambuf_iterator<char>(cf)), std::istreambuf_iterator<char>());
+        dump_bytes("ctrlfile", ctext);
+    }
     Rng r(1000 + (uint64_t)cname.size() * 7919u);
     // weno tables are static and built by the first weno_nug_func object
     {

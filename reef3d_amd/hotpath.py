@@ -63,6 +63,10 @@ def library():
         "reef_ctx_destroy": ([vp], None),
         "reef_set_control": ([vp, ctypes.c_char_p, ctypes.c_double], ctypes.c_int),
         "reef_last_error": ([], ctypes.c_char_p),
+        "reef_case_last_error": ([], ctypes.c_char_p),
+        "reef_grid_read": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(GridDesc)], ctypes.c_int),
+        "reef_grid_file_free": ([vp], None),
+        "reef_ctrl_read": ([ctypes.c_char_p, ctypes.POINTER(Params), dp], ctypes.c_int),
         "reef_field_put": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
         "reef_field_get": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
         "reef_field_zero": ([vp, ctypes.c_int], ctypes.c_int),
@@ -141,6 +145,35 @@ def _check(rc):
         raise ReefError(f"reef call failed ({rc}): {msg}")
 
 
+def _case_check(rc):
+    if rc != 0:
+        msg = library().reef_case_last_error().decode(errors="replace")
+        raise ReefError(f"reef case file ({rc}): {msg}")
+
+
+def read_grid_file(path):
+    """lexer::read_grid of a DIVEMesh grid file (reef_grid_read): a dict with nx, ny, nz,
+    jdir, XN, YN, ZN, DYM, DZM and group (boundary group of sides 1..6)."""
+    L = library()
+    h, gd = ctypes.c_void_p(), GridDesc()
+    _case_check(L.reef_grid_read(os.fsencode(path), ctypes.byref(h), ctypes.byref(gd)))
+    try:
+        out = dict(nx=gd.knox, ny=gd.knoy, nz=gd.knoz, jdir=gd.j_dir, DYM=gd.DYM, DZM=gd.DZM,
+                   group=[gd.group[q] for q in range(7)])
+        for key, ptr, n in (("XN", gd.XN, gd.knox), ("YN", gd.YN, gd.knoy), ("ZN", gd.ZN, gd.knoz)):
+            out[key] = np.ctypeslib.as_array(ptr, shape=(n + 11,)).copy()
+    finally:
+        L.reef_grid_file_free(h)
+    return out
+
+
+def read_ctrl_file(path):
+    """The hot-path keys of a ctrl.txt (reef_ctrl_read): (Params, dict F49, N48, N47, N40)."""
+    prm, c4 = Params(), np.zeros(4)
+    _case_check(library().reef_ctrl_read(os.fsencode(path), ctypes.byref(prm), _dp(c4)))
+    return prm, dict(F49=int(c4[0]), N48=int(c4[1]), N47=float(c4[2]), N40=int(c4[3]))
+
+
 def grid_desc(nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21):
     """reef_grid_desc of a box; the node arrays are returned too (they must stay alive)."""
     xn = [np.ascontiguousarray(a, dtype=np.float64) for a in (XN, YN, ZN)]
@@ -193,6 +226,24 @@ class Domain:
         self.h = h
         if F49 != 1:
             self.set_control("F49", F49)
+
+    @classmethod
+    def from_case(cls, case_dir):
+        """A domain from a REEF3D case directory: DIVEMesh_Grid/grid-000001.dat and ctrl.txt
+        (the hot-path keys; N48 = 1 turns on the adaptive time step of every step)."""
+        g = read_grid_file(os.path.join(case_dir, "DIVEMesh_Grid", "grid-000001.dat"))
+        prm, ctrl = read_ctrl_file(os.path.join(case_dir, "ctrl.txt"))
+        if g["group"][1:6] != [21] * 5:
+            raise ReefError(f"reef case: sides 1..5 must be walls (group 21), got {g['group'][1:6]}")
+        D = cls(g["nx"], g["ny"], g["nz"], g["XN"], g["YN"], g["ZN"], g["DYM"], g["DZM"], jdir=g["jdir"],
+                top_group=g["group"][6], rho=(prm.W1, prm.W3), visc=(prm.W2, prm.W4),
+                g=(prm.W20, prm.W21, prm.W22), F43=prm.F43, F45=prm.F45, dt=prm.dt, N44=prm.N44, N46=prm.N46,
+                F49=ctrl["F49"])
+        if ctrl["N48"]:
+            D.set_control("N47", ctrl["N47"])
+            D.set_control("N48", 1)
+        D.ctrl = ctrl
+        return D
 
     def set_control(self, key, value):
         """lexer control key of a hot-path variant (reef_set_control), e.g. F49 = 0."""

@@ -1,0 +1,73 @@
+"""REEF3D case files at the boundary, host only (no GPU): the DIVEMesh grid reader and the
+ctrl.txt hot-path keys, against the files the reference's own run read (the fixture
+harness stores them beside the grid arrays the reference built from them)."""
+import numpy as np
+import pytest
+
+from tests._cases import CASES, load
+
+HEADER_BYTES = 74 * 4 + 22 * 8          # read_grid.cpp:76-295: 74 ints and 22 doubles
+
+
+def _write(tmp_path, name, data):
+    p = tmp_path / name
+    p.write_bytes(bytes(data))
+    return str(p)
+
+
+@pytest.mark.parametrize("case", ["box", "nonuni", "vortex", "tank", "tank2d", "tankfsf"])
+def test_grid_file_gives_the_references_grid(case, tmp_path):
+    from reef3d_amd.hotpath import read_grid_file
+    d = load(case)
+    c = CASES[case]
+    g = read_grid_file(_write(tmp_path, "grid.dat", d["gridfile"]))
+    assert (g["nx"], g["ny"], g["nz"], g["jdir"]) == (c["nx"], c["ny"], c["nz"], c["jdir"])
+    # the node arrays the reference read from the same file (lexer::XN, ...), bit for bit
+    for key, n in (("XN", c["nx"]), ("YN", c["ny"]), ("ZN", c["nz"])):
+        assert np.array_equal(g[key], d[key][:n + 11]), key
+    gs = d["grid_scalars"]
+    assert (g["DYM"], g["DZM"]) == (gs[1], gs[2])
+    assert g["group"][1:] == [21, 21, 21, 21, 21, c["top_group"]]
+
+
+def test_grid_file_refusals(tmp_path):
+    from reef3d_amd import ReefError
+    from reef3d_amd.hotpath import read_grid_file
+    raw = bytearray(load("box")["gridfile"].tobytes())
+    solid = bytearray(raw)
+    solid[HEADER_BYTES:HEADER_BYTES + 4] = np.int32(-1).tobytes()     # flag4 of cell (0,0,0)
+    with pytest.raises(ReefError, match="solid"):
+        read_grid_file(_write(tmp_path, "solid.dat", solid))
+    with pytest.raises(ReefError, match="truncated"):
+        read_grid_file(_write(tmp_path, "short.dat", raw[:HEADER_BYTES + 100]))
+    topo = bytearray(raw)
+    off = (4 + 6 + 1 + 20 + 6 + 6 + 6 + 9 + 3 + 1 + 1) * 4 + 16 * 8     # toporead
+    assert np.frombuffer(bytes(topo[off:off + 4]), np.int32)[0] == 0
+    topo[off:off + 4] = np.int32(1).tobytes()
+    with pytest.raises(ReefError, match="topography"):
+        read_grid_file(_write(tmp_path, "topo.dat", topo))
+
+
+@pytest.mark.parametrize("case", ["box", "tank2d", "vortex"])
+def test_ctrl_file_hot_path_keys(case, tmp_path):
+    from reef3d_amd.hotpath import read_ctrl_file
+    d = load(case)
+    c = CASES[case]
+    prm, ctrl = read_ctrl_file(_write(tmp_path, "ctrl.txt", d["ctrlfile"]))
+    assert (prm.W1, prm.W2, prm.W3, prm.W4) == (1000.0, 1.0e-6, 1.205, 1.4938e-5)
+    assert (prm.W20, prm.W21, prm.W22) == c["g"]
+    assert (prm.dt, prm.N44) == (c["dt"], c["N44"])
+    assert (prm.F43, prm.F45) == (0.55, 2.1)        # ini.cpp defaults, not in the file
+    assert ctrl["F49"] == 1 and ctrl["N48"] == 0 and ctrl["N40"] == (13 if case == "vortex" else 3)
+
+
+def test_ctrl_file_refuses_other_schemes(tmp_path):
+    from reef3d_amd import ReefError
+    from reef3d_amd.hotpath import read_ctrl_file
+    base = "N 10 1\nD 20 0\nN 40 3\n"
+    prm, ctrl = read_ctrl_file(_write(tmp_path, "ok.txt", (base + "/ comment\nN 48 1\nN 47 0.2\n").encode()))
+    assert ctrl["N48"] == 1 and ctrl["N47"] == 0.2 and prm.N46 == 250 and prm.W1 == 998.2
+    for text, key in (("D 20 0\nN 40 3\n", "N 10"), ("N 10 1\nN 40 3\n", "D 20"),
+                      (base + "D 10 5\n", "D 10"), (base + "N 40 2\n", "N 40")):
+        with pytest.raises(ReefError, match=key):
+            read_ctrl_file(_write(tmp_path, "bad.txt", text.encode()))

@@ -318,3 +318,22 @@ def test_set_timestep_matches_fixed_dt_context():
     B.momentum_fc3()
     for n in ("phi", "u", "v", "w", "press"):
         assert bitexact(A.get(n), B.get(n)), n
+
+
+def test_domain_from_case_files(tmp_path):
+    """A domain built from the case files the reference itself read (the DIVEMesh grid file
+    and ctrl.txt of the fixture run) steps like the reference."""
+    from reef3d_amd import Domain as DDomain
+    d = load("tank")
+    (tmp_path / "DIVEMesh_Grid").mkdir()
+    (tmp_path / "DIVEMesh_Grid" / "grid-000001.dat").write_bytes(d["gridfile"].tobytes())
+    (tmp_path / "ctrl.txt").write_bytes(d["ctrlfile"].tobytes())
+    D = DDomain.from_case(str(tmp_path))
+    for n in ("phi", "u", "v", "w", "press", "ro", "visc"):
+        D.put(n, d[f"tk_{n}_0"])
+    it, vol = D.momentum_fc3()
+    assert it > 0
+    for n in ("phi", "u", "v", "w", "press"):
+        got, ref = inner(D.get(n)), inner(d[f"tk_{n}_1"])
+        err = np.abs(got - ref).max() / (np.abs(ref).max() or 1.0)
+        assert err < SOLVED_RTOL, f"{n}: rel err {err:.2e}"
