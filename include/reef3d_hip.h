@@ -179,6 +179,12 @@ int reef_fc3_step(reef_ctx* ctx, int* iters, double* vol2);
  * following steps.  Inflow, floating-body and EARSM terms are zero on the supported box
  * domains.  out4 (may be null): dt, umax, vmax, wmax. */
 int reef_timestep(reef_ctx* c, double N47, int ini, double* out4);
+/* FP64 checkpoint of a context (no counterpart in the reference, whose restart files are
+ * FP32, cfd_state_write.cpp:34-146): grid size, the device dt and every public field with
+ * its ghost layers.  reef_checkpoint_read restores them into a context of the same grid;
+ * the steps that follow are bit-identical to those of the uninterrupted run. */
+int reef_checkpoint_write(reef_ctx* c, const char* path);
+int reef_checkpoint_read(reef_ctx* c, const char* path);
 /* the dt of the following steps (the p->dt that fixtimestep::start, ietimestep::start or the
  * caller sets before momentum::start, fixtimestep.cpp:38-114) */
 int reef_set_timestep(reef_ctx* c, double dt);

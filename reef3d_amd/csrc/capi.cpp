@@ -392,6 +392,81 @@ int reef_set_timestep(reef_ctx* c, double dt)
     return rc ? eng(e, rc) : 0;
 }
 
+// FP64 checkpoint: "REEF3DCK", version, nx, ny, nz, j_dir, the device dt, the number of
+// fields, then each public field (id, (nx+6)(ny+6)(nz+6) doubles, i fastest with ghosts)
+int reef_checkpoint_write(reef_ctx* c, const char* path)
+{
+    if(check_ctx(c) || !path) return fail(REEF_E_ARG, "bad argument");
+    Engine& e = *c->b.e;
+    TankHost* t = e.tank(c->tank);
+    if(!t) return fail(REEF_E_ARG, "no tank");
+    const HostGrid& h = t->hg;
+    const size_t n = (size_t)(h.nx + 6) * (h.ny + 6) * (h.nz + 6);
+    std::vector<double> buf(n);
+    double dt = 0.0;
+    int rc;
+    if((rc = e.get(c->tank, REEF_U, buf.data(), REEF_LAYOUT_IFAST))) return eng(e, rc);   // synchronises
+    HIPCHK(hipMemcpy(&dt, t->dev.tstep, sizeof(double), hipMemcpyDeviceToHost));
+    FILE* f = std::fopen(path, "wb");
+    if(!f) return fail(REEF_E_ARG, std::string("cannot write ") + path);
+    const int head[6] = {1, h.nx, h.ny, h.nz, h.jdir, REEF_NFIELDS};
+    bool ok = std::fwrite("REEF3DCK", 1, 8, f) == 8 && std::fwrite(head, sizeof(int), 5, f) == 5 &&
+              std::fwrite(&dt, sizeof(double), 1, f) == 1 && std::fwrite(head + 5, sizeof(int), 1, f) == 1;
+    for(int id = 0; ok && id < REEF_NFIELDS; ++id)
+    {
+        if((rc = e.get(c->tank, id, buf.data(), REEF_LAYOUT_IFAST)))
+        {
+            std::fclose(f);
+            return eng(e, rc);
+        }
+        ok = std::fwrite(&id, sizeof(int), 1, f) == 1 && std::fwrite(buf.data(), sizeof(double), n, f) == n;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? 0 : fail(REEF_E_ARG, std::string("write failed: ") + path);
+}
+
+int reef_checkpoint_read(reef_ctx* c, const char* path)
+{
+    if(check_ctx(c) || !path) return fail(REEF_E_ARG, "bad argument");
+    Engine& e = *c->b.e;
+    TankHost* t = e.tank(c->tank);
+    if(!t) return fail(REEF_E_ARG, "no tank");
+    const HostGrid& h = t->hg;
+    const size_t n = (size_t)(h.nx + 6) * (h.ny + 6) * (h.nz + 6);
+    FILE* f = std::fopen(path, "rb");
+    if(!f) return fail(REEF_E_ARG, std::string("cannot open ") + path);
+    char magic[8];
+    int head[6];
+    double dt = 0.0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "REEF3DCK", 8) == 0 &&
+              std::fread(head, sizeof(int), 5, f) == 5 && std::fread(&dt, sizeof(double), 1, f) == 1 &&
+              std::fread(head + 5, sizeof(int), 1, f) == 1;
+    if(!ok || head[0] != 1)
+    {
+        std::fclose(f);
+        return fail(REEF_E_ARG, "not a checkpoint of this library");
+    }
+    if(head[1] != h.nx || head[2] != h.ny || head[3] != h.nz || head[4] != h.jdir || head[5] != REEF_NFIELDS)
+    {
+        std::fclose(f);
+        return fail(REEF_E_ARG, "checkpoint of another grid");
+    }
+    std::vector<std::vector<double>> fields(REEF_NFIELDS, std::vector<double>(n));
+    for(int q = 0; ok && q < REEF_NFIELDS; ++q)
+    {
+        int id = -1;
+        ok = std::fread(&id, sizeof(int), 1, f) == 1 && id == q && std::fread(fields[q].data(), sizeof(double), n, f) == n;
+    }
+    std::fclose(f);
+    if(!ok) return fail(REEF_E_ARG, "checkpoint is truncated");
+    int rc;
+    for(int q = 0; q < REEF_NFIELDS; ++q)
+        if((rc = e.put(c->tank, q, fields[q].data(), REEF_LAYOUT_IFAST))) return eng(e, rc);
+    HIPCHK(hipMemcpy(t->dev.tstep, &dt, sizeof(double), hipMemcpyHostToDevice));
+    t->prm.dt = dt;
+    return 0;
+}
+
 int reef_timestep(reef_ctx* c, double N47, int ini, double* out4)
 {
     if(check_ctx(c)) return REEF_E_ARG;

@@ -85,6 +85,8 @@ def library():
         "reef_vortex_step": ([vp, ctypes.c_double, dp], ctypes.c_int),
         "reef_timestep": ([vp, ctypes.c_double, ctypes.c_int, dp], ctypes.c_int),
         "reef_set_timestep": ([vp, ctypes.c_double], ctypes.c_int),
+        "reef_checkpoint_write": ([vp, ctypes.c_char_p], ctypes.c_int),
+        "reef_checkpoint_read": ([vp, ctypes.c_char_p], ctypes.c_int),
         "reef_phi_rk3": ([vp], ctypes.c_int),
         "reef_sync": ([vp], ctypes.c_int),
         "reef_timing_enable": ([vp, ctypes.c_int], ctypes.c_int),
@@ -357,6 +359,14 @@ class Domain:
         out = np.zeros(4)
         _check(library().reef_timestep(self.h, float(N47), 1 if ini else 0, _dp(out)))
         return tuple(float(x) for x in out)
+
+    def checkpoint(self, path):
+        """FP64 checkpoint of every public field and the step size (reef_checkpoint_write)."""
+        _check(library().reef_checkpoint_write(self.h, os.fsencode(path)))
+
+    def restore(self, path):
+        """Restores a checkpoint of a domain with the same grid (reef_checkpoint_read)."""
+        _check(library().reef_checkpoint_read(self.h, os.fsencode(path)))
 
     def set_timestep(self, dt):
         """dt of the following steps (p->dt)."""
