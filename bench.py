@@ -160,6 +160,28 @@ STENCIL_BYTES = {
     "hj": (40 + 48 + 48) / 3.0,      # phi_s, u, v, w [, phi_n] in; phi out
     "reini": (16 + 24 + 24) / 3.0,   # phi_s [, phi_0] in; phi out
 }
+# the WENO stencils are bound by FP64 VALU issue, not by HBM: FP64 flops per cell
+# (add + mul + 2 fma + trans) counted with SQ_INSTS_VALU_*_F64 on the 512x64x128 tank
+# (profiles/wave_tank_512x64x128_stencil_valu.md); the same kernels run here
+STENCIL_FP64_FLOPS = {"hj": 1072.0, "reini": 1082.0, "momentum": 1488.0}
+FP64_PEAK_TFLOPS = 78.6
+MOMENTUM_BYTES = (56 + 64 + 64) / 3.0   # q_in, u, v, w, p, phi [, q_old] in; q out
+
+
+def stencil_rooflines(fams, cells):
+    """HBM and FP64-VALU rates of the WENO stencil kernels from their live launch times."""
+    out = {}
+    for k, bpc in (("hj", STENCIL_BYTES["hj"]), ("reini", STENCIL_BYTES["reini"]), ("momentum", MOMENTUM_BYTES)):
+        ms, n = fams.get(k, (0.0, 0))
+        if not n:
+            continue
+        avg_s = ms / 1e3 / n
+        gbs = bpc * cells / avg_s / 1e9
+        tf = STENCIL_FP64_FLOPS[k] * cells / avg_s / 1e12
+        out[k] = {"avg_launch_us": round(avg_s * 1e6, 2), "hbm_GBs": round(gbs, 1),
+                  "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "fp64_TFLOPs": round(tf, 2),
+                  "fp64_frac": round(tf / FP64_PEAK_TFLOPS, 4), "bound": "valu"}
+    return out
 
 
 def vortex(args):
@@ -224,7 +246,9 @@ def vortex(args):
                    "rel_err_vs_exact": float(vol_end[0] / exact - 1.0)},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_cell": round(STENCIL_BYTES[dom], 2)},
+                     "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_cell": round(STENCIL_BYTES[dom], 2),
+                     # what actually bounds it: FP64 VALU issue (see STENCIL_FP64_FLOPS)
+                     "valu": stencil_rooflines(fams, cells).get(dom)},
         "kernel_ms_per_step": {k: round(v[0] / steps, 4) for k, v in fams.items() if v[1]},
     }
     if args.check > 0:
@@ -345,7 +369,15 @@ def tanks(args):
     B.run(args.warmup)
     B.sync()
     t0 = time.perf_counter()
-    B.run(args.steps)
+    done = 0
+    while done < args.steps:
+        # chunks of 20 steps, each reported when it ends (every tank finishes a chunk
+        # before the next one starts)
+        n = min(20, args.steps - done)
+        B.run(n)
+        done += n
+        if args.steps > 20:
+            print(f"# tanks scheduled: {done} steps", file=sys.stderr, flush=True)
     B.sync()
     t_sched = time.perf_counter() - t0
     same = None
@@ -369,6 +401,7 @@ def tanks(args):
         seq_time += time.perf_counter() - t1
         seq_cells += sp["nx"] * sp["nz"]
         S.close()
+        print(f"# tanks sequential: {sp['nx']}x{sp['nz']} done", file=sys.stderr, flush=True)
     lock_rate = cells * args.steps / t_lock / 1e6
     sched_rate = cells * args.steps / t_sched / 1e6
     seq_rate = seq_cells * args.steps / seq_time / 1e6
@@ -697,6 +730,7 @@ def main():
             "bytes_per_cell": SOLVER_BYTES[dom],
         },
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in fams.items() if v[1]},
+        "stencil_rooflines": stencil_rooflines(fams, cells),
     }
     if not args.no_mixed:
         del D
