@@ -166,6 +166,20 @@ __device__ __forceinline__ bool tile_cell(const DevGrid& g, const TileWalk& w, i
     return true;
 }
 
+// tile_cell for the per-cell stencils: a wavefront covers one row of the tile, so j and k
+// are the same in all its lanes and are made scalar; the y / z grid tables they index then
+// load through the scalar unit
+__device__ __forceinline__ bool tile_cell_row(const DevGrid& g, const TileWalk& w, int t, int& i, int& j, int& k,
+                                              long& c)
+{
+    tile_cell_of(w.s, g.nx, g.ny, g.nz, t, (int)threadIdx.x, i, j, k);
+    j = __builtin_amdgcn_readfirstlane(j);
+    k = __builtin_amdgcn_readfirstlane(k);
+    if(i >= g.nx || j >= g.ny || k >= g.nz) return false;
+    c = g.base + i + g.sx * j + g.sxy * k;
+    return true;
+}
+
 // block sum of M values (wavefront butterfly, lane 0's sum, 4-wave combine) into dst[0..M)
 template<int M>
 __device__ __forceinline__ void block_sum_to(double (&v)[M], double* __restrict__ dst)
@@ -338,7 +352,7 @@ __global__ void __launch_bounds__(NT, 4) kb_hj_stage(TankSet b, int stage, int p
         {
             int i, j, k;
             long c;
-            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            if(!tile_cell_row(g, tw, t, i, j, k, c)) continue;
             const double L = hj_L(g, phi_s, u, v, w, c, i, j, k);
             const double La = 0.0 + L;
             if(stage == 0)
@@ -371,7 +385,7 @@ __global__ void __launch_bounds__(NT, 4) kb_reini_stage(TankSet b, int stage, in
         {
             int i, j, k;
             long c;
-            if(!tile_cell(g, tw, t, i, j, k, c)) continue;
+            if(!tile_cell_row(g, tw, t, i, j, k, c)) continue;
             const double lsv = f[c];
             const double L = reini_L(g, f, c, i, j, k, fsf);
             const double dtc = reini_dtc(g, i, j, k, F43);
@@ -406,7 +420,7 @@ __global__ void __launch_bounds__(NT, 4) kb_flux_acc(TankSet b, int ipol, int q_
         {
             int i, j, k;
             long c;
-            if(!tile_cell(g, tw, t, i, j, k, c) || !comp_cell(g, ipol, i, j, k)) continue;
+            if(!tile_cell_row(g, tw, t, i, j, k, c) || !comp_cell(g, ipol, i, j, k)) continue;
             acc[c] += flux_div(g, ipol, qf, u, v, w, c, i, j, k);
         }
     }
@@ -439,7 +453,7 @@ __global__ void __launch_bounds__(NT, 2) kb_momentum(TankSet b, int stage, int q
         {
             int i, j, k;
             long c;
-            if(!tile_cell(g, tw, t, i, j, k, c) || !comp_cell(g, comp, i, j, k)) continue;
+            if(!tile_cell_row(g, tw, t, i, j, k, c) || !comp_cell(g, comp, i, j, k)) continue;
             const double F = momentum_F(g, comp, qin, u, v, w, press, phi, grav, rw, ra, c, i, j, k);
             const double df = qin[c];
             if(stage == 1)
