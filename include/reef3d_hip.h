@@ -185,6 +185,24 @@ int reef_timestep(reef_ctx* c, double N47, int ini, double* out4);
  * the steps that follow are bit-identical to those of the uninterrupted run. */
 int reef_checkpoint_write(reef_ctx* c, const char* path);
 int reef_checkpoint_read(reef_ctx* c, const char* path);
+/* The reference's FP32 state file (cfd_state, REEF3D_CFD_STATE/REEF3D-CFD-State-<num>-<rank>.r3d,
+ * cfd_state_filename.cpp:27) of a single-rank box, for Paraview and restart interop.
+ * meta8 = count, printcount, simtime, printtime, sedprinttime, fsfprinttime, probeprinttime,
+ * stateprinttime (the two counts as whole numbers).
+ * reef_cfd_state_write replaces cfd_state::write_result (cfd_state_write.cpp:34-146) for
+ * the context's u, v, w, press and phi; the fields the hot path does not hold are written
+ * as the reference holds them on such a case: topo 1 (no topography), kin, eps, eddyv 0
+ * (laminar), qbe 0 (no sediment), conc 0.
+ * reef_cfd_state_read replaces cfd_state::read (cfd_state_read.cpp:46-192): meta8 and the
+ * loop cells of u, v, w, press, phi as double(float), then the reader's ghost fills (u 10,
+ * v 11, w 12, press 40, phi 52 for F50 = 2). */
+int reef_cfd_state_write(reef_ctx* c, const char* path, const double* meta8);
+int reef_cfd_state_read(reef_ctx* c, const char* path, double* meta8);
+/* the same two on host fields (REEF_LAYOUT_IFAST of an nx x ny x nz box; fields = u, v, w,
+ * press, phi); no device work; errors in reef_case_last_error */
+int reef_cfd_state_write_host(int nx, int ny, int nz, const char* path, const double* const* fields,
+                              const double* meta8);
+int reef_cfd_state_read_host(int nx, int ny, int nz, const char* path, double* const* fields, double* meta8);
 /* the dt of the following steps (the p->dt that fixtimestep::start, ietimestep::start or the
  * caller sets before momentum::start, fixtimestep.cpp:38-114) */
 int reef_set_timestep(reef_ctx* c, double dt);

@@ -65,6 +65,8 @@
 #include"convection_void.h"
 #include"benchmark_vortex3D.h"
 #include"initialize.h"
+#include"cfd_state.h"
+#include"sediment_void.h"
 
 using std::string;
 using std::vector;
@@ -854,6 +856,43 @@ static void op_timestep(Env& e, Rng& r, double amp)
     write_scalars("ts_ini", {p->dt, p->umax, p->vmax, p->wmax, p->N47});
 }
 
+// cfd_state::write (cfd_state.cpp:203-225, which writes only the result file:
+// cfd_state_write.cpp:34-146) on the final tank state, laminar (kepsilon_void) and without
+// sediment (sediment_void); the print counters and times are set to distinct values first
+static void op_state(Env& e)
+{
+    lexer* p = e.p;
+    fdm* a = e.a;
+    p->count = 7;
+    p->printcount = 3;
+    p->printtime = 0.25;
+    p->sedprinttime = 0.5;
+    p->fsfprinttime = 0.75;
+    p->probeprinttime = 1.0;
+    p->stateprinttime = 1.25;
+    dump_field("st_u", a->u);
+    dump_field("st_v", a->v);
+    dump_field("st_w", a->w);
+    dump_field("st_press", a->press);
+    dump_field("st_phi", a->phi);
+    write_scalars("st_meta", {double(p->count), double(p->printcount), p->simtime, p->printtime, p->sedprinttime,
+                              p->fsfprinttime, p->probeprinttime, p->stateprinttime});
+    kepsilon_void turb(p, a, e.pgc);
+    sediment_void sed;
+    {
+        cfd_state st(p, a, e.pgc, 0);
+        st.write(p, a, e.pgc, &turb, &sed);
+    }
+    std::ifstream sf("REEF3D_CFD_STATE/REEF3D-CFD-State-00000000-000001.r3d", std::ios::binary);
+    vector<char> bytes((std::istreambuf_iterator<char>(sf)), std::istreambuf_iterator<char>());
+    if(bytes.empty())
+    {
+        fprintf(stderr, "no state file written\n");
+        exit(1);
+    }
+    dump_bytes("state_file", bytes);
+}
+
 int main(int argc, char** argv)
 {
     if(argc < 3)
@@ -958,6 +997,7 @@ int main(int argc, char** argv)
     else if(cname == "tank2d")
         run_tank(e, 3, 0.35, 0.02, 0.6);
     if(cname == "tank2d") op_timestep(e, r, 1.3);
+    if(cname == "tank" || cname == "tank2d") op_state(e);
     MPI_Finalize();
     return 0;
 }

@@ -467,6 +467,59 @@ int reef_checkpoint_read(reef_ctx* c, const char* path)
     return 0;
 }
 
+// cfd_state::write_result of the context's u, v, w, press, phi (the host writer does the
+// format; the fields come back from the device first)
+int reef_cfd_state_write(reef_ctx* c, const char* path, const double* meta8)
+{
+    if(check_ctx(c) || !path || !meta8) return fail(REEF_E_ARG, "bad argument");
+    Engine& e = *c->b.e;
+    TankHost* t = e.tank(c->tank);
+    if(!t) return fail(REEF_E_ARG, "no tank");
+    const HostGrid& h = t->hg;
+    const size_t n = (size_t)(h.nx + 6) * (h.ny + 6) * (h.nz + 6);
+    std::vector<std::vector<double>> f(5, std::vector<double>(n));
+    const int ids[5] = {REEF_U, REEF_V, REEF_W, REEF_PRESS, REEF_PHI};
+    const double* fp[5];
+    int rc;
+    for(int q = 0; q < 5; ++q)
+    {
+        if((rc = e.get(c->tank, ids[q], f[q].data(), REEF_LAYOUT_IFAST))) return eng(e, rc);
+        fp[q] = f[q].data();
+    }
+    if((rc = reef_cfd_state_write_host(h.nx, h.ny, h.nz, path, fp, meta8)))
+        return fail(rc, reef_case_last_error());
+    return 0;
+}
+
+// cfd_state::read (cfd_state_read.cpp:46-192): the loop cells of u, v, w, press, phi from the
+// file, then the reader's ghost fills: start1 u 10, start2 v 11, start3 w 12, start4 press
+// 40, start4 phi 52 (F50 = 2, ini.cpp:416)
+int reef_cfd_state_read(reef_ctx* c, const char* path, double* meta8)
+{
+    if(check_ctx(c) || !path || !meta8) return fail(REEF_E_ARG, "bad argument");
+    Engine& e = *c->b.e;
+    TankHost* t = e.tank(c->tank);
+    if(!t) return fail(REEF_E_ARG, "no tank");
+    const HostGrid& h = t->hg;
+    const size_t n = (size_t)(h.nx + 6) * (h.ny + 6) * (h.nz + 6);
+    std::vector<std::vector<double>> f(5, std::vector<double>(n));
+    const int ids[5] = {REEF_U, REEF_V, REEF_W, REEF_PRESS, REEF_PHI};
+    const int kind_gcv[5][2] = {{1, 10}, {2, 11}, {3, 12}, {4, 40}, {4, 52}};
+    double* fp[5];
+    int rc;
+    for(int q = 0; q < 5; ++q)
+    {
+        if((rc = e.get(c->tank, ids[q], f[q].data(), REEF_LAYOUT_IFAST))) return eng(e, rc);
+        fp[q] = f[q].data();
+    }
+    if((rc = reef_cfd_state_read_host(h.nx, h.ny, h.nz, path, fp, meta8))) return fail(rc, reef_case_last_error());
+    for(int q = 0; q < 5; ++q)
+        if((rc = e.put(c->tank, ids[q], fp[q], REEF_LAYOUT_IFAST))) return eng(e, rc);
+    for(int q = 0; q < 5; ++q)
+        if((rc = e.ghost(ids[q], kind_gcv[q][0], kind_gcv[q][1]))) return eng(e, rc);
+    return 0;
+}
+
 int reef_timestep(reef_ctx* c, double N47, int ini, double* out4)
 {
     if(check_ctx(c)) return REEF_E_ARG;

@@ -1,10 +1,12 @@
 // REEF3D case files at the boundary: the DIVEMesh grid file and the hot-path keys of
-// ctrl.txt, so a context can be built from a REEF3D case directory.  Host code only.
+// ctrl.txt, so a context can be built from a REEF3D case directory, and the FP32 state
+// file of cfd_state (Paraview and restart interop).  Host code only.
 #include "../../include/reef3d_hip.h"
 
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -20,6 +22,38 @@ int fail(int code, const std::string& m)
 }
 
 constexpr int MARGE = 5;
+
+// The cfd_state result file of a single-rank box (cfd_state_write.cpp:34-146): two ints
+// (count, printcount) and six doubles (simtime, printtime, sedprinttime, fsfprinttime,
+// probeprinttime, stateprinttime), then FP32 values in the reference's loop order, i outer
+// and k inner (IJKLOOP, looping.h:129):
+//   topo over ALOOP; u over ULOOP (i < knox - 1), v over VLOOP (j < knoy - 1), w over WLOOP
+//   (k < knoz - 1) (looping.h:149-151 with ulast = vlast = wlast = 1, lexer::vellast for a
+//   rank without neighbours or periodic sides); press, phi, kin, eps, eddyv over LOOP;
+//   qbe over SLICELOOP4 (i, j); conc over LOOP.
+// A box without solids has every cell in each loop.  Fields are REEF_LAYOUT_IFAST.
+struct StateLayout
+{
+    int nx, ny, nz;
+    long at(int i, int j, int k) const { return (long)(k + 3) * (ny + 6) * (nx + 6) + (long)(j + 3) * (nx + 6) + (i + 3); }
+    long cells() const { return (long)nx * ny * nz; }
+    // values of the staggered loops: u, v, w
+    long ucells(int comp) const
+    {
+        return (long)(nx - (comp == 0)) * (ny - (comp == 1)) * (nz - (comp == 2));
+    }
+    long bytes() const
+    {
+        return 8 + 48 + 4 * (cells() + ucells(0) + ucells(1) + ucells(2) + 5 * cells() + (long)nx * ny + cells());
+    }
+    template<class F>
+    void loop(int di, int dj, int dk, F f) const
+    {
+        for(int i = 0; i < nx - di; ++i)
+            for(int j = 0; j < ny - dj; ++j)
+                for(int k = 0; k < nz - dk; ++k) f(at(i, j, k));
+    }
+};
 
 }  // namespace
 
@@ -214,6 +248,77 @@ int reef_ctrl_read(const char* path, reef_params* prm, double* ctrl4)
         ctrl4[2] = N47;
         ctrl4[3] = N40;
     }
+    return 0;
+}
+
+// cfd_state::write_result (cfd_state_write.cpp:34-146) of a single-rank box domain
+int reef_cfd_state_write_host(int nx, int ny, int nz, const char* path, const double* const* fields,
+                              const double* meta8)
+{
+    if(!path || !fields || !meta8) return fail(REEF_E_ARG, "null argument");
+    for(int q = 0; q < 5; ++q)
+        if(!fields[q]) return fail(REEF_E_ARG, "null field");
+    if(nx < 1 || ny < 1 || nz < 1) return fail(REEF_E_ARG, "bad grid size");
+    const StateLayout L{nx, ny, nz};
+    std::vector<char> out;
+    out.reserve((size_t)L.bytes());
+    auto put = [&](const void* p, size_t n) { out.insert(out.end(), (const char*)p, (const char*)p + n); };
+    const int head[2] = {(int)meta8[0], (int)meta8[1]};
+    put(head, sizeof head);
+    put(meta8 + 2, 6 * sizeof(double));
+    auto putf = [&](float v) { put(&v, sizeof v); };
+    L.loop(0, 0, 0, [&](long) { putf(1.0f); });                                  // topo (no topography)
+    L.loop(1, 0, 0, [&](long c) { putf((float)fields[0][c]); });                 // u
+    L.loop(0, 1, 0, [&](long c) { putf((float)fields[1][c]); });                 // v
+    L.loop(0, 0, 1, [&](long c) { putf((float)fields[2][c]); });                 // w
+    L.loop(0, 0, 0, [&](long c) { putf((float)fields[3][c]); });                 // press
+    L.loop(0, 0, 0, [&](long c) { putf((float)fields[4][c]); });                 // phi
+    for(int q = 0; q < 3; ++q) L.loop(0, 0, 0, [&](long) { putf(0.0f); });      // kin, eps, eddyv (laminar)
+    for(long q = 0; q < (long)nx * ny; ++q) putf(0.0f);                          // qbe (no sediment)
+    L.loop(0, 0, 0, [&](long) { putf(0.0f); });                                  // conc
+    FILE* f = std::fopen(path, "wb");
+    if(!f) return fail(REEF_E_ARG, std::string("cannot write ") + path);
+    const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    if(std::fclose(f) != 0 || !ok) return fail(REEF_E_ARG, std::string("write failed: ") + path);
+    return 0;
+}
+
+// cfd_state::read (cfd_state_read.cpp:46-147) of a single-rank box domain: meta8 and the
+// loop cells of u, v, w, press, phi (double of the FP32 value); other cells are unchanged
+int reef_cfd_state_read_host(int nx, int ny, int nz, const char* path, double* const* fields, double* meta8)
+{
+    if(!path || !fields || !meta8) return fail(REEF_E_ARG, "null argument");
+    for(int q = 0; q < 5; ++q)
+        if(!fields[q]) return fail(REEF_E_ARG, "null field");
+    if(nx < 1 || ny < 1 || nz < 1) return fail(REEF_E_ARG, "bad grid size");
+    const StateLayout L{nx, ny, nz};
+    std::ifstream f(path, std::ios::binary);
+    if(!f) return fail(REEF_E_ARG, std::string("cannot open state file ") + path);
+    std::vector<char> in((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    if((long)in.size() != L.bytes())
+        return fail(REEF_E_ARG, "state file of " + std::to_string(in.size()) + " bytes; a " + std::to_string(nx) + "x" +
+                                    std::to_string(ny) + "x" + std::to_string(nz) + " box needs " +
+                                    std::to_string(L.bytes()));
+    const char* p = in.data();
+    int head[2];
+    std::memcpy(head, p, sizeof head);
+    p += sizeof head;
+    meta8[0] = head[0];
+    meta8[1] = head[1];
+    std::memcpy(meta8 + 2, p, 6 * sizeof(double));
+    p += 6 * sizeof(double);
+    auto getf = [&]() {
+        float v;
+        std::memcpy(&v, p, sizeof v);
+        p += sizeof v;
+        return (double)v;
+    };
+    p += 4 * L.cells();                                                          // topo
+    L.loop(1, 0, 0, [&](long c) { fields[0][c] = getf(); });
+    L.loop(0, 1, 0, [&](long c) { fields[1][c] = getf(); });
+    L.loop(0, 0, 1, [&](long c) { fields[2][c] = getf(); });
+    L.loop(0, 0, 0, [&](long c) { fields[3][c] = getf(); });
+    L.loop(0, 0, 0, [&](long c) { fields[4][c] = getf(); });
     return 0;
 }
 

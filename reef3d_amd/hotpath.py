@@ -87,6 +87,10 @@ def library():
         "reef_set_timestep": ([vp, ctypes.c_double], ctypes.c_int),
         "reef_checkpoint_write": ([vp, ctypes.c_char_p], ctypes.c_int),
         "reef_checkpoint_read": ([vp, ctypes.c_char_p], ctypes.c_int),
+        "reef_cfd_state_write": ([vp, ctypes.c_char_p, dp], ctypes.c_int),
+        "reef_cfd_state_read": ([vp, ctypes.c_char_p, dp], ctypes.c_int),
+        "reef_cfd_state_write_host": ([ctypes.c_int] * 3 + [ctypes.c_char_p, ctypes.POINTER(dp), dp], ctypes.c_int),
+        "reef_cfd_state_read_host": ([ctypes.c_int] * 3 + [ctypes.c_char_p, ctypes.POINTER(dp), dp], ctypes.c_int),
         "reef_phi_rk3": ([vp], ctypes.c_int),
         "reef_sync": ([vp], ctypes.c_int),
         "reef_timing_enable": ([vp, ctypes.c_int], ctypes.c_int),
@@ -174,6 +178,39 @@ def read_ctrl_file(path):
     prm, c4 = Params(), np.zeros(4)
     _case_check(library().reef_ctrl_read(os.fsencode(path), ctypes.byref(prm), _dp(c4)))
     return prm, dict(F49=int(c4[0]), N48=int(c4[1]), N47=float(c4[2]), N40=int(c4[3]))
+
+
+def cfd_state_name(num, rank=1):
+    """File name of cfd_state's result file (cfd_state_filename.cpp:27)."""
+    return os.path.join("REEF3D_CFD_STATE", "REEF3D-CFD-State-%08i-%06i.r3d" % (num, rank))
+
+
+def _state_meta(count, printcount, times):
+    meta = np.zeros(8)
+    meta[0], meta[1] = int(count), int(printcount)
+    meta[2:8] = np.asarray(times, dtype=np.float64).reshape(6)
+    return meta
+
+
+def write_cfd_state_host(path, fields, count=0, printcount=0, times=(0.0,) * 6):
+    """cfd_state::write_result from host fields (reef_cfd_state_write_host): fields = u, v,
+    w, press, phi in the i-fast layout (nz + 6, ny + 6, nx + 6); times = simtime, printtime,
+    sedprinttime, fsfprinttime, probeprinttime, stateprinttime."""
+    fs = [np.ascontiguousarray(f, dtype=np.float64) for f in fields]
+    nz, ny, nx = (d - 6 for d in fs[0].shape)
+    arr = (ctypes.POINTER(ctypes.c_double) * 5)(*[_dp(f) for f in fs])
+    _case_check(library().reef_cfd_state_write_host(nx, ny, nz, os.fsencode(path), arr,
+                                                    _dp(_state_meta(count, printcount, times))))
+
+
+def read_cfd_state_host(path, nx, ny, nz):
+    """cfd_state::read into zeroed host fields (reef_cfd_state_read_host): (u, v, w, press,
+    phi), count, printcount, times."""
+    fs = [np.zeros((nz + 6, ny + 6, nx + 6)) for _ in range(5)]
+    meta = np.zeros(8)
+    arr = (ctypes.POINTER(ctypes.c_double) * 5)(*[_dp(f) for f in fs])
+    _case_check(library().reef_cfd_state_read_host(nx, ny, nz, os.fsencode(path), arr, _dp(meta)))
+    return tuple(fs), int(meta[0]), int(meta[1]), tuple(float(x) for x in meta[2:])
 
 
 def grid_desc(nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21):
@@ -367,6 +404,19 @@ class Domain:
     def restore(self, path):
         """Restores a checkpoint of a domain with the same grid (reef_checkpoint_read)."""
         _check(library().reef_checkpoint_read(self.h, os.fsencode(path)))
+
+    def write_cfd_state(self, path, count=0, printcount=0, times=(0.0,) * 6):
+        """The reference's FP32 state file of u, v, w, press, phi (reef_cfd_state_write);
+        times = simtime, printtime, sedprinttime, fsfprinttime, probeprinttime,
+        stateprinttime."""
+        _check(library().reef_cfd_state_write(self.h, os.fsencode(path), _dp(_state_meta(count, printcount, times))))
+
+    def read_cfd_state(self, path):
+        """Loads a state file (reef_cfd_state_read): u, v, w, press, phi with their ghost
+        layers refilled; returns (count, printcount, times)."""
+        meta = np.zeros(8)
+        _check(library().reef_cfd_state_read(self.h, os.fsencode(path), _dp(meta)))
+        return int(meta[0]), int(meta[1]), tuple(float(x) for x in meta[2:])
 
     def set_timestep(self, dt):
         """dt of the following steps (p->dt)."""

@@ -71,3 +71,62 @@ def test_ctrl_file_refuses_other_schemes(tmp_path):
                       (base + "D 10 5\n", "D 10"), (base + "N 40 2\n", "N 40")):
         with pytest.raises(ReefError, match=key):
             read_ctrl_file(_write(tmp_path, "bad.txt", text.encode()))
+
+
+# ---- the reference's FP32 state file (cfd_state)
+STATE_CASES = ["tank", "tank2d"]
+
+
+def _state_args(d):
+    fields = [d["st_" + n] for n in ("u", "v", "w", "press", "phi")]
+    m = d["st_meta"]
+    return fields, int(m[0]), int(m[1]), tuple(m[2:8])
+
+
+@pytest.mark.parametrize("case", STATE_CASES)
+def test_cfd_state_file_is_the_references(case, tmp_path):
+    """The state file written from the reference's final tank fields and print counters is
+    byte for byte the file cfd_state::write wrote for them in the fixture run."""
+    from reef3d_amd.hotpath import write_cfd_state_host
+    d = load(case)
+    fields, count, printcount, times = _state_args(d)
+    path = str(tmp_path / "state.r3d")
+    write_cfd_state_host(path, fields, count, printcount, times)
+    got = np.fromfile(path, dtype=np.uint8)
+    ref = d["state_file"]
+    assert got.size == ref.size
+    assert np.array_equal(got, ref), f"first difference at byte {int(np.argmax(got != ref))}"
+
+
+@pytest.mark.parametrize("case", STATE_CASES)
+def test_cfd_state_read_of_the_references_file(case, tmp_path):
+    """Reading the reference's state file gives the FP32-rounded fields on the loop cells
+    (ULOOP / VLOOP / WLOOP leave the last staggered row, LOOP every cell) and the counters."""
+    from reef3d_amd.hotpath import read_cfd_state_host
+    d = load(case)
+    c = CASES[case]
+    nx, ny, nz = c["nx"], c["ny"], c["nz"]
+    fields, count, printcount, times = _state_args(d)
+    path = _write(tmp_path, "state.r3d", d["state_file"])
+    got, gc, gp, gt = read_cfd_state_host(path, nx, ny, nz)
+    assert (gc, gp) == (count, printcount) and gt == times
+    drop = ((0, 0, 1), (0, 1, 0), (1, 0, 0), (0, 0, 0), (0, 0, 0))   # (k, j, i) rows left out
+    for q, (dk, dj, di) in enumerate(drop):
+        box = (slice(3, 3 + nz - dk), slice(3, 3 + ny - dj), slice(3, 3 + nx - di))
+        want = fields[q][box].astype(np.float32).astype(np.float64)
+        assert np.array_equal(got[q][box], want), q
+        mask = np.ones(got[q].shape, bool)
+        mask[box] = False
+        assert not got[q][mask].any(), q                              # untouched cells stay zero
+
+
+def test_cfd_state_refusals(tmp_path):
+    from reef3d_amd import ReefError
+    from reef3d_amd.hotpath import read_cfd_state_host
+    d = load("tank")
+    path = _write(tmp_path, "state.r3d", d["state_file"])
+    c = CASES["tank"]
+    with pytest.raises(ReefError, match="needs"):
+        read_cfd_state_host(path, c["nx"] + 1, c["ny"], c["nz"])
+    with pytest.raises(ReefError, match="cannot open"):
+        read_cfd_state_host(str(tmp_path / "missing.r3d"), c["nx"], c["ny"], c["nz"])

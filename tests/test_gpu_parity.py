@@ -363,3 +363,31 @@ def test_checkpoint_restart_is_bit_identical(tmp_path):
     from reef3d_amd import ReefError
     with pytest.raises(ReefError, match="another grid"):
         C.restore(path)
+
+
+@pytest.mark.parametrize("case", ["tank", "tank2d"])
+def test_cfd_state_file_from_device_and_back(case, tmp_path):
+    """reef_cfd_state_write from the device fields writes the file cfd_state::write wrote in
+    the reference's run, byte for byte; reef_cfd_state_read loads it into a fresh context as
+    cfd_state::read does: the FP32-rounded loop cells, then the reader's ghost fills."""
+    from reef3d_amd import Domain as DDomain
+    from reef3d_amd.hotpath import grid_desc, host_ghost_apply, read_cfd_state_host
+    d = load(case)
+    args = domain_args(case, d)
+    names = ("u", "v", "w", "press", "phi")
+    m = d["st_meta"]
+    A = DDomain(**args)
+    for n in names:
+        A.put(n, d["st_" + n])
+    path = str(tmp_path / "state.r3d")
+    A.write_cfd_state(path, int(m[0]), int(m[1]), tuple(m[2:8]))
+    got = np.fromfile(path, dtype=np.uint8)
+    assert np.array_equal(got, d["state_file"])
+    B = DDomain(**args)
+    count, printcount, times = B.read_cfd_state(path)
+    assert (count, printcount, times) == (int(m[0]), int(m[1]), tuple(m[2:8]))
+    loaded = read_cfd_state_host(path, args["nx"], args["ny"], args["nz"])[0]
+    gd, keep = grid_desc(args["nx"], args["ny"], args["nz"], args["XN"], args["YN"], args["ZN"], args["DYM"],
+                         args["DZM"], jdir=args["jdir"], top_group=args["top_group"])
+    for n, f, (kind, gcv) in zip(names, loaded, ((1, 10), (2, 11), (3, 12), (4, 40), (4, 52))):
+        assert bitexact(B.get(n), host_ghost_apply(gd, kind, gcv, f)), n
