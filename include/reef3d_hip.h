@@ -85,10 +85,11 @@ typedef struct reef_grid_file reef_grid_file;
 int reef_grid_read(const char* path, reef_grid_file** file, reef_grid_desc* desc);
 void reef_grid_file_free(reef_grid_file* file);
 /* the hot-path keys of a ctrl.txt (control::read_control, read_control.cpp:28-70) over the
- * defaults of ini.cpp: prm gets W1-W4, W20-W22, F43, F45, N44, N46 and N49 as dt; ctrl4
- * (may be null) gets F49, N48, N47, N40.  Keys choosing another scheme than the hot
- * path's (D10, D20, D30, F30, F35, F40, N10, N40, M10) give REEF_E_UNSUPPORTED. */
-int reef_ctrl_read(const char* path, reef_params* prm, double* ctrl4);
+ * defaults of ini.cpp: prm gets W1-W4, W20-W22, F43, F45, N44, N46 and N49 as dt; ctrl5
+ * (may be null) gets F49, N48, N47, N40, D20.  Keys choosing another scheme than the hot
+ * path's (D10, D20 other than 0 or 1, D30, F30, F35, F40, N10, N40, M10, and D20 = 1 with
+ * N48 = 1) give REEF_E_UNSUPPORTED. */
+int reef_ctrl_read(const char* path, reef_params* prm, double* ctrl5);
 /* message of the last failed reef_grid_read / reef_ctrl_read of this thread */
 const char* reef_case_last_error(void);
 
@@ -106,6 +107,10 @@ int reef_ctx_create(const reef_grid_desc* grid, const reef_params* prm, reef_ctx
  * reef_fc3_step first computes its own dt on the device as reef_timestep does (ietimestep
  * ::start, selected for N48 = 1 with D20 = 0 by driver_logic_cfd.cpp:59-66); "N47": its
  * factor (ini.cpp:593, default 0.3).
+ * "D20": momentum diffusion, 0 = diff_void (default here), 1 = ediff2, the explicit viscous
+ * terms of ediff2::diff_u/v/w (ediff2.cpp:78-190) added to F/G/H after the convection in
+ * every momentum_FC3 predictor (momentum_FC3.cpp:170, 187, 204, 272-306, 370-404), laminar
+ * (eddyv 0); not with N48 = 1, which the reference steps with another time-step class.
  * Other keys: REEF_E_UNSUPPORTED. */
 int reef_set_control(reef_ctx* ctx, const char* key, double value);
 void reef_ctx_destroy(reef_ctx* ctx);

@@ -76,7 +76,7 @@ class Domain:
 
     def __init__(self, nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21,
                  rho=(1000.0, 1.205), visc=(1.0e-6, 1.4938e-5), g=(0.0, 0.0, 0.0),
-                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000, F49=1):
+                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000, F49=1, D20=0):
         self.nx, self.ny, self.nz, self.jdir = nx, ny, nz, jdir
         self.shape = (nz + 6, ny + 6, nx + 6)
         groups = (ctypes.c_int * 7)(0, 21, 21, 21, 21, 21, top_group)
@@ -86,6 +86,8 @@ class Domain:
                                   DYM, DZM, groups, ctypes.byref(self.prm))
         if F49 != 1 and lib().orc_set_control(self.h, b"F49", float(F49)) != 0:
             raise ValueError(f"F49={F49} not supported")
+        if D20 != 0 and lib().orc_set_control(self.h, b"D20", float(D20)) != 0:
+            raise ValueError(f"D20={D20} not supported")
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -39,6 +39,7 @@ struct orc_domain
 {
     int nx, ny, nz, jdir;
     int F49 = 1;                // reinit discretisation: 1 reinidisc_f, 0 reinidisc_fsf
+    int D20 = 0;                // momentum diffusion: 0 diff_void, 1 ediff2 (explicit)
     int dot_ext = 0;            // 1: BiCGStab dot products accumulated in long double
     long sx, sy, sz;
     int group[7];
@@ -999,7 +1000,56 @@ template<class F> void cell_loop(orc_domain* d, F fn)
         fn(d->id(i, j, k));
 }
 
-// momentum_FC3::start (momentum_FC3.cpp:122-430) with D20=0 (diff_void), B10=0,
+// ediff2::diff_u / diff_v / diff_w (ediff2.cpp:78-190): the explicit viscous term of
+// component comp at cell c, added to F after the convection.  Laminar: eddyv = 0, so each
+// visc + eddyv sum of the reference equals visc (adding +0.0 is exact for visc > 0).
+static double ediff_term(const orc_domain* d, int comp, const double* u, const double* v, const double* w,
+                         const double* nu, long c, int i, int j, int k)
+{
+    const long X = 1, Y = d->sx, Z = d->sx * d->sy;
+    const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
+    const double *DXN = d->DN[0].data(), *DYN = d->DN[1].data(), *DZN = d->DN[2].data();
+    const double *DXP = d->DP[0].data(), *DYP = d->DP[1].data(), *DZP = d->DP[2].data();
+    auto V = [&](long o) { return nu[c + o]; };
+    if(comp == 1)
+    {
+        const double uc = u[c];
+        const double ddy_p = (((V(0) + V(X)) + V(Y)) + V(X + Y)) * 0.25;
+        const double ddy_m = (((V(-Y) + V(X - Y)) + V(0)) + V(X)) * 0.25;
+        const double ddz_p = (((V(0) + V(X)) + V(Z)) + V(X + Z)) * 0.25;
+        const double ddz_m = (((V(-Z) + V(X - Z)) + V(0)) + V(X)) * 0.25;
+        return 2.0 * ((u[c + X] - uc) * (V(X) / DXN[ip + 1]) - (uc - u[c - X]) * (V(0) / DXN[ip])) / DXP[ip]
+               + ((u[c + Y] - uc) * (ddy_p / DYP[jp]) - (uc - u[c - Y]) * (ddy_m / DYP[jp - 1])) / DYN[jp]
+               + ((u[c + Z] - uc) * (ddz_p / DZP[kp]) - (uc - u[c - Z]) * (ddz_m / DZP[kp - 1])) / DZN[kp]
+               + ((v[c + X] - v[c]) * (ddy_p / DXP[ip]) - (v[c + X - Y] - v[c - Y]) * (ddy_m / DXP[ip])) / DYN[jp]
+               + ((w[c + X] - w[c]) * (ddz_p / DXP[ip]) - (w[c + X - Z] - w[c - Z]) * (ddz_m / DXP[ip])) / DZN[kp];
+    }
+    if(comp == 2)
+    {
+        const double vc = v[c];
+        const double ddx_p = (((V(0) + V(Y)) + V(X)) + V(X + Y)) * 0.25;
+        const double ddx_m = (((V(-X) + V(Y - X)) + V(0)) + V(Y)) * 0.25;
+        const double ddz_p = (((V(0) + V(Y)) + V(Z)) + V(Y + Z)) * 0.25;
+        const double ddz_m = (((V(-Z) + V(Y - Z)) + V(0)) + V(Y)) * 0.25;
+        return ((v[c + X] - vc) * (ddx_p / DXP[ip]) - (vc - v[c - X]) * (ddx_m / DXP[ip - 1])) / DXN[ip]
+               + 2.0 * ((v[c + Y] - vc) * (V(Y) / DYN[jp + 1]) - (vc - v[c - Y]) * (V(0) / DYN[jp])) / DYP[jp]
+               + ((v[c + Z] - vc) * (ddz_p / DZP[kp]) - (vc - v[c - Z]) * (ddz_m / DZP[kp - 1])) / DZN[kp]
+               + ((u[c + Y] - u[c]) * (ddx_p / DYP[jp]) - (u[c - X + Y] - u[c - X]) * (ddx_m / DYP[jp])) / DXN[ip]
+               + ((w[c + Y] - w[c]) * (ddz_p / DYP[jp]) - (w[c + Y - Z] - w[c - Z]) * (ddz_m / DYP[jp])) / DZN[kp];
+    }
+    const double wc = w[c];
+    const double ddx_p = (((V(0) + V(Z)) + V(X)) + V(X + Z)) * 0.25;
+    const double ddx_m = (((V(-X) + V(Z - X)) + V(0)) + V(Z)) * 0.25;
+    const double ddy_p = (((V(0) + V(Z)) + V(Y)) + V(Y + Z)) * 0.25;
+    const double ddy_m = (((V(-Y) + V(Z - Y)) + V(0)) + V(Z)) * 0.25;
+    return ((w[c + X] - wc) * (ddx_p / DXP[ip]) - (wc - w[c - X]) * (ddx_m / DXP[ip - 1])) / DXN[ip]
+           + ((w[c + Y] - wc) * (ddy_p / DYP[jp]) - (wc - w[c - Y]) * (ddy_m / DYP[jp - 1])) / DYN[jp]
+           + 2.0 * ((w[c + Z] - wc) * (V(Z) / DZN[kp + 1]) - (wc - w[c - Z]) * (V(0) / DZN[kp])) / DZP[kp]
+           + ((u[c + Z] - u[c]) * (ddx_p / DZP[kp]) - (u[c - X + Z] - u[c - X]) * (ddx_m / DZP[kp])) / DXN[ip]
+           + ((v[c + Z] - v[c]) * (ddy_p / DZP[kp]) - (v[c - Y + Z] - v[c - Y]) * (ddy_m / DZP[kp])) / DYN[jp];
+}
+
+// momentum_FC3::start (momentum_FC3.cpp:122-430) with D20=0 (diff_void) or 1 (ediff2), B10=0,
 // laminar (kepsilon_void zeroes F/G/H), void ioflow and forcing, pjm_corr projection.
 // The working field ls is a member of momentum_FC3 whose ghost cells are never filled,
 // so the stage-1 HJ-WENO reads zeros there (momentum_FC3.cpp:132-139).
@@ -1008,6 +1058,7 @@ struct FC3
     orc_domain* d;
     long n;
     std::vector<double> ls, frk1, frk2, L, F, urk1, urk2, vrk1, vrk2, wrk1, wrk2, udiff, vdiff, wdiff, pcorr;
+    const double* nu = nullptr;     // the mixture viscosity (fluid_update) the diffusion reads
 
     explicit FC3(orc_domain* dd) : d(dd), n(dd->sx * dd->sy * dd->sz)
     {
@@ -1031,12 +1082,16 @@ struct FC3
         // irhs/jrhs/krhs (momentum_FC3.cpp:432-472), rhsvec = 0, Fext = 0, W29 = 0
         vel_loop(d, comp, [&](long c, int, int, int) { F[c] += (0.0 + g + 0.0 + 0.0) * 1.0; });
         flux(d, comp, q, uu, vv, ww, F.data());
+        // pdiff->diff_u / diff_v / diff_w (momentum_FC3.cpp:170, 187, 204 and the stages)
+        if(d->D20 == 1)
+            vel_loop(d, comp, [&](long c, int i, int j, int k) { F[c] += ediff_term(d, comp, uu, vv, ww, nu, c, i, j, k); });
     }
 
     int step(double* phi, double* u, double* v, double* w, double* press, double* ro, double* visc, double* vol)
     {
         const double dt = d->prm.dt;
         int iters = 0;
+        nu = visc;
         // ---- stage 1
         cell_loop(d, [&](long c) { L[c] = 0.0; ls[c] = phi[c]; });
         hj(d, ls.data(), u, v, w, L.data());
@@ -1311,6 +1366,11 @@ int orc_set_control(orc_domain* d, const char* key, double value)
     if(std::string(key) == "F49" && (value == 0.0 || value == 1.0))
     {
         d->F49 = (int)value;
+        return 0;
+    }
+    if(std::string(key) == "D20" && (value == 0.0 || value == 1.0))
+    {
+        d->D20 = (int)value;
         return 0;
     }
     if(std::string(key) == "dot_ext" && (value == 0.0 || value == 1.0))

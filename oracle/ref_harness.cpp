@@ -54,6 +54,7 @@
 #include"momentum_FC3.h"
 #include"momentum_RK3.h"
 #include"diff_void.h"
+#include"ediff2.h"
 #include"kepsilon_void.h"
 #include"heat_void.h"
 #include"concentration_void.h"
@@ -151,6 +152,8 @@ struct Case
     double W22;
     double N44;
     int N46;
+    double W2 = 1.0e-6, W4 = 1.4938e-5;   // kinematic viscosity of water and air
+    int D20 = 0;                          // 0 diff_void, 1 ediff2 (explicit diffusion)
 };
 
 static double node_coord(int i, int n, double x0, double d, int nonuni)
@@ -238,9 +241,9 @@ static void write_ctrl(const Case& c)
     fprintf(f, "A 10 6\nM 10 1\n");
     fprintf(f, "N 10 1\nN 40 %d\nN 44 %.17g\nN 46 %d\nN 48 0\nN 49 %.17g\n", c.N40, c.N44, c.N46, c.dt);
     fprintf(f, "N 45 1000000\nP 12 1000000\n");
-    fprintf(f, "D 10 4\nD 20 0\nD 30 1\n");
+    fprintf(f, "D 10 4\nD 20 %d\nD 30 1\n", c.D20);
     fprintf(f, "F 30 3\nF 35 5\nF 40 3\nF 44 3\nF 49 1\n");
-    fprintf(f, "W 1 1000.0\nW 2 1.0e-6\nW 3 1.205\nW 4 1.4938e-5\nW 22 %.17g\n", c.W22);
+    fprintf(f, "W 1 1000.0\nW 2 %.17g\nW 3 1.205\nW 4 %.17g\nW 22 %.17g\n", c.W2, c.W4, c.W22);
     fclose(f);
 }
 
@@ -798,6 +801,8 @@ static void run_tank(Env& e, int nsteps, double depth, double amp, double wlen)
     weno_flux_nug conv(p);
     weno_hj_df_nug fsfdisc(p);
     diff_void dvoid;
+    ediff2 ediff(p);
+    diffusion* pdiff = p->D20 == 1 ? (diffusion*)&ediff : (diffusion*)&dvoid;
     poisson_pcorr pois(p, ph, pc);
     pjm_corr press(p, a, pgc, ph, pc);
     kepsilon_void turb(p, a, pgc);
@@ -805,7 +810,7 @@ static void run_tank(Env& e, int nsteps, double depth, double amp, double wlen)
     p->F49 = G_f49;
     reini_RK3 ri(p, 1);
     fsi_void fsi(p, pgc);
-    momentum_FC3 mom(p, a, pgc, &conv, &fsfdisc, &dvoid, &press, &pois, &turb, &solv, &solv, e.pflow, ph, pc, &ri, &fsi);
+    momentum_FC3 mom(p, a, pgc, &conv, &fsfdisc, pdiff, &press, &pois, &turb, &solv, &solv, e.pflow, ph, pc, &ri, &fsi);
     vrans_v vr(p, pgc);
     sixdof_void sd(p, pgc);
     sd.initialize(p, a, pgc);
@@ -936,12 +941,18 @@ int main(int argc, char** argv)
         c.N40 = 13;
         c.dt = 0.5 / (16.0 * 2.0);
     }
-    else if(cname == "tank" || cname == "tank2d" || cname == "tankfsf")
+    else if(cname == "tank" || cname == "tank2d" || cname == "tankfsf" || cname == "tankvisc")
     {
         c.nx = cname == "tank2d" ? 24 : 16;
         c.ny = cname == "tank2d" ? 1 : 6;
         c.jdir = cname == "tank2d" ? 0 : 1;
         if(cname == "tankfsf") G_f49 = 0;
+        if(cname == "tankvisc")
+        {
+            // explicit diffusion (ediff2) with a viscosity large enough to matter in 3 steps
+            c.D20 = 1;
+            c.W2 = c.W4 = 0.01;
+        }
         c.nz = 12;
         c.dx = c.dy = c.dz = 0.05;
         c.topgroup = 3;
@@ -992,7 +1003,7 @@ int main(int argc, char** argv)
     }
     else if(cname == "vortex")
         run_vortex(e, 4);
-    else if(cname == "tank" || cname == "tankfsf")
+    else if(cname == "tank" || cname == "tankfsf" || cname == "tankvisc")
         run_tank(e, 3, 0.35, 0.02, 0.8);
     else if(cname == "tank2d")
         run_tank(e, 3, 0.35, 0.02, 0.6);

@@ -73,8 +73,9 @@ void bl_hj_stage(const TankSet& b, int stage, int phi_s, int phi_n, int u, int v
 void bl_reini_stage(const TankSet& b, int stage, int fs, int f0, int out, double F43, int fsf,
                     hipStream_t s);
 void bl_flux_acc(const TankSet& b, int ipol, int q, int u, int v, int w, int acc, hipStream_t s);
+// diff = 1: with the explicit diffusion of ediff2 (D20 = 1)
 void bl_momentum(const TankSet& b, int comp, int stage, int qin, int qold, int u, int v, int w,
-                 double grav, double cfac, int out, double rw, double ra, hipStream_t s);
+                 double grav, double cfac, int out, double rw, double ra, int diff, hipStream_t s);
 void bl_heaviside(const TankSet& b, const double* prm4, hipStream_t s);
 void bl_poisson_rhs(const TankSet& b, int u, int v, int w, double afac, hipStream_t s);
 void bl_face_rho(const TankSet& b, double rw, double ra, int with_aii, hipStream_t s);

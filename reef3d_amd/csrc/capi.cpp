@@ -88,9 +88,21 @@ int control(Engine& e, const char* key, double value)
         e.graphs = value != 0.0;
         return 0;
     }
+    if(std::strcmp(key, "D20") == 0)
+    {
+        if(value != 0.0 && value != 1.0)
+            return fail(REEF_E_UNSUPPORTED, "D20 must be 0 (diff_void) or 1 (ediff2); 2 (idiff2_FS) is not implemented");
+        if(value == 1.0 && e.n48)
+            return fail(REEF_E_UNSUPPORTED, "D20 = 1 with N48 = 1: the reference steps it with another time-step class");
+        e.drop_graphs();
+        e.d20 = value != 0.0;
+        return 0;
+    }
     if(std::strcmp(key, "N48") == 0)
     {
         if(value != 0.0 && value != 1.0) return fail(REEF_E_UNSUPPORTED, "N48 must be 0 (fixed) or 1 (adaptive)");
+        if(value == 1.0 && e.d20)
+            return fail(REEF_E_UNSUPPORTED, "N48 = 1 with D20 = 1: the reference steps it with another time-step class");
         e.drop_graphs();
         e.n48 = value != 0.0;
         return 0;

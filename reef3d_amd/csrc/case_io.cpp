@@ -155,7 +155,7 @@ void reef_grid_file_free(reef_grid_file* file) { delete file; }
 // The hot-path keys of ctrl.txt (control::read_control, read_control.cpp:28-70: a key
 // letter, its number and its value; '/' starts a comment), over the defaults of ini.cpp.
 // Keys that choose another scheme than the hot path's are refused.
-int reef_ctrl_read(const char* path, reef_params* prm, double* ctrl4)
+int reef_ctrl_read(const char* path, reef_params* prm, double* ctrl5)
 {
     if(!path || !prm) return fail(REEF_E_ARG, "null argument");
     std::ifstream f(path);
@@ -231,22 +231,28 @@ int reef_ctrl_read(const char* path, reef_params* prm, double* ctrl4)
     };
     const Need need[] = {{"D 10 (convection: 4, weno_flux_nug)", D10, 4}, {"D 30 (pressure: 1, pjm_corr)", D30, 1},
                          {"F 30 (level set: 3)", F30, 3},           {"F 35 (level-set convection: 5)", F35, 5},
-                         {"F 40 (reinit: 3, reini_RK3)", F40, 3},   {"N 10 (solver: 1, bicgstab_ijk)", N10, 1},
-                         {"D 20 (diffusion: 0)", D20, 0}};
+                         {"F 40 (reinit: 3, reini_RK3)", F40, 3},   {"N 10 (solver: 1, bicgstab_ijk)", N10, 1}};
     for(const Need& n : need)
         if(n.have != n.want)
             return fail(REEF_E_UNSUPPORTED, std::string("ctrl.txt key outside the hot path: ") + n.key);
+    if(D20 != 0 && D20 != 1)
+        return fail(REEF_E_UNSUPPORTED, "ctrl.txt key outside the hot path: D 20 (diffusion: 0 or 1; 2, idiff2_FS, "
+                                        "is not implemented)");
+    if(D20 == 1 && N48 == 1)
+        return fail(REEF_E_UNSUPPORTED, "ctrl.txt D 20 1 with N 48 1: the reference steps it with another "
+                                        "time-step class");
     if(N40 != 3 && N40 != 13) return fail(REEF_E_UNSUPPORTED, "ctrl.txt N 40 must be 3 (momentum_FC3) or 13");
     if(M10 > 1) return fail(REEF_E_UNSUPPORTED, "ctrl.txt M 10: one rank");
     if(F49 != 0 && F49 != 1) return fail(REEF_E_UNSUPPORTED, "ctrl.txt F 49 must be 0 or 1");
     if(N48 != 0 && N48 != 1) return fail(REEF_E_UNSUPPORTED, "ctrl.txt N 48 must be 0 or 1");
     *prm = p;
-    if(ctrl4)
+    if(ctrl5)
     {
-        ctrl4[0] = F49;
-        ctrl4[1] = N48;
-        ctrl4[2] = N47;
-        ctrl4[3] = N40;
+        ctrl5[0] = F49;
+        ctrl5[1] = N48;
+        ctrl5[2] = N47;
+        ctrl5[3] = N40;
+        ctrl5[4] = D20;
     }
     return 0;
 }

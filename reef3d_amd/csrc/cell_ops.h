@@ -541,6 +541,55 @@ __device__ __forceinline__ double momentum_F(const DevGrid& g, int comp, const d
     return F;
 }
 
+// ediff2::diff_u / diff_v / diff_w (ediff2.cpp:78-190): the explicit viscous term of
+// component comp at cell c, which momentum_FC3 adds to F after the convection (D20 = 1).
+// Laminar: eddyv = 0, and visc + 0.0 is visc exactly, so the reference's visc + eddyv sums
+// are the visc sums in the same order.
+__device__ __forceinline__ double ediff_term(const DevGrid& g, int comp, const double* __restrict__ u,
+                                             const double* __restrict__ v, const double* __restrict__ w,
+                                             const double* __restrict__ nu, long c, int i, int j, int k)
+{
+    const long X = 1, Y = g.sx, Z = g.sxy;
+    const int ip = i + MARGE, jp = j + MARGE, kp = k + MARGE;
+    auto V = [&](long o) { return nu[c + o]; };
+    if(comp == 1)
+    {
+        const double uc = u[c];
+        const double ddy_p = (((V(0) + V(X)) + V(Y)) + V(X + Y)) * 0.25;
+        const double ddy_m = (((V(-Y) + V(X - Y)) + V(0)) + V(X)) * 0.25;
+        const double ddz_p = (((V(0) + V(X)) + V(Z)) + V(X + Z)) * 0.25;
+        const double ddz_m = (((V(-Z) + V(X - Z)) + V(0)) + V(X)) * 0.25;
+        return 2.0 * ((u[c + X] - uc) * (V(X) / g.DXN[ip + 1]) - (uc - u[c - X]) * (V(0) / g.DXN[ip])) / g.DXP[ip]
+               + ((u[c + Y] - uc) * (ddy_p / g.DYP[jp]) - (uc - u[c - Y]) * (ddy_m / g.DYP[jp - 1])) / g.DYN[jp]
+               + ((u[c + Z] - uc) * (ddz_p / g.DZP[kp]) - (uc - u[c - Z]) * (ddz_m / g.DZP[kp - 1])) / g.DZN[kp]
+               + ((v[c + X] - v[c]) * (ddy_p / g.DXP[ip]) - (v[c + X - Y] - v[c - Y]) * (ddy_m / g.DXP[ip])) / g.DYN[jp]
+               + ((w[c + X] - w[c]) * (ddz_p / g.DXP[ip]) - (w[c + X - Z] - w[c - Z]) * (ddz_m / g.DXP[ip])) / g.DZN[kp];
+    }
+    if(comp == 2)
+    {
+        const double vc = v[c];
+        const double ddx_p = (((V(0) + V(Y)) + V(X)) + V(X + Y)) * 0.25;
+        const double ddx_m = (((V(-X) + V(Y - X)) + V(0)) + V(Y)) * 0.25;
+        const double ddz_p = (((V(0) + V(Y)) + V(Z)) + V(Y + Z)) * 0.25;
+        const double ddz_m = (((V(-Z) + V(Y - Z)) + V(0)) + V(Y)) * 0.25;
+        return ((v[c + X] - vc) * (ddx_p / g.DXP[ip]) - (vc - v[c - X]) * (ddx_m / g.DXP[ip - 1])) / g.DXN[ip]
+               + 2.0 * ((v[c + Y] - vc) * (V(Y) / g.DYN[jp + 1]) - (vc - v[c - Y]) * (V(0) / g.DYN[jp])) / g.DYP[jp]
+               + ((v[c + Z] - vc) * (ddz_p / g.DZP[kp]) - (vc - v[c - Z]) * (ddz_m / g.DZP[kp - 1])) / g.DZN[kp]
+               + ((u[c + Y] - u[c]) * (ddx_p / g.DYP[jp]) - (u[c - X + Y] - u[c - X]) * (ddx_m / g.DYP[jp])) / g.DXN[ip]
+               + ((w[c + Y] - w[c]) * (ddz_p / g.DYP[jp]) - (w[c + Y - Z] - w[c - Z]) * (ddz_m / g.DYP[jp])) / g.DZN[kp];
+    }
+    const double wc = w[c];
+    const double ddx_p = (((V(0) + V(Z)) + V(X)) + V(X + Z)) * 0.25;
+    const double ddx_m = (((V(-X) + V(Z - X)) + V(0)) + V(Z)) * 0.25;
+    const double ddy_p = (((V(0) + V(Z)) + V(Y)) + V(Y + Z)) * 0.25;
+    const double ddy_m = (((V(-Y) + V(Z - Y)) + V(0)) + V(Z)) * 0.25;
+    return ((w[c + X] - wc) * (ddx_p / g.DXP[ip]) - (wc - w[c - X]) * (ddx_m / g.DXP[ip - 1])) / g.DXN[ip]
+           + ((w[c + Y] - wc) * (ddy_p / g.DYP[jp]) - (wc - w[c - Y]) * (ddy_m / g.DYP[jp - 1])) / g.DYN[jp]
+           + 2.0 * ((w[c + Z] - wc) * (V(Z) / g.DZN[kp + 1]) - (wc - w[c - Z]) * (V(0) / g.DZN[kp])) / g.DZP[kp]
+           + ((u[c + Z] - u[c]) * (ddx_p / g.DZP[kp]) - (u[c - X + Z] - u[c - X]) * (ddx_m / g.DZP[kp])) / g.DXN[ip]
+           + ((v[c + Z] - v[c]) * (ddy_p / g.DZP[kp]) - (v[c - Y + Z] - v[c - Y]) * (ddy_m / g.DZP[kp])) / g.DYN[jp];
+}
+
 // smoothed Heaviside of fluid_update_fsf (fluid_update_fsf.cpp:52-62)
 __device__ __forceinline__ double heaviside_H(double p, double psi)
 {

@@ -975,7 +975,7 @@ int Engine::momentum_stage(int stage, const int qin[3], const int qold[3], const
         tbegin("momentum");
         each([&](const TankSet& S) {
             bl_momentum(S, comp, stage, qin[comp - 1], qold[comp - 1], vel[0], vel[1], vel[2], grav[comp - 1], cfac,
-                        out[comp - 1], t0->prm.W1, t0->prm.W3, stream);
+                        out[comp - 1], t0->prm.W1, t0->prm.W3, d20, stream);
         });
         tend();
     }

@@ -95,6 +95,8 @@ public:
     // with the factor N47) computed on the device; 0: the fixed dt of each tank
     int n48 = 0;
     double n47 = 0.3;
+    // D20 = 1: the momentum predictors add ediff2's explicit viscous terms; 0: diff_void
+    int d20 = 0;
     double delta = 0.1;             // reliable-update threshold of the mixed solve
     // replay fc3_step's launch sequences (the four segments between its pressure solves)
     // and the BiCGStab iteration batches as hipGraphs, captured on first use; the graphs

@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(NT, 4) kb_flux_acc(TankSet b, int ipol, int q_
 
 // velocity predictor of one component (momentum_FC3.cpp:160-208 and the later stages);
 // cdt = cfac * dt of the tank
-template<int comp>
+template<int comp, bool DIFF>
 __global__ void __launch_bounds__(NT, 2) kb_momentum(TankSet b, int stage, int qin_, int qold_, int u_, int v_, int w_,
                                                    double grav, double cfac, int out_, double rw, double ra)
 {
@@ -454,7 +454,8 @@ __global__ void __launch_bounds__(NT, 2) kb_momentum(TankSet b, int stage, int q
             int i, j, k;
             long c;
             if(!tile_cell_row(g, tw, t, i, j, k, c) || !comp_cell(g, comp, i, j, k)) continue;
-            const double F = momentum_F(g, comp, qin, u, v, w, press, phi, grav, rw, ra, c, i, j, k);
+            double F = momentum_F(g, comp, qin, u, v, w, press, phi, grav, rw, ra, c, i, j, k);
+            if constexpr(DIFF) F += ediff_term(g, comp, u, v, w, T.f[REEF_VISC], c, i, j, k);
             const double df = qin[c];
             if(stage == 1)
                 out[c] = df + cdt * 1.0 * F;
@@ -1991,14 +1992,17 @@ void bl_flux_acc(const TankSet& b, int ipol, int q, int u, int v, int w, int acc
     hipLaunchKernelGGL(kb_flux_acc, tgrid(b), dim3(NT), 0, s, b, ipol, q, u, v, w, acc);
 }
 void bl_momentum(const TankSet& b, int comp, int stage, int qin, int qold, int u, int v, int w, double grav,
-                 double cfac, int out, double rw, double ra, hipStream_t s)
+                 double cfac, int out, double rw, double ra, int diff, hipStream_t s)
 {
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, tgrid(b), dim3(NT), 0, s, b, stage, qin, qold, u, v, w, grav, cfac, out, rw, ra);
+    };
     if(comp == 1)
-        hipLaunchKernelGGL(kb_momentum<1>, tgrid(b), dim3(NT), 0, s, b, stage, qin, qold, u, v, w, grav, cfac, out, rw, ra);
+        diff ? go(kb_momentum<1, true>) : go(kb_momentum<1, false>);
     else if(comp == 2)
-        hipLaunchKernelGGL(kb_momentum<2>, tgrid(b), dim3(NT), 0, s, b, stage, qin, qold, u, v, w, grav, cfac, out, rw, ra);
+        diff ? go(kb_momentum<2, true>) : go(kb_momentum<2, false>);
     else
-        hipLaunchKernelGGL(kb_momentum<3>, tgrid(b), dim3(NT), 0, s, b, stage, qin, qold, u, v, w, grav, cfac, out, rw, ra);
+        diff ? go(kb_momentum<3, true>) : go(kb_momentum<3, false>);
 }
 void bl_heaviside(const TankSet& b, const double* prm4, hipStream_t s)
 {

@@ -174,10 +174,10 @@ def read_grid_file(path):
 
 
 def read_ctrl_file(path):
-    """The hot-path keys of a ctrl.txt (reef_ctrl_read): (Params, dict F49, N48, N47, N40)."""
-    prm, c4 = Params(), np.zeros(4)
-    _case_check(library().reef_ctrl_read(os.fsencode(path), ctypes.byref(prm), _dp(c4)))
-    return prm, dict(F49=int(c4[0]), N48=int(c4[1]), N47=float(c4[2]), N40=int(c4[3]))
+    """The hot-path keys of a ctrl.txt (reef_ctrl_read): (Params, dict F49, N48, N47, N40, D20)."""
+    prm, c5 = Params(), np.zeros(5)
+    _case_check(library().reef_ctrl_read(os.fsencode(path), ctypes.byref(prm), _dp(c5)))
+    return prm, dict(F49=int(c5[0]), N48=int(c5[1]), N47=float(c5[2]), N40=int(c5[3]), D20=int(c5[4]))
 
 
 def cfd_state_name(num, rank=1):
@@ -249,7 +249,7 @@ class Domain:
 
     def __init__(self, nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21,
                  rho=(1000.0, 1.205), visc=(1.0e-6, 1.4938e-5), g=(0.0, 0.0, 0.0),
-                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000, F49=1):
+                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000, F49=1, D20=0):
         L = library()
         self.nx, self.ny, self.nz, self.jdir = nx, ny, nz, jdir
         self.shape = (nz + 6, ny + 6, nx + 6)
@@ -265,6 +265,8 @@ class Domain:
         self.h = h
         if F49 != 1:
             self.set_control("F49", F49)
+        if D20 != 0:
+            self.set_control("D20", D20)
 
     @classmethod
     def from_case(cls, case_dir):
@@ -277,7 +279,7 @@ class Domain:
         D = cls(g["nx"], g["ny"], g["nz"], g["XN"], g["YN"], g["ZN"], g["DYM"], g["DZM"], jdir=g["jdir"],
                 top_group=g["group"][6], rho=(prm.W1, prm.W3), visc=(prm.W2, prm.W4),
                 g=(prm.W20, prm.W21, prm.W22), F43=prm.F43, F45=prm.F45, dt=prm.dt, N44=prm.N44, N46=prm.N46,
-                F49=ctrl["F49"])
+                F49=ctrl["F49"], D20=ctrl["D20"])
         if ctrl["N48"]:
             D.set_control("N47", ctrl["N47"])
             D.set_control("N48", 1)

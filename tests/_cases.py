@@ -14,6 +14,9 @@ CASES = {
     "tank2d": dict(nx=24, ny=1, nz=12, jdir=0, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14),
     # the tank with reinidisc_fsf (F49 = 0)
     "tankfsf": dict(nx=16, ny=6, nz=12, jdir=1, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14, F49=0),
+    # the tank with explicit diffusion (D20 = 1, ediff2) and viscosity 0.01 in both fluids
+    "tankvisc": dict(nx=16, ny=6, nz=12, jdir=1, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14,
+                     visc=(0.01, 0.01), D20=1),
 }
 
 
@@ -26,7 +29,7 @@ def domain_args(case, d):
     gs = d["grid_scalars"]
     return dict(nx=c["nx"], ny=c["ny"], nz=c["nz"], XN=d["XN"], YN=d["YN"], ZN=d["ZN"], DYM=gs[1], DZM=gs[2],
                 jdir=c["jdir"], top_group=c["top_group"], g=c["g"], dt=c["dt"], N44=c["N44"], N46=5000,
-                F49=c.get("F49", 1))
+                F49=c.get("F49", 1), visc=c.get("visc", (1.0e-6, 1.4938e-5)), D20=c.get("D20", 0))
 
 
 def inner(a):

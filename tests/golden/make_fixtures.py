@@ -15,7 +15,7 @@ import tempfile
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-CASES = ["box", "nonuni", "vortex", "tank", "tank2d", "tankfsf"]
+CASES = ["box", "nonuni", "vortex", "tank", "tank2d", "tankfsf", "tankvisc"]
 
 
 def main(cases):

@@ -73,6 +73,19 @@ def test_ctrl_file_refuses_other_schemes(tmp_path):
             read_ctrl_file(_write(tmp_path, "bad.txt", text.encode()))
 
 
+def test_ctrl_file_explicit_diffusion(tmp_path):
+    """D 20 1 (ediff2) is read from the fixture run's ctrl.txt with its viscosities; it is
+    refused together with N 48 1, and D 20 2 (idiff2_FS) is refused."""
+    from reef3d_amd import ReefError
+    from reef3d_amd.hotpath import read_ctrl_file
+    prm, ctrl = read_ctrl_file(_write(tmp_path, "ctrl.txt", load("tankvisc")["ctrlfile"]))
+    assert ctrl["D20"] == 1 and (prm.W2, prm.W4) == (0.01, 0.01)
+    assert read_ctrl_file(_write(tmp_path, "c0.txt", load("tank")["ctrlfile"]))[1]["D20"] == 0
+    for text in ("N 10 1\nN 40 3\nD 20 1\nN 48 1\n", "N 10 1\nN 40 3\nD 20 2\n"):
+        with pytest.raises(ReefError, match="D 20"):
+            read_ctrl_file(_write(tmp_path, "bad.txt", text.encode()))
+
+
 # ---- the reference's FP32 state file (cfd_state)
 STATE_CASES = ["tank", "tank2d"]
 
