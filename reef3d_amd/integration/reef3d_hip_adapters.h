@@ -95,6 +95,9 @@ inline reef_ctx* context(lexer* p)
         reef_grid_desc d = grid_from_lexer(p);
         reef_params r = params_from_lexer(p);
         check(reef_ctx_create(&d, &r, &ctx), "reef_ctx_create");
+        // the scheme controls the case chose (reini_RK3.cpp:68-72, driver_logic_cfd.cpp:332-335)
+        check(reef_set_control(ctx, "F49", p->F49 == 0 ? 0.0 : 1.0), "F49");
+        if(p->D20 == 1) check(reef_set_control(ctx, "D20", 1.0), "D20");
     }
     return ctx;
 }
