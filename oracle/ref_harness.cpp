@@ -941,13 +941,15 @@ int main(int argc, char** argv)
         c.N40 = 13;
         c.dt = 0.5 / (16.0 * 2.0);
     }
-    else if(cname == "tank" || cname == "tank2d" || cname == "tankfsf" || cname == "tankvisc")
+    else if(cname == "tank" || cname == "tank2d" || cname == "tankfsf" || cname == "tankvisc" ||
+            cname == "tank2dvisc")
     {
-        c.nx = cname == "tank2d" ? 24 : 16;
-        c.ny = cname == "tank2d" ? 1 : 6;
-        c.jdir = cname == "tank2d" ? 0 : 1;
+        const bool two = cname == "tank2d" || cname == "tank2dvisc";
+        c.nx = two ? 24 : 16;
+        c.ny = two ? 1 : 6;
+        c.jdir = two ? 0 : 1;
         if(cname == "tankfsf") G_f49 = 0;
-        if(cname == "tankvisc")
+        if(cname == "tankvisc" || cname == "tank2dvisc")
         {
             // explicit diffusion (ediff2) with a viscosity large enough to matter in 3 steps
             c.D20 = 1;
@@ -1005,7 +1007,7 @@ int main(int argc, char** argv)
         run_vortex(e, 4);
     else if(cname == "tank" || cname == "tankfsf" || cname == "tankvisc")
         run_tank(e, 3, 0.35, 0.02, 0.8);
-    else if(cname == "tank2d")
+    else if(cname == "tank2d" || cname == "tank2dvisc")
         run_tank(e, 3, 0.35, 0.02, 0.6);
     if(cname == "tank2d") op_timestep(e, r, 1.3);
     if(cname == "tank" || cname == "tank2d") op_state(e);

@@ -17,6 +17,8 @@ CASES = {
     # the tank with explicit diffusion (D20 = 1, ediff2) and viscosity 0.01 in both fluids
     "tankvisc": dict(nx=16, ny=6, nz=12, jdir=1, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14,
                      visc=(0.01, 0.01), D20=1),
+    "tank2dvisc": dict(nx=24, ny=1, nz=12, jdir=0, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14,
+                       visc=(0.01, 0.01), D20=1),
 }
 
 

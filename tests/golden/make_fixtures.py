@@ -15,7 +15,7 @@ import tempfile
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-CASES = ["box", "nonuni", "vortex", "tank", "tank2d", "tankfsf", "tankvisc"]
+CASES = ["box", "nonuni", "vortex", "tank", "tank2d", "tankfsf", "tankvisc", "tank2dvisc"]
 
 
 def main(cases):

@@ -140,7 +140,7 @@ def test_bicgstab_first_iterations_match_reference_path():
 SOLVED_RTOL = 1e-8
 
 
-@pytest.mark.parametrize("case", ["tank", "tank2d", "tankfsf", "tankvisc"])
+@pytest.mark.parametrize("case", ["tank", "tank2d", "tankfsf", "tankvisc", "tank2dvisc"])
 def test_fc3_step_matches_reference(case):
     d = load(case)
     D = dev(case, d)

@@ -130,7 +130,7 @@ def test_bicgstab_iteration_path(it):
     assert same(inner(x), inner(d[f"bs_x_it{it}"]))
 
 
-@pytest.mark.parametrize("case", ["tank", "tank2d", "tankfsf", "tankvisc"])
+@pytest.mark.parametrize("case", ["tank", "tank2d", "tankfsf", "tankvisc", "tank2dvisc"])
 def test_momentum_fc3_steps(case):
     d = load(case)
     O = oracle(case, d)
