@@ -245,7 +245,7 @@ int reef_batch_put(reef_batch* b, int tank, int field_id, const double* host, in
 int reef_batch_get(reef_batch* b, int tank, int field_id, double* host, int layout);
 int reef_batch_ghost_fill(reef_batch* b, int field_id, int kind, int gcv);
 int reef_batch_fluid_update(reef_batch* b);
-/* the keys of reef_set_control ("F49", "graphs", "N48", "N47") for every tank of the batch */
+/* the keys of reef_set_control ("F49", "graphs", "N48", "N47", "D20") for every tank of the batch */
 int reef_batch_control(reef_batch* b, const char* key, double value);
 int reef_batch_solver_config(reef_batch* b, int precision, double delta);
 /* Tile plans per size bucket (no counterpart in the reference).  A bucket is (jdir,
