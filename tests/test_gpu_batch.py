@@ -47,7 +47,7 @@ def _prepare(B):
     B.fluid_update()
 
 
-def _run(tanks, steps, precision, graphs=False, every_step=False):
+def _run(tanks, steps, precision, graphs=False, every_step=False, d20=0):
     from reef3d_amd import Batch
     B = Batch()
     ids = [B.add(**kw) for kw, _ in tanks]
@@ -55,6 +55,7 @@ def _run(tanks, steps, precision, graphs=False, every_step=False):
         _initial(B, tid, kw, f)
     B.solver_config(precision)
     B.set_control("graphs", 1 if graphs else 0)
+    B.set_control("D20", d20)
     _prepare(B)
     iters, history = [], []
     for _ in range(steps):
@@ -75,6 +76,18 @@ def test_batch_results_identical_to_alone(precision):
         for n in FIELDS:
             assert np.array_equal(mixed[q][n], alone[0][n]), f"tank {q} {n} differs between batch and alone"
         assert [s[q] for s in it_mixed] == [s[0] for s in it_alone]
+
+
+def test_batch_with_explicit_diffusion_identical_to_alone():
+    """D20 = 1 (ediff2) on a batch: each tank's steps are the bits of the tank run alone,
+    and the control reaches the batched launches (the results differ from D20 = 0)."""
+    mixed, _ = _run(TANKS, 2, "fp64", graphs=True, d20=1)
+    plain, _ = _run(TANKS, 2, "fp64", graphs=True)
+    for q, t in enumerate(TANKS):
+        alone, _ = _run([t], 2, "fp64", d20=1)
+        for n in FIELDS:
+            assert np.array_equal(mixed[q][n], alone[0][n]), f"tank {q} {n} differs between batch and alone"
+        assert not np.array_equal(mixed[q]["u"], plain[q]["u"]), f"tank {q}: D20 = 1 changed nothing"
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
