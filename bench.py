@@ -146,13 +146,13 @@ def timed_steps(D, steps):
     D.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        it, _ = D.momentum_fc3()
+        it, vol = D.momentum_fc3()
         iters += it
     D.sync()
     elapsed = time.perf_counter() - t0
     fams = {name: D.timing_get(name) for name in TIMED_FAMILIES}
     D.timing(False)
-    return elapsed, iters, fams
+    return elapsed, iters, fams, vol
 
 
 # algorithmic bytes per interior cell, averaged over the three stages of one launch family
@@ -276,32 +276,14 @@ def vortex(args):
     print(json.dumps(out), flush=True)
 
 
-def c3_specs(n=64, seed=1003):
-    """Seeded 2-D tank set of the batched workload."""
-    rng = np.random.default_rng(seed)
-    specs = []
-    for _ in range(n):
-        nx = int(rng.choice(np.arange(256, 2049, 128)))
-        nz = int(rng.choice(np.arange(64, 257, 32)))
-        dx = float(rng.uniform(0.01, 0.05))
-        depth = 0.6 * nz * dx
-        amp = float(rng.uniform(0.01, 0.1)) * depth
-        lam = float(rng.uniform(1.0, 4.0))
-        specs.append(dict(nx=nx, nz=nz, dx=dx, depth=depth, amp=amp, wavelength=lam))
-    return specs
-
-
 def tank_batch(specs, precision, graphs=True, rtol=1e-10):
     from reef3d_amd import Batch
-    from reef3d_amd.cases import wave_tank
+    from reef3d_amd.cases import tank_fields, tank_kwargs
     B = Batch()
     ids = []
     for sp in specs:
-        f = wave_tank(sp["nx"], 1, sp["nz"], sp["dx"], sp["depth"], sp["amp"], sp["wavelength"])
-        umax = max(float(np.abs(f["u"]).max()), float(np.abs(f["w"]).max()))
-        dt = 0.25 * sp["dx"] / max(math.sqrt(9.81 * sp["depth"]), umax)
-        tid = B.add(nx=sp["nx"], ny=1, nz=sp["nz"], XN=f["XN"], YN=f["YN"], ZN=f["ZN"], DYM=sp["dx"], DZM=sp["dx"],
-                    jdir=0, top_group=3, g=(0.0, 0.0, -9.81), dt=dt, N44=-rtol, N46=20000)
+        f = tank_fields(sp)
+        tid = B.add(**tank_kwargs(sp, f, rtol))
         for n in ("phi", "u", "v", "w", "press"):
             B.put(tid, n, f[n])
         ids.append(tid)
@@ -321,6 +303,7 @@ def tanks(args):
     """Batched 2-D tanks: lockstep steps replayed from hipGraphs, the same steps with
     iteration-level scheduling, and the same tanks one at a time with the plain FP64
     drop-in."""
+    from reef3d_amd.cases import c3_specs
     specs = c3_specs()
     cells = sum(sp["nx"] * sp["nz"] for sp in specs)
     graphs = not args.no_graphs
@@ -439,40 +422,16 @@ def tanks(args):
     print(json.dumps(out), flush=True)
 
 
-def c4_requests(n, step_div, seed=1004, rate=200.0):
-    """Seeded request stream of the served workload."""
-    rng = np.random.default_rng(seed)
-    t, reqs = 0.0, []
-    for _ in range(n):
-        t += float(rng.exponential(1.0 / rate))
-        if rng.random() < 0.7:
-            nx, ny, nz, jdir = int(rng.choice(np.arange(256, 2049, 128))), 1, int(rng.choice(np.arange(64, 257, 32))), 0
-        else:
-            nx, ny, nz, jdir = (int(rng.choice(np.arange(128, 257, 32))), int(rng.choice(np.arange(32, 65, 8))),
-                                int(rng.choice(np.arange(64, 129, 16))), 1)
-        dx = float(rng.uniform(0.01, 0.05))
-        depth = 0.6 * nz * dx
-        amp = float(rng.uniform(0.01, 0.1)) * depth
-        lam = float(rng.uniform(1.0, 4.0))
-        steps = int(rng.integers(50, 501))
-        reqs.append(dict(arrival=t, nx=nx, ny=ny, nz=nz, jdir=jdir, dx=dx, depth=depth, amp=amp, wavelength=lam,
-                         steps=max(1, steps // step_div), steps_full=steps))
-    return reqs
-
-
-def request_fields(r):
-    from reef3d_amd.cases import wave_tank
-    f = wave_tank(r["nx"], r["ny"], r["nz"], r["dx"], r["depth"], r["amp"], r["wavelength"])
-    umax = max(float(np.abs(f["u"]).max()), float(np.abs(f["w"]).max()))
-    f["dt"] = 0.25 * r["dx"] / max(math.sqrt(9.81 * r["depth"]), umax)
-    return f
-
-
 def serve(args):
     """Served request stream versus the same requests solved one at a time in FP64."""
+    print(json.dumps(run_serve(args)), flush=True)
+
+
+def run_serve(args):
     from reef3d_amd import Batch, Server
+    from reef3d_amd.cases import c4_requests, tank_fields
     reqs = c4_requests(args.requests, args.step_div)
-    fields = [request_fields(r) for r in reqs]
+    fields = [tank_fields(r) for r in reqs]
     # tile plans of the served tanks: 2-D chunks of --cpt tiles (default 4, the fastest in a
     # sweep of 2, 4 and 8 on this stream) per size bucket, or with --tune the autotuner's
     # plans for the size buckets of the stream (tuned once before serving, each bucket alone)
@@ -571,11 +530,15 @@ def serve(args):
             "service_s_total": round(float(sum(service)), 3)},
         "speedup_vs_sequential": round((len(reqs) / makespan) / (len(reqs) / seq_span), 3) if service else None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def solves(args, cfg):
     """Standalone FP64 vs mixed-precision solves on the wave tank's step-0 operator."""
+    print(json.dumps(run_solves(args, cfg)), flush=True)
+
+
+def run_solves(args, cfg):
     D, _ = make_domain(cfg)
     D.solver_config("fp64", args.delta)   # the reliable-update threshold of the mixed solves
     # both solvers replay their 8-iteration batches as hipGraphs (the same for both)
@@ -637,7 +600,7 @@ def solves(args, cfg):
         "max_abs_p_minus_p64": float(a[:, 7].max()),
         "max_rel_p_minus_p64": float((a[:, 7] / a[:, 8]).max()),
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -647,6 +610,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison leg")
+    ap.add_argument("--no-solves", action="store_true", help="skip the standalone-solve (mixed vs FP64) leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--workload", choices=("fc3", "solves", "vortex", "tanks", "serve"), default="fc3")
     ap.add_argument("--requests", type=int, default=16, help="requests of --workload serve")
     ap.add_argument("--step-div", type=int, default=10, help="--workload serve: steps per request divided by this")
@@ -685,9 +650,11 @@ def main():
         return
     D, _ = make_domain(cfg)
     cells = cfg["nx"] * cfg["ny"] * cfg["nz"]
+    vol_init = D.fluid_update()
     for _ in range(args.warmup):
         D.momentum_fc3()
-    elapsed, iters, fams = timed_steps(D, args.steps)
+    vol_start = D.fluid_update()
+    elapsed, iters, fams, vol_end = timed_steps(D, args.steps)
     dom = max(("bicg_p", "bicg_v", "bicg_t", "bicg_x"), key=lambda k: fams[k][0])
     ms, n = fams[dom]
     avg_s = ms / 1e3 / max(n, 1)
@@ -731,6 +698,13 @@ def main():
         },
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in fams.items() if v[1]},
         "stencil_rooflines": stencil_rooflines(fams, cells),
+        # enclosed water volume (volume1 of fluid_update_fsf) of the tank: at set-up, before
+        # and after the timed steps
+        "interface_volume": {"initial": float(vol_init[0]), "before_timed": float(vol_start[0]),
+                             "after_timed": float(vol_end[0]),
+                             "rel_change_timed": float(vol_end[0] / vol_start[0] - 1.0),
+                             "rel_change_total": float(vol_end[0] / vol_init[0] - 1.0),
+                             "steps_total": args.warmup + args.steps},
     }
     if not args.no_mixed:
         del D
@@ -738,7 +712,7 @@ def main():
         Dm.solver_config("mixed")
         for _ in range(args.warmup):
             Dm.momentum_fc3()
-        el_m, it_m, fams_m = timed_steps(Dm, args.steps)
+        el_m, it_m, fams_m, vol_m = timed_steps(Dm, args.steps)
         _, _, rel = Dm.solver_last()
         dm = max(("mix_v", "bicg32_t", "bicg32_x"), key=lambda k: fams_m[k][0])
         ms_m, n_m = fams_m[dm]
@@ -749,11 +723,25 @@ def main():
             "speedup_vs_fp64": round(elapsed / el_m, 3),
             "iters_per_step": it_m / args.steps,
             "last_solve_rel_residual": rel,
+            "interface_volume_rel_change_total": float(vol_m[0] / vol_init[0] - 1.0),
             "dominant_kernel": dm,
             "dominant_achieved_GBs": round(SOLVER_BYTES[dm] * cells / avg_m / 1e9, 1),
             "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in fams_m.items() if v[1]},
         }
         del Dm
+    if not args.no_solves:
+        # C2: standalone solves on the step-0 operator, FP64 against mixed precision
+        sv = run_solves(args, cfg)
+        out["mixed_solves"] = {k: sv[k] for k in ("solves", "fp64", "mixed", "speedup_mixed_vs_fp64",
+                                                  "reliable_update_delta", "max_rel_p_minus_p64")}
+        out["mixed_solves"]["workload"] = sv["config"]["workload"]
+        out["mixed_solves"]["graphs"] = sv["config"]["graphs"]
+    if not args.no_serve:
+        # C4: a bounded sample of the served stream against sequential plain FP64 solves
+        sr = run_serve(args)
+        out["served"] = {"solves_per_s": sr["value"], "latency_s": sr["latency_s"], "makespan_s": sr["makespan_s"],
+                         "sequential_fp64": sr["sequential_fp64"], "speedup_vs_sequential": sr["speedup_vs_sequential"],
+                         "tank_steps": sr["tank_steps"], "workload": sr["config"]["workload"]}
     if rank == 0 and not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, iters / args.steps)
     if rank == 0:

@@ -133,6 +133,12 @@ int reef_convection(reef_ctx* ctx, int ipol, int q_id, int u_id, int v_id, int w
  * p->count > 0: iters TVD-RK3 pseudo steps of reinidisc_f, in place, ghosts refilled. */
 int reef_reini(reef_ctx* ctx, int phi_id, int iters);
 
+/* reini::start with p->count == 0, the initial reinitialisation of driver_ini_cfd
+ * (reini_RK3.cpp:88-101): start4(phi, gcv), then iters TVD-RK3 pseudo steps (the caller
+ * passes 2*int(p->maxlength/(p->F43*p->DXM))) with the ghost rule gcv = gcval_iniphi
+ * (50; 52 when F50 = 2 and F60 or F62 is set, reini_RK3.cpp:44-57) after every stage. */
+int reef_reini_ini(reef_ctx* ctx, int phi_id, int iters, int gcv);
+
 /* fluid_update_fsf::start (fluid_update_fsf.cpp:39-104): REEF_RO/REEF_VISC from
  * REEF_PHI, ghosts refilled; vol2 (may be NULL) receives volume1, volume2. */
 int reef_fluid_update(reef_ctx* ctx, double* vol2);
@@ -157,6 +163,22 @@ int reef_poisson_solve(reef_ctx* ctx, int x_id, int rhs_id, double stop, int max
  * rel_residual: final ||b - A x||_2 / ||b - A x0||_2 in FP64.  x_id, rhs_id must differ. */
 int reef_poisson_solve_mixed(reef_ctx* ctx, int x_id, int rhs_id, double stop, int maxiter,
                              int* iters, int* updates, double* rel_residual);
+
+/* solver::start(p, a, pgc, f, rhsvec, var) (solver.h:42) of bicgstab_ijk, or of
+ * bicgstab_ijk_2D on a 2-D context (driver_logic_cfd.cpp:464-468), on the caller's operator
+ * (bicgstab_ijk.cpp:51-123, bicgstab_ijk_2D.cpp:49-94): M7 = {M.p, M.n, M.s, M.w, M.e, M.t,
+ * M.b} of matrix_diag and rhs, each reef_solver_veclen(ctx, var) doubles in the reference's
+ * vec order (the FLEXLOOP cells of var, i outermost and k innermost; var 1, 2, 3 leave out
+ * the last i, j, k row of the u, v, w grid, var 4, 5 take every cell).  Field x_id holds
+ * the initial guess (fillxvec) and receives the solution on those cells (finalize); its
+ * other cells are not touched.  stop: the caller's p->N43 (var 1..3) or p->N44 (var 4, 5)
+ * on ||r||_2 / cells (negative: |stop| * ||b - A x0||_2 / cells); maxiter: p->N46; iters
+ * receives p->solveriter, residual the final ||r||_2 / cells.  The operator's arrays are
+ * allocated on the device at the first call. */
+int reef_solver_start(reef_ctx* ctx, int var, const double* const* M7, const double* rhs, int x_id, double stop,
+                      int maxiter, int* iters, double* residual);
+/* entries of the vec of var (1..5) on the context's grid, -1 on a bad argument */
+long reef_solver_veclen(reef_ctx* ctx, int var);
 
 /* ||rhs - A x||_2 in FP64 with the operator assembled from REEF_PHI: res_calc
  * (bicgstab_ijk.cpp:314-338) without its 1/N. */

@@ -64,6 +64,8 @@ int reef_server_release(reef_server* s, long id);
 /* queued and live requests, scheduler rounds run, device bytes held by live tanks; returns the
  * worker's error once it has stopped on one */
 int reef_server_stats(reef_server* s, int* queued, int* live, long* steps, double* bytes_in_use);
+/* the largest device byte count the live tanks held at once since the server was created */
+int reef_server_peak_bytes(reef_server* s, double* peak);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,7 @@ def lib():
         L.orc_hj.argtypes = [ctypes.c_void_p] + [dp] * 5
         L.orc_flux.argtypes = [ctypes.c_void_p, ctypes.c_int] + [dp] * 5
         L.orc_reini.argtypes = [ctypes.c_void_p, dp, ctypes.c_int]
+        L.orc_reini_gcv.argtypes = [ctypes.c_void_p, dp, ctypes.c_int, ctypes.c_int]
         L.orc_fluid_update.argtypes = [ctypes.c_void_p] + [dp] * 4
         L.orc_poisson.argtypes = [ctypes.c_void_p] + [dp] * 4
         L.orc_set_control.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double]
@@ -59,6 +60,13 @@ def lib():
         L.orc_pjm.restype = ctypes.c_int
         L.orc_fc3_step.argtypes = [ctypes.c_void_p] + [dp] * 8
         L.orc_fc3_step.restype = ctypes.c_int
+        L.orc_bicgstab_var.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp, dp, ctypes.c_double, ctypes.c_int, dp]
+        L.orc_bicgstab_var.restype = ctypes.c_int
+        L.orc_veclen.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_veclen.restype = ctypes.c_long
+        L.orc_idiff_assemble.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double] + [dp] * 9
+        L.orc_idiff.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double] + [dp] * 8
+        L.orc_idiff.restype = ctypes.c_int
         L.orc_vortex_step.argtypes = [ctypes.c_void_p] + [dp] * 6 + [ctypes.c_double, dp]
         L.orc_phi_rk3.argtypes = [ctypes.c_void_p] + [dp] * 4
         L.orc_timestep.argtypes = [ctypes.c_void_p] + [dp] * 3 + [ctypes.c_double, ctypes.c_int, dp]
@@ -76,7 +84,7 @@ class Domain:
 
     def __init__(self, nx, ny, nz, XN, YN, ZN, DYM, DZM, jdir=1, top_group=21,
                  rho=(1000.0, 1.205), visc=(1.0e-6, 1.4938e-5), g=(0.0, 0.0, 0.0),
-                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000, F49=1, D20=0):
+                 F43=0.55, F45=2.1, dt=0.01, N44=1e-10, N46=5000, F49=1, D20=0, N43=None):
         self.nx, self.ny, self.nz, self.jdir = nx, ny, nz, jdir
         self.shape = (nz + 6, ny + 6, nx + 6)
         groups = (ctypes.c_int * 7)(0, 21, 21, 21, 21, 21, top_group)
@@ -88,6 +96,8 @@ class Domain:
             raise ValueError(f"F49={F49} not supported")
         if D20 != 0 and lib().orc_set_control(self.h, b"D20", float(D20)) != 0:
             raise ValueError(f"D20={D20} not supported")
+        if N43 is not None and lib().orc_set_control(self.h, b"N43", float(N43)) != 0:
+            raise ValueError(f"N43={N43} not supported")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -123,8 +133,8 @@ class Domain:
     def flux(self, ipol, q, u, v, w, acc):
         lib().orc_flux(self.h, ipol, _p(q), _p(u), _p(v), _p(w), _p(acc))
 
-    def reini(self, phi, iters):
-        lib().orc_reini(self.h, _p(phi), iters)
+    def reini(self, phi, iters, gcv=52):
+        lib().orc_reini_gcv(self.h, _p(phi), iters, gcv)
 
     def fluid_update(self, phi, ro, visc):
         vol = np.zeros(2)
@@ -142,6 +152,34 @@ class Domain:
         M = np.ascontiguousarray(M)
         it = lib().orc_bicgstab(self.h, _p(x), _p(M), _p(rhs), stop, maxiter, _p(res))
         return it, res[0]
+
+    def bicgstab_var(self, var, x, M, rhs, stop, maxiter):
+        """bicgstab_ijk::start with var 1..5 on M (7 rows of veclen(var)) and rhs (vec order)."""
+        res = np.zeros(1)
+        M = np.ascontiguousarray(M, dtype=np.float64)
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        it = lib().orc_bicgstab_var(self.h, int(var), _p(x), _p(M), _p(rhs), stop, maxiter, _p(res))
+        return it, res[0]
+
+    def idiff_assemble(self, comp, alpha, vin, u, v, w, visc, au=None, av=None):
+        """idiff2_FS(_2D) diff_u/v/w operator of component comp: (M 7 x n, rhs n)."""
+        n = self.veclen(comp)
+        M, rhs = np.zeros(7 * n), np.zeros(n)
+        au = u if au is None else au
+        av = v if av is None else av
+        lib().orc_idiff_assemble(self.h, int(comp), float(alpha), _p(vin), _p(u), _p(v), _p(w), _p(au), _p(av),
+                                 _p(visc), _p(M), _p(rhs))
+        return M.reshape(7, n), rhs
+
+    def idiff(self, comp, alpha, diff, vin, u, v, w, visc, au=None, av=None):
+        """the whole diff_u/v/w call into diff; returns the iterations"""
+        au = u if au is None else au
+        av = v if av is None else av
+        return lib().orc_idiff(self.h, int(comp), float(alpha), _p(diff), _p(vin), _p(u), _p(v), _p(w), _p(au), _p(av),
+                               _p(visc))
+
+    def veclen(self, var):
+        return lib().orc_veclen(self.h, int(var))
 
     def set_control(self, key, value):
         if lib().orc_set_control(self.h, key.encode(), float(value)) != 0:

@@ -39,7 +39,14 @@ struct orc_domain
 {
     int nx, ny, nz, jdir;
     int F49 = 1;                // reinit discretisation: 1 reinidisc_f, 0 reinidisc_fsf
-    int D20 = 0;                // momentum diffusion: 0 diff_void, 1 ediff2 (explicit)
+    int D20 = 0;                // momentum diffusion: 0 diff_void, 1 ediff2, 2 idiff2_FS (implicit)
+    double N43 = 1.0e-5;        // stop criterion of the diffusion solves (ini.cpp:589)
+    int df_set = 0;             // direct-forcing flags DF1..3 set (after the first forcing call)
+    // private x of the solver of the diffusion solves (psolv, one bicgstab_ijk object for
+    // var 1..3, driver_logic_cfd.cpp:464-468): fillxvec writes only the FLEXLOOP cells of
+    // each call, so the other cells keep what earlier calls of other vars left there and
+    // the matvec of a restart reads them (bicgstab_ijk.cpp:280-294, 366-378)
+    std::vector<double> psolv_x;
     int dot_ext = 0;            // 1: BiCGStab dot products accumulated in long double
     long sx, sy, sz;
     int group[7];
@@ -623,8 +630,10 @@ void reini_disc(orc_domain* d, const double* f, double* L)
     }
 }
 
-// reini_RK3::start/time_preproc (reini_RK3.cpp:77-160) for count>0 (gcval_phi = 52)
-void reini(orc_domain* d, double* f, int iters)
+// reini_RK3::start/time_preproc (reini_RK3.cpp:77-160): for count>0 gcv = gcval_phi (52 for
+// F50 = 2); the initial reinitialisation at count==0 runs with gcv = gcval_iniphi (50) after a
+// start4(f, gcv) of its own (reini_RK3.cpp:93-101), which the caller does
+void reini(orc_domain* d, double* f, int iters, int gcv = 52)
 {
     long n = d->sx * d->sy * d->sz;
     std::vector<double> L(n, 0.0), frk1(n, 0.0), frk2(n, 0.0), dtf(n, 0.0);
@@ -646,13 +655,13 @@ void reini(orc_domain* d, double* f, int iters)
     {
         reini_disc(d, f, L.data());
         interior([&](long c) { frk1[c] = f[c] + dtf[c] * L[c]; });
-        ghost(d, frk1.data(), 4, 52);
+        ghost(d, frk1.data(), 4, gcv);
         reini_disc(d, frk1.data(), L.data());
         interior([&](long c) { frk2[c] = 0.75 * f[c] + 0.25 * frk1[c] + 0.25 * dtf[c] * L[c]; });
-        ghost(d, frk2.data(), 4, 52);
+        ghost(d, frk2.data(), 4, gcv);
         reini_disc(d, frk2.data(), L.data());
         interior([&](long c) { f[c] = (1.0 / 3.0) * f[c] + (2.0 / 3.0) * frk2[c] + (2.0 / 3.0) * dtf[c] * L[c]; });
-        ghost(d, f, 4, 52);
+        ghost(d, f, 4, gcv);
     }
 }
 
@@ -740,22 +749,29 @@ void poisson(orc_domain* d, const double* phi, const double* press, double* M, d
     }
 }
 
-// bicgstab_ijk::solve and helpers (bicgstab_ijk.cpp:125-395), var 5: FLEXLOOP over flag4
+// bicgstab_ijk::solve and helpers (bicgstab_ijk.cpp:125-395), FLEXLOOP of var
+// (bicgstab_ijk.cpp:55-85, looping.h:55-57,159): var 1..3 over flag1..3 with ulast / vlast /
+// wlast, 4 and 5 over flag4
 struct Bicg
 {
     orc_domain* d;
     const double* M;
-    long N;
+    long N;                       // entries of the vec (matrix_diag rows)
+    int var = 5;
     std::vector<double> rhs, x, rj, r0, pj, vj, sj, tj, ph, sh, aii;
 
     template<class F> void loop(F fn)
     {
+        const int il = var == 1 ? d->last[1] : 0, jl = var == 2 ? d->last[2] : 0, kl = var == 3 ? d->last[3] : 0;
+        const std::vector<int>& fl = d->flag[var <= 3 ? var : 4];
         long n = 0;
-        for(int i = 0; i < d->nx; ++i)
-        for(int j = 0; j < d->ny; ++j)
-        for(int k = 0; k < d->nz; ++k)
+        for(int i = 0; i < d->nx - il; ++i)
+        for(int j = 0; j < d->ny - jl; ++j)
+        for(int k = 0; k < d->nz - kl; ++k)
         {
-            fn(d->id(i, j, k), n);
+            const long c = d->id(i, j, k);
+            if(fl[c] <= 0) continue;
+            fn(c, n);
             ++n;
         }
     }
@@ -878,18 +894,39 @@ struct Bicg
     }
 };
 
-int bicgstab(orc_domain* d, double* xf, const double* M, const double* rhsvec, double stop, int maxiter, double* res)
+// entries of the vec of var: its FLEXLOOP cells
+long veclen(orc_domain* d, int var)
+{
+    Bicg b;
+    b.d = d;
+    b.var = var;
+    long n = 0;
+    b.loop([&](long, long) { ++n; });
+    return n;
+}
+
+int bicgstab(orc_domain* d, double* xf, const double* M, const double* rhsvec, double stop, int maxiter, double* res,
+             int var = 5)
 {
     long n = d->sx * d->sy * d->sz;
     Bicg b;
     b.d = d;
     b.M = M;
-    b.N = d->cellnum;
+    b.var = var;
+    b.N = veclen(d, var);
     for(auto* v : {&b.rhs, &b.x, &b.rj, &b.r0, &b.pj, &b.vj, &b.sj, &b.tj, &b.ph, &b.sh, &b.aii})
         v->assign(n, 0.0);
+    // var 1..3: the persistent private x of the diffusion solver (var 4, 5 cover every
+    // interior cell, so the pressure solver's stale cells are only its ghosts, always 0)
+    if(var <= 3)
+    {
+        if(d->psolv_x.size() != (size_t)n) d->psolv_x.assign(n, 0.0);
+        b.x.swap(d->psolv_x);
+    }
     b.loop([&](long c, long k) { b.x[c] = xf[c]; b.rhs[c] = rhsvec[k]; });
     int it = b.solve(stop, maxiter, res);
     b.loop([&](long c, long) { xf[c] = b.x[c]; });
+    if(var <= 3) b.x.swap(d->psolv_x);
     return it;
 }
 
@@ -1049,6 +1086,150 @@ static double ediff_term(const orc_domain* d, int comp, const double* u, const d
            + ((v[c + Z] - v[c]) * (ddy_p / DZP[kp]) - (v[c - Y + Z] - v[c - Y]) * (ddy_m / DZP[kp])) / DYN[jp];
 }
 
+// idiff2_FS::diff_u / diff_v / diff_w (idiff2_FSu.cpp:40-278, idiff2_FSv.cpp:29-265,
+// idiff2_FSw.cpp:29-265) and idiff2_FS_2D::diff_u / diff_w (idiff2_FSu_2D.cpp:40-218,
+// idiff2_FSw_2D.cpp:29-207): the implicit viscous operator of component comp over its
+// staggered loop, in vec order, into M (7 rows) and rhs (which the caller zeroed, as irhs
+// leaves it).  Laminar: eddyv = 0, so each visc + eddyv of the reference equals visc;
+// no porosity: CPOR = 1; D22 = 1 (ini.cpp:393): a row whose neighbour lies outside the
+// component's grid moves that coupling times the cell's own value to the right-hand side,
+// but only once the direct-forcing flags are set (DF1..3 > 0 on the staggered loops after
+// the first ghostcell::solid_forcing_flag_update, momentum_forcing.cpp:102; they are 0
+// before, so the rows stay unfolded).  diff_w reads the fdm's u and v (au, av) at k+1 and
+// the argument u, v elsewhere (idiff2_FSw.cpp:94-95).
+void idiff_assemble(orc_domain* d, int comp, double alpha, const double* vin, const double* u, const double* v,
+                    const double* w, const double* au, const double* av, const double* nu, double* M, double* rhs)
+{
+    const long N = veclen(d, comp);
+    double *Mp = M, *Mn = M + N, *Ms = M + 2 * N, *Mw = M + 3 * N, *Me = M + 4 * N, *Mt = M + 5 * N, *Mb = M + 6 * N;
+    const long X = 1, Y = d->sx, Z = d->sx * d->sy;
+    const double *DXN = d->DN[0].data(), *DYN = d->DN[1].data(), *DZN = d->DN[2].data();
+    const double *DXP = d->DP[0].data(), *DYP = d->DP[1].data(), *DZP = d->DP[2].data();
+    const double dt = d->prm.dt;
+    const double CPOR = 1.0;
+    const bool two = d->jdir == 0;
+    const int ie = d->nx - (comp == 1 ? d->last[1] : 0), je = d->ny - (comp == 2 ? d->last[2] : 0),
+              ke = d->nz - (comp == 3 ? d->last[3] : 0);
+    auto outside = [&](int i, int j, int k) { return i < 0 || i >= ie || j < 0 || j >= je || k < 0 || k >= ke; };
+    const double* own = comp == 1 ? u : comp == 2 ? v : w;
+    long n = 0;
+    vel_loop(d, comp, [&](long c, int i, int j, int k) {
+        const int IP = i + MARGE, JP = j + MARGE, KP = k + MARGE;
+        auto V = [&](long o) { return nu[c + o]; };
+        if(comp == 1)
+        {
+            const double ddy_p = 0.25 * (((V(0) + V(X)) + V(Y)) + V(X + Y));
+            const double ddy_m = 0.25 * (((V(-Y) + V(X - Y)) + V(0)) + V(X));
+            const double ddz_p = 0.25 * (((V(0) + V(X)) + V(Z)) + V(X + Z));
+            const double ddz_m = 0.25 * (((V(-Z) + V(X - Z)) + V(0)) + V(X));
+            if(!two)
+            {
+                Mp[n] = 2.0 * V(X) / (DXN[IP + 1] * DXP[IP]) + 2.0 * V(0) / (DXN[IP] * DXP[IP])
+                      + ddy_p / (DYP[JP] * DYN[JP]) + ddy_m / (DYP[JP - 1] * DYN[JP]) + ddz_p / (DZP[KP] * DZN[KP])
+                      + ddz_m / (DZP[KP - 1] * DZN[KP]) + CPOR / (alpha * dt);
+                rhs[n] += ((v[c + X] - v[c]) * ddy_p - (v[c + X - Y] - v[c - Y]) * ddy_m) / (DXP[IP] * DYN[JP])
+                        + ((w[c + X] - w[c]) * ddz_p - (w[c + X - Z] - w[c - Z]) * ddz_m) / (DXP[IP] * DZN[KP])
+                        + (CPOR * vin[c]) / (alpha * dt);
+                Me[n] = -ddy_m / (DYP[JP - 1] * DYN[JP]);
+                Mw[n] = -ddy_p / (DYP[JP] * DYN[JP]);
+            }
+            else
+            {
+                Mp[n] = 2.0 * V(X) / (DXN[IP + 1] * DXP[IP]) + 2.0 * V(0) / (DXN[IP] * DXP[IP])
+                      + ddz_p / (DZP[KP] * DZN[KP]) + ddz_m / (DZP[KP - 1] * DZN[KP]) + CPOR / (alpha * dt);
+                rhs[n] += ((w[c + X] - w[c]) * ddz_p - (w[c + X - Z] - w[c - Z]) * ddz_m) / (DXP[IP] * DZN[KP])
+                        + (CPOR * vin[c]) / (alpha * dt);
+            }
+            Ms[n] = -2.0 * V(0) / (DXN[IP] * DXP[IP]);
+            Mn[n] = -2.0 * V(X) / (DXN[IP + 1] * DXP[IP]);
+            Mb[n] = -ddz_m / (DZP[KP - 1] * DZN[KP]);
+            Mt[n] = -ddz_p / (DZP[KP] * DZN[KP]);
+        }
+        else if(comp == 2)
+        {
+            const double ddx_p = 0.25 * (((V(0) + V(Y)) + V(X)) + V(X + Y));
+            const double ddx_m = 0.25 * (((V(-X) + V(Y - X)) + V(0)) + V(Y));
+            const double ddz_p = 0.25 * (((V(0) + V(Y)) + V(Z)) + V(Y + Z));
+            const double ddz_m = 0.25 * (((V(-Z) + V(Y - Z)) + V(0)) + V(Y));
+            Mp[n] = 2.0 * V(Y) / (DYN[JP + 1] * DYP[JP]) + 2.0 * V(0) / (DYN[JP] * DYP[JP])
+                  + ddx_p / (DXP[IP] * DXN[IP]) + ddx_m / (DXP[IP - 1] * DXN[IP]) + ddz_p / (DZP[KP] * DZN[KP])
+                  + ddz_m / (DZP[KP - 1] * DZN[KP]) + CPOR / (alpha * dt);
+            rhs[n] += ((u[c + Y] - u[c]) * ddx_p - (u[c - X + Y] - u[c - X]) * ddx_m) / (DYP[JP] * DXN[IP])
+                    + ((w[c + Y] - w[c]) * ddz_p - (w[c + Y - Z] - w[c - Z]) * ddz_m) / (DYP[JP] * DZN[KP])
+                    + (CPOR * vin[c]) / (alpha * dt);
+            Ms[n] = -ddx_m / (DXP[IP - 1] * DXN[IP]);
+            Mn[n] = -ddx_p / (DXP[IP] * DXN[IP]);
+            Me[n] = -2.0 * V(0) / (DYN[JP] * DYP[JP]);
+            Mw[n] = -2.0 * V(Y) / (DYN[JP + 1] * DYP[JP]);
+            Mb[n] = -ddz_m / (DZP[KP - 1] * DZN[KP]);
+            Mt[n] = -ddz_p / (DZP[KP] * DZN[KP]);
+        }
+        else
+        {
+            const double ddx_p = 0.25 * (((V(0) + V(Z)) + V(X)) + V(X + Z));
+            const double ddx_m = 0.25 * (((V(-X) + V(Z - X)) + V(0)) + V(Z));
+            if(!two)
+            {
+                const double ddy_p = 0.25 * (((V(0) + V(Z)) + V(Y)) + V(Y + Z));
+                const double ddy_m = 0.25 * (((V(-Y) + V(Z - Y)) + V(0)) + V(Z));
+                Mp[n] = 2.0 * V(Z) / (DZN[KP + 1] * DZP[KP]) + 2.0 * V(0) / (DZN[KP] * DZP[KP])
+                      + ddx_p / (DXP[IP] * DXN[IP]) + ddx_m / (DXP[IP - 1] * DXN[IP]) + ddy_p / (DYP[JP] * DYN[JP])
+                      + ddy_m / (DYP[JP - 1] * DYN[JP]) + CPOR / (alpha * dt);
+                rhs[n] += ((au[c + Z] - u[c]) * ddx_p - (u[c - X + Z] - u[c - X]) * ddx_m) / (DZP[KP] * DXN[IP])
+                        + ((av[c + Z] - v[c]) * ddy_p - (v[c - Y + Z] - v[c - Y]) * ddy_m) / (DZP[KP] * DYN[JP])
+                        + (CPOR * vin[c]) / (alpha * dt);
+                Me[n] = -ddy_m / (DYP[JP - 1] * DYN[JP]);
+                Mw[n] = -ddy_p / (DYP[JP] * DYN[JP]);
+            }
+            else
+            {
+                Mp[n] = 2.0 * V(Z) / (DZN[KP + 1] * DZP[KP]) + 2.0 * V(0) / (DZN[KP] * DZP[KP])
+                      + ddx_p / (DXP[IP] * DXN[IP]) + ddx_m / (DXP[IP - 1] * DXN[IP]) + CPOR / (alpha * dt);
+                rhs[n] += ((u[c + Z] - u[c]) * ddx_p - (u[c - X + Z] - u[c - X]) * ddx_m) / (DZP[KP] * DXN[IP])
+                        + (CPOR * vin[c]) / (alpha * dt);
+            }
+            Ms[n] = -ddx_m / (DXP[IP - 1] * DXN[IP]);
+            Mn[n] = -ddx_p / (DXP[IP] * DXN[IP]);
+            Mb[n] = -2.0 * V(0) / (DZN[KP] * DZP[KP]);
+            Mt[n] = -2.0 * V(Z) / (DZN[KP + 1] * DZP[KP]);
+        }
+        // D22 = 1 wall folding (idiff2_FSu.cpp:120-170 and the v, w, 2-D forms): s, n, e, w,
+        // b, t in that order, with the cell's own value of the component
+        if(d->df_set)
+        {
+            const double x = own[c];
+            if(outside(i - 1, j, k)) { rhs[n] -= Ms[n] * x; Ms[n] = 0.0; }
+            if(outside(i + 1, j, k)) { rhs[n] -= Mn[n] * x; Mn[n] = 0.0; }
+            if(!two)
+            {
+                if(outside(i, j - 1, k)) { rhs[n] -= Me[n] * x; Me[n] = 0.0; }
+                if(outside(i, j + 1, k)) { rhs[n] -= Mw[n] * x; Mw[n] = 0.0; }
+            }
+            if(outside(i, j, k - 1)) { rhs[n] -= Mb[n] * x; Mb[n] = 0.0; }
+            if(outside(i, j, k + 1)) { rhs[n] -= Mt[n] * x; Mt[n] = 0.0; }
+        }
+        ++n;
+    });
+}
+
+// the whole diff_u / diff_v / diff_w call: diff = vin on the component's loop, its ghosts
+// (start1..3 with gcval 10, 11, 12), the operator, bicgstab_ijk (bicgstab_ijk_2D in 2-D)
+// with var comp and stop N43, the ghosts again; returns the iterations
+int idiff(orc_domain* d, int comp, double alpha, double* diff, const double* vin, const double* u, const double* v,
+          const double* w, const double* au, const double* av, const double* nu)
+{
+    const int gcv = comp == 1 ? 10 : comp == 2 ? 11 : 12;
+    vel_loop(d, comp, [&](long c, int, int, int) { diff[c] = vin[c]; });
+    ghost(d, diff, comp, gcv);
+    const long N = veclen(d, comp);
+    std::vector<double> M(7 * N, 0.0), rhs(N, 0.0);
+    idiff_assemble(d, comp, alpha, vin, u, v, w, au, av, nu, M.data(), rhs.data());
+    double res;
+    const int it = bicgstab(d, diff, M.data(), rhs.data(), d->N43, d->prm.N46, &res, comp);
+    ghost(d, diff, comp, gcv);
+    return it;
+}
+
 // momentum_FC3::start (momentum_FC3.cpp:122-430) with D20=0 (diff_void) or 1 (ediff2), B10=0,
 // laminar (kepsilon_void zeroes F/G/H), void ioflow and forcing, pjm_corr projection.
 // The working field ls is a member of momentum_FC3 whose ghost cells are never filled,
@@ -1102,6 +1283,9 @@ struct FC3
         ghost(d, phi, 4, 52);
         fluid_update(d, phi, ro, visc, vol);
         velocity_stage(phi, press, u, v, w, u, v, w, urk1, vrk1, wrk1, 1.0, 0);
+        // momentum_forcing_start (momentum_FC3.cpp:212) ends with solid_forcing_flag_update:
+        // the direct-forcing flags the implicit diffusion reads are set from here on
+        d->df_set = 1;
         iters += pjm(d, phi, urk1.data(), vrk1.data(), wrk1.data(), press, pcorr.data(), 1.0);
         ghost(d, urk1.data(), 1, 10);
         ghost(d, vrk1.data(), 2, 11);
@@ -1145,7 +1329,7 @@ struct FC3
     // stage 1 (kind 0) and stage 2 (kind 1) velocity predictors
     void velocity_stage(const double* phi, const double* press, const double* un, const double* vn, const double* wn,
                         const double* uu, const double* vv, const double* ww, std::vector<double>& ur,
-                        std::vector<double>& vr, std::vector<double>& wr, double, int kind)
+                        std::vector<double>& vr, std::vector<double>& wr, double alpha, int kind)
     {
         const double dt = d->prm.dt;
         std::vector<double>* out[4] = {nullptr, &ur, &vr, &wr};
@@ -1161,8 +1345,14 @@ struct FC3
             }
             rhs_comp(comp, phi, press, uu, vv, ww);
             std::vector<double>& df = *dif[comp];
-            vel_loop(d, comp, [&](long c, int, int, int) { df[c] = in[comp][c]; });
-            ghost(d, df.data(), comp, comp == 1 ? 10 : comp == 2 ? 11 : 12);
+            if(d->D20 == 2)
+                // pdiff->diff_u/v/w with idiff2_FS: the fdm's u, v are the step's (un, vn)
+                idiff(d, comp, alpha, df.data(), in[comp], uu, vv, ww, un, vn, nu);
+            else
+            {
+                vel_loop(d, comp, [&](long c, int, int, int) { df[c] = in[comp][c]; });
+                ghost(d, df.data(), comp, comp == 1 ? 10 : comp == 2 ? 11 : 12);
+            }
             std::vector<double>& o = *out[comp];
             if(kind == 0)
                 vel_loop(d, comp, [&](long c, int, int, int) { o[c] = df[c] + dt * 1.0 * F[c]; });
@@ -1184,8 +1374,14 @@ struct FC3
             if(comp == 2 && d->jdir == 0) continue;
             rhs_comp(comp, phi, press, urk2.data(), vrk2.data(), wrk2.data());
             std::vector<double>& df = *dif[comp];
-            vel_loop(d, comp, [&](long c, int, int, int) { df[c] = in[comp][c]; });
-            ghost(d, df.data(), comp, comp == 1 ? 10 : comp == 2 ? 11 : 12);
+            if(d->D20 == 2)
+                // the fdm's u and v are already this stage's results when w is solved
+                idiff(d, comp, 2.0 / 3.0, df.data(), in[comp], urk2.data(), vrk2.data(), wrk2.data(), u, v, nu);
+            else
+            {
+                vel_loop(d, comp, [&](long c, int, int, int) { df[c] = in[comp][c]; });
+                ghost(d, df.data(), comp, comp == 1 ? 10 : comp == 2 ? 11 : 12);
+            }
             double* t = tgt[comp];
             vel_loop(d, comp, [&](long c, int, int, int) {
                 t[c] = (1.0 / 3.0) * t[c] + (2.0 / 3.0) * df[c] + (2.0 / 3.0) * dt * 1.0 * F[c];
@@ -1346,6 +1542,8 @@ void orc_flux(orc_domain* d, int ipol, const double* q, const double* u, const d
 
 void orc_reini(orc_domain* d, double* phi, int iters) { reini(d, phi, iters); }
 
+void orc_reini_gcv(orc_domain* d, double* phi, int iters, int gcv) { reini(d, phi, iters, gcv); }
+
 void orc_fluid_update(orc_domain* d, const double* phi, double* ro, double* visc, double* vol2)
 {
     fluid_update(d, phi, ro, visc, vol2);
@@ -1361,6 +1559,27 @@ int orc_bicgstab(orc_domain* d, double* x, const double* M7, const double* rhs, 
     return bicgstab(d, x, M7, rhs, stop, maxiter, residual);
 }
 
+int orc_bicgstab_var(orc_domain* d, int var, double* x, const double* M7, const double* rhs, double stop, int maxiter,
+                     double* residual)
+{
+    if(var < 1 || var > 5) return -1;
+    return bicgstab(d, x, M7, rhs, stop, maxiter, residual, var);
+}
+
+long orc_veclen(orc_domain* d, int var) { return var < 1 || var > 5 ? -1 : veclen(d, var); }
+
+void orc_idiff_assemble(orc_domain* d, int comp, double alpha, const double* vin, const double* u, const double* v,
+                        const double* w, const double* au, const double* av, const double* visc, double* M7, double* rhs)
+{
+    idiff_assemble(d, comp, alpha, vin, u, v, w, au, av, visc, M7, rhs);
+}
+
+int orc_idiff(orc_domain* d, int comp, double alpha, double* diff, const double* vin, const double* u, const double* v,
+              const double* w, const double* au, const double* av, const double* visc)
+{
+    return idiff(d, comp, alpha, diff, vin, u, v, w, au, av, visc);
+}
+
 int orc_set_control(orc_domain* d, const char* key, double value)
 {
     if(std::string(key) == "F49" && (value == 0.0 || value == 1.0))
@@ -1368,7 +1587,17 @@ int orc_set_control(orc_domain* d, const char* key, double value)
         d->F49 = (int)value;
         return 0;
     }
-    if(std::string(key) == "D20" && (value == 0.0 || value == 1.0))
+    if(std::string(key) == "N43" && value > 0.0)
+    {
+        d->N43 = value;
+        return 0;
+    }
+    if(std::string(key) == "df_set" && (value == 0.0 || value == 1.0))
+    {
+        d->df_set = (int)value;
+        return 0;
+    }
+    if(std::string(key) == "D20" && (value == 0.0 || value == 1.0 || value == 2.0))
     {
         d->D20 = (int)value;
         return 0;

@@ -54,6 +54,9 @@ void orc_flux(orc_domain*, int ipol, const double* q, const double* u, const dou
 
 /* reini_RK3::start with count>0, iters pseudo steps (phi in place, ghosts refilled) */
 void orc_reini(orc_domain*, double* phi, int iters);
+/* the same pseudo steps with the level-set ghost rule gcv after every stage (50 = gcval_iniphi
+ * of the initial reinitialisation, 52 = gcval_phi) */
+void orc_reini_gcv(orc_domain*, double* phi, int iters, int gcv);
 
 /* ietimestep::start (ini = 0) / ::ini (ini = 1), N50 = 1: out4 = dt, umax, vmax, wmax */
 void orc_timestep(orc_domain*, const double* u, const double* v, const double* w, double N47, int ini,
@@ -73,6 +76,20 @@ int orc_set_control(orc_domain*, const char* key, double value);
 double orc_residual(orc_domain*, const double* x, const double* M7, const double* rhs, double* r);
 int orc_bicgstab(orc_domain*, double* x, const double* M7, const double* rhs, double stop,
                  int maxiter, double* residual);
+/* bicgstab_ijk::start with var 1..5 (FLEXLOOP of flag1..4, bicgstab_ijk.cpp:51-123): M7 rows
+ * and rhs of orc_veclen(var) entries in vec order; x in field layout, solved in place */
+int orc_bicgstab_var(orc_domain*, int var, double* x, const double* M7, const double* rhs, double stop,
+                     int maxiter, double* residual);
+long orc_veclen(orc_domain*, int var);
+/* idiff2_FS / idiff2_FS_2D diff_u/v/w of component comp (1..3): the operator (M7 rows and
+ * rhs in vec order, rhs zeroed by the caller), and the whole call (diff = vin, ghosts, the
+ * var-comp solve with stop N43, ghosts) returning the iterations; au, av are the fdm's u,
+ * v that diff_w reads at k+1 */
+void orc_idiff_assemble(orc_domain*, int comp, double alpha, const double* vin, const double* u, const double* v,
+                        const double* w, const double* au, const double* av, const double* visc, double* M7,
+                        double* rhs);
+int orc_idiff(orc_domain*, int comp, double alpha, double* diff, const double* vin, const double* u, const double* v,
+              const double* w, const double* au, const double* av, const double* visc);
 
 /* pjm_corr::start with alpha: u,v,w corrected in place, press updated, pcorr out */
 int orc_pjm(orc_domain*, const double* phi, double* u, double* v, double* w, double* press,

@@ -45,6 +45,11 @@
 #include"reini_RK3.h"
 #include"levelset_RK3.h"
 #include"bicgstab_ijk.h"
+#include"bicgstab_ijk_2D.h"
+#include"idiff2_FS.h"
+#include"idiff2_FS_2D.h"
+#include"matrix_diag.h"
+#include"vec.h"
 #include"poisson_pcorr.h"
 #include"pjm_corr.h"
 #include"fluid_update_fsf.h"
@@ -153,7 +158,7 @@ struct Case
     double N44;
     int N46;
     double W2 = 1.0e-6, W4 = 1.4938e-5;   // kinematic viscosity of water and air
-    int D20 = 0;                          // 0 diff_void, 1 ediff2 (explicit diffusion)
+    int D20 = 0;                          // 0 diff_void, 1 ediff2 (explicit), 2 idiff2_FS (implicit)
 };
 
 static double node_coord(int i, int n, double x0, double d, int nonuni)
@@ -688,6 +693,96 @@ static void op_bicgstab(Env& e, Rng& r)
 }
 
 // ---------------------------------------------------------------- level-set benchmark (C0-style)
+// A solver that dumps what solver::start receives (the caller's matrix_diag rows and
+// rhsvec over the vec of var, the initial guess) and what the wrapped bicgstab returns
+// (the solution field and solveriter) under the prefix tag.
+class dump_solver : public solver
+{
+public:
+    solver* inner = nullptr;
+    string tag;
+    void start(lexer* p, fdm* a, ghostcell* pgc, field& f, vec& rhs, int var) override
+    {
+        int ul = var == 1 ? p->ulast : 0, vl = var == 2 ? p->vlast : 0, wl = var == 3 ? p->wlast : 0;
+        int* flag = var == 1 ? p->flag1 : var == 2 ? p->flag2 : var == 3 ? p->flag3 : p->flag4;
+        long n = 0;
+        for(int i = 0; i < p->knox - ul; ++i)
+        for(int j = 0; j < p->knoy - vl; ++j)
+        for(int k = 0; k < p->knoz - wl; ++k)
+            if(flag[(i - p->imin) * p->jmax * p->kmax + (j - p->jmin) * p->kmax + k - p->kmin] > 0) ++n;
+        vector<double> M(7 * n), r(n);
+        double* rows[7] = {a->M.p, a->M.n, a->M.s, a->M.w, a->M.e, a->M.t, a->M.b};
+        for(int q = 0; q < 7; ++q)
+            for(long m = 0; m < n; ++m) M[q * n + m] = rows[q][m];
+        for(long m = 0; m < n; ++m) r[m] = rhs.V[m];
+        write_npy(tag + "_M", M.data(), {7, n});
+        write_npy(tag + "_rhs", r.data(), {n});
+        dump_field(tag + "_x0", f);
+        inner->start(p, a, pgc, f, rhs, var);
+        dump_field(tag + "_x1", f);
+        write_scalars(tag + "_it", {double(p->solveriter), double(var), double(n)});
+    }
+    void startf(lexer*, ghostcell*, field&, vec&, matrix_diag&, int) override {}
+    void startF(lexer*, ghostcell*, double*, vec&, matrix_diag&, int) override {}
+    void startV(lexer*, ghostcell*, double*, vec&, matrix_diag&, int) override {}
+    void startM(lexer*, ghostcell*, double*, double*, double*, int) override {}
+};
+
+// implicit momentum diffusion (D20 = 2): idiff2_FS::diff_u/v/w (idiff2_FSu.cpp:40-278 and
+// v, w) in 3-D, idiff2_FS_2D in 2-D, on random staggered velocities and a random positive
+// viscosity, with the bicgstab the driver would pair with them (driver_logic_cfd.cpp:
+// 464-468); pass 0 with the direct-forcing flags DF1..DF3 as a run starts (0: no wall
+// folding), pass 1 after ghostcell::solid_forcing_flag_update set them (the state after
+// the first forcing call, momentum_forcing.cpp:102)
+static void op_idiff(Env& e, Rng& r)
+{
+    lexer* p = e.p;
+    fdm* a = e.a;
+    ghostcell* pgc = e.pgc;
+    set_velocities(e, r, 0.8);
+    for(int k = -3; k < G_nz + 3; ++k)
+    for(int j = -3; j < G_ny + 3; ++j)
+    for(int i = -3; i < G_nx + 3; ++i)
+        a->visc(i, j, k) = 0.002 + 0.01 * r.uni();
+    dump_field("id_u", a->u);
+    dump_field("id_v", a->v);
+    dump_field("id_w", a->w);
+    dump_field("id_visc", a->visc);
+    dump_field("id_eddyv", a->eddyv);
+    const double alpha = 0.25;
+    // a stop criterion tight enough that the solves take several iterations
+    const double N43 = p->N43;
+    p->N43 = 1.0e-13;
+    write_scalars("id_scalars", {p->dt, alpha, p->N43, double(p->N46), double(p->D22)});
+    bicgstab_ijk s3(p, a, pgc);
+    bicgstab_ijk_2D s2(p, a, pgc);
+    dump_solver ds;
+    ds.inner = p->j_dir == 1 ? (solver*)&s3 : (solver*)&s2;
+    idiff2_FS d3(p);
+    idiff2_FS_2D d2(p);
+    diffusion* pd = p->j_dir == 1 ? (diffusion*)&d3 : (diffusion*)&d2;
+    field1 udiff(p);
+    field2 vdiff(p);
+    field3 wdiff(p);
+    for(int pass = 0; pass < 2; ++pass)
+    {
+        if(pass == 1) pgc->solid_forcing_flag_update(p, a);
+        dump_iarr("id_DF1_" + std::to_string(pass), p->DF1);
+        for(int comp = 1; comp <= 3; ++comp)
+        {
+            if(comp == 2 && p->j_dir == 0) continue;
+            for(int n = 0; n < p->imax * p->jmax * p->kmax; ++n) a->rhsvec.V[n] = 0.0;
+            ds.tag = "id" + std::to_string(comp) + "_" + std::to_string(pass);
+            field& df = comp == 1 ? (field&)udiff : comp == 2 ? (field&)vdiff : (field&)wdiff;
+            if(comp == 1) pd->diff_u(p, a, pgc, &ds, df, a->u, a->u, a->v, a->w, alpha);
+            if(comp == 2) pd->diff_v(p, a, pgc, &ds, df, a->v, a->u, a->v, a->w, alpha);
+            if(comp == 3) pd->diff_w(p, a, pgc, &ds, df, a->w, a->u, a->v, a->w, alpha);
+            dump_field(ds.tag + "_diff", df);
+        }
+    }
+    p->N43 = N43;
+}
+
 static void run_vortex(Env& e, int nsteps)
 {
     lexer* p = e.p;
@@ -802,20 +897,32 @@ static void run_tank(Env& e, int nsteps, double depth, double amp, double wlen)
     weno_hj_df_nug fsfdisc(p);
     diff_void dvoid;
     ediff2 ediff(p);
+    idiff2_FS idiff(p);
+    idiff2_FS_2D idiff2d(p);
     diffusion* pdiff = p->D20 == 1 ? (diffusion*)&ediff : (diffusion*)&dvoid;
+    if(p->D20 == 2) pdiff = p->j_dir == 1 ? (diffusion*)&idiff : (diffusion*)&idiff2d;
     poisson_pcorr pois(p, ph, pc);
     pjm_corr press(p, a, pgc, ph, pc);
     kepsilon_void turb(p, a, pgc);
-    bicgstab_ijk solv(p, a, pgc);
+    bicgstab_ijk solv3(p, a, pgc), psolv3(p, a, pgc);
+    bicgstab_ijk_2D solv2(p, a, pgc), psolv2(p, a, pgc);
+    // the D20 = 2 runs use two solver objects, psolv for the diffusion and ppoissonsolv for
+    // the pressure, and pair a 2-D tank with bicgstab_ijk_2D, as driver_logic_cfd.cpp:464-478
+    // does (psolv keeps its private x between the diffusion solves); the earlier fixture runs
+    // (D20 0, 1: psolv unused) kept one bicgstab_ijk for both
+    const bool two = p->D20 == 2 && p->j_dir == 0;
+    solver& solv = two ? (solver&)solv2 : (solver&)solv3;
+    solver& dsolv = p->D20 != 2 ? solv : two ? (solver&)psolv2 : (solver&)psolv3;
     p->F49 = G_f49;
     reini_RK3 ri(p, 1);
     fsi_void fsi(p, pgc);
-    momentum_FC3 mom(p, a, pgc, &conv, &fsfdisc, pdiff, &press, &pois, &turb, &solv, &solv, e.pflow, ph, pc, &ri, &fsi);
+    momentum_FC3 mom(p, a, pgc, &conv, &fsfdisc, pdiff, &press, &pois, &turb, &dsolv, &solv, e.pflow, ph, pc, &ri, &fsi);
     vrans_v vr(p, pgc);
     sixdof_void sd(p, pgc);
     sd.initialize(p, a, pgc);
     vr.initialize_cfd(p, a, pgc);
 
+    if(p->D20 == 2) dump_iarr("tk_DF1_0", p->DF1);
     dump_field("tk_phi_0", a->phi);
     dump_field("tk_u_0", a->u);
     dump_field("tk_v_0", a->v);
@@ -838,6 +945,7 @@ static void run_tank(Env& e, int nsteps, double depth, double amp, double wlen)
         dump_field("tk_ro_" + t, a->ro);
         dump_field("tk_visc_" + t, a->visc);
         write_scalars("tk_vol_" + t, {p->volume1, p->volume2, double(p->solveriter)});
+        if(p->D20 == 2 && s == 1) dump_iarr("tk_DF1_1", p->DF1);
     }
 }
 
@@ -941,10 +1049,17 @@ int main(int argc, char** argv)
         c.N40 = 13;
         c.dt = 0.5 / (16.0 * 2.0);
     }
-    else if(cname == "tank" || cname == "tank2d" || cname == "tankfsf" || cname == "tankvisc" ||
-            cname == "tank2dvisc")
+    else if(cname == "box2d")
     {
-        const bool two = cname == "tank2d" || cname == "tank2dvisc";
+        c.nx = 14; c.ny = 1; c.nz = 9;
+        c.jdir = 0;
+        c.dx = c.dy = c.dz = 0.1;
+        c.nonuniform = 1;
+    }
+    else if(cname == "tank" || cname == "tank2d" || cname == "tankfsf" || cname == "tankvisc" ||
+            cname == "tank2dvisc" || cname == "tankidiff" || cname == "tank2didiff")
+    {
+        const bool two = cname == "tank2d" || cname == "tank2dvisc" || cname == "tank2didiff";
         c.nx = two ? 24 : 16;
         c.ny = two ? 1 : 6;
         c.jdir = two ? 0 : 1;
@@ -953,6 +1068,12 @@ int main(int argc, char** argv)
         {
             // explicit diffusion (ediff2) with a viscosity large enough to matter in 3 steps
             c.D20 = 1;
+            c.W2 = c.W4 = 0.01;
+        }
+        if(cname == "tankidiff" || cname == "tank2didiff")
+        {
+            // implicit diffusion (idiff2_FS / idiff2_FS_2D), the default D20 of ini.cpp:391
+            c.D20 = 2;
             c.W2 = c.W4 = 0.01;
         }
         c.nz = 12;
@@ -1002,12 +1123,15 @@ int main(int argc, char** argv)
             op_bicgstab(e, r);
         }
         op_timestep(e, r, 0.7);
+        op_idiff(e, r);
     }
+    else if(cname == "box2d")
+        op_idiff(e, r);
     else if(cname == "vortex")
         run_vortex(e, 4);
-    else if(cname == "tank" || cname == "tankfsf" || cname == "tankvisc")
+    else if(cname == "tank" || cname == "tankfsf" || cname == "tankvisc" || cname == "tankidiff")
         run_tank(e, 3, 0.35, 0.02, 0.8);
-    else if(cname == "tank2d" || cname == "tank2dvisc")
+    else if(cname == "tank2d" || cname == "tank2dvisc" || cname == "tank2didiff")
         run_tank(e, 3, 0.35, 0.02, 0.6);
     if(cname == "tank2d") op_timestep(e, r, 1.3);
     if(cname == "tank" || cname == "tank2d") op_state(e);

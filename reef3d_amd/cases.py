@@ -88,3 +88,59 @@ def vortex3d(n):
 C0 = dict(name="vortex3d_128", n=128, T=3.0, dt=1.0 / 512.0)
 C1 = dict(name="wave_tank_512x64x128", nx=512, ny=64, nz=128, dx=0.02, depth=1.6, amp=0.05,
           wavelength=2.0, dt=0.005, steps=20, rtol=1e-10)
+
+
+def c3_specs(n=64, seed=1003):
+    """Seeded 2-D tank set of the batched workload: nx in {256..2048 step 128}, nz in
+    {64..256 step 32}, dx = dz ~ U[0.01, 0.05], depth = 0.6 nz dz, amplitude ~ U[0.01, 0.1]
+    depth, wavelength ~ U[1, 4] m."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n):
+        nx = int(rng.choice(np.arange(256, 2049, 128)))
+        nz = int(rng.choice(np.arange(64, 257, 32)))
+        dx = float(rng.uniform(0.01, 0.05))
+        depth = 0.6 * nz * dx
+        amp = float(rng.uniform(0.01, 0.1)) * depth
+        lam = float(rng.uniform(1.0, 4.0))
+        specs.append(dict(nx=nx, ny=1, nz=nz, jdir=0, dx=dx, depth=depth, amp=amp, wavelength=lam))
+    return specs
+
+
+def c4_requests(n, step_div=1, seed=1004, rate=200.0):
+    """Seeded request stream of the served workload: Poisson arrivals (rate per second),
+    70 % 2-D tanks (nx 256..2048 step 128, nz 64..256 step 32), 30 % 3-D tanks (nx
+    128..256 step 32, ny 32..64 step 8, nz 64..128 step 16), dx, depth, amplitude and
+    wavelength as in c3_specs, 50..500 steps each (divided by step_div, at least 1)."""
+    rng = np.random.default_rng(seed)
+    t, reqs = 0.0, []
+    for _ in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        if rng.random() < 0.7:
+            nx, ny, nz, jdir = int(rng.choice(np.arange(256, 2049, 128))), 1, int(rng.choice(np.arange(64, 257, 32))), 0
+        else:
+            nx, ny, nz, jdir = (int(rng.choice(np.arange(128, 257, 32))), int(rng.choice(np.arange(32, 65, 8))),
+                                int(rng.choice(np.arange(64, 129, 16))), 1)
+        dx = float(rng.uniform(0.01, 0.05))
+        depth = 0.6 * nz * dx
+        amp = float(rng.uniform(0.01, 0.1)) * depth
+        lam = float(rng.uniform(1.0, 4.0))
+        steps = int(rng.integers(50, 501))
+        reqs.append(dict(arrival=t, nx=nx, ny=ny, nz=nz, jdir=jdir, dx=dx, depth=depth, amp=amp, wavelength=lam,
+                         steps=max(1, steps // step_div), steps_full=steps))
+    return reqs
+
+
+def tank_fields(spec, g=9.81):
+    """wave_tank fields of a c3_specs / c4_requests entry plus its fixed time step
+    dt = 0.25 dx / max(sqrt(g depth), max |u|, |w| of the initial fields)."""
+    f = wave_tank(spec["nx"], spec["ny"], spec["nz"], spec["dx"], spec["depth"], spec["amp"], spec["wavelength"], g)
+    umax = max(float(np.abs(f["u"]).max()), float(np.abs(f["w"]).max()))
+    f["dt"] = 0.25 * spec["dx"] / max(math.sqrt(g * spec["depth"]), umax)
+    return f
+
+
+def tank_kwargs(spec, f, rtol=1e-10, N46=20000):
+    """Batch.add / Domain keyword arguments of a c3_specs / c4_requests tank."""
+    return dict(nx=spec["nx"], ny=spec["ny"], nz=spec["nz"], XN=f["XN"], YN=f["YN"], ZN=f["ZN"], DYM=spec["dx"],
+                DZM=spec["dx"], jdir=spec["jdir"], top_group=3, g=(0.0, 0.0, -9.81), dt=f["dt"], N44=-rtol, N46=N46)

@@ -43,6 +43,11 @@ struct TankDev
     int gnops[NPROG];
     BiState* st;                   // solver state (FP64 or mixed)
     double* vol;                   // volume1, volume2 of the last fluid update
+    // explicit 7-diagonal operator of the matrix solve (solver::start with the caller's
+    // a->M) in field layout: M.p, M.n, M.s, M.w, M.e, M.t, M.b, the right-hand side, and the
+    // private x of the solver of var 1..3 (m[8]) and of var 4, 5 (m[9]); null until the
+    // tank first needs them
+    double* m[10];
 };
 
 // The tanks one launch covers, passed by value (it lives in the kernel-argument segment,
@@ -73,7 +78,7 @@ void bl_hj_stage(const TankSet& b, int stage, int phi_s, int phi_n, int u, int v
 void bl_reini_stage(const TankSet& b, int stage, int fs, int f0, int out, double F43, int fsf,
                     hipStream_t s);
 void bl_flux_acc(const TankSet& b, int ipol, int q, int u, int v, int w, int acc, hipStream_t s);
-// diff = 1: with the explicit diffusion of ediff2 (D20 = 1)
+// diff = 1: with the explicit diffusion of ediff2 (D20 = 1); 2: F only into out (D20 = 2)
 void bl_momentum(const TankSet& b, int comp, int stage, int qin, int qold, int u, int v, int w,
                  double grav, double cfac, int out, double rw, double ra, int diff, hipStream_t s);
 void bl_heaviside(const TankSet& b, const double* prm4, hipStream_t s);
@@ -96,6 +101,19 @@ void bl_timestep(const TankSet& b, int u, int v, int w, double N47, int ini, hip
 // momentum_RK3 update of one component
 void bl_vortex_velocity(const TankSet& b, int comp, int id, double simtime, double DXM, hipStream_t s);
 void bl_void_rk3(const TankSet& b, int comp, int id, double Fv, hipStream_t s);
+// bicgstab_ijk / bicgstab_ijk_2D on each tank's explicit 7-diagonal operator (TankDev::m)
+// over the FLEXLOOP cells of var (1..3: the u / v / w grids, 4, 5: every cell), iterating
+// on the private x m[8]; stop / maxiter per call.  bl_mat_copy_x: fillxvec (dir 0, field
+// f into the private x) or finalize (dir 1) on those cells.
+void bl_mat_copy_x(const TankSet& b, int f, int var, int dir, hipStream_t s);
+// implicit diffusion (D20 = 2) of component comp in a momentum_FC3 stage: the idiff2_FS
+// operator and the private x (= qin on the loop); after the var-comp solve, the stage
+// combination from the solved diff and F; bl_df_set marks the direct-forcing flags set
+void bl_idiff(const TankSet& b, int comp, double alpha, int qin, int u, int v, int w, hipStream_t s);
+void bl_idiff_combine(const TankSet& b, int comp, int stage, int qold, int F, double cfac, int out, hipStream_t s);
+void bl_df_set(const TankSet& b, hipStream_t s);
+void bl_mat_init(const TankSet& b, int var, double stop, int maxiter, hipStream_t s);
+void bl_mat_phase(const TankSet& b, int var, int phase, int part, hipStream_t s);
 // solver states of the set's tanks into dst[slot]
 void bl_gather_states(const TankSet& b, BiState* dst, hipStream_t s);
 // ||rhs - A x||_2 per tank into norms[slot]

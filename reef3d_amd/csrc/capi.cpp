@@ -90,18 +90,23 @@ int control(Engine& e, const char* key, double value)
     }
     if(std::strcmp(key, "D20") == 0)
     {
-        if(value != 0.0 && value != 1.0)
-            return fail(REEF_E_UNSUPPORTED, "D20 must be 0 (diff_void) or 1 (ediff2); 2 (idiff2_FS) is not implemented");
+        if(value != 0.0 && value != 1.0 && value != 2.0)
+            return fail(REEF_E_UNSUPPORTED, "D20 must be 0 (diff_void), 1 (ediff2) or 2 (idiff2_FS)");
         if(value == 1.0 && e.n48)
             return fail(REEF_E_UNSUPPORTED, "D20 = 1 with N48 = 1: the reference steps it with another time-step class");
+        return eng(e, e.set_d20((int)value));
+    }
+    if(std::strcmp(key, "N43") == 0)
+    {
+        if(!(value > 0.0)) return fail(REEF_E_ARG, "N43 must be > 0");
         e.drop_graphs();
-        e.d20 = value != 0.0;
+        e.n43 = value;
         return 0;
     }
     if(std::strcmp(key, "N48") == 0)
     {
         if(value != 0.0 && value != 1.0) return fail(REEF_E_UNSUPPORTED, "N48 must be 0 (fixed) or 1 (adaptive)");
-        if(value == 1.0 && e.d20)
+        if(value == 1.0 && e.d20 == 1)
             return fail(REEF_E_UNSUPPORTED, "N48 = 1 with D20 = 1: the reference steps it with another time-step class");
         e.drop_graphs();
         e.n48 = value != 0.0;
@@ -152,8 +157,15 @@ int reef_ctx_create(const reef_grid_desc* grid, const reef_params* prm, reef_ctx
     int rc;
     if((rc = make_engine(c->b))) return rc;
     // one tank: the pool holds exactly its arrays
-    c->b.e->pool.chunk_bytes = Engine::tank_bytes(*grid) + ((size_t)1 << 20);
-    if((rc = c->b.e->add_tank(*grid, *prm, &c->tank)))
+    TankBuild tb;
+    std::string err;
+    if((rc = Engine::build_tank(*grid, *prm, tb, err)))
+    {
+        drop_engine(c->b);
+        return fail(rc, err);
+    }
+    c->b.e->pool.chunk_bytes = Engine::built_bytes(tb) + ((size_t)1 << 20);
+    if((rc = c->b.e->add_built(tb, &c->tank)))
     {
         eng(*c->b.e, rc);
         drop_engine(c->b);
@@ -293,6 +305,16 @@ int reef_reini(reef_ctx* c, int phi_id, int iters)
     return eng(*c->b.e, c->b.e->reini(phi_id, iters));
 }
 
+int reef_reini_ini(reef_ctx* c, int phi_id, int iters, int gcv)
+{
+    if(check_ctx(c)) return REEF_E_ARG;
+    if(!field_ok(phi_id) || iters < 0) return fail(REEF_E_ARG, "bad field id or iteration count");
+    if(gcv != 50 && gcv != 52) return fail(REEF_E_UNSUPPORTED, "initial reinitialisation ghost rule must be 50 or 52");
+    int rc;
+    if((rc = eng(*c->b.e, c->b.e->ghost(phi_id, 4, gcv)))) return rc;
+    return eng(*c->b.e, c->b.e->reini(phi_id, iters, gcv));
+}
+
 int reef_fluid_update(reef_ctx* c, double* vol2)
 {
     if(check_ctx(c)) return REEF_E_ARG;
@@ -333,6 +355,47 @@ int reef_poisson_solve_mixed(reef_ctx* c, int x_id, int rhs_id, double stop, int
     if(iters) *iters = s.iter;
     if(updates) *updates = s.nru;
     if(rel_residual) *rel_residual = s.res0 > 0.0 ? s.residual * double(s.ncell) / s.res0 : 0.0;
+    return 0;
+}
+
+long reef_solver_veclen(reef_ctx* c, int var)
+{
+    if(check_ctx(c) || var < 1 || var > 5) return -1;
+    const HostGrid& h = c->b.e->tank(c->tank)->hg;
+    return (long)(h.nx - (var == 1)) * (h.ny - (var == 2)) * (h.nz - (var == 3));
+}
+
+int reef_solver_start(reef_ctx* c, int var, const double* const* M7, const double* rhs, int x_id, double stop,
+                      int maxiter, int* iters, double* residual)
+{
+    if(check_ctx(c)) return REEF_E_ARG;
+    if(var < 1 || var > 5 || !field_ok(x_id) || !M7 || !rhs || maxiter < 0) return fail(REEF_E_ARG, "bad argument");
+    for(int q = 0; q < 7; ++q)
+        if(!M7[q]) return fail(REEF_E_ARG, "null diagonal");
+    Engine& E = *c->b.e;
+    int rc;
+    if((rc = eng(E, E.ensure_matrix(c->tank)))) return rc;
+    const TankHost* t = E.tank(c->tank);
+    const HostGrid& h = t->hg;
+    const long X = h.nx + 6, Y = h.ny + 6, Z = h.nz + 6;
+    const int ie = h.nx - (var == 1), je = h.ny - (var == 2), ke = h.nz - (var == 3);
+    // vec order (fillxvec, bicgstab_ijk.cpp:366-378): i outermost, k innermost
+    std::vector<double> box((size_t)(X * Y * Z));
+    const double* src[8] = {M7[0], M7[1], M7[2], M7[3], M7[4], M7[5], M7[6], rhs};
+    for(int q = 0; q < 8; ++q)
+    {
+        std::fill(box.begin(), box.end(), 0.0);
+        long n = 0;
+        for(int i = 0; i < ie; ++i)
+        for(int j = 0; j < je; ++j)
+        for(int k = 0; k < ke; ++k)
+            box[(size_t)((i + 3) + X * ((j + 3) + Y * (k + 3)))] = src[q][n++];
+        if((rc = eng(E, E.put_array(c->tank, t->dev.m[q], box.data(), REEF_LAYOUT_IFAST)))) return rc;
+    }
+    if((rc = eng(E, E.mat_solve(var, x_id, stop, maxiter)))) return rc;
+    const BiState& st = E.last_state[c->tank];
+    if(iters) *iters = st.iter;
+    if(residual) *residual = st.residual;
     return 0;
 }
 

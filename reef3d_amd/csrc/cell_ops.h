@@ -626,4 +626,126 @@ __device__ __forceinline__ void correct_cell(const DevGrid& g, const double* __r
               * ((pcorr[c + g.sxy] - pcorr[c]) / (g.DZP[k + MARGE] * roface(phi[c], phi[c + g.sxy], g.psi, rw, ra)));
 }
 
+
+// One row of the implicit viscous operator of idiff2_FS::diff_u / diff_v / diff_w
+// (idiff2_FSu.cpp:40-170, idiff2_FSv.cpp:29-160, idiff2_FSw.cpp:29-160) and of idiff2_FS_2D
+// diff_u / diff_w (idiff2_FSu_2D.cpp, idiff2_FSw_2D.cpp) at cell c of component comp's loop,
+// in the reference's operation order.  Laminar (eddyv = 0: each visc + eddyv sum is the visc
+// sum), no porosity (CPOR = 1); rhs starts from the 0 that irhs left.  D22 = 1: once the
+// direct-forcing flags are set (df), a coupling to a neighbour outside the component's grid
+// (outside [0, ie) x [0, je) x [0, ke)) moves to the right-hand side times the cell's own
+// value, in the order s, n, e, w, b, t.  diff_w reads the fdm's u and v (au, av) at k + 1.
+struct MatRow
+{
+    double p, n, s, w, e, t, b, rhs;
+};
+
+__device__ __forceinline__ MatRow idiff_row(const DevGrid& g, int comp, double alpha, double dt,
+                                            const double* __restrict__ vin, const double* __restrict__ u,
+                                            const double* __restrict__ v, const double* __restrict__ w,
+                                            const double* __restrict__ au, const double* __restrict__ av,
+                                            const double* __restrict__ nu, bool df, int ie, int je, int ke, long c, int i,
+                                            int j, int k)
+{
+    const long X = 1, Y = g.sx, Z = g.sxy;
+    const int IP = i + MARGE, JP = j + MARGE, KP = k + MARGE;
+    const double *DXN = g.DXN, *DYN = g.DYN, *DZN = g.DZN, *DXP = g.DXP, *DYP = g.DYP, *DZP = g.DZP;
+    const double CPOR = 1.0;
+    const bool two = g.jdir == 0;
+    auto V = [&](long o) { return nu[c + o]; };
+    MatRow r;
+    r.w = r.e = 0.0;
+    if(comp == 1)
+    {
+        const double ddz_p = 0.25 * (((V(0) + V(X)) + V(Z)) + V(X + Z));
+        const double ddz_m = 0.25 * (((V(-Z) + V(X - Z)) + V(0)) + V(X));
+        if(!two)
+        {
+            const double ddy_p = 0.25 * (((V(0) + V(X)) + V(Y)) + V(X + Y));
+            const double ddy_m = 0.25 * (((V(-Y) + V(X - Y)) + V(0)) + V(X));
+            r.p = 2.0 * V(X) / (DXN[IP + 1] * DXP[IP]) + 2.0 * V(0) / (DXN[IP] * DXP[IP]) + ddy_p / (DYP[JP] * DYN[JP])
+                + ddy_m / (DYP[JP - 1] * DYN[JP]) + ddz_p / (DZP[KP] * DZN[KP]) + ddz_m / (DZP[KP - 1] * DZN[KP])
+                + CPOR / (alpha * dt);
+            r.rhs = 0.0 + (((v[c + X] - v[c]) * ddy_p - (v[c + X - Y] - v[c - Y]) * ddy_m) / (DXP[IP] * DYN[JP])
+                           + ((w[c + X] - w[c]) * ddz_p - (w[c + X - Z] - w[c - Z]) * ddz_m) / (DXP[IP] * DZN[KP])
+                           + (CPOR * vin[c]) / (alpha * dt));
+            r.e = -ddy_m / (DYP[JP - 1] * DYN[JP]);
+            r.w = -ddy_p / (DYP[JP] * DYN[JP]);
+        }
+        else
+        {
+            r.p = 2.0 * V(X) / (DXN[IP + 1] * DXP[IP]) + 2.0 * V(0) / (DXN[IP] * DXP[IP]) + ddz_p / (DZP[KP] * DZN[KP])
+                + ddz_m / (DZP[KP - 1] * DZN[KP]) + CPOR / (alpha * dt);
+            r.rhs = 0.0 + (((w[c + X] - w[c]) * ddz_p - (w[c + X - Z] - w[c - Z]) * ddz_m) / (DXP[IP] * DZN[KP])
+                           + (CPOR * vin[c]) / (alpha * dt));
+        }
+        r.s = -2.0 * V(0) / (DXN[IP] * DXP[IP]);
+        r.n = -2.0 * V(X) / (DXN[IP + 1] * DXP[IP]);
+        r.b = -ddz_m / (DZP[KP - 1] * DZN[KP]);
+        r.t = -ddz_p / (DZP[KP] * DZN[KP]);
+    }
+    else if(comp == 2)
+    {
+        const double ddx_p = 0.25 * (((V(0) + V(Y)) + V(X)) + V(X + Y));
+        const double ddx_m = 0.25 * (((V(-X) + V(Y - X)) + V(0)) + V(Y));
+        const double ddz_p = 0.25 * (((V(0) + V(Y)) + V(Z)) + V(Y + Z));
+        const double ddz_m = 0.25 * (((V(-Z) + V(Y - Z)) + V(0)) + V(Y));
+        r.p = 2.0 * V(Y) / (DYN[JP + 1] * DYP[JP]) + 2.0 * V(0) / (DYN[JP] * DYP[JP]) + ddx_p / (DXP[IP] * DXN[IP])
+            + ddx_m / (DXP[IP - 1] * DXN[IP]) + ddz_p / (DZP[KP] * DZN[KP]) + ddz_m / (DZP[KP - 1] * DZN[KP])
+            + CPOR / (alpha * dt);
+        r.rhs = 0.0 + (((u[c + Y] - u[c]) * ddx_p - (u[c - X + Y] - u[c - X]) * ddx_m) / (DYP[JP] * DXN[IP])
+                       + ((w[c + Y] - w[c]) * ddz_p - (w[c + Y - Z] - w[c - Z]) * ddz_m) / (DYP[JP] * DZN[KP])
+                       + (CPOR * vin[c]) / (alpha * dt));
+        r.s = -ddx_m / (DXP[IP - 1] * DXN[IP]);
+        r.n = -ddx_p / (DXP[IP] * DXN[IP]);
+        r.e = -2.0 * V(0) / (DYN[JP] * DYP[JP]);
+        r.w = -2.0 * V(Y) / (DYN[JP + 1] * DYP[JP]);
+        r.b = -ddz_m / (DZP[KP - 1] * DZN[KP]);
+        r.t = -ddz_p / (DZP[KP] * DZN[KP]);
+    }
+    else
+    {
+        const double ddx_p = 0.25 * (((V(0) + V(Z)) + V(X)) + V(X + Z));
+        const double ddx_m = 0.25 * (((V(-X) + V(Z - X)) + V(0)) + V(Z));
+        if(!two)
+        {
+            const double ddy_p = 0.25 * (((V(0) + V(Z)) + V(Y)) + V(Y + Z));
+            const double ddy_m = 0.25 * (((V(-Y) + V(Z - Y)) + V(0)) + V(Z));
+            r.p = 2.0 * V(Z) / (DZN[KP + 1] * DZP[KP]) + 2.0 * V(0) / (DZN[KP] * DZP[KP]) + ddx_p / (DXP[IP] * DXN[IP])
+                + ddx_m / (DXP[IP - 1] * DXN[IP]) + ddy_p / (DYP[JP] * DYN[JP]) + ddy_m / (DYP[JP - 1] * DYN[JP])
+                + CPOR / (alpha * dt);
+            r.rhs = 0.0 + (((au[c + Z] - u[c]) * ddx_p - (u[c - X + Z] - u[c - X]) * ddx_m) / (DZP[KP] * DXN[IP])
+                           + ((av[c + Z] - v[c]) * ddy_p - (v[c - Y + Z] - v[c - Y]) * ddy_m) / (DZP[KP] * DYN[JP])
+                           + (CPOR * vin[c]) / (alpha * dt));
+            r.e = -ddy_m / (DYP[JP - 1] * DYN[JP]);
+            r.w = -ddy_p / (DYP[JP] * DYN[JP]);
+        }
+        else
+        {
+            r.p = 2.0 * V(Z) / (DZN[KP + 1] * DZP[KP]) + 2.0 * V(0) / (DZN[KP] * DZP[KP]) + ddx_p / (DXP[IP] * DXN[IP])
+                + ddx_m / (DXP[IP - 1] * DXN[IP]) + CPOR / (alpha * dt);
+            r.rhs = 0.0 + (((u[c + Z] - u[c]) * ddx_p - (u[c - X + Z] - u[c - X]) * ddx_m) / (DZP[KP] * DXN[IP])
+                           + (CPOR * vin[c]) / (alpha * dt));
+        }
+        r.s = -ddx_m / (DXP[IP - 1] * DXN[IP]);
+        r.n = -ddx_p / (DXP[IP] * DXN[IP]);
+        r.b = -2.0 * V(0) / (DZN[KP] * DZP[KP]);
+        r.t = -2.0 * V(Z) / (DZN[KP + 1] * DZP[KP]);
+    }
+    if(df)
+    {
+        const double x = (comp == 1 ? u : comp == 2 ? v : w)[c];
+        if(i - 1 < 0) { r.rhs -= r.s * x; r.s = 0.0; }
+        if(i + 1 >= ie) { r.rhs -= r.n * x; r.n = 0.0; }
+        if(!two)
+        {
+            if(j - 1 < 0) { r.rhs -= r.e * x; r.e = 0.0; }
+            if(j + 1 >= je) { r.rhs -= r.w * x; r.w = 0.0; }
+        }
+        if(k - 1 < 0) { r.rhs -= r.b * x; r.b = 0.0; }
+        if(k + 1 >= ke) { r.rhs -= r.t * x; r.t = 0.0; }
+    }
+    return r;
+}
+
 }  // namespace reef

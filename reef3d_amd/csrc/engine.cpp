@@ -168,17 +168,36 @@ Engine::~Engine()
     if(d_norms_) (void)hipFree(d_norms_);
 }
 
+// device bytes add_built takes for a build: the pool blocks of the FP64 and FP32 arrays,
+// the metadata block (tables, FP32 1/DN, ghost programs), the solver state and the
+// time-step block, each rounded to the pool's granule
+static size_t meta_block_bytes(const TankBuild& b)
+{
+    return up(b.meta.size() * 8) + up(b.inv.size() * 4) + b.ops.size() * sizeof(GhostOp);
+}
+
+size_t Engine::built_bytes(const TankBuild& b)
+{
+    const size_t n = (size_t)b.hg.nalloc;
+    return (size_t)NARR64 * up(n * 8) + (size_t)NARR32 * up(n * 4) + up(meta_block_bytes(b)) + up(sizeof(BiState)) +
+           up(8 * sizeof(double));
+}
+
 size_t Engine::tank_bytes(const reef_grid_desc& gd)
+{
+    reef_params p{};
+    p.F45 = 2.1;
+    TankBuild b;
+    std::string e;
+    if(build_tank(gd, p, b, e)) return 0;
+    return built_bytes(b);
+}
+
+size_t Engine::tank_bytes_min(const reef_grid_desc& gd)
 {
     HostGrid h;
     if(!build_grid(h, gd, 2.1).empty()) return 0;
-    size_t b = (size_t)NARR64 * up((size_t)h.nalloc * 8) + (size_t)NARR32 * up((size_t)h.nalloc * 4);
-    size_t meta = 0;
-    for(int a = 0; a < 3; ++a)
-        meta += (h.P[a].size() * 3 + h.qf[a].size() + h.cf[a].size() + h.isf[a].size()) * 8 + h.DN[a].size() * 4;
-    b += up(meta) + (size_t)2 * 8 * (size_t)(h.nx + h.ny + h.nz) * 64;   // ghost programs (estimate)
-    b += up(sizeof(BiState));
-    return b;
+    return (size_t)NARR64 * up((size_t)h.nalloc * 8) + (size_t)NARR32 * up((size_t)h.nalloc * 4);
 }
 
 int Engine::build_tank(const reef_grid_desc& gd, const reef_params& p, TankBuild& b, std::string& err)
@@ -233,8 +252,19 @@ int Engine::add_tank(const reef_grid_desc& gd, const reef_params& p, int* id)
     return add_built(b, id);
 }
 
+// the reef_params values a batch shares (everything except the per-tank dt and F45)
+static bool same_shared(const reef_params& a, const reef_params& b)
+{
+    return a.W1 == b.W1 && a.W2 == b.W2 && a.W3 == b.W3 && a.W4 == b.W4 && a.W20 == b.W20 && a.W21 == b.W21 &&
+           a.W22 == b.W22 && a.F43 == b.F43 && a.N44 == b.N44 && a.N46 == b.N46;
+}
+
 int Engine::add_built(TankBuild& b, int* id)
 {
+    if(!first_used()) has_shared_ = false;
+    if(has_shared_ && !same_shared(shared_, b.prm))
+        return fail(REEF_E_ARG, "tank parameters differ from the batch's shared ones (W1-W4, W20-W22, F43, N44, N46; "
+                                "only dt and F45 may differ per tank)");
     TankHost t;
     t.used = true;
     t.prm = b.prm;
@@ -284,7 +314,7 @@ int Engine::add_built(TankBuild& b, int* id)
     const std::vector<GhostOp>& ops = b.ops;
     const size_t metabytes = up(meta.size() * 8);
     void* mb;
-    if((rc = take(metabytes + inv.size() * 4 + ops.size() * sizeof(GhostOp) + 64, &mb))) return release(), rc;
+    if((rc = take(meta_block_bytes(b), &mb))) return release(), rc;
     // the solver state stays with the tank (iterations in flight survive table rebuilds)
     void* stb;
     if((rc = take(sizeof(BiState), &stb))) return release(), rc;
@@ -344,8 +374,23 @@ int Engine::add_built(TankBuild& b, int* id)
         tanks.emplace_back();
     }
     tanks[slot] = std::move(t);
+    if(!has_shared_)
+    {
+        shared_ = tanks[slot].prm;
+        has_shared_ = true;
+    }
     dirty_ = true;
     if(id) *id = slot;
+    if(d20 == 2)
+    {
+        const int rc2 = ensure_matrix(slot);
+        if(rc2)
+        {
+            const std::string keep = err;
+            remove_tank(slot);
+            return fail(rc2, keep);
+        }
+    }
     return 0;
 }
 
@@ -358,6 +403,7 @@ int Engine::remove_tank(int id)
     if((rc = drain())) return rc;
     for(void* m : t->mem) pool.free(m);
     *t = TankHost();
+    if(!first_used()) has_shared_ = false;
     dirty_ = true;
     return 0;
 }
@@ -510,6 +556,13 @@ int Engine::put(int id, int field, const double* host, int layout)
 {
     TankHost* t = tank(id);
     if(!t || field < 0 || field >= NARR64) return fail(REEF_E_ARG, "bad tank or field id");
+    return put_array(id, t->dev.f[field], host, layout);
+}
+
+int Engine::put_array(int id, double* dev, const double* host, int layout)
+{
+    TankHost* t = tank(id);
+    if(!t || !dev) return fail(REEF_E_ARG, "bad tank or array");
     if(layout != REEF_LAYOUT_IFAST && layout != REEF_LAYOUT_REF) return fail(REEF_E_ARG, "bad layout");
     const HostGrid& h = t->hg;
     const long X = h.nx + 6, Y = h.ny + 6, Z = h.nz + 6, sx = h.sx;
@@ -525,7 +578,7 @@ int Engine::put(int id, int field, const double* host, int layout)
     }
     else
         for(long r = 0; r < Y * Z; ++r) std::memcpy(st + (size_t)r * sx, host + (size_t)r * X, X * sizeof(double));
-    double* dst0 = t->dev.f[field] + (h.base - 3 - 3 * h.sx - 3 * h.sxy);
+    double* dst0 = dev + (h.base - 3 - 3 * h.sx - 3 * h.sxy);
     EHIP(hipStreamSynchronize(stream));
     EHIP(hipMemcpy(dst0, st, span * sizeof(double), hipMemcpyHostToDevice));
     return 0;
@@ -649,27 +702,27 @@ int Engine::convection(int ipol, int q, int u, int v, int w)
     return 0;
 }
 
-int Engine::reini(int f, int iters)
+int Engine::reini(int f, int iters, int gcv)
 {
     int rc;
     if((rc = sync_tables())) return rc;
-    // F43 is a control value shared by the batch (the first used tank's)
-    const TankHost* t0 = first_used();
-    const double f43 = t0 ? t0->prm.F43 : 0.55;
+    // F43 is a control value shared by the batch
+    const reef_params* sp = shared();
+    const double f43 = sp ? sp->F43 : 0.55;
     for(int q = 0; q < iters; ++q)
     {
         tbegin("reini");
         each([&](const TankSet& S) { bl_reini_stage(S, 1, f, f, A_RK1, f43, fsf, stream); });
         tend();
-        if((rc = ghost(A_RK1, 4, 52))) return rc;
+        if((rc = ghost(A_RK1, 4, gcv))) return rc;
         tbegin("reini");
         each([&](const TankSet& S) { bl_reini_stage(S, 2, A_RK1, f, A_RK2, f43, fsf, stream); });
         tend();
-        if((rc = ghost(A_RK2, 4, 52))) return rc;
+        if((rc = ghost(A_RK2, 4, gcv))) return rc;
         tbegin("reini");
         each([&](const TankSet& S) { bl_reini_stage(S, 3, A_RK2, f, f, f43, fsf, stream); });
         tend();
-        if((rc = ghost(f, 4, 52))) return rc;
+        if((rc = ghost(f, 4, gcv))) return rc;
     }
     return 0;
 }
@@ -678,9 +731,8 @@ int Engine::fluid_update()
 {
     int rc;
     if((rc = sync_tables())) return rc;
-    const TankHost* t0 = first_used();
-    if(!t0) return 0;
-    const reef_params* p = &t0->prm;
+    const reef_params* p = shared();
+    if(!p) return 0;
     const double prm4[4] = {p->W1, p->W2, p->W3, p->W4};
     tbegin("heaviside");
     each([&](const TankSet& S) { bl_heaviside(S, prm4, stream); });
@@ -768,13 +820,16 @@ int Engine::replay(const GraphKey& key, Body body)
 // ------------------------------------------------------------------ BiCGStab iterations
 static const char* const BICG_NAMES[5] = {"bicg_p", "bicg_v", "bicg_t", "bicg_x", ""};
 static const char* const MIX_NAMES[4] = {"mix_v", "bicg32_t", "bicg32_x", "mix_ru"};
+static const char* const MAT_NAMES[4] = {"mat_p", "mat_v", "mat_t", "mat_x"};
 
 // 8 iterations of the tanks of the current sets, then their states gathered into ring
 // slot `ring` and copied to the host.  Iterations after a tank's end are no-ops for it.
-int Engine::enqueue_batch(bool mixed, int x, int rhs, int ring, int iters)
+// kind: SOLVE_FP64 (the Poisson operator), SOLVE_MIXED, or SOLVE_MAT (the explicit
+// 7-diagonal operator; rhs is then the FLEXLOOP var)
+int Engine::enqueue_batch(int kind, int x, int rhs, int ring, int iters)
 {
     const int nph = 4;
-    const char* const* names = mixed ? MIX_NAMES : BICG_NAMES;
+    const char* const* names = kind == SOLVE_MIXED ? MIX_NAMES : kind == SOLVE_MAT ? MAT_NAMES : BICG_NAMES;
     auto body = [&]() -> int {
         for(int q = 0; q < iters; ++q)
             for(int ph = 0; ph < nph; ++ph)
@@ -782,8 +837,10 @@ int Engine::enqueue_batch(bool mixed, int x, int rhs, int ring, int iters)
                 {
                     if(part == 0) tbegin(names[ph]);
                     each([&](const TankSet& S) {
-                        if(mixed)
+                        if(kind == SOLVE_MIXED)
                             bl_mixed_phase(S, x, rhs, ph, part, stream);
+                        else if(kind == SOLVE_MAT)
+                            bl_mat_phase(S, rhs, ph, part, stream);
                         else
                             bl_bicg_phase(S, x, rhs, ph, part, stream);
                     });
@@ -795,11 +852,11 @@ int Engine::enqueue_batch(bool mixed, int x, int rhs, int ring, int iters)
     // the lockstep sets only change with the tables; the scheduler's solving sets change
     // from round to round, and a set's batch graph is captured when the set comes back
     if(cur_ == &all_)
-        rc = replay({1, mixed ? 1 : 0, x, rhs * 64 + iters}, body);
+        rc = replay({1, kind, x, rhs * 64 + iters}, body);
     else if(graphs && !timing && cur_ == &solving_sets_)
     {
         std::vector<int> key(solving_key_);
-        key.insert(key.end(), {-1, mixed ? 1 : 0, x, rhs, iters});
+        key.insert(key.end(), {-1, kind, x, rhs, iters});
         if(set_graphs_.size() > 4096) drop_graphs();
         SetGraph& sg = set_graphs_[key];
         if(!sg.exec && ++sg.seen >= 2 && (rc = capture(body, &sg.exec))) return rc;
@@ -825,16 +882,16 @@ int Engine::enqueue_batch(bool mixed, int x, int rhs, int ring, int iters)
 // BiCGStab iterations of the current sets until every tank of them has finished.  Two
 // batches are in flight: batch k+1 is enqueued before the host waits for the states of
 // batch k.
-int Engine::run_iterations(bool mixed, int x, int rhs, int maxiter)
+int Engine::run_iterations(int kind, int x, int rhs, int maxiter)
 {
     std::vector<int> members;
     for(const TankSet& S : *cur_)
         for(int p = 0; p < S.n; ++p) members.push_back(S.slot[p]);
     int rc;
-    if((rc = enqueue_batch(mixed, x, rhs, 0, 8))) return rc;
+    if((rc = enqueue_batch(kind, x, rhs, 0, 8))) return rc;
     for(long k = 1;; ++k)
     {
-        if((rc = enqueue_batch(mixed, x, rhs, (int)(k & 1), 8))) return rc;
+        if((rc = enqueue_batch(kind, x, rhs, (int)(k & 1), 8))) return rc;
         EHIP(hipEventSynchronize(st_ev_[(k - 1) & 1]));
         const BiState* hs = h_status_ + (size_t)((k - 1) & 1) * cap_tanks_;
         bool all = true;
@@ -856,8 +913,8 @@ int Engine::run_iterations(bool mixed, int x, int rhs, int maxiter)
 // solver's init)
 int Engine::solve_begin(int x, int rhs, double stop, int maxiter, bool mixed)
 {
-    const TankHost* t0 = first_used();
-    const double rw = t0 ? t0->prm.W1 : 1000.0, ra = t0 ? t0->prm.W3 : 1.205;
+    const reef_params* sp = shared();
+    const double rw = sp ? sp->W1 : 1000.0, ra = sp ? sp->W3 : 1.205;
     tbegin("face_rho");
     each([&](const TankSet& S) { bl_face_rho(S, rw, ra, mixed ? 0 : 1, stream); });
     tend();
@@ -877,7 +934,10 @@ int Engine::solve_begin(int x, int rhs, double stop, int maxiter, bool mixed)
     return 0;
 }
 
-int Engine::solve_iterate(int x, int rhs, int maxiter, bool mixed) { return run_iterations(mixed, x, rhs, maxiter); }
+int Engine::solve_iterate(int x, int rhs, int maxiter, bool mixed)
+{
+    return run_iterations(mixed ? SOLVE_MIXED : SOLVE_FP64, x, rhs, maxiter);
+}
 
 // the mixed solve keeps its FP64 iterate split until the end: x64 += x32
 int Engine::solve_end(int x, bool mixed)
@@ -900,12 +960,55 @@ int Engine::solve(int x, int rhs, double stop, int maxiter, bool mixed)
     return solve_end(x, mixed);
 }
 
+// the explicit-operator arrays of a tank (M.p .. M.b, the right-hand side and the solver's
+// private x), zeroed, from the pool the first time a tank needs them
+int Engine::ensure_matrix(int id)
+{
+    TankHost* t = tank(id);
+    if(!t) return fail(REEF_E_ARG, "no such tank");
+    if(t->dev.m[0]) return 0;
+    const size_t bytes = (size_t)t->hg.nalloc * 8;
+    for(int q = 0; q < 10; ++q)
+    {
+        void* m = nullptr;
+        const int rc = pool.alloc(bytes, &m);
+        if(rc) return fail(rc, "device pool: out of memory (budget) for the explicit operator");
+        t->mem.push_back(m);
+        t->bytes += up(bytes);
+        t->dev.m[q] = (double*)m;
+        EHIP(hipMemsetAsync(m, 0, bytes, stream));
+    }
+    dirty_ = true;
+    return 0;
+}
+
+// solver::start (bicgstab_ijk / bicgstab_ijk_2D, bicgstab_ijk.cpp:51-123) on the explicit
+// operator of every active tank: var 1..3 over the u / v / w grids (stop = N43), 4, 5 over
+// every cell (N44); the solution in place in array x
+int Engine::mat_solve(int var, int x, double stop, int maxiter)
+{
+    if(var < 1 || var > 5) return fail(REEF_E_ARG, "var must be 1..5");
+    int rc;
+    for(const TankHost& t : tanks)
+        if(t.used && t.active && !t.dev.m[0]) return fail(REEF_E_ARG, "tank without an explicit operator");
+    if((rc = sync_tables()) || (rc = drain())) return rc;
+    tbegin("mat_init");
+    each([&](const TankSet& S) {
+        bl_mat_copy_x(S, x, var, 0, stream);
+        bl_mat_init(S, var, stop, maxiter, stream);
+    });
+    tend();
+    if((rc = run_iterations(SOLVE_MAT, -1, var, maxiter))) return rc;
+    each([&](const TankSet& S) { bl_mat_copy_x(S, x, var, 1, stream); });
+    return 0;
+}
+
 int Engine::residual(int x, int rhs, std::vector<double>& norms)
 {
     int rc;
     if((rc = sync_tables())) return rc;
-    const TankHost* t0 = first_used();
-    const double rw = t0 ? t0->prm.W1 : 1000.0, ra = t0 ? t0->prm.W3 : 1.205;
+    const reef_params* sp = shared();
+    const double rw = sp ? sp->W1 : 1000.0, ra = sp ? sp->W3 : 1.205;
     each([&](const TankSet& S) {
         bl_face_rho(S, rw, ra, 0, stream);
         bl_residual(S, x, rhs, d_norms_, stream);
@@ -922,27 +1025,27 @@ int Engine::pjm_pre(int u, int v, int w, double alpha, bool mixed)
 {
     int rc;
     if((rc = ghost(u, 1, 7)) || (rc = ghost(v, 2, 8)) || (rc = ghost(w, 3, 9))) return rc;
-    const TankHost* t0 = first_used();
-    if(!t0) return 0;
+    const reef_params* sp = shared();
+    if(!sp) return 0;
     tbegin("poisson_rhs");
     each([&](const TankSet& S) {
         bl_poisson_rhs(S, u, v, w, alpha, stream);
         bl_fill(S, REEF_PCORR, 0.0, stream);
     });
     tend();
-    return solve_begin(REEF_PCORR, A_RHS, t0->prm.N44, t0->prm.N46, mixed);
+    return solve_begin(REEF_PCORR, A_RHS, sp->N44, sp->N46, mixed);
 }
 
 // pjm_corr::start after the solve: p' ghosts, velocity and pressure correction
 int Engine::pjm_post(int u, int v, int w, double alpha, bool mixed)
 {
     int rc;
-    const TankHost* t0 = first_used();
-    if(!t0) return 0;
+    const reef_params* sp = shared();
+    if(!sp) return 0;
     if((rc = solve_end(REEF_PCORR, mixed))) return rc;
     if((rc = ghost(REEF_PCORR, 4, 40))) return rc;
     tbegin("correct");
-    each([&](const TankSet& S) { bl_correct(S, t0->prm.W1, t0->prm.W3, alpha, u, v, w, stream); });
+    each([&](const TankSet& S) { bl_correct(S, sp->W1, sp->W3, alpha, u, v, w, stream); });
     tend();
     return ghost(REEF_PRESS, 4, 40);
 }
@@ -952,11 +1055,11 @@ int Engine::pjm(int u, int v, int w, double alpha)
 {
     int rc;
     if((rc = sync_tables()) || (rc = drain())) return rc;
-    const TankHost* t0 = first_used();
+    const reef_params* sp = shared();
     const bool mixed = precision == REEF_SOLVER_MIXED;
     if((rc = pjm_pre(u, v, w, alpha, mixed))) return rc;
-    if(!t0) return 0;
-    if((rc = solve_iterate(REEF_PCORR, A_RHS, t0->prm.N46, mixed))) return rc;
+    if(!sp) return 0;
+    if((rc = solve_iterate(REEF_PCORR, A_RHS, sp->N46, mixed))) return rc;
     for(const TankSet& S : all_)
         for(int p = 0; p < S.n; ++p) step_iters[S.slot[p]] += last_state[S.slot[p]].iter;
     return pjm_post(u, v, w, alpha, mixed);
@@ -965,23 +1068,67 @@ int Engine::pjm(int u, int v, int w, double alpha)
 // velocity predictor of the three components (momentum_FC3.cpp:160-208 and the later stages)
 int Engine::momentum_stage(int stage, const int qin[3], const int qold[3], const int vel[3], const int out[3])
 {
-    const TankHost* t0 = first_used();
-    if(!t0) return 0;
-    const double grav[3] = {t0->prm.W20, t0->prm.W21, t0->prm.W22};
+    const reef_params* sp = shared();
+    if(!sp) return 0;
+    const double grav[3] = {sp->W20, sp->W21, sp->W22};
     const double cfac = stage == 1 ? 1.0 : stage == 2 ? 0.25 : (2.0 / 3.0);
     int rc;
-    for(int comp = 1; comp <= 3; ++comp)
+    if(d20 == 2)
     {
-        tbegin("momentum");
-        each([&](const TankSet& S) {
-            bl_momentum(S, comp, stage, qin[comp - 1], qold[comp - 1], vel[0], vel[1], vel[2], grav[comp - 1], cfac,
-                        out[comp - 1], t0->prm.W1, t0->prm.W3, d20, stream);
-        });
-        tend();
+        // idiff2_FS: per component F (irhs + convection) into F / G / H, the implicit
+        // viscous solve (var = comp, stop N43) on the private x, then the stage combination
+        static const int FGH[3] = {REEF_F, REEF_G, REEF_H};
+        for(int comp = 1; comp <= 3; ++comp)
+        {
+            const int q = comp - 1;
+            tbegin("momentum");
+            each([&](const TankSet& S) {
+                bl_momentum(S, comp, stage, qin[q], qold[q], vel[0], vel[1], vel[2], grav[q], cfac, FGH[q], sp->W1,
+                            sp->W3, 2, stream);
+            });
+            tend();
+            tbegin("idiff");
+            each([&](const TankSet& S) {
+                bl_idiff(S, comp, cfac, qin[q], vel[0], vel[1], vel[2], stream);
+                bl_mat_init(S, comp, n43, sp->N46, stream);
+            });
+            tend();
+            if((rc = run_iterations(SOLVE_MAT, -1, comp, sp->N46))) return rc;
+            tbegin("idiff");
+            each([&](const TankSet& S) { bl_idiff_combine(S, comp, stage, qold[q], FGH[q], cfac, out[q], stream); });
+            tend();
+        }
+        if(stage == 1) each([&](const TankSet& S) { bl_df_set(S, stream); });
     }
+    else
+        for(int comp = 1; comp <= 3; ++comp)
+        {
+            tbegin("momentum");
+            each([&](const TankSet& S) {
+                bl_momentum(S, comp, stage, qin[comp - 1], qold[comp - 1], vel[0], vel[1], vel[2], grav[comp - 1], cfac,
+                            out[comp - 1], sp->W1, sp->W3, d20, stream);
+            });
+            tend();
+        }
     const int kinds[3] = {1, 2, 3}, gcvs[3] = {10, 11, 12};
     for(int comp = 0; comp < 3; ++comp)
         if((rc = ghost(out[comp], kinds[comp], gcvs[comp]))) return rc;
+    return 0;
+}
+
+// D20 = 2 needs each tank's explicit-operator arrays
+int Engine::set_d20(int v)
+{
+    if(v < 0 || v > 2) return fail(REEF_E_ARG, "D20 must be 0, 1 or 2");
+    drop_graphs();
+    if(v == 2)
+        for(size_t q = 0; q < tanks.size(); ++q)
+            if(tanks[q].used)
+            {
+                const int rc = ensure_matrix((int)q);
+                if(rc) return rc;
+            }
+    d20 = v;
     return 0;
 }
 
@@ -1068,14 +1215,20 @@ int Engine::fc3_step()
     int rc;
     if((rc = sync_tables()) || (rc = drain())) return rc;
     std::fill(step_iters.begin(), step_iters.end(), 0L);
-    const TankHost* t0 = first_used();
-    if(!t0) return 0;
+    const reef_params* sp = shared();
+    if(!sp) return 0;
     const bool mixed = precision == REEF_SOLVER_MIXED;
     for(int s = 0; s < 4; ++s)
     {
-        if((rc = replay({0, s, mixed ? 1 : 0, fsf}, [&] { return fc3_segment(s, mixed); }))) return rc;
+        // the implicit diffusion (D20 = 2) synchronises with the host inside its segments,
+        // which are then launched directly
+        if(d20 == 2)
+            rc = fc3_segment(s, mixed);
+        else
+            rc = replay({0, s, mixed ? 1 : 0, fsf}, [&] { return fc3_segment(s, mixed); });
+        if(rc) return rc;
         if(s == 3) break;
-        if((rc = solve_iterate(REEF_PCORR, A_RHS, t0->prm.N46, mixed))) return rc;
+        if((rc = solve_iterate(REEF_PCORR, A_RHS, sp->N46, mixed))) return rc;
         for(const TankSet& S : all_)
             for(int p = 0; p < S.n; ++p) step_iters[S.slot[p]] += last_state[S.slot[p]].iter;
     }
@@ -1121,6 +1274,8 @@ int Engine::start_steps(int id, int n)
 {
     TankHost* t = tank(id);
     if(!t || n < 0) return fail(REEF_E_ARG, "bad tank or step count");
+    if(d20 == 2)
+        return fail(REEF_E_UNSUPPORTED, "D20 = 2 (implicit diffusion) runs in lockstep steps only (reef_batch_fc3_step)");
     t->steps_left += n;
     if(t->pc < 0 && t->steps_left > 0) t->pc = 0;
     return 0;
@@ -1223,7 +1378,7 @@ int Engine::pump(std::vector<int>& stepped)
         cur_ = &solving_sets_;
         // many tanks solving: a deeper batch per round amortises the round's segment
         // launches and host synchronisation (iterations after a tank's end are no-ops)
-        rc = enqueue_batch(mixed, REEF_PCORR, A_RHS, ring, (int)solving.size() >= 8 ? 32 : 8);
+        rc = enqueue_batch(mixed ? SOLVE_MIXED : SOLVE_FP64, REEF_PCORR, A_RHS, ring, (int)solving.size() >= 8 ? 32 : 8);
         cur_ = &all_;
         if(rc) return rc;
         Inflight f;
@@ -1279,8 +1434,9 @@ int Engine::vortex_step(double simtime)
     if((rc = phi_rk3())) return rc;
     if((rc = reini(REEF_PHI, 3))) return rc;
     const TankHost* t0 = first_used();
-    if(!t0) return 0;
-    const double grav[3] = {t0->prm.W20, t0->prm.W21, t0->prm.W22};
+    const reef_params* sp = shared();
+    if(!t0 || !sp) return 0;
+    const double grav[3] = {sp->W20, sp->W21, sp->W22};
     for(int comp = 1; comp <= 3; ++comp)
     {
         if(comp == 2 && t0->hg.jdir == 0) continue;
@@ -1325,7 +1481,12 @@ int Engine::probe(double& stencil_ms, double& solver_ms)
     });
     if((rc = ghost(A_FRK1, 4, 52)) || (rc = reini(A_FRK1, 1))) return rc;
     const int qin[3] = {REEF_U, REEF_V, REEF_W}, out[3] = {A_URK1, A_VRK1, A_WRK1};
-    if((rc = momentum_stage(1, qin, qin, qin, out))) return rc;
+    // the predictor without the implicit diffusion, whose solves would move the private x
+    const int keep_d20 = d20;
+    d20 = d20 == 2 ? 0 : d20;
+    rc = momentum_stage(1, qin, qin, qin, out);
+    d20 = keep_d20;
+    if(rc) return rc;
     EHIP(hipEventRecord(ev[1].e, stream));
     if((rc = ghost(A_URK1, 1, 7)) || (rc = ghost(A_VRK1, 2, 8)) || (rc = ghost(A_WRK1, 3, 9))) return rc;
     each([&](const TankSet& S) {

@@ -24,6 +24,14 @@
 
 namespace reef {
 
+// the three iterations of the solver kernels
+enum SolveKind
+{
+    SOLVE_FP64 = 0,    // bicgstab_ijk on the Poisson operator (face densities)
+    SOLVE_MIXED = 1,   // FP32 iteration with FP64 reliable updates, Poisson operator
+    SOLVE_MAT = 2      // bicgstab_ijk on an explicit 7-diagonal operator (TankDev::m)
+};
+
 // first-fit suballocator over large device chunks (256-byte granules)
 class DevicePool
 {
@@ -95,8 +103,12 @@ public:
     // with the factor N47) computed on the device; 0: the fixed dt of each tank
     int n48 = 0;
     double n47 = 0.3;
-    // D20 = 1: the momentum predictors add ediff2's explicit viscous terms; 0: diff_void
+    // D20 = 1: the momentum predictors add ediff2's explicit viscous terms; 2: idiff2_FS,
+    // the implicit viscous solve of each component (stop N43); 0: diff_void.  Set with
+    // set_d20 (D20 = 2 gives every tank its explicit-operator arrays).
     int d20 = 0;
+    double n43 = 1.0e-5;
+    int set_d20(int v);
     double delta = 0.1;             // reliable-update threshold of the mixed solve
     // replay fc3_step's launch sequences (the four segments between its pressure solves)
     // and the BiCGStab iteration batches as hipGraphs, captured on first use; the graphs
@@ -129,7 +141,13 @@ public:
     int autotune(int reps, std::vector<TuneRecord>* log);
 
     // ---- tanks
+    // device bytes of one tank of this grid: exactly what add_built takes from the pool
+    // (builds the tank's host half, ghost programs included); 0 if the grid is refused
     static size_t tank_bytes(const reef_grid_desc& g);
+    // the same for a build already made
+    static size_t built_bytes(const TankBuild& b);
+    // a lower bound without the ghost programs (the field arrays only), cheap
+    static size_t tank_bytes_min(const reef_grid_desc& g);
     int add_tank(const reef_grid_desc& g, const reef_params& p, int* id);
     // add_tank in two halves: the host build (thread-safe, no engine state) and the device
     // part; add_built leaves b usable again when it fails
@@ -146,15 +164,24 @@ public:
 
     // ---- host <-> device fields of one tank (REEF_LAYOUT_IFAST / REEF_LAYOUT_REF)
     int put(int id, int field, const double* host, int layout);
+    // the same into any FP64 array of the tank's layout (e.g. TankDev::m)
+    int put_array(int id, double* dev, const double* host, int layout);
     int get(int id, int field, double* host, int layout);
     int zero(int field);
 
     // ---- operations on every active tank
     int ghost(int field, int kind, int gcv);
     int convection(int ipol, int q, int u, int v, int w);
-    int reini(int field, int iters);
+    // reini_RK3 pseudo steps with the ghost rule gcv of the level set (52 = gcval_phi for
+    // F50 = 2; 50 = gcval_iniphi of the initial reinitialisation)
+    int reini(int field, int iters, int gcv = 52);
     int fluid_update();
     int solve(int x, int rhs, double stop, int maxiter, bool mixed);
+    // the tank's explicit-operator arrays (TankDev::m), from the pool on first use
+    int ensure_matrix(int id);
+    // solver::start on the explicit operators of the active tanks (bicgstab_ijk with var
+    // 1..5), the solution in place in array x
+    int mat_solve(int var, int x, double stop, int maxiter);
     int residual(int x, int rhs, std::vector<double>& norms);
     int pjm(int u, int v, int w, double alpha);
     int fc3_step();
@@ -252,11 +279,17 @@ private:
     int capture(Body body, hipGraphExec_t* out);
     template<class Body>
     int replay(const GraphKey& key, Body body);
-    int enqueue_batch(bool mixed, int x, int rhs, int ring, int iters);
+    int enqueue_batch(int kind, int x, int rhs, int ring, int iters);
     int drain();
     int read_batch(const Inflight& b);
-    int run_iterations(bool mixed, int x, int rhs, int maxiter);
+    int run_iterations(int kind, int x, int rhs, int maxiter);
     const TankHost* first_used() const;
+    // the parameters every tank of the engine shares (all of reef_params except dt and
+    // F45): those of the first tank added to an empty engine; add_built refuses a tank
+    // whose values differ.  Null while the engine holds no tank.
+    reef_params shared_{};
+    bool has_shared_ = false;
+    const reef_params* shared() const { return has_shared_ ? &shared_ : nullptr; }
     int solve_begin(int x, int rhs, double stop, int maxiter, bool mixed);
     int solve_iterate(int x, int rhs, int maxiter, bool mixed);
     int solve_end(int x, bool mixed);

@@ -223,7 +223,10 @@ std::string build_grid(HostGrid& g, const reef_grid_desc& d, double F45)
     g.sxy = g.sx * g.sy;
     g.base = 16 + NG * g.sx + NG * g.sxy;        // (-3, j, k) at offset 13 within a row
     g.nalloc = round_up(16 + g.sxy * g.sz + 64, 64);
-    if(g.nalloc > (1L << 30)) return "field too large for 32-bit ghost programs";
+    // the solver walks address an array through 32-bit byte offsets from its base (FP64:
+    // at most 2^29 - 1 elements) and the ghost programs through 32-bit element indices
+    if(g.nalloc * (long)sizeof(double) >= (1L << 32))
+        return "field too large: at most 2^29 - 1 doubles per field (32-bit byte offsets of the solver kernels)";
     g.last[0] = 0;
     g.last[1] = g.last[2] = g.last[3] = 1;        // lexer::vellast, no neighbour ranks
     // flags (gc_flagfield.cpp:26-98): 10 inside, -20 outside; u/v/w lose their last row

@@ -427,8 +427,10 @@ __global__ void __launch_bounds__(NT, 4) kb_flux_acc(TankSet b, int ipol, int q_
 }
 
 // velocity predictor of one component (momentum_FC3.cpp:160-208 and the later stages);
-// cdt = cfac * dt of the tank
-template<int comp, bool DIFF>
+// cdt = cfac * dt of the tank.  MODE 0: diff_void, 1: ediff2's explicit viscous terms added
+// to F, 2 (idiff2_FS, D20 = 2): only F (irhs + convection) into out, the implicit solve and
+// the stage combination follow (km_idiff, km_combine)
+template<int comp, int MODE>
 __global__ void __launch_bounds__(NT, 2) kb_momentum(TankSet b, int stage, int qin_, int qold_, int u_, int v_, int w_,
                                                    double grav, double cfac, int out_, double rw, double ra)
 {
@@ -455,7 +457,12 @@ __global__ void __launch_bounds__(NT, 2) kb_momentum(TankSet b, int stage, int q
             long c;
             if(!tile_cell_row(g, tw, t, i, j, k, c) || !comp_cell(g, comp, i, j, k)) continue;
             double F = momentum_F(g, comp, qin, u, v, w, press, phi, grav, rw, ra, c, i, j, k);
-            if constexpr(DIFF) F += ediff_term(g, comp, u, v, w, T.f[REEF_VISC], c, i, j, k);
+            if constexpr(MODE == 1) F += ediff_term(g, comp, u, v, w, T.f[REEF_VISC], c, i, j, k);
+            if constexpr(MODE == 2)
+            {
+                out[c] = F;
+                continue;
+            }
             const double df = qin[c];
             if(stage == 1)
                 out[c] = df + cdt * 1.0 * F;
@@ -1724,6 +1731,411 @@ __global__ void kb_fin_mix_ru(TankSet b)
     }
 }
 
+// ------------------------------------------------------------------ explicit 7-diagonal operator
+// bicgstab_ijk::start / bicgstab_ijk_2D::start (bicgstab_ijk.cpp:51-123, bicgstab_ijk_2D.cpp:
+// 49-94) on a caller's matrix_diag: M.p, M.n, M.s, M.w, M.e, M.t, M.b and the right-hand
+// side of every FLEXLOOP cell sit in field layout in TankDev::m[0..7], the solver's private
+// x in m[8].  The FLEXLOOP cells of var (looping.h:55-57,159) on a box are i < nx - ulast,
+// j < ny - vlast, k < nz - wlast (ulast / vlast / wlast = 1 for var 1 / 2 / 3, whose
+// flag1..3 drop the last row of their grid; var 4, 5: every interior cell).  As in the
+// reference, fillxvec copies the field into the private x on those cells only: its other
+// interior cells keep what the calls of other vars left there (they are read by the
+// restart's A x), its ghosts stay 0.  ph and sh are 0 outside the FLEXLOOP cells (the
+// reference zeroes them after every solve, bicgstab_ijk.cpp:271-275), so their neighbours
+// there contribute M * 0.0 and are not read.  The iteration is the state machine of the
+// Poisson solve (the same finalize kernels); 3-D tanks use the 7-point product of
+// matvec_std (bicgstab_ijk.cpp:296-310), 2-D tanks the 5-point one of bicgstab_ijk_2D.cpp:
+// 308-320.
+struct MatMask
+{
+    int ie, je, ke;    // exclusive ends of the FLEXLOOP box
+};
+
+__device__ __forceinline__ MatMask mat_mask(const DevGrid& g, int var)
+{
+    MatMask m;
+    m.ie = g.nx - (var == 1 ? 1 : 0);
+    m.je = g.ny - (var == 2 ? 1 : 0);
+    m.ke = g.nz - (var == 3 ? 1 : 0);
+    return m;
+}
+
+struct MatOp
+{
+    const double *p, *n, *s, *w, *e, *t, *b;
+};
+
+// the private x of the solve: m[8] for var 1..3 (psolv, the diffusion solves), m[9] for
+// var 4, 5 (ppoissonsolv, a separate bicgstab_ijk object in the driver)
+__device__ __forceinline__ int mat_xslot(int var) { return var <= 3 ? 8 : 9; }
+
+__device__ __forceinline__ MatOp mat_op(CTank& T)
+{
+    MatOp o;
+    o.p = T.m[0];
+    o.n = T.m[1];
+    o.s = T.m[2];
+    o.w = T.m[3];
+    o.e = T.m[4];
+    o.t = T.m[5];
+    o.b = T.m[6];
+    return o;
+}
+
+// (A y)(c); MASK: y(cc) = yv(cc) inside the FLEXLOOP box and 0 outside it (ph, sh), else
+// yv(cc) everywhere (the private x)
+template<bool MASK, class YV>
+__device__ __forceinline__ double mat_apply(const DevGrid& g, const MatOp& M, const MatMask& mk, YV yv, long c, int i,
+                                            int j, int k)
+{
+    const double yn = !MASK || i + 1 < mk.ie ? yv(c + 1) : 0.0;
+    const double ys = !MASK || i > 0 ? yv(c - 1) : 0.0;
+    const double yt = !MASK || k + 1 < mk.ke ? yv(c + g.sxy) : 0.0;
+    const double yb = !MASK || k > 0 ? yv(c - g.sxy) : 0.0;
+    if(g.jdir == 1)
+    {
+        const double yw = !MASK || j + 1 < mk.je ? yv(c + g.sx) : 0.0;
+        const double ye = !MASK || j > 0 ? yv(c - g.sx) : 0.0;
+        return M.p[c] * yv(c) + M.n[c] * yn + M.s[c] * ys + M.w[c] * yw + M.e[c] * ye + M.t[c] * yt + M.b[c] * yb;
+    }
+    return M.p[c] * yv(c) + M.n[c] * yn + M.s[c] * ys + M.t[c] * yt + M.b[c] * yb;
+}
+
+__device__ __forceinline__ bool mat_cell(const DevGrid& g, const MatMask& mk, const TileWalk& tw, int t, int& i, int& j,
+                                         int& k, long& c)
+{
+    return tile_cell(g, tw, t, i, j, k, c) && i < mk.ie && j < mk.je && k < mk.ke;
+}
+
+// restart label (bicgstab_ijk.cpp:134-148): r = rhs - A x, r0 = p = r, ph = aii p; on the
+// first pass also precon_setup (:356-364), aii = -1/(M.p + 1e-19)
+__device__ __forceinline__ void mat_restart(CTank& T, const DevGrid& g, const MatMask& mk, int var,
+                                            const TileWalk& tw, bool setup, double& rr)
+{
+    const double* __restrict__ x = T.m[mat_xslot(var)];
+    const MatOp M = mat_op(T);
+    const double* __restrict__ rhs = T.m[7];
+    double* __restrict__ r = T.f[A_R];
+    double* __restrict__ r0 = T.f[A_R0];
+    double* __restrict__ p = T.f[A_P];
+    double* __restrict__ ph = T.f[A_PH];
+    double* __restrict__ aii = T.f[A_AII];
+    #pragma unroll 1
+    for(int t = tw.t; t < tw.t1; ++t)
+    {
+        int i, j, k;
+        long c;
+        if(!mat_cell(g, mk, tw, t, i, j, k, c)) continue;
+        if(setup) aii[c] = -1.0 / (M.p[c] + 1.0e-19);
+        const double rv = rhs[c] - (mat_apply<false>(g, M, mk, [&](long cc) { return x[cc]; }, c, i, j, k));
+        r[c] = rv;
+        r0[c] = rv;
+        p[c] = rv;
+        ph[c] = rv * aii[c];
+        rr += rv * rv;
+    }
+}
+
+// fillxvec (bicgstab_ijk.cpp:366-378): private x = field f on the FLEXLOOP cells (dir 0);
+// finalize (:380-384): f = private x there (dir 1)
+__global__ void __launch_bounds__(NT) km_copy_x(TankSet b, int f_, int var, int dir)
+{
+    #pragma unroll 1
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
+    {
+        CTank& T = tank_at(b, q.p);
+        const DevGrid g = T.g;
+        const MatMask mk = mat_mask(g, var);
+        double* __restrict__ f = T.f[f_];
+        double* __restrict__ x = T.m[mat_xslot(var)];
+        const TileWalk tw = tile_walk(T, q.t);
+        int i, j, k;
+        long c;
+        if(!mat_cell(g, mk, tw, q.t, i, j, k, c)) continue;
+        if(dir == 0)
+            x[c] = f[c];
+        else
+            f[c] = x[c];
+    }
+}
+
+__global__ void __launch_bounds__(NT, OCC_A64) km_init(TankSet b, int var)
+{
+    #pragma unroll 1
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    {
+        CTank& T = tank_at(b, q.p);
+        const DevGrid g = T.g;
+        double vals[1] = {0.0};
+        mat_restart(T, g, mat_mask(g, var), var, item_walk(T, b, q), true, vals[0]);
+        block_sum_to<1>(vals, b.partials + (size_t)item_slot(T, b, q));
+    }
+}
+
+// the pending restart, or p = r + beta (p - w v), ph = aii p (bicgstab_ijk.cpp:234-235, 151)
+__global__ void __launch_bounds__(NT, OCC_A64) km_p(TankSet b, int var)
+{
+    #pragma unroll 1
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    {
+        CTank& T = tank_at(b, q.p);
+        const DevGrid g = T.g;
+        const BiState* st = T.st;
+        if(st->done || st->mode == 2)
+        {
+            skip_tank(q, b);
+            continue;
+        }
+        const MatMask mk = mat_mask(g, var);
+        const TileWalk tw = item_walk(T, b, q);
+        if(st->mode == 1)
+        {
+            double vals[1] = {0.0};
+            mat_restart(T, g, mk, var, tw, false, vals[0]);
+            block_sum_to<1>(vals, b.partials + (size_t)item_slot(T, b, q));
+            continue;
+        }
+        const double beta = st->beta, w = st->w;
+        const double* __restrict__ r = T.f[A_R];
+        const double* __restrict__ vv = T.f[A_V];
+        const double* __restrict__ aii = T.f[A_AII];
+        double* __restrict__ p = T.f[A_P];
+        double* __restrict__ ph = T.f[A_PH];
+        #pragma unroll 4
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!mat_cell(g, mk, tw, t, i, j, k, c)) continue;
+            const double pn = r[c] + beta * (p[c] - w * vv[c]);
+            p[c] = pn;
+            ph[c] = pn * aii[c];
+        }
+    }
+}
+
+// v = A ph; sigma = (v, r0), |v|^2, |r|^2 (bicgstab_ijk.cpp:151-160)
+__global__ void __launch_bounds__(NT, OCC_A64) km_v(TankSet b, int var)
+{
+    #pragma unroll 1
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    {
+        CTank& T = tank_at(b, q.p);
+        const DevGrid g = T.g;
+        const BiState* st = T.st;
+        if(st->done || st->mode == 1 || st->mode == 3)
+        {
+            skip_tank(q, b);
+            continue;
+        }
+        const MatMask mk = mat_mask(g, var);
+        const MatOp M = mat_op(T);
+        const TileWalk tw = item_walk(T, b, q);
+        const double* __restrict__ ph = T.f[A_PH];
+        const double* __restrict__ r = T.f[A_R];
+        const double* __restrict__ r0 = T.f[A_R0];
+        double* __restrict__ vo = T.f[A_V];
+        double vals[3] = {0.0, 0.0, 0.0};
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!mat_cell(g, mk, tw, t, i, j, k, c)) continue;
+            const double v = mat_apply<true>(g, M, mk, [&](long cc) { return ph[cc]; }, c, i, j, k);
+            vo[c] = v;
+            const double rc = r[c];
+            vals[0] += v * r0[c];
+            vals[1] += v * v;
+            vals[2] += rc * rc;
+        }
+        block_sum_to<3>(vals, b.partials + (size_t)item_slot(T, b, q) * 3);
+    }
+}
+
+// s = r - alpha v, sh = aii s (at the stencil points), t = A sh; (t, s), (t, t), (s, s)
+// (bicgstab_ijk.cpp:193-221)
+__global__ void __launch_bounds__(NT, OCC_A64) km_t(TankSet b, int var)
+{
+    #pragma unroll 1
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    {
+        CTank& T = tank_at(b, q.p);
+        const DevGrid g = T.g;
+        const BiState* st = T.st;
+        if(st->done || st->mode == 1 || st->mode == 3)
+        {
+            skip_tank(q, b);
+            continue;
+        }
+        const MatMask mk = mat_mask(g, var);
+        const MatOp M = mat_op(T);
+        const TileWalk tw = item_walk(T, b, q);
+        const double alpha = st->alpha;
+        const double* __restrict__ r = T.f[A_R];
+        const double* __restrict__ v = T.f[A_V];
+        const double* __restrict__ a = T.f[A_AII];
+        double* __restrict__ to = T.f[A_T];
+        double vals[3] = {0.0, 0.0, 0.0};
+        #pragma unroll 1
+        for(int tt = tw.t; tt < tw.t1; ++tt)
+        {
+            int i, j, k;
+            long c;
+            if(!mat_cell(g, mk, tw, tt, i, j, k, c)) continue;
+            const double sc = r[c] - alpha * v[c];
+            const double t =
+                mat_apply<true>(g, M, mk, [&](long cc) { return (r[cc] - alpha * v[cc]) * a[cc]; }, c, i, j, k);
+            to[c] = t;
+            vals[0] += t * sc;
+            vals[1] += t * t;
+            vals[2] += sc * sc;
+        }
+        block_sum_to<3>(vals, b.partials + (size_t)item_slot(T, b, q) * 3);
+    }
+}
+
+// x and r updates, (r, r0), |r|^2 (bicgstab_ijk.cpp:223-268)
+__global__ void __launch_bounds__(NT, OCC_FULL) km_x(TankSet b, int var)
+{
+    #pragma unroll 1
+    for(ItemIter q = set_items(b); q.q < q.hi; next_item(q, b))
+    {
+        CTank& T = tank_at(b, q.p);
+        const DevGrid g = T.g;
+        const BiState* st = T.st;
+        if(st->done || st->mode == 1 || st->mode == 3)
+        {
+            skip_tank(q, b);
+            continue;
+        }
+        const MatMask mk = mat_mask(g, var);
+        const TileWalk tw = item_walk(T, b, q);
+        const double alpha = st->alpha, w = st->w;
+        const int branch = st->branch;
+        double* __restrict__ xv = T.m[mat_xslot(var)];
+        double* __restrict__ r = T.f[A_R];
+        const double* __restrict__ vv = T.f[A_V];
+        const double* __restrict__ aii = T.f[A_AII];
+        const double* __restrict__ ph = T.f[A_PH];
+        const double* __restrict__ tv = T.f[A_T];
+        const double* __restrict__ r0 = T.f[A_R0];
+        double vals[2] = {0.0, 0.0};
+        #pragma unroll 4
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!mat_cell(g, mk, tw, t, i, j, k, c)) continue;
+            const double sv = r[c] - alpha * vv[c];
+            double rn;
+            if(branch)
+            {
+                const double sh = sv * aii[c];
+                xv[c] += alpha * ph[c] + w * sh;
+                rn = sv - w * tv[c];
+            }
+            else
+            {
+                xv[c] += alpha * ph[c];
+                rn = sv;
+            }
+            r[c] = rn;
+            vals[0] += rn * r0[c];
+            vals[1] += rn * rn;
+        }
+        block_sum_to<2>(vals, b.partials + (size_t)item_slot(T, b, q) * 2);
+    }
+}
+
+// ------------------------------------------------------------------ implicit diffusion (D20 = 2)
+// pdiff->diff_u / diff_v / diff_w of idiff2_FS (idiff2_FS_2D on 2-D tanks) inside a
+// momentum_FC3 stage (momentum_FC3.cpp:170-204, 272-306, 370-404): the operator rows and
+// right-hand side of component comp into TankDev::m[0..7], and fillxvec of the diff field
+// (= u_in on the loop, idiff2_FSu.cpp:47-48) into the private x m[8].  The direct-forcing
+// flags are set in TankDev::tstep[5] after the first stage-1 predictor of the run.
+template<int comp>
+__global__ void __launch_bounds__(NT, 2) km_idiff(TankSet b, double alpha, int qin_, int u_, int v_, int w_)
+{
+    #pragma unroll 1
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
+    {
+        CTank& T = tank_at(b, q.p);
+        const DevGrid g = T.g;
+        if(comp == 2 && g.jdir == 0) continue;
+        const MatMask mk = mat_mask(g, comp);
+        const double* __restrict__ vin = T.f[qin_];
+        const double* __restrict__ u = T.f[u_];
+        const double* __restrict__ v = T.f[v_];
+        const double* __restrict__ w = T.f[w_];
+        const double dt = T.tstep[0];
+        const bool df = T.tstep[5] != 0.0;
+        const TileWalk tw = tile_walk(T, q.t);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell_row(g, tw, t, i, j, k, c) || !comp_cell(g, comp, i, j, k)) continue;
+            const MatRow r = idiff_row(g, comp, alpha, dt, vin, u, v, w, T.f[REEF_U], T.f[REEF_V], T.f[REEF_VISC], df,
+                                       mk.ie, mk.je, mk.ke, c, i, j, k);
+            T.m[0][c] = r.p;
+            T.m[1][c] = r.n;
+            T.m[2][c] = r.s;
+            T.m[3][c] = r.w;
+            T.m[4][c] = r.e;
+            T.m[5][c] = r.t;
+            T.m[6][c] = r.b;
+            T.m[7][c] = r.rhs;
+            T.m[8][c] = vin[c];
+        }
+    }
+}
+
+// the stage combination of component comp with the solved diff (the private x, which
+// finalize would copy into diff): stage 1 urk1 = udiff + dt F, stage 2 urk2 = 0.75 u +
+// 0.25 udiff + 0.25 dt F, stage 3 u = u / 3 + 2/3 udiff + 2/3 dt F (momentum_FC3.cpp:
+// 172-174, 274-276, 372-374); F is the component's F / G / H
+template<int comp>
+__global__ void __launch_bounds__(NT) km_combine(TankSet b, int stage, int qold_, int F_, double cfac, int out_)
+{
+    #pragma unroll 1
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
+    {
+        CTank& T = tank_at(b, q.p);
+        const DevGrid g = T.g;
+        if(comp == 2 && g.jdir == 0) continue;
+        const double* __restrict__ qold = T.f[qold_];
+        const double* __restrict__ F = T.f[F_];
+        const double* __restrict__ df = T.m[8];
+        double* __restrict__ out = T.f[out_];
+        const double cdt = cfac * T.tstep[0];
+        const TileWalk tw = tile_walk(T, q.t);
+        #pragma unroll 1
+        for(int t = tw.t; t < tw.t1; ++t)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell_row(g, tw, t, i, j, k, c) || !comp_cell(g, comp, i, j, k)) continue;
+            if(stage == 1)
+                out[c] = df[c] + cdt * 1.0 * F[c];
+            else if(stage == 2)
+                out[c] = 0.75 * qold[c] + 0.25 * df[c] + cdt * 1.0 * F[c];
+            else
+                out[c] = (1.0 / 3.0) * qold[c] + (2.0 / 3.0) * df[c] + cdt * 1.0 * F[c];
+        }
+    }
+}
+
+// ghostcell::solid_forcing_flag_update at the end of momentum_forcing_start
+// (momentum_forcing.cpp:102): from here on the diffusion rows fold their wall couplings
+__global__ void km_df_set(TankSet b)
+{
+    const int tk = blockIdx.x * blockDim.x + threadIdx.x;
+    if(tk >= b.n) return;
+    tank_at(b, tk).tstep[5] = 1.0;
+}
+
 // ------------------------------------------------------------------ adaptive time step
 // ietimestep::start (ietimestep.cpp:38-207) and ::ini (:209-304) with N50 = 1, first pass:
 // per work item the largest |u| over the u loop (i < nx - 1), |v| over the v loop
@@ -1998,11 +2410,11 @@ void bl_momentum(const TankSet& b, int comp, int stage, int qin, int qold, int u
         hipLaunchKernelGGL(kernel, tgrid(b), dim3(NT), 0, s, b, stage, qin, qold, u, v, w, grav, cfac, out, rw, ra);
     };
     if(comp == 1)
-        diff ? go(kb_momentum<1, true>) : go(kb_momentum<1, false>);
+        diff == 2 ? go(kb_momentum<1, 2>) : diff ? go(kb_momentum<1, 1>) : go(kb_momentum<1, 0>);
     else if(comp == 2)
-        diff ? go(kb_momentum<2, true>) : go(kb_momentum<2, false>);
+        diff == 2 ? go(kb_momentum<2, 2>) : diff ? go(kb_momentum<2, 1>) : go(kb_momentum<2, 0>);
     else
-        diff ? go(kb_momentum<3, true>) : go(kb_momentum<3, false>);
+        diff == 2 ? go(kb_momentum<3, 2>) : diff ? go(kb_momentum<3, 1>) : go(kb_momentum<3, 0>);
 }
 void bl_heaviside(const TankSet& b, const double* prm4, hipStream_t s)
 {
@@ -2049,6 +2461,58 @@ void bl_bicg_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStr
                     hipLaunchKernelGGL((kb_bicg_t<double, false>), pgrid_direct(b, OCC_A64), dim3(NT), 0, s, b, x, rhs);
                 break;
             default: hipLaunchKernelGGL(kb_bicg_x<double>, grid, dim3(NT), 0, s, b, x, rhs); break;
+        }
+        return;
+    }
+    switch(phase)
+    {
+        case 0: hipLaunchKernelGGL(kb_fin_p, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 1: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 2: hipLaunchKernelGGL(kb_fin_t, tankgrid(b), dim3(NT), 0, s, b); break;
+        default: hipLaunchKernelGGL(kb_fin_x, tankgrid(b), dim3(NT), 0, s, b); break;
+    }
+}
+
+void bl_idiff(const TankSet& b, int comp, double alpha, int qin, int u, int v, int w, hipStream_t s)
+{
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, tgrid(b), dim3(NT), 0, s, b, alpha, qin, u, v, w); };
+    comp == 1 ? go(km_idiff<1>) : comp == 2 ? go(km_idiff<2>) : go(km_idiff<3>);
+}
+
+void bl_idiff_combine(const TankSet& b, int comp, int stage, int qold, int F, double cfac, int out, hipStream_t s)
+{
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, tgrid(b), dim3(NT), 0, s, b, stage, qold, F, cfac, out); };
+    comp == 1 ? go(km_combine<1>) : comp == 2 ? go(km_combine<2>) : go(km_combine<3>);
+}
+
+void bl_df_set(const TankSet& b, hipStream_t s)
+{
+    hipLaunchKernelGGL(km_df_set, setup_grid(b), dim3(64), 0, s, b);
+}
+
+void bl_mat_copy_x(const TankSet& b, int f, int var, int dir, hipStream_t s)
+{
+    hipLaunchKernelGGL(km_copy_x, tgrid(b), dim3(NT), 0, s, b, f, var, dir);
+}
+
+void bl_mat_init(const TankSet& b, int var, double stop, int maxiter, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_setup, setup_grid(b), dim3(64), 0, s, b, stop, maxiter, 0, 0.0);
+    hipLaunchKernelGGL(km_init, pgrid(b, OCC_A64), dim3(NT), 0, s, b, var);
+    hipLaunchKernelGGL(kb_fin_init, tankgrid(b), dim3(NT), 0, s, b);
+}
+
+void bl_mat_phase(const TankSet& b, int var, int phase, int part, hipStream_t s)
+{
+    const dim3 grid64 = pgrid(b, OCC_A64);
+    if(part == 0)
+    {
+        switch(phase)
+        {
+            case 0: hipLaunchKernelGGL(km_p, grid64, dim3(NT), 0, s, b, var); break;
+            case 1: hipLaunchKernelGGL(km_v, grid64, dim3(NT), 0, s, b, var); break;
+            case 2: hipLaunchKernelGGL(km_t, grid64, dim3(NT), 0, s, b, var); break;
+            default: hipLaunchKernelGGL(km_x, pgrid(b), dim3(NT), 0, s, b, var); break;
         }
         return;
     }

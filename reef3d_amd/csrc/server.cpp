@@ -94,6 +94,7 @@ struct reef_server
     std::string fatal_msg;
     long steps = 0;
     size_t bytes_in_use = 0;
+    size_t peak_bytes = 0;            // largest bytes_in_use since creation
     std::thread worker;
     std::vector<std::thread> builders;
     int device = 0;
@@ -160,6 +161,7 @@ int reef_server::round(std::vector<RequestPtr>& admitted)
             live.push_back(r);
         }
         bytes_in_use = E.pool.in_use();
+        peak_bytes = std::max(peak_bytes, bytes_in_use);
     }
     for(RequestPtr& r : admitted)
         if((rc = E.start_steps(r->tank, r->steps))) return rc;
@@ -228,7 +230,13 @@ void reef_server::build_loop()
         p.dt = r->dt;
         std::unique_ptr<TankBuild> b(new TankBuild());
         std::string err;
-        const int rc = Engine::build_tank(r->gd, p, *b, err);
+        int rc = Engine::build_tank(r->gd, p, *b, err);
+        const size_t exact = rc ? 0 : Engine::built_bytes(*b);
+        if(!rc && cfg.hbm_budget_bytes > 0.0 && (double)exact > cfg.hbm_budget_bytes)
+        {
+            rc = REEF_E_NOMEM;
+            err = "tank larger than the HBM budget";
+        }
         {
             std::lock_guard<std::mutex> lk(mu);
             if(rc)
@@ -237,7 +245,10 @@ void reef_server::build_loop()
                 r->err = err;
             }
             else
+            {
                 r->built = std::move(b);
+                r->bytes = exact;
+            }
             r->ready = true;
         }
         cv_work.notify_all();
@@ -373,7 +384,9 @@ int reef_server_submit(reef_server* s, const reef_grid_desc* grid, double dt, in
     if(!s || !grid || !grid->XN || !grid->YN || !grid->ZN || !phi || !u || !v || !w || !press || !id)
         return sfail(REEF_E_ARG, "null argument");
     if(steps < 1 || !(dt > 0.0)) return sfail(REEF_E_ARG, "steps must be >= 1 and dt > 0");
-    const size_t bytes = Engine::tank_bytes(*grid);
+    // the field arrays alone (a lower bound, cheap); the builder replaces it by the exact
+    // bytes, ghost programs included, before the request can be admitted
+    const size_t bytes = Engine::tank_bytes_min(*grid);
     if(bytes == 0) return sfail(REEF_E_UNSUPPORTED, "grid outside the implemented subset");
     if(s->cfg.hbm_budget_bytes > 0.0 && (double)bytes > s->cfg.hbm_budget_bytes)
         return sfail(REEF_E_NOMEM, "tank larger than the HBM budget");
@@ -479,6 +492,14 @@ int reef_server_stats(reef_server* s, int* queued, int* live, long* steps, doubl
     if(steps) *steps = s->steps;
     if(bytes_in_use) *bytes_in_use = (double)s->bytes_in_use;
     if(s->fatal) return sfail(s->fatal, s->fatal_msg);
+    return 0;
+}
+
+int reef_server_peak_bytes(reef_server* s, double* peak)
+{
+    if(!s || !peak) return sfail(REEF_E_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *peak = (double)s->peak_bytes;
     return 0;
 }
 

@@ -73,11 +73,15 @@ def library():
         "reef_ghost_fill": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "reef_convection": ([vp] + [ctypes.c_int] * 5, ctypes.c_int),
         "reef_reini": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "reef_reini_ini": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "reef_fluid_update": ([vp, dp], ctypes.c_int),
         "reef_poisson_solve": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ip, dp], ctypes.c_int),
         "reef_poisson_solve_mixed": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ip, ip, dp],
                                      ctypes.c_int),
         "reef_poisson_residual": ([vp, ctypes.c_int, ctypes.c_int, dp], ctypes.c_int),
+        "reef_solver_start": ([vp, ctypes.c_int, ctypes.POINTER(dp), dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                               ip, dp], ctypes.c_int),
+        "reef_solver_veclen": ([vp, ctypes.c_int], ctypes.c_long),
         "reef_solver_config": ([vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "reef_solver_last": ([vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long), dp], ctypes.c_int),
         "reef_pjm_corr": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ip], ctypes.c_int),
@@ -129,6 +133,7 @@ def library():
         "reef_server_request_info": ([vp, ctypes.c_long, dp, ip, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "reef_server_release": ([vp, ctypes.c_long], ctypes.c_int),
         "reef_server_stats": ([vp, ip, ip, ctypes.POINTER(ctypes.c_long), dp], ctypes.c_int),
+        "reef_server_peak_bytes": ([vp, dp], ctypes.c_int),
         "reef_host_grid_arrays": ([ctypes.POINTER(GridDesc), ctypes.c_double, ctypes.c_int] + [dp] * 7, ctypes.c_int),
         "reef_host_ghost_apply": ([ctypes.POINTER(GridDesc), ctypes.c_double, ctypes.c_int, ctypes.c_int, dp],
                                   ctypes.c_int),
@@ -341,6 +346,13 @@ class Domain:
         """reini::start with p->count > 0 and p->F44 = iters."""
         _check(library().reef_reini(self.h, FIELDS[name], iters))
 
+    def reini_ini(self, name="phi", iters=None, gcv=50, maxlength=None, F43=0.55, DXM=None):
+        """reini::start with p->count == 0: start4(phi, gcv), then 2*int(maxlength/(F43*DXM))
+        pseudo steps (or iters) with the ghost rule gcv (reini_RK3.cpp:88-101)."""
+        if iters is None:
+            iters = 2 * int(maxlength / (F43 * DXM))
+        _check(library().reef_reini_ini(self.h, FIELDS[name], iters, gcv))
+
     def fluid_update(self):
         vol = np.zeros(2)
         _check(library().reef_fluid_update(self.h, _dp(vol)))
@@ -361,6 +373,22 @@ class Domain:
         _check(library().reef_poisson_solve_mixed(self.h, FIELDS[x], FIELDS[rhs], stop, maxiter,
                                                   ctypes.byref(it), ctypes.byref(nru), _dp(rel)))
         return it.value, nru.value, float(rel[0])
+
+    def solver_start(self, var, M7, rhs, x, stop, maxiter):
+        """solver::start(p, a, pgc, f, rhsvec, var) on the caller's matrix_diag: M7 rows
+        (M.p, M.n, M.s, M.w, M.e, M.t, M.b) and rhs in vec order (the FLEXLOOP cells of var,
+        i outermost); field x holds the initial guess and receives the solution.  Returns
+        (iterations, final ||r||_2 / cells)."""
+        n = library().reef_solver_veclen(self.h, int(var))
+        rows = [np.ascontiguousarray(M7[q], dtype=np.float64) for q in range(7)]
+        b = np.ascontiguousarray(rhs, dtype=np.float64)
+        if any(r.shape != (n,) for r in rows) or b.shape != (n,):
+            raise ValueError(f"operator rows and rhs must have {n} entries for var {var}")
+        ptrs = (ctypes.POINTER(ctypes.c_double) * 7)(*(_dp(r) for r in rows))
+        it, res = ctypes.c_int(), np.zeros(1)
+        _check(library().reef_solver_start(self.h, int(var), ptrs, _dp(b), FIELDS[x], float(stop), int(maxiter),
+                                           ctypes.byref(it), _dp(res)))
+        return it.value, float(res[0])
 
     def residual(self, x, rhs):
         """||rhs - A x||_2 in FP64 on the device (operator assembled from phi)."""
@@ -662,4 +690,6 @@ class Server:
     def stats(self):
         q, lv, st, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_long(), np.zeros(1)
         _scheck(library().reef_server_stats(self.h, ctypes.byref(q), ctypes.byref(lv), ctypes.byref(st), _dp(b)))
-        return dict(queued=q.value, live=lv.value, rounds=st.value, bytes_in_use=float(b[0]))
+        pk = np.zeros(1)
+        _scheck(library().reef_server_peak_bytes(self.h, _dp(pk)))
+        return dict(queued=q.value, live=lv.value, rounds=st.value, bytes_in_use=float(b[0]), peak_bytes=float(pk[0]))

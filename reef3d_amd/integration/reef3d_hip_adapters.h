@@ -147,7 +147,10 @@ public:
     }
 };
 
-// reini::start for F40 = 3 (reini_RK3 with reinidisc_f), p->count > 0
+// reini::start for F40 = 3 (reini_RK3, reinidisc_f or reinidisc_fsf by F49): at
+// p->count == 0 the initial reinitialisation of driver_ini_cfd (2*int(maxlength/(F43 DXM))
+// pseudo steps with gcval_iniphi, reini_RK3.cpp:88-101), afterwards p->F44 steps with
+// gcval_phi
 class hip_reini : public reini
 {
 public:
@@ -158,8 +161,21 @@ public:
         (void)pgc;
         (void)pflow;
         reef_ctx* c = context(p);
+        if(p->F50 != 2)
+        {
+            std::fprintf(stderr, "reef3d_hip: hip_reini supports F50 = 2 only\n");
+            std::exit(1);
+        }
         put(c, REEF_WORK1, f);
-        check(reef_reini(c, REEF_WORK1, p->F44), "reef_reini");
+        if(p->count == 0)
+        {
+            // gcval_iniphi of reini_RK3::reini_RK3 (reini_RK3.cpp:44-57) for F50 = 2
+            const int gcv = (p->F62 > 1.0e-20 || p->F60 > 1.0e-20) ? 52 : 50;
+            const int iters = 2 * int(p->maxlength / (p->F43 * p->DXM));
+            check(reef_reini_ini(c, REEF_WORK1, iters, gcv), "reef_reini_ini");
+        }
+        else
+            check(reef_reini(c, REEF_WORK1, p->F44), "reef_reini");
         get(c, REEF_WORK1, f);
     }
 };

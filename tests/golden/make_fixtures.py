@@ -15,7 +15,8 @@ import tempfile
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-CASES = ["box", "nonuni", "vortex", "tank", "tank2d", "tankfsf", "tankvisc", "tank2dvisc"]
+CASES = ["box", "nonuni", "vortex", "tank", "tank2d", "tankfsf", "tankvisc", "tank2dvisc", "box2d", "tankidiff",
+         "tank2didiff"]
 
 
 def main(cases):

@@ -137,6 +137,8 @@ def test_batch_slots_and_budget():
     assert c == a                      # the freed slot is reused
     slots, active, used = B.info()
     assert slots == 2 and active == 2 and used <= 2.5 * need
+    # tank_bytes is exactly what a tank takes from the device pool, ghost programs included
+    assert used == 2 * need
     # a tank added after others were stepped is advanced only once made active with them
     for t in (b, c):
         _initial(B, t, kw0, f0)
@@ -204,3 +206,24 @@ def test_scheduled_run_identical_to_lockstep(precision, graphs):
     for q in range(len(TANKS)):
         for n in FIELDS:
             assert np.array_equal(sched[q][n], lock[q][n]), f"tank {q} {n}: scheduled != lockstep"
+
+
+def test_batch_refuses_differing_shared_parameters():
+    """Only dt and F45 may differ between the tanks of a batch; the shared values stay
+    those of the first tank added to the empty batch, also after that tank is removed."""
+    from reef3d_amd import Batch, ReefError
+    kw0, _ = TANKS[0]
+    kw2, _ = TANKS[2]
+    B = Batch()
+    a = B.add(**kw0)
+    b = B.add(**dict(kw2, dt=0.001, F45=1.5))
+    for bad in (dict(rho=(998.0, 1.205)), dict(g=(0.0, 0.0, -9.8)), dict(N44=-1e-9), dict(F43=0.5),
+                dict(N46=100), dict(visc=(2.0e-6, 1.4938e-5))):
+        with pytest.raises(ReefError):
+            B.add(**dict(kw2, **bad))
+    B.remove(a)
+    with pytest.raises(ReefError):
+        B.add(**dict(kw2, rho=(998.0, 1.205)))
+    B.remove(b)
+    B.add(**dict(kw2, rho=(998.0, 1.205)))   # an empty batch takes new shared values
+    B.close()

@@ -116,6 +116,56 @@ def test_pjm_corr_within_solver_tolerance():
         np.testing.assert_allclose(got, ref, rtol=0, atol=1e-9 * np.abs(ref).max(), err_msg=n)
 
 
+@pytest.mark.parametrize("gcv", [50, 52])
+def test_reini_initial_bitexact(gcv):
+    """reini::start at p->count == 0 (reini_RK3.cpp:88-101): start4(phi, gcval_iniphi), then
+    pseudo steps with that ghost rule, bit-exact against the oracle."""
+    from oracle.oracle import Domain as ODomain
+    d = load("nonuni")
+    D = dev("nonuni", d)
+    O = ODomain(**domain_args("nonuni", d))
+    phi = d["hj_phi"].copy()
+    D.put("phi", phi)
+    D.reini_ini("phi", iters=4, gcv=gcv)
+    O.ghost(phi, 4, gcv)
+    O.reini(phi, 4, gcv)
+    assert bitexact(D.get("phi"), phi)
+
+
+@pytest.mark.parametrize("case", ["box", "nonuni", "box2d"])
+def test_solver_start_on_reference_operator(case):
+    """solver::start(p, a, pgc, f, rhsvec, var) on the matrix_diag and rhsvec that the
+    reference's idiff2_FS(_2D) handed to bicgstab_ijk(_2D), var 1..3 with the N43 stop, in the
+    reference's call order (the solver's private x carries over between the calls): the
+    iteration count within one of the reference's and the solved cells within the solver
+    tolerance; the cells outside the FLEXLOOP of var keep the caller's values."""
+    d = load(case)
+    D = dev(case, d)
+    sc = d["id_scalars"]
+    N43, maxiter = float(sc[2]), int(sc[3])
+    nx, ny, nz = (CASES[case][k] for k in ("nx", "ny", "nz"))
+    comps = (1, 3) if CASES[case]["jdir"] == 0 else (1, 2, 3)
+    for df_pass, comp in ((p, c) for p in (0, 1) for c in comps):
+        tag = f"id{comp}_{df_pass}"
+        vin = d["id_" + "uvw"[comp - 1]]
+        it_ref, var, n = (int(x) for x in d[tag + "_it"])
+        M = d[tag + "_M"].copy()
+        if CASES[case]["jdir"] == 0:
+            M[3:5] = 0.0                          # M.w, M.e are not set in 2-D
+        D.put("work0", vin)
+        it, res = D.solver_start(comp, M, d[tag + "_rhs"], "work0", N43, maxiter)
+        assert abs(it - it_ref) <= 1, f"{tag}: {it} iterations against {it_ref}"
+        got, ref = inner(D.get("work0")), inner(d[tag + "_diff"])
+        ie, je, ke = nx - (comp == 1), ny - (comp == 2), nz - (comp == 3)
+        sel = (slice(0, ke), slice(0, je), slice(0, ie))
+        scale = np.abs(ref[sel]).max() or 1.0
+        err = np.abs(got[sel] - ref[sel]).max() / scale
+        assert err < 1e-10, f"{tag}: solved cells rel err {err:.2e}"
+        outside = np.ones(got.shape, bool)
+        outside[sel] = False
+        assert np.array_equal(got[outside], inner(vin)[outside])
+
+
 def test_bicgstab_first_iterations_match_reference_path():
     """The first iterations follow the reference's alpha/omega/beta sequence to rounding."""
     d = load("box")
@@ -140,7 +190,7 @@ def test_bicgstab_first_iterations_match_reference_path():
 SOLVED_RTOL = 1e-8
 
 
-@pytest.mark.parametrize("case", ["tank", "tank2d", "tankfsf", "tankvisc", "tank2dvisc"])
+@pytest.mark.parametrize("case", ["tank", "tank2d", "tankfsf", "tankvisc", "tank2dvisc", "tankidiff", "tank2didiff"])
 def test_fc3_step_matches_reference(case):
     d = load(case)
     D = dev(case, d)

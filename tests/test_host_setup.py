@@ -48,3 +48,40 @@ def test_unsupported_geometry_is_rejected():
                          jdir=1)
     with pytest.raises(ReefError):
         host_ghost_apply(gd, 4, 52, np.zeros((14, 7, 14)))
+
+
+def test_field_size_limit_of_32bit_offsets():
+    """The solver walks address FP64 arrays through 32-bit byte offsets: a field of 2^29 or
+    more doubles (here a 1024 x 512 x 1024 tank, which would fit in HBM) is refused at
+    set-up instead of addressing the wrong memory."""
+    from reef3d_amd.hotpath import Batch, ReefError, grid_desc, host_grid_arrays
+    from tests._cases import uniform_nodes
+    n = (1024, 512, 1024)
+    nodes = [uniform_nodes(m, 0.01) for m in n]
+    gd, keep = grid_desc(*n, *nodes, 0.01, 0.01, jdir=1, top_group=3)
+    with pytest.raises(ReefError, match="too large"):
+        host_grid_arrays(gd, 0, n[0])
+    assert Batch.tank_bytes(*n, *nodes, 0.01, 0.01, jdir=1, top_group=3) == 0
+    # just below the limit the set-up is accepted
+    n = (1000, 500, 1000)
+    nodes = [uniform_nodes(m, 0.01) for m in n]
+    gd, keep = grid_desc(*n, *nodes, 0.01, 0.01, jdir=1, top_group=3)
+    P = host_grid_arrays(gd, 0, n[0])[0]
+    assert P.shape == (n[0] + 10,)
+
+
+def test_tank_bytes_include_ghost_programs():
+    """reef_batch_tank_bytes counts the compiled ghost programs (16 bytes per op, about
+    5-6 % of a 3-D tank), not only the field arrays."""
+    from reef3d_amd.hotpath import Batch
+    from tests._cases import uniform_nodes
+    nx, ny, nz = 128, 32, 64
+    nodes = [uniform_nodes(m, 0.02) for m in (nx, ny, nz)]
+    total = Batch.tank_bytes(nx, ny, nz, *nodes, 0.02, 0.02, jdir=1, top_group=3)
+    sx = (nx + 6 + 15) // 16 * 16
+    nalloc = (16 + sx * (ny + 6) * (nz + 6) + 64 + 63) // 64 * 64
+    fields = 38 * nalloc * 8 + 13 * nalloc * 4
+    # every ghost program writes at least one ghost layer over the 6 faces
+    faces = 2 * (nx * ny + ny * nz + nx * nz)
+    assert total >= fields + 13 * faces * 16
+    assert total < 1.2 * fields

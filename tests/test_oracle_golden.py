@@ -130,7 +130,42 @@ def test_bicgstab_iteration_path(it):
     assert same(inner(x), inner(d[f"bs_x_it{it}"]))
 
 
-@pytest.mark.parametrize("case", ["tank", "tank2d", "tankfsf", "tankvisc", "tank2dvisc"])
+@pytest.mark.parametrize("case", ["box", "nonuni", "box2d"])
+def test_implicit_diffusion_operator_and_solve(case):
+    """idiff2_FS (3-D) / idiff2_FS_2D (2-D) diff_u/v/w: the matrix_diag rows and rhsvec the
+    reference hands to solver::start, the bicgstab_ijk(_2D) solve with var 1..3 (its masks,
+    N43 stop and the private x it keeps between calls) and the field after the closing
+    ghost fill, bit-exact, in the reference's call order; pass 0 with the direct-forcing
+    flags unset (no wall folding), pass 1 with them set."""
+    from oracle.oracle import Domain
+    d = load(case)
+    sc = d["id_scalars"]
+    # the time step the reference held then (the fixture's earlier ietimestep calls set it)
+    O = Domain(**dict(domain_args(case, d), dt=float(sc[0])))
+    alpha, N43 = float(sc[1]), float(sc[2])
+    assert float(sc[4]) == 1.0                     # D22
+    O.set_control("N43", N43)
+    u, v, w, visc = (d["id_" + n] for n in ("u", "v", "w", "visc"))
+    assert not d["id_eddyv"].any()
+    comps = (1, 3) if CASES[case]["jdir"] == 0 else (1, 2, 3)
+    for df_pass, comp in ((p, c) for p in (0, 1) for c in comps):
+        O.set_control("df_set", df_pass)
+        tag = f"id{comp}_{df_pass}"
+        vin = (u, v, w)[comp - 1]
+        M, rhs = O.idiff_assemble(comp, alpha, vin, u, v, w, visc)
+        it_ref, var, n = (int(x) for x in d[tag + "_it"])
+        assert var == comp and M.shape == (7, n)
+        rows = (0, 1, 2, 5, 6) if CASES[case]["jdir"] == 0 else range(7)   # 2-D leaves M.w, M.e unset
+        for q in rows:
+            assert same(M[q], d[tag + "_M"][q]), f"{tag} row {q}"
+        assert same(rhs, d[tag + "_rhs"]), f"{tag} rhs"
+        diff = np.zeros(O.shape)
+        it = O.idiff(comp, alpha, diff, vin, u, v, w, visc)
+        assert it == it_ref
+        assert same(inner(diff), inner(d[tag + "_diff"])), f"{tag} diff"
+
+
+@pytest.mark.parametrize("case", ["tank", "tank2d", "tankfsf", "tankvisc", "tank2dvisc", "tankidiff", "tank2didiff"])
 def test_momentum_fc3_steps(case):
     d = load(case)
     O = oracle(case, d)
