@@ -48,6 +48,10 @@ struct orc_domain
     // the matvec of a restart reads them (bicgstab_ijk.cpp:280-294, 366-378)
     std::vector<double> psolv_x;
     int dot_ext = 0;            // 1: BiCGStab dot products accumulated in long double
+    // threads of the per-cell loops (stencils, stage combinations, BiCGStab sweeps): 1 keeps
+    // every result bit-exact with the reference; more keep the per-cell results and sum the
+    // BiCGStab dot products per i slab (a different rounding); volume sums stay serial
+    int threads = 1;
     long sx, sy, sz;
     int group[7];
     orc_params prm;
@@ -505,6 +509,7 @@ double flux_face(const orc_domain* d, const double* f, int i, int j, int k, int 
 // flux_HJ_CDS2 ipol=4 cell-centred velocities (flux_HJ_CDS2.cpp:59-63,90-95,122-127)
 void hj(orc_domain* d, const double* phi, const double* u, const double* v, const double* w, double* L)
 {
+    #pragma omp parallel for schedule(static) num_threads(d->threads) if(d->threads > 1)
     for(int i = 0; i < d->nx; ++i)
     for(int j = 0; j < d->ny; ++j)
     for(int k = 0; k < d->nz; ++k)
@@ -531,6 +536,7 @@ void flux(orc_domain* d, int ipol, const double* q, const double* u, const doubl
     const double* DY = ipol == 2 ? d->DP[1].data() : d->DN[1].data();
     const double* DZ = ipol == 3 ? d->DP[2].data() : d->DN[2].data();
     int ufx = ipol == 1, ufy = ipol == 2, ufz = ipol == 3;
+    #pragma omp parallel for schedule(static) num_threads(d->threads) if(d->threads > 1)
     for(int i = 0; i < d->nx - il; ++i)
     for(int j = 0; j < d->ny - jl; ++j)
     for(int k = 0; k < d->nz - kl; ++k)
@@ -582,6 +588,7 @@ void flux(orc_domain* d, int ipol, const double* q, const double* u, const doubl
 void reini_disc(orc_domain* d, const double* f, double* L)
 {
     const long sxy = d->sx * d->sy;
+    #pragma omp parallel for schedule(static) num_threads(d->threads) if(d->threads > 1)
     for(int i = 0; i < d->nx; ++i)
     for(int j = 0; j < d->ny; ++j)
     for(int k = 0; k < d->nz; ++k)
@@ -646,6 +653,7 @@ void reini(orc_domain* d, double* f, int iters, int gcv = 52)
         dtf[d->id(i, j, k)] = d->prm.F43 * m;
     }
     auto interior = [&](auto fn) {
+        #pragma omp parallel for schedule(static) num_threads(d->threads) if(d->threads > 1)
         for(int i = 0; i < d->nx; ++i)
         for(int j = 0; j < d->ny; ++j)
         for(int k = 0; k < d->nz; ++k)
@@ -775,6 +783,74 @@ struct Bicg
             ++n;
         }
     }
+    // vec index of the first FLEXLOOP cell of every i slab (for the slab-parallel loops)
+    std::vector<long> n0;
+    void slabs()
+    {
+        const int il = var == 1 ? d->last[1] : 0;
+        n0.assign(d->nx - il + 1, 0);
+        loop([&](long c, long n) {
+            const int i = (int)(c % d->sx) - 3;
+            if(n0[i + 1] <= n) n0[i + 1] = n + 1;
+        });
+        for(int i = 1; i <= d->nx - il; ++i)
+            if(n0[i] < n0[i - 1]) n0[i] = n0[i - 1];
+    }
+    // the cells of loop(), i slabs in parallel when the domain runs more than one thread;
+    // fn writes only the cell it is given
+    template<class F> void ploop(F fn)
+    {
+        const int il = var == 1 ? d->last[1] : 0, jl = var == 2 ? d->last[2] : 0, kl = var == 3 ? d->last[3] : 0;
+        const std::vector<int>& fl = d->flag[var <= 3 ? var : 4];
+        #pragma omp parallel for schedule(static) num_threads(d->threads) if(d->threads > 1)
+        for(int i = 0; i < d->nx - il; ++i)
+        {
+            long n = n0[i];
+            for(int j = 0; j < d->ny - jl; ++j)
+            for(int k = 0; k < d->nz - kl; ++k)
+            {
+                const long c = d->id(i, j, k);
+                if(fl[c] <= 0) continue;
+                fn(c, n);
+                ++n;
+            }
+        }
+    }
+    // K sums over the cells of loop(): with one thread in the reference's serial loop order
+    // (bit-exact); with more, per-slab partial sums added in slab order (deterministic for
+    // a given thread count, a different rounding from the reference's)
+    template<class A, int K, class F> void psum(A* out, F fn)
+    {
+        if(d->threads <= 1)
+        {
+            A acc[K];
+            for(int q = 0; q < K; ++q) acc[q] = 0.0;
+            loop([&](long c, long n) { fn(c, n, acc); });
+            for(int q = 0; q < K; ++q) out[q] = acc[q];
+            return;
+        }
+        const int il = var == 1 ? d->last[1] : 0, jl = var == 2 ? d->last[2] : 0, kl = var == 3 ? d->last[3] : 0;
+        const std::vector<int>& fl = d->flag[var <= 3 ? var : 4];
+        const int ni = d->nx - il;
+        std::vector<A> part((size_t)ni * K, (A)0.0);
+        #pragma omp parallel for schedule(static) num_threads(d->threads)
+        for(int i = 0; i < ni; ++i)
+        {
+            A* acc = part.data() + (size_t)i * K;
+            long n = n0[i];
+            for(int j = 0; j < d->ny - jl; ++j)
+            for(int k = 0; k < d->nz - kl; ++k)
+            {
+                const long c = d->id(i, j, k);
+                if(fl[c] <= 0) continue;
+                fn(c, n, acc);
+                ++n;
+            }
+        }
+        for(int q = 0; q < K; ++q) out[q] = 0.0;
+        for(int i = 0; i < ni; ++i)
+            for(int q = 0; q < K; ++q) out[q] += part[(size_t)i * K + q];
+    }
     double Ax(const std::vector<double>& xx, long c, long n) const
     {
         const long sx = d->sx, sxy = d->sx * d->sy;
@@ -790,29 +866,30 @@ struct Bicg
     // solution moves when a different summation order is used (a device reduction)
     template<class A> double res_calc()
     {
-        A resi = 0.0;
-        loop([&](long c, long n) {
+        A resi[1];
+        psum<A, 1>(resi, [&](long c, long n, A* a) {
             double y = rhs[c] - (Ax(x, c, n));
-            resi += y * y;
+            a[0] += y * y;
         });
-        return sqrt((double)resi) / double(d->cellnum);
+        return sqrt((double)resi[0]) / double(d->cellnum);
     }
     int solve(double stop, int maxiter, double* resout)
     {
+        slabs();
         return d->dot_ext ? solve_t<long double>(stop, maxiter, resout) : solve_t<double>(stop, maxiter, resout);
     }
     template<class A> int solve_t(double stop, int maxiter, double* resout)
     {
         int solveriter = 0;
         double residual = 1.0e9;
-        loop([&](long c, long n) { aii[c] = -1.0 / (M[n] + 1.0e-19); });
+        ploop([&](long c, long n) { aii[c] = -1.0 / (M[n] + 1.0e-19); });
         double r_j, norm_r0, r_j1 = 0.0;
     restart:
         {
-            A acc = 0.0;
-            loop([&](long c, long n) { rj[c] = rhs[c] - (Ax(x, c, n)); });
-            loop([&](long c, long) { r0[c] = pj[c] = rj[c]; acc += rj[c] * r0[c]; });
-            r_j = (double)acc;
+            A acc[1];
+            ploop([&](long c, long n) { rj[c] = rhs[c] - (Ax(x, c, n)); });
+            psum<A, 1>(acc, [&](long c, long, A* a) { r0[c] = pj[c] = rj[c]; a[0] += rj[c] * r0[c]; });
+            r_j = (double)acc[0];
         }
         norm_r0 = sqrt(r_j);
         // negative stop: relative to the initial residual (harness mapping of a relative
@@ -822,17 +899,17 @@ struct Bicg
         {
             do
             {
-                A sig = 0.0, nvj = 0.0, nrj = 0.0;
-                loop([&](long c, long) { ph[c] = pj[c] * aii[c]; });
-                loop([&](long c, long n) { vj[c] = Ax(ph, c, n); });
-                loop([&](long c, long) {
-                    sig += vj[c] * r0[c];
-                    nvj += vj[c] * vj[c];
-                    nrj += rj[c] * rj[c];
+                A s3[3];
+                ploop([&](long c, long) { ph[c] = pj[c] * aii[c]; });
+                ploop([&](long c, long n) { vj[c] = Ax(ph, c, n); });
+                psum<A, 3>(s3, [&](long c, long, A* a) {
+                    a[0] += vj[c] * r0[c];
+                    a[1] += vj[c] * vj[c];
+                    a[2] += rj[c] * rj[c];
                 });
-                const double sigma = (double)sig;
-                const double norm_vj = sqrt((double)nvj);
-                const double norm_rj = sqrt((double)nrj);
+                const double sigma = (double)s3[0];
+                const double norm_vj = sqrt((double)s3[1]);
+                const double norm_rj = sqrt((double)s3[2]);
                 double alpha = r_j / sigma;
                 if(fabs(sigma) <= (1.0e-12 * (norm_vj * norm_r0)))
                 {
@@ -846,46 +923,46 @@ struct Bicg
                     ++solveriter;
                     goto restart;
                 }
-                A nsj = 0.0;
-                loop([&](long c, long) {
+                A nsj[1];
+                psum<A, 1>(nsj, [&](long c, long, A* a) {
                     sj[c] = rj[c] - alpha * vj[c];
-                    nsj += sj[c] * sj[c];
+                    a[0] += sj[c] * sj[c];
                 });
-                const double norm_sj = sqrt((double)nsj);
+                const double norm_sj = sqrt((double)nsj[0]);
                 if(norm_sj > stop)
                 {
-                    loop([&](long c, long) { sh[c] = sj[c] * aii[c]; });
-                    loop([&](long c, long n) { tj[c] = Ax(sh, c, n); });
-                    A w1 = 0.0, w2 = 0.0;
-                    loop([&](long c, long) {
-                        w1 += tj[c] * sj[c];
-                        w2 += tj[c] * tj[c];
+                    ploop([&](long c, long) { sh[c] = sj[c] * aii[c]; });
+                    ploop([&](long c, long n) { tj[c] = Ax(sh, c, n); });
+                    A ww[2];
+                    psum<A, 2>(ww, [&](long c, long, A* a) {
+                        a[0] += tj[c] * sj[c];
+                        a[1] += tj[c] * tj[c];
                     });
-                    double w = (double)w1 / ((double)w2 == 0 ? 1.0e-15 : (double)w2);
-                    A rj1 = 0.0;
-                    loop([&](long c, long) {
+                    double w = (double)ww[0] / ((double)ww[1] == 0 ? 1.0e-15 : (double)ww[1]);
+                    A rj1[1];
+                    psum<A, 1>(rj1, [&](long c, long, A* a) {
                         x[c] += alpha * ph[c] + w * sh[c];
                         rj[c] = sj[c] - w * tj[c];
-                        rj1 += rj[c] * r0[c];
+                        a[0] += rj[c] * r0[c];
                     });
-                    r_j1 = (double)rj1;
+                    r_j1 = (double)rj1[0];
                     double beta = alpha * r_j1 / (w * r_j == 0 ? 1.0e-15 : (w * r_j));
-                    loop([&](long c, long) { pj[c] = rj[c] + beta * (pj[c] - w * vj[c]); });
+                    ploop([&](long c, long) { pj[c] = rj[c] + beta * (pj[c] - w * vj[c]); });
                 }
                 if(norm_sj <= stop)
                 {
-                    A rj1 = 0.0;
-                    loop([&](long c, long) {
+                    A rj1[1];
+                    psum<A, 1>(rj1, [&](long c, long, A* a) {
                         x[c] += alpha * ph[c];
                         rj[c] = sj[c];
-                        rj1 += rj[c] * r0[c];
+                        a[0] += rj[c] * r0[c];
                     });
-                    r_j1 = (double)rj1;
+                    r_j1 = (double)rj1[0];
                 }
                 r_j = r_j1;
-                A res = 0.0;
-                loop([&](long c, long) { res += rj[c] * rj[c]; });
-                residual = sqrt((double)res) / double(d->cellnum);
+                A res[1];
+                psum<A, 1>(res, [&](long c, long, A* a) { a[0] += rj[c] * rj[c]; });
+                residual = sqrt((double)res[0]) / double(d->cellnum);
                 ++solveriter;
             } while((residual >= stop) && (solveriter < maxiter));
         }
@@ -1029,8 +1106,24 @@ template<class F> void vel_loop(orc_domain* d, int comp, F fn)
     }
 }
 
+// vel_loop with i slabs in parallel when the domain runs threads > 1 (fn writes only cell c)
+template<class F> void vel_loop_p(orc_domain* d, int comp, F fn)
+{
+    int il = comp == 1 ? d->last[1] : 0, jl = comp == 2 ? d->last[2] : 0, kl = comp == 3 ? d->last[3] : 0;
+    const std::vector<int>& fl = d->flag[comp];
+    #pragma omp parallel for schedule(static) num_threads(d->threads) if(d->threads > 1)
+    for(int i = 0; i < d->nx - il; ++i)
+    for(int j = 0; j < d->ny - jl; ++j)
+    for(int k = 0; k < d->nz - kl; ++k)
+    {
+        long c = d->id(i, j, k);
+        if(fl[c] > 0) fn(c, i, j, k);
+    }
+}
+
 template<class F> void cell_loop(orc_domain* d, F fn)
 {
+    #pragma omp parallel for schedule(static) num_threads(d->threads) if(d->threads > 1)
     for(int i = 0; i < d->nx; ++i)
     for(int j = 0; j < d->ny; ++j)
     for(int k = 0; k < d->nz; ++k)
@@ -1254,18 +1347,18 @@ struct FC3
         long off = comp == 1 ? 1 : comp == 2 ? d->sx : d->sx * d->sy;
         const double* D = d->DP[comp - 1].data();
         double g = comp == 1 ? d->prm.gi : comp == 2 ? d->prm.gj : d->prm.gk;
-        vel_loop(d, comp, [&](long c, int i, int j, int k) { F[c] = 0.0; });
+        vel_loop_p(d, comp, [&](long c, int i, int j, int k) { F[c] = 0.0; });
         // pjm_corr::upgrad/vpgrad/wpgrad (pjm_corr.cpp:185-204)
-        vel_loop(d, comp, [&](long c, int i, int j, int k) {
+        vel_loop_p(d, comp, [&](long c, int i, int j, int k) {
             int ix = comp == 1 ? i : comp == 2 ? j : k;
             F[c] -= 1.0 * (press[c + off] - press[c]) / (D[ix + MARGE] * roface(d, phi, c, off));
         });
         // irhs/jrhs/krhs (momentum_FC3.cpp:432-472), rhsvec = 0, Fext = 0, W29 = 0
-        vel_loop(d, comp, [&](long c, int, int, int) { F[c] += (0.0 + g + 0.0 + 0.0) * 1.0; });
+        vel_loop_p(d, comp, [&](long c, int, int, int) { F[c] += (0.0 + g + 0.0 + 0.0) * 1.0; });
         flux(d, comp, q, uu, vv, ww, F.data());
         // pdiff->diff_u / diff_v / diff_w (momentum_FC3.cpp:170, 187, 204 and the stages)
         if(d->D20 == 1)
-            vel_loop(d, comp, [&](long c, int i, int j, int k) { F[c] += ediff_term(d, comp, uu, vv, ww, nu, c, i, j, k); });
+            vel_loop_p(d, comp, [&](long c, int i, int j, int k) { F[c] += ediff_term(d, comp, uu, vv, ww, nu, c, i, j, k); });
     }
 
     int step(double* phi, double* u, double* v, double* w, double* press, double* ro, double* visc, double* vol)
@@ -1350,14 +1443,14 @@ struct FC3
                 idiff(d, comp, alpha, df.data(), in[comp], uu, vv, ww, un, vn, nu);
             else
             {
-                vel_loop(d, comp, [&](long c, int, int, int) { df[c] = in[comp][c]; });
+                vel_loop_p(d, comp, [&](long c, int, int, int) { df[c] = in[comp][c]; });
                 ghost(d, df.data(), comp, comp == 1 ? 10 : comp == 2 ? 11 : 12);
             }
             std::vector<double>& o = *out[comp];
             if(kind == 0)
-                vel_loop(d, comp, [&](long c, int, int, int) { o[c] = df[c] + dt * 1.0 * F[c]; });
+                vel_loop_p(d, comp, [&](long c, int, int, int) { o[c] = df[c] + dt * 1.0 * F[c]; });
             else
-                vel_loop(d, comp, [&](long c, int, int, int) {
+                vel_loop_p(d, comp, [&](long c, int, int, int) {
                     o[c] = 0.75 * old[comp][c] + 0.25 * df[c] + 0.25 * dt * 1.0 * F[c];
                 });
         }
@@ -1379,11 +1472,11 @@ struct FC3
                 idiff(d, comp, 2.0 / 3.0, df.data(), in[comp], urk2.data(), vrk2.data(), wrk2.data(), u, v, nu);
             else
             {
-                vel_loop(d, comp, [&](long c, int, int, int) { df[c] = in[comp][c]; });
+                vel_loop_p(d, comp, [&](long c, int, int, int) { df[c] = in[comp][c]; });
                 ghost(d, df.data(), comp, comp == 1 ? 10 : comp == 2 ? 11 : 12);
             }
             double* t = tgt[comp];
-            vel_loop(d, comp, [&](long c, int, int, int) {
+            vel_loop_p(d, comp, [&](long c, int, int, int) {
                 t[c] = (1.0 / 3.0) * t[c] + (2.0 / 3.0) * df[c] + (2.0 / 3.0) * dt * 1.0 * F[c];
             });
         }
@@ -1600,6 +1693,11 @@ int orc_set_control(orc_domain* d, const char* key, double value)
     if(std::string(key) == "D20" && (value == 0.0 || value == 1.0 || value == 2.0))
     {
         d->D20 = (int)value;
+        return 0;
+    }
+    if(std::string(key) == "threads" && value >= 1.0 && value <= 256.0)
+    {
+        d->threads = (int)value;
         return 0;
     }
     if(std::string(key) == "dot_ext" && (value == 0.0 || value == 1.0))

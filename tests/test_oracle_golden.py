@@ -202,3 +202,38 @@ def test_adaptive_timestep(case):
     for ini, key in ((False, "ts_start"), (True, "ts_ini")):
         got = O.timestep(d["ts_u"], d["ts_v"], d["ts_w"], N47=float(d[key][4]), ini=ini)
         assert got == tuple(float(x) for x in d[key][:4]), key
+
+
+def test_threaded_oracle_vortex_bitexact():
+    """The oracle with 4 threads (per-cell loops over i slabs in parallel) still reproduces
+    the reference's level-set benchmark steps bit for bit: those steps hold no dot product."""
+    d = load("vortex")
+    O = oracle("vortex", d)
+    O.set_control("threads", 4)
+    phi = d["vx_phi_0"].copy()
+    u, v, w, ro, visc = (np.zeros(O.shape) for _ in range(5))
+    dt = CASES["vortex"]["dt"]
+    for s in range(1, 5):
+        vol = O.vortex_step(phi, u, v, w, ro, visc, (s - 1) * dt)
+        assert same(phi, d[f"vx_phi_{s}"]), f"step {s}"
+        assert same(vol, d[f"vx_vol_{s}"][:2])
+
+
+@pytest.mark.parametrize("case", ["tank", "tank2didiff"])
+def test_threaded_oracle_fc3_steps(case):
+    """With 4 threads the BiCGStab dot products are summed per i slab, so the steps leave the
+    reference's bits only through the solves: within the solver tolerance of the fixtures,
+    and identical from run to run."""
+    d = load(case)
+    runs = []
+    for _ in range(2):
+        O = oracle(case, d)
+        O.set_control("threads", 4)
+        f = {n: d[f"tk_{n}_0"].copy() for n in ("phi", "u", "v", "w", "press", "ro", "visc")}
+        O.fc3_step(f["phi"], f["u"], f["v"], f["w"], f["press"], f["ro"], f["visc"])
+        runs.append(f)
+    for n in ("phi", "u", "v", "w", "press"):
+        assert same(runs[0][n], runs[1][n]), n
+        ref = inner(d[f"tk_{n}_1"])
+        err = np.abs(inner(runs[0][n]) - ref).max() / (np.abs(ref).max() or 1.0)
+        assert err < 1e-8, f"{n}: {err:.2e}"
