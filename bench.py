@@ -152,6 +152,11 @@ def cpu_baseline(cfg, threads=None):
     }
 
 
+def note(msg):
+    """progress line on stderr (the JSON line alone goes to stdout)"""
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
 def timed_steps(D, steps):
     """steps momentum_FC3 steps with per-kernel HIP event timing; (seconds, iterations, families)"""
     D.sync()
@@ -663,6 +668,7 @@ def main():
     if args.workload == "serve":
         serve(args)
         return
+    note("fp64 wave tank: set-up")
     D, _ = make_domain(cfg)
     cells = cfg["nx"] * cfg["ny"] * cfg["nz"]
     vol_init = D.fluid_update()
@@ -670,6 +676,7 @@ def main():
         D.momentum_fc3()
     vol_start = D.fluid_update()
     elapsed, iters, fams, vol_end = timed_steps(D, args.steps)
+    note(f"fp64 wave tank: {args.steps} timed steps, {elapsed / args.steps * 1e3:.1f} ms/step")
     dom = max(("bicg_p", "bicg_v", "bicg_t", "bicg_x"), key=lambda k: fams[k][0])
     ms, n = fams[dom]
     avg_s = ms / 1e3 / max(n, 1)
@@ -728,6 +735,7 @@ def main():
         for _ in range(args.warmup):
             Dm.momentum_fc3()
         el_m, it_m, fams_m, vol_m = timed_steps(Dm, args.steps)
+        note(f"mixed wave tank: {args.steps} timed steps, {el_m / args.steps * 1e3:.1f} ms/step")
         _, _, rel = Dm.solver_last()
         dm = max(("mix_v", "bicg32_t", "bicg32_x"), key=lambda k: fams_m[k][0])
         ms_m, n_m = fams_m[dm]
@@ -746,6 +754,7 @@ def main():
         del Dm
     if not args.no_solves:
         # C2: standalone solves on the step-0 operator, FP64 against mixed precision
+        note(f"standalone solves: {args.solves} right-hand sides")
         sv = run_solves(args, cfg)
         out["mixed_solves"] = {k: sv[k] for k in ("solves", "fp64", "mixed", "speedup_mixed_vs_fp64",
                                                   "reliable_update_delta", "max_rel_p_minus_p64")}
@@ -753,11 +762,13 @@ def main():
         out["mixed_solves"]["graphs"] = sv["config"]["graphs"]
     if not args.no_serve:
         # C4: a bounded sample of the served stream against sequential plain FP64 solves
+        note("served stream")
         sr = run_serve(args)
         out["served"] = {"solves_per_s": sr["value"], "latency_s": sr["latency_s"], "makespan_s": sr["makespan_s"],
                          "sequential_fp64": sr["sequential_fp64"], "speedup_vs_sequential": sr["speedup_vs_sequential"],
                          "tank_steps": sr["tank_steps"], "workload": sr["config"]["workload"]}
     if rank == 0 and not args.no_cpu:
+        note("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(cfg)
     if rank == 0:
         print(json.dumps(out), flush=True)

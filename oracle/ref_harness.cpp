@@ -16,6 +16,16 @@
 //          vortex (C0-style level-set benchmark, 16^3, a few steps)
 //          tank (momentum_FC3 wave-tank steps, 16x6x12, free-surface top)
 //          tank2d (momentum_FC3 wave-tank steps, 24x1x12, j_dir=0)
+//
+// Built with -DREEF_ADAPTERS (oracle/_ref/adapter_harness, linked with the GPU library)
+// it takes a third argument that runs the tank cases through the reference-side adapters
+// of reef3d_amd/integration instead of the reference's own operators, inside the
+// reference's own momentum_FC3 and driver set-up:
+//   ops   hip_weno_flux, hip_weno_hj, hip_reini, and hip_bicgstab for every solve
+//         (poisson_pcorr still assembles a->M on the host)
+//   press hip_pjm_corr for the projection (and hip_bicgstab for the diffusion solves)
+//   step  hip_momentum_FC3: the whole step on the device
+// The dumps have the names of the plain run, so a test compares them with its fixture.
 #include<mpi.h>
 #include<cstdio>
 #include<cstdlib>
@@ -73,6 +83,9 @@
 #include"initialize.h"
 #include"cfd_state.h"
 #include"sediment_void.h"
+#ifdef REEF_ADAPTERS
+#include"reef3d_hip_adapters.h"
+#endif
 
 using std::string;
 using std::vector;
@@ -363,6 +376,7 @@ static Env setup(const Case& c)
 // ---------------------------------------------------------------- dump helpers
 static int G_nx, G_ny, G_nz;
 static int G_f49 = 1;       // reinit discretisation of the tank runs (F49)
+static string G_adapt;      // adapter mode of the tank runs (REEF_ADAPTERS builds)
 
 static vector<long> fshape() { return {G_nz + 6, G_ny + 6, G_nx + 6}; }
 
@@ -916,7 +930,42 @@ static void run_tank(Env& e, int nsteps, double depth, double amp, double wlen)
     p->F49 = G_f49;
     reini_RK3 ri(p, 1);
     fsi_void fsi(p, pgc);
-    momentum_FC3 mom(p, a, pgc, &conv, &fsfdisc, pdiff, &press, &pois, &turb, &dsolv, &solv, e.pflow, ph, pc, &ri, &fsi);
+    convection* pconv = &conv;
+    convection* pfsf = &fsfdisc;
+    reini* preini = &ri;
+    pressure* ppress = &press;
+    solver* psolv = &dsolv;
+    solver* ppsolv = &solv;
+#ifdef REEF_ADAPTERS
+    using namespace reef_adapt;
+    hip_bicgstab* hsolv = nullptr;
+    hip_bicgstab* hpsolv = nullptr;
+    if(G_adapt == "ops" || G_adapt == "press")
+    {
+        hsolv = new hip_bicgstab(p);
+        psolv = hsolv;
+    }
+    if(G_adapt == "ops")
+    {
+        pconv = new hip_weno_flux(p);
+        pfsf = new hip_weno_hj(p);
+        preini = new hip_reini(p);
+        hpsolv = new hip_bicgstab(p);
+        ppsolv = hpsolv;
+    }
+    if(G_adapt == "press") ppress = new hip_pjm_corr(p);
+    if(!G_adapt.empty() && G_adapt != "ops" && G_adapt != "press" && G_adapt != "step")
+    {
+        fprintf(stderr, "unknown adapter mode %s\n", G_adapt.c_str());
+        exit(2);
+    }
+#endif
+    momentum_FC3 mom_ref(p, a, pgc, pconv, pfsf, pdiff, ppress, &pois, &turb, psolv, ppsolv, e.pflow, ph, pc, preini,
+                         &fsi);
+    momentum* pmom = &mom_ref;
+#ifdef REEF_ADAPTERS
+    if(G_adapt == "step") pmom = new hip_momentum_FC3(p);
+#endif
     vrans_v vr(p, pgc);
     sixdof_void sd(p, pgc);
     sd.initialize(p, a, pgc);
@@ -934,7 +983,7 @@ static void run_tank(Env& e, int nsteps, double depth, double amp, double wlen)
     for(int s = 1; s <= nsteps; ++s)
     {
         ++p->count;
-        mom.start(p, a, pgc, &vr, &sd);
+        pmom->start(p, a, pgc, &vr, &sd);
         p->simtime += p->dt;
         string t = std::to_string(s);
         dump_field("tk_phi_" + t, a->phi);
@@ -1015,6 +1064,15 @@ int main(int argc, char** argv)
     }
     string cname = argv[1];
     g_out = argv[2];
+    if(argc > 3)
+    {
+#ifdef REEF_ADAPTERS
+        G_adapt = argv[3];
+#else
+        fprintf(stderr, "adapter modes need the REEF_ADAPTERS build (oracle/_ref/adapter_harness)\n");
+        return 2;
+#endif
+    }
     mkdir(g_out.c_str(), 0755);
     char cwd[4096];
     if(g_out[0] != '/' && getcwd(cwd, sizeof(cwd)))

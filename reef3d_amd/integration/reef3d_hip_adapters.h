@@ -22,9 +22,13 @@
 #include "field.h"
 #include "ghostcell.h"
 #include "lexer.h"
+#include "matrix_diag.h"
 #include "momentum.h"
+#include "pressure.h"
 #include "reini.h"
+#include "solver.h"
 #include "timestep.h"
+#include "vec.h"
 #include "reef3d_hip.h"
 
 namespace reef_adapt {
@@ -97,7 +101,8 @@ inline reef_ctx* context(lexer* p)
         check(reef_ctx_create(&d, &r, &ctx), "reef_ctx_create");
         // the scheme controls the case chose (reini_RK3.cpp:68-72, driver_logic_cfd.cpp:332-335)
         check(reef_set_control(ctx, "F49", p->F49 == 0 ? 0.0 : 1.0), "F49");
-        if(p->D20 == 1) check(reef_set_control(ctx, "D20", 1.0), "D20");
+        if(p->D20 == 1 || p->D20 == 2) check(reef_set_control(ctx, "D20", double(p->D20)), "D20");
+        if(p->D20 == 2) check(reef_set_control(ctx, "N43", p->N43), "N43");
     }
     return ctx;
 }
@@ -177,6 +182,96 @@ public:
         else
             check(reef_reini(c, REEF_WORK1, p->F44), "reef_reini");
         get(c, REEF_WORK1, f);
+    }
+};
+
+// solver::start for N10 = 1 (bicgstab_ijk, or bicgstab_ijk_2D on a 2-D case; driver_logic_cfd.cpp:
+// 464-478) on the caller's a->M and rhsvec: the seven diagonals and the right-hand side
+// go to the device over the vec of var, the field is the initial guess and receives the
+// solution on the FLEXLOOP cells of var (bicgstab_ijk.cpp:51-123).  The driver keeps two
+// solver objects, psolv (var 1..3) and ppoissonsolv (var 4, 5); the device keeps one
+// private x for each of the two roles, so two instances of this class behave like them.
+// The other entry points are not on the hot path (solver.h:43-46): the reference's own
+// startf / startF / startM are empty, startV (sigma grids) is refused.
+class hip_bicgstab : public solver
+{
+public:
+    explicit hip_bicgstab(lexer* p) { context(p); }
+    void start(lexer* p, fdm* a, ghostcell* pgc, field& f, vec& rhsvec, int var) override
+    {
+        (void)pgc;
+        reef_ctx* c = context(p);
+        p->preconiter = 0;
+        const double* M7[7] = {a->M.p, a->M.n, a->M.s, a->M.w, a->M.e, a->M.t, a->M.b};
+        const double stop = var <= 3 ? p->N43 : p->N44;
+        put(c, REEF_WORK2, f);
+        int iters = 0;
+        double res = 0.0;
+        check(reef_solver_start(c, var, M7, rhsvec.V, REEF_WORK2, stop, p->N46, &iters, &res), "reef_solver_start");
+        get(c, REEF_WORK2, f);
+        p->solveriter = iters;
+    }
+    void startf(lexer*, ghostcell*, field&, vec&, matrix_diag&, int) override {}
+    void startF(lexer*, ghostcell*, double*, vec&, matrix_diag&, int) override {}
+    void startV(lexer*, ghostcell*, double*, vec&, matrix_diag&, int) override
+    {
+        std::fprintf(stderr, "reef3d_hip: hip_bicgstab::startV (sigma grids) is not supported\n");
+        std::exit(1);
+    }
+    void startM(lexer*, ghostcell*, double*, double*, double*, int) override {}
+};
+
+// pressure::start for D30 = 1 (pjm_corr with poisson_pcorr, driver_logic_cfd.cpp:385-395):
+// the projection of u, v, w with the face densities of a->phi, assembled, solved and
+// applied on the device (pjm_corr.cpp:90-121); a->press is updated and p->solveriter /
+// p->poissoniter receive the iterations.  The ppois and psolv the driver passes are not
+// used: the assembly and the solve are part of the device projection.  The pressure-
+// gradient members serve the sigma-grid and 2-D shallow-water paths only and are refused.
+class hip_pjm_corr : public pressure
+{
+public:
+    explicit hip_pjm_corr(lexer* p) { context(p); }
+    void start(fdm* a, lexer* p, poisson* ppois, solver* psolv, ghostcell* pgc, ioflow* pflow, field& u, field& v,
+               field& w, double alpha) override
+    {
+        (void)ppois;
+        (void)psolv;
+        (void)pgc;
+        (void)pflow;
+        reef_ctx* c = context(p);
+        check(reef_set_timestep(c, p->dt), "reef_set_timestep");
+        put(c, REEF_U, u);
+        put(c, REEF_V, v);
+        put(c, REEF_W, w);
+        put(c, REEF_PRESS, a->press);
+        put(c, REEF_PHI, a->phi);
+        int iters = 0;
+        check(reef_pjm_corr(c, REEF_U, REEF_V, REEF_W, alpha, &iters), "reef_pjm_corr");
+        get(c, REEF_U, u);
+        get(c, REEF_V, v);
+        get(c, REEF_W, w);
+        get(c, REEF_PRESS, a->press);
+        p->solveriter = iters;
+        p->poissoniter = iters;
+    }
+    // pjm_corr::ini sets up the pressure reference of B30 (pressure_reference_ini.cpp:28);
+    // the device projection has none (B30 = 0, reference_start a no-op)
+    void ini(lexer* p, fdm*, ghostcell*) override
+    {
+        if(p->B30 != 0) refuse("ini with B30 != 0");
+    }
+    void upgrad(lexer*, fdm*, slice&, slice&) override { refuse("upgrad"); }
+    void vpgrad(lexer*, fdm*, slice&, slice&) override { refuse("vpgrad"); }
+    void wpgrad(lexer*, fdm*, slice&, slice&) override { refuse("wpgrad"); }
+    void ucorr(lexer*, fdm*, field&, double) override { refuse("ucorr"); }
+    void vcorr(lexer*, fdm*, field&, double) override { refuse("vcorr"); }
+    void wcorr(lexer*, fdm*, field&, double) override { refuse("wcorr"); }
+
+private:
+    static void refuse(const char* what)
+    {
+        std::fprintf(stderr, "reef3d_hip: hip_pjm_corr::%s is not on the supported path\n", what);
+        std::exit(1);
     }
 };
 
