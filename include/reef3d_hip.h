@@ -74,6 +74,30 @@ typedef struct reef_params
     int N46;                    /* maximum BiCGStab iterations */
 } reef_params;
 
+/* Wave generation by relaxation zones and the numerical beach (iowave with B98 = 2, B99 =
+ * 1, 2; iowave.h:44, iowave_relax.cpp:28-395, iowave_precalc_relax.cpp:27-160) for linear
+ * waves (wave_lib_linear, B92 = 2): the ctrl.txt keys it reads.  The zones are the
+ * reference's defaults (B106 = B107 = B108 = 0): a generation band of half width B96_1
+ * around x = xmin and a beach band of half width B96_2 around x = xmax, the wave's origin
+ * at the domain's lower corner (B105 = 0) with direction B105_1. */
+typedef struct reef_wave_desc
+{
+    int B92;                    /* wave theory: 2 (linear) */
+    int B91;                    /* 1: wave height B91_1 and length B91_2 given */
+    double B91_1, B91_2;
+    int B93;                    /* 1: wave height B93_1 and period B93_2 given */
+    double B93_1, B93_2;
+    double B96_1, B96_2;        /* half widths of the generation and the beach zone */
+    int B98;                    /* 2: relaxation generation, 0: none */
+    int B99;                    /* beach: 0 none, 1 u, v, w, p, 2 also phi */
+    int B101;                   /* ramp: 0 none, 1 over B102 periods, 2 over B102 seconds */
+    double B102;
+    double B105_1;              /* wave direction in degrees (0: along +x) */
+    double B120;                /* phase shift in degrees (ini.cpp: -90) */
+    double F60;                 /* still-water level (p->phimean) */
+    double W50;                 /* air inflow velocity (0) */
+} reef_wave_desc;
+
 typedef struct reef_ctx reef_ctx;
 
 /* ---- REEF3D case files (host only; no device work) */
@@ -87,9 +111,13 @@ void reef_grid_file_free(reef_grid_file* file);
 /* the hot-path keys of a ctrl.txt (control::read_control, read_control.cpp:28-70) over the
  * defaults of ini.cpp: prm gets W1-W4, W20-W22, F43, F45, N44, N46 and N49 as dt; ctrl5
  * (may be null) gets F49, N48, N47, N40, D20.  Keys choosing another scheme than the hot
- * path's (D10, D20 other than 0 or 1, D30, F30, F35, F40, N10, N40, M10, and D20 = 1 with
- * N48 = 1) give REEF_E_UNSUPPORTED. */
+ * path's (D10, D20 other than 0, 1 or 2, D30, F30, F35, F40, N10, N40, M10, and D20 = 1
+ * with N48 = 1) give REEF_E_UNSUPPORTED. */
 int reef_ctrl_read(const char* path, reef_params* prm, double* ctrl5);
+/* the values of one ctrl.txt key (letter, number; e.g. 'N', 43 or 'B', 96) from the last
+ * line that sets it (read_control.cpp:28-70): up to capacity values into values, *count
+ * = the number of values on that line, 0 when no line sets the key */
+int reef_ctrl_key(const char* path, char letter, int num, double* values, int capacity, int* count);
 /* message of the last failed reef_grid_read / reef_ctrl_read of this thread */
 const char* reef_case_last_error(void);
 

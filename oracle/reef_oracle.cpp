@@ -52,6 +52,19 @@ struct orc_domain
     // every result bit-exact with the reference; more keep the per-cell results and sum the
     // BiCGStab dot products per i slab (a different rounding); volume sums stay serial
     int threads = 1;
+    // iowave relaxation-zone wave generation (B98 = 2) and numerical beach (B99), linear
+    // waves; on = 0: the void ioflow of the other cases
+    struct Wave
+    {
+        int on = 0;
+        orc_wave_desc w;
+        double wa, wk, ww, wdt, wT, wL, pshift, singamma, cosgamma, phimean, psi, epsi, tan_alpha, x0, y0;
+        double G[4][2], Gs[2], Ge[2], B[4][2], Bs[2], Be[2];   // zone corners and lines
+        double dist2_fac;
+        std::vector<double> eta;                // slice4 eta (i, j), zero until written
+        std::vector<double> uval, vval, wval, lsval;   // per cell of the precalc
+    } wave;
+    double simtime = 0.0;       // p->simtime of the wave runs
     long sx, sy, sz;
     int group[7];
     orc_params prm;
@@ -1323,6 +1336,386 @@ int idiff(orc_domain* d, int comp, double alpha, double* diff, const double* vin
     return it;
 }
 
+// ------------------------------------------------------------------ iowave (B98 = 2)
+// iowave::intriangle (iowave_dist_ini.cpp:141-190)
+int intriangle(double Ax, double Ay, double Bx, double By, double Cx, double Cy, double x0, double y0)
+{
+    double Px = x0, Py = y0, Pz = -10.0, Qx = x0, Qy = y0, Qz = +10.0;
+    double PQx = Qx - Px, PQy = Qy - Py, PQz = Qz - Pz;
+    double Mx = PQy * Pz - PQz * Py;
+    double My = PQz * Px - PQx * Pz;
+    double u = PQz * (Cx * By - Cy * Bx) + Mx * (Cx - Bx) + My * (Cy - By);
+    double v = PQz * (Ax * Cy - Ay * Cx) + Mx * (Ax - Cx) + My * (Ay - Cy);
+    double w = PQz * (Bx * Ay - By * Ax) + Mx * (Bx - Ax) + My * (By - Ay);
+    int check = 1;
+    if(u == 0.0 && v == 0.0 && w == 0.0) check = 0;
+    // the reference's condition, && binding tighter than ||
+    if((u >= 0.0 && v >= 0.0 && w >= 0.0) || ((u < 0.0 && v < 0.0 && w < 0.0) && check == 1)) return 1;
+    return 0;
+}
+
+double wmin(double a, double b) { return a < b ? a : b; }
+double wmax(double a, double b) { return a > b ? a : b; }
+
+// distbeach_ini / distgen_ini (iowave_dist_ini.cpp:27-139): corners 1..4 of one zone
+void zone_corners(const double* S, const double* E, double dist, double (*C)[2])
+{
+    double dx = E[0] - S[0], dy = E[1] - S[1];
+    double n1x = -dy, n1y = dx, n2x = dy, n2y = -dx;
+    double l1 = sqrt(pow(n1x, 2.0) + pow(n1y, 2.0));
+    double l2 = sqrt(pow(n2x, 2.0) + pow(n2y, 2.0));
+    l1 = fabs(l1) > 1.0e-20 ? l1 : 1.0e20;
+    l2 = fabs(l2) > 1.0e-20 ? l2 : 1.0e20;
+    C[0][0] = S[0] + (n1x / l1) * dist;
+    C[0][1] = S[1] + (n1y / l1) * dist;
+    C[1][0] = S[0] + (n2x / l2) * dist;
+    C[1][1] = S[1] + (n2y / l2) * dist;
+    C[2][0] = E[0] + (n1x / l1) * dist;
+    C[2][1] = E[1] + (n1y / l1) * dist;
+    C[3][0] = E[0] + (n2x / l2) * dist;
+    C[3][1] = E[1] + (n2y / l2) * dist;
+}
+
+// distgen / distbeach (iowave_dist.cpp:130-190) at (x0, y0): the distance to the zone's line
+// inside the zone, 1e20 outside
+double zone_dist(const double (*C)[2], const double* S, const double* E, double x0, double y0)
+{
+    double dist = 1.0e20;
+    int test1 = intriangle(C[0][0], C[0][1], C[2][0], C[2][1], C[1][0], C[1][1], x0, y0);
+    int test2 = intriangle(C[2][0], C[2][1], C[3][0], C[3][1], C[1][0], C[1][1], x0, y0);
+    if(test1 == 1 || test2 == 1)
+    {
+        double denom = sqrt(pow(E[1] - S[1], 2.0) + pow(E[0] - S[0], 2.0));
+        denom = denom > 1.0e-20 ? denom : 1.0e20;
+        dist = wmin(fabs((E[1] - S[1]) * x0 - (E[0] - S[0]) * y0 + E[0] * S[1] - E[1] * S[0]) / denom, dist);
+    }
+    return dist;
+}
+
+const double EE_REF = 2.71828182846;   // definitions.h:38
+
+// rb1_ext (iowave_relax_functions.cpp:28-95), one generation zone
+double rb1_ext(const orc_domain* d, double x0, double y0)
+{
+    const orc_domain::Wave& W = d->wave;
+    double x = -1.0e20, r = 0.0;
+    int t1 = intriangle(W.G[0][0], W.G[0][1], W.G[2][0], W.G[2][1], W.G[1][0], W.G[1][1], x0, y0);
+    int t2 = intriangle(W.G[2][0], W.G[2][1], W.G[3][0], W.G[3][1], W.G[1][0], W.G[1][1], x0, y0);
+    if(!(t1 == 1 || t2 == 1)) return 1.0;
+    double denom = sqrt(pow(W.Ge[1] - W.Gs[1], 2.0) + pow(W.Ge[0] - W.Gs[0], 2.0));
+    denom = denom > 1.0e-20 ? denom : 1.0e20;
+    double xdist = wmin(fabs((W.Ge[1] - W.Gs[1]) * x0 - (W.Ge[0] - W.Gs[0]) * y0 + W.Ge[0] * W.Gs[1] - W.Ge[1] * W.Gs[0]) /
+                            denom,
+                        1.0e20);
+    x = wmax(1.0 - xdist / W.w.B96_1, x);
+    x = wmax(x, 0.0);
+    r = 1.0 - (exp(pow(x, 3.5)) - 1.0) / (EE_REF - 1.0);
+    return r;
+}
+
+// rb3_ext (iowave_relax_functions.cpp:143-200), one beach zone
+double rb3_ext(const orc_domain* d, double x0, double y0)
+{
+    const orc_domain::Wave& W = d->wave;
+    int t1 = intriangle(W.B[0][0], W.B[0][1], W.B[2][0], W.B[2][1], W.B[1][0], W.B[1][1], x0, y0);
+    int t2 = intriangle(W.B[2][0], W.B[2][1], W.B[3][0], W.B[3][1], W.B[1][0], W.B[1][1], x0, y0);
+    if(!(t1 == 1 || t2 == 1)) return 1.0;
+    double denom = sqrt(pow(W.Be[1] - W.Bs[1], 2.0) + pow(W.Be[0] - W.Bs[0], 2.0));
+    denom = denom > 1.0e-20 ? denom : 1.0e20;
+    double x = wmin(fabs((W.Be[1] - W.Bs[1]) * x0 - (W.Be[0] - W.Bs[0]) * y0 + W.Be[0] * W.Bs[1] - W.Be[1] * W.Bs[0]) /
+                        denom,
+                    1.0e20);
+    double dist2 = W.w.B96_2;
+    x = (dist2 - fabs(x)) / (dist2 * W.dist2_fac);
+    x = wmax(x, 0.0);
+    double r = 0.0;
+    r += 1.0 - (exp(pow(x, 3.5)) - 1.0) / (EE_REF - 1.0);
+    r /= double(1);
+    return r;
+}
+
+// iowave::xgen (iowave_dist.cpp:27-44) at (x1, y1)
+double xgen(const orc_domain* d, double x1, double y1)
+{
+    const orc_domain::Wave& W = d->wave;
+    return fabs(y1 - W.tan_alpha * x1 + W.tan_alpha * W.x0 - W.y0) / sqrt(pow(W.tan_alpha, 2.0) + 1.0);
+}
+
+// iowave::iowave (iowave.cpp:32-263) and wave_lib_parameters (wave_lib_parameters.cpp:28-160)
+// for linear waves (B92 = 2) with B91 or B93, the default zones (B106 = B107 = B108 = 0)
+void wave_setup(orc_domain* d, const orc_wave_desc& w)
+{
+    orc_domain::Wave& W = d->wave;
+    W.on = 1;
+    W.w = w;
+    W.phimean = w.F60;
+    W.wdt = W.phimean;
+    W.pshift = w.B120 * (PI_REF / 180.0);
+    if(w.B91 == 1)
+    {
+        W.wa = 0.5 * w.B91_1;
+        double wL = w.B91_2;
+        W.wL = wL;
+        W.wk = (2.0 * PI_REF) / (wL > 1.0e-20 ? wL : 1.0e20);
+        W.ww = sqrt(fabs(9.81 * W.wk * tanh(W.wk * (W.wdt))));
+        double wf = W.ww / (2.0 * PI_REF);
+        W.wT = 1.0 / wf;
+    }
+    if(w.B93 == 1)
+    {
+        W.wa = 0.5 * w.B93_1;
+        W.wT = w.B93_2;
+        double wL0 = (9.81 / (2.0 * PI_REF)) * W.wT * W.wT;
+        double k0 = (2.0 * PI_REF) / wL0;
+        double S0 = sqrt(k0 * W.wdt) * (1.0 + (k0 * W.wdt) / 6.0 + (k0 * k0 * W.wdt * W.wdt) / 30.0);
+        double wL = wL0 * tanh(S0);
+        for(int qn = 0; qn < 500; ++qn) wL = wL0 * tanh(2.0 * PI_REF * W.wdt / wL);
+        W.wL = wL;
+        double wf = 1.0 / W.wT;
+        W.ww = wf * 2.0 * PI_REF;
+        W.wk = (2.0 * PI_REF) / (wL > 1.0e-20 ? wL : 1.0e20);
+    }
+    W.singamma = sin((w.B105_1) * (PI_REF / 180.0));
+    W.cosgamma = cos((w.B105_1) * (PI_REF / 180.0));
+    W.psi = 0.6 * d->DXM;
+    W.epsi = 3.0 * d->DXM;
+    W.dist2_fac = w.B99 == 1 ? 2.0 : 1.0;
+    const double xmin = d->X[0][MARGE], xmax = d->X[0][d->nx + MARGE];
+    const double ymin = d->X[1][MARGE], ymax = d->X[1][d->ny + MARGE];
+    W.Gs[0] = xmin;
+    W.Gs[1] = ymin - 10.0 * d->DXM;
+    W.Ge[0] = xmin;
+    W.Ge[1] = ymax + 10.0 * d->DXM;
+    W.Bs[0] = xmax;
+    W.Bs[1] = ymin - 10.0 * d->DXM;
+    W.Be[0] = xmax;
+    W.Be[1] = ymax + 10.0 * d->DXM;
+    zone_corners(W.Bs, W.Be, w.B96_2, W.B);
+    zone_corners(W.Gs, W.Ge, w.B96_1, W.G);
+    W.x0 = xmin;
+    W.y0 = ymin;
+    double alpha = (w.B105_1 + 90.0) * (PI_REF / 180.0);
+    W.tan_alpha = tan(alpha);
+    const long n = d->sx * d->sy * d->sz;
+    W.eta.assign((size_t)d->nx * d->ny, 0.0);
+    W.uval.assign(n, 0.0);
+    W.vval.assign(n, 0.0);
+    W.wval.assign(n, 0.0);
+    W.lsval.assign(n, 0.0);
+}
+
+// the ramp (iowave_relax_functions.cpp:239-254)
+double wave_ramp(const orc_domain* d)
+{
+    const orc_domain::Wave& W = d->wave;
+    double f = 1.0;
+    if(W.w.B101 == 1 && d->simtime < W.w.B102 * W.wT)
+        f = d->simtime / (W.w.B102 * W.wT) - (1.0 / PI_REF) * sin(PI_REF * (d->simtime / (W.w.B102 * W.wT)));
+    if(W.w.B101 == 2 && d->simtime < W.w.B102)
+        f = d->simtime / (W.w.B102) - (1.0 / PI_REF) * sin(PI_REF * (d->simtime / (W.w.B102)));
+    return f;
+}
+
+// wave_lib_linear (wave_lib_linear.cpp:47-85) through wave_interface (wave_interface.cpp:
+// 137-219): wts = 0, wte = 1e20, wavetime = simtime
+double wave_horzvel(const orc_domain* d, double x, double z)
+{
+    const orc_domain::Wave& W = d->wave;
+    double teta = W.wk * x - W.ww * (d->simtime) + W.pshift;
+    return W.ww * W.wa * (cosh(W.wk * (W.wdt + z)) / sinh(W.wk * W.wdt)) * cos(teta);
+}
+double wave_u(const orc_domain* d, double x, double z)
+{
+    z = wmax(z, -d->wave.phimean);
+    double uvel = 0.0;
+    if(d->simtime >= 0.0 && d->simtime <= 1.0e20) uvel = d->wave.cosgamma * wave_horzvel(d, x, z);
+    return uvel;
+}
+double wave_v(const orc_domain* d, double x, double z)
+{
+    z = wmax(z, -d->wave.phimean);
+    double vvel = 0.0;
+    if(d->simtime >= 0.0 && d->simtime <= 1.0e20) vvel = d->wave.singamma * wave_horzvel(d, x, z);
+    return vvel;
+}
+double wave_w(const orc_domain* d, double x, double z)
+{
+    const orc_domain::Wave& W = d->wave;
+    z = wmax(z, -W.phimean);
+    double wvel = 0.0;
+    if(d->simtime >= 0.0 && d->simtime <= 1.0e20)
+    {
+        double teta = W.wk * x - W.ww * (d->simtime) + W.pshift;
+        wvel = W.ww * W.wa * (sinh(W.wk * (W.wdt + z)) / sinh(W.wk * W.wdt)) * sin(teta);
+    }
+    return wvel;
+}
+double wave_eta(const orc_domain* d, double x)
+{
+    const orc_domain::Wave& W = d->wave;
+    double eta = 0.0;
+    if(d->simtime >= 0.0 && d->simtime <= 1.0e20) eta = W.wa * cos(W.wk * x - W.ww * (d->simtime) + W.pshift);
+    return eta;
+}
+
+// iowave::wavegen_precalc_relax (iowave_precalc_relax.cpp:27-160) at d->simtime: eta in the
+// generation zone's columns (gcsl_start4 fills only slice ghosts, which the relaxation loops
+// do not read), then the target values of every generation-zone cell of the u, v, w and
+// cell loops, kept per cell instead of by count
+void wave_precalc(orc_domain* d)
+{
+    orc_domain::Wave& W = d->wave;
+    const int nx = d->nx, ny = d->ny;
+    const double *XN = d->X[0].data(), *YN = d->X[1].data(), *XP = d->P[0].data(), *YP = d->P[1].data();
+    const double *ZN = d->X[2].data(), *ZP = d->P[2].data();
+    auto dg = [&](int i, int j) { return zone_dist(W.G, W.Gs, W.Ge, XP[i + MARGE], YP[j + MARGE]); };
+    auto eta = [&](int i, int j) -> double& { return W.eta[(size_t)i + (size_t)nx * j]; };
+    for(int i = 0; i < nx; ++i)
+    for(int j = 0; j < ny; ++j)
+        if(W.w.B98 == 2 && dg(i, j) < 1.0e20) eta(i, j) = wave_eta(d, xgen(d, XP[i + MARGE], YP[j + MARGE]));
+    vel_loop(d, 1, [&](long c, int i, int j, int k) {
+        double xg = xgen(d, XN[i + 1 + MARGE], YP[j + MARGE]);
+        double zloc1 = ZP[k + MARGE];
+        double fsfloc = 0.5 * (eta(i, j) + eta(i + 1, j)) + W.phimean, z = 0.0;
+        if(zloc1 <= fsfloc) z = zloc1 - W.phimean;
+        if(zloc1 > fsfloc) z = 0.5 * (eta(i, j) + eta(i + 1, j));
+        if(W.w.B98 == 2 && dg(i, j) < 1.0e20)
+        {
+            if(zloc1 <= fsfloc + W.epsi) W.uval[c] = wave_u(d, xg, z) + 0.0;
+            if(zloc1 > fsfloc + W.epsi) W.uval[c] = 0.0;
+        }
+    });
+    vel_loop(d, 2, [&](long c, int i, int j, int k) {
+        double xg = xgen(d, XP[i + MARGE], YN[j + 1 + MARGE]);
+        double zloc2 = ZP[k + MARGE];
+        double fsfloc = 0.5 * (eta(i, j) + eta(i, j + 1)) + W.phimean, z = 0.0;
+        if(zloc2 <= fsfloc) z = zloc2 - W.phimean;
+        if(zloc2 > fsfloc) z = 0.5 * (eta(i, j) + eta(i, j + 1));
+        if(W.w.B98 == 2 && dg(i, j) < 1.0e20)
+        {
+            if(zloc2 <= fsfloc + W.epsi) W.vval[c] = wave_v(d, xg, z);
+            if(zloc2 > fsfloc + W.epsi) W.vval[c] = 0.0;
+        }
+    });
+    vel_loop(d, 3, [&](long c, int i, int j, int k) {
+        double xg = xgen(d, XP[i + MARGE], YP[j + MARGE]);
+        double zloc3 = ZN[k + 1 + MARGE];
+        double fsfloc = eta(i, j) + W.phimean, z = 0.0;
+        if(zloc3 <= fsfloc) z = zloc3 - W.phimean;
+        if(zloc3 > fsfloc) z = eta(i, j);
+        if(W.w.B98 == 2 && dg(i, j) < 1.0e20)
+        {
+            if(zloc3 <= fsfloc + W.epsi) W.wval[c] = wave_w(d, xg, z);
+            if(zloc3 > fsfloc + W.epsi) W.wval[c] = 0.0;
+        }
+    });
+    for(int i = 0; i < nx; ++i)
+    for(int j = 0; j < ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+        if(W.w.B98 == 2 && dg(i, j) < 1.0e20) W.lsval[d->id(i, j, k)] = eta(i, j) + W.phimean - ZP[k + MARGE];
+}
+
+// the band value H (= G) of the relaxation of velocity (iowave_relax.cpp:38-55): 1 above
+// -psi, 0 below -epsi, a smoothed step between
+void wave_HG(const orc_domain* d, double phival, double& H, double& G)
+{
+    const orc_domain::Wave& W = d->wave;
+    H = G = 0.0;
+    if(phival >= -W.psi) { H = 1.0; G = 1.0; }
+    if(phival < -W.epsi) { H = 0.0; G = 0.0; }
+    if(phival >= -W.epsi && phival < -W.psi)
+    {
+        H = 0.5 * (1.0 + phival / W.epsi + (1.0 / PI_REF) * sin((PI_REF * phival) / W.epsi));
+        G = H;
+    }
+}
+
+// iowave::u_relax / v_relax / w_relax (iowave_relax.cpp:28-330): comp 1..3 of field vel,
+// with the band of phi on the component's face
+void wave_vel_relax(orc_domain* d, int comp, const double* phi, double* vel)
+{
+    orc_domain::Wave& W = d->wave;
+    const double *XP = d->P[0].data(), *YP = d->P[1].data(), *XN = d->X[0].data(), *YN = d->X[1].data();
+    const long off = comp == 1 ? 1 : comp == 2 ? d->sx : d->sx * d->sy;
+    const double* val = comp == 1 ? W.uval.data() : comp == 2 ? W.vval.data() : W.wval.data();
+    const double ramp = wave_ramp(d);
+    const bool beach = W.w.B99 == 1 || W.w.B99 == 2;
+    vel_loop(d, comp, [&](long c, int i, int j, int k) {
+        double x0 = XP[i + MARGE], y0 = YP[j + MARGE];
+        double dg = zone_dist(W.G, W.Gs, W.Ge, x0, y0);
+        double db = zone_dist(W.B, W.Bs, W.Be, x0, y0);
+        double phival = 0.5 * (phi[c] + phi[c + off]);
+        double H, G;
+        wave_HG(d, phival, H, G);
+        // relax1/2/4 on the component's point (pos1, pos2, pos3 = cell centre in x, y)
+        double xr = comp == 1 ? XN[i + 1 + MARGE] : XP[i + MARGE];
+        double yr = comp == 2 ? YN[j + 1 + MARGE] : YP[j + MARGE];
+        if(W.w.B98 == 2 && dg < 1.0e20)
+        {
+            double r = rb1_ext(d, xr, yr);
+            vel[c] = (1.0 - r) * ramp * val[c] * H + r * H * vel[c] + (1.0 - G) * vel[c];
+            if(comp == 1) vel[c] += (1.0 - r) * ramp * W.w.W50 * (1.0 - H) + r * (1.0 - H) * W.w.W50;
+        }
+        if(beach && db < (comp == 1 ? 1.0e19 : 1.0e20)) vel[c] = rb3_ext(d, xr, yr) * vel[c];
+    });
+}
+
+// iowave::p_relax (iowave_relax.cpp:332-356)
+void wave_p_relax(orc_domain* d, const double* ro, double* press)
+{
+    orc_domain::Wave& W = d->wave;
+    if(!(W.w.B99 == 1 || W.w.B99 == 2)) return;
+    const double *XP = d->P[0].data(), *YP = d->P[1].data(), *ZP = d->P[2].data();
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        long c = d->id(i, j, k);
+        double x0 = XP[i + MARGE], y0 = YP[j + MARGE];
+        if(zone_dist(W.B, W.Bs, W.Be, x0, y0) < 1.0e20)
+        {
+            double r = rb3_ext(d, x0, y0);
+            press[c] = (1.0 - r) * ((W.phimean - ZP[k + MARGE]) * ro[c] * fabs(d->prm.gk)) + r * press[c];
+        }
+    }
+}
+
+// iowave::phi_relax (iowave_relax.cpp:358-395)
+void wave_phi_relax(orc_domain* d, double* f)
+{
+    orc_domain::Wave& W = d->wave;
+    const double *XP = d->P[0].data(), *YP = d->P[1].data(), *ZP = d->P[2].data();
+    const double ramp = wave_ramp(d);
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+    {
+        long c = d->id(i, j, k);
+        double x0 = XP[i + MARGE], y0 = YP[j + MARGE];
+        if(W.w.B98 == 2 && zone_dist(W.G, W.Gs, W.Ge, x0, y0) < 1.0e20)
+        {
+            double r = rb1_ext(d, x0, y0);
+            f[c] = (1.0 - r) * ramp * W.lsval[c] + r * f[c];
+        }
+        if(W.w.B99 == 2 && zone_dist(W.B, W.Bs, W.Be, x0, y0) < 1.0e20)
+        {
+            double r = rb3_ext(d, x0, y0);
+            f[c] = (1.0 - r) * (W.phimean - ZP[k + MARGE]) + r * f[c];
+        }
+    }
+}
+
+// u_relax, v_relax, w_relax and p_relax in the order momentum_FC3 calls them
+// (momentum_FC3.cpp:218-221)
+void wave_relax_uvwp(orc_domain* d, const double* phi, const double* ro, double* u, double* v, double* w,
+                     double* press)
+{
+    wave_vel_relax(d, 1, phi, u);
+    wave_vel_relax(d, 2, phi, v);
+    wave_vel_relax(d, 3, phi, w);
+    if(press) wave_p_relax(d, ro, press);
+}
+
 // momentum_FC3::start (momentum_FC3.cpp:122-430) with D20=0 (diff_void) or 1 (ediff2), B10=0,
 // laminar (kepsilon_void zeroes F/G/H), void ioflow and forcing, pjm_corr projection.
 // The working field ls is a member of momentum_FC3 whose ghost cells are never filled,
@@ -1370,6 +1763,7 @@ struct FC3
         cell_loop(d, [&](long c) { L[c] = 0.0; ls[c] = phi[c]; });
         hj(d, ls.data(), u, v, w, L.data());
         cell_loop(d, [&](long c) { frk1[c] = ls[c] + dt * L[c]; });
+        if(d->wave.on) wave_phi_relax(d, frk1.data());      // pflow->phi_relax (momentum_FC3.cpp:145)
         ghost(d, frk1.data(), 4, 52);
         reini(d, frk1.data(), 2);
         cell_loop(d, [&](long c) { phi[c] = frk1[c]; });
@@ -1380,6 +1774,7 @@ struct FC3
         // the direct-forcing flags the implicit diffusion reads are set from here on
         d->df_set = 1;
         iters += pjm(d, phi, urk1.data(), vrk1.data(), wrk1.data(), press, pcorr.data(), 1.0);
+        if(d->wave.on) wave_relax_uvwp(d, phi, ro, urk1.data(), vrk1.data(), wrk1.data(), press);   // :218-221
         ghost(d, urk1.data(), 1, 10);
         ghost(d, vrk1.data(), 2, 11);
         ghost(d, wrk1.data(), 3, 12);
@@ -1388,6 +1783,7 @@ struct FC3
         cell_loop(d, [&](long c) { L[c] = 0.0; });
         hj(d, frk1.data(), urk1.data(), vrk1.data(), wrk1.data(), L.data());
         cell_loop(d, [&](long c) { frk2[c] = 0.75 * ls[c] + 0.25 * frk1[c] + 0.25 * dt * L[c]; });
+        if(d->wave.on) wave_phi_relax(d, frk2.data());      // momentum_FC3.cpp:244
         ghost(d, frk2.data(), 4, 52);
         reini(d, frk2.data(), 2);
         cell_loop(d, [&](long c) { phi[c] = frk2[c]; });
@@ -1396,6 +1792,7 @@ struct FC3
         fluid_update(d, phi, ro, visc, vol);
         velocity_stage(phi, press, u, v, w, urk1.data(), vrk1.data(), wrk1.data(), urk2, vrk2, wrk2, 0.25, 1);
         iters += pjm(d, phi, urk2.data(), vrk2.data(), wrk2.data(), press, pcorr.data(), 0.25);
+        if(d->wave.on) wave_relax_uvwp(d, phi, ro, urk2.data(), vrk2.data(), wrk2.data(), press);   // :320-323
         ghost(d, urk2.data(), 1, 10);
         ghost(d, vrk2.data(), 2, 11);
         ghost(d, wrk2.data(), 3, 12);
@@ -1404,6 +1801,7 @@ struct FC3
         cell_loop(d, [&](long c) { L[c] = 0.0; });
         hj(d, frk2.data(), urk2.data(), vrk2.data(), wrk2.data(), L.data());
         cell_loop(d, [&](long c) { ls[c] = (1.0 / 3.0) * ls[c] + (2.0 / 3.0) * frk2[c] + (2.0 / 3.0) * dt * L[c]; });
+        if(d->wave.on) wave_phi_relax(d, ls.data());        // momentum_FC3.cpp:346
         cell_loop(d, [&](long c) { phi[c] = ls[c]; });
         ghost(d, phi, 4, 52);
         reini(d, phi, 3);
@@ -1412,6 +1810,7 @@ struct FC3
         std::vector<double> un(u, u + n), vn(v, v + n), wn(w, w + n);
         velocity_stage3(phi, press, u, v, w);
         iters += pjm(d, phi, u, v, w, press, pcorr.data(), 2.0 / 3.0);
+        if(d->wave.on) wave_relax_uvwp(d, phi, ro, u, v, w, press);    // momentum_FC3.cpp:418-421
         ghost(d, u, 1, 10);
         ghost(d, v, 2, 11);
         ghost(d, w, 3, 12);
@@ -1721,8 +2120,47 @@ int orc_pjm(orc_domain* d, const double* phi, double* u, double* v, double* w, d
 int orc_fc3_step(orc_domain* d, double* phi, double* u, double* v, double* w, double* press, double* ro, double* visc,
                  double* vol2)
 {
+    if(d->wave.on)
+    {
+        // driver::loop_cfd before momentum (driver_loop_cfd.cpp:78-92): the wave precalc at
+        // the step's simtime, relaxation of u, v, w, levelset_void::update (the fluid update)
+        wave_precalc(d);
+        wave_vel_relax(d, 1, phi, u);
+        wave_vel_relax(d, 2, phi, v);
+        wave_vel_relax(d, 3, phi, w);
+        double vol[2];
+        fluid_update(d, phi, ro, visc, vol);
+    }
     FC3 s(d);
-    return s.step(phi, u, v, w, press, ro, visc, vol2);
+    const int it = s.step(phi, u, v, w, press, ro, visc, vol2);
+    if(d->wave.on) d->simtime += d->prm.dt;
+    return it;
+}
+
+void orc_wave_set(orc_domain* d, const orc_wave_desc* w)
+{
+    wave_setup(d, *w);
+}
+
+void orc_wave_params(const orc_domain* d, double* out5)
+{
+    const orc_domain::Wave& W = d->wave;
+    out5[0] = W.wT;
+    out5[1] = W.wL;
+    out5[2] = W.wk;
+    out5[3] = W.ww;
+    out5[4] = W.wa;
+}
+
+void orc_set_simtime(orc_domain* d, double t) { d->simtime = t; }
+
+void orc_wave_relax(orc_domain* d, double simtime, const double* phi, const double* ro, double* u, double* v, double* w,
+                    double* press, double* f)
+{
+    d->simtime = simtime;
+    wave_precalc(d);
+    wave_relax_uvwp(d, phi, ro, u, v, w, press);
+    wave_phi_relax(d, f);
 }
 
 void orc_vortex_step(orc_domain* d, double* phi, double* u, double* v, double* w, double* ro, double* visc,

@@ -107,6 +107,31 @@ void orc_vortex_step(orc_domain*, double* phi, double* u, double* v, double* w, 
 /* HJ-WENO phi transport TVD-RK3 only (levelset_RK3.cpp:109-160, no reinit) */
 void orc_phi_rk3(orc_domain*, double* phi, const double* u, const double* v, const double* w);
 
+/* iowave wave generation by relaxation zones (B98 = 2) and numerical beach (B99 = 1, 2)
+ * for linear waves (B92 = 2) with B91 (height, length) or B93 (height, period); the
+ * default zones along x = xmin and x = xmax (B96_1, B96_2 long); with it set,
+ * orc_fc3_step runs the driver loop's wave prologue (precalc at the domain's simtime,
+ * relaxation of u, v, w, fluid update) before the step and advances simtime by dt */
+typedef struct orc_wave_desc
+{
+    int B92;
+    int B91;
+    double B91_1, B91_2;
+    int B93;
+    double B93_1, B93_2;
+    double B96_1, B96_2;
+    int B98, B99, B101;
+    double B102, B105_1, B120, F60, W50;
+} orc_wave_desc;
+void orc_wave_set(orc_domain*, const orc_wave_desc*);
+/* wT, wL, wk, ww, wA */
+void orc_wave_params(const orc_domain*, double* out5);
+void orc_set_simtime(orc_domain*, double t);
+/* wavegen_precalc at simtime, then u/v/w_relax (face band of phi), p_relax (ro) and
+ * phi_relax of f, in place */
+void orc_wave_relax(orc_domain*, double simtime, const double* phi, const double* ro, double* u, double* v, double* w,
+                    double* press, double* f);
+
 /* number of cells of the field layout */
 long orc_field_size(const orc_domain*);
 

@@ -6,6 +6,7 @@
 
 #include "reef_internal.h"
 #include "tiling.h"
+#include "wave.h"
 
 namespace reef {
 
@@ -48,6 +49,9 @@ struct TankDev
     // private x of the solver of var 1..3 (m[8]) and of var 4, 5 (m[9]); null until the
     // tank first needs them
     double* m[10];
+    // relaxation-zone wave generator and numerical beach (iowave); wave.col null: none.
+    // Its simulation time is tstep[6].
+    WaveDev wave;
 };
 
 // The tanks one launch covers, passed by value (it lives in the kernel-argument segment,
@@ -82,6 +86,13 @@ void bl_flux_acc(const TankSet& b, int ipol, int q, int u, int v, int w, int acc
 void bl_momentum(const TankSet& b, int comp, int stage, int qin, int qold, int u, int v, int w,
                  double grav, double cfac, int out, double rw, double ra, int diff, hipStream_t s);
 void bl_heaviside(const TankSet& b, const double* prm4, hipStream_t s);
+// iowave of the tanks with a wave generator (the others are skipped): the surface
+// elevation of wavegen_precalc at each tank's simtime (tstep[6]); the relaxation of the
+// velocities u, v, w (u_ >= 0), of the pressure (p_ >= 0) and of the level-set field f
+// (f_ >= 0), with phi's band and rho of the tank; simtime += dt
+void bl_wave_eta(const TankSet& b, hipStream_t s);
+void bl_wave_relax(const TankSet& b, int u, int v, int w, int p, int f, hipStream_t s);
+void bl_wave_time(const TankSet& b, hipStream_t s);
 void bl_poisson_rhs(const TankSet& b, int u, int v, int w, double afac, hipStream_t s);
 void bl_face_rho(const TankSet& b, double rw, double ra, int with_aii, hipStream_t s);
 void bl_correct(const TankSet& b, double rw, double ra, double afac, int u, int v, int w,

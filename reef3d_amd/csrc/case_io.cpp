@@ -5,6 +5,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <fstream>
 #include <iterator>
 #include <sstream>
@@ -235,9 +236,9 @@ int reef_ctrl_read(const char* path, reef_params* prm, double* ctrl5)
     for(const Need& n : need)
         if(n.have != n.want)
             return fail(REEF_E_UNSUPPORTED, std::string("ctrl.txt key outside the hot path: ") + n.key);
-    if(D20 != 0 && D20 != 1)
-        return fail(REEF_E_UNSUPPORTED, "ctrl.txt key outside the hot path: D 20 (diffusion: 0 or 1; 2, idiff2_FS, "
-                                        "is not implemented)");
+    if(D20 != 0 && D20 != 1 && D20 != 2)
+        return fail(REEF_E_UNSUPPORTED, "ctrl.txt key outside the hot path: D 20 (diffusion: 0 diff_void, 1 ediff2, "
+                                        "2 idiff2_FS)");
     if(D20 == 1 && N48 == 1)
         return fail(REEF_E_UNSUPPORTED, "ctrl.txt D 20 1 with N 48 1: the reference steps it with another "
                                         "time-step class");
@@ -253,6 +254,31 @@ int reef_ctrl_read(const char* path, reef_params* prm, double* ctrl5)
         ctrl5[2] = N47;
         ctrl5[3] = N40;
         ctrl5[4] = D20;
+    }
+    return 0;
+}
+
+// The values of one ctrl.txt key (read_control.cpp:28-70 reads a key letter, its number and
+// the key's values), from the last line that sets it; *count = 0 when no line does.
+int reef_ctrl_key(const char* path, char letter, int num, double* values, int capacity, int* count)
+{
+    if(!path || !count || capacity < 0 || (capacity > 0 && !values)) return fail(REEF_E_ARG, "null argument");
+    std::ifstream f(path);
+    if(!f) return fail(REEF_E_ARG, std::string("cannot open control file ") + path);
+    *count = 0;
+    std::string line;
+    while(std::getline(f, line))
+    {
+        std::istringstream in(line);
+        char c = 0;
+        int n = 0;
+        if(!(in >> c) || c != letter || !(in >> n) || n != num) continue;
+        std::vector<double> v;
+        double x;
+        while(in >> x) v.push_back(x);
+        const int m = (int)std::min<size_t>(v.size(), (size_t)capacity);
+        for(int q = 0; q < m; ++q) values[q] = v[q];
+        *count = (int)v.size();
     }
     return 0;
 }

@@ -1171,6 +1171,10 @@ int Engine::fc3_segment(int s, bool mixed)
         each([&](const TankSet& S) { bl_copy(S, A_FRK1, REEF_PHI, stream); });
         tend();
         if((rc = ghost(REEF_PHI, 4, 52))) return rc;
+        // the stage's fluid update (pupdate, momentum_FC3.cpp:157-159): the diffusion reads the
+        // mixture viscosity of this stage's phi; without diffusion nothing in the step reads
+        // rho or nu before the final update (face densities come from phi)
+        if(d20 && (rc = fluid_update())) return rc;
         const int qin[3] = {REEF_U, REEF_V, REEF_W};
         if((rc = momentum_stage(1, qin, qin, qin, VEL[0]))) return rc;
     }
@@ -1187,6 +1191,7 @@ int Engine::fc3_segment(int s, bool mixed)
         tend();
         if((rc = ghost(REEF_PHI, 4, 52))) return rc;
         if((rc = reini(REEF_PHI, 2))) return rc;
+        if(d20 && (rc = fluid_update())) return rc;   // momentum_FC3.cpp:260
         const int qold[3] = {REEF_U, REEF_V, REEF_W};
         if((rc = momentum_stage(2, VEL[0], qold, VEL[0], VEL[1]))) return rc;
     }
@@ -1200,6 +1205,7 @@ int Engine::fc3_segment(int s, bool mixed)
         tend();
         if((rc = ghost(REEF_PHI, 4, 52))) return rc;
         if((rc = reini(REEF_PHI, 3))) return rc;
+        if(d20 && (rc = fluid_update())) return rc;   // momentum_FC3.cpp:358
         const int qold[3] = {REEF_U, REEF_V, REEF_W};
         if((rc = momentum_stage(3, VEL[1], qold, VEL[1], VEL[2]))) return rc;
     }

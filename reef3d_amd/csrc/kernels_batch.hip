@@ -2358,6 +2358,149 @@ __global__ void __launch_bounds__(NT) kb_void_rk3(TankSet b, int comp, int id, d
     }
 }
 
+// ------------------------------------------------------------------ wave generation (iowave)
+// Relaxation-zone wave generation (B98 = 2) and numerical beach (B99) of linear waves.  The
+// static part (zones, relaxation factors, distances along the wave) comes per column from
+// the host; the kernels evaluate the wave at the tank's simtime (tstep[6]), the value
+// wavegen_precalc gives at the start of the step, where a relaxation reads it.
+
+// iowave::ramp (iowave_relax_functions.cpp:239-254); ramp_T = B102 wT (B101 = 1) or B102
+__device__ __forceinline__ double wave_ramp(const WaveDev& W, double t)
+{
+    double f = 1.0;
+    if(W.ramp_mode == 1 && t < W.ramp_T) f = t / (W.ramp_T) - (1.0 / PI_REF) * sin(PI_REF * (t / (W.ramp_T)));
+    if(W.ramp_mode == 2 && t < W.ramp_T) f = t / (W.ramp_T) - (1.0 / PI_REF) * sin(PI_REF * (t / (W.ramp_T)));
+    return f;
+}
+
+// wave_interface (wave_interface.cpp:137-219) over wave_lib_linear (wave_lib_linear.cpp:
+// 47-85): the start and end times are 0 and 1e20, the depth clamp is wD
+__device__ __forceinline__ double wave_horz(const WaveDev& W, double x, double z, double t)
+{
+    const double teta = W.wk * x - W.ww * (t) + W.pshift;
+    return W.ww * W.wa * (cosh(W.wk * (W.wdt + z)) / sinh(W.wk * W.wdt)) * cos(teta);
+}
+__device__ __forceinline__ double wave_target(const WaveDev& W, int comp, double x, double z, double t)
+{
+    z = z > -W.wD ? z : -W.wD;
+    if(!(t >= 0.0 && t <= 1.0e20)) return 0.0;
+    if(comp == 1) return W.cosg * wave_horz(W, x, z, t);
+    if(comp == 2) return W.sing * wave_horz(W, x, z, t);
+    const double teta = W.wk * x - W.ww * (t) + W.pshift;
+    return W.ww * W.wa * (sinh(W.wk * (W.wdt + z)) / sinh(W.wk * W.wdt)) * sin(teta);
+}
+
+// eta(i, j) of wavegen_precalc_relax (iowave_precalc_relax.cpp:37-48) in the generation
+// zone's columns; the other columns keep 0, as the reference's slice does.  One block per
+// tank of the set.
+__global__ void __launch_bounds__(NT) kb_wave_eta(TankSet b)
+{
+    CTank& T = tank_at(b, blockIdx.x);
+    const WaveDev W = T.wave;
+    if(!W.col || !W.gen) return;
+    const long ncol = (long)T.g.nx * T.g.ny;
+    const double t = T.tstep[6];
+    const double* __restrict__ gen = W.col + WC_GEN * ncol;
+    const double* __restrict__ xg = W.col + WC_XG4 * ncol;
+    for(long q = threadIdx.x; q < ncol; q += NT)
+        if(gen[q] != 0.0) W.eta[q] = (t >= 0.0 && t <= 1.0e20) ? W.wa * cos(W.wk * xg[q] - W.ww * (t) + W.pshift) : 0.0;
+}
+
+// iowave::u_relax, v_relax, w_relax, p_relax, phi_relax (iowave_relax.cpp:28-395) with the
+// target values of wavegen_precalc_relax (iowave_precalc_relax.cpp:50-160), per cell of the
+// zones' columns: velocity (u_ >= 0: the fields u_, v_, w_), pressure (p_ >= 0) and level set
+// (f_ >= 0)
+__global__ void __launch_bounds__(NT) kb_wave_relax(TankSet b, int u_, int v_, int w_, int p_, int f_)
+{
+#pragma unroll 1
+    for(TileIter q = set_tile(b); q.ok; q.ok = false)
+    {
+        CTank& T = tank_at(b, q.p);
+        const WaveDev W = T.wave;
+        if(!W.col) continue;
+        const DevGrid g = T.g;
+        const long ncol = (long)g.nx * g.ny;
+        const double t = T.tstep[6];
+        const double ramp = wave_ramp(W, t);
+        const double* __restrict__ phi = T.f[REEF_PHI];
+        const TileWalk tw = tile_walk(T, q.t);
+#pragma unroll 1
+        for(int tt = tw.t; tt < tw.t1; ++tt)
+        {
+            int i, j, k;
+            long c;
+            if(!tile_cell(g, tw, tt, i, j, k, c)) continue;
+            const long col = i + (long)g.nx * j;
+            const bool gen = W.gen && W.col[WC_GEN * ncol + col] != 0.0;
+            const bool beach = W.beach && W.col[WC_BEACH * ncol + col] != 0.0;
+            if(!gen && !beach) continue;
+            const double zp = g.ZP[k + MARGE];
+            if(u_ >= 0)
+                for(int comp = 1; comp <= 3; ++comp)
+                {
+                    if(!comp_cell(g, comp, i, j, k)) continue;
+                    double* __restrict__ vel = T.f[comp == 1 ? u_ : comp == 2 ? v_ : w_];
+                    const long off = comp == 1 ? 1 : comp == 2 ? g.sx : g.sxy;
+                    const double phival = 0.5 * (phi[c] + phi[c + off]);
+                    double H = 0.0, G = 0.0;
+                    if(phival >= -W.psi) { H = 1.0; G = 1.0; }
+                    if(phival < -W.epsi) { H = 0.0; G = 0.0; }
+                    if(phival >= -W.epsi && phival < -W.psi)
+                    {
+                        H = 0.5 * (1.0 + phival / W.epsi + (1.0 / PI_REF) * sin((PI_REF * phival) / W.epsi));
+                        G = H;
+                    }
+                    double v = vel[c];
+                    if(gen)
+                    {
+                        // the target of wavegen_precalc_relax at this point
+                        const double e0 = W.eta[col];
+                        const double e1 = comp == 1 ? W.eta[col + 1] : comp == 2 ? W.eta[col + g.nx] : 0.0;
+                        const double zloc = comp == 3 ? W.zn[k + 1 + MARGE] : zp;
+                        const double emid = comp == 3 ? e0 : 0.5 * (e0 + e1);
+                        const double fsfloc = emid + W.phimean;
+                        const double z = zloc <= fsfloc ? zloc - W.phimean : emid;
+                        const double xg = W.col[(comp == 1 ? WC_XG1 : comp == 2 ? WC_XG2 : WC_XG4) * ncol + col];
+                        double val = 0.0;
+                        if(zloc <= fsfloc + W.epsi) val = wave_target(W, comp, xg, z, t) + (comp == 1 ? W.Ui : 0.0);
+                        const double r = W.col[(comp == 1 ? WC_R1WG : comp == 2 ? WC_R2WG : WC_R4WG) * ncol + col];
+                        v = (1.0 - r) * ramp * val * H + r * H * v + (1.0 - G) * v;
+                        if(comp == 1) v += (1.0 - r) * ramp * W.W50 * (1.0 - H) + r * (1.0 - H) * W.W50;
+                    }
+                    if(beach) v = W.col[(comp == 1 ? WC_R1NB : comp == 2 ? WC_R2NB : WC_R4NB) * ncol + col] * v;
+                    vel[c] = v;
+                }
+            if(p_ >= 0 && beach)
+            {
+                double* __restrict__ press = T.f[p_];
+                const double r = W.col[WC_R4NB * ncol + col];
+                press[c] = (1.0 - r) * ((W.phimean - zp) * T.f[REEF_RO][c] * W.gabs) + r * press[c];
+            }
+            if(f_ >= 0)
+            {
+                double* __restrict__ f = T.f[f_];
+                if(gen)
+                {
+                    const double r = W.col[WC_R4WG * ncol + col];
+                    f[c] = (1.0 - r) * ramp * (W.eta[col] + W.phimean - zp) + r * f[c];
+                }
+                if(beach && W.beach_phi)
+                {
+                    const double r = W.col[WC_R4NB * ncol + col];
+                    f[c] = (1.0 - r) * (W.phimean - zp) + r * f[c];
+                }
+            }
+        }
+    }
+}
+
+// the driver's p->simtime += p->dt after the step (driver_loop_cfd.cpp:100)
+__global__ void kb_wave_time(TankSet b)
+{
+    CTank& T = tank_at(b, blockIdx.x);
+    if(T.wave.col && threadIdx.x == 0) T.tstep[6] = T.tstep[6] + T.tstep[0];
+}
+
 }  // namespace bk
 
 // ------------------------------------------------------------------ launchers
@@ -2421,6 +2564,7 @@ void bl_heaviside(const TankSet& b, const double* prm4, hipStream_t s)
     hipLaunchKernelGGL(kb_heaviside, pgrid(b), dim3(NT), 0, s, b, prm4[0], prm4[1], prm4[2], prm4[3]);
     hipLaunchKernelGGL(kb_heaviside_fin, tankgrid(b), dim3(NT), 0, s, b);
 }
+
 void bl_poisson_rhs(const TankSet& b, int u, int v, int w, double afac, hipStream_t s)
 {
     hipLaunchKernelGGL(kb_poisson_rhs, tgrid(b), dim3(NT), 0, s, b, u, v, w, afac);
@@ -2585,6 +2729,19 @@ void bl_vortex_velocity(const TankSet& b, int comp, int id, double simtime, doub
 void bl_void_rk3(const TankSet& b, int comp, int id, double Fv, hipStream_t s)
 {
     hipLaunchKernelGGL(kb_void_rk3, tgrid(b), dim3(NT), 0, s, b, comp, id, Fv);
+}
+
+void bl_wave_eta(const TankSet& b, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_wave_eta, tankgrid(b), dim3(NT), 0, s, b);
+}
+void bl_wave_relax(const TankSet& b, int u, int v, int w, int p, int f, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_wave_relax, tgrid(b), dim3(NT), 0, s, b, u, v, w, p, f);
+}
+void bl_wave_time(const TankSet& b, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_wave_time, tankgrid(b), dim3(64), 0, s, b);
 }
 
 void bl_gather_states(const TankSet& b, BiState* dst, hipStream_t s)

@@ -67,6 +67,7 @@ def library():
         "reef_grid_read": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(GridDesc)], ctypes.c_int),
         "reef_grid_file_free": ([vp], None),
         "reef_ctrl_read": ([ctypes.c_char_p, ctypes.POINTER(Params), dp], ctypes.c_int),
+        "reef_ctrl_key": ([ctypes.c_char_p, ctypes.c_char, ctypes.c_int, dp, ctypes.c_int, ip], ctypes.c_int),
         "reef_field_put": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
         "reef_field_get": ([vp, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
         "reef_field_zero": ([vp, ctypes.c_int], ctypes.c_int),
@@ -185,6 +186,15 @@ def read_ctrl_file(path):
     return prm, dict(F49=int(c5[0]), N48=int(c5[1]), N47=float(c5[2]), N40=int(c5[3]), D20=int(c5[4]))
 
 
+def read_ctrl_key(path, key):
+    """The values of one ctrl.txt key such as "N 43" or "B 96" (reef_ctrl_key): a list,
+    empty when the file does not set the key."""
+    letter, num = key.split()
+    vals, n = np.zeros(16), ctypes.c_int(0)
+    _case_check(library().reef_ctrl_key(os.fsencode(path), letter.encode(), int(num), _dp(vals), 16, ctypes.byref(n)))
+    return [float(x) for x in vals[:min(n.value, 16)]]
+
+
 def cfd_state_name(num, rank=1):
     """File name of cfd_state's result file (cfd_state_filename.cpp:27)."""
     return os.path.join("REEF3D_CFD_STATE", "REEF3D-CFD-State-%08i-%06i.r3d" % (num, rank))
@@ -285,6 +295,9 @@ class Domain:
                 top_group=g["group"][6], rho=(prm.W1, prm.W3), visc=(prm.W2, prm.W4),
                 g=(prm.W20, prm.W21, prm.W22), F43=prm.F43, F45=prm.F45, dt=prm.dt, N44=prm.N44, N46=prm.N46,
                 F49=ctrl["F49"], D20=ctrl["D20"])
+        if ctrl["D20"] == 2:
+            n43 = read_ctrl_key(os.path.join(case_dir, "ctrl.txt"), "N 43")
+            D.set_control("N43", n43[0] if n43 else 1.0e-5)   # ini.cpp:589
         if ctrl["N48"]:
             D.set_control("N47", ctrl["N47"])
             D.set_control("N48", 1)

@@ -18,10 +18,13 @@
 #include <cstdlib>
 
 #include "convection.h"
+#include "density_f.h"
 #include "fdm.h"
 #include "field.h"
 #include "ghostcell.h"
+#include "increment.h"
 #include "lexer.h"
+#include "looping.h"
 #include "matrix_diag.h"
 #include "momentum.h"
 #include "pressure.h"
@@ -225,12 +228,16 @@ public:
 // the projection of u, v, w with the face densities of a->phi, assembled, solved and
 // applied on the device (pjm_corr.cpp:90-121); a->press is updated and p->solveriter /
 // p->poissoniter receive the iterations.  The ppois and psolv the driver passes are not
-// used: the assembly and the solve are part of the device projection.  The pressure-
-// gradient members serve the sigma-grid and 2-D shallow-water paths only and are refused.
-class hip_pjm_corr : public pressure
+// used: the assembly and the solve are part of the device projection.  upgrad / vpgrad /
+// wpgrad add the pressure gradient to the host's a->F, a->G, a->H, which momentum_FC3's
+// irhs/jrhs/krhs assemble on the host (momentum_FC3.cpp:167, :184, :201): they run here, on
+// the host, with the caller's own density_f::roface, as pjm_corr::upgrad..wpgrad do
+// (pjm_corr.cpp:185-201).  ucorr / vcorr / wcorr are only called from inside
+// pjm_corr::start and are refused.
+class hip_pjm_corr : public pressure, public increment
 {
 public:
-    explicit hip_pjm_corr(lexer* p) { context(p); }
+    explicit hip_pjm_corr(lexer* p) : pd_(p) { context(p); }
     void start(fdm* a, lexer* p, poisson* ppois, solver* psolv, ghostcell* pgc, ioflow* pflow, field& u, field& v,
                field& w, double alpha) override
     {
@@ -260,9 +267,22 @@ public:
     {
         if(p->B30 != 0) refuse("ini with B30 != 0");
     }
-    void upgrad(lexer*, fdm*, slice&, slice&) override { refuse("upgrad"); }
-    void vpgrad(lexer*, fdm*, slice&, slice&) override { refuse("vpgrad"); }
-    void wpgrad(lexer*, fdm*, slice&, slice&) override { refuse("wpgrad"); }
+    void upgrad(lexer* p, fdm* a, slice&, slice&) override
+    {
+        ULOOP
+        a->F(i, j, k) -= PORVAL1 * (a->press(i + 1, j, k) - a->press(i, j, k)) / (p->DXP[IP] * pd_.roface(p, a, 1, 0, 0));
+    }
+    void vpgrad(lexer* p, fdm* a, slice&, slice&) override
+    {
+        if(p->j_dir == 1)
+            VLOOP
+            a->G(i, j, k) -= PORVAL2 * (a->press(i, j + 1, k) - a->press(i, j, k)) / (p->DYP[JP] * pd_.roface(p, a, 0, 1, 0));
+    }
+    void wpgrad(lexer* p, fdm* a, slice&, slice&) override
+    {
+        WLOOP
+        a->H(i, j, k) -= PORVAL3 * (a->press(i, j, k + 1) - a->press(i, j, k)) / (p->DZP[KP] * pd_.roface(p, a, 0, 0, 1));
+    }
     void ucorr(lexer*, fdm*, field&, double) override { refuse("ucorr"); }
     void vcorr(lexer*, fdm*, field&, double) override { refuse("vcorr"); }
     void wcorr(lexer*, fdm*, field&, double) override { refuse("wcorr"); }
@@ -273,6 +293,7 @@ private:
         std::fprintf(stderr, "reef3d_hip: hip_pjm_corr::%s is not on the supported path\n", what);
         std::exit(1);
     }
+    density_f pd_;
 };
 
 // timestep for N48 = 1 with D20 = 0 or >= 2 (ietimestep, driver_logic_cfd.cpp:59-66):

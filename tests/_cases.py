@@ -14,18 +14,18 @@ CASES = {
     "tank2d": dict(nx=24, ny=1, nz=12, jdir=0, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14),
     # the tank with reinidisc_fsf (F49 = 0)
     "tankfsf": dict(nx=16, ny=6, nz=12, jdir=1, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14, F49=0),
-    # the tank with explicit diffusion (D20 = 1, ediff2) and viscosity 0.01 in both fluids
+    # the tank with explicit diffusion (D20 = 1, ediff2), viscosity 0.01 in water and 0.002 in air
     "tankvisc": dict(nx=16, ny=6, nz=12, jdir=1, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14,
-                     visc=(0.01, 0.01), D20=1),
+                     visc=(0.01, 0.002), D20=1),
     "tank2dvisc": dict(nx=24, ny=1, nz=12, jdir=0, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14,
-                       visc=(0.01, 0.01), D20=1),
+                       visc=(0.01, 0.002), D20=1),
     # operator-level implicit-diffusion fixtures on a stretched 2-D (x-z) box
     "box2d": dict(nx=14, ny=1, nz=9, jdir=0, top_group=21, g=(0.0, 0.0, 0.0), dt=0.01, N44=1e-10),
     # the tanks with implicit diffusion (D20 = 2, idiff2_FS / idiff2_FS_2D, N43 = 1e-5)
     "tankidiff": dict(nx=16, ny=6, nz=12, jdir=1, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14,
-                      visc=(0.01, 0.01), D20=2),
+                      visc=(0.01, 0.002), D20=2),
     "tank2didiff": dict(nx=24, ny=1, nz=12, jdir=0, top_group=3, g=(0.0, 0.0, -9.81), dt=0.005, N44=1e-14,
-                        visc=(0.01, 0.01), D20=2),
+                        visc=(0.01, 0.002), D20=2),
 }
 
 

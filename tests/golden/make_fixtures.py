@@ -16,7 +16,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 CASES = ["box", "nonuni", "vortex", "tank", "tank2d", "tankfsf", "tankvisc", "tank2dvisc", "box2d", "tankidiff",
-         "tank2didiff"]
+         "tank2didiff", "wavebox", "tankwave", "tank2dwave"]
 
 
 def main(cases):

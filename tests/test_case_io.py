@@ -67,23 +67,32 @@ def test_ctrl_file_refuses_other_schemes(tmp_path):
     base = "N 10 1\nD 20 0\nN 40 3\n"
     prm, ctrl = read_ctrl_file(_write(tmp_path, "ok.txt", (base + "/ comment\nN 48 1\nN 47 0.2\n").encode()))
     assert ctrl["N48"] == 1 and ctrl["N47"] == 0.2 and prm.N46 == 250 and prm.W1 == 998.2
-    for text, key in (("D 20 0\nN 40 3\n", "N 10"), ("N 10 1\nN 40 3\n", "D 20"),
+    # without a D 20 line the default idiff2_FS (D20 = 2, ini.cpp:391) is read
+    assert read_ctrl_file(_write(tmp_path, "d.txt", b"N 10 1\nN 40 3\n"))[1]["D20"] == 2
+    for text, key in (("D 20 0\nN 40 3\n", "N 10"), ("N 10 1\nN 40 3\nD 20 4\n", "D 20"),
                       (base + "D 10 5\n", "D 10"), (base + "N 40 2\n", "N 40")):
         with pytest.raises(ReefError, match=key):
             read_ctrl_file(_write(tmp_path, "bad.txt", text.encode()))
 
 
-def test_ctrl_file_explicit_diffusion(tmp_path):
-    """D 20 1 (ediff2) is read from the fixture run's ctrl.txt with its viscosities; it is
-    refused together with N 48 1, and D 20 2 (idiff2_FS) is refused."""
+def test_ctrl_file_diffusion(tmp_path):
+    """D 20 1 (ediff2) and D 20 2 (idiff2_FS) are read from the fixture runs' ctrl.txt with
+    their viscosities; D 20 1 is refused together with N 48 1, D 20 3 is refused; single
+    keys (N 43, the wave keys) come from reef_ctrl_key."""
     from reef3d_amd import ReefError
-    from reef3d_amd.hotpath import read_ctrl_file
+    from reef3d_amd.hotpath import read_ctrl_file, read_ctrl_key
     prm, ctrl = read_ctrl_file(_write(tmp_path, "ctrl.txt", load("tankvisc")["ctrlfile"]))
-    assert ctrl["D20"] == 1 and (prm.W2, prm.W4) == (0.01, 0.01)
+    assert ctrl["D20"] == 1 and (prm.W2, prm.W4) == (0.01, 0.002)
+    prm, ctrl = read_ctrl_file(_write(tmp_path, "ci.txt", load("tankidiff")["ctrlfile"]))
+    assert ctrl["D20"] == 2 and (prm.W2, prm.W4) == (0.01, 0.002)
     assert read_ctrl_file(_write(tmp_path, "c0.txt", load("tank")["ctrlfile"]))[1]["D20"] == 0
-    for text in ("N 10 1\nN 40 3\nD 20 1\nN 48 1\n", "N 10 1\nN 40 3\nD 20 2\n"):
+    for text in ("N 10 1\nN 40 3\nD 20 1\nN 48 1\n", "N 10 1\nN 40 3\nD 20 3\n"):
         with pytest.raises(ReefError, match="D 20"):
             read_ctrl_file(_write(tmp_path, "bad.txt", text.encode()))
+    path = _write(tmp_path, "keys.txt", b"N 43 1e-7\nB 96 0.5 0.75\nB 96 0.25 1.5\n/ B 98 2\n")
+    assert read_ctrl_key(path, "N 43") == [1e-7]
+    assert read_ctrl_key(path, "B 96") == [0.25, 1.5]
+    assert read_ctrl_key(path, "B 98") == []
 
 
 # ---- the reference's FP32 state file (cfd_state)

@@ -125,7 +125,10 @@ def test_batched_tank_matches_oracle():
 def test_batch_slots_and_budget():
     from reef3d_amd import Batch, ReefError
     kw0, f0 = TANKS[0]
-    need = Batch.tank_bytes(**{k: kw0[k] for k in ("nx", "ny", "nz", "XN", "YN", "ZN", "DYM", "DZM", "jdir")})
+    # the grid with its boundary groups (the ghost programs of a free-surface top differ
+    # from a wall's)
+    need = Batch.tank_bytes(**{k: kw0[k] for k in ("nx", "ny", "nz", "XN", "YN", "ZN", "DYM", "DZM", "jdir",
+                                                   "top_group")})
     assert need > 0
     B = Batch(hbm_budget_bytes=2.5 * need)
     a = B.add(**kw0)
