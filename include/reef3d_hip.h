@@ -29,7 +29,8 @@ enum reef_status
     REEF_E_ARG = -1,        /* invalid argument (null pointer, bad id, bad size) */
     REEF_E_UNSUPPORTED = -2,/* geometry or option outside the implemented subset */
     REEF_E_HIP = -3,        /* HIP runtime error (message via reef_last_error) */
-    REEF_E_NOMEM = -4       /* device allocation failed */
+    REEF_E_NOMEM = -4,      /* device allocation failed */
+    REEF_E_STATE = -5       /* call out of order (an allocation-free call on stale tables) */
 };
 
 enum reef_field_id
@@ -170,6 +171,21 @@ int reef_reini_ini(reef_ctx* ctx, int phi_id, int iters, int gcv);
 /* fluid_update_fsf::start (fluid_update_fsf.cpp:39-104): REEF_RO/REEF_VISC from
  * REEF_PHI, ghosts refilled; vol2 (may be NULL) receives volume1, volume2. */
 int reef_fluid_update(reef_ctx* ctx, double* vol2);
+
+/* reef_fluid_update without any device allocation.  No counterpart in the reference: this
+ * entry point is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same kernels and the
+ * same results as reef_fluid_update (bit-identical REEF_RO, REEF_VISC, volume1, volume2),
+ * with the reduction partials and the volumes in a static device array of the library
+ * instead of library-allocated memory: 2 doubles per work item, at most 65536 work items
+ * (1 MiB; about 16.7M cells of 256-cell tiles); a larger tank gives REEF_E_ARG.  The array
+ * is resolved per GPU by reef_ctx_create and held by each GPU until the process ends.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (two host
+ * threads, two contexts) race and must be serialised by the caller.  The context's device
+ * tables must be current: they are after reef_ctx_create and after any other call;
+ * reef_set_control("D20", 2) or a first reef_solver_start rebuilds them on their next call,
+ * and this call returns REEF_E_STATE until then. */
+int reef_fluid_update_noalloc(reef_ctx* ctx, double* vol2);
 
 /* poisson::start + solver::start (poisson_pcorr.cpp:78-231, bicgstab_ijk.cpp:51-278):
  * assembles the 7-point operator on REEF_PHI and solves A x = rhs for field x_id.

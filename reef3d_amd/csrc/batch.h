@@ -93,6 +93,17 @@ void bl_heaviside(const TankSet& b, const double* prm4, hipStream_t s);
 void bl_wave_eta(const TankSet& b, hipStream_t s);
 void bl_wave_relax(const TankSet& b, int u, int v, int w, int p, int f, hipStream_t s);
 void bl_wave_time(const TankSet& b, hipStream_t s);
+// allocation-free fluid update: the partials and the volumes in the device array `scratch`
+// of kernels_batch.hip (NOALLOC_DOUBLES doubles); scratch_dev is its address on the current
+// device from noalloc_scratch.  The set's tanks need item0 + nitems <= NOALLOC_MAX_ITEMS and
+// at most NOALLOC_MAX_TANKS tanks; volumes land at scratch_dev + NOALLOC_VOL0.
+constexpr int NOALLOC_MAX_ITEMS = 65536;
+constexpr int NOALLOC_MAX_TANKS = 64;
+constexpr int NOALLOC_VOL0 = 2 * NOALLOC_MAX_ITEMS;
+constexpr int NOALLOC_DOUBLES = NOALLOC_VOL0 + 2 * NOALLOC_MAX_TANKS;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+int noalloc_scratch(double** out);
+void bl_heaviside_noalloc(const TankSet& b, const double* prm4, double* scratch_dev, hipStream_t s);
 void bl_poisson_rhs(const TankSet& b, int u, int v, int w, double afac, hipStream_t s);
 void bl_face_rho(const TankSet& b, double rw, double ra, int with_aii, hipStream_t s);
 void bl_correct(const TankSet& b, double rw, double ra, double afac, int u, int v, int w,

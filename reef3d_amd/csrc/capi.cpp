@@ -171,6 +171,16 @@ int reef_ctx_create(const reef_grid_desc* grid, const reef_params* prm, reef_ctx
         drop_engine(c->b);
         return rc;
     }
+    // set-up: the device tables, and the address of the allocation-free scratch on this GPU
+    // (reef_fluid_update_noalloc)
+    double* scratch_dev = nullptr;
+    if((rc = c->b.e->prepare()) || (rc = noalloc_scratch(&scratch_dev)))
+    {
+        if(rc == REEF_E_HIP && c->b.e->err.empty()) c->b.e->err = "hipGetSymbolAddress of the scratch array failed";
+        eng(*c->b.e, rc);
+        drop_engine(c->b);
+        return rc;
+    }
     HIPCHK(hipStreamSynchronize(c->b.stream));
     *out = c.release();
     return 0;
@@ -327,6 +337,16 @@ int reef_fluid_update(reef_ctx* c, double* vol2)
         vol2[0] = e.vol[2 * c->tank];
         vol2[1] = e.vol[2 * c->tank + 1];
     }
+    return 0;
+}
+
+int reef_fluid_update_noalloc(reef_ctx* c, double* vol2)
+{
+    if(check_ctx(c)) return REEF_E_ARG;
+    if(!vol2) return fail(REEF_E_ARG, "null vol2");
+    Engine& e = *c->b.e;
+    int rc;
+    if((rc = e.fluid_update_noalloc(c->tank, vol2))) return eng(e, rc);
     return 0;
 }
 

@@ -741,6 +741,56 @@ int Engine::fluid_update()
     return ghost(REEF_VISC, 4, 1);
 }
 
+// fluid_update of one active tank with its reduction memory in the allocation-free scratch
+// (bl_heaviside_noalloc): no device allocation, no table rebuild, no timing events.  The
+// tables must be current (the context syncs them at creation); vol2 receives volume1,
+// volume2 of the tank.
+int Engine::fluid_update_noalloc(int slot, double* vol2)
+{
+    if(dirty_)
+        return fail(REEF_E_STATE, "allocation-free fluid update: the device tables are out of date (tanks, plans or "
+                                  "operators changed since the last call); call any other entry point once first");
+    const reef_params* p = shared();
+    const TankHost* t = tank(slot);
+    if(!p || !t || !t->active) return fail(REEF_E_ARG, "allocation-free fluid update: no such active tank");
+    int set_of = -1, idx = -1;
+    for(size_t q = 0; q < cur_->size(); ++q)
+    {
+        const TankSet& S = (*cur_)[q];
+        if(S.n > NOALLOC_MAX_TANKS)
+            return fail(REEF_E_ARG, "allocation-free fluid update: more than " + std::to_string(NOALLOC_MAX_TANKS) +
+                                        " tanks in one launch set");
+        for(int m = 0; m < S.n; ++m)
+        {
+            const TankDev& d = tanks[S.slot[m]].dev;
+            if((long)d.item0 + d.shape.nitems > NOALLOC_MAX_ITEMS)
+                return fail(REEF_E_ARG, "allocation-free fluid update: the tanks need " +
+                                            std::to_string((long)d.item0 + d.shape.nitems) +
+                                            " reduction slots, the scratch holds " + std::to_string(NOALLOC_MAX_ITEMS));
+            if(S.slot[m] == slot) set_of = (int)q, idx = m;
+        }
+    }
+    if(set_of < 0) return fail(REEF_E_ARG, "allocation-free fluid update: tank not in the active sets");
+    double* sp = nullptr;
+    if(noalloc_scratch(&sp)) return fail(REEF_E_HIP, "hipGetSymbolAddress of the scratch array failed");
+    const double prm4[4] = {p->W1, p->W2, p->W3, p->W4};
+    for(size_t q = 0; q < cur_->size(); ++q)
+    {
+        bl_heaviside_noalloc((*cur_)[q], prm4, sp, stream);
+        // the next set's launch reuses the volume slots: copy this set's first
+        if((int)q == set_of)
+            EHIP(hipMemcpyAsync(vol2, sp + NOALLOC_VOL0 + 2 * idx, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    const int q_ro = prog_index(4, 1);
+    if(q_ro < 0) return fail(REEF_E_UNSUPPORTED, "ghost program (4, 1) not compiled");
+    each([&](const TankSet& S) {
+        bl_ghost(S, q_ro, REEF_RO, stream);
+        bl_ghost(S, q_ro, REEF_VISC, stream);
+    });
+    EHIP(hipStreamSynchronize(stream));
+    return 0;
+}
+
 int Engine::read_volumes()
 {
     int rc;

@@ -176,6 +176,9 @@ public:
     // F50 = 2; 50 = gcval_iniphi of the initial reinitialisation)
     int reini(int field, int iters, int gcv = 52);
     int fluid_update();
+    // fluid_update of tank slot with the reduction memory in the allocation-free scratch
+    // array (no device allocation); vol2 receives its volume1, volume2
+    int fluid_update_noalloc(int slot, double* vol2);
     int solve(int x, int rhs, double stop, int maxiter, bool mixed);
     // the tank's explicit-operator arrays (TankDev::m), from the pool on first use
     int ensure_matrix(int id);

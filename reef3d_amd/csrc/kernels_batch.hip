@@ -14,6 +14,7 @@
 #include "cell_ops.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace reef {
 namespace bk {
@@ -505,15 +506,18 @@ __global__ void __launch_bounds__(NT) kb_heaviside(TankSet b, double rw, double 
     }
 }
 
-__global__ void __launch_bounds__(NT) kb_heaviside_fin(TankSet b)
+// volume1, volume2 of each tank of the set: into the tank's volume slot, or with out set
+// into out[2 * (tank's index in the set) + 0, 1]
+__global__ void __launch_bounds__(NT) kb_heaviside_fin(TankSet b, double* __restrict__ out)
 {
     CTank& T = tank_at(b, blockIdx.x);
     double tot[2];
     tank_totals<2>(T, b, tot);
+    double* dst = out ? out + 2 * blockIdx.x : T.vol;
     if(threadIdx.x == 0)
     {
-        T.vol[0] = tot[0];
-        T.vol[1] = tot[1];
+        dst[0] = tot[0];
+        dst[1] = tot[1];
     }
 }
 
@@ -2562,9 +2566,46 @@ void bl_momentum(const TankSet& b, int comp, int stage, int qin, int qold, int u
 void bl_heaviside(const TankSet& b, const double* prm4, hipStream_t s)
 {
     hipLaunchKernelGGL(kb_heaviside, pgrid(b), dim3(NT), 0, s, b, prm4[0], prm4[1], prm4[2], prm4[3]);
-    hipLaunchKernelGGL(kb_heaviside_fin, tankgrid(b), dim3(NT), 0, s, b);
+    hipLaunchKernelGGL(kb_heaviside_fin, tankgrid(b), dim3(NT), 0, s, b, (double*)nullptr);
 }
 
+// ------------------------------------------------------------------ allocation-free scratch
+// The reduction memory of the allocation-free fluid update (reef_fluid_update_noalloc): a
+// file-scope device array instead of library-allocated memory.  Layout: the block partials
+// (2 doubles per work item of the set, the slots kb_heaviside writes) from element 0, the
+// set's volumes (2 doubles per tank) from element NOALLOC_VOL0.  Every GPU the library
+// runs on holds its own copy until the process ends.  Host code never passes the array's
+// name to a kernel (in host code the name is a stand-in, not a device address): it takes
+// the address hipGetSymbolAddress gives for the current device, once per device.
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+}
+
+int noalloc_scratch(double** out)
+{
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return REEF_E_HIP;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if(!g_scratch_ptr[dev])
+    {
+        void* ptr = nullptr;
+        if(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return REEF_E_HIP;
+        g_scratch_ptr[dev] = (double*)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+void bl_heaviside_noalloc(const TankSet& b, const double* prm4, double* scratch_dev, hipStream_t s)
+{
+    TankSet t = b;
+    t.partials = scratch_dev;
+    hipLaunchKernelGGL(kb_heaviside, pgrid(t), dim3(NT), 0, s, t, prm4[0], prm4[1], prm4[2], prm4[3]);
+    hipLaunchKernelGGL(kb_heaviside_fin, tankgrid(t), dim3(NT), 0, s, t, scratch_dev + NOALLOC_VOL0);
+}
 void bl_poisson_rhs(const TankSet& b, int u, int v, int w, double afac, hipStream_t s)
 {
     hipLaunchKernelGGL(kb_poisson_rhs, tgrid(b), dim3(NT), 0, s, b, u, v, w, afac);

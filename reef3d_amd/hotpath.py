@@ -76,6 +76,7 @@ def library():
         "reef_reini": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "reef_reini_ini": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "reef_fluid_update": ([vp, dp], ctypes.c_int),
+        "reef_fluid_update_noalloc": ([vp, dp], ctypes.c_int),
         "reef_poisson_solve": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ip, dp], ctypes.c_int),
         "reef_poisson_solve_mixed": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ip, ip, dp],
                                      ctypes.c_int),
@@ -369,6 +370,13 @@ class Domain:
     def fluid_update(self):
         vol = np.zeros(2)
         _check(library().reef_fluid_update(self.h, _dp(vol)))
+        return vol
+
+    def fluid_update_noalloc(self):
+        """fluid_update with no device allocation (reef_fluid_update_noalloc): the reduction
+        memory is the library's static device scratch; volume1, volume2 returned."""
+        vol = np.zeros(2)
+        _check(library().reef_fluid_update_noalloc(self.h, _dp(vol)))
         return vol
 
     # ---------------------------------------------------------------- projection
