@@ -139,8 +139,10 @@ int reef_ctx_create(const reef_grid_desc* grid, const reef_params* prm, reef_ctx
  * "D20": momentum diffusion, 0 = diff_void (default here), 1 = ediff2, the explicit viscous
  * terms of ediff2::diff_u/v/w (ediff2.cpp:78-190) added to F/G/H after the convection in
  * every momentum_FC3 predictor (momentum_FC3.cpp:170, 187, 204, 272-306, 370-404), laminar
- * (eddyv 0); not with N48 = 1, which the reference steps with another time-step class.
- * Other keys: REEF_E_UNSUPPORTED. */
+ * (eddyv 0); not with N48 = 1, which the reference steps with another time-step class;
+ * 2 = idiff2_FS, the implicit viscous solves (idiff2_FSu.cpp:40-278, var 1..3, stop "N43",
+ * default 1e-5, ini.cpp:589).  "simtime": the simulation time of the wave generator
+ * (reef_wave_set; p->simtime).  Other keys: REEF_E_UNSUPPORTED. */
 int reef_set_control(reef_ctx* ctx, const char* key, double value);
 void reef_ctx_destroy(reef_ctx* ctx);
 const char* reef_last_error(void);
@@ -167,6 +169,23 @@ int reef_reini(reef_ctx* ctx, int phi_id, int iters);
  * passes 2*int(p->maxlength/(p->F43*p->DXM))) with the ghost rule gcv = gcval_iniphi
  * (50; 52 when F50 = 2 and F60 or F62 is set, reini_RK3.cpp:44-57) after every stage. */
 int reef_reini_ini(reef_ctx* ctx, int phi_id, int iters, int gcv);
+
+/* ioflow = iowave (B90 = 1, driver_logic_cfd.cpp:512-513) with relaxation-zone generation
+ * and the numerical beach (iowave::iowave + ini, iowave.cpp:32-263, iowave_ini.cpp:27-46):
+ * the static zones and factors of the context's tank; params5 (may be null) gets wT, wL,
+ * wk, ww and the amplitude.  From then on reef_fc3_step runs, as driver::loop_cfd does
+ * around momentum_FC3 (driver_loop_cfd.cpp:78-100), wavegen_precalc at the tank's simtime,
+ * u/v/w_relax of the fields and the fluid update before the step, the relaxations
+ * momentum_FC3 calls inside it (phi_relax after each phi stage, u/v/w/p_relax after each
+ * projection, momentum_FC3.cpp:145, 218-221, 244, 320-323, 346, 418-421), and
+ * simtime += dt after it.  The simtime starts at 0; reef_set_control(ctx, "simtime", t)
+ * sets it.  Outside the zones nothing changes. */
+int reef_wave_set(reef_ctx* ctx, const reef_wave_desc* w, double* params5);
+/* wavegen_precalc at simtime (iowave_precalc_relax.cpp:27-160), then iowave's relaxations
+ * (iowave_relax.cpp:28-395): u_relax, v_relax, w_relax of fields u_id, v_id, w_id with the
+ * band of REEF_PHI (u_id < 0: skipped), p_relax of press_id with REEF_RO (< 0: skipped),
+ * phi_relax of f_id (< 0: skipped); the tank's simtime becomes simtime. */
+int reef_wave_relax(reef_ctx* ctx, double simtime, int u_id, int v_id, int w_id, int press_id, int f_id);
 
 /* fluid_update_fsf::start (fluid_update_fsf.cpp:39-104): REEF_RO/REEF_VISC from
  * REEF_PHI, ghosts refilled; vol2 (may be NULL) receives volume1, volume2. */
@@ -304,6 +323,8 @@ void reef_batch_destroy(reef_batch* b);
 long reef_batch_tank_bytes(const reef_grid_desc* grid);
 int reef_batch_add(reef_batch* b, const reef_grid_desc* grid, const reef_params* prm, int* tank);
 int reef_batch_remove(reef_batch* b, int tank);
+/* reef_wave_set for one tank of a batch (its simtime, advanced by its own dt) */
+int reef_batch_wave_set(reef_batch* b, int tank, const reef_wave_desc* w, double* params5);
 /* tanks the next operations advance (n = 0: every tank) */
 int reef_batch_set_active(reef_batch* b, const int* tanks, int n);
 int reef_batch_info(reef_batch* b, int* slots, int* active, double* bytes_in_use);

@@ -21,6 +21,15 @@ class Params(ctypes.Structure):
                 ("dt", ctypes.c_double), ("N44", ctypes.c_double), ("N46", ctypes.c_int)]
 
 
+class WaveDesc(ctypes.Structure):
+    """orc_wave_desc (oracle/reef_oracle.h)."""
+    _fields_ = [("B92", ctypes.c_int), ("B91", ctypes.c_int), ("B91_1", ctypes.c_double), ("B91_2", ctypes.c_double),
+                ("B93", ctypes.c_int), ("B93_1", ctypes.c_double), ("B93_2", ctypes.c_double),
+                ("B96_1", ctypes.c_double), ("B96_2", ctypes.c_double), ("B98", ctypes.c_int), ("B99", ctypes.c_int),
+                ("B101", ctypes.c_int), ("B102", ctypes.c_double), ("B105_1", ctypes.c_double),
+                ("B120", ctypes.c_double), ("F60", ctypes.c_double), ("W50", ctypes.c_double)]
+
+
 def build():
     import subprocess
     root = os.path.dirname(_HERE)
@@ -70,6 +79,10 @@ def lib():
         L.orc_vortex_step.argtypes = [ctypes.c_void_p] + [dp] * 6 + [ctypes.c_double, dp]
         L.orc_phi_rk3.argtypes = [ctypes.c_void_p] + [dp] * 4
         L.orc_timestep.argtypes = [ctypes.c_void_p] + [dp] * 3 + [ctypes.c_double, ctypes.c_int, dp]
+        L.orc_wave_set.argtypes = [ctypes.c_void_p, ctypes.POINTER(WaveDesc), dp]
+        L.orc_wave_params.argtypes = [ctypes.c_void_p, dp]
+        L.orc_set_simtime.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_wave_relax.argtypes = [ctypes.c_void_p, ctypes.c_double] + [dp] * 7
         _LIB = L
     return _LIB
 
@@ -205,6 +218,21 @@ class Domain:
 
     def phi_rk3(self, phi, u, v, w):
         lib().orc_phi_rk3(self.h, _p(phi), _p(u), _p(v), _p(w))
+
+    def wave_set(self, desc, phi):
+        """iowave set-up from a WaveDesc (fields as reef_wave_desc) and psi from phi (inipsi
+        with B90 > 0); returns (wT, wL, wk, ww, wA)."""
+        lib().orc_wave_set(self.h, ctypes.byref(desc), _p(np.ascontiguousarray(phi)))
+        out = np.zeros(5)
+        lib().orc_wave_params(self.h, _p(out))
+        return tuple(float(x) for x in out)
+
+    def set_simtime(self, t):
+        lib().orc_set_simtime(self.h, float(t))
+
+    def wave_relax(self, simtime, phi, ro, u, v, w, press, f):
+        """wavegen_precalc at simtime, then u/v/w_relax, p_relax and phi_relax in place."""
+        lib().orc_wave_relax(self.h, float(simtime), _p(phi), _p(ro), _p(u), _p(v), _p(w), _p(press), _p(f))
 
     def timestep(self, u, v, w, N47=0.3, ini=False):
         """ietimestep::start / ::ini: (dt, umax, vmax, wmax)."""

@@ -2137,9 +2137,22 @@ int orc_fc3_step(orc_domain* d, double* phi, double* u, double* v, double* w, do
     return it;
 }
 
-void orc_wave_set(orc_domain* d, const orc_wave_desc* w)
+void orc_wave_set(orc_domain* d, const orc_wave_desc* w, const double* phi)
 {
     wave_setup(d, *w);
+    // initialize::inipsi with B90 > 0 (ini_psi.cpp:40-56): psi from the mean DZN of the
+    // cells within 5 DZM of the interface, in LOOP order
+    int count = 0;
+    double psim = 0.0;
+    for(int i = 0; i < d->nx; ++i)
+    for(int j = 0; j < d->ny; ++j)
+    for(int k = 0; k < d->nz; ++k)
+        if(fabs(phi[d->id(i, j, k)]) < 5.0 * d->DZM)
+        {
+            psim += d->DN[2][k + MARGE];
+            ++count;
+        }
+    d->psi = d->prm.F45 * psim / double(count);
 }
 
 void orc_wave_params(const orc_domain* d, double* out5)

@@ -123,7 +123,8 @@ typedef struct orc_wave_desc
     int B98, B99, B101;
     double B102, B105_1, B120, F60, W50;
 } orc_wave_desc;
-void orc_wave_set(orc_domain*, const orc_wave_desc*);
+/* also sets psi from phi as initialize::inipsi does for wave runs (B90 > 0) */
+void orc_wave_set(orc_domain*, const orc_wave_desc*, const double* phi);
 /* wT, wL, wk, ww, wA */
 void orc_wave_params(const orc_domain*, double* out5);
 void orc_set_simtime(orc_domain*, double t);

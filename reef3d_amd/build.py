@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libreef3d_hip.so")
 OBJ = os.path.join(HERE, "_obj")
-SOURCES = ["grid_host.cpp", "engine.cpp", "capi.cpp", "server.cpp", "case_io.cpp", "kernels_batch.hip"]
+SOURCES = ["grid_host.cpp", "engine.cpp", "capi.cpp", "server.cpp", "case_io.cpp", "wave_host.cpp", "kernels_batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

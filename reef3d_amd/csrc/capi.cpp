@@ -81,6 +81,8 @@ int control(Engine& e, const char* key, double value)
         e.fsf = value == 0.0 ? 1 : 0;
         return 0;
     }
+    if(std::strcmp(key, "simtime") == 0)
+        return eng(e, e.set_simtime(value));
     if(std::strcmp(key, "graphs") == 0)
     {
         if(value != 0.0 && value != 1.0) return fail(REEF_E_ARG, "graphs must be 0 or 1");
@@ -348,6 +350,29 @@ int reef_fluid_update_noalloc(reef_ctx* c, double* vol2)
     int rc;
     if((rc = e.fluid_update_noalloc(c->tank, vol2))) return eng(e, rc);
     return 0;
+}
+
+int reef_wave_set(reef_ctx* c, const reef_wave_desc* w, double* params5)
+{
+    if(check_ctx(c)) return REEF_E_ARG;
+    if(!w) return fail(REEF_E_ARG, "null argument");
+    return eng(*c->b.e, c->b.e->wave_set(c->tank, *w, params5));
+}
+
+int reef_wave_relax(reef_ctx* c, double simtime, int u_id, int v_id, int w_id, int press_id, int f_id)
+{
+    if(check_ctx(c)) return REEF_E_ARG;
+    const bool vel = u_id >= 0;
+    if((vel && (!field_ok(u_id) || !field_ok(v_id) || !field_ok(w_id))) || press_id >= REEF_NFIELDS ||
+       f_id >= REEF_NFIELDS)
+        return fail(REEF_E_ARG, "bad field id");
+    Engine& e = *c->b.e;
+    const TankHost* t = e.tank(c->tank);
+    if(!t || !t->dev.wave.col) return fail(REEF_E_ARG, "no wave generator on this context (reef_wave_set)");
+    int rc;
+    if((rc = e.set_simtime(simtime))) return eng(e, rc);
+    return eng(e, e.wave_relax(vel ? u_id : -1, vel ? v_id : -1, vel ? w_id : -1, press_id < 0 ? -1 : press_id,
+                               f_id < 0 ? -1 : f_id));
 }
 
 int reef_poisson_solve(reef_ctx* c, int x_id, int rhs_id, double stop, int maxiter, int* iters, double* residual)
@@ -710,6 +735,13 @@ int reef_batch_add(reef_batch* b, const reef_grid_desc* grid, const reef_params*
     if(check_batch(b)) return REEF_E_ARG;
     if(!grid || !prm || !grid->XN || !grid->YN || !grid->ZN) return fail(REEF_E_ARG, "null argument");
     return eng(*b->e, b->e->add_tank(*grid, *prm, tank));
+}
+
+int reef_batch_wave_set(reef_batch* b, int tank, const reef_wave_desc* w, double* params5)
+{
+    if(check_batch(b)) return REEF_E_ARG;
+    if(!w) return fail(REEF_E_ARG, "null argument");
+    return eng(*b->e, b->e->wave_set(tank, *w, params5));
 }
 
 int reef_batch_remove(reef_batch* b, int tank)

@@ -1032,6 +1032,101 @@ int Engine::ensure_matrix(int id)
     return 0;
 }
 
+int Engine::wave_set(int id, const reef_wave_desc& w, double* params5)
+{
+    TankHost* t = tank(id);
+    if(!t) return fail(REEF_E_ARG, "no such tank");
+    if(t->dev.wave.col) return fail(REEF_E_ARG, "the tank already has a wave generator");
+    WaveSetup ws;
+    const std::string e = build_wave(t->hg, w, ws);
+    if(!e.empty()) return fail(REEF_E_UNSUPPORTED, e);
+    HostGrid& h = t->hg;
+    const size_t ncol = (size_t)h.nx * h.ny, nzn = h.X[2].size();
+    const reef_params* sp = shared();
+    ws.dev.gabs = std::fabs(sp ? sp->W22 : t->prm.W22);
+    void* m[3] = {nullptr, nullptr, nullptr};
+    const size_t bytes[3] = {WC_N * ncol * 8, ncol * 8, nzn * 8};
+    int rc;
+    if((rc = drain())) return rc;
+    EHIP(hipStreamSynchronize(stream));
+    // initialize::inipsi with an ioflow of waves (B90 > 0, ini_psi.cpp:40-56): the interface
+    // half width from the mean DZN of the cells within 5 DZM of the current phi, summed in
+    // LOOP order
+    {
+        std::vector<double> phi((size_t)(h.nx + 6) * (h.ny + 6) * (h.nz + 6));
+        if((rc = get(id, REEF_PHI, phi.data(), REEF_LAYOUT_IFAST))) return rc;
+        double psim = 0.0;
+        int count = 0;
+        for(int i = 0; i < h.nx; ++i)
+            for(int j = 0; j < h.ny; ++j)
+                for(int k = 0; k < h.nz; ++k)
+                    if(std::fabs(phi[(size_t)h.cid(i, j, k)]) < 5.0 * h.DZM)
+                    {
+                        psim += h.DN[2][k + MARGE];
+                        ++count;
+                    }
+        if(count == 0) return fail(REEF_E_ARG, "wave set-up: no cell within 5 DZM of the interface (set phi first)");
+        h.psi = t->prm.F45 * psim / double(count);
+        t->dev.g.psi = h.psi;
+    }
+    for(int q = 0; q < 3; ++q)
+    {
+        if((rc = pool.alloc(bytes[q], &m[q])))
+        {
+            for(int r = 0; r < q; ++r) pool.free(m[r]);
+            return fail(rc, "device pool: out of memory (budget) for the wave generator");
+        }
+    }
+    const std::vector<double> zero(ncol, 0.0);
+    EHIP(hipMemcpy(m[0], ws.col.data(), bytes[0], hipMemcpyHostToDevice));
+    EHIP(hipMemcpy(m[1], zero.data(), bytes[1], hipMemcpyHostToDevice));
+    EHIP(hipMemcpy(m[2], h.X[2].data(), bytes[2], hipMemcpyHostToDevice));
+    for(int q = 0; q < 3; ++q)
+    {
+        t->mem.push_back(m[q]);
+        t->bytes += up(bytes[q]);
+    }
+    ws.dev.col = (const double*)m[0];
+    ws.dev.eta = (double*)m[1];
+    ws.dev.zn = (const double*)m[2];
+    t->dev.wave = ws.dev;
+    if(params5)
+    {
+        params5[0] = ws.wT;
+        params5[1] = ws.wL;
+        params5[2] = ws.dev.wk;
+        params5[3] = ws.dev.ww;
+        params5[4] = ws.dev.wa;
+    }
+    any_wave_ = true;
+    drop_graphs();      // captured steps have no wave launches
+    dirty_ = true;
+    return 0;
+}
+
+int Engine::set_simtime(double tm)
+{
+    int rc;
+    if((rc = sync_tables()) || (rc = drain())) return rc;
+    EHIP(hipStreamSynchronize(stream));
+    for(TankHost& t : tanks)
+        if(t.used && t.active && t.dev.wave.col) EHIP(hipMemcpy(t.dev.tstep + 6, &tm, sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int Engine::wave_relax(int u, int v, int w, int p, int f)
+{
+    int rc;
+    if((rc = sync_tables()) || (rc = drain())) return rc;
+    tbegin("wave");
+    each([&](const TankSet& S) {
+        bl_wave_eta(S, stream);
+        bl_wave_relax(S, u, v, w, p, f, stream);
+    });
+    tend();
+    return 0;
+}
+
 // solver::start (bicgstab_ijk / bicgstab_ijk_2D, bicgstab_ijk.cpp:51-123) on the explicit
 // operator of every active tank: var 1..3 over the u / v / w grids (stop = N43), 4, 5 over
 // every cell (N44); the solution in place in array x
@@ -1197,6 +1292,13 @@ int Engine::fc3_segment(int s, bool mixed)
     {
         const int* vel = VEL[s - 1];
         if((rc = pjm_post(vel[0], vel[1], vel[2], ALPHA[s - 1], mixed))) return rc;
+        // pflow->u_relax, v_relax, w_relax, p_relax (momentum_FC3.cpp:218-221, 320-323, 418-421)
+        if(any_wave_)
+        {
+            tbegin("wave");
+            each([&](const TankSet& S) { bl_wave_relax(S, vel[0], vel[1], vel[2], REEF_PRESS, -1, stream); });
+            tend();
+        }
         if((rc = ghost(vel[0], 1, 10)) || (rc = ghost(vel[1], 2, 11)) || (rc = ghost(vel[2], 3, 12))) return rc;
     }
     if(s == 0 && n48)
@@ -1205,6 +1307,19 @@ int Engine::fc3_segment(int s, bool mixed)
         tbegin("timestep");
         each([&](const TankSet& S) { bl_timestep(S, REEF_U, REEF_V, REEF_W, n47, 0, stream); });
         tend();
+    }
+    if(s == 0 && any_wave_)
+    {
+        // the driver loop before momentum (driver_loop_cfd.cpp:78-92): wavegen_precalc at the
+        // step's simtime, u_relax, v_relax, w_relax of the fdm's velocities, then
+        // levelset_void::update, the fluid update (levelset_void.cpp:86-89)
+        tbegin("wave");
+        each([&](const TankSet& S) {
+            bl_wave_eta(S, stream);
+            bl_wave_relax(S, REEF_U, REEF_V, REEF_W, -1, -1, stream);
+        });
+        tend();
+        if((rc = fluid_update())) return rc;
     }
     if(s == 0)
     {
@@ -1215,6 +1330,7 @@ int Engine::fc3_segment(int s, bool mixed)
         tbegin("hj");
         each([&](const TankSet& S) { bl_hj_stage(S, 1, A_LS, -1, REEF_U, REEF_V, REEF_W, A_FRK1, 1.0, stream); });
         tend();
+        if(any_wave_) wave_phi(A_FRK1);         // pflow->phi_relax (momentum_FC3.cpp:145)
         if((rc = ghost(A_FRK1, 4, 52))) return rc;
         if((rc = reini(A_FRK1, 2))) return rc;
         tbegin("copy");
@@ -1222,9 +1338,10 @@ int Engine::fc3_segment(int s, bool mixed)
         tend();
         if((rc = ghost(REEF_PHI, 4, 52))) return rc;
         // the stage's fluid update (pupdate, momentum_FC3.cpp:157-159): the diffusion reads the
-        // mixture viscosity of this stage's phi; without diffusion nothing in the step reads
-        // rho or nu before the final update (face densities come from phi)
-        if(d20 && (rc = fluid_update())) return rc;
+        // mixture viscosity of this stage's phi, the beach's pressure relaxation its density;
+        // otherwise nothing in the step reads rho or nu before the final update (face
+        // densities come from phi)
+        if((d20 || any_wave_) && (rc = fluid_update())) return rc;
         const int qin[3] = {REEF_U, REEF_V, REEF_W};
         if((rc = momentum_stage(1, qin, qin, qin, VEL[0]))) return rc;
     }
@@ -1234,6 +1351,7 @@ int Engine::fc3_segment(int s, bool mixed)
         tbegin("hj");
         each([&](const TankSet& S) { bl_hj_stage(S, 2, A_FRK1, A_LS, A_URK1, A_VRK1, A_WRK1, A_FRK2, 0.25, stream); });
         tend();
+        if(any_wave_) wave_phi(A_FRK2);         // momentum_FC3.cpp:244
         if((rc = ghost(A_FRK2, 4, 52))) return rc;
         if((rc = reini(A_FRK2, 2))) return rc;
         tbegin("copy");
@@ -1241,7 +1359,7 @@ int Engine::fc3_segment(int s, bool mixed)
         tend();
         if((rc = ghost(REEF_PHI, 4, 52))) return rc;
         if((rc = reini(REEF_PHI, 2))) return rc;
-        if(d20 && (rc = fluid_update())) return rc;   // momentum_FC3.cpp:260
+        if((d20 || any_wave_) && (rc = fluid_update())) return rc;   // momentum_FC3.cpp:260
         const int qold[3] = {REEF_U, REEF_V, REEF_W};
         if((rc = momentum_stage(2, VEL[0], qold, VEL[0], VEL[1]))) return rc;
     }
@@ -1253,15 +1371,29 @@ int Engine::fc3_segment(int s, bool mixed)
             bl_hj_stage(S, 3, A_FRK2, A_LS, A_URK2, A_VRK2, A_WRK2, REEF_PHI, 2.0 / 3.0, stream);
         });
         tend();
+        if(any_wave_) wave_phi(REEF_PHI);       // momentum_FC3.cpp:346
         if((rc = ghost(REEF_PHI, 4, 52))) return rc;
         if((rc = reini(REEF_PHI, 3))) return rc;
-        if(d20 && (rc = fluid_update())) return rc;   // momentum_FC3.cpp:358
+        if((d20 || any_wave_) && (rc = fluid_update())) return rc;   // momentum_FC3.cpp:358
         const int qold[3] = {REEF_U, REEF_V, REEF_W};
         if((rc = momentum_stage(3, VEL[1], qold, VEL[1], VEL[2]))) return rc;
     }
     else
-        return fluid_update();
+    {
+        if((rc = fluid_update())) return rc;
+        // p->simtime += p->dt after the step (driver_loop_cfd.cpp:100)
+        if(any_wave_) each([&](const TankSet& S) { bl_wave_time(S, stream); });
+        return 0;
+    }
     return pjm_pre(VEL[s][0], VEL[s][1], VEL[s][2], ALPHA[s], mixed);
+}
+
+// iowave::phi_relax (iowave_relax.cpp:358-395) of field f on the active wave tanks
+void Engine::wave_phi(int f)
+{
+    tbegin("wave");
+    each([&](const TankSet& S) { bl_wave_relax(S, -1, -1, -1, -1, f, stream); });
+    tend();
 }
 
 // momentum_FC3::start (momentum_FC3.cpp:122-430): three TVD-RK3 stages of phi transport,

@@ -182,6 +182,16 @@ public:
     int solve(int x, int rhs, double stop, int maxiter, bool mixed);
     // the tank's explicit-operator arrays (TankDev::m), from the pool on first use
     int ensure_matrix(int id);
+    // relaxation-zone wave generation and numerical beach on tank id (iowave, B98 = 2): the
+    // static set-up from the pool; params5 (may be null) receives wT, wL, wk, ww, wA.  From
+    // then on every fc3 step of the tank runs the driver loop's wave prologue and the
+    // relaxations inside momentum_FC3, and advances the tank's simtime by its dt.
+    int wave_set(int id, const reef_wave_desc& w, double* params5);
+    // the simulation time of the active wave tanks (p->simtime)
+    int set_simtime(double t);
+    // wavegen_precalc at the tanks' simtime, then the relaxations of iowave on the given
+    // fields of every active wave tank (u >= 0: u, v, w; p >= 0: pressure; f >= 0: phi_relax)
+    int wave_relax(int u, int v, int w, int p, int f);
     // solver::start on the explicit operators of the active tanks (bicgstab_ijk with var
     // 1..5), the solution in place in array x
     int mat_solve(int var, int x, double stop, int maxiter);
@@ -242,6 +252,7 @@ private:
     hipEvent_t st_ev_[2] = {nullptr, nullptr};
     long batch_no_ = 0;
     bool dirty_ = true;
+    bool any_wave_ = false;                // some tank has a wave generator
     std::vector<TankSet> all_;             // the active tanks (chunks of SET_MAX)
     const std::vector<TankSet>* cur_ = &all_;
 
@@ -299,6 +310,7 @@ private:
     int pjm_pre(int u, int v, int w, double alpha, bool mixed);
     int pjm_post(int u, int v, int w, double alpha, bool mixed);
     int fc3_segment(int s, bool mixed);
+    void wave_phi(int f);
     int probe(double& stencil_ms, double& solver_ms);
     int momentum_stage(int stage, const int qin[3], const int qold[3], const int vel[3], const int out[3]);
 };

@@ -43,6 +43,25 @@ class Params(ctypes.Structure):
                 ("dt", ctypes.c_double), ("N44", ctypes.c_double), ("N46", ctypes.c_int)]
 
 
+class WaveDesc(ctypes.Structure):
+    """reef_wave_desc: the iowave keys of ctrl.txt (B91..B120, F60, W50)."""
+    _fields_ = [("B92", ctypes.c_int), ("B91", ctypes.c_int), ("B91_1", ctypes.c_double), ("B91_2", ctypes.c_double),
+                ("B93", ctypes.c_int), ("B93_1", ctypes.c_double), ("B93_2", ctypes.c_double),
+                ("B96_1", ctypes.c_double), ("B96_2", ctypes.c_double), ("B98", ctypes.c_int), ("B99", ctypes.c_int),
+                ("B101", ctypes.c_int), ("B102", ctypes.c_double), ("B105_1", ctypes.c_double),
+                ("B120", ctypes.c_double), ("F60", ctypes.c_double), ("W50", ctypes.c_double)]
+
+
+def wave_desc(F60, height, length=None, period=None, gen=0.0, beach=0.0, B99=2, B101=0, B102=1.0, angle=0.0,
+              B120=-90.0, W50=0.0, B98=2):
+    """A reef_wave_desc of linear waves (B92 = 2) of the given height and either length
+    (B91) or period (B93) over still water at F60; gen / beach: the zone half widths B96."""
+    if (length is None) == (period is None):
+        raise ValueError("give either the wave length or the period")
+    return WaveDesc(2, 1 if length is not None else 0, height, length or 0.0, 1 if period is not None else 0, height,
+                    period or 0.0, gen, beach, B98, B99, B101, B102, angle, B120, F60, W50)
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_double), ("max_live", ctypes.c_int), ("precision", ctypes.c_int),
                 ("delta", ctypes.c_double), ("graphs", ctypes.c_int), ("F49", ctypes.c_int), ("shared", Params),
@@ -108,6 +127,9 @@ def library():
         "reef_batch_tank_bytes": ([ctypes.POINTER(GridDesc)], ctypes.c_long),
         "reef_batch_add": ([vp, ctypes.POINTER(GridDesc), ctypes.POINTER(Params), ip], ctypes.c_int),
         "reef_batch_remove": ([vp, ctypes.c_int], ctypes.c_int),
+        "reef_batch_wave_set": ([vp, ctypes.c_int, ctypes.POINTER(WaveDesc), dp], ctypes.c_int),
+        "reef_wave_set": ([vp, ctypes.POINTER(WaveDesc), dp], ctypes.c_int),
+        "reef_wave_relax": ([vp, ctypes.c_double] + [ctypes.c_int] * 5, ctypes.c_int),
         "reef_batch_set_active": ([vp, ip, ctypes.c_int], ctypes.c_int),
         "reef_batch_info": ([vp, ip, ip, dp], ctypes.c_int),
         "reef_batch_put": ([vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int], ctypes.c_int),
@@ -372,6 +394,19 @@ class Domain:
         _check(library().reef_fluid_update(self.h, _dp(vol)))
         return vol
 
+    # ---------------------------------------------------------------- ioflow = iowave
+    def wave_set(self, desc):
+        """Relaxation-zone wave generation and beach (reef_wave_set) from a WaveDesc
+        (wave_desc(...)); returns (wT, wL, wk, ww, wA)."""
+        out = np.zeros(5)
+        _check(library().reef_wave_set(self.h, ctypes.byref(desc), _dp(out)))
+        return tuple(float(x) for x in out)
+
+    def wave_relax(self, simtime, u=None, v=None, w=None, press=None, f=None):
+        """wavegen_precalc at simtime and iowave's relaxations of the named fields."""
+        fid = lambda n: -1 if n is None else FIELDS[n]
+        _check(library().reef_wave_relax(self.h, float(simtime), fid(u), fid(v), fid(w), fid(press), fid(f)))
+
     def fluid_update_noalloc(self):
         """fluid_update with no device allocation (reef_fluid_update_noalloc): the reduction
         memory is the library's static device scratch; volume1, volume2 returned."""
@@ -543,6 +578,12 @@ class Batch:
         _check(library().reef_batch_add(self.h, ctypes.byref(gd), ctypes.byref(prm), ctypes.byref(tid)))
         self._keep[tid.value] = ((nz + 6, ny + 6, nx + 6), keep)
         return tid.value
+
+    def wave_set(self, tank, desc):
+        """reef_batch_wave_set: a wave generator on one tank; returns (wT, wL, wk, ww, wA)."""
+        out = np.zeros(5)
+        _check(library().reef_batch_wave_set(self.h, tank, ctypes.byref(desc), _dp(out)))
+        return tuple(float(x) for x in out)
 
     def remove(self, tank):
         _check(library().reef_batch_remove(self.h, tank))

@@ -8,7 +8,7 @@ fixtures.
 import numpy as np
 import pytest
 
-from tests._cases import CASES, domain_args, ghost_cases, inner, load
+from tests._cases import CASES, WAVES, domain_args, ghost_cases, inner, load
 
 
 def oracle(case, d):
@@ -237,3 +237,51 @@ def test_threaded_oracle_fc3_steps(case):
         ref = inner(d[f"tk_{n}_1"])
         err = np.abs(inner(runs[0][n]) - ref).max() / (np.abs(ref).max() or 1.0)
         assert err < 1e-8, f"{n}: {err:.2e}"
+
+
+# ------------------------------------------------------------------ iowave
+def _owave(case, d, phi_key):
+    from oracle.oracle import WaveDesc
+    from reef3d_amd.hotpath import wave_desc
+    O = oracle(case, d)
+    w = wave_desc(**WAVES[case])
+    params = O.wave_set(WaveDesc.from_buffer_copy(w), d[phi_key])
+    return O, params
+
+
+@pytest.mark.parametrize("case,key", [("wavebox", "wv_scalars"), ("tankwave", "tk_wave"), ("tank2dwave", "tk_wave")])
+def test_wave_parameters(case, key):
+    """wave_lib_parameters for linear waves (B91): wT, wL, wk, ww and the amplitude."""
+    d = load(case)
+    _, params = _owave(case, d, "wv0_phi" if case == "wavebox" else "tk_phi_0")
+    assert params == tuple(d[key][:5])
+
+
+def test_wave_relaxation_operators():
+    """u/v/w_relax, p_relax and phi_relax after wavegen_precalc, during the ramp and after it."""
+    d = load("wavebox")
+    O, _ = _owave("wavebox", d, "wv0_phi")
+    for q in (0, 1):
+        t = f"wv{q}_"
+        u, v, w, p = (d[t + n + "_in"].copy() for n in ("u", "v", "w", "press"))
+        f = d[t + "phi"].copy()
+        O.wave_relax(float(d[t + "time"][0]), d[t + "phi"].copy(), d[t + "ro"].copy(), u, v, w, p, f)
+        for n, a in (("u", u), ("v", v), ("w", w), ("press", p), ("f", f)):
+            assert same(a, d[t + n + "_out"]), f"{t}{n}"
+
+
+@pytest.mark.parametrize("case", ["tankwave", "tank2dwave"])
+def test_wave_tank_steps(case):
+    """Three driver-loop steps with the wave generator (prologue + momentum_FC3 with its
+    relaxations), each from the reference's state of the previous step."""
+    d = load(case)
+    O, _ = _owave(case, d, "tk_phi_0")
+    # the reference's psi of the run (initialize::inipsi from the initial phi)
+    assert O.scalars()["psi"] == d["tk_scalars"][0]
+    dt = CASES[case]["dt"]
+    for s in (1, 2, 3):
+        O.set_simtime((s - 1) * dt)
+        f = {n: d[f"tk_{n}_{s - 1}"].copy() for n in ("phi", "u", "v", "w", "press", "ro", "visc")}
+        O.fc3_step(f["phi"], f["u"], f["v"], f["w"], f["press"], f["ro"], f["visc"])
+        for n in ("phi", "u", "v", "w", "press", "ro", "visc"):
+            assert same(inner(f[n]), inner(d[f"tk_{n}_{s}"])), f"step {s} {n}"
