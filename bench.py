@@ -12,8 +12,9 @@ host, uploaded once and stay in HBM; the timed region contains the steps only.
 Prints one JSON line: value = interior cells x steps / wall time (Mcell-updates/s).
 The roofline object is for the dominant kernel (the largest total time among the solver
 kernels), measured with HIP events on the library stream inside the timed region; the
-cpu_baseline object times the CPU oracle (a single-threaded restatement of the
-reference) on a y-slab of the same tank and scales it to the same metric.  The
+cpu_baseline object times the CPU oracle (the C++ restatement of the reference's step,
+OpenMP over i slabs with the host's CPU share of threads) on one whole step of the same
+tank from the same initial fields.  The
 mixed_projection object repeats the timed steps from the same initial state with the
 mixed-precision projection (FP32 BiCGStab with FP64 reliable updates, to the same 1e-10
 FP64 residual) and compares.

@@ -907,13 +907,22 @@ int Engine::enqueue_batch(int kind, int x, int rhs, int ring, int iters)
     {
         std::vector<int> key(solving_key_);
         key.insert(key.end(), {-1, kind, x, rhs, iters});
-        if(set_graphs_.size() > 4096) drop_graphs();
-        SetGraph& sg = set_graphs_[key];
-        if(!sg.exec && ++sg.seen >= 2 && (rc = capture(body, &sg.exec))) return rc;
-        if(sg.exec)
+        // a full cache captures no more sets (their batches launch directly) rather than
+        // destroying executable graphs in the middle of a run: graph execs are destroyed only
+        // when the tables change (drop_graphs, after a stream synchronisation) and at teardown
+        auto it = set_graphs_.find(key);
+        if(it == set_graphs_.end() && set_graphs_.size() < SET_GRAPHS_MAX) it = set_graphs_.emplace(key, SetGraph{}).first;
+        if(it != set_graphs_.end())
         {
-            EHIP(hipGraphLaunch(sg.exec, stream));
-            rc = 0;
+            SetGraph& sg = it->second;
+            if(!sg.exec && ++sg.seen >= 2 && (rc = capture(body, &sg.exec))) return rc;
+            if(sg.exec)
+            {
+                EHIP(hipGraphLaunch(sg.exec, stream));
+                rc = 0;
+            }
+            else
+                rc = body();
         }
         else
             rc = body();

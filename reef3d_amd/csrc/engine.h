@@ -289,6 +289,7 @@ private:
         hipGraphExec_t exec = nullptr;
     };
     std::map<std::vector<int>, SetGraph> set_graphs_;
+    static constexpr size_t SET_GRAPHS_MAX = 4096;   // solving sets with a graph entry
     template<class Body>
     int capture(Body body, hipGraphExec_t* out);
     template<class Body>
