@@ -64,7 +64,7 @@ sys.path.insert(0, ROOT)
 # kernel must read or write once; stencil neighbours are served on chip
 SOLVER_BYTES = {
     "bicg_p": 48,   # r, p, v, aii in; p, ph out
-    "bicg_v": 56,   # ph, 3 face densities, r0, r in; v out
+    "bicg_v": 48,   # ph, 3 face densities, r0 in; v out (|r|^2 comes from the x phase)
     "bicg_t": 56,   # r, v, aii, 3 face densities in; t out
     "bicg_x": 72,   # x, ph, r, v, aii, t, r0 in; x, r out
 }
@@ -203,6 +203,92 @@ def stencil_rooflines(fams, cells):
                   "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "fp64_TFLOPs": round(tf, 2),
                   "fp64_frac": round(tf / FP64_PEAK_TFLOPS, 4), "bound": "valu"}
     return out
+
+
+def hbm_copy(nbytes=1 << 30, reps=5):
+    """Achievable HBM bandwidth: a device-to-device copy of nbytes (1 GiB), read + written
+    bytes over the fastest of `reps` copies timed with HIP events."""
+    import torch
+    src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    ok = bool(torch.equal(dst[:: 1 << 20], src[:: 1 << 20]))
+    del src, dst
+    torch.cuda.empty_cache()
+    gbs = 2.0 * nbytes / (best / 1e3) / 1e9
+    return {"bytes": nbytes, "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "best_ms": round(best, 4),
+            "checked": ok}
+
+
+# kernel name -> the family of the bench's SOLVER_BYTES (the traffic tables of the profiles)
+TRAFFIC_KERNELS = (("kb_bicg_p<double>", "bicg_p"), ("kb_bicg_v<double, true>", "bicg_v"),
+                   ("kb_bicg_t<double, true>", "bicg_t"), ("kb_bicg_x<double>", "bicg_x"),
+                   ("kb_mix_v<true>", "mix_v"), ("kb_bicg_t<float, true>", "bicg32_t"),
+                   ("kb_bicg_x<float>", "bicg32_x"), ("kb_hj_stage", "hj"), ("kb_reini_stage", "reini"))
+
+
+def traffic_tables(fetch_dir, write_dir, cells=512 * 64 * 128, out_dir=None):
+    """HBM bytes per launch of the solver and stencil kernels from two rocprofv3 --pmc runs
+    of `bench.py --steps 1 --warmup 0 --no-cpu --no-solves --no-serve` (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch), each counter scaled by what it reads for the 1 GiB copy
+    of the same run (hbm_copy: 1 GiB read, 1 GiB written).  Writes profiles/traffic_<kernel>.json
+    (read by the default run's roofline object) and prints a markdown table."""
+    import csv
+    import glob
+
+    def per_kernel(d, counter):
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            raise SystemExit(f"no counter_collection.csv under {d}")
+        sums, cnt, copy_best = {}, {}, 0.0
+        for fn in files:
+            with open(fn) as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name, val = row["Kernel_Name"], float(row["Counter_Value"])
+                    if "copy" in name.lower() and "kb_" not in name:
+                        copy_best = max(copy_best, val)
+                    key = (name, row.get("Dispatch_Id", row.get("Correlation_Id")))
+                    sums[key] = sums.get(key, 0.0) + val
+        fam = {}
+        for (name, _), val in sums.items():
+            for kname, f in TRAFFIC_KERNELS:
+                if kname in name:
+                    fam.setdefault(f, []).append(val)
+        return {f: sum(v) / len(v) for f, v in fam.items()}, copy_best
+
+    fetch, copy_f = per_kernel(fetch_dir, "FETCH_SIZE")
+    write, copy_w = per_kernel(write_dir, "WRITE_SIZE")
+    gib_kib = float(1 << 20)
+    if copy_f <= 0 or copy_w <= 0:
+        raise SystemExit("the 1 GiB copy is missing from the counter runs")
+    fs, ws = gib_kib / copy_f, gib_kib / copy_w
+    print(f"calibration (1 GiB copy): FETCH_SIZE {copy_f:.0f} KiB -> scale {fs:.4f}; WRITE_SIZE {copy_w:.0f} KiB -> "
+          f"scale {ws:.4f}")
+    print("| kernel | FETCH_SIZE KiB/disp | WRITE_SIZE KiB/disp | HBM MB/launch (scaled) | algorithmic MB | ratio |")
+    print("|---|---:|---:|---:|---:|---:|")
+    for _, f in TRAFFIC_KERNELS:
+        if f not in fetch or f not in write:
+            continue
+        hbm = 1024.0 * (fs * fetch[f] + ws * write[f])
+        alg = SOLVER_BYTES.get(f)
+        alg_mb = alg * cells / 1e6 if alg else None
+        print(f"| {f} | {fetch[f]:.0f} | {write[f]:.0f} | {hbm / 1e6:.1f} | "
+              f"{'' if alg_mb is None else f'{alg_mb:.1f}'} | {'' if alg_mb is None else f'{hbm / 1e6 / alg_mb:.3f}'} |")
+        if alg:
+            with open(os.path.join(out_dir or os.path.join(ROOT, "profiles"), "traffic_" + f + ".json"), "w") as fh:
+                json.dump({"kernel": f, "bytes_per_launch": int(hbm), "fetch_kib": fetch[f], "write_kib": write[f],
+                           "fetch_scale": fs, "write_scale": ws, "calibration": "1 GiB device copy"}, fh, indent=1)
 
 
 def vortex(args):
@@ -646,7 +732,12 @@ def main():
     ap.add_argument("--check", type=int, default=3, help="oracle parity steps of --workload vortex")
     ap.add_argument("--solves", type=int, default=100, help="right-hand sides of --workload solves")
     ap.add_argument("--delta", type=float, default=0.1, help="reliable-update threshold of the mixed solve")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/traffic_*.json from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE)")
     args = ap.parse_args()
+    if args.traffic:
+        traffic_tables(*args.traffic)
+        return
     args.steps_given = args.steps is not None
     if args.steps is None:
         args.steps = 3
@@ -669,6 +760,8 @@ def main():
     if args.workload == "serve":
         serve(args)
         return
+    note("hbm copy: 1 GiB")
+    copy = hbm_copy()
     note("fp64 wave tank: set-up")
     D, _ = make_domain(cfg)
     cells = cfg["nx"] * cfg["ny"] * cfg["nz"]
@@ -719,6 +812,8 @@ def main():
             "avg_launch_us": round(avg_s * 1e6, 2),
             "bytes_per_cell": SOLVER_BYTES[dom],
         },
+        # achievable bandwidth: a 1 GiB device copy, read + written bytes
+        "hbm_copy": copy,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in fams.items() if v[1]},
         "stencil_rooflines": stencil_rooflines(fams, cells),
         # enclosed water volume (volume1 of fluid_update_fsf) of the tank: at set-up, before

@@ -180,12 +180,14 @@ __device__ __forceinline__ void block_partials_w(double (&v)[M], double* __restr
             partials[(size_t)blockIdx.x * M + m] = ((ws[m][0] + ws[m][1]) + ws[m][2]) + ws[m][3];
 }
 
-// totals of nb block partials, summed in block order by one block of NT threads
+// totals of nb block partials by one block of NT threads: each thread adds a strided
+// subset in a fixed order, the wavefronts combine by a butterfly and the four wave sums
+// in a fixed order, so every thread gets the same totals with one block barrier
 template<int M>
 __device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int nb, double (&tot)[M])
 {
-    __shared__ double sh[M][NT];
-    const int t = threadIdx.x;
+    __shared__ double ws[M][NT / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     double acc[M];
     for(int m = 0; m < M; ++m) acc[m] = 0.0;
     int b = t;
@@ -199,15 +201,12 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ partials
     }
     for(; b < nb; b += NT)
         for(int m = 0; m < M; ++m) acc[m] += partials[(size_t)b * M + m];
-    for(int m = 0; m < M; ++m) sh[m][t] = acc[m];
+    for(int m = 0; m < M; ++m)
+        for(int off = 32; off > 0; off >>= 1) acc[m] += __shfl_xor(acc[m], off);
+    if(lane == 0)
+        for(int m = 0; m < M; ++m) ws[m][wave] = acc[m];
     __syncthreads();
-    for(int s = NT / 2; s > 0; s >>= 1)
-    {
-        if(t < s)
-            for(int m = 0; m < M; ++m) sh[m][t] += sh[m][t + s];
-        __syncthreads();
-    }
-    for(int m = 0; m < M; ++m) tot[m] = sh[m][0];
+    for(int m = 0; m < M; ++m) tot[m] = ((ws[m][0] + ws[m][1]) + ws[m][2]) + ws[m][3];
 }
 
 // Couplings of poisson_pcorr.cpp:95-123 before the boundary folding: T[0..5] for the

@@ -135,7 +135,7 @@ struct TileWalk
 
 __device__ __forceinline__ TileWalk item_walk(CTank& T, const TankSet& b, const ItemIter& q)
 {
-    const int li = q.q - b.item0[q.p];
+    const int li = item_at(T.shape, q.q - b.item0[q.p]);
     TileWalk w;
     w.s = T.shape;
     w.t = li * T.shape.cpt;
@@ -147,7 +147,7 @@ __device__ __forceinline__ TileWalk item_walk(CTank& T, const TankSet& b, const 
 // reduction slot of the current item of a persistent walk
 __device__ __forceinline__ int item_slot(CTank& T, const TankSet& b, const ItemIter& q)
 {
-    return T.item0 + (q.q - b.item0[q.p]);
+    return T.item0 + item_at(T.shape, q.q - b.item0[q.p]);
 }
 
 __device__ __forceinline__ TileWalk tile_walk(CTank& T, int t)
@@ -626,9 +626,22 @@ __global__ void __launch_bounds__(NT) kb_correct(TankSet b, double rw, double ra
 // plane walk and the FP32 / elementwise kernels; 5 for the kernels that apply the FP64
 // operator cell by cell (about 100 registers), whose grids are sized to match.
 constexpr int OCC_FULL = 8, OCC_A64 = 5;
-// plane walk: 4 blocks per CU (it keeps one plane of loads in flight per block, about 100
-// registers); the default plan of a 3-D tank gives it one work item per block
+// plane walk: 4 blocks per CU for 4-byte operands (two planes of loads in flight per
+// block); the default plan of a 3-D tank gives it one work item per block.  8-byte operands
+// keep PW64_DEPTH planes of loads in flight at OCC_PW64 blocks per CU: the FP64 walk is
+// bound by the latency of its loads, and a deeper prefetch at a lower occupancy keeps more
+// bytes in flight per CU than one plane at four blocks per CU
+#ifndef REEF_PW64_DEPTH
+#define REEF_PW64_DEPTH 1
+#endif
+#ifndef REEF_OCC_PW64
+#define REEF_OCC_PW64 4
+#endif
 constexpr int OCC_PW = REEF_OCC_PW;
+constexpr int PW64_DEPTH = REEF_PW64_DEPTH;
+constexpr int OCC_PW64 = REEF_OCC_PW64;
+template<class V>
+constexpr int occ_pw() { return sizeof(V) == 4 ? OCC_PW : OCC_PW64; }
 
 // The state machine of bicgstab_ijk::solve (bicgstab_ijk.cpp:125-278) per tank: mode 1 = a
 // restart is pending, 2 = p and ph were just set from the residual, 3 = reliable update
@@ -695,13 +708,16 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
     using E = decltype(pre(0u));
     using AF = AxisFactor<Op>;
     using In = PlaneIn<V, R, E, AF>;
-    constexpr int D = sizeof(V) == 4 ? 2 : 1;
+    constexpr int D = sizeof(V) == 4 ? 2 : PW64_DEPTH;
+    static_assert(D == 1 || D == 2, "plane walk prefetch depth");
     __shared__ PlaneLDS<V> L[2];
     const V* __restrict__ fxa = face_x(op);
     const V* __restrict__ fya = face_y(op);
     const V* __restrict__ fza = face_z(op);
     const int h = threadIdx.x;
-    const int lane = h & (TX - 1), row = h >> 6;
+    // one tile row per wavefront: the row (and with it j and the column's y grid factors)
+    // is wave-uniform and lives in scalar registers
+    const int lane = h & (TX - 1), row = __builtin_amdgcn_readfirstlane(h >> 6);
     // the halo cell of thread h relative to the tile origin (h < 136), and its LDS slot:
     // h < 64 the south row (j0-1, i0+h), 64 <= h < 128 the north row (j0+TY, i0+h-64),
     // 128 <= h < 132 the west column (j0+h-128, i0-1), 132 <= h < 136 the east column
@@ -738,12 +754,33 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
     }
     const bool has_halo = h < 2 * TX + 2 * TY;
     const bool fh_y = h < TX, fh_x = h >= 2 * TX && h < 2 * TX + TY;
-    auto geo = [&](int t, int& i0, int& j0, int& k, int& col) {
-        col = t / tw.s.nbz;
-        k = t - col * tw.s.nbz;
-        const int by = col / tw.s.nbx, bx = col - by * tw.s.nbx;
-        i0 = bx * TX;
-        j0 = by * TY;
+    // position of a tile of the walk: plane k of column col = (bx, by).  The walk moves one
+    // tile at a time, so positions advance by increments (one division per item, at its
+    // start, instead of one per plane)
+    struct Pos
+    {
+        int k, col, bx, by;
+    };
+    auto pos_of = [&](int tt) {
+        Pos q;
+        q.col = tt / tw.s.nbz;
+        q.k = tt - q.col * tw.s.nbz;
+        q.by = q.col / tw.s.nbx;
+        q.bx = q.col - q.by * tw.s.nbx;
+        return q;
+    };
+    auto adv = [&](Pos q) {
+        if(++q.k == tw.s.nbz)
+        {
+            q.k = 0;
+            ++q.col;
+            if(++q.bx == tw.s.nbx)
+            {
+                q.bx = 0;
+                ++q.by;
+            }
+        }
+        return q;
     };
     // per-column state: the thread's cell, which halo/face loads are real
     int i0, j0, k, col, i, j;
@@ -785,7 +822,11 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
     AF ax, ay;
     int t = tw.t;
     if(t >= tw.t1) return;
-    geo(t, i0, j0, k, col);
+    Pos pc = pos_of(t);   // plane t
+    i0 = pc.bx * TX;
+    j0 = pc.by * TY;
+    k = pc.k;
+    col = pc.col;
     bool v1 = false;   // D = 2: the buffer after the current one holds plane t+1
     auto start = [&](In& P0, In& P1) {
         column_setup();
@@ -802,11 +843,10 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
         if constexpr(D == 2)
             if(t + 1 < tw.t1)
             {
-                int a, b2, kk, cc;
-                geo(t + 1, a, b2, kk, cc);
-                if(cc == col)
+                const Pos q = adv(pc);
+                if(q.col == col)
                 {
-                    fetch(kk, P1);
+                    fetch(q.k, P1);
                     v1 = true;
                 }
             }
@@ -831,11 +871,10 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
         bool vf = false;
         if(t + D < tw.t1 && (D == 1 || v1))
         {
-            int a, b2, kk, cc;
-            geo(t + D, a, b2, kk, cc);
-            if(cc == col)
+            const Pos q = D == 1 ? adv(pc) : adv(adv(pc));
+            if(q.col == col)
             {
-                fetch(kk, nf);
+                fetch(q.k, nf);
                 vf = true;
             }
         }
@@ -854,24 +893,23 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
         }
         buf ^= 1;
         if(t + 1 >= tw.t1) return false;
-        int i2, j2, k2, col2;
-        geo(t + 1, i2, j2, k2, col2);
         ++t;
-        if(col2 == col)
+        pc = adv(pc);
+        if(pc.col == col)
         {
             xd = xc;
             xc = xu;
             auxc = auxu;
             fzd = in ? cur.f4 : V(0);
-            k = k2;
+            k = pc.k;
             v1 = vf;
         }
         else
         {
-            i0 = i2;
-            j0 = j2;
-            k = k2;
-            col = col2;
+            i0 = pc.bx * TX;
+            j0 = pc.by * TY;
+            k = pc.k;
+            col = pc.col;
             // the next step's current buffer (and its successor) restart the column
             if constexpr(D == 1)
                 start(nf, n1);
@@ -1025,7 +1063,7 @@ __global__ void kb_setup(TankSet b, double stop, int maxiter, int mixed, double 
     st->mode = 0;
     st->branch = 0;
     st->mixed = mixed;
-    st->r_j_prev = st->rmax = st->res0 = 0.0;
+    st->r_j_prev = st->rmax = st->res0 = st->rr = 0.0;
     st->delta = delta;
     st->pending = 0;
     st->nru = 0;
@@ -1056,6 +1094,7 @@ __global__ void kb_fin_init(TankSet b)
     if(threadIdx.x == 0)
     {
         st->r_j = tot[0];
+        st->rr = tot[0];
         st->norm_r0 = sqrt(tot[0]);
         // a negative stop asks for ||r||_2 / N < |stop| * ||b - A x0||_2 / N
         if(st->stop < 0.0) st->stop = -st->stop * sqrt(tot[0]) / double(st->ncell);
@@ -1120,6 +1159,7 @@ __global__ void kb_fin_p(TankSet b)
     if(threadIdx.x == 0)
     {
         st->r_j = tot[0];
+        st->rr = tot[0];
         st->norm_r0 = sqrt(tot[0]);
         st->residual = sqrt(tot[0]) / double(st->ncell);
         st->iter += 1;
@@ -1128,10 +1168,12 @@ __global__ void kb_fin_p(TankSet b)
     }
 }
 
-// v = A ph; sigma = (v, r0), |v|^2, |r|^2 (bicgstab_ijk.cpp:151-160)
+// v = A ph; sigma = (v, r0), |v|^2 (bicgstab_ijk.cpp:151-160).  |r|^2 of the same loop is
+// the square norm of the residual the x phase (or the restart) has just formed and summed:
+// kb_fin_v takes it from the state instead of reading r again here.
 // PW: items of the tanks walked by the plane walk, else the others (direct stencil)
 template<class V, bool PW>
-__global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL : OCC_A64)) kb_bicg_v(TankSet b, int x,
+__global__ void __launch_bounds__(NT, PW ? occ_pw<V>() : (sizeof(V) == 4 ? OCC_FULL : OCC_A64)) kb_bicg_v(TankSet b, int x,
                                                                                                int rhs)
 {
     #pragma unroll 1
@@ -1150,48 +1192,44 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL :
         const TileWalk tw = item_walk(T, b, q);
         double vals[3] = {0.0, 0.0, 0.0};
         const V* __restrict__ ph = bv.ph;
-        const V* __restrict__ r = bv.r;
         const V* __restrict__ r0 = bv.r0;
         V* __restrict__ vo = bv.v;
         auto epilogue = [&](long c, V v) {
             vo[c] = v;
-            const double vd = v, rc = r[c];
+            const double vd = v;
             vals[0] += vd * double(r0[c]);
             vals[1] += vd * vd;
-            vals[2] += rc * rc;
         };
         if constexpr(PW)
         {
             struct EpiIn
             {
-                V r0, r;
+                V r0;
             };
             plane_walk<V>(
                 g, op, tw, [&](unsigned cc) { return ld32(ph, cc); }, [](V x, V&) { return x; },
-                [&](unsigned c) { return EpiIn{ld32(r0, c), ld32(r, c)}; },
+                [&](unsigned c) { return EpiIn{ld32(r0, c)}; },
                 [&](unsigned c, V Ax, V, const EpiIn& e) {
                     st32(vo, c, Ax);
-                    const double vd = Ax, rc = e.r;
+                    const double vd = Ax;
                     vals[0] += vd * double(e.r0);
                     vals[1] += vd * vd;
-                    vals[2] += rc * rc;
                 });
         }
         else if(row_walk_ok(g, tw))
         {
             struct EpiIn
             {
-                V r0, r;
+                V r0;
             };
             row_walk<V>(
                 g, op, tw, [&](unsigned cc) { return ld32(ph, cc); }, [](V x, V&) { return x; },
-                [&](unsigned c) { return EpiIn{ld32(r0, c), ld32(r, c)}; },
+                [&](unsigned c) { return EpiIn{ld32(r0, c)}; },
                 [&](unsigned c, V Ax, V, const EpiIn& e) {
                     st32(vo, c, Ax);
-                    const double vd = Ax, rc = e.r;
+                    const double vd = Ax;
                     vals[0] += vd * double(e.r0);
                     vals[1] += vd * vd;
-                    vals[2] += rc * rc;
                 });
         }
         else
@@ -1318,19 +1356,22 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : OCC_FULL) kb_mix_v(TankSet b
     }
 }
 
-// alpha and the restart tests (bicgstab_ijk.cpp:162-191)
-__global__ void kb_fin_v(TankSet b)
+// alpha and the restart tests (bicgstab_ijk.cpp:162-191); rr_state: |r|^2 from the state
+// (kb_bicg_v), else the phase's third sum (kb_mix_v, km_v)
+__global__ void kb_fin_v(TankSet b, int rr_state)
 {
     CTank& T = tank_at(b, blockIdx.x);
     BiState* st = T.st;
-    if(st->done || st->mode == 1 || st->mode == 3) return;
+    // the partials' loads go out together with the state's (one memory latency, not two)
+    const int done = st->done, mode = st->mode;
     double tot[3];
     tank_totals<3>(T, b, tot);
+    if(done || mode == 1 || mode == 3) return;
     if(threadIdx.x == 0)
     {
         const double sigma = tot[0];
         const double norm_vj = sqrt(tot[1]);
-        const double norm_rj = sqrt(tot[2]);
+        const double norm_rj = sqrt(rr_state ? st->rr : tot[2]);
         const double alpha = st->r_j / sigma;
         st->sigma = sigma;
         st->norm_vj = norm_vj;
@@ -1353,7 +1394,7 @@ __global__ void kb_fin_v(TankSet b)
 
 // s = r - alpha v, sh = aii s, t = A sh; (t,s), (t,t), (s,s) (bicgstab_ijk.cpp:193-221)
 template<class V, bool PW>
-__global__ void __launch_bounds__(NT, PW ? OCC_PW : (sizeof(V) == 4 ? OCC_FULL : OCC_A64)) kb_bicg_t(TankSet b, int x,
+__global__ void __launch_bounds__(NT, PW ? occ_pw<V>() : (sizeof(V) == 4 ? OCC_FULL : OCC_A64)) kb_bicg_t(TankSet b, int x,
                                                                                                int rhs)
 {
     #pragma unroll 1
@@ -1451,9 +1492,10 @@ __global__ void kb_fin_t(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
     BiState* st = T.st;
-    if(st->done || st->mode == 1 || st->mode == 3) return;
+    const int done = st->done, mode = st->mode;
     double tot[3];
     tank_totals<3>(T, b, tot);
+    if(done || mode == 1 || mode == 3) return;
     if(threadIdx.x == 0)
     {
         st->w1 = tot[0];
@@ -1527,11 +1569,12 @@ __global__ void kb_fin_x(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
     BiState* st = T.st;
-    if(st->done || st->mode == 1 || st->mode == 3) return;
+    const int done = st->done, mode = st->mode;
     const double alpha = st->alpha, w = st->w;
     const int branch = st->branch;
     double tot[2];
     tank_totals<2>(T, b, tot);
+    if(done || mode == 1 || mode == 3) return;
     if(threadIdx.x == 0)
     {
         const double r_j1 = tot[0];
@@ -1539,6 +1582,7 @@ __global__ void kb_fin_x(TankSet b)
         if(branch) st->beta = alpha * r_j1 / (w * st->r_j == 0 ? 1.0e-15 : (w * st->r_j));
         st->r_j1 = r_j1;
         st->r_j = r_j1;
+        st->rr = tot[1];
         st->residual = sqrt(tot[1]) / double(st->ncell);
         st->iter += 1;
         st->mode = 0;
@@ -2636,12 +2680,12 @@ void bl_bicg_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStr
         {
             case 0: hipLaunchKernelGGL(kb_bicg_p<double>, grid64, dim3(NT), 0, s, b, x, rhs); break;
             case 1:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<double, true>), pgrid_pw(b, OCC_PW), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_v<double, true>), pgrid_pw(b, OCC_PW64), dim3(NT), 0, s, b, x, rhs);
                 if(b.has_direct)
                     hipLaunchKernelGGL((kb_bicg_v<double, false>), pgrid_direct(b, OCC_A64), dim3(NT), 0, s, b, x, rhs);
                 break;
             case 2:
-                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<double, true>), pgrid_pw(b, OCC_PW), dim3(NT), 0, s, b, x, rhs);
+                if(b.has_pw) hipLaunchKernelGGL((kb_bicg_t<double, true>), pgrid_pw(b, OCC_PW64), dim3(NT), 0, s, b, x, rhs);
                 if(b.has_direct)
                     hipLaunchKernelGGL((kb_bicg_t<double, false>), pgrid_direct(b, OCC_A64), dim3(NT), 0, s, b, x, rhs);
                 break;
@@ -2652,7 +2696,7 @@ void bl_bicg_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStr
     switch(phase)
     {
         case 0: hipLaunchKernelGGL(kb_fin_p, tankgrid(b), dim3(NT), 0, s, b); break;
-        case 1: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 1: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b, 1); break;
         case 2: hipLaunchKernelGGL(kb_fin_t, tankgrid(b), dim3(NT), 0, s, b); break;
         default: hipLaunchKernelGGL(kb_fin_x, tankgrid(b), dim3(NT), 0, s, b); break;
     }
@@ -2704,7 +2748,7 @@ void bl_mat_phase(const TankSet& b, int var, int phase, int part, hipStream_t s)
     switch(phase)
     {
         case 0: hipLaunchKernelGGL(kb_fin_p, tankgrid(b), dim3(NT), 0, s, b); break;
-        case 1: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 1: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b, 0); break;
         case 2: hipLaunchKernelGGL(kb_fin_t, tankgrid(b), dim3(NT), 0, s, b); break;
         default: hipLaunchKernelGGL(kb_fin_x, tankgrid(b), dim3(NT), 0, s, b); break;
     }
@@ -2745,7 +2789,7 @@ void bl_mixed_phase(const TankSet& b, int x, int rhs, int phase, int part, hipSt
     }
     switch(phase)
     {
-        case 0: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 0: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b, 0); break;
         case 1: hipLaunchKernelGGL(kb_fin_t, tankgrid(b), dim3(NT), 0, s, b); break;
         case 2: hipLaunchKernelGGL(kb_fin_x, tankgrid(b), dim3(NT), 0, s, b); break;
         default: hipLaunchKernelGGL(kb_fin_mix_ru, tankgrid(b), dim3(NT), 0, s, b); break;

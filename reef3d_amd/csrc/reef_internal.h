@@ -80,6 +80,8 @@ struct BiState
     // reliable updates
     double r_j_prev, rmax, delta;
     double res0;                       // ||b - A x0||_2 (FP64) of a mixed solve
+    double rr;                         // |r|^2 of the current residual, summed where r was
+                                       // formed (initial residual, restart, x phase)
     int pending, nru;
     int buf;                           // mixed precision: current half of the FP32 p / v buffers
     long ncell;

@@ -36,6 +36,14 @@ struct TileShape
     int ntiles;
     int cpt;             // tiles per work item
     int nitems;
+    // 0, or the z-chunks per column when the items are walked z-slab-major: 64 x 4 x 1
+    // tiles whose columns split into whole items (nbz % cpt == 0, more than one chunk).
+    // Item position l then stands for item (l % columns) * zslab + l / columns, so one XCD
+    // group's contiguous share of the positions is a z-slab of every column: the rows of
+    // the y-neighbour tiles a plane walk stages in its halo belong to blocks of the same
+    // XCD walking the same planes at the same time, and are served by that XCD's L2.
+    // Items and their reduction slots are unchanged, only the block that walks them is.
+    int zslab;
 };
 
 REEF_HD TileShape make_shape(int nx, int ny, int nz, int ty, int tz, int cpt)
@@ -50,6 +58,7 @@ REEF_HD TileShape make_shape(int nx, int ny, int nz, int ty, int tz, int cpt)
     s.ntiles = s.nbx * s.nby * s.nbz;
     s.cpt = cpt;
     s.nitems = (s.ntiles + cpt - 1) / cpt;
+    s.zslab = (ty == 4 && tz == 1 && s.nbz % cpt == 0 && s.nbz / cpt > 1) ? s.nbz / cpt : 0;
     return s;
 }
 
@@ -66,6 +75,15 @@ REEF_HD bool tile_cell_of(const TileShape& s, int nx, int ny, int nz, int t, int
     j = by * s.ty + ry;
     k = bz * s.tz + rz;
     return i < nx && j < ny && k < nz;
+}
+
+// the item at position l of a tank's item range (see TileShape::zslab)
+REEF_HD int item_at(const TileShape& s, int l)
+{
+    if(s.zslab == 0) return l;
+    const int ncol = s.nbx * s.nby;
+    const int zc = l / ncol;
+    return (l - zc * ncol) * s.zslab + zc;
 }
 
 // items [lo, hi) walked by block group `xcd` (0..7) of a launch over nitems items
