@@ -181,10 +181,10 @@ STENCIL_BYTES = {
     "hj": (40 + 48 + 48) / 3.0,      # phi_s, u, v, w [, phi_n] in; phi out
     "reini": (16 + 24 + 24) / 3.0,   # phi_s [, phi_0] in; phi out
 }
-# the WENO stencils are bound by FP64 VALU issue, not by HBM: FP64 flops per cell
-# (add + mul + 2 fma + trans) counted with SQ_INSTS_VALU_*_F64 on the 512x64x128 tank
+# the WENO stencils are bound by VALU issue and load latency, not by HBM: FP64 flops per
+# cell (add + mul + 2 fma + trans) counted with SQ_INSTS_VALU_*_F64 on the 512x64x128 tank
 # (profiles/wave_tank_512x64x128_stencil_valu.md); the same kernels run here
-STENCIL_FP64_FLOPS = {"hj": 1072.0, "reini": 1082.0, "momentum": 1488.0}
+STENCIL_FP64_FLOPS = {"hj": 554.0, "reini": 733.0, "momentum": 798.0}
 FP64_PEAK_TFLOPS = 78.6
 MOMENTUM_BYTES = (56 + 64 + 64) / 3.0   # q_in, u, v, w, p, phi [, q_old] in; q out
 

@@ -14,31 +14,67 @@ constexpr int TX = 64, TY = 4, NT = 256;
 // ------------------------------------------------------------------ WENO5 core
 // Smoothness indicators, (IS+psi)^-2 weights without alpha normalisation and the
 // non-uniform reconstruction, in the operation order of weno_nug_func.h:48-196 and
-// weno_flux_nug.cpp:136-160 / weno_hj_df_nug.cpp:110-177.
+// weno_flux_nug.cpp:136-160 / weno_hj_df_nug.cpp:110-177.  The stencil side (minus: the
+// upwind stencil from below) is chosen per lane and every lane of a wavefront runs the
+// same instructions, so lanes that take different sides do not run the reconstruction
+// twice.  Each side's operations and their order are the reference's: the two sides
+// differ only in which table entries and differences enter a term, and the reference's
+// "x - Q1 (q5 - q4)" is computed as x + (-Q1) (q5 - q4), which IEEE arithmetic rounds
+// identically (negation is exact and x - y is x + (-y)).
+// Table entry e of I, C, Q is at [e * st]: st = 1 for a cell's own table row (y, z), the
+// row length for the entry-major x tables.
 __device__ __forceinline__ double weno_recon(const double* __restrict__ I, const double* __restrict__ C,
-                                             const double* __restrict__ Q, bool minus, double q1, double q2,
+                                             const double* __restrict__ Q, long st, bool minus, double q1, double q2,
                                              double q3, double q4, double q5)
 {
     const double dq12 = q1 - q2, dq23 = q2 - q3, dq32 = q3 - q2, dq34 = q3 - q4, dq43 = q4 - q3, dq54 = q5 - q4;
     const int r = minus ? 0 : 3;
-    const double* I1 = I + r * 3;
-    const double is1 = I1[0] * dq54 * dq54 + I1[1] * (dq54) * (dq34) + I1[2] * dq34 * dq34;
-    const double is2 = I1[3] * dq23 * dq23 + I1[4] * (dq43) * (dq23) + I1[5] * dq43 * dq43;
-    const double is3 = I1[6] * dq12 * dq12 + I1[7] * (dq32) * (dq12) + I1[8] * dq32 * dq32;
+    const double* I1 = I + (long)(r * 3) * st;
+    const double is1 = I1[0] * dq54 * dq54 + I1[st] * (dq54) * (dq34) + I1[2 * st] * dq34 * dq34;
+    const double is2 = I1[3 * st] * dq23 * dq23 + I1[4 * st] * (dq43) * (dq23) + I1[5 * st] * dq43 * dq43;
+    const double is3 = I1[6 * st] * dq12 * dq12 + I1[7 * st] * (dq32) * (dq12) + I1[8 * st] * dq32 * dq32;
     const double a1 = is1 + 1.0e-6, a2 = is2 + 1.0e-6, a3 = is3 + 1.0e-6;
-    const double c0 = C[r], c1 = C[r + 1], c2 = C[r + 2];
+    const double c0 = C[r * st], c1 = C[(r + 1) * st], c2 = C[(r + 2) * st];
     const double w1 = c0 / (0.0 + (a1 * a1) * (c0 / (a1 * a1) + c1 / (a2 * a2) + c2 / (a3 * a3)));
     const double w2 = c1 / (0.0 + (a2 * a2) * (c0 / (a1 * a1) + c1 / (a2 * a2) + c2 / (a3 * a3)));
     const double w3 = c2 / (0.0 + (a3 * a3) * (c0 / (a1 * a1) + c1 / (a2 * a2) + c2 / (a3 * a3)));
-    if(minus)
-        return w1 * (q4 + Q[0] * (q3 - q4) - Q[1] * (q5 - q4)) + w2 * (q3 + Q[2] * (q4 - q3) - Q[3] * (q2 - q3))
-             + w3 * (q2 + Q[4] * (q1 - q2) + Q[5] * (q3 - q2));
-    return w1 * (q4 + Q[6] * (q3 - q4) + Q[7] * (q5 - q4)) + w2 * (q3 + Q[8] * (q2 - q3) - Q[9] * (q4 - q3))
-         + w3 * (q2 + Q[10] * (q3 - q2) - Q[11] * (q1 - q2));
+    // minus: w1 (q4 + Q0 (q3 - q4) - Q1 (q5 - q4)) + w2 (q3 + Q2 (q4 - q3) - Q3 (q2 - q3))
+    //        + w3 (q2 + Q4 (q1 - q2) + Q5 (q3 - q2))
+    // plus:  w1 (q4 + Q6 (q3 - q4) + Q7 (q5 - q4)) + w2 (q3 + Q8 (q2 - q3) - Q9 (q4 - q3))
+    //        + w3 (q2 + Q10 (q3 - q2) - Q11 (q1 - q2))
+    const double* Qs = Q + (long)(minus ? 0 : 6) * st;
+    const double A1 = Qs[0], B1 = minus ? -Qs[st] : Qs[st];
+    const double A2 = Qs[2 * st], B2 = Qs[3 * st];
+    const double u2 = minus ? dq43 : dq23, v2 = minus ? dq23 : dq43;
+    const double A3 = Qs[4 * st], B3 = minus ? Qs[5 * st] : -Qs[5 * st];
+    const double u3 = minus ? dq12 : dq32, v3 = minus ? dq32 : dq12;
+    return w1 * (q4 + A1 * dq34 + B1 * dq54) + w2 * (q3 + A2 * u2 - B2 * v2) + w3 * (q2 + A3 * u3 + B3 * v3);
 }
 
-// HJ gradient along one axis at index ix (stride s), divided differences over DXP
-// (weno_hj_df_nug.cpp:344-396 / ddweno_nug_sf.cpp:186-224)
+// HJ gradient along one axis at index ix (stride s) on the side `minus` chosen per lane,
+// given the two central divided differences dd2 = (f0 - fm1) / D(ip-1) and
+// dd3 = (fp1 - f0) / D(ip) (weno_hj_df_nug.cpp:344-396 / ddweno_nug_sf.cpp:186-224): the
+// minus stencil takes (fm2 - fm3) / D(ip-3), dd1, dd2, dd3, dd4, the plus stencil dd1, dd2,
+// dd3, dd4, (fp3 - fp2) / D(ip+2); only the outer quotient of the lane's side is formed
+__device__ __forceinline__ double hj_grad_dd(const DevGrid& g, int axis, const double* __restrict__ f, long c, long s,
+                                             int ix, bool minus, double dd2, double dd3)
+{
+    const int ip = ix + MARGE;
+    const double* D = axis == 0 ? g.DXP : axis == 1 ? g.DYP : g.DZP;
+    const long st = axis == 0 ? g.nT : 1;
+    const double* I = axis == 0 ? g.isfT + ip : g.isf[axis] + (size_t)ip * 36;
+    const double* C = axis == 0 ? g.cfT + ip : g.cf[axis] + (size_t)ip * 12;
+    const double* Q = axis == 0 ? g.qfT + ip : g.qf[axis] + (size_t)ip * 24;
+    const double fm2 = f[c - 2 * s], fm1 = f[c - s], fp1 = f[c + s], fp2 = f[c + 2 * s];
+    const double fo = f[minus ? c - 3 * s : c + 3 * s];
+    const double dd1 = (fm1 - fm2) / D[ip - 2];
+    const double dd4 = (fp2 - fp1) / D[ip + 1];
+    const double ddo = (minus ? fm2 - fo : fo - fp2) / D[minus ? ip - 3 : ip + 2];
+    return weno_recon(I, C, Q, st, minus, minus ? ddo : dd1, minus ? dd1 : dd2, minus ? dd2 : dd3, minus ? dd3 : dd4,
+                      minus ? dd4 : ddo);
+}
+
+// the HJ gradient upwind of advec (minus for advec > 0, plus for advec < 0, 0 otherwise)
 __device__ __forceinline__ double hj_grad(const DevGrid& g, int axis, const double* __restrict__ f, long c, long s,
                                           int ix, double advec)
 {
@@ -46,32 +82,28 @@ __device__ __forceinline__ double hj_grad(const DevGrid& g, int axis, const doub
     if(!minus && !(advec < 0.0)) return 0.0;
     const int ip = ix + MARGE;
     const double* D = axis == 0 ? g.DXP : axis == 1 ? g.DYP : g.DZP;
-    const double* I = g.isf[axis] + (size_t)ip * 36;
-    const double* C = g.cf[axis] + (size_t)ip * 12;
-    const double* Q = g.qf[axis] + (size_t)ip * 24;
-    const double fm3 = f[c - 3 * s], fm2 = f[c - 2 * s], fm1 = f[c - s], f0 = f[c], fp1 = f[c + s], fp2 = f[c + 2 * s],
-                 fp3 = f[c + 3 * s];
-    if(minus)
-        return weno_recon(I, C, Q, true, (fm2 - fm3) / D[ip - 3], (fm1 - fm2) / D[ip - 2], (f0 - fm1) / D[ip - 1],
-                          (fp1 - f0) / D[ip], (fp2 - fp1) / D[ip + 1]);
-    return weno_recon(I, C, Q, false, (fm1 - fm2) / D[ip - 2], (f0 - fm1) / D[ip - 1], (fp1 - f0) / D[ip],
-                      (fp2 - fp1) / D[ip + 1], (fp3 - fp2) / D[ip + 2]);
+    const double f0 = f[c];
+    return hj_grad_dd(g, axis, f, c, s, ix, minus, (f0 - f[c - s]) / D[ip - 1], (f[c + s] - f0) / D[ip]);
 }
 
 // ------------------------------------------------------------------ WENO flux
 // weno_flux_nug::fx/fy/fz with iqmin..kqmax (weno_flux_nug.cpp:136-313): values; the
-// x upwind-from-above stencil repeats f(i+2) (weno_flux_nug.cpp:288-295).
+// x upwind-from-above stencil repeats f(i+2) (weno_flux_nug.cpp:288-295).  The side is
+// chosen per lane (weno_recon).
 __device__ __forceinline__ double flux_face(const DevGrid& g, int axis, int uf, const double* __restrict__ f, long c,
                                             long s, int ix, double advec)
 {
     const bool minus = advec > 0.0;
     if(!minus && !(advec < 0.0)) return 0.0;
     const int ip = ix + MARGE;
-    const double* I = g.isf[axis] + (size_t)ip * 36 + uf * 18;
-    const double* C = g.cf[axis] + (size_t)ip * 12 + uf * 6;
-    const double* Q = g.qf[axis] + (size_t)ip * 24 + uf * 12;
-    if(minus) return weno_recon(I, C, Q, true, f[c - 2 * s], f[c - s], f[c], f[c + s], f[c + 2 * s]);
-    return weno_recon(I, C, Q, false, f[c - s], f[c], f[c + s], f[c + 2 * s], axis == 0 ? f[c + 2 * s] : f[c + 3 * s]);
+    const long st = axis == 0 ? g.nT : 1;
+    const double* I = axis == 0 ? g.isfT + (long)(uf * 18) * st + ip : g.isf[axis] + (size_t)ip * 36 + uf * 18;
+    const double* C = axis == 0 ? g.cfT + (long)(uf * 6) * st + ip : g.cf[axis] + (size_t)ip * 12 + uf * 6;
+    const double* Q = axis == 0 ? g.qfT + (long)(uf * 12) * st + ip : g.qf[axis] + (size_t)ip * 24 + uf * 12;
+    const double fm1 = f[c - s], f0 = f[c], fp1 = f[c + s], fp2 = f[c + 2 * s];
+    const double fo = f[minus ? c - 2 * s : (axis == 0 ? c + 2 * s : c + 3 * s)];
+    return weno_recon(I, C, Q, st, minus, minus ? fo : fm1, minus ? fm1 : f0, minus ? f0 : fp1, minus ? fp1 : fp2,
+                      minus ? fp2 : fo);
 }
 
 // flux divergence of component ipol at cell c (weno_flux_nug.cpp:110-134) with the
@@ -482,24 +514,22 @@ __device__ __forceinline__ double reini_L(const DevGrid& g, const double* __rest
         double dx = 0.0, dy = 0.0, dz = 0.0;
         double lsSig = lsv / sqrt(lsv * lsv);
         if(fabs(lsv) < 1.0e-8) lsSig = 1.0;
-        const double xmin = (lsv - f[c - 1]) / g.DXP[ip - 1];
-        const double xplus = (f[c + 1] - lsv) / g.DXP[ip];
-        if(xmin * lsSig > 0.0 && xplus * lsSig > -xmin * lsSig) dx = hj_grad(g, 0, f, c, 1, i, 1.0);
-        if(xplus * lsSig < 0.0 && xmin * lsSig < -xplus * lsSig) dx = hj_grad(g, 0, f, c, 1, i, -1.0);
-        if(xplus * lsSig > 0.0 && xmin * lsSig < 0.0) dx = 0.0;
-        if(g.jdir == 1)
-        {
-            const double ymin = (lsv - f[c - g.sx]) / g.DYP[jp - 1];
-            const double yplus = (f[c + g.sx] - lsv) / g.DYP[jp];
-            if(ymin * lsSig > 0.0 && yplus * lsSig > -ymin * lsSig) dy = hj_grad(g, 1, f, c, g.sx, j, 1.0);
-            if(yplus * lsSig < 0.0 && ymin * lsSig < -yplus * lsSig) dy = hj_grad(g, 1, f, c, g.sx, j, -1.0);
-            if(yplus * lsSig > 0.0 && ymin * lsSig < 0.0) dy = 0.0;
-        }
-        const double zmin = (lsv - f[c - g.sxy]) / g.DZP[kp - 1];
-        const double zplus = (f[c + g.sxy] - lsv) / g.DZP[kp];
-        if(zmin * lsSig > 0.0 && zplus * lsSig > -zmin * lsSig) dz = hj_grad(g, 2, f, c, g.sxy, k, 1.0);
-        if(zplus * lsSig < 0.0 && zmin * lsSig < -zplus * lsSig) dz = hj_grad(g, 2, f, c, g.sxy, k, -1.0);
-        if(zplus * lsSig > 0.0 && zmin * lsSig < 0.0) dz = 0.0;
+        // the side of each axis (reinidisc_f.cpp:78-104): minus, plus or none (the three tests
+        // exclude each other but for the last, which zeroes the gradient); the quotients of
+        // the tests are the central divided differences of the WENO stencil
+        auto axis_grad = [&](int axis, long st, int ix, const double* DP, int pp) {
+            const double dmin = (lsv - f[c - st]) / DP[pp - 1];
+            const double dplus = (f[c + st] - lsv) / DP[pp];
+            const bool cm = dmin * lsSig > 0.0 && dplus * lsSig > -dmin * lsSig;
+            const bool cp = dplus * lsSig < 0.0 && dmin * lsSig < -dplus * lsSig;
+            const bool cz = dplus * lsSig > 0.0 && dmin * lsSig < 0.0;
+            double d = 0.0;
+            if((cm || cp) && !cz) d = hj_grad_dd(g, axis, f, c, st, ix, cm, dmin, dplus);
+            return d;
+        };
+        dx = axis_grad(0, 1, i, g.DXP, ip);
+        if(g.jdir == 1) dy = axis_grad(1, g.sx, j, g.DYP, jp);
+        dz = axis_grad(2, g.sxy, k, g.DZP, kp);
         const double dnorm = sqrt(dx * dx + dy * dy + dz * dz);
         double deltax;
         if(g.jdir == 0)

@@ -234,6 +234,20 @@ int Engine::build_tank(const reef_grid_desc& gd, const reef_params& p, TankBuild
         b.meta.insert(b.meta.end(), h.cf[a].begin(), h.cf[a].end());
         b.offisf[a] = b.meta.size();
         b.meta.insert(b.meta.end(), h.isf[a].begin(), h.isf[a].end());
+        if(a == 0)
+        {
+            // the x tables again, entry-major: entry e of every IP in a row
+            auto transpose = [&](const std::vector<double>& t, int w, size_t& off) {
+                const size_t n = t.size() / (size_t)w;
+                off = b.meta.size();
+                for(int e = 0; e < w; ++e)
+                    for(size_t ip = 0; ip < n; ++ip) b.meta.push_back(t[ip * (size_t)w + (size_t)e]);
+                b.nT = (int)n;
+            };
+            transpose(h.qf[0], 24, b.offqfT);
+            transpose(h.cf[0], 12, b.offcfT);
+            transpose(h.isf[0], 36, b.offisfT);
+        }
     }
     for(int a = 0; a < 3; ++a)
     {
@@ -351,6 +365,10 @@ int Engine::add_built(TankBuild& b, int* id)
         g.cf[a] = md + b.offcf[a];
         g.isf[a] = md + b.offisf[a];
     }
+    g.qfT = md + b.offqfT;
+    g.cfT = md + b.offcfT;
+    g.isfT = md + b.offisfT;
+    g.nT = b.nT;
     D.iDXN32 = invd + b.offinv[0];
     D.iDYN32 = invd + b.offinv[1];
     D.iDZN32 = invd + b.offinv[2];

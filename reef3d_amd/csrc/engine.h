@@ -86,6 +86,8 @@ struct TankBuild
     std::vector<double> meta;           // P, DN, DP and WENO tables per axis
     std::vector<float> inv;             // FP32 1/DN per axis
     size_t offP[3] = {}, offDN[3] = {}, offDP[3] = {}, offqf[3] = {}, offcf[3] = {}, offisf[3] = {}, offinv[3] = {};
+    size_t offqfT = 0, offcfT = 0, offisfT = 0;   // the x tables entry-major (DevGrid::qfT)
+    int nT = 0;
 };
 
 class Engine

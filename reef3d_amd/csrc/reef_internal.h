@@ -30,6 +30,11 @@ struct DevGrid
     const double *DXP, *DYP, *DZP, *DXN, *DYN, *DZN, *XP, *YP, *ZP;
     // WENO tables per axis: qf [IP][uf][12], cf [IP][uf][6], isf [IP][uf][18]
     const double *qf[3], *cf[3], *isf[3];
+    // the x tables entry-major: qfT [uf * 12 + e][IP], cfT [uf * 6 + e][IP], isfT [uf * 18 + e][IP]
+    // with nT = IP entries per row, so a wavefront's 64 cells of a row read one table entry
+    // as one coalesced 512-byte access
+    const double *qfT, *cfT, *isfT;
+    int nT;
 };
 
 // One ghost operation: f[dst] = (src < 0) ? 0 : (scaled ? 0.0 + w*f[src] : f[src])
