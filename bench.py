@@ -636,6 +636,10 @@ def run_serve(args):
                           "p99": round(float(np.percentile(seq_lat, 99)), 3)},
             "service_s_total": round(float(sum(service)), 3)},
         "speedup_vs_sequential": round((len(reqs) / makespan) / (len(reqs) / seq_span), 3) if service else None,
+        # per request: nx, ny, nz, steps, served latency (submit to done), FP64 service alone
+        "requests": [[r["nx"], r["ny"], r["nz"], r["steps"], round(float(l_), 3),
+                      round(float(service[q_]), 3) if service else None]
+                     for q_, (r, l_) in enumerate(zip(reqs, lat))],
     }
     return out
 
