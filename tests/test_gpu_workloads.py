@@ -9,8 +9,9 @@
   smallest tank agrees with the FP64 oracle within the mixed solve's tolerance.
 * The served request stream (seed 1004, 70 % 2-D / 30 % 3-D, one 256x64x128 request)
   under an HBM budget that forces queueing: every request gives the same bits alone as in
-  the mix, admission waits for memory to free up, and the live tanks never hold more
-  device bytes than the budget.
+  the mix, admission waits for memory to free up, the live tanks never hold more device
+  bytes than the budget, and the smallest request agrees with the FP64 oracle within the
+  mixed solve's tolerance.
 """
 import time
 
@@ -239,3 +240,21 @@ def test_c4_served_stream_alone_equals_mix_and_budget():
         assert A.info(rid)["iters"] == info[q]["iters"]
         A.close()
     print("c4: every request alone == in the mix", flush=True)
+    # the smallest served request against the FP64 oracle, step for step as served
+    from oracle.oracle import Domain as ODomain
+    from reef3d_amd.cases import tank_kwargs
+    q = int(np.argmin([r["nx"] * r["ny"] * r["nz"] for r in reqs]))
+    r, f = reqs[q], fields[q]
+    O = ODomain(**tank_kwargs(r, f))
+    st = {n: f[n].copy() for n in ("phi", "u", "v", "w", "press")}
+    for n, kind, gcv in (("phi", 4, 52), ("u", 1, 10), ("v", 2, 11), ("w", 3, 12), ("press", 4, 40)):
+        O.ghost(st[n], kind, gcv)
+    st["ro"], st["visc"] = np.zeros_like(st["phi"]), np.zeros_like(st["phi"])
+    O.fluid_update(st["phi"], st["ro"], st["visc"])
+    for _ in range(r["steps"]):
+        O.fc3_step(*(st[n] for n in FIELDS))
+    for n in ("phi", "u", "w", "press"):
+        err = _rel(inner(mix[q][n]), inner(st[n]))
+        print(f"c4: request {q} ({r['nx']}x{r['ny']}x{r['nz']}, {r['steps']} steps) {n} vs oracle {err:.2e}",
+              flush=True)
+        assert err < MIXED_RTOL, f"request {q} {n}: {err:.2e} against the oracle"
