@@ -216,13 +216,15 @@ std::string build_grid(HostGrid& g, const reef_grid_desc& d, double F45)
                 weno_coefficients(uf == 0 ? g.X[a].data() : g.P[a].data(), ip, &g.qf[a][(size_t)ip * 24 + uf * 12],
                                   &g.cf[a][(size_t)ip * 12 + uf * 6], &g.isf[a][(size_t)ip * 36 + uf * 18]);
     }
-    // device layout: rows padded to 16 doubles, (0,j,k) on a 128-byte boundary
-    g.sx = round_up(g.nx + 2 * NG, 16);
+    // device layout: rows padded to 32 elements, (0,j,k) at a multiple of 32 elements, so
+    // every row starts on a 128-byte boundary in the FP32 arrays as in the FP64 ones (a
+    // wavefront's 64-cell row is then 2 whole cache lines of FP32, 4 of FP64)
+    g.sx = round_up(g.nx + 2 * NG, 32);
     g.sy = g.ny + 2 * NG;
     g.sz = g.nz + 2 * NG;
     g.sxy = g.sx * g.sy;
-    g.base = 16 + NG * g.sx + NG * g.sxy;        // (-3, j, k) at offset 13 within a row
-    g.nalloc = round_up(16 + g.sxy * g.sz + 64, 64);
+    g.base = 32 + NG * g.sx + NG * g.sxy;        // (-3, j, k) at offset 29 within a row
+    g.nalloc = round_up(32 + g.sxy * g.sz + 64, 64);
     // the solver walks address an array through 32-bit byte offsets from its base (FP64:
     // at most 2^29 - 1 elements) and the ghost programs through 32-bit element indices
     if(g.nalloc * (long)sizeof(double) >= (1L << 32))

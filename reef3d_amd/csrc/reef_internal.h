@@ -17,7 +17,8 @@ constexpr int NG = 3;           // ghost layers of a field (grid.h:70)
 constexpr double PI_REF = 3.14159265359;   // definitions.h:37
 
 // Device view of one domain.  Element (i,j,k), i,j,k in [-3, n+3), lives at
-// base + i + sx*j + sxy*k.  Rows are padded so that every i = 0 is 128-byte aligned.
+// base + i + sx*j + sxy*k.  Rows are padded so that every i = 0 is 128-byte aligned, in
+// the FP64 and in the FP32 arrays (sx and base are multiples of 32 elements).
 struct DevGrid
 {
     int nx, ny, nz, jdir;

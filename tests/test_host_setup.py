@@ -78,8 +78,8 @@ def test_tank_bytes_include_ghost_programs():
     nx, ny, nz = 128, 32, 64
     nodes = [uniform_nodes(m, 0.02) for m in (nx, ny, nz)]
     total = Batch.tank_bytes(nx, ny, nz, *nodes, 0.02, 0.02, jdir=1, top_group=3)
-    sx = (nx + 6 + 15) // 16 * 16
-    nalloc = (16 + sx * (ny + 6) * (nz + 6) + 64 + 63) // 64 * 64
+    sx = (nx + 6 + 31) // 32 * 32
+    nalloc = (32 + sx * (ny + 6) * (nz + 6) + 64 + 63) // 64 * 64
     fields = 38 * nalloc * 8 + 13 * nalloc * 4
     # every ghost program writes at least one ghost layer over the 6 faces
     faces = 2 * (nx * ny + ny * nz + nx * nz)
