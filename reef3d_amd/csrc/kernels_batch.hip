@@ -843,7 +843,7 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
         if constexpr(D == 2)
             if(t + 1 < tw.t1)
             {
-                const Pos q = adv(pc);
+                const Pos q = D == 1 ? adv(pc) : pos_of(t + 1);
                 if(q.col == col)
                 {
                     fetch(q.k, P1);
@@ -871,7 +871,7 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
         bool vf = false;
         if(t + D < tw.t1 && (D == 1 || v1))
         {
-            const Pos q = D == 1 ? adv(pc) : adv(adv(pc));
+            const Pos q = D == 1 ? adv(pc) : pos_of(t + 2);
             if(q.col == col)
             {
                 fetch(q.k, nf);
@@ -894,7 +894,7 @@ __device__ __forceinline__ void plane_walk(const DevGrid& g, const Op& op, const
         buf ^= 1;
         if(t + 1 >= tw.t1) return false;
         ++t;
-        pc = adv(pc);
+        pc = D == 1 ? adv(pc) : pos_of(t);
         if(pc.col == col)
         {
             xd = xc;
