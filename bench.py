@@ -17,7 +17,13 @@ OpenMP over i slabs with the host's CPU share of threads) on one whole step of t
 tank from the same initial fields.  The
 mixed_projection object repeats the timed steps from the same initial state with the
 mixed-precision projection (FP32 BiCGStab with FP64 reliable updates, to the same 1e-10
-FP64 residual) and compares.
+FP64 residual) and compares; mixed_solves (100 standalone solves, FP64 against mixed) and
+served (a 16-request sample of the served stream against sequential FP64) add the other
+half of the metric; hbm_copy times a 1 GiB device copy (the achievable bandwidth).
+
+--traffic FETCH_DIR WRITE_DIR: no GPU work; turns two rocprofv3 --pmc runs (FETCH_SIZE,
+WRITE_SIZE) of `bench.py --steps 1 --warmup 0 --no-cpu --no-solves --no-serve` into
+profiles/traffic_<kernel>.json (the roofline's traffic) and a markdown table.
 
 --workload vortex: the level-set vortex deformation benchmark instead: unit cube, 128^3
 cells, sphere r = 0.15 at (0.35, 0.35, 0.35), levelset_RK3 + 3 reinit iterations per
