@@ -166,13 +166,13 @@ def note(msg):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def timed_steps(D, steps, kernel_steps=1):
+def timed_steps(D, steps, kernel_steps=1, graphs=True):
     """steps momentum_FC3 steps on the wall clock, their launch sequences replayed from
     hipGraphs (segments and iteration batches, captured in the warm-up) with no per-kernel
     events in the timed region; then kernel_steps more steps launched one by one with HIP
     events around each launch for the per-kernel breakdown.  (seconds, iterations,
     families of the instrumented steps, volumes after the timed steps)"""
-    D.set_control("graphs", 1)
+    D.set_control("graphs", 1 if graphs else 0)
     D.sync()
     iters = 0
     t0 = time.perf_counter()
@@ -334,16 +334,25 @@ def vortex(args):
     D = fresh()
     vol0 = D.fluid_update()
     D.sync()
-    D.timing(True)
-    D.timing_reset()
+    # the timed run carries no per-kernel events; the breakdown comes from a separate run
+    # of the first kernel_steps steps with HIP events around each launch
     t0 = time.perf_counter()
     for s in range(steps):
         D.vortex_step(s * dt, volumes=False)
     D.sync()
     elapsed = time.perf_counter() - t0
-    fams = {name: D.timing_get(name) for name in TIMED_FAMILIES}
-    D.timing(False)
     vol_end = D.fluid_update()
+    kernel_steps = min(steps, 64)
+    Dk = fresh()
+    Dk.sync()
+    Dk.timing(True)
+    Dk.timing_reset()
+    for s in range(kernel_steps):
+        Dk.vortex_step(s * dt, volumes=False)
+    Dk.sync()
+    fams = {name: Dk.timing_get(name) for name in TIMED_FAMILIES}
+    Dk.timing(False)
+    del Dk
     exact = 4.0 / 3.0 * np.pi * 0.15 ** 3
     dom = max(STENCIL_BYTES, key=lambda k: fams[k][0])
     ms, nl = fams[dom]
@@ -371,7 +380,7 @@ def vortex(args):
                      "avg_launch_us": round(avg_s * 1e6, 2), "bytes_per_cell": round(STENCIL_BYTES[dom], 2),
                      # what actually bounds it: FP64 VALU issue (see STENCIL_FP64_FLOPS)
                      "valu": stencil_rooflines(fams, cells).get(dom)},
-        "kernel_ms_per_step": {k: round(v[0] / steps, 4) for k, v in fams.items() if v[1]},
+        "kernel_ms_per_step": {k: round(v[0] / kernel_steps, 4) for k, v in fams.items() if v[1]},
     }
     if args.check > 0:
         Dp = fresh()
@@ -743,7 +752,7 @@ def main():
     ap.add_argument("--step-div", type=int, default=10, help="--workload serve: steps per request divided by this")
     ap.add_argument("--serve-precision", choices=("fp64", "mixed"), default="mixed", help="--workload serve: projection")
     ap.add_argument("--sequential", type=int, default=8, help="tanks run one at a time in --workload tanks")
-    ap.add_argument("--no-graphs", action="store_true", help="--workload tanks / solves / serve: launch by launch")
+    ap.add_argument("--no-graphs", action="store_true", help="launch by launch instead of hipGraph replay")
     ap.add_argument("--no-tune", action="store_true", help="--workload tanks: default tile plans")
     ap.add_argument("--tune", action="store_true", help="--workload serve: autotuned tile plans")
     ap.add_argument("--serve-only", action="store_true", help="--workload serve: skip the sequential FP64 leg")
@@ -785,11 +794,11 @@ def main():
     D, _ = make_domain(cfg)
     cells = cfg["nx"] * cfg["ny"] * cfg["nz"]
     vol_init = D.fluid_update()
-    D.set_control("graphs", 1)
+    D.set_control("graphs", 0 if args.no_graphs else 1)
     for _ in range(args.warmup):
         D.momentum_fc3()
     vol_start = D.fluid_update()
-    elapsed, iters, fams, vol_end = timed_steps(D, args.steps)
+    elapsed, iters, fams, vol_end = timed_steps(D, args.steps, graphs=not args.no_graphs)
     note(f"fp64 wave tank: {args.steps} timed steps, {elapsed / args.steps * 1e3:.1f} ms/step")
     dom = max(("bicg_p", "bicg_v", "bicg_t", "bicg_x"), key=lambda k: fams[k][0])
     ms, n = fams[dom]
@@ -820,6 +829,7 @@ def main():
             "cells": cells,
             "dt": cfg["dt"],
             "bicgstab_iters_per_step": iters / args.steps,
+            "graphs": not args.no_graphs,
         },
         "roofline": {
             "kernel": dom,
@@ -849,10 +859,10 @@ def main():
         del D
         Dm, _ = make_domain(cfg)
         Dm.solver_config("mixed")
-        Dm.set_control("graphs", 1)
+        Dm.set_control("graphs", 0 if args.no_graphs else 1)
         for _ in range(args.warmup):
             Dm.momentum_fc3()
-        el_m, it_m, fams_m, vol_m = timed_steps(Dm, args.steps)
+        el_m, it_m, fams_m, vol_m = timed_steps(Dm, args.steps, graphs=not args.no_graphs)
         note(f"mixed wave tank: {args.steps} timed steps, {el_m / args.steps * 1e3:.1f} ms/step")
         _, _, rel = Dm.solver_last()
         dm = max(("mix_v", "bicg32_t", "bicg32_x"), key=lambda k: fams_m[k][0])
