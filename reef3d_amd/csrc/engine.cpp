@@ -1661,6 +1661,8 @@ int Engine::vortex_step(double simtime)
     }
     for(int comp = 1; comp <= 3; ++comp)
         each([&](const TankSet& S) { bl_vortex_velocity(S, comp, REEF_U + comp - 1, simtime, t0->hg.DXM, stream); });
+    // benchmark_vortex3D.cpp:98-100 fills u (10), v (11) and then v again with gcv 12; w's
+    // ghosts keep the values of the void RK3 fill above.  Kept as the reference has it.
     if((rc = ghost(REEF_U, 1, 10))) return rc;
     if((rc = ghost(REEF_V, 2, 11))) return rc;
     if((rc = ghost(REEF_V, 2, 12))) return rc;
