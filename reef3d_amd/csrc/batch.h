@@ -62,6 +62,8 @@ struct TankSet
 {
     const TankDev* tanks;          // device tank table (every slot)
     double* partials;              // reduction partials: [TankDev::item0 + item] * 4
+    double* partials_ru;           // partials of the mixed solve's reliable-update phase:
+                                   // [TankDev::item0 + item] * 2, read by the next v phase
     int n;                         // tanks in the set
     int ntiles, nitems;            // tiles and work items of the set
     int has_pw, has_direct;        // tanks walked by the 3-D plane walk / by the direct stencil
@@ -116,6 +118,9 @@ void bl_coef32(const TankSet& b, hipStream_t s);
 void bl_mixed_init(const TankSet& b, int x, int rhs, double stop, int maxiter, double delta,
                    hipStream_t s);
 void bl_mixed_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStream_t s);
+// after the last iteration of a batch: stores the outcome of a reliable-update phase that
+// no v phase has taken yet (phase 3 has no finalize of its own)
+void bl_mixed_settle(const TankSet& b, hipStream_t s);
 void bl_add32(const TankSet& b, int x, hipStream_t s);
 // ietimestep::start (ini = 0) / ::ini (ini = 1): velocity maxima, then dt into tstep[0]
 void bl_timestep(const TankSet& b, int u, int v, int w, double N47, int ini, hipStream_t s);

@@ -491,11 +491,12 @@ int Engine::sync_tables()
         host[q] = t.dev;
     }
     if(!host.empty()) EHIP(hipMemcpy(d_tanks_, host.data(), sizeof(TankDev) * nslot, hipMemcpyHostToDevice));
-    if(4 * std::max(items, (size_t)64) > cap_partials_)
+    // two halves: 4 partials per item, then the reliable-update phase's 2 per item
+    if(8 * std::max(items, (size_t)64) > cap_partials_)
     {
         if(d_partials_) (void)hipFree(d_partials_);
         d_partials_ = nullptr;
-        cap_partials_ = 8 * std::max(items, (size_t)64);
+        cap_partials_ = 16 * std::max(items, (size_t)64);
         EHIP(hipMalloc(&d_partials_, sizeof(double) * cap_partials_));
     }
     if(!st_ev_[0])
@@ -532,6 +533,7 @@ std::vector<TankSet> Engine::make_sets(const std::vector<int>& slots_in) const
         std::memset(&S, 0, sizeof(S));
         S.tanks = d_tanks_;
         S.partials = d_partials_;
+        S.partials_ru = d_partials_ + cap_partials_ / 2;
         const size_t b = std::min(slots.size(), a + (size_t)SET_MAX);
         int nt = 0, ni = 0;
         for(size_t q = a; q < b; ++q)
@@ -914,6 +916,7 @@ int Engine::enqueue_batch(int kind, int x, int rhs, int ring, int iters)
                     });
                     if(part == 0) tend();
                 }
+        if(kind == SOLVE_MIXED) each([&](const TankSet& S) { bl_mixed_settle(S, stream); });
         return 0;
     };
     int rc;
@@ -1720,6 +1723,7 @@ int Engine::probe(double& stencil_ms, double& solver_ms)
                     else
                         bl_bicg_phase(S, REEF_PCORR, A_RHS, ph, part, stream);
                 });
+    if(mixed) each([&](const TankSet& S) { bl_mixed_settle(S, stream); });
     if((rc = solve_end(REEF_PCORR, mixed))) return rc;
     EHIP(hipEventRecord(ev[2].e, stream));
     EHIP(hipEventSynchronize(ev[2].e));

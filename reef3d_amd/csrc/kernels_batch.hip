@@ -204,6 +204,75 @@ __device__ __forceinline__ void tank_totals(CTank& T, const TankSet& b, double (
     sum_partials<M>(b.partials + (size_t)T.item0 * M, T.shape.nitems, tot);
 }
 
+// ------------------------------------------------------------------ reliable-update outcome
+// The state that a reliable update (mode 3) or restart (mode 1) of the mixed solve leaves,
+// from the FP64 sums of its residual phase tot = {|r|^2, (r0, r)} (kb_mix_ru).  The phase
+// has no finalize launch of its own: the blocks of the next v phase evaluate the outcome
+// from the same sums (they need beta and the mode), kb_fin_v stores it, and
+// kb_fin_mix_ru stores it after the last iteration of a batch, so the states the host
+// reads are settled.  An idle phase then costs one launch, not two.
+struct RuNext
+{
+    double rn, residual, r_j, norm_r0, beta;
+    int iter, done, mode;
+};
+
+__device__ __forceinline__ void tank_totals_ru(CTank& T, const TankSet& b, double (&tot)[2])
+{
+    sum_partials<2>(b.partials_ru + (size_t)T.item0 * 2, T.shape.nitems, tot);
+}
+
+__device__ __forceinline__ RuNext ru_next(const BiState* st, const double (&tot)[2])
+{
+    RuNext n;
+    n.rn = sqrt(tot[0]);
+    n.residual = n.rn / double(st->ncell);
+    n.iter = st->iter + (st->mode == 1 ? 1 : 0);   // a restart counts as an iteration (bicgstab_ijk.cpp:183-191)
+    n.done = !((n.residual >= st->stop) && (n.iter < st->maxiter));
+    n.mode = st->mode == 1 ? 2 : 0;
+    n.r_j = tot[1];
+    n.norm_r0 = st->norm_r0;
+    n.beta = st->beta;
+    if(st->mode == 1)
+        n.norm_r0 = sqrt(tot[1]);
+    else
+    {
+        const double wr = st->w * st->r_j_prev;
+        n.beta = st->alpha * tot[1] / (wr == 0 ? 1.0e-15 : wr);
+    }
+    return n;
+}
+
+// the outcome as the v phase needs it: mode (0 or 2; 4 for done) in the high word, the FP32
+// beta's bits in the low word.  Out of line, so that the walk of the calling kernel keeps
+// its registers.
+__device__ __noinline__ unsigned long long ru_outcome(const double* __restrict__ parts, int nitems, const BiState* st)
+{
+    double tot[2];
+    sum_partials<2>(parts, nitems, tot);
+    const RuNext n = ru_next(st, tot);
+    return ((unsigned long long)(n.done ? 4u : (unsigned)n.mode) << 32) | __builtin_bit_cast(unsigned, (float)n.beta);
+}
+
+// stores the outcome (one thread)
+__device__ __forceinline__ void ru_store(BiState* st, const RuNext& n)
+{
+    st->residual = n.residual;
+    st->nru += 1;
+    st->pending = 1;
+    st->rmax = n.rn;
+    st->iter = n.iter;
+    if(n.done)
+    {
+        st->done = 1;
+        return;
+    }
+    st->r_j = n.r_j;
+    st->norm_r0 = n.norm_r0;
+    st->beta = n.beta;
+    st->mode = n.mode;
+}
+
 // ------------------------------------------------------------------ per-precision views
 template<class V>
 struct Prec;
@@ -1252,7 +1321,8 @@ __global__ void __launch_bounds__(NT, PW ? occ_pw<V>() : (sizeof(V) == 4 ? OCC_F
 // precon_solve) so that no separate p pass runs; in mode 2 (p, ph fresh from the start
 // or a restart) ph is read.  The new p and v go to the other half of their double
 // buffers (kb_fin_v swaps the halves), ph is written at the cell for the x phase;
-// (v, r0), |v|^2, |r|^2 as kb_bicg_v.
+// (v, r0), |v|^2, |r|^2 as kb_bicg_v.  A tank still in mode 1 / 3 has just had its
+// reliable-update phase: the walk runs on that phase's outcome (RuNext).
 template<bool PW>
 __global__ void __launch_bounds__(NT, PW ? OCC_PW : OCC_FULL) kb_mix_v(TankSet b, int x)
 {
@@ -1262,13 +1332,30 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : OCC_FULL) kb_mix_v(TankSet b
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const BiState* st = T.st;
-        if(st->done || st->mode == 1 || st->mode == 3 || plane_walk_ok(T) != PW)
+        if(st->done || plane_walk_ok(T) != PW)
         {
             skip_tank(q, b);
             continue;
         }
-        const int upd = st->mode == 0, pend = st->pending, hsel = st->buf;
-        const float beta = (float)st->beta, w = (float)st->w;
+        int mode = st->mode, pend = st->pending;
+        float beta = (float)st->beta;
+        if(mode == 1 || mode == 3)
+        {
+            // the reliable-update phase ran last: its outcome from its sums (every thread
+            // holds the same totals; the decisions are made scalar)
+            const unsigned long long o = ru_outcome(b.partials_ru + (size_t)T.item0 * 2, T.shape.nitems, st);
+            const int om = __builtin_amdgcn_readfirstlane((int)(o >> 32));
+            if(om == 4)
+            {
+                skip_tank(q, b);
+                continue;
+            }
+            mode = om;
+            beta = __builtin_bit_cast(float, (unsigned)o);
+            pend = 1;
+        }
+        const int upd = mode == 0, hsel = st->buf;
+        const float w = (float)st->w;
         const SolverOp32 op = Prec<float>::op(T);
         const float* __restrict__ r = T.s[S_R];
         const float* __restrict__ r0 = T.s[S_R0];
@@ -1357,18 +1444,28 @@ __global__ void __launch_bounds__(NT, PW ? OCC_PW : OCC_FULL) kb_mix_v(TankSet b
 }
 
 // alpha and the restart tests (bicgstab_ijk.cpp:162-191); rr_state: |r|^2 from the state
-// (kb_bicg_v), else the phase's third sum (kb_mix_v, km_v)
+// (kb_bicg_v), else the phase's third sum (kb_mix_v, km_v).  A mixed-precision tank in
+// mode 1 / 3 first takes the outcome of its reliable-update phase, as kb_mix_v did.
 __global__ void kb_fin_v(TankSet b, int rr_state)
 {
     CTank& T = tank_at(b, blockIdx.x);
     BiState* st = T.st;
     // the partials' loads go out together with the state's (one memory latency, not two)
     const int done = st->done, mode = st->mode;
+    const bool ru = st->mixed && (mode == 1 || mode == 3);
     double tot[3];
     tank_totals<3>(T, b, tot);
-    if(done || mode == 1 || mode == 3) return;
+    double rtot[2];
+    if(ru && !done) tank_totals_ru(T, b, rtot);
+    if(done || (!ru && (mode == 1 || mode == 3))) return;
     if(threadIdx.x == 0)
     {
+        if(ru)
+        {
+            const RuNext n = ru_next(st, rtot);
+            ru_store(st, n);
+            if(n.done) return;
+        }
         const double sigma = tot[0];
         const double norm_vj = sqrt(tot[1]);
         const double norm_rj = sqrt(rr_state ? st->rr : tot[2]);
@@ -1739,44 +1836,20 @@ __global__ void __launch_bounds__(NT, OCC_A64) kb_mix_ru(TankSet b, int x, int r
             vals[0] += rv * rv;
             vals[1] += double(r0v) * double(r);
         }
-        block_sum_to<2>(vals, b.partials + (size_t)item_slot(T, b, q) * 2);
+        block_sum_to<2>(vals, b.partials_ru + (size_t)item_slot(T, b, q) * 2);
     }
 }
 
+// the outcome of a reliable-update phase that no v phase has taken yet (launched after the
+// last iteration of a batch)
 __global__ void kb_fin_mix_ru(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
     BiState* st = T.st;
     if(st->done || (st->mode != 1 && st->mode != 3)) return;
     double tot[2];
-    tank_totals<2>(T, b, tot);
-    if(threadIdx.x == 0)
-    {
-        const double rn = sqrt(tot[0]);
-        st->residual = rn / double(st->ncell);
-        st->nru += 1;
-        st->pending = 1;
-        st->rmax = rn;
-        if(st->mode == 1) st->iter += 1;     // a restart counts as an iteration (bicgstab_ijk.cpp:183-191)
-        if(!((st->residual >= st->stop) && (st->iter < st->maxiter)))
-        {
-            st->done = 1;
-            return;
-        }
-        if(st->mode == 1)
-        {
-            st->r_j = tot[1];
-            st->norm_r0 = sqrt(tot[1]);
-            st->mode = 2;
-        }
-        else
-        {
-            const double wr = st->w * st->r_j_prev;
-            st->beta = st->alpha * tot[1] / (wr == 0 ? 1.0e-15 : wr);
-            st->r_j = tot[1];
-            st->mode = 0;
-        }
-    }
+    tank_totals_ru(T, b, tot);
+    if(threadIdx.x == 0) ru_store(st, ru_next(st, tot));
 }
 
 // ------------------------------------------------------------------ explicit 7-diagonal operator
@@ -2792,8 +2865,13 @@ void bl_mixed_phase(const TankSet& b, int x, int rhs, int phase, int part, hipSt
         case 0: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b, 0); break;
         case 1: hipLaunchKernelGGL(kb_fin_t, tankgrid(b), dim3(NT), 0, s, b); break;
         case 2: hipLaunchKernelGGL(kb_fin_x, tankgrid(b), dim3(NT), 0, s, b); break;
-        default: hipLaunchKernelGGL(kb_fin_mix_ru, tankgrid(b), dim3(NT), 0, s, b); break;
+        default: break;   // the reliable-update phase is finalized by the next v phase
     }
+}
+
+void bl_mixed_settle(const TankSet& b, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_fin_mix_ru, tankgrid(b), dim3(NT), 0, s, b);
 }
 
 void bl_add32(const TankSet& b, int x, hipStream_t s)
