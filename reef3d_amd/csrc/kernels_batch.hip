@@ -254,6 +254,29 @@ __device__ __noinline__ unsigned long long ru_outcome(const double* __restrict__
     return ((unsigned long long)(n.done ? 4u : (unsigned)n.mode) << 32) | __builtin_bit_cast(unsigned, (float)n.beta);
 }
 
+// The FP64 solve's restart (kb_bicg_p in mode 1) has no finalize launch of its own either:
+// the v phase that follows it in the same iteration needs only its stop decision, taken
+// from its sum |r|^2 (restart_done), and kb_fin_v stores its outcome (restart_store), as
+// kb_fin_p does for the explicit-operator solve (bicgstab_ijk.cpp:183-191).
+__device__ __noinline__ int restart_done(const double* __restrict__ parts, int nitems, const BiState* st)
+{
+    double tot[1];
+    sum_partials<1>(parts, nitems, tot);
+    const double residual = sqrt(tot[0]) / double(st->ncell);
+    return !((residual >= st->stop) && (st->iter + 1 < st->maxiter));
+}
+
+__device__ __forceinline__ void restart_store(BiState* st, double rr)
+{
+    st->r_j = rr;
+    st->rr = rr;
+    st->norm_r0 = sqrt(rr);
+    st->residual = sqrt(rr) / double(st->ncell);
+    st->iter += 1;
+    if(!((st->residual >= st->stop) && (st->iter < st->maxiter))) st->done = 1;
+    st->mode = 2;
+}
+
 // stores the outcome (one thread)
 __device__ __forceinline__ void ru_store(BiState* st, const RuNext& n)
 {
@@ -1194,7 +1217,7 @@ __global__ void __launch_bounds__(NT, OCC_A64) kb_bicg_p(TankSet b, int x, int r
         {
             double vals[1] = {0.0};
             restart_body(g, Prec<V>::op(T), v, tw, vals[0]);
-            block_sum_to<1>(vals, b.partials + (size_t)item_slot(T, b, q));
+            block_sum_to<1>(vals, b.partials_ru + (size_t)item_slot(T, b, q));
             continue;
         }
         const V beta = (V)st->beta, w = (V)st->w;
@@ -1217,7 +1240,8 @@ __global__ void __launch_bounds__(NT, OCC_A64) kb_bicg_p(TankSet b, int x, int r
     }
 }
 
-// totals of a restart: ++solveriter, residual = |r|/N (bicgstab_ijk.cpp:183-191)
+// totals of a restart of the explicit-operator solve (km_p): ++solveriter, residual = |r|/N
+// (bicgstab_ijk.cpp:183-191); the Poisson solve's restarts are finalized by its v phase
 __global__ void kb_fin_p(TankSet b)
 {
     CTank& T = tank_at(b, blockIdx.x);
@@ -1251,7 +1275,15 @@ __global__ void __launch_bounds__(NT, PW ? occ_pw<V>() : (sizeof(V) == 4 ? OCC_F
         CTank& T = tank_at(b, q.p);
         const DevGrid g = T.g;
         const BiState* st = T.st;
-        if(st->done || st->mode == 1 || st->mode == 3 || plane_walk_ok(T) != PW)
+        if(st->done || st->mode == 3 || plane_walk_ok(T) != PW)
+        {
+            skip_tank(q, b);
+            continue;
+        }
+        // mode 1: this iteration's p phase has restarted the tank (ph = aii r, fresh)
+        if(st->mode == 1
+           && (sizeof(V) != 8
+               || __builtin_amdgcn_readfirstlane(restart_done(b.partials_ru + (size_t)T.item0, T.shape.nitems, st))))
         {
             skip_tank(q, b);
             continue;
@@ -1453,11 +1485,15 @@ __global__ void kb_fin_v(TankSet b, int rr_state)
     // the partials' loads go out together with the state's (one memory latency, not two)
     const int done = st->done, mode = st->mode;
     const bool ru = st->mixed && (mode == 1 || mode == 3);
+    // the FP64 Poisson solve (rr_state): mode 1 is the restart of this iteration's p phase
+    const bool rs = rr_state && mode == 1;
     double tot[3];
     tank_totals<3>(T, b, tot);
     double rtot[2];
     if(ru && !done) tank_totals_ru(T, b, rtot);
-    if(done || (!ru && (mode == 1 || mode == 3))) return;
+    double stot[1];
+    if(rs && !done) sum_partials<1>(b.partials_ru + (size_t)T.item0, T.shape.nitems, stot);
+    if(done || (!ru && !rs && (mode == 1 || mode == 3))) return;
     if(threadIdx.x == 0)
     {
         if(ru)
@@ -1465,6 +1501,11 @@ __global__ void kb_fin_v(TankSet b, int rr_state)
             const RuNext n = ru_next(st, rtot);
             ru_store(st, n);
             if(n.done) return;
+        }
+        if(rs)
+        {
+            restart_store(st, stot[0]);
+            if(st->done) return;
         }
         const double sigma = tot[0];
         const double norm_vj = sqrt(tot[1]);
@@ -2768,7 +2809,7 @@ void bl_bicg_phase(const TankSet& b, int x, int rhs, int phase, int part, hipStr
     }
     switch(phase)
     {
-        case 0: hipLaunchKernelGGL(kb_fin_p, tankgrid(b), dim3(NT), 0, s, b); break;
+        case 0: break;   // a restart of the p phase is finalized by the v phase
         case 1: hipLaunchKernelGGL(kb_fin_v, tankgrid(b), dim3(NT), 0, s, b, 1); break;
         case 2: hipLaunchKernelGGL(kb_fin_t, tankgrid(b), dim3(NT), 0, s, b); break;
         default: hipLaunchKernelGGL(kb_fin_x, tankgrid(b), dim3(NT), 0, s, b); break;
